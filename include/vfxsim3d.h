@@ -1,0 +1,103 @@
+/* vfxsim3d.h — C ABI of the MI355X (gfx950) sim3d smoke/fire solver library (libvfxsim3d.so).
+ *
+ * The drop-in boundary for the reference's `vfx.sim3d.Solver3D` (vfx/sim3d/solver.py): the
+ * Python host layer (neural-vfx_amd/vfx/sim3d/solver.py) keeps the reference API and calls these
+ * entry points through ctypes.  Plain pointers and sizes only:
+ *   - field pointers are device pointers borrowed from the caller (torch.Tensor.data_ptr());
+ *     `data` is (6, N, N, N) float32 [c][z][y][x] with channels density, vel_x, vel_y, vel_z,
+ *     temperature, fuel (vfx/core/fields3d.py:12-16); `solid` is (N, N, N) float32, 1 = obstacle;
+ *   - `prm_host` is a host array of vfx_p_size() floats in the reference's parameter layout
+ *     (vfx/sim3d/kernels.py:11-42, filled like Solver3D._upload, solver.py:331-359);
+ *   - `stream` is a hipStream_t (torch.cuda.current_stream().cuda_stream), 0 = null stream.
+ * Every function returns 0 on success or a VFX_E_* code; vfx_last_error() gives the message.
+ * The library allocates device memory only in vfx_ctx_create; calls on one context must not
+ * overlap (one context per Solver3D, as the reference keeps one solver per grid).
+ */
+#ifndef VFXSIM3D_H
+#define VFXSIM3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFX_OK 0
+#define VFX_E_ARG 1     /* shape / pointer / size assertion (the reference's AssertionError) */
+#define VFX_E_VALUE 2   /* bad value (the reference's ValueError, e.g. >32 emitters) */
+#define VFX_E_HIP 3     /* HIP runtime error */
+
+/* stage selectors of vfx_step_stage */
+#define VFX_STAGE_ADVECT 0   /* obstacle setup + MacCormack advection + mass ratio: solver.py:422-430 */
+#define VFX_STAGE_FORCES 1   /* sources, diffusion, vorticity confinement, forces: solver.py:432-455 */
+#define VFX_STAGE_PROJECT 2  /* pressure projection: pressure.py:940-950 */
+
+/* options of vfx_set_option */
+#define VFX_OPT_TRUE_RESIDUAL 1  /* 1: compute ||b - K phi|| / ||b|| after each projection */
+#define VFX_OPT_RESTARTS 2       /* CG restarts from the true residual while it exceeds the
+                                    relative tolerance (needs P_REL_TOL > 0); default 0 */
+#define VFX_OPT_COARSE_M 3       /* multigrid levels with m <= this run in one workgroup */
+
+typedef struct vfx_ctx vfx_ctx;
+
+/* Number of floats of the parameter buffer (kernels.py:42 P_SIZE = 576). */
+int vfx_p_size(void);
+
+/* Allocate every buffer of a res^3 solver (Solver3D.__init__, solver.py:86-125;
+ * PressureSolver3D.__init__, pressure.py:789-818).  res must be even (pressure.py:790). */
+int vfx_ctx_create(int res, vfx_ctx** out);
+int vfx_ctx_destroy(vfx_ctx* ctx);
+
+/* One whole step in place: advection, forces, projection (Solver3D.step, solver.py:146-155). */
+int vfx_step(vfx_ctx* ctx, float* data, const float* solid, const float* prm_host, void* stream);
+
+/* One stage of the step (Solver3D.step_profiled's graphs, solver.py:157-184).  The advection
+ * stage uploads prm_host; the later stages reuse the uploaded parameters (prm_host may be NULL). */
+int vfx_step_stage(vfx_ctx* ctx, int stage, float* data, const float* solid, const float* prm_host, void* stream);
+
+/* Projection only, with dt = 0 forces (Solver3D.project, solver.py:266-272). */
+int vfx_project(vfx_ctx* ctx, float* data, const float* solid, const float* prm_host, void* stream);
+
+/* MacCormack advection of C channels by `vel` (3,N,N,N), no obstacles, no mass correction
+ * (Solver3D.advect, solver.py:289-308).  out is (C,N,N,N) and must not alias field. */
+int vfx_advect(vfx_ctx* ctx, const float* field, int C, const float* vel, float* out, float dt, void* stream);
+
+/* |div u| dx per cell, 0 in solids (record_div + k_div_abs, pressure.py:952-956, :156-163);
+ * the caller normalises by max|u| like Solver3D.divergence (solver.py:274-287). */
+int vfx_div_abs(vfx_ctx* ctx, const float* data, const float* solid, float* out, void* stream);
+
+/* PCG iterations of the last projection (Solver3D.pressure_iters, solver.py:383-386); waits for
+ * the last projection to finish. */
+int vfx_pressure_iters(vfx_ctx* ctx, int* out);
+
+/* Diagnostics of the last projection (waits for it): out[0] iterations (all restarts),
+ * out[1] max|u*|, out[2] ||b||_2, out[3] recurrence ||r||_2, out[4] max|r|,
+ * out[5] true ||b - K phi||_2 (when VFX_OPT_TRUE_RESIDUAL is on, else -1), out[6] restarts. */
+int vfx_pressure_stats(vfx_ctx* ctx, double* out, int n);
+
+int vfx_set_option(vfx_ctx* ctx, int key, double value);
+
+/* Copy the warm-start pressure phi (N^3 floats, device) out of / into the solver. */
+int vfx_get_phi(vfx_ctx* ctx, float* dst, void* stream);
+int vfx_set_phi(vfx_ctx* ctx, const float* src, void* stream);
+
+/* Copy the advected state of the last advection stage (6,N,N,N) and the mass ratio. */
+int vfx_get_advected(vfx_ctx* ctx, float* dst, double* mass_ratio, void* stream);
+
+/* Operator access for parity tests: rebuild masks and multigrid operators for `solid`, then
+ * out = K p (vfx_kmul), z = M^-1 r (vfx_precond, the V(1,1) preconditioner), or the level-l
+ * operators in component-fastest layout diag [m^3][8], off [m^3][24] (vfx_get_ops; returns
+ * the level count in *nlev and the level size in *m). */
+int vfx_kmul(vfx_ctx* ctx, const float* solid, const float* p, float* out, void* stream);
+int vfx_precond(vfx_ctx* ctx, const float* solid, const float* r, float* z, void* stream);
+int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float* off, int* nlev, int* m,
+                void* stream);
+
+/* Message of the last failed call on this thread. */
+int vfx_last_error(char* buf, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFXSIM3D_H */

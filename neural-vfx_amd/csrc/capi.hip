@@ -1,0 +1,500 @@
+// C ABI of the sim3d library (include/vfxsim3d.h): context, buffers, parameter upload ring and
+// the host-side step / projection sequencing (Solver3D, vfx/sim3d/solver.py:79-459).
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/vfxsim3d.h"
+#include "sim3d_kernels.hpp"
+
+namespace vfx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+bool launch_ok(const char* what) { return hip_ok(hipGetLastError(), what); }
+
+Geom make_geom(int N, int zc) {
+    Geom g;
+    g.N = N;
+    g.n3 = (size_t)N * N * N;
+    g.bx = (N % 64 == 0 || N > 32) ? 64 : 32;
+    g.by = 256 / g.bx;
+    if (zc <= 0) {
+        // bound the block count (and so the reduction partials) to about 4096
+        const size_t per = (size_t)256 * 4096;
+        zc = (int)((g.n3 + per - 1) / per);
+        if (zc < 1) zc = 1;
+    }
+    g.zc = zc;
+    g.block = dim3(g.bx, g.by, 1);
+    g.grid = dim3((N + g.bx - 1) / g.bx, (N + g.by - 1) / g.by, (N + zc - 1) / zc);
+    return g;
+}
+
+__global__ void k_reset_cg(DevState* st) {
+    if (threadIdx.x == 0) {
+        st->it = 0;
+        st->stop_at = INT_MAX;
+    }
+}
+
+}  // namespace vfx
+
+using namespace vfx;
+
+constexpr int HOST_RING = 4;   // pinned parameter slots (solver.py:81, :331-359)
+constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag
+
+struct vfx_ctx {
+    int N = 0;
+    Geom g;
+    float inv_dx = 0.0f, inv2dx = 0.0f;
+    // device buffers
+    void* pool = nullptr;
+    float *hat = nullptr, *adv = nullptr, *omega = nullptr, *omega_s = nullptr, *force = nullptr;
+    float *tmp_a = nullptr, *tmp_b = nullptr;
+    float* prm = nullptr;      // parameters of the step
+    float* prm_adv = nullptr;  // parameters of the standalone advect()
+    double* mass_part = nullptr;
+    DevState* st = nullptr;
+    Pressure P;
+    // host side
+    float* ring = nullptr;   // HOST_RING * P_SIZE pinned floats
+    hipEvent_t ring_ev[HOST_RING] = {};
+    bool ring_used[HOST_RING] = {};
+    int ring_i = 0;
+    float prm_last[P_SIZE];
+    int* stop_pin = nullptr;  // STOP_RING pinned ints
+    hipEvent_t stop_ev[STOP_RING] = {};
+    hipEvent_t done_ev = nullptr;
+    bool projected = false;
+    // options
+    bool opt_true_res = false;
+    int opt_restarts = 0;
+    // diagnostics of the last projection
+    int last_iters = 0, last_restarts = 0;
+    bool last_true_valid = false;
+};
+
+static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define CHECK_HIP(expr, what)                                    \
+    do {                                                         \
+        if (!hip_ok((expr), what)) return VFX_E_HIP;             \
+    } while (0)
+#define CHECK_LAUNCH(what)                                       \
+    do {                                                         \
+        if (!launch_ok(what)) return VFX_E_HIP;                  \
+    } while (0)
+
+static int arg_error(const std::string& m) {
+    set_error(m);
+    return VFX_E_ARG;
+}
+
+static int upload(vfx_ctx* c, const float* host, float* dst, hipStream_t s) {
+    const int i = c->ring_i;
+    c->ring_i = (i + 1) % HOST_RING;
+    if (c->ring_used[i]) CHECK_HIP(hipEventSynchronize(c->ring_ev[i]), "parameter ring wait");
+    float* slot = c->ring + (size_t)i * P_SIZE;
+    std::memcpy(slot, host, sizeof(float) * P_SIZE);
+    CHECK_HIP(hipMemcpyAsync(dst, slot, sizeof(float) * P_SIZE, hipMemcpyHostToDevice, s), "parameter upload");
+    CHECK_HIP(hipEventRecord(c->ring_ev[i], s), "parameter ring event");
+    c->ring_used[i] = true;
+    return VFX_OK;
+}
+
+static int validate_prm(const float* prm) {
+    if (prm == nullptr) return arg_error("prm_host is NULL");
+    if (prm[P_N_EM] < 0 || prm[P_N_EM] > MAX_EMITTERS || prm[P_N_IMP] < 0 || prm[P_N_IMP] > MAX_IMPULSES) {
+        set_error("at most 32 emitters / 32 impulses");
+        return VFX_E_VALUE;
+    }
+    return VFX_OK;
+}
+
+static int stage_advect(vfx_ctx* c, float* data, const float* solid, hipStream_t s) {
+    c->P.setup(solid, s);
+    CHECK_LAUNCH("obstacle setup");
+    launch_advect(c->g, 6, data, data + c->g.n3, c->prm, c->inv_dx, c->hat, c->adv, c->P.fm, c->mass_part, s);
+    launch_mass_fin(c->g, c->mass_part, c->st, s);
+    CHECK_LAUNCH("advection");
+    return VFX_OK;
+}
+
+static int stage_forces(vfx_ctx* c, float* data, hipStream_t s) {
+    launch_sources_diffuse(c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, data, s);
+    launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega, c->omega_s, c->force, c->tmp_a, c->tmp_b,
+                       s);
+    CHECK_LAUNCH("forces");
+    return VFX_OK;
+}
+
+// The PCG loop (PressureSolver3D.record, pressure.py:940-950).  The reference runs the loop as a
+// CUDA-graph conditional; here the host enqueues iteration k, then waits only for iteration k-1's
+// stop flag, so the device always has the next iteration queued and stopped iterations are
+// no-ops on the device.
+static int run_cg(vfx_ctx* c, hipStream_t s, int* iters) {
+    Pressure& P = c->P;
+    const int maxit = (int)c->prm_last[P_MAXIT];
+    P.iteration(0, c->prm, s);
+    CHECK_LAUNCH("pcg init");
+    CHECK_HIP(hipMemcpyAsync(&c->stop_pin[0], &c->st->stop_at, sizeof(int), hipMemcpyDeviceToHost, s), "stop read");
+    CHECK_HIP(hipEventRecord(c->stop_ev[0], s), "stop event");
+    int k = 1;
+    for (; k <= maxit; ++k) {
+        P.iteration(k, c->prm, s);
+        CHECK_LAUNCH("pcg iteration");
+        const int slot = k % STOP_RING;
+        CHECK_HIP(hipMemcpyAsync(&c->stop_pin[slot], &c->st->stop_at, sizeof(int), hipMemcpyDeviceToHost, s),
+                  "stop read");
+        CHECK_HIP(hipEventRecord(c->stop_ev[slot], s), "stop event");
+        const int prev = (k - 1) % STOP_RING;
+        CHECK_HIP(hipEventSynchronize(c->stop_ev[prev]), "stop wait");
+        if (c->stop_pin[prev] <= k - 1) break;
+    }
+    *iters = k;   // upper bound; the exact count is DevState::it
+    return VFX_OK;
+}
+
+static int stage_project(vfx_ctx* c, float* data, hipStream_t s) {
+    Pressure& P = c->P;
+    float* vel = data + c->g.n3;
+    P.rhs(vel, c->prm, s);
+    CHECK_LAUNCH("pressure rhs");
+    int upper = 0;
+    int rc = run_cg(c, s, &upper);
+    if (rc) return rc;
+    c->last_restarts = 0;
+    c->last_true_valid = false;
+    int total = 0;
+    const double rel_tol = (double)c->prm_last[P_REL_TOL];
+    if (c->opt_true_res || (c->opt_restarts > 0 && rel_tol > 0.0)) {
+        for (;;) {
+            P.true_residual(vel, s);
+            CHECK_LAUNCH("true residual");
+            c->last_true_valid = true;
+            if (!(c->opt_restarts > c->last_restarts && rel_tol > 0.0)) break;
+            DevState hs;
+            CHECK_HIP(hipMemcpyAsync(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "state read");
+            CHECK_HIP(hipStreamSynchronize(s), "state sync");
+            if (std::sqrt(hs.s[S_TRUE_RR]) <= rel_tol * std::sqrt(hs.s[S_BB])) break;
+            total += hs.it;
+            c->last_restarts += 1;
+            // restart from the true residual: warm start from the current phi
+            float saved = c->prm_last[P_WARM];
+            c->prm_last[P_WARM] = 1.0f;
+            rc = upload(c, c->prm_last, c->prm, s);
+            c->prm_last[P_WARM] = saved;
+            if (rc) return rc;
+            P.rhs(vel, c->prm, s);
+            CHECK_LAUNCH("pressure restart rhs");
+            rc = run_cg(c, s, &upper);
+            if (rc) return rc;
+        }
+    }
+    c->last_iters = total;   // plus DevState::it of the final run, read lazily
+    P.apply(vel, s);
+    CHECK_LAUNCH("pressure apply");
+    CHECK_HIP(hipEventRecord(c->done_ev, s), "done event");
+    c->projected = true;
+    return VFX_OK;
+}
+
+static int check_ctx(vfx_ctx* c) {
+    if (c == nullptr) return arg_error("null context");
+    return VFX_OK;
+}
+
+extern "C" {
+
+int vfx_p_size(void) { return P_SIZE; }
+
+int vfx_last_error(char* buf, int n) {
+    if (buf && n > 0) {
+        std::snprintf(buf, (size_t)n, "%s", g_err.c_str());
+    }
+    return VFX_OK;
+}
+
+int vfx_ctx_create(int res, vfx_ctx** out) {
+    if (out == nullptr) return arg_error("out is NULL");
+    *out = nullptr;
+    if (res < 2 || res % 2 != 0) return arg_error("even grid size >= 2 required");
+    if (res > 1024) return arg_error("grid size above 1024");
+    vfx_ctx* c = new (std::nothrow) vfx_ctx();
+    if (!c) return arg_error("out of host memory");
+    const int N = res;
+    c->N = N;
+    c->g = make_geom(N, 0);
+    const double dx = 1.0 / (double)N;   // DOMAIN_SIZE / res (fields.py:25)
+    const double inv = 1.0 / dx;
+    c->inv_dx = (float)inv;
+    c->inv2dx = (float)(0.5 * inv);
+    Pressure& P = c->P;
+    P.init_levels(N);
+    P.geom = c->g;
+    P.nparts = nblocks(c->g);
+    const size_t n3 = c->g.n3;
+    // one pool: floats first, then doubles, then the mask
+    const size_t nf = n3 * (6 + 6 + 3 + 3 + 3 + 4 + 4) + 2 * (size_t)P_SIZE + n3 * 8 + P.level_floats();
+    const size_t nd = (size_t)P.nparts * (2 + 3 + 1) + sizeof(DevState) / sizeof(double) + 8;
+    const size_t bytes = nf * 4 + 64 + nd * 8 + 64 + n3 + 64;
+    if (!hip_ok(hipMalloc(&c->pool, bytes), "device allocation")) { delete c; return VFX_E_HIP; }
+    if (!hip_ok(hipMemset(c->pool, 0, bytes), "device clear")) { hipFree(c->pool); delete c; return VFX_E_HIP; }
+    float* f = (float*)c->pool;
+    c->hat = f; f += 6 * n3;
+    c->adv = f; f += 6 * n3;
+    c->omega = f; f += 3 * n3;
+    c->omega_s = f; f += 3 * n3;
+    c->force = f; f += 3 * n3;
+    c->tmp_a = f; f += 4 * n3;
+    c->tmp_b = f; f += 4 * n3;
+    c->prm = f; f += P_SIZE;
+    c->prm_adv = f; f += P_SIZE;
+    P.r = f; f += n3;
+    P.z = f; f += n3;
+    P.p = f; f += n3;
+    P.q = f; f += n3;
+    P.phi = f; f += n3;
+    P.z0 = f; f += n3;
+    P.rr = f; f += n3;
+    P.zt = f; f += n3;
+    P.bind_levels(f); f += P.level_floats();
+    uintptr_t a = ((uintptr_t)f + 63) & ~(uintptr_t)63;
+    double* d = (double*)a;
+    c->mass_part = d; d += 2 * (size_t)P.nparts;
+    P.part = d; d += 3 * (size_t)P.nparts;
+    P.part_pq = d; d += (size_t)P.nparts;
+    a = ((uintptr_t)d + 63) & ~(uintptr_t)63;
+    c->st = (DevState*)a;
+    P.st = c->st;
+    a = (a + sizeof(DevState) + 63) & ~(uintptr_t)63;
+    P.fm = (uint8_t*)a;
+    bool ok = hip_ok(hipHostMalloc((void**)&c->ring, sizeof(float) * P_SIZE * HOST_RING, hipHostMallocDefault),
+                     "pinned allocation") &&
+              hip_ok(hipHostMalloc((void**)&c->stop_pin, sizeof(int) * STOP_RING, hipHostMallocDefault),
+                     "pinned allocation");
+    for (int i = 0; ok && i < HOST_RING; ++i)
+        ok = hip_ok(hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming), "event");
+    for (int i = 0; ok && i < STOP_RING; ++i)
+        ok = hip_ok(hipEventCreateWithFlags(&c->stop_ev[i], hipEventDisableTiming), "event");
+    ok = ok && hip_ok(hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming), "event");
+    std::memset(c->prm_last, 0, sizeof(c->prm_last));
+    if (!ok) { vfx_ctx_destroy(c); return VFX_E_HIP; }
+    *out = c;
+    return VFX_OK;
+}
+
+int vfx_ctx_destroy(vfx_ctx* c) {
+    if (!c) return VFX_OK;
+    hipDeviceSynchronize();
+    for (int i = 0; i < HOST_RING; ++i) if (c->ring_ev[i]) hipEventDestroy(c->ring_ev[i]);
+    for (int i = 0; i < STOP_RING; ++i) if (c->stop_ev[i]) hipEventDestroy(c->stop_ev[i]);
+    if (c->done_ev) hipEventDestroy(c->done_ev);
+    if (c->ring) hipHostFree(c->ring);
+    if (c->stop_pin) hipHostFree(c->stop_pin);
+    if (c->pool) hipFree(c->pool);
+    delete c;
+    return VFX_OK;
+}
+
+int vfx_set_option(vfx_ctx* c, int key, double value) {
+    if (int rc = check_ctx(c)) return rc;
+    switch (key) {
+        case VFX_OPT_TRUE_RESIDUAL: c->opt_true_res = value != 0.0; return VFX_OK;
+        case VFX_OPT_RESTARTS: c->opt_restarts = value < 0 ? 0 : (int)value; return VFX_OK;
+        case VFX_OPT_COARSE_M: {
+            const int cm = (int)value;
+            if (cm < 1) return arg_error("coarse_m must be >= 1");
+            c->P.coarse_m = cm;
+            c->P.S = c->P.nlev - 1;
+            for (int l = 0; l < c->P.nlev; ++l)
+                if (c->P.lv[l].m <= cm) { c->P.S = l; break; }
+            return VFX_OK;
+        }
+        default: set_error("unknown option"); return VFX_E_VALUE;
+    }
+}
+
+int vfx_step_stage(vfx_ctx* c, int stage, float* data, const float* solid, const float* prm_host, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!data || !solid) return arg_error("null field pointer");
+    hipStream_t s = S(stream);
+    if (stage == VFX_STAGE_ADVECT || prm_host != nullptr) {
+        if (int rc = validate_prm(prm_host)) return rc;
+        std::memcpy(c->prm_last, prm_host, sizeof(c->prm_last));
+        if (int rc = upload(c, prm_host, c->prm, s)) return rc;
+    }
+    switch (stage) {
+        case VFX_STAGE_ADVECT: return stage_advect(c, data, solid, s);
+        case VFX_STAGE_FORCES: return stage_forces(c, data, s);
+        case VFX_STAGE_PROJECT:
+            c->P.setup(solid, s);
+            CHECK_LAUNCH("obstacle setup");
+            return stage_project(c, data, s);
+        default: set_error("unknown stage"); return VFX_E_VALUE;
+    }
+}
+
+int vfx_step(vfx_ctx* c, float* data, const float* solid, const float* prm_host, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!data || !solid) return arg_error("null field pointer");
+    if (int rc = validate_prm(prm_host)) return rc;
+    hipStream_t s = S(stream);
+    std::memcpy(c->prm_last, prm_host, sizeof(c->prm_last));
+    if (int rc = upload(c, prm_host, c->prm, s)) return rc;
+    if (int rc = stage_advect(c, data, solid, s)) return rc;
+    if (int rc = stage_forces(c, data, s)) return rc;
+    return stage_project(c, data, s);
+}
+
+int vfx_project(vfx_ctx* c, float* data, const float* solid, const float* prm_host, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!data || !solid) return arg_error("null field pointer");
+    if (int rc = validate_prm(prm_host)) return rc;
+    hipStream_t s = S(stream);
+    std::memcpy(c->prm_last, prm_host, sizeof(c->prm_last));
+    if (int rc = upload(c, prm_host, c->prm, s)) return rc;
+    c->P.setup(solid, s);
+    CHECK_LAUNCH("obstacle setup");
+    return stage_project(c, data, s);
+}
+
+int vfx_advect(vfx_ctx* c, const float* field, int C, const float* vel, float* out, float dt, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!field || !vel || !out) return arg_error("null pointer");
+    if (C < 1) return arg_error("channel count must be >= 1");
+    hipStream_t s = S(stream);
+    float h[P_SIZE];
+    std::memset(h, 0, sizeof(h));
+    h[P_DT] = dt;
+    if (int rc = upload(c, h, c->prm_adv, s)) return rc;
+    const size_t n3 = c->g.n3;
+    for (int c0 = 0; c0 < C; c0 += 6) {
+        const int cc = C - c0 < 6 ? C - c0 : 6;
+        launch_advect(c->g, cc, field + c0 * n3, vel, c->prm_adv, c->inv_dx, c->hat, out + c0 * n3, nullptr, nullptr,
+                      s);
+    }
+    CHECK_LAUNCH("advect");
+    return VFX_OK;
+}
+
+int vfx_div_abs(vfx_ctx* c, const float* data, const float* solid, float* out, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!data || !solid || !out) return arg_error("null pointer");
+    launch_div_abs(c->g, data + c->g.n3, solid, out, S(stream));
+    CHECK_LAUNCH("divergence");
+    return VFX_OK;
+}
+
+int vfx_pressure_iters(vfx_ctx* c, int* out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!out) return arg_error("null pointer");
+    if (!c->projected) { *out = 0; return VFX_OK; }
+    CHECK_HIP(hipEventSynchronize(c->done_ev), "projection wait");
+    DevState hs;
+    CHECK_HIP(hipMemcpy(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost), "state read");
+    *out = c->last_iters + hs.it;
+    return VFX_OK;
+}
+
+int vfx_pressure_stats(vfx_ctx* c, double* out, int n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!out || n < 1) return arg_error("null pointer");
+    double v[7] = {0, 0, 0, 0, 0, -1, 0};
+    if (c->projected) {
+        CHECK_HIP(hipEventSynchronize(c->done_ev), "projection wait");
+        DevState hs;
+        CHECK_HIP(hipMemcpy(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost), "state read");
+        v[0] = c->last_iters + hs.it;
+        v[1] = hs.s[S_MAXU];
+        v[2] = std::sqrt(hs.s[S_BB]);
+        v[3] = std::sqrt(hs.s[S_RR]);
+        v[4] = hs.s[S_MAXR];
+        v[5] = c->last_true_valid ? std::sqrt(hs.s[S_TRUE_RR]) : -1.0;
+        v[6] = c->last_restarts;
+    }
+    for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
+    return VFX_OK;
+}
+
+int vfx_get_phi(vfx_ctx* c, float* dst, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!dst) return arg_error("null pointer");
+    CHECK_HIP(hipMemcpyAsync(dst, c->P.phi, c->g.n3 * 4, hipMemcpyDeviceToDevice, S(stream)), "phi copy");
+    return VFX_OK;
+}
+
+int vfx_set_phi(vfx_ctx* c, const float* src, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!src) return arg_error("null pointer");
+    CHECK_HIP(hipMemcpyAsync(c->P.phi, src, c->g.n3 * 4, hipMemcpyDeviceToDevice, S(stream)), "phi copy");
+    return VFX_OK;
+}
+
+int vfx_get_advected(vfx_ctx* c, float* dst, double* mass_ratio, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    hipStream_t s = S(stream);
+    if (dst) CHECK_HIP(hipMemcpyAsync(dst, c->adv, 6 * c->g.n3 * 4, hipMemcpyDeviceToDevice, s), "adv copy");
+    if (mass_ratio) {
+        DevState hs;
+        CHECK_HIP(hipMemcpyAsync(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "state read");
+        CHECK_HIP(hipStreamSynchronize(s), "state sync");
+        *mass_ratio = hs.s[S_MASS_RATIO];
+    }
+    return VFX_OK;
+}
+
+int vfx_kmul(vfx_ctx* c, const float* solid, const float* p, float* out, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!solid || !p || !out) return arg_error("null pointer");
+    hipStream_t s = S(stream);
+    c->P.setup(solid, s);
+    launch_kmul(c->g, p, c->P.fm, out, s);
+    CHECK_LAUNCH("kmul");
+    return VFX_OK;
+}
+
+int vfx_precond(vfx_ctx* c, const float* solid, const float* r, float* z, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!solid || !r || !z) return arg_error("null pointer");
+    hipStream_t s = S(stream);
+    const size_t n3 = c->g.n3;
+    c->P.setup(solid, s);
+    CHECK_HIP(hipMemcpyAsync(c->P.r, r, n3 * 4, hipMemcpyDeviceToDevice, s), "r copy");
+    hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
+    c->P.precond(0, s);
+    CHECK_LAUNCH("precond");
+    CHECK_HIP(hipMemcpyAsync(z, c->P.z, n3 * 4, hipMemcpyDeviceToDevice, s), "z copy");
+    return VFX_OK;
+}
+
+int vfx_get_ops(vfx_ctx* c, const float* solid, int level, float* diag, float* off, int* nlev, int* m,
+                void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!solid) return arg_error("null pointer");
+    if (nlev) *nlev = c->P.nlev;
+    if (level < 0 || level >= c->P.nlev) return arg_error("level out of range");
+    const Lvl& L = c->P.lv[level];
+    if (m) *m = L.m;
+    hipStream_t s = S(stream);
+    c->P.setup(solid, s);
+    CHECK_LAUNCH("setup");
+    const size_t m3 = (size_t)L.m * L.m * L.m;
+    if (diag) CHECK_HIP(hipMemcpyAsync(diag, L.diag, m3 * 8 * 4, hipMemcpyDeviceToDevice, s), "diag copy");
+    if (off) CHECK_HIP(hipMemcpyAsync(off, L.off, m3 * 24 * 4, hipMemcpyDeviceToDevice, s), "off copy");
+    return VFX_OK;
+}
+
+}  // extern "C"

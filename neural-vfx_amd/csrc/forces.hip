@@ -1,0 +1,361 @@
+// Sources, heat/fuel diffusion, vorticity confinement, buoyancy, wind and the smoothed force.
+//
+// Restates vfx/sim3d/kernels.py:181-457 (sources, diffuse_heat_fuel, curl, body_forces, the
+// separable fluid-renormalised [1,2,1]^3 smoothing sep_x / sep_y / sep_z_store / sep_z_apply)
+// in the launch order of solver.py:432-455.  "Solid" is read from the uint8 face mask
+// (mask == 0 <=> solid > 0.5, pressure.py:115-143); cells outside the box count as solid.
+#include "sim3d_kernels.hpp"
+
+namespace vfx {
+
+__device__ __forceinline__ float clampf_(float g, float lo, float hi) {
+    return g < lo ? lo : (g > hi ? hi : g);
+}
+
+// Philox-4x32-10 (Salmon et al., SC'11), first output word; key (seed, emitter), counter
+// (cell, step, 'EMIT', 0).
+__device__ __forceinline__ uint32_t philox_u32(uint32_t seed, uint32_t em, uint32_t cell, uint32_t step) {
+    uint32_t c0 = cell, c1 = step, c2 = 0x454D4954u, c3 = 0u, k0 = seed, k1 = em;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    return c0;
+}
+
+// sources (kernels.py:182-250) with the ignition ramp width and per-cell emitter noise as
+// parameters (reference behaviour: ramp 100 K, noise off).
+__global__ void __launch_bounds__(256) k_sources(int N, int zc, const float* __restrict__ adv,
+                                                 const uint8_t* __restrict__ fm, const float* __restrict__ prm,
+                                                 const DevState* __restrict__ st, float* __restrict__ out) {
+    const size_t n3 = (size_t)N * N * N;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= N || y >= N) return;
+    const float amb = prm[P_AMB], dt = prm[P_DT], dx = prm[P_DX];
+    const int n_em = (int)prm[P_N_EM], n_imp = (int)prm[P_N_IMP];
+    const float ramp = prm[P_IGN_RAMP], nlo = prm[P_NOISE_LO], nhi = prm[P_NOISE_HI];
+    const bool noisy = !(nlo == 1.0f && nhi == 1.0f);
+    const uint32_t seed = (uint32_t)prm[P_SEED], stepno = (uint32_t)prm[P_STEPNO];
+    const float ratio = (float)st->s[S_MASS_RATIO];
+    const float tmax = prm[P_TMAX];
+    const float e_burn = expf(-prm[P_BURN] * dt);
+    const float e_cool = expf(-prm[P_COOL] * dt);
+    const float e_diss = expf(-prm[P_DDISS] * dt);
+    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
+    for (int z = blockIdx.z * zc; z < z1; ++z) {
+        const size_t i = idx3(N, z, y, x);
+        if (fm[i] == 0) {
+            out[i] = 0.0f; out[n3 + i] = 0.0f; out[2 * n3 + i] = 0.0f; out[3 * n3 + i] = 0.0f;
+            out[4 * n3 + i] = amb; out[5 * n3 + i] = 0.0f;
+            continue;
+        }
+        const float px = ((float)x + 0.5f) * dx, py = ((float)y + 0.5f) * dx, pz = ((float)z + 0.5f) * dx;
+        float d = adv[i] * ratio;
+        float vx = adv[n3 + i], vy = adv[2 * n3 + i], vz = adv[3 * n3 + i];
+        float T = adv[4 * n3 + i], Fu = adv[5 * n3 + i];
+        for (int e = 0; e < n_em; ++e) {
+            const float* b = prm + P_EM0 + e * EM_STRIDE;
+            const float rad = b[3];
+            const float ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
+            const float r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
+            if (r2 < 1.0f) {
+                const float w = (1.0f - r2) * (1.0f - r2);
+                float drate = b[4], frate = b[6];
+                const float te = b[5];
+                float nu = 1.0f;
+                if (noisy) {
+                    const uint32_t u = philox_u32(seed, (uint32_t)e, (uint32_t)i, stepno);
+                    nu = nlo + (nhi - nlo) * ((float)(u >> 8) * (1.0f / 16777216.0f));
+                    drate = drate * nu;
+                    frate = frate * nu;
+                }
+                d += drate * w * dt;
+                Fu += frate * w * dt;
+                if (te > 0.0f) {
+                    const float tex = noisy ? (te - amb) * nu : (te - amb);
+                    const float tt = amb + tex * w;
+                    T = T > tt ? T : tt;
+                }
+                const float evx = b[7], evy = b[8], evz = b[9];
+                if (evx * evx + evy * evy + evz * evz > 0.0f) {
+                    vx = vx + (evx - vx) * w;
+                    vy = vy + (evy - vy) * w;
+                    vz = vz + (evz - vz) * w;
+                }
+            }
+        }
+        for (int m = 0; m < n_imp; ++m) {
+            const float* b = prm + P_IMP0 + m * IMP_STRIDE;
+            const float rad = b[3];
+            const float ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
+            const float r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
+            if (r2 < 9.0f) {
+                const float gk = expf(-r2);
+                vx = vx + gk * b[4];
+                vy = vy + gk * b[5];
+                vz = vz + gk * b[6];
+            }
+        }
+        const float ign = clampf_((T - prm[P_IGN]) / ramp, 0.0f, 1.0f);
+        const float burned = Fu * (1.0f - e_burn) * ign;
+        Fu -= burned;
+        const float room = clampf_((tmax - T) / (tmax - amb), 0.0f, 1.0f);
+        T += prm[P_BHEAT] * burned * room;
+        d += prm[P_SMOKE] * burned;
+        T = amb + (T - amb) * e_cool;
+        d = d * e_diss;
+        out[i] = d > 0.0f ? d : 0.0f;
+        out[n3 + i] = vx;
+        out[2 * n3 + i] = vy;
+        out[3 * n3 + i] = vz;
+        out[4 * n3 + i] = T < tmax ? T : tmax;
+        out[5 * n3 + i] = Fu > 0.0f ? Fu : 0.0f;
+    }
+}
+
+__device__ __forceinline__ bool solid_at(const uint8_t* __restrict__ fm, int N, int z, int y, int x) {
+    if (x < 0 || y < 0 || z < 0 || x >= N || y >= N || z >= N) return true;
+    return fm[idx3(N, z, y, x)] == 0;
+}
+
+// diffuse_heat_fuel (kernels.py:268-286): channels 0..3 copied, T and F one explicit substep.
+__global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const float* __restrict__ src,
+                                                 const uint8_t* __restrict__ fm, const float* __restrict__ prm,
+                                                 float nsub, float* __restrict__ dst) {
+    const size_t n3 = (size_t)N * N * N;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= N || y >= N) return;
+    const float dx = prm[P_DX];
+    float lam = prm[P_KAPPA] * prm[P_DT] / (nsub * dx * dx);
+    lam = lam < LAM_CAP ? lam : LAM_CAP;
+    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
+    for (int z = blockIdx.z * zc; z < z1; ++z) {
+        const size_t i = idx3(N, z, y, x);
+        dst[i] = src[i];
+        dst[n3 + i] = src[n3 + i];
+        dst[2 * n3 + i] = src[2 * n3 + i];
+        dst[3 * n3 + i] = src[3 * n3 + i];
+        const bool solid = fm[i] == 0;
+        const bool sxm = solid_at(fm, N, z, y, x - 1), sxp = solid_at(fm, N, z, y, x + 1);
+        const bool sym = solid_at(fm, N, z, y - 1, x), syp = solid_at(fm, N, z, y + 1, x);
+        const bool szm = solid_at(fm, N, z - 1, y, x), szp = solid_at(fm, N, z + 1, y, x);
+        for (int c = 4; c <= 5; ++c) {
+            const float* f = src + c * n3;
+            const float v = f[i];
+            if (solid || lam <= 0.0f) {
+                dst[c * n3 + i] = v;
+            } else {
+                const float s = (sxm ? v : f[i - 1]) + (sxp ? v : f[i + 1]) + (sym ? v : f[i - N]) +
+                                (syp ? v : f[i + N]) + (szm ? v : f[i - (size_t)N * N]) +
+                                (szp ? v : f[i + (size_t)N * N]);
+                dst[c * n3 + i] = v + lam * (s - 6.0f * v);
+            }
+        }
+    }
+}
+
+// curl (kernels.py:297-310): walls and solids count as zero velocity.
+__global__ void __launch_bounds__(256) k_curl(int N, int zc, const float* __restrict__ data,
+                                              const uint8_t* __restrict__ fm, float inv2dx,
+                                              float* __restrict__ w) {
+    const size_t n3 = (size_t)N * N * N;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= N || y >= N) return;
+    const float* vx = data + n3;
+    const float* vy = data + 2 * n3;
+    const float* vz = data + 3 * n3;
+    const size_t sz = (size_t)N * N;
+    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
+    for (int z = blockIdx.z * zc; z < z1; ++z) {
+        const size_t i = idx3(N, z, y, x);
+        const bool sxm = solid_at(fm, N, z, y, x - 1), sxp = solid_at(fm, N, z, y, x + 1);
+        const bool sym = solid_at(fm, N, z, y - 1, x), syp = solid_at(fm, N, z, y + 1, x);
+        const bool szm = solid_at(fm, N, z - 1, y, x), szp = solid_at(fm, N, z + 1, y, x);
+        const float dwdy = (syp ? 0.0f : vz[i + N]) - (sym ? 0.0f : vz[i - N]);
+        const float dvdz = (szp ? 0.0f : vy[i + sz]) - (szm ? 0.0f : vy[i - sz]);
+        const float dudz = (szp ? 0.0f : vx[i + sz]) - (szm ? 0.0f : vx[i - sz]);
+        const float dwdx = (sxp ? 0.0f : vz[i + 1]) - (sxm ? 0.0f : vz[i - 1]);
+        const float dvdx = (sxp ? 0.0f : vy[i + 1]) - (sxm ? 0.0f : vy[i - 1]);
+        const float dudy = (syp ? 0.0f : vx[i + N]) - (sym ? 0.0f : vx[i - N]);
+        w[i] = (dwdy - dvdz) * inv2dx;
+        w[n3 + i] = (dudz - dwdx) * inv2dx;
+        w[2 * n3 + i] = (dvdx - dudy) * inv2dx;
+    }
+}
+
+// sep_x (kernels.py:392-406): out[0:3] = [1,2,1]_x (w a), out[3] = [1,2,1]_x w.
+__global__ void __launch_bounds__(256) k_sep_x(int N, int zc, const float* __restrict__ a,
+                                               const uint8_t* __restrict__ fm, float* __restrict__ out) {
+    const size_t n3 = (size_t)N * N * N;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= N || y >= N) return;
+    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
+    for (int z = blockIdx.z * zc; z < z1; ++z) {
+        const size_t i = idx3(N, z, y, x);
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
+#pragma unroll
+        for (int k = -1; k <= 1; ++k) {
+            if (!solid_at(fm, N, z, y, x + k)) {
+                const float q = (float)(2 - (k < 0 ? -k : k));
+                s0 += q * a[i + k];
+                s1 += q * a[n3 + i + k];
+                s2 += q * a[2 * n3 + i + k];
+                ws += q;
+            }
+        }
+        out[i] = s0; out[n3 + i] = s1; out[2 * n3 + i] = s2; out[3 * n3 + i] = ws;
+    }
+}
+
+// sep_y (kernels.py:409-419)
+__global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const float* __restrict__ a, float* __restrict__ out) {
+    const size_t n3 = (size_t)N * N * N;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= N || y >= N) return;
+    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
+    for (int z = blockIdx.z * zc; z < z1; ++z) {
+        const size_t i = idx3(N, z, y, x);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float* f = a + c * n3;
+            float v = 2.0f * f[i];
+            if (y > 0) v += f[i - N];
+            if (y + 1 < N) v += f[i + N];
+            out[c * n3 + i] = v;
+        }
+    }
+}
+
+// sep_z_ratio + sep_z_store / sep_z_apply (kernels.py:422-457).  mode 0: out (3ch) = ratio;
+// mode 1: fluid cells of the state get velocity += dt * ratio.
+__global__ void __launch_bounds__(256) k_sep_z(int N, int zc, const float* __restrict__ a,
+                                               const uint8_t* __restrict__ fm, const float* __restrict__ prm,
+                                               int mode, float* __restrict__ out) {
+    const size_t n3 = (size_t)N * N * N;
+    const size_t sz = (size_t)N * N;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= N || y >= N) return;
+    const float dt = prm[P_DT];
+    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
+    for (int z = blockIdx.z * zc; z < z1; ++z) {
+        const size_t i = idx3(N, z, y, x);
+        if (mode == 1 && fm[i] == 0) continue;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
+#pragma unroll
+        for (int k = -1; k <= 1; ++k) {
+            const int zz = z + k;
+            if (zz >= 0 && zz < N) {
+                const float q = (float)(2 - (k < 0 ? -k : k));
+                const size_t j = i + (ptrdiff_t)k * (ptrdiff_t)sz;
+                s0 += q * a[j];
+                s1 += q * a[n3 + j];
+                s2 += q * a[2 * n3 + j];
+                ws += q * a[3 * n3 + j];
+            }
+        }
+        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+        if (ws > 0.0f) { r0 = s0 / ws; r1 = s1 / ws; r2 = s2 / ws; }
+        if (mode == 0) {
+            out[i] = r0; out[n3 + i] = r1; out[2 * n3 + i] = r2;
+        } else {
+            out[n3 + i] = out[n3 + i] + dt * r0;
+            out[2 * n3 + i] = out[2 * n3 + i] + dt * r1;
+            out[3 * n3 + i] = out[3 * n3 + i] + dt * r2;
+        }
+    }
+}
+
+__device__ __forceinline__ float wmag(const float* __restrict__ o, size_t n3, size_t i) {
+    const float a = o[i], b = o[n3 + i], c = o[2 * n3 + i];
+    return sqrtf(a * a + b * b + c * c);
+}
+
+// body_forces (kernels.py:346-387): buoyancy + vorticity confinement + wind marker force.
+__global__ void __launch_bounds__(256) k_body_forces(int N, int zc, const float* __restrict__ f,
+                                                     const uint8_t* __restrict__ fm, const float* __restrict__ om,
+                                                     const float* __restrict__ prm, float* __restrict__ force) {
+    const size_t n3 = (size_t)N * N * N;
+    const size_t sz = (size_t)N * N;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= N || y >= N) return;
+    const float amb = prm[P_AMB], alpha = prm[P_ALPHA], beta = prm[P_BETA], eps = prm[P_EPS];
+    const float cfloor = prm[P_CFLOOR], dxp = prm[P_DX];
+    const float wdx = prm[P_WIND_X], wdy = prm[P_WIND_Y], wdz = prm[P_WIND_Z], wmark = prm[P_WMARK];
+    const bool wind = wdx * wdx + wdy * wdy + wdz * wdz > 0.0f;
+    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
+    for (int z = blockIdx.z * zc; z < z1; ++z) {
+        const size_t i = idx3(N, z, y, x);
+        if (fm[i] == 0) {
+            force[i] = 0.0f; force[n3 + i] = 0.0f; force[2 * n3 + i] = 0.0f;
+            continue;
+        }
+        const float d = f[i], T = f[4 * n3 + i];
+        float fx = 0.0f, fy = -alpha * d + beta * (T - amb), fz = 0.0f;
+        if (eps != 0.0f) {
+            const int xr = min(x + 1, N - 1), xl = max(x - 1, 0);
+            const int yt = min(y + 1, N - 1), yb = max(y - 1, 0);
+            const int zf = min(z + 1, N - 1), zb = max(z - 1, 0);
+            const size_t rowb = (size_t)z * sz + (size_t)y * N;
+            const float gx = (wmag(om, n3, rowb + xr) - wmag(om, n3, rowb + xl)) / (float)max(xr - xl, 1);
+            const float gy = (wmag(om, n3, (size_t)z * sz + (size_t)yt * N + x) -
+                              wmag(om, n3, (size_t)z * sz + (size_t)yb * N + x)) / (float)max(yt - yb, 1);
+            const float gz = (wmag(om, n3, (size_t)zf * sz + (size_t)y * N + x) -
+                              wmag(om, n3, (size_t)zb * sz + (size_t)y * N + x)) / (float)max(zf - zb, 1);
+            const float wx = om[i], wy = om[n3 + i], wz = om[2 * n3 + i];
+            const float fl = cfloor * sqrtf(wx * wx + wy * wy + wz * wz);
+            const float mag = sqrtf(gx * gx + gy * gy + gz * gz + fl * fl) + 1.0e-20f;
+            const float k = eps * dxp / mag;
+            fx += k * (gy * wz - gz * wy);
+            fy += k * (gz * wx - gx * wz);
+            fz += k * (gx * wy - gy * wx);
+        }
+        if (wind) {
+            const float mk = clampf_(d + f[5 * n3 + i] + (T - amb) / wmark, 0.0f, 1.0f);
+            fx += wdx * mk;
+            fy += wdy * mk;
+            fz += wdz * mk;
+        }
+        force[i] = fx; force[n3 + i] = fy; force[2 * n3 + i] = fz;
+    }
+}
+
+void launch_sources_diffuse(const Geom& g, const float* adv, const uint8_t* fm, const float* prm,
+                            const DevState* st, float* hat, float* data, hipStream_t s) {
+    // n_diff = max(1, ceil(N / 256)) substeps, ping-pong ending in data (solver.py:107-109,436-442)
+    const int nd = g.N > 256 ? (g.N + 255) / 256 : 1;
+    float* b0 = nd % 2 == 0 ? data : hat;
+    float* b1 = nd % 2 == 0 ? hat : data;
+    hipLaunchKernelGGL(k_sources, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, st, b0);
+    for (int k = 0; k < nd; ++k) {
+        float* a = k % 2 == 0 ? b0 : b1;
+        float* b = k % 2 == 0 ? b1 : b0;
+        hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, a, fm, prm, (float)nd, b);
+    }
+}
+
+void launch_vort_forces(const Geom& g, float* data, const uint8_t* fm, const float* prm, float inv2dx,
+                        float* omega, float* omega_s, float* force, float* tmp_a, float* tmp_b, hipStream_t s) {
+    hipLaunchKernelGGL(k_curl, g.grid, g.block, 0, s, g.N, g.zc, data, fm, inv2dx, omega);
+    hipLaunchKernelGGL(k_sep_x, g.grid, g.block, 0, s, g.N, g.zc, omega, fm, tmp_a);
+    hipLaunchKernelGGL(k_sep_y, g.grid, g.block, 0, s, g.N, g.zc, tmp_a, tmp_b);
+    hipLaunchKernelGGL(k_sep_z, g.grid, g.block, 0, s, g.N, g.zc, tmp_b, fm, prm, 0, omega_s);
+    hipLaunchKernelGGL(k_body_forces, g.grid, g.block, 0, s, g.N, g.zc, data, fm, omega_s, prm, force);
+    hipLaunchKernelGGL(k_sep_x, g.grid, g.block, 0, s, g.N, g.zc, force, fm, tmp_a);
+    hipLaunchKernelGGL(k_sep_y, g.grid, g.block, 0, s, g.N, g.zc, tmp_a, tmp_b);
+    hipLaunchKernelGGL(k_sep_z, g.grid, g.block, 0, s, g.N, g.zc, tmp_b, fm, prm, 1, data);
+}
+
+}  // namespace vfx

@@ -1,0 +1,130 @@
+// Shared definitions of the MI355X (gfx950) sim3d library: parameter-buffer slots, device
+// scalar slots, launch geometry, the collocated-grid helpers and block reductions.
+//
+// Layout of every field: (C, D, H, W) float32, index [c][z][y][x], x fastest (the reference's
+// FieldState3D, vfx/core/fields3d.py:1-10).  Masks are uint8 open-face bit sets.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace vfx {
+
+// ---------------------------------------------------------------- parameter buffer (floats)
+// Slot numbers of the reference (vfx/sim3d/kernels.py:11-42) plus extension slots in the
+// ranges the reference leaves unused.
+enum : int {
+    P_DT = 0, P_WIND_X = 1, P_WIND_Y = 2, P_WIND_Z = 3, P_N_EM = 4, P_N_IMP = 5, P_DX = 6,
+    P_AMB = 7, P_KAPPA = 8, P_CFLOOR = 9, P_WARM = 10,
+    P_IGN_RAMP = 11, P_NOISE_LO = 12, P_NOISE_HI = 13, P_SEED = 14, P_STEPNO = 15,
+    P_ALPHA = 16, P_BETA = 17, P_EPS = 18, P_DDISS = 19, P_COOL = 20, P_BURN = 21, P_IGN = 22,
+    P_BHEAT = 23, P_SMOKE = 24, P_TOL = 25, P_MAXIT = 26, P_TMAX = 27, P_WMARK = 28,
+    P_REL_TOL = 29,
+    P_EM0 = 32, EM_STRIDE = 10, MAX_EMITTERS = 32,
+    P_IMP0 = P_EM0 + MAX_EMITTERS * EM_STRIDE, IMP_STRIDE = 7, MAX_IMPULSES = 32,
+    P_SIZE = P_IMP0 + MAX_IMPULSES * IMP_STRIDE,
+};
+
+// ---------------------------------------------------------------- device scalar state
+// doubles written by the finalize kernels (the reference keeps the same quantities as float
+// slots, kernels.py:44-53); the CG inner products, alpha and beta are FP64 here.
+enum : int {
+    S_MASS_RATIO = 0, S_MAXU = 1, S_RZ = 2, S_ALPHA = 3, S_BETA = 4, S_MAXR = 5, S_THR = 6,
+    S_BEST = 7, S_BB = 8, S_RR = 9, S_TRUE_RR = 10, S_COUNT = 16,
+};
+
+struct DevState {
+    double s[S_COUNT];
+    int it;        // CG iterations of the current / last projection
+    int stop_at;   // iteration at which the stop test fired (INT_MAX while running)
+    int changed;   // solid differs from the cached face mask -> rebuild operators
+    int pad;
+};
+
+constexpr int FLUID = 64;          // face-mask fluid bit (pressure.py:32-33)
+constexpr float LAM_CAP = 0.16f;   // kernels.py:63
+constexpr float HS = 0.35355339059327373f;   // 1/sqrt(8), pressure.py:30
+constexpr float OMEGA_JAC = 0.8f;  // pressure.py:785
+constexpr float GALERKIN_SCALE = 1.0f / 16.0f;   // pressure.py:783
+
+// ---------------------------------------------------------------- launch geometry
+// Cell kernels use 256-thread blocks of BX x (256/BX) threads over (x, y) and walk ZC z-planes
+// per thread, so a wavefront reads whole x-rows (coalesced) and a block reuses y-neighbours.
+struct Geom {
+    int N;
+    int bx, by, zc;
+    dim3 grid, block;
+    size_t n3;
+};
+
+Geom make_geom(int N, int zc = 1);
+inline int nblocks(const Geom& g) { return (int)(g.grid.x * g.grid.y * g.grid.z); }
+
+__device__ __forceinline__ size_t idx3(int N, int z, int y, int x) {
+    return ((size_t)z * N + y) * N + x;
+}
+
+__device__ __forceinline__ int block_linear() {
+    return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+
+// ---------------------------------------------------------------- reductions
+// Fixed-order block reductions: wavefront (64 lanes) butterfly via DPP/shuffles, then LDS across
+// the waves of the block.  Deterministic for a fixed launch shape.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum over the block; result valid in thread 0.  `sh` needs blockDim/64 doubles.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (t == 0)
+        for (int w = 0; w < nw; ++w) r += sh[w];
+    return r;
+}
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
+    v = wave_max(v);
+    __syncthreads();
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (t == 0)
+        for (int w = 0; w < nw; ++w) r = fmax(r, sh[w]);
+    return r;
+}
+
+// Reduce `n` block partials in a fixed order inside one 256-thread block; result in thread 0.
+__device__ __forceinline__ double reduce_parts_sum(const double* p, int n, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
+    return block_sum(v, sh);
+}
+__device__ __forceinline__ double reduce_parts_max(const double* p, int n, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, p[i]);
+    return block_max(v, sh);
+}
+
+// ---------------------------------------------------------------- error plumbing
+void set_error(const std::string& msg);
+bool hip_ok(hipError_t e, const char* what);
+bool launch_ok(const char* what);
+
+}  // namespace vfx

@@ -1,0 +1,68 @@
+// Host-side launchers and the per-resolution solver objects of the sim3d library.
+#pragma once
+
+#include <cmath>
+
+#include "sim3d_common.hpp"
+
+namespace vfx {
+
+// One multigrid level: m^3 blocks x 8 Hadamard components (component-fastest storage).
+struct Lvl {
+    int m;
+    float dmin;   // diag <= dmin -> x = 0 (pressure.py:803)
+    float* x;     // [m^3][8]
+    float* r;     // [m^3][8]
+    float* diag;  // [m^3][8]
+    float* off;   // [m^3][24]: ax*8 + c couples to the +ax neighbour
+};
+
+constexpr int MAX_LEVELS = 16;
+
+struct CoarseLevels {
+    int n;
+    Lvl lv[MAX_LEVELS];
+};
+
+// ---------------------------------------------------------------- advection / forces
+void launch_advect(const Geom& g, int C, const float* src, const float* vel, const float* prm, float inv_dx,
+                   float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s);
+void launch_mass_fin(const Geom& g, const double* part, DevState* st, hipStream_t s);
+void launch_sources_diffuse(const Geom& g, const float* adv, const uint8_t* fm, const float* prm,
+                            const DevState* st, float* hat, float* data, hipStream_t s);
+void launch_vort_forces(const Geom& g, float* data, const uint8_t* fm, const float* prm, float inv2dx,
+                        float* omega, float* omega_s, float* force, float* tmp_a, float* tmp_b, hipStream_t s);
+
+// ---------------------------------------------------------------- projection
+void launch_div_abs(const Geom& g, const float* vel, const float* solid, float* out, hipStream_t s);
+void launch_kmul(const Geom& g, const float* p, const uint8_t* fm, float* out, hipStream_t s);
+
+// Buffers and launch sequence of the MG-preconditioned CG projection for one resolution
+// (PressureSolver3D, pressure.py:779-956).
+struct Pressure {
+    int N = 0;
+    int nlev = 0;
+    int S = 0;            // first level handled by the single-workgroup coarse kernel
+    int coarse_m = 8;     // levels with m <= coarse_m run inside that kernel
+    Lvl lv[MAX_LEVELS];
+    Geom geom;
+    uint8_t* fm = nullptr;
+    float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *phi = nullptr;
+    float *z0 = nullptr, *rr = nullptr, *zt = nullptr;
+    double* part = nullptr;      // 3 * nparts
+    double* part_pq = nullptr;   // nparts
+    int nparts = 0;
+    DevState* st = nullptr;
+
+    void init_levels(int N);
+    size_t level_floats() const;
+    void bind_levels(float* base);
+    void setup(const float* solid, hipStream_t s);
+    void rhs(const float* vel, const float* prm, hipStream_t s);
+    void precond(int k, hipStream_t s);
+    void iteration(int k, const float* prm, hipStream_t s);
+    void apply(float* vel, hipStream_t s);
+    void true_residual(const float* vel, hipStream_t s);
+};
+
+}  // namespace vfx

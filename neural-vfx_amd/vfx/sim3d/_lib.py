@@ -1,0 +1,82 @@
+"""ctypes binding of libvfxsim3d.so (C ABI in include/vfxsim3d.h).
+
+The library must be built (neural-vfx_amd/vfx/sim3d/_build.py); there is no fallback path.
+torch is imported first so the library resolves to the HIP runtime torch already loaded.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads torch's HIP runtime before the library binds to it)
+
+from ._build import LIB_PATH
+
+VFX_OK, VFX_E_ARG, VFX_E_VALUE, VFX_E_HIP = 0, 1, 2, 3
+STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
+OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M = 1, 2, 3
+
+# every entry point of include/vfxsim3d.h with its argument types
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+SIGNATURES = {
+    "vfx_p_size": [],
+    "vfx_ctx_create": [_I, ctypes.POINTER(_P)],
+    "vfx_ctx_destroy": [_P],
+    "vfx_step": [_P, _P, _P, _P, _P],
+    "vfx_step_stage": [_P, _I, _P, _P, _P, _P],
+    "vfx_project": [_P, _P, _P, _P, _P],
+    "vfx_advect": [_P, _P, _I, _P, _P, ctypes.c_float, _P],
+    "vfx_div_abs": [_P, _P, _P, _P, _P],
+    "vfx_pressure_iters": [_P, ctypes.POINTER(_I)],
+    "vfx_pressure_stats": [_P, ctypes.POINTER(ctypes.c_double), _I],
+    "vfx_set_option": [_P, _I, ctypes.c_double],
+    "vfx_get_phi": [_P, _P, _P],
+    "vfx_set_phi": [_P, _P, _P],
+    "vfx_get_advected": [_P, _P, ctypes.POINTER(ctypes.c_double), _P],
+    "vfx_kmul": [_P, _P, _P, _P, _P],
+    "vfx_precond": [_P, _P, _P, _P, _P],
+    "vfx_get_ops": [_P, _P, _I, _P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I), _P],
+    "vfx_last_error": [ctypes.c_char_p, _I],
+}
+
+_lib = None
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load libvfxsim3d.so; raises ImportError if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                          f"(python neural-vfx_amd/vfx/sim3d/_build.py) first")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    load().vfx_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str = "") -> None:
+    """Map a status code to the reference's exception types (solver.py:311-315, :347-348)."""
+    if rc == VFX_OK:
+        return
+    msg = f"{what}: {last_error()}" if what else last_error()
+    if rc == VFX_E_ARG:
+        raise AssertionError(msg)
+    if rc == VFX_E_VALUE:
+        raise ValueError(msg)
+    raise SolverError(msg)

@@ -1,0 +1,279 @@
+"""numpy/ctypes front end of the C oracle (oracle/sim3d_oracle.c): a CPU restatement of the
+reference vfx.sim3d step in FP32 and FP64.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+use it, as the checker (and as the timed CPU path); the product never imports it.
+
+Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
+  advect          closed forms (constant field, integer shift, linear field), scipy
+                  map_coordinates at the traced points, clamp bounds, multichannel = per-channel
+  mass_ratio      closed form (numpy sums)
+  sources         per-cell closed forms of each term in isolation (emitter footprint, impulse,
+                  combustion, cooling, dissipation, solid reset), Philox known-answer vectors
+  diffuse         conservation, constant fixed point, single-spike stencil, lambda cap
+  curl            exact curl of linear fields in the interior; solid / wall zero rule
+  smooth3         equals the reference's own 27-tap form (binom3) and a numpy scatter form
+  body_forces     buoyancy closed form; confinement vs a numpy restatement on a field with
+                  known gradient; wind marker closed form
+  mask/kmul/div2  K assembled column by column equals D D^T with D assembled independently;
+                  symmetric, PSD, constant null space; pure-gradient removal
+  probe/galerkin  T K T^T of the assembled K (component-diagonal blocks), Galerkin formula from
+                  explicit prolongation matrices
+  precond         symmetric M^-1, positive definite, error reduction per application
+  project         exact FP64 sparse solve (scipy) of the same system; d(u) = 0 to tolerance
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle (gcc, OpenMP, no FP contraction)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _dt(prec: str):
+    return np.float32 if prec == "f32" else np.float64
+
+
+def _a(x, dtype):
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+def _p(x):
+    return x.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fn(name: str, prec: str, restype=None):
+    f = getattr(_load(), f"{name}_{prec}" if prec else name)
+    f.restype = restype
+    return f
+
+
+def _real(prec):
+    return ctypes.c_float if prec == "f32" else ctypes.c_double
+
+
+def p_size() -> int:
+    return int(_load().or_p_size())
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    _load().or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def advect(src, vel, s, prec="f32"):
+    dt = _dt(prec)
+    src = _a(src, dt)
+    vel = _a(vel, dt)
+    C, N = src.shape[0], src.shape[-1]
+    hat = np.empty_like(src)
+    out = np.empty_like(src)
+    _fn("or_advect", prec)(ctypes.c_int(N), ctypes.c_int(C), _p(src), _p(vel), _real(prec)(s), _p(hat), _p(out))
+    return out
+
+
+def mass_ratio(pre, post, solid, prec="f32"):
+    dt = _dt(prec)
+    pre, post, solid = _a(pre, dt), _a(post, dt), _a(solid, dt)
+    return float(_fn("or_mass_ratio", prec, ctypes.c_double)(ctypes.c_int(solid.shape[-1]), _p(pre), _p(post),
+                                                              _p(solid)))
+
+
+def sources(adv, solid, prm, ratio, prec="f32"):
+    dt = _dt(prec)
+    adv, solid = _a(adv, dt), _a(solid, dt)
+    prm = _a(prm, np.float32)
+    out = np.empty_like(adv)
+    _fn("or_sources", prec)(ctypes.c_int(solid.shape[-1]), _p(adv), _p(solid), _p(prm), _real(prec)(ratio), _p(out))
+    return out
+
+
+def diffuse(src, solid, prm, nsub=1.0, prec="f32"):
+    dt = _dt(prec)
+    src, solid = _a(src, dt), _a(solid, dt)
+    prm = _a(prm, np.float32)
+    out = np.empty_like(src)
+    _fn("or_diffuse", prec)(ctypes.c_int(solid.shape[-1]), _p(src), _p(solid), _p(prm), _real(prec)(nsub), _p(out))
+    return out
+
+
+def curl(data, solid, inv2dx, prec="f32"):
+    dt = _dt(prec)
+    data, solid = _a(data, dt), _a(solid, dt)
+    N = solid.shape[-1]
+    w = np.empty((3, N, N, N), dt)
+    _fn("or_curl", prec)(ctypes.c_int(N), _p(data), _p(solid), _real(prec)(inv2dx), _p(w))
+    return w
+
+
+def smooth3(a, solid, mode=0, dt_=0.0, out=None, prec="f32"):
+    """mode 0: returns the smoothed 3 channels; mode 1: adds dt * smoothed to out[1:4] (a copy)."""
+    dt = _dt(prec)
+    a, solid = _a(a, dt), _a(solid, dt)
+    N = solid.shape[-1]
+    ta = np.empty((4, N, N, N), dt)
+    tb = np.empty((4, N, N, N), dt)
+    o = np.empty((3, N, N, N), dt) if mode == 0 else _a(out, dt).copy()
+    _fn("or_smooth3", prec)(ctypes.c_int(N), _p(a), _p(solid), _p(ta), _p(tb), ctypes.c_int(mode), _real(prec)(dt_),
+                            _p(o))
+    return o
+
+
+def binom3(a, solid, prec="f32"):
+    dt = _dt(prec)
+    a, solid = _a(a, dt), _a(solid, dt)
+    N = solid.shape[-1]
+    o = np.empty((3, N, N, N), dt)
+    _fn("or_binom3", prec)(ctypes.c_int(N), _p(a), _p(solid), _p(o))
+    return o
+
+
+def body_forces(f, solid, om, prm, prec="f32"):
+    dt = _dt(prec)
+    f, solid, om = _a(f, dt), _a(solid, dt), _a(om, dt)
+    prm = _a(prm, np.float32)
+    N = solid.shape[-1]
+    force = np.empty((3, N, N, N), dt)
+    _fn("or_body_forces", prec)(ctypes.c_int(N), _p(f), _p(solid), _p(om), _p(prm), _p(force))
+    return force
+
+
+def mask(solid, prec="f32"):
+    solid = _a(solid, _dt(prec))
+    fm = np.empty(solid.shape, np.int32)
+    _fn("or_mask", prec)(ctypes.c_int(solid.shape[-1]), _p(solid), _p(fm))
+    return fm
+
+
+def kmul(fm, p, prec="f32"):
+    fm = _a(fm, np.int32)
+    p = _a(p, _dt(prec))
+    out = np.empty_like(p)
+    _fn("or_kmul", prec)(ctypes.c_int(fm.shape[-1]), _p(fm), _p(p), _p(out))
+    return out
+
+
+def div2(fm, vel3, prec="f32"):
+    fm = _a(fm, np.int32)
+    vel3 = _a(vel3, _dt(prec))
+    out = np.empty(fm.shape, _dt(prec))
+    _fn("or_div2", prec)(ctypes.c_int(fm.shape[-1]), _p(fm), _p(vel3), _p(out))
+    return out
+
+
+def apply(fm, phi, vel3, prec="f32"):
+    fm = _a(fm, np.int32)
+    phi = _a(phi, _dt(prec))
+    v = _a(vel3, _dt(prec)).copy()
+    _fn("or_apply", prec)(ctypes.c_int(fm.shape[-1]), _p(fm), _p(phi), _p(v))
+    return v
+
+
+def div_abs(fm, vel3, prec="f32"):
+    fm = _a(fm, np.int32)
+    vel3 = _a(vel3, _dt(prec))
+    out = np.empty(fm.shape, _dt(prec))
+    _fn("or_div_abs", prec)(ctypes.c_int(fm.shape[-1]), _p(fm), _p(vel3), _p(out))
+    return out
+
+
+def level_sizes(N: int):
+    """m of every multigrid level (pressure.py:799-809)."""
+    ms = []
+    m = N // 2
+    while True:
+        ms.append(m)
+        if m % 2 == 1 or m == 1:
+            break
+        m //= 2
+    return ms
+
+
+def build_ops(fm, prec="f32"):
+    """[(diag (8,m,m,m), off (24,m,m,m)), ...] per level, reference layout."""
+    fm = _a(fm, np.int32)
+    N = fm.shape[-1]
+    ms = level_sizes(N)
+    total = sum(32 * m ** 3 for m in ms)
+    buf = np.empty(total, _dt(prec))
+    n = _fn("or_build_ops", prec, ctypes.c_int)(ctypes.c_int(N), _p(fm), _p(buf))
+    assert n == len(ms)
+    out, o = [], 0
+    for m in ms:
+        d = buf[o:o + 8 * m ** 3].reshape(8, m, m, m); o += 8 * m ** 3
+        f = buf[o:o + 24 * m ** 3].reshape(24, m, m, m); o += 24 * m ** 3
+        out.append((d, f))
+    return out
+
+
+def precond_apply(fm, r, prec="f32"):
+    fm = _a(fm, np.int32)
+    r = _a(r, _dt(prec))
+    z = np.empty_like(r)
+    rz = _fn("or_precond_apply", prec, ctypes.c_double)(ctypes.c_int(fm.shape[-1]), _p(fm), _p(r), _p(z))
+    return z, float(rz)
+
+
+def project(data, solid, prm, phi, prec="f32"):
+    """Project data[1:4] in place; phi (warm start) updated in place.  Returns (iters, stats)."""
+    assert data.dtype == _dt(prec) and data.flags.c_contiguous
+    assert phi.dtype == _dt(prec) and phi.flags.c_contiguous
+    solid = _a(solid, _dt(prec))
+    prm = _a(prm, np.float32)
+    stats = np.zeros(3, np.float64)
+    it = _fn("or_project", prec, ctypes.c_int)(ctypes.c_int(solid.shape[-1]), _p(data), _p(solid), _p(prm), _p(phi),
+                                               _p(stats))
+    return int(it), {"b_norm": stats[0], "r_norm": stats[1], "max_u": stats[2]}
+
+
+def stage_advect(data, solid, prm, inv_dx, prec="f32"):
+    dt = _dt(prec)
+    data, solid = _a(data, dt), _a(solid, dt)
+    prm = _a(prm, np.float32)
+    adv = np.empty_like(data)
+    ratio = _fn("or_stage_advect", prec, ctypes.c_double)(ctypes.c_int(solid.shape[-1]), _p(data), _p(solid), _p(prm),
+                                                           _real(prec)(inv_dx), _p(adv))
+    return adv, float(ratio)
+
+
+def stage_forces(adv, solid, prm, ratio, inv_dx, prec="f32"):
+    dt = _dt(prec)
+    adv, solid = _a(adv, dt), _a(solid, dt)
+    prm = _a(prm, np.float32)
+    data = np.empty_like(adv)
+    r = np.float32(ratio) if prec == "f32" else float(ratio)
+    _fn("or_stage_forces", prec)(ctypes.c_int(solid.shape[-1]), _p(adv), _p(solid), _p(prm), _real(prec)(r),
+                                 _real(prec)(inv_dx), _p(data))
+    return data
+
+
+def step(data, solid, prm, phi, inv_dx, prec="f32"):
+    """One whole step (advect, forces, project); returns (new data, iters)."""
+    adv, ratio = stage_advect(data, solid, prm, inv_dx, prec)
+    out = stage_forces(adv, solid, prm, ratio, inv_dx, prec)
+    it, _ = project(out, solid, prm, phi, prec)
+    return out, it
