@@ -93,6 +93,28 @@ int vfx_precond(vfx_ctx* ctx, const float* solid, const float* r, float* z, void
 int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float* off, int* nlev, int* m,
                 void* stream);
 
+/* kernel ids of vfx_time_kernel */
+#define VFX_K_ADV_SL 0     /* semi-Lagrangian pass of the advection */
+#define VFX_K_ADV_MC 1     /* MacCormack pass + clamp + mass partials */
+#define VFX_K_SOURCES 2
+#define VFX_K_DIFFUSE 3
+#define VFX_K_MATVEC 4     /* q = K p and p.q */
+#define VFX_K_POST_RZ 5    /* post-Jacobi of the preconditioner + r.z, max|r|, r.r */
+#define VFX_K_SMOOTH0 6    /* one colour of GS on the finest multigrid level */
+#define VFX_K_RESTRICT0 7  /* residual restriction finest -> next level */
+#define VFX_K_PROLONG0 8   /* prolongation + black sweep on the finest level */
+#define VFX_K_PRE 9        /* pre-Jacobi of the preconditioner */
+#define VFX_K_RESID 10     /* Jacobi residual r - K z0 */
+#define VFX_K_FWD 11       /* Hadamard transform to level 1 + red sweep */
+#define VFX_K_INV 12       /* inverse transform */
+#define VFX_K_UPD_P 13     /* p / phi update */
+#define VFX_K_COARSE 14    /* single-workgroup coarse V-cycle */
+
+/* Average device time (ms) of `reps` back-to-back launches of one kernel on the solver's current
+ * buffers, bracketed by HIP events on `stream` (measurement support for the benchmark; call it
+ * after a step so the buffers hold live data).  Leaves the solver state as it found it. */
+int vfx_time_kernel(vfx_ctx* ctx, int which, int reps, void* stream, float* ms_avg);
+
 /* Message of the last failed call on this thread. */
 int vfx_last_error(char* buf, int n);
 

@@ -11,9 +11,11 @@ __device__ __forceinline__ float clampf_(float g, float lo, float hi) {
     return g < lo ? lo : (g > hi ? hi : g);
 }
 
-// base_idx / frac (kernels.py:67-74): clamp to [0, n-1], floor, keep one cell of room.
+// base_idx / frac (kernels.py:67-74): clamp to [0, n-1], floor, keep one cell of room.  The
+// clamp uses fmaxf/fminf so that a NaN coordinate also lands on cell 0 (equal to the reference's
+// clamp for every other input).
 __device__ __forceinline__ int base_idx(float g, int n) {
-    int i = (int)floorf(clampf_(g, 0.0f, (float)(n - 1)));
+    int i = (int)floorf(fminf(fmaxf(g, 0.0f), (float)(n - 1)));
     return i < n - 2 ? i : n - 2;
 }
 
@@ -150,6 +152,15 @@ void launch_advect(const Geom& g, int C, const float* src, const float* vel, con
     hipLaunchKernelGGL(k_adv_sl, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat);
     hipLaunchKernelGGL(k_adv_mc, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat, out, fm, part,
                        nblocks(g));
+}
+
+void launch_adv_pass(int pass, const Geom& g, int C, const float* src, const float* vel, const float* prm,
+                     float inv_dx, float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
+    if (pass == 0)
+        hipLaunchKernelGGL(k_adv_sl, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat);
+    else
+        hipLaunchKernelGGL(k_adv_mc, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat, out, fm, part,
+                           nblocks(g));
 }
 
 void launch_mass_fin(const Geom& g, const double* part, DevState* st, hipStream_t s) {

@@ -480,6 +480,55 @@ int vfx_precond(vfx_ctx* c, const float* solid, const float* r, float* z, void* 
     return VFX_OK;
 }
 
+int vfx_time_kernel(vfx_ctx* c, int which, int reps, void* stream, float* ms_avg) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!ms_avg || reps < 1) return arg_error("bad arguments");
+    hipStream_t s = S(stream);
+    DevState saved;
+    CHECK_HIP(hipMemcpyAsync(&saved, c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "state save");
+    CHECK_HIP(hipStreamSynchronize(s), "state save");
+    hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
+    hipEvent_t e0, e1;
+    CHECK_HIP(hipEventCreate(&e0), "event");
+    CHECK_HIP(hipEventCreate(&e1), "event");
+    // advection is timed on the advected copy of the last step (adv -> hat -> omega..omega_s, which
+    // are contiguous 6-channel scratch); the projection kernels on the solver's own buffers
+    auto one = [&](void) -> bool {
+        const size_t n3 = c->g.n3;
+        switch (which) {
+            case VFX_K_ADV_SL:
+                launch_adv_pass(0, c->g, 6, c->adv, c->adv + n3, c->prm, c->inv_dx, c->hat, nullptr, nullptr,
+                                nullptr, s);
+                return true;
+            case VFX_K_ADV_MC:
+                launch_adv_pass(1, c->g, 6, c->adv, c->adv + n3, c->prm, c->inv_dx, c->hat, c->omega, c->P.fm,
+                                c->mass_part, s);
+                return true;
+            case VFX_K_SOURCES: launch_sources_pass(0, c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, s); return true;
+            case VFX_K_DIFFUSE: launch_sources_pass(1, c->g, c->hat, c->P.fm, c->prm, c->st, c->omega, s); return true;
+            default: return which >= VFX_K_MATVEC && c->P.launch_single(which - VFX_K_MATVEC, s);
+        }
+    };
+    if (!one()) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        set_error("unknown kernel id (or a level the grid does not have)");
+        return VFX_E_VALUE;
+    }
+    CHECK_HIP(hipEventRecord(e0, s), "event");
+    for (int i = 0; i < reps; ++i) one();
+    CHECK_HIP(hipEventRecord(e1, s), "event");
+    CHECK_LAUNCH("timed kernel");
+    CHECK_HIP(hipEventSynchronize(e1), "event");
+    float ms = 0.0f;
+    CHECK_HIP(hipEventElapsedTime(&ms, e0, e1), "event");
+    *ms_avg = ms / (float)reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    CHECK_HIP(hipMemcpy(c->st, &saved, sizeof(DevState), hipMemcpyHostToDevice), "state restore");
+    return VFX_OK;
+}
+
 int vfx_get_ops(vfx_ctx* c, const float* solid, int level, float* diag, float* off, int* nlev, int* m,
                 void* stream) {
     if (int rc = check_ctx(c)) return rc;

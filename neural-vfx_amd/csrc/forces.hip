@@ -346,6 +346,14 @@ void launch_sources_diffuse(const Geom& g, const float* adv, const uint8_t* fm, 
     }
 }
 
+void launch_sources_pass(int pass, const Geom& g, const float* src, const uint8_t* fm, const float* prm,
+                         const DevState* st, float* dst, hipStream_t s) {
+    if (pass == 0)
+        hipLaunchKernelGGL(k_sources, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, st, dst);
+    else
+        hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, 1.0f, dst);
+}
+
 void launch_vort_forces(const Geom& g, float* data, const uint8_t* fm, const float* prm, float inv2dx,
                         float* omega, float* omega_s, float* force, float* tmp_a, float* tmp_b, hipStream_t s) {
     hipLaunchKernelGGL(k_curl, g.grid, g.block, 0, s, g.N, g.zc, data, fm, inv2dx, omega);

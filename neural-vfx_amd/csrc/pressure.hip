@@ -799,6 +799,51 @@ void Pressure::true_residual(const float* vel, hipStream_t s) {
     hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(256), 0, s, part, nparts, st);
 }
 
+// One kernel of the projection on the current buffers (iteration index 0 semantics, so no CG
+// state is advanced), for timing.  Returns false for an unknown id.
+bool Pressure::launch_single(int which, hipStream_t s) {
+    const Geom& g = geom;
+    const size_t m0 = (size_t)lv[0].m;
+    const size_t half0 = m0 * m0 * (m0 / 2) * 8;
+    switch (which) {
+        case 0: hipLaunchKernelGGL(k_matvec_dot, g.grid, g.block, 0, s, N, g.zc, p, fm, q, part_pq, st, 1); break;
+        case 1: hipLaunchKernelGGL(k_post_rz, g.grid, g.block, 0, s, N, g.zc, zt, r, fm, z, part, nparts, st, 0); break;
+        case 2:
+            if (S < 1) return false;
+            hipLaunchKernelGGL(k_smooth_half, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s, lv[0], 1, st, 0);
+            break;
+        case 3: {
+            if (S < 1) return false;
+            const size_t items = (size_t)lv[1].m * lv[1].m * lv[1].m * 8;
+            hipLaunchKernelGGL(k_restrict, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, lv[0], lv[1],
+                               S > 1 ? 1 : 0, st, 0);
+            break;
+        }
+        case 4:
+            if (S < 1) return false;
+            hipLaunchKernelGGL(k_prolong_black, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s, lv[0], lv[1],
+                               st, 0);
+            break;
+        case 5: hipLaunchKernelGGL(k_pre, g.grid, g.block, 0, s, N, g.zc, r, q, fm, z0, st, 0); break;
+        case 6: hipLaunchKernelGGL(k_resid, g.grid, g.block, 0, s, N, g.zc, r, z0, fm, rr, st, 0); break;
+        case 7:
+            hipLaunchKernelGGL(k_fwd, dim3((unsigned)((m0 * m0 * m0 * 8 + 255) / 256)), dim3(256), 0, s, N, rr, lv[0],
+                               S > 0 ? 1 : 0, st, 0);
+            break;
+        case 8: hipLaunchKernelGGL(k_inv, g.grid, g.block, 0, s, N, g.zc, lv[0], fm, z0, zt, st, 0); break;
+        case 9: hipLaunchKernelGGL(k_upd_p, g.grid, g.block, 0, s, N, g.zc, phi, p, z, st, 0); break;
+        case 10: {
+            CoarseLevels cl{};
+            cl.n = nlev - S;
+            for (int j = 0; j < cl.n; ++j) cl.lv[j] = lv[S + j];
+            hipLaunchKernelGGL(k_coarse, dim3(1), dim3(CB_THREADS), 0, s, cl, st, 0);
+            break;
+        }
+        default: return false;
+    }
+    return true;
+}
+
 void launch_div_abs(const Geom& g, const float* vel, const float* solid, float* out, hipStream_t s) {
     hipLaunchKernelGGL(k_div_abs, g.grid, g.block, 0, s, g.N, g.zc, vel, solid, out);
 }

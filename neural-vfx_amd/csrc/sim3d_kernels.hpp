@@ -27,6 +27,10 @@ struct CoarseLevels {
 // ---------------------------------------------------------------- advection / forces
 void launch_advect(const Geom& g, int C, const float* src, const float* vel, const float* prm, float inv_dx,
                    float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s);
+void launch_adv_pass(int pass, const Geom& g, int C, const float* src, const float* vel, const float* prm,
+                     float inv_dx, float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s);
+void launch_sources_pass(int pass, const Geom& g, const float* src, const uint8_t* fm, const float* prm,
+                         const DevState* st, float* dst, hipStream_t s);
 void launch_mass_fin(const Geom& g, const double* part, DevState* st, hipStream_t s);
 void launch_sources_diffuse(const Geom& g, const float* adv, const uint8_t* fm, const float* prm,
                             const DevState* st, float* hat, float* data, hipStream_t s);
@@ -63,6 +67,7 @@ struct Pressure {
     void iteration(int k, const float* prm, hipStream_t s);
     void apply(float* vel, hipStream_t s);
     void true_residual(const float* vel, hipStream_t s);
+    bool launch_single(int which, hipStream_t s);
 };
 
 }  // namespace vfx

@@ -15,6 +15,8 @@ from ._build import LIB_PATH
 VFX_OK, VFX_E_ARG, VFX_E_VALUE, VFX_E_HIP = 0, 1, 2, 3
 STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
 OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M = 1, 2, 3
+KERNELS = ("adv_sl", "adv_mc", "sources", "diffuse", "matvec", "post_rz", "smooth0", "restrict0", "prolong0", "pre",
+           "resid", "fwd", "inv", "upd_p", "coarse")
 
 # every entry point of include/vfxsim3d.h with its argument types
 _P = ctypes.c_void_p
@@ -37,6 +39,7 @@ SIGNATURES = {
     "vfx_kmul": [_P, _P, _P, _P, _P],
     "vfx_precond": [_P, _P, _P, _P, _P],
     "vfx_get_ops": [_P, _P, _I, _P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I), _P],
+    "vfx_time_kernel": [_P, _I, _I, _P, ctypes.POINTER(ctypes.c_float)],
     "vfx_last_error": [ctypes.c_char_p, _I],
 }
 

@@ -91,12 +91,12 @@ class Solver3D:
             raise NotImplementedError("scalar_res (the reference's opt-in finer scalar grid) is not built")
         self.res = int(res)
         self.config = config if config is not None else SolverConfig3D()
+        self.scalar_res = None
+        self._structure()
         self.device = str(device)
         if self.device == "cuda":
             self.device = f"cuda:{torch.cuda.current_device()}"
         self.dx = DOMAIN_SIZE / self.res
-        self.scalar_res = None
-        self._structure()
         self._lib = _lib.load()
         self._ctx = ctypes.c_void_p()
         with torch.cuda.device(torch.device(self.device)):

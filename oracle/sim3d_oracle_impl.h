@@ -18,7 +18,8 @@
 static inline R F(clampr)(R g, R lo, R hi) { return g < lo ? lo : (g > hi ? hi : g); }
 
 static inline int F(base_idx)(R g, int n) {
-    int i = (int)FLOORR(F(clampr)(g, (R)0, (R)(n - 1)));
+    /* NaN lands on cell 0 (same as the clamp for every other input) */
+    int i = (int)FLOORR(g >= (R)0 ? (g > (R)(n - 1) ? (R)(n - 1) : g) : (R)0);
     return i < n - 2 ? i : n - 2;
 }
 
@@ -895,10 +896,16 @@ int F(or_project)(int N, R* data, const R* solid, const float* prm, R* phi, doub
     const int maxit = (int)prm[P_MAXIT];
     const double rel_tol = (double)prm[P_REL_TOL];
 
+    /* per-z-plane partial sums, combined in plane order: independent of the thread count */
+    double* pa = (double*)calloc((size_t)N, sizeof(double));
+    double* pb = (double*)calloc((size_t)N, sizeof(double));
+
     /* k_rhs: r = -d(u*) - K phi0; max|u*| over fluid; ||b||^2 */
     double maxu = 0, bb = 0;
     if (!warm) memset(phi, 0, n3 * sizeof(R));
-    for (int zz = 0; zz < N; ++zz)
+#pragma omp parallel for schedule(static)
+    for (int zz = 0; zz < N; ++zz) {
+        double sb = 0, mu = 0;
         for (int y = 0; y < N; ++y)
             for (int x = 0; x < N; ++x) {
                 size_t i = IDX(zz, y, x);
@@ -907,50 +914,72 @@ int F(or_project)(int N, R* data, const R* solid, const float* prm, R* phi, doub
                 R rv = b;
                 if (warm) rv -= F(kmul)(phi, fm, N, zz, y, x);
                 r[i] = rv;
-                bb += (double)b * (double)b;
+                sb += (double)b * (double)b;
                 R ux = vel[i], uy = vel[n3 + i], uz = vel[2 * n3 + i];
                 double sp = (double)SQRTR(ux * ux + uy * uy + uz * uz);
-                maxu = sp > maxu ? sp : maxu;
+                mu = sp > mu ? sp : mu;
             }
-    const double thr = 2.0 * (double)tol * (double)(float)maxu;
+        pa[zz] = sb;
+        pb[zz] = mu;
+    }
+    for (int zz = 0; zz < N; ++zz) { bb += pa[zz]; maxu = pb[zz] > maxu ? pb[zz] : maxu; }
+    const double thr = (double)(2.0f * tol * (float)maxu);
     const double bnorm = sqrt(bb);
+
+#define RNORM(out)                                                            \
+    do {                                                                      \
+        _Pragma("omp parallel for schedule(static)")                         \
+        for (int zz = 0; zz < N; ++zz) {                                      \
+            double s_ = 0;                                                    \
+            const R* rp = r + (size_t)zz * N * N;                             \
+            for (size_t j = 0; j < (size_t)N * N; ++j) s_ += (double)rp[j] * (double)rp[j]; \
+            pa[zz] = s_;                                                      \
+        }                                                                     \
+        double t_ = 0;                                                        \
+        for (int zz = 0; zz < N; ++zz) t_ += pa[zz];                          \
+        out = sqrt(t_);                                                       \
+    } while (0)
 
     double rz, mr, alpha = 0, beta = 0, best;
     F(precond)(N, fm, &L, r, z0, rr, zt, z, &rz, &mr);
     double rn = 0;
-    for (size_t i = 0; i < n3; ++i) rn += (double)r[i] * (double)r[i];
-    rn = sqrt(rn);
+    RNORM(rn);
     best = mr;
     int it = 0;
 #define GO() (it < maxit && rz > 0.0 && mr <= 4.0 * best && \
               (rel_tol > 0.0 ? rn > rel_tol * bnorm : mr > thr))
     int go = GO();
-    for (size_t i = 0; i < n3; ++i) p[i] = z[i];
+    memcpy(p, z, n3 * sizeof(R));
     while (go) {
         /* k_matvec_dot */
         double pq = 0;
-        for (int zz = 0; zz < N; ++zz)
+#pragma omp parallel for schedule(static)
+        for (int zz = 0; zz < N; ++zz) {
+            double s_ = 0;
             for (int y = 0; y < N; ++y)
                 for (int x = 0; x < N; ++x) {
                     size_t i = IDX(zz, y, x);
                     R v = fm[i] ? F(kmul)(p, fm, N, zz, y, x) : (R)0;
                     q[i] = v;
-                    pq += (double)v * (double)p[i];
+                    s_ += (double)v * (double)p[i];
                 }
+            pa[zz] = s_;
+        }
+        for (int zz = 0; zz < N; ++zz) pq += pa[zz];
         alpha = pq > 0.0 ? rz / pq : 0.0;
         R af = (R)alpha;
+#pragma omp parallel for schedule(static)
         for (size_t i = 0; i < n3; ++i) r[i] = r[i] - af * q[i];
         double rz_new;
         F(precond)(N, fm, &L, r, z0, rr, zt, z, &rz_new, &mr);
-        rn = 0;
-        for (size_t i = 0; i < n3; ++i) rn += (double)r[i] * (double)r[i];
-        rn = sqrt(rn);
+        RNORM(rn);
         beta = rz > 0.0 ? rz_new / rz : 0.0;
         rz = rz_new;
         it += 1;
         best = mr < best ? mr : best;
         go = GO();
         R bf = (R)beta;
+#pragma omp parallel for schedule(static)
         for (size_t i = 0; i < n3; ++i) {
             R po = p[i];
             phi[i] = phi[i] + af * po;
@@ -958,8 +987,10 @@ int F(or_project)(int N, R* data, const R* solid, const float* prm, R* phi, doub
         }
     }
 #undef GO
+#undef RNORM
     F(or_apply)(N, fm, phi, vel);
     if (stats) { stats[0] = bnorm; stats[1] = rn; stats[2] = maxu; }
+    free(pa); free(pb);
     free(r); free(z); free(p); free(q); free(z0); free(rr); free(zt); free(fm);
     F(levels_free)(&L);
     return it;
