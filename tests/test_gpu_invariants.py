@@ -1,0 +1,148 @@
+"""Size-independent properties of the HIP path, ported from the reference's gate tests
+(tests3d/test_sim3d_layer1.py), at 64^3 and at the full 256^3 size.
+
+Thresholds are the reference's: normalised divergence < 1e-4 after projection (:102-145),
+mass within 0.1% in a closed box (:169-195; run for 300 instead of 1000 steps here), bit-identical
+reruns (:375-393), obstacles stay empty (:334-355), buoyancy direction (:279-330).
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from simhelp import blob, smooth_noise, to_dev
+from vfx.core.fields import AMBIENT_TEMP, SIM_DT
+from vfx.core.fields3d import Emitter3D, Forces3D, Impulse3D, Obstacle3D
+from vfx.sim3d import Solver3D, SolverConfig3D, _lib
+
+pytestmark = pytest.mark.gpu
+DIV_TOL = 1e-4
+MASS_TOL = 1e-3
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def sphere():
+    return [Obstacle3D(kind="sphere", pos=(0.5, 0.45, 0.5), size=(0.14, 0.14, 0.14))]
+
+
+def _mass(st):
+    d = st.ch("density") * (st.solid < 0.5)
+    return float(d.double().sum().item()) * st.meta.dx ** 3
+
+
+@pytest.mark.parametrize("obst", [False, True], ids=["open", "sphere"])
+def test_divergence_after_project_default_config(cuda, obst):
+    N = 64
+    s = Solver3D(N)
+    st = s.reset(sphere() if obst else [])
+    v = smooth_noise(N, 3, 1, amp=2.0) + np.array([0.8, -0.3, 0.4], np.float32)[:, None, None, None]
+    st.velocity.copy_(to_dev(v) * (st.solid < 0.5))
+    before = s.divergence(st).max().item()
+    s.project(st)
+    div = s.divergence(st)
+    m = div.max().item()
+    _say(f"64^3 {'sphere' if obst else 'open'}: div {before:.3e} -> {m:.3e} in {s.pressure_iters()} iterations")
+    assert before > 1e-2 and m < DIV_TOL
+    assert torch.all(div[st.solid > 0.5] == 0)
+    assert torch.isfinite(st.data).all()
+
+
+@pytest.mark.parametrize("obst", [False, True], ids=["open", "sphere"])
+def test_divergence_after_steps(cuda, obst):
+    N = 64
+    s = Solver3D(N)
+    st = s.reset(sphere() if obst else [])
+    f = Forces3D(emitters=[Emitter3D(pos=(0.5, 0.15, 0.5), radius=0.07, density_rate=2.0, temperature=900.0,
+                                     velocity=(0.0, 1.0, 0.0))])
+    for _ in range(60):
+        s.step(st, f, SIM_DT)
+    m = s.divergence(st).max().item()
+    _say(f"after 60 steps: div {m:.3e}, iterations {s.pressure_iters()}")
+    assert m < DIV_TOL
+
+
+def test_mass_conserved_closed_box(cuda):
+    N = 64
+    s = Solver3D(N, SolverConfig3D(density_dissipation=0.0))
+    st = s.reset(sphere())
+    fl = (st.solid < 0.5).float()
+    st.ch("density").copy_(to_dev(blob(N, 0.5, 0.2, 0.5, 0.08)) * fl)
+    st.ch("temperature").copy_(AMBIENT_TEMP + 400.0 * to_dev(blob(N, 0.5, 0.2, 0.5, 0.08)))
+    st.velocity.copy_(to_dev(smooth_noise(N, 3, 3, amp=0.5)) * fl)
+    m0 = _mass(st)
+    for _ in range(300):
+        s.step(st, Forces3D(), SIM_DT)
+    rel = abs(_mass(st) - m0) / m0
+    _say(f"mass rel change over 300 steps {rel:.3e}")
+    assert torch.isfinite(st.data).all() and rel < MASS_TOL
+
+
+def _scripted(N=64, steps=60):
+    s = Solver3D(N)
+    st = s.reset(sphere())
+    for i in range(steps):
+        f = Forces3D(emitters=[Emitter3D(pos=(0.5, 0.12, 0.5), radius=0.06, density_rate=2.0, temperature=1300.0,
+                                         fuel_rate=1.5, velocity=(0.0, 1.0, 0.0))],
+                     impulses=[Impulse3D(pos=(0.3, 0.5, 0.4), radius=0.1, velocity=(2.0, 0.5, -1.0))]
+                     if i % 17 == 0 else [], wind=(0.2, 0.0, 0.1))
+        s.step(st, f, SIM_DT)
+    return st
+
+
+def test_bit_identical_reruns(cuda):
+    a = _scripted().data.clone()
+    b = _scripted().data.clone()
+    assert a.abs().sum().item() > 0
+    assert torch.equal(a, b)
+
+
+def test_no_flow_into_obstacles_and_buoyancy(cuda):
+    N = 64
+    s = Solver3D(N)
+    obs = [Obstacle3D("box", (0.5, 0.55, 0.5), (0.18, 0.06, 0.18))]
+    st = s.reset(obs)
+    f = Forces3D(emitters=[Emitter3D(pos=(0.5, 0.2, 0.5), radius=0.07, density_rate=3.0, temperature=1200.0,
+                                     fuel_rate=1.0, velocity=(0.0, 2.0, 0.0))])
+    deep = (-torch.nn.functional.max_pool3d(-st.solid[None, None], 3, 1, 1)[0, 0]) > 0.5
+    for _ in range(100):
+        s.step(st, f, SIM_DT)
+    umax = st.velocity.abs().max().item()
+    assert umax > 0.1
+    assert st.velocity[:, deep].abs().max().item() <= 1e-3 * umax
+    assert st.ch("density")[deep].abs().max().item() < 1e-3
+    # the hot plume rises
+    w = st.ch("density") / st.ch("density").sum()
+    assert (st.velocity[1] * w).sum().item() > 0
+
+
+@pytest.mark.parametrize("N", [128, 256])
+def test_full_size_step_converges(cuda, N):
+    """The full-size step: finite state, the true relative residual meets 1e-5 (with restarts
+    from the true residual when the recurrence drifts), divergence gate."""
+    cfg = SolverConfig3D(buoyancy_alpha=0.0, buoyancy_beta=1.5, vorticity_eps=0.3, pressure_rel_tol=1e-5,
+                         pressure_max_iters=200, pressure_restarts=3, emitter_noise=0.2)
+    s = Solver3D(N, cfg)
+    s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
+    st = s.reset()
+    f = Forces3D(emitters=[Emitter3D(pos=(0.5, 0.0, 0.5), radius=0.1, density_rate=1.0, temperature=294.0,
+                                     fuel_rate=1.0)])
+    for i in range(8):
+        s.step(st, f, 1.0 / 60.0)
+        ps = s.pressure_stats()
+        _say(f"{N}^3 step {i}: iters {ps['iters']} rel {ps['rel_res']:.2e} true {ps['true_rel_res']:.2e} "
+             f"restarts {ps['restarts']}")
+    assert torch.isfinite(st.data).all()
+    assert ps["true_rel_res"] <= 1e-5
+    st.velocity.add_(to_dev(smooth_noise(N, 3, 5, amp=0.5)))
+    s.project(st)
+    ps = s.pressure_stats()
+    div = s.divergence(st).max().item()
+    _say(f"{N}^3 project of a perturbed field: iters {ps['iters']} true rel {ps['true_rel_res']:.2e} div {div:.2e}")
+    assert ps["true_rel_res"] <= 1e-5 and div < DIV_TOL

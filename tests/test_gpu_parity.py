@@ -128,7 +128,8 @@ def test_operator_parity(cuda, hiplib, N):
     sd = to_dev(solid)
     p = smooth_noise(N, 1, 41)[0]
     out = torch.empty(N, N, N, device=DEV)
-    _lib.check(hiplib.vfx_kmul(s._ctx, _ptr(sd), _ptr(to_dev(p)), _ptr(out), _stream()))
+    pd = to_dev(p)
+    _lib.check(hiplib.vfx_kmul(s._ctx, _ptr(sd), _ptr(pd), _ptr(out), _stream()))
     fm = oracle.mask(solid)
     ref = oracle.kmul(fm, p)
     assert rel_err(out.cpu().numpy(), ref) <= 1e-6
@@ -159,7 +160,8 @@ def test_precond_parity(cuda, hiplib, N, coarse_m):
     fm = oracle.mask(solid)
     r = smooth_noise(N, 1, 51)[0] * (fm > 0)
     z = torch.empty(N, N, N, device=DEV)
-    _lib.check(hiplib.vfx_precond(s._ctx, _ptr(to_dev(solid)), _ptr(to_dev(r)), _ptr(z), _stream()))
+    sd, rd = to_dev(solid), to_dev(r)     # keep both alive until the copies have run
+    _lib.check(hiplib.vfx_precond(s._ctx, _ptr(sd), _ptr(rd), _ptr(z), _stream()))
     torch.cuda.synchronize()
     zref, _ = oracle.precond_apply(fm, r)
     e = rel_err(z.cpu().numpy(), zref)
