@@ -104,9 +104,8 @@ int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float*
 #define VFX_K_RESTRICT0 7  /* residual restriction finest -> next level */
 #define VFX_K_PROLONG0 8   /* prolongation + black sweep on the finest level */
 #define VFX_K_PRE 9        /* pre-Jacobi of the preconditioner */
-#define VFX_K_RESID 10     /* Jacobi residual r - K z0 */
-#define VFX_K_FWD 11       /* Hadamard transform to level 1 + red sweep */
-#define VFX_K_INV 12       /* inverse transform */
+#define VFX_K_RESID_FWD 10 /* Jacobi residual r - K z0 + Hadamard transform to level 1 + red sweep */
+#define VFX_K_RED_INV 12   /* last red sweep on level 1 + inverse transform */
 #define VFX_K_UPD_P 13     /* p / phi update */
 #define VFX_K_COARSE 14    /* single-workgroup coarse V-cycle */
 

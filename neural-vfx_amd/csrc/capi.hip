@@ -175,6 +175,7 @@ static int stage_project(vfx_ctx* c, float* data, hipStream_t s) {
     int upper = 0;
     int rc = run_cg(c, s, &upper);
     if (rc) return rc;
+    P.finish(s);
     c->last_restarts = 0;
     c->last_true_valid = false;
     int total = 0;
@@ -201,6 +202,7 @@ static int stage_project(vfx_ctx* c, float* data, hipStream_t s) {
             CHECK_LAUNCH("pressure restart rhs");
             rc = run_cg(c, s, &upper);
             if (rc) return rc;
+            P.finish(s);
         }
     }
     c->last_iters = total;   // plus DevState::it of the final run, read lazily
@@ -244,10 +246,12 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     Pressure& P = c->P;
     P.init_levels(N);
     P.geom = c->g;
-    P.nparts = nblocks(c->g);
+    P.init_tiles();
+    P.nparts = nblocks(c->g) > P.ntiles ? nblocks(c->g) : P.ntiles;
     const size_t n3 = c->g.n3;
-    // one pool: floats first, then doubles, then the mask
-    const size_t nf = n3 * (6 + 6 + 3 + 3 + 3 + 4 + 4) + 2 * (size_t)P_SIZE + n3 * 8 + P.level_floats();
+    // one pool: floats first (every array a multiple of 8 floats, so 32-byte aligned), then
+    // doubles, then the mask
+    const size_t nf = n3 * (6 + 6 + 3 + 3 + 3 + 4 + 4) + 2 * (size_t)P_SIZE + n3 * 7 + P.level_floats();
     const size_t nd = (size_t)P.nparts * (2 + 3 + 1) + sizeof(DevState) / sizeof(double) + 8;
     const size_t bytes = nf * 4 + 64 + nd * 8 + 64 + n3 + 64;
     if (!hip_ok(hipMalloc(&c->pool, bytes), "device allocation")) { delete c; return VFX_E_HIP; }
@@ -262,14 +266,13 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     c->tmp_b = f; f += 4 * n3;
     c->prm = f; f += P_SIZE;
     c->prm_adv = f; f += P_SIZE;
-    P.r = f; f += n3;
-    P.z = f; f += n3;
-    P.p = f; f += n3;
+    P.R[0] = f; f += n3;
+    P.R[1] = f; f += n3;
+    P.P[0] = f; f += n3;
+    P.P[1] = f; f += n3;
     P.q = f; f += n3;
+    P.z = f; f += n3;
     P.phi = f; f += n3;
-    P.z0 = f; f += n3;
-    P.rr = f; f += n3;
-    P.zt = f; f += n3;
     P.bind_levels(f); f += P.level_floats();
     uintptr_t a = ((uintptr_t)f + 63) & ~(uintptr_t)63;
     double* d = (double*)a;
@@ -472,7 +475,7 @@ int vfx_precond(vfx_ctx* c, const float* solid, const float* r, float* z, void* 
     hipStream_t s = S(stream);
     const size_t n3 = c->g.n3;
     c->P.setup(solid, s);
-    CHECK_HIP(hipMemcpyAsync(c->P.r, r, n3 * 4, hipMemcpyDeviceToDevice, s), "r copy");
+    CHECK_HIP(hipMemcpyAsync(c->P.R[0], r, n3 * 4, hipMemcpyDeviceToDevice, s), "r copy");
     hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
     c->P.precond(0, s);
     CHECK_LAUNCH("precond");

@@ -51,23 +51,31 @@ struct Pressure {
     Lvl lv[MAX_LEVELS];
     Geom geom;
     uint8_t* fm = nullptr;
-    float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *phi = nullptr;
-    float *z0 = nullptr, *rr = nullptr, *zt = nullptr;
+    float* R[2] = {nullptr, nullptr};   // residual, ping-pong by iteration parity
+    float* P[2] = {nullptr, nullptr};   // search direction, ping-pong
+    float *q = nullptr, *z = nullptr, *phi = nullptr;
     double* part = nullptr;      // 3 * nparts
     double* part_pq = nullptr;   // nparts
-    int nparts = 0;
+    int nparts = 0;              // partial-sum slots: max(cell-kernel blocks, tiles)
+    dim3 tgrid;                  // LDS tiles of the fine grid
+    int ntiles = 0;
     DevState* st = nullptr;
 
     void init_levels(int N);
+    void init_tiles();
+    float* r_buf(int k) const;
     size_t level_floats() const;
     void bind_levels(float* base);
     void setup(const float* solid, hipStream_t s);
     void rhs(const float* vel, const float* prm, hipStream_t s);
     void precond(int k, hipStream_t s);
     void iteration(int k, const float* prm, hipStream_t s);
+    void finish(hipStream_t s);
     void apply(float* vel, hipStream_t s);
     void true_residual(const float* vel, hipStream_t s);
+    void vcycle_levels(int k, hipStream_t s);
     bool launch_single(int which, hipStream_t s);
+    bool launch_single_mg(int which, hipStream_t s);
 };
 
 }  // namespace vfx
