@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profiled", type=int, default=5, help="extra step_profiled steps for the stage split")
     ap.add_argument("--roofline-kernel", default="adv_mc")
+    ap.add_argument("--time-kernels", default="", help="comma list of extra kernels to time (us)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -138,6 +139,11 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
                 "kernel_ms": round(kms.value, 4), "bytes_per_launch": kbytes}
+    others = {}
+    for name in [n for n in args.time_kernels.split(",") if n]:
+        t = ctypes.c_float()
+        if s._lib.vfx_time_kernel(s._ctx, _lib.KERNELS.index(name), 20, s._stream(), ctypes.byref(t)) == 0:
+            others[name] = round(t.value * 1000.0, 1)
 
     cpu = None
     if not args.no_cpu:
@@ -155,6 +161,7 @@ def main():
         "profiled_total_ms": med("total"), "pcg_iters": iters, "rel_res": [float(f"{r:.3g}") for r in rel],
         "true_rel_res": [float(f"{r:.3g}") for r in true_rel],
         "roofline": roofline, "cpu_baseline": cpu,
+        **({"kernel_us": others} if others else {}),
     }
     print(json.dumps(out))
 

@@ -27,6 +27,10 @@ namespace vfx {
             const int i = (cz * N + y) * N + x;
 #define CELL_LOOP_END }
 
+constexpr int HN_IT = (HN + TILE_THREADS - 1) / TILE_THREADS;          // halo cells per thread
+constexpr int OWN_IT = TX * TY * TZ / TILE_THREADS;                    // own cells per thread
+constexpr int HBN8_IT = (HBN * 8 + TILE_THREADS - 1) / TILE_THREADS;  // level-1 halo values per thread
+
 __device__ __forceinline__ int tile_linear() { return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)); }
 
 // halo cell h of the current tile -> global cell index, or -1 outside the box
@@ -60,9 +64,9 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const float* __restr
         if (!warm) phi[i] = 0.0f;
         const int m = fm[i];
         if (m == 0) { r[i] = 0.0f; continue; }
-        const float b = -div2(vel, n3, m, N, i);
+        const float b = -div2_xyz(vel, n3, m, N, x, y, cz, i);
         float rv = b;
-        if (warm) rv -= kmul_m(phi, fm, N, i, m);
+        if (warm) rv -= kmul_xyz(phi, fm, N, x, y, cz, i, m);
         r[i] = rv;
         bb += (double)b * (double)b;
         const float ux = vel[i], uy = vel[n3 + i], uz = vel[2 * n3 + i];
@@ -104,20 +108,21 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
     __shared__ double sh[8];
     if (cg_inactive(st, k)) return;
     const float a = (float)st->s[S_ALPHA], b = (float)st->s[S_BETA];
-    for (int h = threadIdx.x; h < HN; h += TILE_THREADS) {
+    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
+        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
+        if (h >= HN) break;
         const int i = halo_global(N, h);
-        float v = 0.0f;
-        int m = 0;
-        if (i >= 0) {
-            m = fm[i];
-            v = upd ? z[i] + b * p_in[i] : p_in[i];
-        }
-        sp[h] = v;
-        sm[h] = (uint8_t)m;
+        const int ic = i >= 0 ? i : 0;   // every load in bounds; outside cells staged as mask 0, value 0
+        const int m = fm[ic];
+        const float zv = z[ic], pv = p_in[ic];
+        const float v = upd ? zv + b * pv : pv;
+        sp[h] = i >= 0 ? v : 0.0f;
+        sm[h] = (uint8_t)(i >= 0 ? m : 0);
     }
     __syncthreads();
     double s = 0.0;
-    for (int j = threadIdx.x; j < TX * TY * TZ; j += TILE_THREADS) {
+    _Pragma("unroll") for (int ji_ = 0; ji_ < OWN_IT; ++ji_) {
+        const int j = (int)threadIdx.x + ji_ * TILE_THREADS;
         int h;
         const int i = own_global(N, j, h);
         if (i < 0) continue;
@@ -156,19 +161,15 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
     __shared__ uint8_t sm[HN];
     if (cg_inactive(st, k)) return;
     const float a = (float)st->s[S_ALPHA];
-    for (int h = threadIdx.x; h < HN; h += TILE_THREADS) {
+    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
+        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
+        if (h >= HN) break;
         const int i = halo_global(N, h);
-        float v = 0.0f;
-        int m = 0;
-        if (i >= 0) {
-            m = fm[i];
-            if (m) {
-                float rv = r_in[i];
-                if (k > 0) rv = rv - a * q[i];
-                v = OMEGA_JAC * rv / kdiag(m);
-            }
-        }
-        sz[h] = v;
+        const int ic = i >= 0 ? i : 0;   // every load in bounds; outside cells staged as mask 0, value 0
+        const int m = i >= 0 ? (int)fm[ic] : 0;
+        const float r0 = r_in[ic], qv = q[ic];
+        const float rv = k > 0 ? r0 - a * qv : r0;
+        sz[h] = m ? OMEGA_JAC * rv / kdiag(m) : 0.0f;
         sm[h] = (uint8_t)m;
     }
     __syncthreads();
@@ -229,34 +230,40 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
     if (cg_inactive(st, k)) return;
     const int mb = L.m;
     const int bx0 = (int)blockIdx.x * BTX - 1, by0 = (int)blockIdx.y * BTY - 1, bz0 = (int)blockIdx.z * BTZ - 1;
-    for (int it = threadIdx.x; it < HBN * 8; it += TILE_THREADS) {
+    _Pragma("unroll") for (int bi_ = 0; bi_ < HBN8_IT; ++bi_) {
+        const int it = (int)threadIdx.x + bi_ * TILE_THREADS;
+        if (it >= HBN * 8) break;
         const int b = it >> 3, c = it & 7;
         const int I = bx0 + b % HBX, J = by0 + (b / HBX) % HBY, K = bz0 + b / (HBX * HBY);
-        float v = 0.0f;
-        if (I >= 0 && J >= 0 && K >= 0 && I < mb && J < mb && K < mb) v = L.x[lcell(mb, K, J, I) * 8 + c];
-        sx[it] = v;
+        const bool in = I >= 0 && J >= 0 && K >= 0 && I < mb && J < mb && K < mb;
+        const float v = L.x[(in ? lcell(mb, K, J, I) : 0) * 8 + c];
+        sx[it] = in ? v : 0.0f;
     }
-    for (int h = threadIdx.x; h < HN; h += TILE_THREADS) {
+    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
+        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
+        if (h >= HN) break;
         const int i = halo_global(N, h);
-        sm[h] = (uint8_t)(i >= 0 ? fm[i] : 0);
+        const int m = fm[i >= 0 ? i : 0];
+        sm[h] = (uint8_t)(i >= 0 ? m : 0);
     }
     __syncthreads();
-    for (int h = threadIdx.x; h < HN; h += TILE_THREADS) {
+    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
+        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
+        if (h >= HN) break;
         const int m = sm[h];
-        float acc = 0.0f;
-        if (m) {
-            const int i = halo_global(N, h);
-            const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-            const int hb = ((hz >> 1) * HBY + (hy >> 1)) * HBX + (hx >> 1);
-            acc = OMEGA_JAC * r[i] / kdiag(m);
+        const int i = halo_global(N, h);
+        const float rv = r[i >= 0 ? i : 0];
+        const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+        const int hb = ((hz >> 1) * HBY + (hy >> 1)) * HBX + (hx >> 1);
+        float acc = m ? OMEGA_JAC * rv / kdiag(m) : 0.0f;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) acc += hsign(c, hz & 1, hy & 1, hx & 1) * sx[hb * 8 + c];
-        }
-        szt[h] = acc;
+        for (int c = 0; c < 8; ++c) acc += hsign(c, hz & 1, hy & 1, hx & 1) * sx[hb * 8 + c];
+        szt[h] = m ? acc : 0.0f;
     }
     __syncthreads();
     double s = 0.0, mx = 0.0, rr = 0.0;
-    for (int j = threadIdx.x; j < TX * TY * TZ; j += TILE_THREADS) {
+    _Pragma("unroll") for (int ji_ = 0; ji_ < OWN_IT; ++ji_) {
+        const int j = (int)threadIdx.x + ji_ * TILE_THREADS;
         int h;
         const int i = own_global(N, j, h);
         if (i < 0) continue;
@@ -334,7 +341,8 @@ __global__ void __launch_bounds__(256) k_apply(int N, int zc, float* __restrict_
             continue;
         }
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) vel[ax * n3 + i] = vel[ax * n3 + i] - g_ax(phi, fm, i, axstride(N, ax), ax);
+        for (int ax = 0; ax < 3; ++ax)
+            vel[ax * n3 + i] = vel[ax * n3 + i] - g_ax_xyz(phi, m, N, i, ax == 0 ? x : (ax == 1 ? y : cz), ax);
     CELL_LOOP_END
 }
 
@@ -348,7 +356,7 @@ __global__ void __launch_bounds__(256) k_true_res(int N, int zc, const float* __
     CELL_LOOP_BEGIN
         const int m = fm[i];
         if (m == 0) continue;
-        const double v = -(double)div2(vel, n3, m, N, i) - (double)kmul_m(phi, fm, N, i, m);
+        const double v = -(double)div2_xyz(vel, n3, m, N, x, y, cz, i) - (double)kmul_xyz(phi, fm, N, x, y, cz, i, m);
         s += v * v;
     CELL_LOOP_END
     const double t = block_sum(s, sh);
@@ -368,7 +376,7 @@ __global__ void __launch_bounds__(256) k_div_abs(int N, int zc, const float* __r
     const int n3 = N * N * N;
     CELL_LOOP_BEGIN
         const int m = mask_from_solid(solid, N, cz, y, x);
-        out[i] = m == 0 ? 0.0f : 0.5f * fabsf(div2(vel, n3, m, N, i));
+        out[i] = m == 0 ? 0.0f : 0.5f * fabsf(div2_xyz(vel, n3, m, N, x, y, cz, i));
     CELL_LOOP_END
 }
 
@@ -377,7 +385,7 @@ __global__ void __launch_bounds__(256) k_kmul(int N, int zc, const float* __rest
                                               const uint8_t* __restrict__ fm, float* __restrict__ out) {
     CELL_LOOP_BEGIN
         const int m = fm[i];
-        out[i] = m ? kmul_m(p, fm, N, i, m) : 0.0f;
+        out[i] = m ? kmul_xyz(p, fm, N, x, y, cz, i, m) : 0.0f;
     CELL_LOOP_END
 }
 

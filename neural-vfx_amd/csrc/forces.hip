@@ -120,16 +120,30 @@ __global__ void __launch_bounds__(256) k_sources(int N, int zc, const float* __r
     }
 }
 
-__device__ __forceinline__ bool solid_at(const uint8_t* __restrict__ fm, int N, int z, int y, int x) {
-    if (x < 0 || y < 0 || z < 0 || x >= N || y >= N || z >= N) return true;
-    return fm[idx3(N, z, y, x)] == 0;
+// Neighbour loads go to in-bounds addresses (clamped to the cell itself outside the box) and are
+// issued unconditionally; the solid / range tests only select values.  A load guarded by a test
+// would wait for the test's own load and serialise the memory traffic of every cell.
+struct Nb6 {
+    int xm, xp, ym, yp, zm, zp;   // clamped neighbour indices
+    bool oxm, oxp, oym, oyp, ozm, ozp;   // neighbour inside the box
+};
+
+__device__ __forceinline__ Nb6 nb6(int N, int x, int y, int z, int i) {
+    Nb6 n;
+    const int sz = N * N;
+    n.oxm = x > 0; n.oxp = x + 1 < N; n.oym = y > 0; n.oyp = y + 1 < N; n.ozm = z > 0; n.ozp = z + 1 < N;
+    n.xm = n.oxm ? i - 1 : i; n.xp = n.oxp ? i + 1 : i;
+    n.ym = n.oym ? i - N : i; n.yp = n.oyp ? i + N : i;
+    n.zm = n.ozm ? i - sz : i; n.zp = n.ozp ? i + sz : i;
+    return n;
 }
 
-// diffuse_heat_fuel (kernels.py:268-286): channels 0..3 copied, T and F one explicit substep.
+// diffuse_heat_fuel (kernels.py:268-286): channels 0..3 copied, T and F one explicit substep with
+// zero flux at walls and solids (a missing or solid neighbour contributes the cell's own value).
 __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const float* __restrict__ src,
                                                  const uint8_t* __restrict__ fm, const float* __restrict__ prm,
                                                  float nsub, float* __restrict__ dst) {
-    const size_t n3 = (size_t)N * N * N;
+    const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
@@ -138,27 +152,28 @@ __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const float* __r
     lam = lam < LAM_CAP ? lam : LAM_CAP;
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const size_t i = idx3(N, z, y, x);
-        dst[i] = src[i];
-        dst[n3 + i] = src[n3 + i];
-        dst[2 * n3 + i] = src[2 * n3 + i];
-        dst[3 * n3 + i] = src[3 * n3 + i];
-        const bool solid = fm[i] == 0;
-        const bool sxm = solid_at(fm, N, z, y, x - 1), sxp = solid_at(fm, N, z, y, x + 1);
-        const bool sym = solid_at(fm, N, z, y - 1, x), syp = solid_at(fm, N, z, y + 1, x);
-        const bool szm = solid_at(fm, N, z - 1, y, x), szp = solid_at(fm, N, z + 1, y, x);
-        for (int c = 4; c <= 5; ++c) {
-            const float* f = src + c * n3;
-            const float v = f[i];
-            if (solid || lam <= 0.0f) {
-                dst[c * n3 + i] = v;
-            } else {
-                const float s = (sxm ? v : f[i - 1]) + (sxp ? v : f[i + 1]) + (sym ? v : f[i - N]) +
-                                (syp ? v : f[i + N]) + (szm ? v : f[i - (size_t)N * N]) +
-                                (szp ? v : f[i + (size_t)N * N]);
-                dst[c * n3 + i] = v + lam * (s - 6.0f * v);
-            }
+        const int i = (z * N + y) * N + x;
+        const Nb6 n = nb6(N, x, y, z, i);
+        const float c0 = src[i], c1 = src[n3 + i], c2 = src[2 * n3 + i], c3 = src[3 * n3 + i];
+        const int m0 = fm[i], mxm = fm[n.xm], mxp = fm[n.xp], mym = fm[n.ym], myp = fm[n.yp], mzm = fm[n.zm],
+                  mzp = fm[n.zp];
+        const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
+        const bool syp = !n.oyp || myp == 0, szm = !n.ozm || mzm == 0, szp = !n.ozp || mzp == 0;
+        float out[2];
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+            const float* f = src + (4 + cc) * n3;
+            const float v = f[i], a0 = f[n.xm], a1 = f[n.xp], a2 = f[n.ym], a3 = f[n.yp], a4 = f[n.zm], a5 = f[n.zp];
+            const float s = (sxm ? v : a0) + (sxp ? v : a1) + (sym ? v : a2) + (syp ? v : a3) + (szm ? v : a4) +
+                            (szp ? v : a5);
+            out[cc] = (m0 == 0 || lam <= 0.0f) ? v : v + lam * (s - 6.0f * v);
         }
+        dst[i] = c0;
+        dst[n3 + i] = c1;
+        dst[2 * n3 + i] = c2;
+        dst[3 * n3 + i] = c3;
+        dst[4 * n3 + i] = out[0];
+        dst[5 * n3 + i] = out[1];
     }
 }
 
@@ -166,26 +181,29 @@ __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const float* __r
 __global__ void __launch_bounds__(256) k_curl(int N, int zc, const float* __restrict__ data,
                                               const uint8_t* __restrict__ fm, float inv2dx,
                                               float* __restrict__ w) {
-    const size_t n3 = (size_t)N * N * N;
+    const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
     const float* vx = data + n3;
     const float* vy = data + 2 * n3;
     const float* vz = data + 3 * n3;
-    const size_t sz = (size_t)N * N;
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const size_t i = idx3(N, z, y, x);
-        const bool sxm = solid_at(fm, N, z, y, x - 1), sxp = solid_at(fm, N, z, y, x + 1);
-        const bool sym = solid_at(fm, N, z, y - 1, x), syp = solid_at(fm, N, z, y + 1, x);
-        const bool szm = solid_at(fm, N, z - 1, y, x), szp = solid_at(fm, N, z + 1, y, x);
-        const float dwdy = (syp ? 0.0f : vz[i + N]) - (sym ? 0.0f : vz[i - N]);
-        const float dvdz = (szp ? 0.0f : vy[i + sz]) - (szm ? 0.0f : vy[i - sz]);
-        const float dudz = (szp ? 0.0f : vx[i + sz]) - (szm ? 0.0f : vx[i - sz]);
-        const float dwdx = (sxp ? 0.0f : vz[i + 1]) - (sxm ? 0.0f : vz[i - 1]);
-        const float dvdx = (sxp ? 0.0f : vy[i + 1]) - (sxm ? 0.0f : vy[i - 1]);
-        const float dudy = (syp ? 0.0f : vx[i + N]) - (sym ? 0.0f : vx[i - N]);
+        const int i = (z * N + y) * N + x;
+        const Nb6 n = nb6(N, x, y, z, i);
+        const int mxm = fm[n.xm], mxp = fm[n.xp], mym = fm[n.ym], myp = fm[n.yp], mzm = fm[n.zm], mzp = fm[n.zp];
+        const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
+        const bool syp = !n.oyp || myp == 0, szm = !n.ozm || mzm == 0, szp = !n.ozp || mzp == 0;
+        const float vz_yp = vz[n.yp], vz_ym = vz[n.ym], vy_zp = vy[n.zp], vy_zm = vy[n.zm];
+        const float vx_zp = vx[n.zp], vx_zm = vx[n.zm], vz_xp = vz[n.xp], vz_xm = vz[n.xm];
+        const float vy_xp = vy[n.xp], vy_xm = vy[n.xm], vx_yp = vx[n.yp], vx_ym = vx[n.ym];
+        const float dwdy = (syp ? 0.0f : vz_yp) - (sym ? 0.0f : vz_ym);
+        const float dvdz = (szp ? 0.0f : vy_zp) - (szm ? 0.0f : vy_zm);
+        const float dudz = (szp ? 0.0f : vx_zp) - (szm ? 0.0f : vx_zm);
+        const float dwdx = (sxp ? 0.0f : vz_xp) - (sxm ? 0.0f : vz_xm);
+        const float dvdx = (sxp ? 0.0f : vy_xp) - (sxm ? 0.0f : vy_xm);
+        const float dudy = (syp ? 0.0f : vx_yp) - (sym ? 0.0f : vx_ym);
         w[i] = (dwdy - dvdz) * inv2dx;
         w[n3 + i] = (dudz - dwdx) * inv2dx;
         w[2 * n3 + i] = (dvdx - dudy) * inv2dx;
@@ -195,23 +213,27 @@ __global__ void __launch_bounds__(256) k_curl(int N, int zc, const float* __rest
 // sep_x (kernels.py:392-406): out[0:3] = [1,2,1]_x (w a), out[3] = [1,2,1]_x w.
 __global__ void __launch_bounds__(256) k_sep_x(int N, int zc, const float* __restrict__ a,
                                                const uint8_t* __restrict__ fm, float* __restrict__ out) {
-    const size_t n3 = (size_t)N * N * N;
+    const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const size_t i = idx3(N, z, y, x);
+        const int i = (z * N + y) * N + x;
+        const int idx[3] = {x > 0 ? i - 1 : i, i, x + 1 < N ? i + 1 : i};
+        const bool inb[3] = {x > 0, true, x + 1 < N};
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
 #pragma unroll
-        for (int k = -1; k <= 1; ++k) {
-            if (!solid_at(fm, N, z, y, x + k)) {
-                const float q = (float)(2 - (k < 0 ? -k : k));
-                s0 += q * a[i + k];
-                s1 += q * a[n3 + i + k];
-                s2 += q * a[2 * n3 + i + k];
-                ws += q;
-            }
+        for (int k = 0; k < 3; ++k) {
+            const int j = idx[k];
+            const int mk = fm[j];
+            const float a0 = a[j], a1 = a[n3 + j], a2 = a[2 * n3 + j];
+            const bool fl = inb[k] && mk != 0;
+            const float q = k == 1 ? 2.0f : 1.0f;
+            s0 = s0 + (fl ? q * a0 : 0.0f);
+            s1 = s1 + (fl ? q * a1 : 0.0f);
+            s2 = s2 + (fl ? q * a2 : 0.0f);
+            ws = ws + (fl ? q : 0.0f);
         }
         out[i] = s0; out[n3 + i] = s1; out[2 * n3 + i] = s2; out[3 * n3 + i] = ws;
     }
@@ -219,19 +241,26 @@ __global__ void __launch_bounds__(256) k_sep_x(int N, int zc, const float* __res
 
 // sep_y (kernels.py:409-419)
 __global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const float* __restrict__ a, float* __restrict__ out) {
-    const size_t n3 = (size_t)N * N * N;
+    const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const size_t i = idx3(N, z, y, x);
+        const int i = (z * N + y) * N + x;
+        const int im = y > 0 ? i - N : i, ip = y + 1 < N ? i + N : i;
+        float c0[4], cm[4], cp[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float* f = a + c * n3;
-            float v = 2.0f * f[i];
-            if (y > 0) v += f[i - N];
-            if (y + 1 < N) v += f[i + N];
+            c0[c] = a[c * n3 + i];
+            cm[c] = a[c * n3 + im];
+            cp[c] = a[c * n3 + ip];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float v = 2.0f * c0[c];
+            v = v + (y > 0 ? cm[c] : 0.0f);
+            v = v + (y + 1 < N ? cp[c] : 0.0f);
             out[c * n3 + i] = v;
         }
     }
@@ -242,37 +271,42 @@ __global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const float* __res
 __global__ void __launch_bounds__(256) k_sep_z(int N, int zc, const float* __restrict__ a,
                                                const uint8_t* __restrict__ fm, const float* __restrict__ prm,
                                                int mode, float* __restrict__ out) {
-    const size_t n3 = (size_t)N * N * N;
-    const size_t sz = (size_t)N * N;
+    const int n3 = N * N * N;
+    const int sz = N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
     const float dt = prm[P_DT];
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const size_t i = idx3(N, z, y, x);
-        if (mode == 1 && fm[i] == 0) continue;
+        const int i = (z * N + y) * N + x;
+        const int idx[3] = {z > 0 ? i - sz : i, i, z + 1 < N ? i + sz : i};
+        const bool inb[3] = {z > 0, true, z + 1 < N};
+        float v[3][4];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[k][c] = a[c * n3 + idx[k]];
+        const int m0 = mode == 1 ? (int)fm[i] : 1;
+        float o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
+        if (mode == 1) { o1 = out[n3 + i]; o2 = out[2 * n3 + i]; o3 = out[3 * n3 + i]; }
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
 #pragma unroll
-        for (int k = -1; k <= 1; ++k) {
-            const int zz = z + k;
-            if (zz >= 0 && zz < N) {
-                const float q = (float)(2 - (k < 0 ? -k : k));
-                const size_t j = i + (ptrdiff_t)k * (ptrdiff_t)sz;
-                s0 += q * a[j];
-                s1 += q * a[n3 + j];
-                s2 += q * a[2 * n3 + j];
-                ws += q * a[3 * n3 + j];
-            }
+        for (int k = 0; k < 3; ++k) {
+            const float q = k == 1 ? 2.0f : 1.0f;
+            s0 = s0 + (inb[k] ? q * v[k][0] : 0.0f);
+            s1 = s1 + (inb[k] ? q * v[k][1] : 0.0f);
+            s2 = s2 + (inb[k] ? q * v[k][2] : 0.0f);
+            ws = ws + (inb[k] ? q * v[k][3] : 0.0f);
         }
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
         if (ws > 0.0f) { r0 = s0 / ws; r1 = s1 / ws; r2 = s2 / ws; }
         if (mode == 0) {
             out[i] = r0; out[n3 + i] = r1; out[2 * n3 + i] = r2;
-        } else {
-            out[n3 + i] = out[n3 + i] + dt * r0;
-            out[2 * n3 + i] = out[2 * n3 + i] + dt * r1;
-            out[3 * n3 + i] = out[3 * n3 + i] + dt * r2;
+        } else if (m0 != 0) {
+            out[n3 + i] = o1 + dt * r0;
+            out[2 * n3 + i] = o2 + dt * r1;
+            out[3 * n3 + i] = o3 + dt * r2;
         }
     }
 }

@@ -144,12 +144,16 @@ __global__ void k_clear_changed(DevState* st) {
 // residual restriction (k_resrestrict :306-316) + optional zero-guess red sweep at the coarse
 // level (k_restrict_red :444-459)
 __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I, bool do_red) {
-    float acc = 0.0f;
+    float res[8];
+#pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int z = 2 * K + (k >> 2), y = 2 * J + ((k >> 1) & 1), x = 2 * I + (k & 1);
         const size_t o = lcell(F.m, z, y, x) * 8 + c;
-        acc += F.r[o] - F.diag[o] * F.x[o] - offsum(F, c, z, y, x);
+        res[k] = F.r[o] - F.diag[o] * F.x[o] - offsum(F, c, z, y, x);
     }
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += res[k];
     acc = 0.125f * acc;
     const size_t oc = lcell(C.m, K, J, I) * 8 + c;
     C.r[oc] = acc;
@@ -162,16 +166,17 @@ __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c,
 // constant prolongation (k_prolong :319-325)
 __device__ __forceinline__ void prolong_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I) {
     const size_t o = lcell(F.m, K, J, I) * 8 + c;
+    const float x = F.x[o], xc = C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
     if (F.diag[o] <= F.dmin) return;
-    F.x[o] = F.x[o] + C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
+    F.x[o] = x + xc;
 }
 
-// x value of a fine cell after the constant prolongation, without storing it
-__device__ __forceinline__ float xp(const Lvl& F, const Lvl& C, int c, int K, int J, int I) {
-    const size_t o = lcell(F.m, K, J, I) * 8 + c;
-    const float x = F.x[o];
-    if (F.diag[o] <= F.dmin) return x;
-    return x + C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
+// x value of a level cell after the constant prolongation (prolong skips cells with
+// diag <= dmin), without storing it
+__device__ __forceinline__ float xp(const Lvl& F, const Lvl& C, int c, size_t o, int K, int J, int I) {
+    const float x = F.x[o * 8 + c], dg = F.diag[o * 8 + c];
+    const float xc = C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
+    return dg <= F.dmin ? x : x + xc;
 }
 
 // prolongation fused into the black post-sweep: equal to prolonging every cell, then sweeping
@@ -181,16 +186,21 @@ __device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, i
     const int m = F.m;
     const size_t o = lcell(m, K, J, I);
     const float d = F.diag[o * 8 + c];
-    if (d <= F.dmin) { F.x[o * 8 + c] = 0.0f; return; }
-    const size_t sy = (size_t)m, sz = (size_t)m * m;
+    const LNb n = level_neighbours(m, o, K, J, I);
+    const int nI[6] = {I + 1, I - 1, I, I, I, I}, nJ[6] = {J, J, J + 1, J - 1, J, J},
+              nK[6] = {K, K, K, K, K + 1, K - 1};
+    float t[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        // coordinates of the clamped neighbour (the cell itself where it is missing)
+        const int kI = n.e[k] ? nI[k] : I, kJ = n.e[k] ? nJ[k] : J, kK = n.e[k] ? nK[k] : K;
+        t[k] = link_coef(F, n, o, k, c) * xp(F, C, c, n.c[k], kK, kJ, kI);
+    }
     float a = 0.0f;
-    if (I + 1 < m) a += F.off[o * 24 + c] * xp(F, C, c, K, J, I + 1);
-    if (I > 0) a += F.off[(o - 1) * 24 + c] * xp(F, C, c, K, J, I - 1);
-    if (J + 1 < m) a += F.off[o * 24 + 8 + c] * xp(F, C, c, K, J + 1, I);
-    if (J > 0) a += F.off[(o - sy) * 24 + 8 + c] * xp(F, C, c, K, J - 1, I);
-    if (K + 1 < m) a += F.off[o * 24 + 16 + c] * xp(F, C, c, K + 1, J, I);
-    if (K > 0) a += F.off[(o - sz) * 24 + 16 + c] * xp(F, C, c, K - 1, J, I);
-    F.x[o * 8 + c] = (F.r[o * 8 + c] - a) / d;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a = a + (n.e[k] ? t[k] : 0.0f);
+    const float r = F.r[o * 8 + c];
+    F.x[o * 8 + c] = d <= F.dmin ? 0.0f : (r - a) / d;
 }
 
 // half-size colour launch: item t -> (c, K, J, I) with (I + J + K) & 1 == color (m even)

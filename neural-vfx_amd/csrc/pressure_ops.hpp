@@ -17,50 +17,53 @@ constexpr int INTERIOR = 128;
 
 __device__ __forceinline__ int axstride(int N, int ax) { return ax == 0 ? 1 : (ax == 1 ? N : N * N); }
 
-// g_ax (pressure.py:54-65) at linear cell i
-__device__ __forceinline__ float g_ax(const float* __restrict__ p, const uint8_t* __restrict__ fm, int i, int st,
-                                      int ax) {
-    const int m = fm[i];
-    const float c = p[i];
-    float g = 0.0f;
-    if ((m >> (2 * ax)) & 1) g += p[i + st] - c;
-    if ((m >> (2 * ax + 1)) & 1) g += c - p[i - st];
-    return g;
+// Branch-free forms.  A load guarded by a bounds or mask test is issued only after the test's own
+// load returns, so each guarded load costs a full memory round trip; instead every load below
+// goes to an in-bounds address (clamped to the cell itself when the neighbour is missing) and the
+// open-face tests become 0/1 multipliers: x * 1 == x and d + (+-0) == d, so the value equals the
+// reference's branching form bit for bit (for finite inputs).
+
+// neighbour indices along axis ax of cell i at coordinate co: +1, +2, -1, -2 (clamped to i)
+struct Nb {
+    int p1, p2, m1, m2;
+};
+
+__device__ __forceinline__ Nb neighbours(int N, int i, int co, int st) {
+    Nb n;
+    n.p1 = co + 1 < N ? i + st : i;
+    n.p2 = co + 2 < N ? i + 2 * st : n.p1;
+    n.m1 = co >= 1 ? i - st : i;
+    n.m2 = co >= 2 ? i - 2 * st : n.m1;
+    return n;
 }
 
-// kmul along one axis with every face open: d += gi + g+; d -= gi + g-
-__device__ __forceinline__ float kmul_axis_open(const float* __restrict__ p, int i, int st, float d) {
-    const float m2 = p[i - 2 * st], m1 = p[i - st], c = p[i], p1 = p[i + st], p2 = p[i + 2 * st];
-    const float gi = (p1 - c) + (c - m1);
-    const float gp = (p2 - p1) + (p1 - c);
-    const float gm = (c - m1) + (m1 - m2);
-    d += gi + gp;
-    d -= gi + gm;
-    return d;
+__device__ __forceinline__ float bitf(int m, int b) { return (float)((m >> b) & 1); }
+
+// g_ax (pressure.py:54-65) at cell i (coordinate co along ax)
+__device__ __forceinline__ float g_ax_xyz(const float* __restrict__ p, int m, int N, int i, int co, int ax) {
+    const Nb n = neighbours(N, i, co, axstride(N, ax));
+    const float c = p[i], p1 = p[n.p1], m1 = p[n.m1];
+    return (0.0f + bitf(m, 2 * ax) * (p1 - c)) + bitf(m, 2 * ax + 1) * (c - m1);
 }
 
-// kmul (pressure.py:68-80): (K p)_i at a fluid cell with mask m = fm[i]
-__device__ __forceinline__ float kmul_m(const float* __restrict__ p, const uint8_t* __restrict__ fm, int N, int i,
-                                        int m) {
+// kmul (pressure.py:68-80): (K p)_i at a fluid cell (x, y, z) with mask m = fm[i]
+__device__ __forceinline__ float kmul_xyz(const float* __restrict__ p, const uint8_t* __restrict__ fm, int N, int x,
+                                          int y, int z, int i, int m) {
     float d = 0.0f;
-    if (m & INTERIOR) {
-        d = kmul_axis_open(p, i, 1, d);
-        d = kmul_axis_open(p, i, N, d);
-        d = kmul_axis_open(p, i, N * N, d);
-        return -d;
-    }
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const int st = axstride(N, ax);
-        const float gi = g_ax(p, fm, i, st, ax);
-        if ((m >> (2 * ax)) & 1) d += gi + g_ax(p, fm, i + st, st, ax);
-        if ((m >> (2 * ax + 1)) & 1) d -= gi + g_ax(p, fm, i - st, st, ax);
+        const int co = ax == 0 ? x : (ax == 1 ? y : z);
+        const Nb n = neighbours(N, i, co, axstride(N, ax));
+        const int mp = fm[n.p1], mm = fm[n.m1];
+        const float c = p[i], p1 = p[n.p1], p2 = p[n.p2], m1 = p[n.m1], m2 = p[n.m2];
+        const float op = bitf(m, 2 * ax), om = bitf(m, 2 * ax + 1);
+        const float gi = (0.0f + op * (p1 - c)) + om * (c - m1);
+        const float gp = (0.0f + bitf(mp, 2 * ax) * (p2 - p1)) + bitf(mp, 2 * ax + 1) * (p1 - c);
+        const float gm = (0.0f + bitf(mm, 2 * ax) * (c - m1)) + bitf(mm, 2 * ax + 1) * (m1 - m2);
+        d = d + op * (gi + gp);
+        d = d - om * (gi + gm);
     }
     return -d;
-}
-
-__device__ __forceinline__ float kmul(const float* __restrict__ p, const uint8_t* __restrict__ fm, int N, int i) {
-    return kmul_m(p, fm, N, i, fm[i]);
 }
 
 // kdiag (pressure.py:83-91)
@@ -74,27 +77,18 @@ __device__ __forceinline__ float kdiag(int m) {
     return s;
 }
 
-// div2 (pressure.py:94-106) of the velocity channels vel[0..2]
-__device__ __forceinline__ float div2(const float* __restrict__ vel, int n3, int m, int N, int i) {
+// div2 (pressure.py:94-106) of the velocity channels vel[0..2] at cell (x, y, z), mask m
+__device__ __forceinline__ float div2_xyz(const float* __restrict__ vel, int n3, int m, int N, int x, int y, int z,
+                                          int i) {
     float d = 0.0f;
-    if (m & INTERIOR) {
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const float* va = vel + ax * n3;
-            const int st = axstride(N, ax);
-            const float u = va[i];
-            d += u + va[i + st];
-            d -= u + va[i - st];
-        }
-        return d;
-    }
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const float* va = vel + ax * n3;
-        const int st = axstride(N, ax);
-        const float u = va[i];
-        if ((m >> (2 * ax)) & 1) d += u + va[i + st];
-        if ((m >> (2 * ax + 1)) & 1) d -= u + va[i - st];
+        const int co = ax == 0 ? x : (ax == 1 ? y : z);
+        const Nb n = neighbours(N, i, co, axstride(N, ax));
+        const float u = va[i], up = va[n.p1], um = va[n.m1];
+        d = d + bitf(m, 2 * ax) * (u + up);
+        d = d - bitf(m, 2 * ax + 1) * (u + um);
     }
     return d;
 }
@@ -167,32 +161,58 @@ __device__ __forceinline__ float kmul_tile(const float* P, const uint8_t* M, int
 // halo index of fine cell (gx, gy, gz) relative to the tile origin (ox, oy, oz) (halo excluded)
 __device__ __forceinline__ int tile_h(int lx, int ly, int lz) { return ((lz + 2) * HY + (ly + 2)) * HX + (lx + 2); }
 
-// offsum (pressure.py:271-287) in the component-fastest layout
-__device__ __forceinline__ float offsum(const Lvl& L, int c, int K, int J, int I) {
-    const int m = L.m;
-    const size_t o = lcell(m, K, J, I);
+// The six neighbour cells of a level cell, clamped to the cell itself where missing, with the
+// existence flags (the tests of offsum, pressure.py:275-286).
+struct LNb {
+    size_t c[6];   // +I, -I, +J, -J, +K, -K
+    bool e[6];
+};
+
+__device__ __forceinline__ LNb level_neighbours(int m, size_t o, int K, int J, int I) {
+    LNb n;
     const size_t sy = (size_t)m, sz = (size_t)m * m;
+    n.e[0] = I + 1 < m; n.c[0] = n.e[0] ? o + 1 : o;
+    n.e[1] = I > 0;     n.c[1] = n.e[1] ? o - 1 : o;
+    n.e[2] = J + 1 < m; n.c[2] = n.e[2] ? o + sy : o;
+    n.e[3] = J > 0;     n.c[3] = n.e[3] ? o - sy : o;
+    n.e[4] = K + 1 < m; n.c[4] = n.e[4] ? o + sz : o;
+    n.e[5] = K > 0;     n.c[5] = n.e[5] ? o - sz : o;
+    return n;
+}
+
+// coupling coefficient of link k (0..5) of component c: +ax links are stored at the cell, -ax
+// links at the -ax neighbour
+__device__ __forceinline__ float link_coef(const Lvl& L, const LNb& n, size_t o, int k, int c) {
+    const int ax = k >> 1;
+    return L.off[((k & 1) ? n.c[k] : o) * 24 + ax * 8 + c];
+}
+
+// offsum (pressure.py:271-287) in the component-fastest layout, every load unconditional
+__device__ __forceinline__ float offsum(const Lvl& L, int c, int K, int J, int I) {
+    const size_t o = lcell(L.m, K, J, I);
+    const LNb n = level_neighbours(L.m, o, K, J, I);
+    float t[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) t[k] = link_coef(L, n, o, k, c) * L.x[n.c[k] * 8 + c];
     float a = 0.0f;
-    if (I + 1 < m) a += L.off[o * 24 + c] * L.x[(o + 1) * 8 + c];
-    if (I > 0) a += L.off[(o - 1) * 24 + c] * L.x[(o - 1) * 8 + c];
-    if (J + 1 < m) a += L.off[o * 24 + 8 + c] * L.x[(o + sy) * 8 + c];
-    if (J > 0) a += L.off[(o - sy) * 24 + 8 + c] * L.x[(o - sy) * 8 + c];
-    if (K + 1 < m) a += L.off[o * 24 + 16 + c] * L.x[(o + sz) * 8 + c];
-    if (K > 0) a += L.off[(o - sz) * 24 + 16 + c] * L.x[(o - sz) * 8 + c];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a = a + (n.e[k] ? t[k] : 0.0f);
     return a;
 }
 
 // one GS update (k_smooth pressure.py:290-303); returns the new value
-__device__ __forceinline__ float smooth_item(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
+__device__ __forceinline__ float smooth_value(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
     const size_t o = lcell(L.m, K, J, I) * 8 + c;
     const float d = L.diag[o];
-    float v = 0.0f;
-    if (d > L.dmin) {
-        float acc = L.r[o];
-        if (!zero_init) acc -= offsum(L, c, K, J, I);
-        v = acc / d;
-    }
-    L.x[o] = v;
+    const float r = L.r[o];
+    const float a = zero_init ? 0.0f : offsum(L, c, K, J, I);
+    const float acc = zero_init ? r : r - a;
+    return d <= L.dmin ? 0.0f : acc / d;
+}
+
+__device__ __forceinline__ float smooth_item(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
+    const float v = smooth_value(L, c, K, J, I, zero_init);
+    L.x[lcell(L.m, K, J, I) * 8 + c] = v;
     return v;
 }
 
