@@ -37,6 +37,8 @@ extern "C" {
 #define VFX_OPT_RESTARTS 2       /* CG restarts from the true residual while it exceeds the
                                     relative tolerance (needs P_REL_TOL > 0); default 0 */
 #define VFX_OPT_COARSE_M 3       /* multigrid levels with m <= this run in one workgroup */
+#define VFX_OPT_STD_STENCIL 4    /* 0: always read the level operators (no standard-cell table);
+                                    results are identical either way, default 1 */
 
 typedef struct vfx_ctx vfx_ctx;
 

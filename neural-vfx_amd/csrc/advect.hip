@@ -3,6 +3,10 @@
 // Restates vfx/sim3d/kernels.py:66-178 (trilinear helpers, RK2 midpoint trace, advect_sl,
 // advect_mc with the 8-corner clamp of Selle et al. 2008) and solver.py:40-76 (fluid density
 // sums before / after advection and their ratio).  All channels share one backtrace.
+//
+// One cell per thread, channel count a template parameter: every gather of a trace step (3
+// channels x 8 corners) and of the final samples (C channels x 8 corners) is loaded before any
+// of it is used, so each RK2 trace costs three dependent memory rounds, not one per load.
 #include "sim3d_kernels.hpp"
 
 namespace vfx {
@@ -20,116 +24,163 @@ __device__ __forceinline__ int base_idx(float g, int n) {
 }
 
 struct Corner {
-    size_t o;      // linear index of the (z0, y0, x0) corner
+    int o;      // linear index of the (z0, y0, x0) corner
     float fx, fy, fz;
 };
 
 __device__ __forceinline__ Corner corner(int N, float px, float py, float pz) {
-    int x0 = base_idx(px, N), y0 = base_idx(py, N), z0 = base_idx(pz, N);
+    const int x0 = base_idx(px, N), y0 = base_idx(py, N), z0 = base_idx(pz, N);
     Corner c;
-    c.o = idx3(N, z0, y0, x0);
+    c.o = (z0 * N + y0) * N + x0;
     c.fx = clampf_(px, 0.0f, (float)(N - 1)) - (float)x0;
     c.fy = clampf_(py, 0.0f, (float)(N - 1)) - (float)y0;
     c.fz = clampf_(pz, 0.0f, (float)(N - 1)) - (float)z0;
     return c;
 }
 
+// the 8 corner values of a trilinear sample, in (z, y, x) bit order 000 .. 111
+struct Oct {
+    float v[8];
+};
+
+__device__ __forceinline__ Oct gather(const float* __restrict__ f, int N, int o) {
+    const int sy = N, sz = N * N;
+    Oct g;
+    g.v[0] = f[o];
+    g.v[1] = f[o + 1];
+    g.v[2] = f[o + sy];
+    g.v[3] = f[o + sy + 1];
+    g.v[4] = f[o + sz];
+    g.v[5] = f[o + sz + 1];
+    g.v[6] = f[o + sz + sy];
+    g.v[7] = f[o + sz + sy + 1];
+    return g;
+}
+
 // tri (kernels.py:77-85): x first, then y, then z, in the reference's operation order.
-__device__ __forceinline__ float tri(const float* __restrict__ f, int N, const Corner& c) {
-    const size_t sy = (size_t)N, sz = (size_t)N * N;
-    const float* p = f + c.o;
-    float a00 = p[0] * (1.0f - c.fx) + p[1] * c.fx;
-    float a01 = p[sy] * (1.0f - c.fx) + p[sy + 1] * c.fx;
-    float a10 = p[sz] * (1.0f - c.fx) + p[sz + 1] * c.fx;
-    float a11 = p[sz + sy] * (1.0f - c.fx) + p[sz + sy + 1] * c.fx;
-    float a0 = a00 * (1.0f - c.fy) + a01 * c.fy;
-    float a1 = a10 * (1.0f - c.fy) + a11 * c.fy;
+__device__ __forceinline__ float lerp3(const Oct& g, const Corner& c) {
+    const float a00 = g.v[0] * (1.0f - c.fx) + g.v[1] * c.fx;
+    const float a01 = g.v[2] * (1.0f - c.fx) + g.v[3] * c.fx;
+    const float a10 = g.v[4] * (1.0f - c.fx) + g.v[5] * c.fx;
+    const float a11 = g.v[6] * (1.0f - c.fx) + g.v[7] * c.fx;
+    const float a0 = a00 * (1.0f - c.fy) + a01 * c.fy;
+    const float a1 = a10 * (1.0f - c.fy) + a11 * c.fy;
     return a0 * (1.0f - c.fz) + a1 * c.fz;
 }
 
-// trace (kernels.py:115-120): RK2 midpoint over signed s (cells per m/s).
-__device__ __forceinline__ void trace(const float* __restrict__ vel, int N, size_t n3, float px, float py,
-                                      float pz, float s, float* q) {
-    Corner c1 = corner(N, px, py, pz);
-    float v1x = tri(vel, N, c1), v1y = tri(vel + n3, N, c1), v1z = tri(vel + 2 * n3, N, c1);
-    float h = 0.5f * s;
-    Corner c2 = corner(N, px + h * v1x, py + h * v1y, pz + h * v1z);
-    float v2x = tri(vel, N, c2), v2y = tri(vel + n3, N, c2), v2z = tri(vel + 2 * n3, N, c2);
-    q[0] = px + s * v2x;
-    q[1] = py + s * v2y;
-    q[2] = pz + s * v2z;
+// velocity at p (sample_vel, kernels.py:100-112)
+__device__ __forceinline__ void sample_vel(const float* __restrict__ vel, int N, int n3, const Corner& c, float* v) {
+    const Oct gx = gather(vel, N, c.o), gy = gather(vel + n3, N, c.o), gz = gather(vel + 2 * n3, N, c.o);
+    v[0] = lerp3(gx, c);
+    v[1] = lerp3(gy, c);
+    v[2] = lerp3(gz, c);
+}
+
+// trace (kernels.py:115-120): RK2 midpoint over signed s (cells per m/s).  Two traces (back and
+// forward) run side by side so their gathers overlap.
+__device__ __forceinline__ void trace2(const float* __restrict__ vel, int N, int n3, float px, float py, float pz,
+                                       float sa, float sb, float* qa, float* qb) {
+    const Corner c0 = corner(N, px, py, pz);
+    float v1[3];
+    sample_vel(vel, N, n3, c0, v1);
+    const float ha = 0.5f * sa, hb = 0.5f * sb;
+    const Corner ca = corner(N, px + ha * v1[0], py + ha * v1[1], pz + ha * v1[2]);
+    const Corner cb = corner(N, px + hb * v1[0], py + hb * v1[1], pz + hb * v1[2]);
+    float va[3], vb[3];
+    sample_vel(vel, N, n3, ca, va);
+    sample_vel(vel, N, n3, cb, vb);
+    qa[0] = px + sa * va[0]; qa[1] = py + sa * va[1]; qa[2] = pz + sa * va[2];
+    qb[0] = px + sb * vb[0]; qb[1] = py + sb * vb[1]; qb[2] = pz + sb * vb[2];
+}
+
+__device__ __forceinline__ void trace1(const float* __restrict__ vel, int N, int n3, float px, float py, float pz,
+                                       float s, float* q) {
+    const Corner c0 = corner(N, px, py, pz);
+    float v1[3];
+    sample_vel(vel, N, n3, c0, v1);
+    const float h = 0.5f * s;
+    const Corner c1 = corner(N, px + h * v1[0], py + h * v1[1], pz + h * v1[2]);
+    float v2[3];
+    sample_vel(vel, N, n3, c1, v2);
+    q[0] = px + s * v2[0];
+    q[1] = py + s * v2[1];
+    q[2] = pz + s * v2[2];
 }
 
 // Pass 1: semi-Lagrangian backward value of C channels -> hat.
-__global__ void __launch_bounds__(256) k_adv_sl(int N, int zc, int C, const float* __restrict__ src,
-                                                const float* __restrict__ vel, const float* __restrict__ prm,
-                                                float inv_dx, float* __restrict__ hat) {
-    const size_t n3 = (size_t)N * N * N;
+template <int C>
+__global__ void __launch_bounds__(256) k_adv_sl(int N, const float* __restrict__ src, const float* __restrict__ vel,
+                                                const float* __restrict__ prm, float inv_dx, float* __restrict__ hat) {
+    const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
     if (x >= N || y >= N) return;
     const float s = prm[P_DT] * inv_dx;
-    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
-    for (int z = blockIdx.z * zc; z < z1; ++z) {
-        float q[3];
-        trace(vel, N, n3, (float)x, (float)y, (float)z, -s, q);
-        Corner c = corner(N, q[0], q[1], q[2]);
-        const size_t i = idx3(N, z, y, x);
-        for (int ch = 0; ch < C; ++ch) hat[ch * n3 + i] = tri(src + ch * n3, N, c);
-    }
+    float q[3];
+    trace1(vel, N, n3, (float)x, (float)y, (float)z, -s, q);
+    const Corner c = corner(N, q[0], q[1], q[2]);
+    Oct g[C];
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) g[ch] = gather(src + ch * n3, N, c.o);
+    const int i = (z * N + y) * N + x;
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) hat[ch * n3 + i] = lerp3(g[ch], c);
 }
 
 // Pass 2: MacCormack correction with the 8-corner clamp; optionally the block partial sums of
 // fluid density before (src[0]) and after (out[0]) advection (solver.py:46-65), in double.
-__global__ void __launch_bounds__(256) k_adv_mc(int N, int zc, int C, const float* __restrict__ src,
-                                                const float* __restrict__ vel, const float* __restrict__ prm,
-                                                float inv_dx, const float* __restrict__ hat,
-                                                float* __restrict__ out, const uint8_t* __restrict__ fm,
-                                                double* __restrict__ part, int nparts) {
+template <int C>
+__global__ void __launch_bounds__(256) k_adv_mc(int N, const float* __restrict__ src, const float* __restrict__ vel,
+                                                const float* __restrict__ prm, float inv_dx,
+                                                const float* __restrict__ hat, float* __restrict__ out,
+                                                const uint8_t* __restrict__ fm, double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
-    const size_t n3 = (size_t)N * N * N;
+    const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
     const bool live = x < N && y < N;
-    const float s = prm[P_DT] * inv_dx;
     double sa = 0.0, sb = 0.0;
-    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     if (live) {
-        for (int z = blockIdx.z * zc; z < z1; ++z) {
-            float q[3], r[3];
-            trace(vel, N, n3, (float)x, (float)y, (float)z, -s, q);
-            trace(vel, N, n3, (float)x, (float)y, (float)z, s, r);
-            const size_t qo = idx3(N, base_idx(q[2], N), base_idx(q[1], N), base_idx(q[0], N));
-            Corner cr = corner(N, r[0], r[1], r[2]);
-            const size_t i = idx3(N, z, y, x);
-            const size_t sy = (size_t)N, sz = (size_t)N * N;
-            for (int ch = 0; ch < C; ++ch) {
-                const float* sc = src + ch * n3;
-                const float h = hat[ch * n3 + i];
-                const float bar = tri(hat + ch * n3, N, cr);
-                const float s0 = sc[i];
-                float val = h + 0.5f * (s0 - bar);
-                const float* p = sc + qo;
-                float lo = p[0], hi = lo, v;
-                v = p[1];            lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                v = p[sy];           lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                v = p[sy + 1];       lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                v = p[sz];           lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                v = p[sz + 1];       lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                v = p[sz + sy];      lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                v = p[sz + sy + 1];  lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                if (val < lo || val > hi) val = h;
-                out[ch * n3 + i] = val;
-                if (ch == 0 && part != nullptr && fm[i] != 0) {
-                    sa += (double)s0;
-                    sb += (double)val;
-                }
+        const float s = prm[P_DT] * inv_dx;
+        float q[3], r[3];
+        trace2(vel, N, n3, (float)x, (float)y, (float)z, -s, s, q, r);
+        const int qo = (base_idx(q[2], N) * N + base_idx(q[1], N)) * N + base_idx(q[0], N);
+        const Corner cr = corner(N, r[0], r[1], r[2]);
+        const int i = (z * N + y) * N + x;
+        Oct gs[C], gh[C];
+        float h[C], s0[C];
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            gs[ch] = gather(src + ch * n3, N, qo);
+            gh[ch] = gather(hat + ch * n3, N, cr.o);
+            h[ch] = hat[ch * n3 + i];
+            s0[ch] = src[ch * n3 + i];
+        }
+        const int fmi = part != nullptr ? (int)fm[i] : 0;
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            const float bar = lerp3(gh[ch], cr);
+            float val = h[ch] + 0.5f * (s0[ch] - bar);
+            float lo = gs[ch].v[0], hi = lo;
+#pragma unroll
+            for (int k = 1; k < 8; ++k) {
+                const float v = gs[ch].v[k];
+                lo = v < lo ? v : lo;
+                hi = v > hi ? v : hi;
+            }
+            if (val < lo || val > hi) val = h[ch];
+            out[ch * n3 + i] = val;
+            if (ch == 0 && fmi != 0) {
+                sa = (double)s0[0];
+                sb = (double)val;
             }
         }
     }
     if (part != nullptr) {
-        double a = block_sum(sa, sh);
-        double b = block_sum(sb, sh);
+        const double a = block_sum(sa, sh);
+        const double b = block_sum(sb, sh);
         if (threadIdx.x == 0 && threadIdx.y == 0) {
             const int bl = block_linear();
             part[bl] = a;
@@ -138,33 +189,77 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, int zc, int C, const floa
     }
 }
 
-// ratio = a / b if a > 0 and b > 1e-30, else 1 (solver.py:67-74).
-__global__ void __launch_bounds__(256) k_mass_fin(const double* __restrict__ part, int nparts,
-                                                  DevState* __restrict__ st) {
+// First reduction level: block b sums partials [b*per, (b+1)*per) of both halves.
+__global__ void __launch_bounds__(256) k_mass_part2(const double* __restrict__ part, int nparts, int per,
+                                                    double* __restrict__ part2, int n2) {
     __shared__ double sh[4];
+    const int b = blockIdx.x;
+    const int lo = b * per, hi = min(lo + per, nparts);
+    double a = 0.0, c = 0.0;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += 256) {
+        a += part[i];
+        c += part[nparts + i];
+    }
+    const double ra = block_sum(a, sh);
+    const double rc = block_sum(c, sh);
+    if (threadIdx.x == 0) {
+        part2[b] = ra;
+        part2[n2 + b] = rc;
+    }
+}
+
+// ratio = a / b if a > 0 and b > 1e-30, else 1 (solver.py:67-74).
+__global__ void __launch_bounds__(FIN_THREADS) k_mass_fin(const double* __restrict__ part, int nparts,
+                                                          DevState* __restrict__ st) {
+    __shared__ double sh[FIN_THREADS / 64];
     double a = reduce_parts_sum(part, nparts, sh);
     double b = reduce_parts_sum(part + nparts, nparts, sh);
     if (threadIdx.x == 0) st->s[S_MASS_RATIO] = (a > 0.0 && b > 1e-30) ? a / b : 1.0;
 }
 
-void launch_advect(const Geom& g, int C, const float* src, const float* vel, const float* prm, float inv_dx,
-                   float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
-    hipLaunchKernelGGL(k_adv_sl, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat);
-    hipLaunchKernelGGL(k_adv_mc, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat, out, fm, part,
-                       nblocks(g));
+static dim3 cell_grid(const Geom& g) { return dim3(g.grid.x, g.grid.y, (unsigned)g.N); }
+
+int advect_parts(const Geom& g) { return (int)(g.grid.x * g.grid.y) * g.N; }
+
+template <int C>
+static void launch_pass_t(int pass, const Geom& g, const float* src, const float* vel, const float* prm, float inv_dx,
+                          float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
+    if (pass == 0)
+        hipLaunchKernelGGL(k_adv_sl<C>, cell_grid(g), g.block, 0, s, g.N, src, vel, prm, inv_dx, hat);
+    else
+        hipLaunchKernelGGL(k_adv_mc<C>, cell_grid(g), g.block, 0, s, g.N, src, vel, prm, inv_dx, hat, out, fm, part,
+                           advect_parts(g));
 }
 
 void launch_adv_pass(int pass, const Geom& g, int C, const float* src, const float* vel, const float* prm,
                      float inv_dx, float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
-    if (pass == 0)
-        hipLaunchKernelGGL(k_adv_sl, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat);
-    else
-        hipLaunchKernelGGL(k_adv_mc, g.grid, g.block, 0, s, g.N, g.zc, C, src, vel, prm, inv_dx, hat, out, fm, part,
-                           nblocks(g));
+    switch (C) {
+        case 1: launch_pass_t<1>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
+        case 2: launch_pass_t<2>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
+        case 3: launch_pass_t<3>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
+        case 4: launch_pass_t<4>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
+        case 5: launch_pass_t<5>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
+        default: launch_pass_t<6>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
+    }
 }
 
-void launch_mass_fin(const Geom& g, const double* part, DevState* st, hipStream_t s) {
-    hipLaunchKernelGGL(k_mass_fin, dim3(1), dim3(256), 0, s, part, nblocks(g), st);
+void launch_advect(const Geom& g, int C, const float* src, const float* vel, const float* prm, float inv_dx,
+                   float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
+    launch_adv_pass(0, g, C, src, vel, prm, inv_dx, hat, out, fm, part, s);
+    launch_adv_pass(1, g, C, src, vel, prm, inv_dx, hat, out, fm, part, s);
 }
+
+// part holds 2 * advect_parts(g) partials; part2 (after them) 2 * MASS_PART2 partials.
+constexpr int MASS_PART2 = 256;
+
+void launch_mass_fin(const Geom& g, double* part, DevState* st, hipStream_t s) {
+    const int n = advect_parts(g);
+    const int per = (n + MASS_PART2 - 1) / MASS_PART2;
+    double* part2 = part + 2 * (size_t)n;
+    hipLaunchKernelGGL(k_mass_part2, dim3(MASS_PART2), dim3(256), 0, s, part, n, per, part2, MASS_PART2);
+    hipLaunchKernelGGL(k_mass_fin, dim3(1), dim3(FIN_THREADS), 0, s, part2, MASS_PART2, st);
+}
+
+size_t mass_part_doubles(const Geom& g) { return 2 * (size_t)advect_parts(g) + 2 * (size_t)MASS_PART2; }
 
 }  // namespace vfx

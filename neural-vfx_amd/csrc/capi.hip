@@ -252,7 +252,7 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     // one pool: floats first (every array a multiple of 8 floats, so 32-byte aligned), then
     // doubles, then the mask
     const size_t nf = n3 * (6 + 6 + 3 + 3 + 3 + 4 + 4) + 2 * (size_t)P_SIZE + n3 * 7 + P.level_floats();
-    const size_t nd = (size_t)P.nparts * (2 + 3 + 1) + sizeof(DevState) / sizeof(double) + 8;
+    const size_t nd = mass_part_doubles(c->g) + (size_t)P.nparts * (3 + 1) + sizeof(DevState) / sizeof(double) + 8;
     const size_t bytes = nf * 4 + 64 + nd * 8 + 64 + n3 + 64;
     if (!hip_ok(hipMalloc(&c->pool, bytes), "device allocation")) { delete c; return VFX_E_HIP; }
     if (!hip_ok(hipMemset(c->pool, 0, bytes), "device clear")) { hipFree(c->pool); delete c; return VFX_E_HIP; }
@@ -276,14 +276,14 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     P.bind_levels(f); f += P.level_floats();
     uintptr_t a = ((uintptr_t)f + 63) & ~(uintptr_t)63;
     double* d = (double*)a;
-    c->mass_part = d; d += 2 * (size_t)P.nparts;
+    c->mass_part = d; d += mass_part_doubles(c->g);
     P.part = d; d += 3 * (size_t)P.nparts;
     P.part_pq = d; d += (size_t)P.nparts;
     a = ((uintptr_t)d + 63) & ~(uintptr_t)63;
     c->st = (DevState*)a;
     P.st = c->st;
     a = (a + sizeof(DevState) + 63) & ~(uintptr_t)63;
-    P.fm = (uint8_t*)a;
+    P.fm = (uint8_t*)a;   // n3 mask bytes, then zeros: fm[n3] == 0 is read for cells outside the box
     bool ok = hip_ok(hipHostMalloc((void**)&c->ring, sizeof(float) * P_SIZE * HOST_RING, hipHostMallocDefault),
                      "pinned allocation") &&
               hip_ok(hipHostMalloc((void**)&c->stop_pin, sizeof(int) * STOP_RING, hipHostMallocDefault),
@@ -324,6 +324,14 @@ int vfx_set_option(vfx_ctx* c, int key, double value) {
             c->P.S = c->P.nlev - 1;
             for (int l = 0; l < c->P.nlev; ++l)
                 if (c->P.lv[l].m <= cm) { c->P.S = l; break; }
+            return VFX_OK;
+        }
+        case VFX_OPT_STD_STENCIL: {
+            const bool on = value != 0.0;
+            if (on != c->P.use_std) {
+                c->P.use_std = on;
+                c->P.force_setup = true;   // the flags are rebuilt by the next setup
+            }
             return VFX_OK;
         }
         default: set_error("unknown option"); return VFX_E_VALUE;

@@ -27,27 +27,85 @@ namespace vfx {
             const int i = (cz * N + y) * N + x;
 #define CELL_LOOP_END }
 
-constexpr int HN_IT = (HN + TILE_THREADS - 1) / TILE_THREADS;          // halo cells per thread
-constexpr int OWN_IT = TX * TY * TZ / TILE_THREADS;                    // own cells per thread
-constexpr int HBN8_IT = (HBN * 8 + TILE_THREADS - 1) / TILE_THREADS;  // level-1 halo values per thread
+constexpr int HB_IT = (HBN + TILE_THREADS - 1) / TILE_THREADS;   // halo blocks staged per thread
 
 __device__ __forceinline__ int tile_linear() { return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)); }
 
-// halo cell h of the current tile -> global cell index, or -1 outside the box
-__device__ __forceinline__ int halo_global(int N, int h) {
-    const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-    const int gx = (int)blockIdx.x * TX - 2 + hx, gy = (int)blockIdx.y * TY - 2 + hy, gz = (int)blockIdx.z * TZ - 2 + hz;
-    if (gx < 0 || gy < 0 || gz < 0 || gx >= N || gy >= N || gz >= N) return -1;
-    return (gz * N + gy) * N + gx;
+// Tiles are staged in 2x2x2 blocks (the level-1 cells): halo block b (0..HBN-1) is the level-1
+// block (I, J, K) = first block of the tile - 1 + (b % HBX, b / HBX % HBY, b / (HBX HBY)), read
+// as four rows of two cells (float2 values, two mask bytes).  N is even, so a block is wholly
+// inside or outside the box; one outside is read at the clamped block (field values, scaled by
+// in = 0) with its mask at n3, the zero bytes past the end of the mask array.  Staging loads are
+// unconditional, so all of a thread's loads are in flight together; only the LDS stores of the
+// last, partial round are predicated.
+struct HBlock {
+    int h0;     // halo index of the block's (0, 0, 0) cell
+    int i0;     // fine index of the (clamped) block's (0, 0, 0) cell
+    int lc;     // level-1 cell of the clamped block
+    bool in;    // block inside the box
+};
+
+__device__ __forceinline__ HBlock halo_block(int N, int b) {
+    const int bi = b % HBX, bj = (b / HBX) % HBY, bk = b / (HBX * HBY);
+    const int mb = N >> 1;
+    const int I = (int)blockIdx.x * BTX - 1 + bi, J = (int)blockIdx.y * BTY - 1 + bj,
+              K = (int)blockIdx.z * BTZ - 1 + bk;
+    const int CI = min(max(I, 0), mb - 1), CJ = min(max(J, 0), mb - 1), CK = min(max(K, 0), mb - 1);
+    HBlock r;
+    r.in = I >= 0 && J >= 0 && K >= 0 && I < mb && J < mb && K < mb;
+    r.h0 = ((2 * bk) * HY + 2 * bj) * HX + 2 * bi;
+    r.i0 = ((2 * CK) * N + 2 * CJ) * N + 2 * CI;
+    r.lc = (CK * mb + CJ) * mb + CI;
+    return r;
 }
 
-// own cell j (0..TX*TY*TZ-1, x fastest) of the current tile -> (global index or -1, halo index)
-__device__ __forceinline__ int own_global(int N, int j, int& h) {
-    const int lx = j % TX, ly = (j / TX) % TY, lz = j / (TX * TY);
-    h = tile_h(lx, ly, lz);
-    const int gx = (int)blockIdx.x * TX + lx, gy = (int)blockIdx.y * TY + ly, gz = (int)blockIdx.z * TZ + lz;
-    if (gx >= N || gy >= N || gz >= N) return -1;
-    return (gz * N + gy) * N + gx;
+// the halo block of own block t (BTX x BTY x BTZ own blocks, one per thread)
+__device__ __forceinline__ int own_hb(int t) {
+    return ((t / (BTX * BTY) + 1) * HBY + (t / BTX) % BTY + 1) * HBX + t % BTX + 1;
+}
+
+// row rw = dy + 2 dz of a block: offset of its dx = 0 cell in the fine grid and in the halo
+__device__ __forceinline__ int row_off(int N, int rw) { return (rw & 1) * N + (rw >> 1) * N * N; }
+__device__ __forceinline__ int row_h(int rw) { return (rw & 1) * HX + (rw >> 1) * HX * HY; }
+
+__device__ __forceinline__ float2 ld2(const float* __restrict__ f, int i) {
+    return *reinterpret_cast<const float2*>(f + i);
+}
+__device__ __forceinline__ void st2(float* __restrict__ f, int i, float a, float b) {
+    *reinterpret_cast<float2*>(f + i) = make_float2(a, b);
+}
+__device__ __forceinline__ int ldm2(const uint8_t* __restrict__ fm, int i) {
+    return *reinterpret_cast<const uint16_t*>(fm + i);
+}
+
+// true when every cell of the thread's own block (halo index h0) is INTERIOR in every lane of the
+// wave: K is then evaluated without the mask (kmul_tile_interior, kdiag = 6)
+__device__ __forceinline__ bool wave_interior(const uint8_t* sm, int h0) {
+    bool in = true;
+#pragma unroll
+    for (int rw = 0; rw < 4; ++rw) {
+        const int m2 = *reinterpret_cast<const uint16_t*>(sm + h0 + row_h(rw));
+        in = in && (m2 & (INTERIOR | (INTERIOR << 8))) == (INTERIOR | (INTERIOR << 8));
+    }
+    return __all(in);
+}
+
+// K p at halo cell h (0 at a solid cell)
+template <bool INT>
+__device__ __forceinline__ float kq(const float* P, const uint8_t* M, int h) {
+    if (INT) return kmul_tile_interior(P, h);
+    return M[h] ? kmul_tile(P, M, h) : 0.0f;
+}
+
+// z0 + T^T x1 at a cell of parity (dz, dy, dx) from hx[c] = HS x_c: hsign(c) x_c = +-(HS x_c)
+// exactly, so the sum equals the reference's c-ordered sum of hsign * x
+__device__ __forceinline__ float inv_sum(float acc, const float* hx, int dz, int dy, int dx) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int e = (c & 1) * dx + ((c >> 1) & 1) * dy + ((c >> 2) & 1) * dz;
+        acc = (e & 1) ? acc - hx[c] : acc + hx[c];
+    }
+    return acc;
 }
 
 // k_rhs (:602-627): r = -d(u*) - K phi0 (phi = 0 without warm start); block max |u*| over fluid
@@ -81,9 +139,9 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const float* __restr
 }
 
 // k_fin_maxu (:730-735) + the CG state reset.
-__global__ void __launch_bounds__(256) k_fin_rhs(const double* __restrict__ part, int nparts,
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restrict__ part, int nparts,
                                                  const float* __restrict__ prm, DevState* __restrict__ st) {
-    __shared__ double sh[4];
+    __shared__ double sh[FIN_THREADS / 64];
     const double mx = reduce_parts_max(part, nparts, sh);
     const double bb = reduce_parts_sum(part + nparts, nparts, sh);
     if (threadIdx.x == 0) {
@@ -96,103 +154,188 @@ __global__ void __launch_bounds__(256) k_fin_rhs(const double* __restrict__ part
     }
 }
 
-// k_upd_p of iteration k-1 (:712-728, when upd) + k_matvec_dot of iteration k (:629-646):
+// k_upd_p of iteration k-1 (:712-728, when UPD) + k_matvec_dot of iteration k (:629-646):
 // p = z + beta p_old, phi += alpha p_old; q = K p; block partial p.q (double).
+template <bool UPD>
 __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float* __restrict__ z,
                                                              const float* __restrict__ p_in, float* __restrict__ p_out,
                                                              float* __restrict__ phi, const uint8_t* __restrict__ fm,
-                                                             float* __restrict__ q, double* __restrict__ part, int upd,
+                                                             float* __restrict__ q, double* __restrict__ part,
                                                              const DevState* __restrict__ st, int k) {
-    __shared__ float sp[HN];
-    __shared__ uint8_t sm[HN];
-    __shared__ double sh[8];
+    __shared__ __align__(16) float sp[HN];
+    __shared__ __align__(16) uint8_t sm[HN];
+    __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
     const float a = (float)st->s[S_ALPHA], b = (float)st->s[S_BETA];
-    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
-        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
-        if (h >= HN) break;
-        const int i = halo_global(N, h);
-        const int ic = i >= 0 ? i : 0;   // every load in bounds; outside cells staged as mask 0, value 0
-        const int m = fm[ic];
-        const float zv = z[ic], pv = p_in[ic];
-        const float v = upd ? zv + b * pv : pv;
-        sp[h] = i >= 0 ? v : 0.0f;
-        sm[h] = (uint8_t)(i >= 0 ? m : 0);
+    const int t = (int)threadIdx.x, n3 = N * N * N;
+    const HBlock ob = halo_block(N, own_hb(t));
+    float2 po[4], ph[4];
+    if (UPD) {
+#pragma unroll
+        for (int rw = 0; rw < 4; ++rw) {
+            po[rw] = ld2(p_in, ob.i0 + row_off(N, rw));
+            ph[rw] = ld2(phi, ob.i0 + row_off(N, rw));
+        }
+    }
+    HBlock hb[HB_IT];
+    float2 hp[HB_IT][4], hz[HB_IT][4];
+    int hm[HB_IT][4];
+#pragma unroll
+    for (int it = 0; it < HB_IT; ++it) {
+        hb[it] = halo_block(N, min(t + it * TILE_THREADS, HBN - 1));
+#pragma unroll
+        for (int rw = 0; rw < 4; ++rw) {
+            const int o = hb[it].i0 + row_off(N, rw);
+            hp[it][rw] = ld2(p_in, o);
+            if (UPD) hz[it][rw] = ld2(z, o);
+            hm[it][rw] = ldm2(fm, hb[it].in ? o : n3);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);   // every load above is issued before the first use
+#pragma unroll
+    for (int it = 0; it < HB_IT; ++it) {
+        if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
+            const float in = hb[it].in ? 1.0f : 0.0f;
+#pragma unroll
+            for (int rw = 0; rw < 4; ++rw) {
+                const float2 pv = hp[it][rw];
+                float v0 = pv.x, v1 = pv.y;
+                if (UPD) {
+                    v0 = hz[it][rw].x + b * pv.x;
+                    v1 = hz[it][rw].y + b * pv.y;
+                }
+                const int h = hb[it].h0 + row_h(rw);
+                st2(sp, h, v0 * in, v1 * in);
+                *reinterpret_cast<uint16_t*>(sm + h) = (uint16_t)hm[it][rw];
+            }
+        }
     }
     __syncthreads();
-    double s = 0.0;
-    _Pragma("unroll") for (int ji_ = 0; ji_ < OWN_IT; ++ji_) {
-        const int j = (int)threadIdx.x + ji_ * TILE_THREADS;
-        int h;
-        const int i = own_global(N, j, h);
-        if (i < 0) continue;
-        const float pn = sp[h];
-        if (upd) {
-            const float po = p_in[i];
-            phi[i] = phi[i] + a * po;
-            p_out[i] = pn;
-        }
-        const float qv = sm[h] ? kmul_tile(sp, sm, h) : 0.0f;
-        q[i] = qv;
-        s += (double)qv * (double)pn;
+    // the thread's own block; a block outside the box is staged as mask 0, value 0 (terms 0)
+    float qv[8];
+    if (wave_interior(sm, ob.h0)) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) qv[c] = kq<true>(sp, sm, ob.h0 + row_h(c >> 1) + (c & 1));
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) qv[c] = kq<false>(sp, sm, ob.h0 + row_h(c >> 1) + (c & 1));
     }
-    const double t = block_sum(s, sh);
-    if (threadIdx.x == 0) part[tile_linear()] = t;
+    double s = 0.0;
+#pragma unroll
+    for (int rw = 0; rw < 4; ++rw) {
+        const int h = ob.h0 + row_h(rw);
+        const float pn0 = sp[h], pn1 = sp[h + 1];
+        s += (double)qv[2 * rw] * (double)pn0;
+        s += (double)qv[2 * rw + 1] * (double)pn1;
+        if (ob.in) {
+            const int i = ob.i0 + row_off(N, rw);
+            if (UPD) {
+                st2(phi, i, ph[rw].x + a * po[rw].x, ph[rw].y + a * po[rw].y);
+                st2(p_out, i, pn0, pn1);
+            }
+            st2(q, i, qv[2 * rw], qv[2 * rw + 1]);
+        }
+    }
+    const double t2 = block_sum(s, sh);
+    if (threadIdx.x == 0) part[tile_linear()] = t2;
 }
 
 // alpha = rz / p.q (the finalize the reference folds into k_pre, :652-659)
-__global__ void __launch_bounds__(256) k_fin_pq(const double* __restrict__ part, int nparts,
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_pq(const double* __restrict__ part, int nparts,
                                                 DevState* __restrict__ st, int k) {
-    __shared__ double sh[4];
+    __shared__ double sh[FIN_THREADS / 64];
     if (cg_inactive(st, k)) return;
     const double pq = reduce_parts_sum(part, nparts, sh);
     if (threadIdx.x == 0) st->s[S_ALPHA] = pq > 0.0 ? st->s[S_RZ] / pq : 0.0;
 }
 
-// k_pre (:648-672) + k_resid (:674-685) + k_fwd_red (:426-441).  Thread t owns one 2x2x2 block
-// of the tile: r -= alpha q on its 8 cells (k > 0; written to r_out), rr = r - K z0 with z0 =
-// omega D^-1 r staged for the tile + halo, then the level-1 components and the zero-guess red
-// sweep.
+// k_pre (:648-672) + k_resid (:674-685) + k_fwd_red (:426-441).  r -= alpha q (UPDR: k > 0;
+// written to r_out) and z0 = omega D^-1 r are staged for the tile + halo; then thread t owns one
+// 2x2x2 block of the tile: rr = r - K z0 on its 8 cells, the level-1 components and the
+// zero-guess red sweep.
+template <bool UPDR>
 __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const float* __restrict__ r_in,
                                                                 float* __restrict__ r_out, const float* __restrict__ q,
                                                                 const uint8_t* __restrict__ fm, Lvl L, int do_red,
                                                                 const DevState* __restrict__ st, int k) {
-    __shared__ float sz[HN];
-    __shared__ uint8_t sm[HN];
+    __shared__ __align__(16) float sz[HN];
+    __shared__ __align__(16) float sr[HN];
+    __shared__ __align__(16) uint8_t sm[HN];
     if (cg_inactive(st, k)) return;
     const float a = (float)st->s[S_ALPHA];
-    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
-        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
-        if (h >= HN) break;
-        const int i = halo_global(N, h);
-        const int ic = i >= 0 ? i : 0;   // every load in bounds; outside cells staged as mask 0, value 0
-        const int m = i >= 0 ? (int)fm[ic] : 0;
-        const float r0 = r_in[ic], qv = q[ic];
-        const float rv = k > 0 ? r0 - a * qv : r0;
-        sz[h] = m ? OMEGA_JAC * rv / kdiag(m) : 0.0f;
-        sm[h] = (uint8_t)m;
-    }
-    __syncthreads();
-    const int t = threadIdx.x;
+    const int t = (int)threadIdx.x, n3 = N * N * N;
     const int bi = t % BTX, bj = (t / BTX) % BTY, bk = t / (BTX * BTY);
     const int mb = L.m;
     const int I = (int)blockIdx.x * BTX + bi, J = (int)blockIdx.y * BTY + bj, K = (int)blockIdx.z * BTZ + bk;
-    if (I >= mb || J >= mb || K >= mb) return;
+    const HBlock ob = halo_block(N, own_hb(t));
+    const size_t cell = (size_t)ob.lc;
+    const bool red = ob.in && do_red && ((I + J + K) & 1) == 0;
+    float4 dg0 = make_float4(0.f, 0.f, 0.f, 0.f), dg1 = dg0;
+    if (red) {
+        const float4* dp = reinterpret_cast<const float4*>(L.diag + cell * 8);
+        dg0 = dp[0];
+        dg1 = dp[1];
+    }
+    HBlock hb[HB_IT];
+    float2 hr[HB_IT][4], hq[HB_IT][4];
+    int hm[HB_IT][4];
+#pragma unroll
+    for (int it = 0; it < HB_IT; ++it) {
+        hb[it] = halo_block(N, min(t + it * TILE_THREADS, HBN - 1));
+#pragma unroll
+        for (int rw = 0; rw < 4; ++rw) {
+            const int o = hb[it].i0 + row_off(N, rw);
+            hr[it][rw] = ld2(r_in, o);
+            if (UPDR) hq[it][rw] = ld2(q, o);
+            hm[it][rw] = ldm2(fm, hb[it].in ? o : n3);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);   // every load above is issued before the first use
+#pragma unroll
+    for (int it = 0; it < HB_IT; ++it) {
+        if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
+            const float in = hb[it].in ? 1.0f : 0.0f;
+#pragma unroll
+            for (int rw = 0; rw < 4; ++rw) {
+                float rv[2] = {hr[it][rw].x, hr[it][rw].y};
+                if (UPDR) {
+                    rv[0] = rv[0] - a * hq[it][rw].x;
+                    rv[1] = rv[1] - a * hq[it][rw].y;
+                }
+                const int h = hb[it].h0 + row_h(rw);
+                float zv[2];
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int m = (hm[it][rw] >> (8 * dx)) & 255;
+                    rv[dx] = rv[dx] * in;
+                    zv[dx] = m ? OMEGA_JAC * rv[dx] / kdiag(m) : 0.0f;
+                }
+                st2(sz, h, zv[0], zv[1]);
+                st2(sr, h, rv[0], rv[1]);
+                *reinterpret_cast<uint16_t*>(sm + h) = (uint16_t)hm[it][rw];
+            }
+        }
+    }
+    __syncthreads();
+    if (!ob.in) return;
+    float kz[8];
+    if (wave_interior(sm, ob.h0)) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) kz[c] = kq<true>(sz, sm, ob.h0 + row_h(c >> 1) + (c & 1));
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) kz[c] = kq<false>(sz, sm, ob.h0 + row_h(c >> 1) + (c & 1));
+    }
     float rr[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int dz = c >> 2, dy = (c >> 1) & 1, dx = c & 1;
-        const int h = tile_h(2 * bi + dx, 2 * bj + dy, 2 * bk + dz);
-        const int i = ((2 * K + dz) * N + 2 * J + dy) * N + 2 * I + dx;
-        float rv = r_in[i];
-        if (k > 0) {
-            rv = rv - a * q[i];
-            r_out[i] = rv;
-        }
-        rr[c] = sm[h] ? rv - kmul_tile(sz, sm, h) : 0.0f;
+    for (int rw = 0; rw < 4; ++rw) {
+        const int h = ob.h0 + row_h(rw);
+        const float r0 = sr[h], r1 = sr[h + 1];
+        if (UPDR) st2(r_out, ob.i0 + row_off(N, rw), r0, r1);
+        rr[2 * rw] = sm[h] ? r0 - kz[2 * rw] : 0.0f;
+        rr[2 * rw + 1] = sm[h + 1] ? r1 - kz[2 * rw + 1] : 0.0f;
     }
-    const size_t cell = lcell(mb, K, J, I);
-    const bool red = do_red && ((I + J + K) & 1) == 0;
+    const float dg[8] = {dg0.x, dg0.y, dg0.z, dg0.w, dg1.x, dg1.y, dg1.z, dg1.w};
     float out[8], xo[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -200,10 +343,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
 #pragma unroll
         for (int s = 0; s < 8; ++s) acc += hsign(c, s >> 2, (s >> 1) & 1, s & 1) * rr[s];
         out[c] = acc;
-        if (red) {
-            const float d = L.diag[cell * 8 + c];
-            xo[c] = d <= L.dmin ? 0.0f : acc / d;
-        }
+        xo[c] = dg[c] <= L.dmin ? 0.0f : acc / dg[c];
     }
     float4* rp = reinterpret_cast<float4*>(L.r + cell * 8);
     rp[0] = make_float4(out[0], out[1], out[2], out[3]);
@@ -216,66 +356,92 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
 }
 
 // k_inv_transform (:352-365) + k_post_rz (:687-710): zt = z0 + T^T x1 (0 in solids) staged for
-// the tile + halo from the level-1 values of the tile's blocks and their neighbours, then
+// the tile + halo, one 2x2x2 block at a time from its 8 level-1 values, then
 // z = zt + omega D^-1 (r - K zt) on the tile and the block partials r.z, max|r|, r.r.  At k = 0
 // z is also written to p_init (the first search direction, k_upd_p init).
 __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const float* __restrict__ r,
                                                            const uint8_t* __restrict__ fm, float* __restrict__ z,
                                                            float* __restrict__ p_init, double* __restrict__ part,
                                                            int nparts, const DevState* __restrict__ st, int k) {
-    __shared__ float sx[HBN * 8];
-    __shared__ float szt[HN];
-    __shared__ uint8_t sm[HN];
-    __shared__ double sh[8];
+    __shared__ __align__(16) float szt[HN];
+    __shared__ __align__(16) float sr[HN];
+    __shared__ __align__(16) uint8_t sm[HN];
+    __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
-    const int mb = L.m;
-    const int bx0 = (int)blockIdx.x * BTX - 1, by0 = (int)blockIdx.y * BTY - 1, bz0 = (int)blockIdx.z * BTZ - 1;
-    _Pragma("unroll") for (int bi_ = 0; bi_ < HBN8_IT; ++bi_) {
-        const int it = (int)threadIdx.x + bi_ * TILE_THREADS;
-        if (it >= HBN * 8) break;
-        const int b = it >> 3, c = it & 7;
-        const int I = bx0 + b % HBX, J = by0 + (b / HBX) % HBY, K = bz0 + b / (HBX * HBY);
-        const bool in = I >= 0 && J >= 0 && K >= 0 && I < mb && J < mb && K < mb;
-        const float v = L.x[(in ? lcell(mb, K, J, I) : 0) * 8 + c];
-        sx[it] = in ? v : 0.0f;
-    }
-    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
-        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
-        if (h >= HN) break;
-        const int i = halo_global(N, h);
-        const int m = fm[i >= 0 ? i : 0];
-        sm[h] = (uint8_t)(i >= 0 ? m : 0);
-    }
-    __syncthreads();
-    _Pragma("unroll") for (int hi_ = 0; hi_ < HN_IT; ++hi_) {
-        const int h = (int)threadIdx.x + hi_ * TILE_THREADS;
-        if (h >= HN) break;
-        const int m = sm[h];
-        const int i = halo_global(N, h);
-        const float rv = r[i >= 0 ? i : 0];
-        const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-        const int hb = ((hz >> 1) * HBY + (hy >> 1)) * HBX + (hx >> 1);
-        float acc = m ? OMEGA_JAC * rv / kdiag(m) : 0.0f;
+    const int t = (int)threadIdx.x, n3 = N * N * N;
+    HBlock hb[HB_IT];
+    float4 xa[HB_IT], xb[HB_IT];
+    float2 hr[HB_IT][4];
+    int hm[HB_IT][4];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc += hsign(c, hz & 1, hy & 1, hx & 1) * sx[hb * 8 + c];
-        szt[h] = m ? acc : 0.0f;
+    for (int it = 0; it < HB_IT; ++it) {
+        hb[it] = halo_block(N, min(t + it * TILE_THREADS, HBN - 1));
+        const float4* xp = reinterpret_cast<const float4*>(L.x + (size_t)hb[it].lc * 8);
+        xa[it] = xp[0];
+        xb[it] = xp[1];
+#pragma unroll
+        for (int rw = 0; rw < 4; ++rw) {
+            const int o = hb[it].i0 + row_off(N, rw);
+            hr[it][rw] = ld2(r, o);
+            hm[it][rw] = ldm2(fm, hb[it].in ? o : n3);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);   // every load above is issued before the first use
+#pragma unroll
+    for (int it = 0; it < HB_IT; ++it) {
+        if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
+            const float in = hb[it].in ? 1.0f : 0.0f;
+            const float hx[8] = {HS * (xa[it].x * in), HS * (xa[it].y * in), HS * (xa[it].z * in),
+                                 HS * (xa[it].w * in), HS * (xb[it].x * in), HS * (xb[it].y * in),
+                                 HS * (xb[it].z * in), HS * (xb[it].w * in)};
+#pragma unroll
+            for (int rw = 0; rw < 4; ++rw) {
+                const float rv[2] = {hr[it][rw].x * in, hr[it][rw].y * in};
+                float zt[2];
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int m = (hm[it][rw] >> (8 * dx)) & 255;
+                    const float acc = inv_sum(m ? OMEGA_JAC * rv[dx] / kdiag(m) : 0.0f, hx, rw >> 1, rw & 1, dx);
+                    zt[dx] = m ? acc : 0.0f;
+                }
+                const int h = hb[it].h0 + row_h(rw);
+                st2(szt, h, zt[0], zt[1]);
+                st2(sr, h, rv[0], rv[1]);
+                *reinterpret_cast<uint16_t*>(sm + h) = (uint16_t)hm[it][rw];
+            }
+        }
     }
     __syncthreads();
+    // the thread's own block; a block outside the box is staged as mask 0, value 0 (terms 0)
+    const HBlock ob = halo_block(N, own_hb(t));
+    float kz[8];
+    const bool fast = wave_interior(sm, ob.h0);
+    if (fast) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) kz[c] = kq<true>(szt, sm, ob.h0 + row_h(c >> 1) + (c & 1));
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) kz[c] = kq<false>(szt, sm, ob.h0 + row_h(c >> 1) + (c & 1));
+    }
     double s = 0.0, mx = 0.0, rr = 0.0;
-    _Pragma("unroll") for (int ji_ = 0; ji_ < OWN_IT; ++ji_) {
-        const int j = (int)threadIdx.x + ji_ * TILE_THREADS;
-        int h;
-        const int i = own_global(N, j, h);
-        if (i < 0) continue;
-        const int m = sm[h];
-        const float rv = r[i];
-        float zv = 0.0f;
-        if (m) zv = szt[h] + OMEGA_JAC * (rv - kmul_tile(szt, sm, h)) / kdiag(m);
-        z[i] = zv;
-        if (k == 0) p_init[i] = zv;
-        s += (double)rv * (double)zv;
-        mx = fmax(mx, fabs((double)rv));
-        rr += (double)rv * (double)rv;
+#pragma unroll
+    for (int rw = 0; rw < 4; ++rw) {
+        const int h = ob.h0 + row_h(rw);
+        float zv[2];
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int m = sm[h + dx];
+            const float rv = sr[h + dx];
+            zv[dx] = m ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : 0.0f;
+            s += (double)rv * (double)zv[dx];
+            mx = fmax(mx, fabs((double)rv));
+            rr += (double)rv * (double)rv;
+        }
+        if (ob.in) {
+            const int i = ob.i0 + row_off(N, rw);
+            st2(z, i, zv[0], zv[1]);
+            if (k == 0) st2(p_init, i, zv[0], zv[1]);
+        }
     }
     const double a = block_sum(s, sh);
     const double b = block_max(mx, sh);
@@ -289,13 +455,20 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
 }
 
 // k_fin_rz (:737-771) with FP64 scalars, plus the optional relative-residual criterion.
-__global__ void __launch_bounds__(256) k_fin_rz(const double* __restrict__ part, int nparts,
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict__ part, int nparts,
                                                 const float* __restrict__ prm, DevState* __restrict__ st, int k) {
-    __shared__ double sh[4];
+    __shared__ double sh[FIN_THREADS / 64];
     if (cg_inactive(st, k)) return;
-    const double rz_new = reduce_parts_sum(part, nparts, sh);
-    const double mr = reduce_parts_max(part + nparts, nparts, sh);
-    const double rr = reduce_parts_sum(part + 2 * nparts, nparts, sh);
+    double va = 0.0, vb = 0.0, vc = 0.0;
+#pragma unroll 8
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        va += part[i];
+        vb = fmax(vb, part[nparts + i]);
+        vc += part[2 * nparts + i];
+    }
+    const double rz_new = block_sum(va, sh);
+    const double mr = block_max(vb, sh);
+    const double rr = block_sum(vc, sh);
     if (threadIdx.x != 0) return;
     double best = mr;
     if (k == 0) {
@@ -363,9 +536,9 @@ __global__ void __launch_bounds__(256) k_true_res(int N, int zc, const float* __
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = t;
 }
 
-__global__ void __launch_bounds__(256) k_fin_true(const double* __restrict__ part, int nparts,
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_true(const double* __restrict__ part, int nparts,
                                                   DevState* __restrict__ st) {
-    __shared__ double sh[4];
+    __shared__ double sh[FIN_THREADS / 64];
     const double s = reduce_parts_sum(part, nparts, sh);
     if (threadIdx.x == 0) st->s[S_TRUE_RR] = s;
 }
@@ -400,8 +573,12 @@ float* Pressure::r_buf(int k) const { return R[k & 1]; }
 void Pressure::precond(int k, hipStream_t s) {
     float* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
     float* r_out = R[k & 1];
-    hipLaunchKernelGGL(k_pre_resid_fwd, tgrid, dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
-                       S > 0 ? 1 : 0, st, k);
+    if (k > 0)
+        hipLaunchKernelGGL(k_pre_resid_fwd<true>, tgrid, dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
+                           S > 0 ? 1 : 0, st, k);
+    else
+        hipLaunchKernelGGL(k_pre_resid_fwd<false>, tgrid, dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
+                           S > 0 ? 1 : 0, st, k);
     vcycle_levels(k, s);
     hipLaunchKernelGGL(k_inv_post, tgrid, dim3(TILE_THREADS), 0, s, N, lv[0], r_out, fm, z, P[0], part, ntiles, st,
                        k);
@@ -410,21 +587,22 @@ void Pressure::precond(int k, hipStream_t s) {
 void Pressure::rhs(const float* vel, const float* prm, hipStream_t s) {
     const Geom& g = geom;
     hipLaunchKernelGGL(k_rhs, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, prm, part, nblocks(g));
-    hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(256), 0, s, part, nblocks(g), prm, st);
+    hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), prm, st);
 }
 
 void Pressure::iteration(int k, const float* prm, hipStream_t s) {
     if (k > 0) {
         // p_{k-1} = z_{k-1} + beta p_{k-2} (k >= 2); p_0 = z_0 was written at k = 0
-        const int upd = k >= 2 ? 1 : 0;
-        const float* p_in = upd ? P[(k - 2) & 1] : P[0];
-        float* p_out = P[(k - 1) & 1];
-        hipLaunchKernelGGL(k_upd_matvec, tgrid, dim3(TILE_THREADS), 0, s, N, z, p_in, p_out, phi, fm, q, part_pq,
-                           upd, st, k);
-        hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(256), 0, s, part_pq, ntiles, st, k);
+        if (k >= 2)
+            hipLaunchKernelGGL(k_upd_matvec<true>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[(k - 2) & 1],
+                               P[(k - 1) & 1], phi, fm, q, part_pq, st, k);
+        else
+            hipLaunchKernelGGL(k_upd_matvec<false>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[0], P[0], phi, fm, q,
+                               part_pq, st, k);
+        hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, ntiles, st, k);
     }
     precond(k, s);
-    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(256), 0, s, part, ntiles, prm, st, k);
+    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k);
 }
 
 void Pressure::finish(hipStream_t s) {
@@ -440,7 +618,7 @@ void Pressure::apply(float* vel, hipStream_t s) {
 void Pressure::true_residual(const float* vel, hipStream_t s) {
     const Geom& g = geom;
     hipLaunchKernelGGL(k_true_res, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi, part);
-    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(256), 0, s, part, nblocks(g), st);
+    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), st);
 }
 
 // One kernel of the projection on the current buffers, for timing (ids relative to
@@ -448,15 +626,15 @@ void Pressure::true_residual(const float* vel, hipStream_t s) {
 bool Pressure::launch_single(int which, hipStream_t s) {
     switch (which) {
         case 0:
-            hipLaunchKernelGGL(k_upd_matvec, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[0], P[1], phi, fm, q, part_pq,
-                               0, st, 1);
+            hipLaunchKernelGGL(k_upd_matvec<false>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[0], P[1], phi, fm, q,
+                               part_pq, st, 1);
             return true;
         case 1:
             hipLaunchKernelGGL(k_inv_post, tgrid, dim3(TILE_THREADS), 0, s, N, lv[0], R[0], fm, z, P[0], part, ntiles,
                                st, 1);
             return true;
         case 5:
-            hipLaunchKernelGGL(k_pre_resid_fwd, tgrid, dim3(TILE_THREADS), 0, s, N, R[0], R[0], q, fm, lv[0],
+            hipLaunchKernelGGL(k_pre_resid_fwd<false>, tgrid, dim3(TILE_THREADS), 0, s, N, R[0], R[0], q, fm, lv[0],
                                S > 0 ? 1 : 0, st, 0);
             return true;
         default: return launch_single_mg(which, s);

@@ -51,9 +51,11 @@ __device__ __forceinline__ int mask_at(const uint8_t* __restrict__ fm, int N, in
     return fm[idx3(N, z, y, x)];
 }
 
+// ALL: every cell fluid with every face open (the neighbourhood of a standard cell)
+template <bool ALL>
 __device__ __forceinline__ float gt(const uint8_t* __restrict__ fm, int N, int c, int BK, int BJ, int BI, int z,
                                     int y, int x, int ax) {
-    const int m = mask_at(fm, N, z, y, x);
+    const int m = ALL ? (FLUID | 63) : mask_at(fm, N, z, y, x);
     const int ex = ax == 0, ey = ax == 1, ez = ax == 2;
     const float t0 = tv(c, BK, BJ, BI, z, y, x);
     float g = 0.0f;
@@ -68,7 +70,38 @@ __device__ __forceinline__ void axcell(int ax, int oz, int oy, int ox, int a, in
     else { z = oz + a; y = oy + c; x = ox + b; }
 }
 
-// k_probe1 (pressure.py:220-246): exact component-diagonal of T K T^T, one (cell, c) per item.
+// k_probe1 (pressure.py:220-246): exact component-diagonal of T K T^T for (block, c).  ALL
+// evaluates it for a block in an all-fluid neighbourhood (the standard-cell table).
+template <bool ALL>
+__device__ void probe_item(const uint8_t* __restrict__ fm, int N, int m, int c, int K, int J, int I, float& dsum,
+                           float* off3) {
+    const int oz = 2 * K, oy = 2 * J, ox = 2 * I;
+    dsum = 0.0f;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int b = 0; b < 2; ++b)
+            for (int cc = 0; cc < 2; ++cc)
+                for (int a = 0; a < 4; ++a) {
+                    int z, y, x;
+                    axcell(ax, oz, oy, ox, a - 1, b, cc, z, y, x);
+                    const float v = gt<ALL>(fm, N, c, K, J, I, z, y, x, ax);
+                    dsum += v * v;
+                }
+    for (int ax = 0; ax < 3; ++ax) {
+        const int nK = K + (ax == 2), nJ = J + (ax == 1), nI = I + (ax == 0);
+        float s = 0.0f;
+        if (ALL || (nK < m && nJ < m && nI < m)) {
+            for (int b = 0; b < 2; ++b)
+                for (int cc = 0; cc < 2; ++cc)
+                    for (int a = 0; a < 2; ++a) {
+                        int z, y, x;
+                        axcell(ax, oz, oy, ox, 1 + a, b, cc, z, y, x);
+                        s += gt<ALL>(fm, N, c, K, J, I, z, y, x, ax) * gt<ALL>(fm, N, c, nK, nJ, nI, z, y, x, ax);
+                    }
+        }
+        off3[ax] = s;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_probe(int N, const uint8_t* __restrict__ fm, Lvl L,
                                                const DevState* __restrict__ st) {
     if (!st->changed) return;
@@ -78,32 +111,10 @@ __global__ void __launch_bounds__(256) k_probe(int N, const uint8_t* __restrict_
         const int c = (int)(t & 7);
         const size_t cell = t >> 3;
         const int I = (int)(cell % m), J = (int)((cell / m) % m), K = (int)(cell / ((size_t)m * m));
-        const int oz = 2 * K, oy = 2 * J, ox = 2 * I;
-        float dsum = 0.0f;
-        for (int ax = 0; ax < 3; ++ax)
-            for (int b = 0; b < 2; ++b)
-                for (int cc = 0; cc < 2; ++cc)
-                    for (int a = 0; a < 4; ++a) {
-                        int z, y, x;
-                        axcell(ax, oz, oy, ox, a - 1, b, cc, z, y, x);
-                        const float v = gt(fm, N, c, K, J, I, z, y, x, ax);
-                        dsum += v * v;
-                    }
-        L.diag[cell * 8 + c] = dsum;
-        for (int ax = 0; ax < 3; ++ax) {
-            const int nK = K + (ax == 2), nJ = J + (ax == 1), nI = I + (ax == 0);
-            float s = 0.0f;
-            if (nK < m && nJ < m && nI < m) {
-                for (int b = 0; b < 2; ++b)
-                    for (int cc = 0; cc < 2; ++cc)
-                        for (int a = 0; a < 2; ++a) {
-                            int z, y, x;
-                            axcell(ax, oz, oy, ox, 1 + a, b, cc, z, y, x);
-                            s += gt(fm, N, c, K, J, I, z, y, x, ax) * gt(fm, N, c, nK, nJ, nI, z, y, x, ax);
-                        }
-            }
-            L.off[cell * 24 + ax * 8 + c] = s;
-        }
+        float d, o3[3];
+        probe_item<false>(fm, N, m, c, K, J, I, d, o3);
+        L.diag[cell * 8 + c] = d;
+        for (int ax = 0; ax < 3; ++ax) L.off[cell * 24 + ax * 8 + c] = o3[ax];
     }
 }
 
@@ -140,16 +151,109 @@ __global__ void k_clear_changed(DevState* st) {
     if (threadIdx.x == 0) st->changed = 0;
 }
 
+// Standard-cell tables of every level (thread c: component c): level 1 from the probe of a block
+// in an all-fluid neighbourhood, each coarser level from k_galerkin's sums over standard fine
+// values, in k_galerkin's order.
+__global__ void __launch_bounds__(64) k_std_tab(CoarseLevels all, int N, const DevState* __restrict__ st) {
+    if (!st->changed) return;
+    const int c = (int)threadIdx.x;
+    if (c >= 8) return;
+    float d, o3[3];
+    probe_item<true>(nullptr, N, all.lv[0].m, c, 1, 1, 1, d, o3);
+    all.lv[0].tab[c] = d;
+    for (int ax = 0; ax < 3; ++ax) all.lv[0].tab[8 + ax * 8 + c] = o3[ax];
+    for (int l = 0; l + 1 < all.n; ++l) {
+        const float* ft = all.lv[l].tab;
+        float* ct = all.lv[l + 1].tab;
+        float dd = 0.0f;
+        for (int k = 0; k < 8; ++k) dd += ft[c];
+        for (int ax = 0; ax < 3; ++ax) {
+            float oi = 0.0f, oo = 0.0f;
+            for (int b = 0; b < 4; ++b) {
+                oi += ft[8 + ax * 8 + c];
+                oo += ft[8 + ax * 8 + c];
+            }
+            dd += 2.0f * oi;
+            ct[8 + ax * 8 + c] = oo * GALERKIN_SCALE;
+        }
+        ct[c] = dd * GALERKIN_SCALE;
+    }
+}
+
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+
+// sf = 1 where the cell has all 6 neighbours, its diag and links and its -ax neighbours' links
+// equal tab bit for bit, and every neighbour has diag > dmin
+__global__ void __launch_bounds__(256) k_std_flags(Lvl L, int use_std, const DevState* __restrict__ st) {
+    if (!st->changed) return;
+    const int m = L.m;
+    const size_t m3 = (size_t)m * m * m;
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < m3; o += (size_t)gridDim.x * blockDim.x) {
+        const int I = (int)(o % m), J = (int)((o / m) % m), K = (int)(o / ((size_t)m * m));
+        bool s = use_std && I > 0 && J > 0 && K > 0 && I + 1 < m && J + 1 < m && K + 1 < m;
+        if (s) {
+            const size_t sy = (size_t)m, sz = (size_t)m * m;
+            const size_t nb[6] = {o + 1, o - 1, o + sy, o - sy, o + sz, o - sz};
+            for (int c = 0; c < 8; ++c) {
+                s = s && same_bits(L.diag[o * 8 + c], L.tab[c]);
+                for (int ax = 0; ax < 3; ++ax) {
+                    s = s && same_bits(L.off[o * 24 + ax * 8 + c], L.tab[8 + ax * 8 + c]);
+                    s = s && same_bits(L.off[nb[2 * ax + 1] * 24 + ax * 8 + c], L.tab[8 + ax * 8 + c]);
+                }
+                for (int k = 0; k < 6; ++k) s = s && L.diag[nb[k] * 8 + c] > L.dmin;
+            }
+        }
+        L.sf[o] = s ? 1 : 0;
+    }
+}
+
 // ---------------------------------------------------------------- multigrid items
 // residual restriction (k_resrestrict :306-316) + optional zero-guess red sweep at the coarse
-// level (k_restrict_red :444-459)
+// level (k_restrict_red :444-459).  A child's neighbour inside the 2x2x2 block is another child
+// (the same address, loaded once); one outside is clamped to the child itself where missing.
 __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I, bool do_red) {
-    float res[8];
+    const int m = F.m;
+    size_t co[8], nb[8][6];
+    bool e[8][6];
+    float r[8], x0[8], xn[8][6];
+    int f = 1;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const int z = 2 * K + (k >> 2), y = 2 * J + ((k >> 1) & 1), x = 2 * I + (k & 1);
-        const size_t o = lcell(F.m, z, y, x) * 8 + c;
-        res[k] = F.r[o] - F.diag[o] * F.x[o] - offsum(F, c, z, y, x);
+        const int dz = k >> 2, dy = (k >> 1) & 1, dx = k & 1;
+        const int z = 2 * K + dz, y = 2 * J + dy, x = 2 * I + dx;
+        e[k][0] = dx == 0 || x + 1 < m;
+        e[k][1] = dx == 1 || x > 0;
+        e[k][2] = dy == 0 || y + 1 < m;
+        e[k][3] = dy == 1 || y > 0;
+        e[k][4] = dz == 0 || z + 1 < m;
+        e[k][5] = dz == 1 || z > 0;
+        co[k] = lcell(m, z, y, x);
+        nb[k][0] = e[k][0] ? lcell(m, z, y, x + 1) : co[k];
+        nb[k][1] = e[k][1] ? lcell(m, z, y, x - 1) : co[k];
+        nb[k][2] = e[k][2] ? lcell(m, z, y + 1, x) : co[k];
+        nb[k][3] = e[k][3] ? lcell(m, z, y - 1, x) : co[k];
+        nb[k][4] = e[k][4] ? lcell(m, z + 1, y, x) : co[k];
+        nb[k][5] = e[k][5] ? lcell(m, z - 1, y, x) : co[k];
+        f &= (int)F.sf[co[k]];
+        r[k] = F.r[co[k] * 8 + c];
+        x0[k] = F.x[co[k] * 8 + c];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) xn[k][j] = F.x[nb[k][j] * 8 + c];
+    }
+    const Tab tb = load_tab(F, c);
+    float res[8];
+    if (__all(f != 0)) {
+        const float cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) res[k] = r[k] - tb.d * x0[k] - offsum_v(cf, xn[k], e[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float cf[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) cf[j] = F.off[((j & 1) ? nb[k][j] : co[k]) * 24 + (j >> 1) * 8 + c];
+            res[k] = r[k] - F.diag[co[k] * 8 + c] * x0[k] - offsum_v(cf, xn[k], e[k]);
+        }
     }
     float acc = 0.0f;
 #pragma unroll
@@ -171,35 +275,44 @@ __device__ __forceinline__ void prolong_item(const Lvl& F, const Lvl& C, int c, 
     F.x[o] = x + xc;
 }
 
-// x value of a level cell after the constant prolongation (prolong skips cells with
-// diag <= dmin), without storing it
-__device__ __forceinline__ float xp(const Lvl& F, const Lvl& C, int c, size_t o, int K, int J, int I) {
-    const float x = F.x[o * 8 + c], dg = F.diag[o * 8 + c];
-    const float xc = C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
-    return dg <= F.dmin ? x : x + xc;
-}
-
 // prolongation fused into the black post-sweep: equal to prolonging every cell, then sweeping
 // black (a GS update ignores the cell's own old value; red cells are rewritten by the red sweep
 // that follows).  Same rounding as the unfused pair (pressure.py:396-423 vs :902-931).
 __device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I) {
     const int m = F.m;
     const size_t o = lcell(m, K, J, I);
-    const float d = F.diag[o * 8 + c];
     const LNb n = level_neighbours(m, o, K, J, I);
     const int nI[6] = {I + 1, I - 1, I, I, I, I}, nJ[6] = {J, J, J + 1, J - 1, J, J},
               nK[6] = {K, K, K, K, K + 1, K - 1};
-    float t[6];
+    const int f = F.sf[o];
+    const float r = F.r[o * 8 + c];
+    float xn[6], xcn[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         // coordinates of the clamped neighbour (the cell itself where it is missing)
         const int kI = n.e[k] ? nI[k] : I, kJ = n.e[k] ? nJ[k] : J, kK = n.e[k] ? nK[k] : K;
-        t[k] = link_coef(F, n, o, k, c) * xp(F, C, c, n.c[k], kK, kJ, kI);
+        xn[k] = F.x[n.c[k] * 8 + c];
+        xcn[k] = C.x[lcell(C.m, kK >> 1, kJ >> 1, kI >> 1) * 8 + c];
     }
-    float a = 0.0f;
+    const Tab tb = load_tab(F, c);
+    float d, cf[6], xpv[6];
+    if (__all(f != 0)) {
+        d = tb.d;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) a = a + (n.e[k] ? t[k] : 0.0f);
-    const float r = F.r[o * 8 + c];
+        for (int k = 0; k < 6; ++k) {
+            cf[k] = tb.o[k >> 1];
+            xpv[k] = xn[k] + xcn[k];   // every neighbour of a standard cell has diag > dmin
+        }
+    } else {
+        d = F.diag[o * 8 + c];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            cf[k] = link_coef(F, n, o, k, c);
+            // x of the neighbour after the constant prolongation (which skips diag <= dmin)
+            xpv[k] = F.diag[n.c[k] * 8 + c] <= F.dmin ? xn[k] : xn[k] + xcn[k];
+        }
+    }
+    const float a = offsum_v(cf, xpv, n.e);
     F.x[o * 8 + c] = d <= F.dmin ? 0.0f : (r - a) / d;
 }
 
@@ -329,11 +442,14 @@ void Pressure::init_levels(int N_) {
         if (lv[l].m <= coarse_m) { S = l; break; }
 }
 
+// floats that hold m3 flag bytes, a multiple of 8 (every level array stays 32-byte aligned)
+static size_t flag_floats(size_t m3) { return ((m3 + 3) / 4 + 7) / 8 * 8; }
+
 size_t Pressure::level_floats() const {
     size_t t = 0;
     for (int l = 0; l < nlev; ++l) {
         const size_t m3 = (size_t)lv[l].m * lv[l].m * lv[l].m;
-        t += m3 * (8 + 8 + 8 + 24);
+        t += m3 * (8 + 8 + 8 + 24) + 32 + flag_floats(m3);
     }
     return t;
 }
@@ -345,11 +461,21 @@ void Pressure::bind_levels(float* base) {
         lv[l].r = base; base += 8 * m3;
         lv[l].diag = base; base += 8 * m3;
         lv[l].off = base; base += 24 * m3;
+        lv[l].tab = base; base += 32;
+        lv[l].sf = reinterpret_cast<uint8_t*>(base); base += flag_floats(m3);
     }
+}
+
+__global__ void k_set_changed(DevState* st) {
+    if (threadIdx.x == 0) st->changed = 1;
 }
 
 void Pressure::setup(const float* solid, hipStream_t s) {
     const size_t n3 = (size_t)N * N * N;
+    if (force_setup) {
+        hipLaunchKernelGGL(k_set_changed, dim3(1), dim3(64), 0, s, st);
+        force_setup = false;
+    }
     hipLaunchKernelGGL(k_solid_check, dim3(grid_for(n3, 4096)), dim3(256), 0, s, n3, solid, fm, st);
     hipLaunchKernelGGL(k_mask, dim3(grid_for(n3, 4096)), dim3(256), 0, s, N, solid, fm, st);
     const size_t m3 = (size_t)lv[0].m * lv[0].m * lv[0].m;
@@ -357,6 +483,14 @@ void Pressure::setup(const float* solid, hipStream_t s) {
     for (int l = 0; l + 1 < nlev; ++l) {
         const size_t mc3 = (size_t)lv[l + 1].m * lv[l + 1].m * lv[l + 1].m;
         hipLaunchKernelGGL(k_galerkin, dim3(grid_for(mc3 * 8, 4096)), dim3(256), 0, s, lv[l], lv[l + 1], st);
+    }
+    CoarseLevels all{};
+    all.n = nlev;
+    for (int l = 0; l < nlev; ++l) all.lv[l] = lv[l];
+    hipLaunchKernelGGL(k_std_tab, dim3(1), dim3(64), 0, s, all, N, st);
+    for (int l = 0; l < nlev; ++l) {
+        const size_t ml3 = (size_t)lv[l].m * lv[l].m * lv[l].m;
+        hipLaunchKernelGGL(k_std_flags, dim3(grid_for(ml3, 4096)), dim3(256), 0, s, lv[l], use_std ? 1 : 0, st);
     }
     hipLaunchKernelGGL(k_clear_changed, dim3(1), dim3(64), 0, s, st);
 }

@@ -66,16 +66,9 @@ __device__ __forceinline__ float kmul_xyz(const float* __restrict__ p, const uin
     return -d;
 }
 
-// kdiag (pressure.py:83-91)
-__device__ __forceinline__ float kdiag(int m) {
-    float s = 0.0f;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        const float oR = (float)((m >> (2 * ax)) & 1), oL = (float)((m >> (2 * ax + 1)) & 1);
-        s += (oL - oR) * (oL - oR) + oL + oR;
-    }
-    return s;
-}
+// kdiag (pressure.py:83-91): sum over axes of (oL - oR)^2 + oL + oR, which is 2 for an axis with
+// any open face and 0 otherwise (small integers, exact in float)
+__device__ __forceinline__ float kdiag(int m) { return (float)(2 * __popc((m | (m >> 1)) & 0x15)); }
 
 // div2 (pressure.py:94-106) of the velocity channels vel[0..2] at cell (x, y, z), mask m
 __device__ __forceinline__ float div2_xyz(const float* __restrict__ vel, int n3, int m, int N, int x, int y, int z,
@@ -158,6 +151,23 @@ __device__ __forceinline__ float kmul_tile(const float* P, const uint8_t* M, int
     return -d;
 }
 
+// kmul_tile at an INTERIOR cell: every open-face multiplier is 1 (x * 1 == x), so the same
+// operations without the mask
+__device__ __forceinline__ float kmul_tile_interior(const float* P, int h) {
+    float d = 0.0f;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int st = ax == 0 ? 1 : (ax == 1 ? HX : HX * HY);
+        const float m2 = P[h - 2 * st], m1 = P[h - st], c = P[h], p1 = P[h + st], p2 = P[h + 2 * st];
+        const float gi = (0.0f + (p1 - c)) + (c - m1);
+        const float gp = (0.0f + (p2 - p1)) + (p1 - c);
+        const float gm = (0.0f + (c - m1)) + (m1 - m2);
+        d = d + (gi + gp);
+        d = d - (gi + gm);
+    }
+    return -d;
+}
+
 // halo index of fine cell (gx, gy, gz) relative to the tile origin (ox, oy, oz) (halo excluded)
 __device__ __forceinline__ int tile_h(int lx, int ly, int lz) { return ((lz + 2) * HY + (ly + 2)) * HX + (lx + 2); }
 
@@ -187,26 +197,58 @@ __device__ __forceinline__ float link_coef(const Lvl& L, const LNb& n, size_t o,
     return L.off[((k & 1) ? n.c[k] : o) * 24 + ax * 8 + c];
 }
 
-// offsum (pressure.py:271-287) in the component-fastest layout, every load unconditional
-__device__ __forceinline__ float offsum(const Lvl& L, int c, int K, int J, int I) {
-    const size_t o = lcell(L.m, K, J, I);
-    const LNb n = level_neighbours(L.m, o, K, J, I);
-    float t[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) t[k] = link_coef(L, n, o, k, c) * L.x[n.c[k] * 8 + c];
+// Standard cells.  Away from walls and solids every cell of a level has the same diag and link
+// coefficients (tab: the same arithmetic on the same values).  A cell whose flag sf is set has
+// all 6 neighbours, its own diag and links and its -ax neighbours' links equal to tab bit for bit,
+// and neighbours with diag > dmin.  When every lane of a wave is at a standard cell, the
+// coefficients come from tab (a 128-byte table) instead of the level arrays; the arithmetic is
+// unchanged, so the result is the same as the general path's.
+struct Tab {
+    float d, o[3];   // diag and the x, y, z link coefficients of component c
+};
+
+__device__ __forceinline__ Tab load_tab(const Lvl& L, int c) {
+    Tab t;
+    t.d = L.tab[c];
+    t.o[0] = L.tab[8 + c];
+    t.o[1] = L.tab[16 + c];
+    t.o[2] = L.tab[24 + c];
+    return t;
+}
+
+// offsum (pressure.py:271-287) from the link coefficients cf and neighbour values xn
+__device__ __forceinline__ float offsum_v(const float* cf, const float* xn, const bool* e) {
     float a = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) a = a + (n.e[k] ? t[k] : 0.0f);
+    for (int k = 0; k < 6; ++k) a = a + (e[k] ? cf[k] * xn[k] : 0.0f);
     return a;
 }
 
 // one GS update (k_smooth pressure.py:290-303); returns the new value
 __device__ __forceinline__ float smooth_value(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
-    const size_t o = lcell(L.m, K, J, I) * 8 + c;
-    const float d = L.diag[o];
-    const float r = L.r[o];
-    const float a = zero_init ? 0.0f : offsum(L, c, K, J, I);
-    const float acc = zero_init ? r : r - a;
+    const size_t o = lcell(L.m, K, J, I);
+    const float r = L.r[o * 8 + c];
+    if (zero_init) {
+        const float d = L.diag[o * 8 + c];
+        return d <= L.dmin ? 0.0f : r / d;
+    }
+    const LNb n = level_neighbours(L.m, o, K, J, I);
+    const int f = L.sf[o];
+    float xn[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xn[k] = L.x[n.c[k] * 8 + c];
+    const Tab tb = load_tab(L, c);
+    float d, cf[6];
+    if (__all(f != 0)) {
+        d = tb.d;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cf[k] = tb.o[k >> 1];
+    } else {
+        d = L.diag[o * 8 + c];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cf[k] = link_coef(L, n, o, k, c);
+    }
+    const float acc = r - offsum_v(cf, xn, n.e);
     return d <= L.dmin ? 0.0f : acc / d;
 }
 

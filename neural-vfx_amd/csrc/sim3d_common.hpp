@@ -110,14 +110,20 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return r;
 }
 
-// Reduce `n` block partials in a fixed order inside one 256-thread block; result in thread 0.
+// Final reductions of block partials run in one workgroup of FIN_THREADS threads; the strided
+// loop is unrolled so that a thread's loads are in flight together.  Result in thread 0, fixed
+// order for a fixed n.
+constexpr int FIN_THREADS = 1024;
+
 __device__ __forceinline__ double reduce_parts_sum(const double* p, int n, double* sh) {
     double v = 0.0;
+#pragma unroll 8
     for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
     return block_sum(v, sh);
 }
 __device__ __forceinline__ double reduce_parts_max(const double* p, int n, double* sh) {
     double v = 0.0;
+#pragma unroll 8
     for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, p[i]);
     return block_max(v, sh);
 }

@@ -15,6 +15,8 @@ struct Lvl {
     float* r;     // [m^3][8]
     float* diag;  // [m^3][8]
     float* off;   // [m^3][24]: ax*8 + c couples to the +ax neighbour
+    float* tab;   // [32]: diag[8], off[24] of a cell in an all-fluid neighbourhood
+    uint8_t* sf;  // [m^3]: 1 where a cell's whole stencil equals tab (see k_std_flags)
 };
 
 constexpr int MAX_LEVELS = 16;
@@ -31,7 +33,8 @@ void launch_adv_pass(int pass, const Geom& g, int C, const float* src, const flo
                      float inv_dx, float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s);
 void launch_sources_pass(int pass, const Geom& g, const float* src, const uint8_t* fm, const float* prm,
                          const DevState* st, float* dst, hipStream_t s);
-void launch_mass_fin(const Geom& g, const double* part, DevState* st, hipStream_t s);
+void launch_mass_fin(const Geom& g, double* part, DevState* st, hipStream_t s);
+size_t mass_part_doubles(const Geom& g);   // partial-sum slots of the advection mass sums
 void launch_sources_diffuse(const Geom& g, const float* adv, const uint8_t* fm, const float* prm,
                             const DevState* st, float* hat, float* data, hipStream_t s);
 void launch_vort_forces(const Geom& g, float* data, const uint8_t* fm, const float* prm, float inv2dx,
@@ -48,9 +51,11 @@ struct Pressure {
     int nlev = 0;
     int S = 0;            // first level handled by the single-workgroup coarse kernel
     int coarse_m = 8;     // levels with m <= coarse_m run inside that kernel
+    bool use_std = true;  // standard-cell flags on (off: every cell reads the level arrays)
+    bool force_setup = false;   // rebuild the operators at the next setup
     Lvl lv[MAX_LEVELS];
     Geom geom;
-    uint8_t* fm = nullptr;
+    uint8_t* fm = nullptr;       // face mask (n3 bytes) followed by at least one zero byte
     float* R[2] = {nullptr, nullptr};   // residual, ping-pong by iteration parity
     float* P[2] = {nullptr, nullptr};   // search direction, ping-pong
     float *q = nullptr, *z = nullptr, *phi = nullptr;

@@ -169,6 +169,27 @@ def test_precond_parity(cuda, hiplib, N, coarse_m):
     assert e <= 1e-6
 
 
+@pytest.mark.parametrize("N", [64, 128])
+def test_precond_standard_stencil_identical(cuda, hiplib, N):
+    """Waves of standard cells take their multigrid coefficients from the per-level table; the
+    preconditioner is bit for bit the one that reads every coefficient from the level arrays."""
+    _say(f"precond standard-stencil on/off N={N}")
+    solid = sphere_solid(N)
+    fm = oracle.mask(solid)
+    r = smooth_noise(N, 1, 7)[0] * (fm > 0)
+    sd, rd = to_dev(solid), to_dev(r)
+    outs = []
+    for on in (1, 0):
+        s = Solver3D(N)
+        s.set_option(_lib.OPT_STD_STENCIL, on)
+        z = torch.empty(N, N, N, device=DEV)
+        _lib.check(hiplib.vfx_precond(s._ctx, _ptr(sd), _ptr(rd), _ptr(z), _stream()))
+        torch.cuda.synchronize()
+        outs.append(z.cpu())
+    assert torch.equal(outs[0], outs[1])
+    assert float(outs[0].abs().max()) > 0.0
+
+
 def _divergent_state(N, solid, seed=1):
     data = random_state(N, seed, solid)
     v = smooth_noise(N, 3, seed, amp=2.0) + np.array([0.8, -0.3, 0.4], np.float32)[:, None, None, None]
