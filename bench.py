@@ -25,19 +25,42 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (compulsory) bytes per grid cell of each kernel, per launch: every distinct input
 # element read once, every output written once, at its storage type (FP32 fields, uint8 mask,
-# FP32 multigrid levels).  Multigrid kernels are per fine-grid cell (level 1 has 8 components on
-# (N/2)^3 blocks = N^3 unknowns).
+# FP32 multigrid levels; the level-1 multigrid has N^3 unknowns, 8 per level-1 cell, so its
+# figures are per fine cell too).  Standard multigrid cells take their coefficients from a
+# 32-float table, so the level kernels read a 1-byte flag per level-1 cell (1/8 byte per unknown)
+# instead of diag/off.  The CG kernels are the steady-state forms (iteration k >= 2).
 KERNEL_BYTES_PER_CELL = {
-    "adv_sl": 24 + 24,            # data (6ch, the velocity is 3 of them) -> hat (6ch)
-    "adv_mc": 24 + 24 + 1 + 24,   # data + hat + mask -> adv
-    "sources": 24 + 1 + 24,       # adv + mask -> hat
-    "diffuse": 24 + 1 + 24,       # hat + mask -> data
-    "matvec": 4 + 1 + 4,          # p + mask -> q
-    "post_rz": 4 + 4 + 1 + 4,     # zt + r + mask -> z
-    "pre": 4 + 4 + 1 + 4 + 4,     # r + q + mask -> r, z0 (k > 0)
-    "resid": 4 + 4 + 1 + 4,       # r + z0 + mask -> rr
-    "upd_p": 4 + 4 + 4 + 4 + 4,   # phi + p + z -> phi, p
+    "adv_sl": 24 + 24,                  # data (6 ch, velocity 3 of them) -> hat (6 ch)
+    "adv_mc": 24 + 24 + 1 + 24,         # data + hat + mask -> adv
+    "sources": 24 + 1 + 24,             # adv + mask -> hat
+    "diffuse": 24 + 1 + 24,             # hat + mask -> data
+    "upd_matvec": 4 * 3 + 1 + 4 * 3,    # z, p_old, phi + mask -> p, phi, q
+    "inv_post": 4 + 4 + 1 + 4,          # level-1 x, r + mask -> z
+    "pre_resid_fwd": 4 + 4 + 1 + 4 + 4 + 2 + 2,   # r, q + mask -> r, level-1 r; red half: diag -> x
+    "smooth0": 2 + 4 + 0.125 + 2,       # one colour: r (half), x (all) + flags -> x (half)
+    "restrict0": 4 + 4 + 0.125 + 0.5 + 0.25 + 0.25,   # level-1 r, x + flags -> level-2 r; red: diag -> x
+    "prolong0": 2 + 4 + 0.5 + 0.125 + 2,   # black: r, x (all), level-2 x + flags -> x (half)
 }
+
+# launches per step of each timed kernel for n CG iterations (the per-step share picks the
+# dominant kernel)
+def launches_per_step(name: str, n: int) -> float:
+    return {"adv_sl": 1, "adv_mc": 1, "sources": 1, "diffuse": 1, "upd_matvec": max(n - 1, 0), "inv_post": n,
+            "pre_resid_fwd": n, "smooth0": 2 * n, "restrict0": n, "prolong0": n, "coarse": n}.get(name, 0)
+
+
+def step_bytes_per_cell(n: int) -> float:
+    """Algorithmic bytes per cell of one whole step with n CG iterations: advection, sources +
+    diffusion, the vorticity/force chain, the projection's fixed part (rhs, phi final, apply) and
+    n iterations of the fused CG kernels plus the level-1 multigrid (coarser levels: 1/7 more)."""
+    adv = KERNEL_BYTES_PER_CELL["adv_sl"] + KERNEL_BYTES_PER_CELL["adv_mc"]
+    frc = KERNEL_BYTES_PER_CELL["sources"] + KERNEL_BYTES_PER_CELL["diffuse"] + 37
+    fixed = 21 + 8 + 29
+    mg = (2 * KERNEL_BYTES_PER_CELL["smooth0"] + KERNEL_BYTES_PER_CELL["restrict0"]
+          + KERNEL_BYTES_PER_CELL["prolong0"]) * 8 / 7
+    it = (KERNEL_BYTES_PER_CELL["upd_matvec"] + KERNEL_BYTES_PER_CELL["inv_post"]
+          + KERNEL_BYTES_PER_CELL["pre_resid_fwd"] + mg)
+    return adv + frc + fixed + n * it
 
 
 def fire_config(seed: int):
@@ -86,8 +109,7 @@ def main():
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profiled", type=int, default=5, help="extra step_profiled steps for the stage split")
-    ap.add_argument("--roofline-kernel", default="adv_mc")
-    ap.add_argument("--time-kernels", default="", help="comma list of extra kernels to time (us)")
+    ap.add_argument("--roofline-kernel", default="", help="kernel for the roofline (default: largest step share)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -129,21 +151,30 @@ def main():
         true_rel.append(ps["true_rel_res"])
     med = lambda k: round(statistics.median(p[k] for p in prof), 3) if prof else None
 
-    # live roofline of the dominant kernel on the solver's stream
-    kname = args.roofline_kernel
-    kid = _lib.KERNELS.index(kname)
-    kms = ctypes.c_float()
-    _lib.check(s._lib.vfx_time_kernel(s._ctx, kid, 20, s._stream(), ctypes.byref(kms)))
+    # live device time of every kernel on the solver's stream (HIP events, 20 launches each); the
+    # roofline is reported for the kernel with the largest share of the step
+    n_it = statistics.median(iters) if iters else 8
+    kern = {}
+    for name in [k for k in _lib.KERNELS if k]:
+        t = ctypes.c_float()
+        if s._lib.vfx_time_kernel(s._ctx, _lib.KERNELS.index(name), 20, s._stream(), ctypes.byref(t)) != 0:
+            continue
+        row = {"us": round(t.value * 1000.0, 1), "per_step_us": round(t.value * 1000.0 * launches_per_step(name, n_it), 1)}
+        if name in KERNEL_BYTES_PER_CELL:
+            gbs = KERNEL_BYTES_PER_CELL[name] * N ** 3 / (t.value * 1e-3) / 1e9
+            row.update(gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 3))
+        kern[name] = row
+    kname = args.roofline_kernel or max((k for k in kern if k in KERNEL_BYTES_PER_CELL),
+                                        key=lambda k: kern[k]["per_step_us"])
     kbytes = KERNEL_BYTES_PER_CELL[kname] * N ** 3
-    achieved = kbytes / (kms.value * 1e-3) / 1e9
+    kms = kern[kname]["us"] / 1000.0
+    achieved = kbytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
-                "kernel_ms": round(kms.value, 4), "bytes_per_launch": kbytes}
-    others = {}
-    for name in [n for n in args.time_kernels.split(",") if n]:
-        t = ctypes.c_float()
-        if s._lib.vfx_time_kernel(s._ctx, _lib.KERNELS.index(name), 20, s._stream(), ctypes.byref(t)) == 0:
-            others[name] = round(t.value * 1000.0, 1)
+                "kernel_ms": round(kms, 4), "bytes_per_launch": int(kbytes)}
+    sb = step_bytes_per_cell(n_it) * N ** 3
+    step_roof = {"bytes_per_step": int(sb), "achieved": round(sb / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+                 "frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     cpu = None
     if not args.no_cpu:
@@ -160,8 +191,7 @@ def main():
         "pressure_ms": med("projection"), "advection_ms": med("advection"), "forces_ms": med("forces"),
         "profiled_total_ms": med("total"), "pcg_iters": iters, "rel_res": [float(f"{r:.3g}") for r in rel],
         "true_rel_res": [float(f"{r:.3g}") for r in true_rel],
-        "roofline": roofline, "cpu_baseline": cpu,
-        **({"kernel_us": others} if others else {}),
+        "roofline": roofline, "step_roofline": step_roof, "cpu_baseline": cpu, "kernels": kern,
     }
     print(json.dumps(out))
 

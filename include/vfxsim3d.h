@@ -96,20 +96,18 @@ int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float*
                 void* stream);
 
 /* kernel ids of vfx_time_kernel */
-#define VFX_K_ADV_SL 0     /* semi-Lagrangian pass of the advection */
-#define VFX_K_ADV_MC 1     /* MacCormack pass + clamp + mass partials */
-#define VFX_K_SOURCES 2
-#define VFX_K_DIFFUSE 3
-#define VFX_K_MATVEC 4     /* q = K p and p.q */
-#define VFX_K_POST_RZ 5    /* post-Jacobi of the preconditioner + r.z, max|r|, r.r */
-#define VFX_K_SMOOTH0 6    /* one colour of GS on the finest multigrid level */
-#define VFX_K_RESTRICT0 7  /* residual restriction finest -> next level */
-#define VFX_K_PROLONG0 8   /* prolongation + black sweep on the finest level */
-#define VFX_K_PRE 9        /* pre-Jacobi of the preconditioner */
-#define VFX_K_RESID_FWD 10 /* Jacobi residual r - K z0 + Hadamard transform to level 1 + red sweep */
-#define VFX_K_RED_INV 12   /* last red sweep on level 1 + inverse transform */
-#define VFX_K_UPD_P 13     /* p / phi update */
-#define VFX_K_COARSE 14    /* single-workgroup coarse V-cycle */
+#define VFX_K_ADV_SL 0     /* semi-Lagrangian pass of the advection (advect_sl, kernels.py:124-140) */
+#define VFX_K_ADV_MC 1     /* MacCormack pass + clamp + mass partials (advect_mc, kernels.py:143-178) */
+#define VFX_K_SOURCES 2    /* sources (kernels.py:182-250) */
+#define VFX_K_DIFFUSE 3    /* heat + fuel diffusion (kernels.py:268-286) */
+#define VFX_K_MATVEC 4     /* p / phi update + q = K p + p.q (k_upd_p + k_matvec_dot, pressure.py:629-646,712-728) */
+#define VFX_K_POST_RZ 5    /* inverse transform + post-Jacobi + r.z, max|r|, r.r (pressure.py:352-365,687-710) */
+#define VFX_K_SMOOTH0 6    /* one colour of GS on the finest multigrid level (k_smooth_h, pressure.py:290-303) */
+#define VFX_K_RESTRICT0 7  /* residual restriction finest -> next level (k_restrict_red, pressure.py:306-316,444-459) */
+#define VFX_K_PROLONG0 8   /* prolongation + black sweep on the finest level (k_prolong_black, pressure.py:396-423) */
+#define VFX_K_PRE 9        /* r -= alpha q, pre-Jacobi, residual, Hadamard transform, red sweep
+                              (k_pre + k_resid + k_fwd_red, pressure.py:648-685,426-441) */
+#define VFX_K_COARSE 14    /* single-workgroup coarse V-cycle (k_coarse_block, pressure.py:550-591) */
 
 /* Average device time (ms) of `reps` back-to-back launches of one kernel on the solver's current
  * buffers, bracketed by HIP events on `stream` (measurement support for the benchmark; call it

@@ -238,7 +238,7 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     if (!c) return arg_error("out of host memory");
     const int N = res;
     c->N = N;
-    c->g = make_geom(N, 0);
+    c->g = make_geom(N, 1);   // one cell per thread: every load of a thread is in flight at once
     const double dx = 1.0 / (double)N;   // DOMAIN_SIZE / res (fields.py:25)
     const double inv = 1.0 / dx;
     c->inv_dx = (float)inv;
@@ -517,7 +517,7 @@ int vfx_time_kernel(vfx_ctx* c, int which, int reps, void* stream, float* ms_avg
                 return true;
             case VFX_K_SOURCES: launch_sources_pass(0, c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, s); return true;
             case VFX_K_DIFFUSE: launch_sources_pass(1, c->g, c->hat, c->P.fm, c->prm, c->st, c->omega, s); return true;
-            default: return which >= VFX_K_MATVEC && c->P.launch_single(which - VFX_K_MATVEC, s);
+            default: return which >= VFX_K_MATVEC && c->P.launch_single(which - VFX_K_MATVEC, c->hat, s);
         }
     };
     if (!one()) {

@@ -622,20 +622,22 @@ void Pressure::true_residual(const float* vel, hipStream_t s) {
 }
 
 // One kernel of the projection on the current buffers, for timing (ids relative to
-// VFX_K_MATVEC).  Only kernels that leave phi and the iteration state untouched are offered.
-bool Pressure::launch_single(int which, hipStream_t s) {
+// VFX_K_MATVEC), in its steady-state form (iteration k >= 2).  What a kernel would write into
+// phi, p, q or r goes to `scratch` (3 N^3 floats) instead, so the solver state is untouched.
+bool Pressure::launch_single(int which, float* scratch, hipStream_t s) {
+    const size_t n3 = (size_t)N * N * N;
     switch (which) {
         case 0:
-            hipLaunchKernelGGL(k_upd_matvec<false>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[0], P[1], phi, fm, q,
-                               part_pq, st, 1);
+            hipLaunchKernelGGL(k_upd_matvec<true>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[0], scratch,
+                               scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2);
             return true;
         case 1:
             hipLaunchKernelGGL(k_inv_post, tgrid, dim3(TILE_THREADS), 0, s, N, lv[0], R[0], fm, z, P[0], part, ntiles,
-                               st, 1);
+                               st, 2);
             return true;
         case 5:
-            hipLaunchKernelGGL(k_pre_resid_fwd<false>, tgrid, dim3(TILE_THREADS), 0, s, N, R[0], R[0], q, fm, lv[0],
-                               S > 0 ? 1 : 0, st, 0);
+            hipLaunchKernelGGL(k_pre_resid_fwd<true>, tgrid, dim3(TILE_THREADS), 0, s, N, R[0], scratch, q, fm, lv[0],
+                               S > 0 ? 1 : 0, st, 2);
             return true;
         default: return launch_single_mg(which, s);
     }

@@ -211,6 +211,7 @@ __global__ void __launch_bounds__(256) k_std_flags(Lvl L, int use_std, const Dev
 // residual restriction (k_resrestrict :306-316) + optional zero-guess red sweep at the coarse
 // level (k_restrict_red :444-459).  A child's neighbour inside the 2x2x2 block is another child
 // (the same address, loaded once); one outside is clamped to the child itself where missing.
+template <bool STD>
 __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I, bool do_red) {
     const int m = F.m;
     size_t co[8], nb[8][6];
@@ -234,15 +235,15 @@ __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c,
         nb[k][3] = e[k][3] ? lcell(m, z, y - 1, x) : co[k];
         nb[k][4] = e[k][4] ? lcell(m, z + 1, y, x) : co[k];
         nb[k][5] = e[k][5] ? lcell(m, z - 1, y, x) : co[k];
-        f &= (int)F.sf[co[k]];
+        if (STD) f &= (int)F.sf[co[k]];
         r[k] = F.r[co[k] * 8 + c];
         x0[k] = F.x[co[k] * 8 + c];
 #pragma unroll
         for (int j = 0; j < 6; ++j) xn[k][j] = F.x[nb[k][j] * 8 + c];
     }
-    const Tab tb = load_tab(F, c);
+    const Tab tb = STD ? load_tab(F, c) : Tab{};
     float res[8];
-    if (__all(f != 0)) {
+    if (STD && __all(f != 0)) {
         const float cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
 #pragma unroll
         for (int k = 0; k < 8; ++k) res[k] = r[k] - tb.d * x0[k] - offsum_v(cf, xn[k], e[k]);
@@ -334,7 +335,7 @@ __global__ void __launch_bounds__(256) k_smooth_half(Lvl L, int color, const Dev
     if (cg_inactive(st, k)) return;
     int c, K, J, I;
     if (!half_item(blockIdx.x * (size_t)blockDim.x + threadIdx.x, L.m, color, c, K, J, I)) return;
-    smooth_item(L, c, K, J, I, false);
+    smooth_item<true>(L, c, K, J, I, false);
 }
 
 __global__ void __launch_bounds__(256) k_restrict(Lvl F, Lvl C, int do_red, const DevState* __restrict__ st, int k) {
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__(256) k_restrict(Lvl F, Lvl C, int do_red, cons
     if (t >= (size_t)m * m * m * 8) return;
     const int c = (int)(t & 7);
     const size_t cell = t >> 3;
-    restrict_item(F, C, c, (int)(cell / ((size_t)m * m)), (int)((cell / m) % m), (int)(cell % m), do_red != 0);
+    restrict_item<true>(F, C, c, (int)(cell / ((size_t)m * m)), (int)((cell / m) % m), (int)(cell % m), do_red != 0);
 }
 
 __global__ void __launch_bounds__(256) k_prolong_black(Lvl F, Lvl C, const DevState* __restrict__ st, int k) {
@@ -366,7 +367,7 @@ __device__ void cb_smooth(const Lvl& L, int color, bool zero_init) {
         const int c = (int)(t & 7);
         const size_t cell = t >> 3;
         const int I = (int)(cell % m), J = (int)((cell / m) % m), K = (int)(cell / ((size_t)m * m));
-        if (((I + J + K) & 1) == color) smooth_item(L, c, K, J, I, zero_init);
+        if (((I + J + K) & 1) == color) smooth_item<false>(L, c, K, J, I, zero_init);
     }
     __syncthreads();
 }
@@ -383,8 +384,8 @@ __global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevels cl, const De
         for (size_t t = threadIdx.x; t < items; t += CB_THREADS) {
             const int c = (int)(t & 7);
             const size_t cell = t >> 3;
-            restrict_item(F, C, c, (int)(cell / ((size_t)C.m * C.m)), (int)((cell / C.m) % C.m), (int)(cell % C.m),
-                          false);
+            restrict_item<false>(F, C, c, (int)(cell / ((size_t)C.m * C.m)), (int)((cell / C.m) % C.m),
+                                 (int)(cell % C.m), false);
         }
         __syncthreads();
     }

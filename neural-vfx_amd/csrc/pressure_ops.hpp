@@ -224,7 +224,9 @@ __device__ __forceinline__ float offsum_v(const float* cf, const float* xn, cons
     return a;
 }
 
-// one GS update (k_smooth pressure.py:290-303); returns the new value
+// one GS update (k_smooth pressure.py:290-303); returns the new value.  STD: waves of standard
+// cells use the table (the single-workgroup coarse levels have few standard cells and skip it).
+template <bool STD>
 __device__ __forceinline__ float smooth_value(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
     const size_t o = lcell(L.m, K, J, I);
     const float r = L.r[o * 8 + c];
@@ -233,13 +235,13 @@ __device__ __forceinline__ float smooth_value(const Lvl& L, int c, int K, int J,
         return d <= L.dmin ? 0.0f : r / d;
     }
     const LNb n = level_neighbours(L.m, o, K, J, I);
-    const int f = L.sf[o];
+    const int f = STD ? L.sf[o] : 0;
     float xn[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) xn[k] = L.x[n.c[k] * 8 + c];
-    const Tab tb = load_tab(L, c);
+    const Tab tb = STD ? load_tab(L, c) : Tab{};
     float d, cf[6];
-    if (__all(f != 0)) {
+    if (STD && __all(f != 0)) {
         d = tb.d;
 #pragma unroll
         for (int k = 0; k < 6; ++k) cf[k] = tb.o[k >> 1];
@@ -252,8 +254,9 @@ __device__ __forceinline__ float smooth_value(const Lvl& L, int c, int K, int J,
     return d <= L.dmin ? 0.0f : acc / d;
 }
 
+template <bool STD>
 __device__ __forceinline__ float smooth_item(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
-    const float v = smooth_value(L, c, K, J, I, zero_init);
+    const float v = smooth_value<STD>(L, c, K, J, I, zero_init);
     L.x[lcell(L.m, K, J, I) * 8 + c] = v;
     return v;
 }

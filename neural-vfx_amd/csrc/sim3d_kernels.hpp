@@ -79,7 +79,7 @@ struct Pressure {
     void apply(float* vel, hipStream_t s);
     void true_residual(const float* vel, hipStream_t s);
     void vcycle_levels(int k, hipStream_t s);
-    bool launch_single(int which, hipStream_t s);
+    bool launch_single(int which, float* scratch, hipStream_t s);
     bool launch_single_mg(int which, hipStream_t s);
 };
 

@@ -16,8 +16,8 @@ VFX_OK, VFX_E_ARG, VFX_E_VALUE, VFX_E_HIP = 0, 1, 2, 3
 STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
 OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL = 1, 2, 3, 4
 # kernel ids of vfx_time_kernel (include/vfxsim3d.h VFX_K_*); "" = unused id
-KERNELS = ("adv_sl", "adv_mc", "sources", "diffuse", "matvec", "post_rz", "smooth0", "restrict0", "prolong0", "pre",
-           "resid_fwd", "", "red_inv", "upd_p", "coarse")
+KERNELS = ("adv_sl", "adv_mc", "sources", "diffuse", "upd_matvec", "inv_post", "smooth0", "restrict0", "prolong0",
+           "pre_resid_fwd", "", "", "", "", "coarse")
 
 # every entry point of include/vfxsim3d.h with its argument types
 _P = ctypes.c_void_p
