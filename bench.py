@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "neural-vfx_amd"))
 
 HBM_PEAK_GBS = 8000.0
+COPY_PROBE_BYTES = 256 << 20   # device copy of known size that scales the PMC byte counters
 
 # Algorithmic (compulsory) bytes per grid cell of each kernel, per launch: every distinct input
 # element read once, every output written once, at its storage type (FP32 fields, uint8 mask,
@@ -111,6 +112,9 @@ def main():
     ap.add_argument("--profiled", type=int, default=5, help="extra step_profiled steps for the stage split")
     ap.add_argument("--roofline-kernel", default="", help="kernel for the roofline (default: largest step share)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="short run for counter collection: one 256 MiB device copy (the scale reference), "
+                         "2 steps, then one launch of every timed kernel; prints no bench line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the 256^3 step is one "
@@ -129,6 +133,22 @@ def main():
     s = Solver3D(N, cfg, device="cuda:0")
     s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
     st = s.reset()
+    if args.pmc_probe:
+        src = torch.ones(COPY_PROBE_BYTES // 4, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        for i in range(2):
+            s.step(st, forces, dt)
+        torch.cuda.synchronize()
+        for name in [k for k in _lib.KERNELS if k]:
+            t = ctypes.c_float()
+            s._lib.vfx_time_kernel(s._ctx, _lib.KERNELS.index(name), 1, s._stream(), ctypes.byref(t))
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "res": N, "copy_bytes": COPY_PROBE_BYTES,
+                          "kernel_bytes_per_launch": {k: int(v * N ** 3) for k, v in KERNEL_BYTES_PER_CELL.items()}}))
+        return
     print(f"# {N}^3 fire: {args.warmup} warm-up steps", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         s.step(st, forces, dt)

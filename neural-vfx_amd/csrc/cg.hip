@@ -29,7 +29,23 @@ namespace vfx {
 
 constexpr int HB_IT = (HBN + TILE_THREADS - 1) / TILE_THREADS;   // halo blocks staged per thread
 
-__device__ __forceinline__ int tile_linear() { return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)); }
+// The tile of this workgroup.  Tiles are launched as a 1D grid and renumbered with xcd_block, so
+// each XCD sweeps a contiguous run of tiles (x fastest, then y, then z) and neighbouring tiles
+// share their halo lines in one L2.
+struct Tile {
+    int x, y, z;   // tile coordinates
+    int lin;       // logical tile index (partial-sum slot)
+};
+
+__device__ __forceinline__ Tile this_tile(int N) {
+    const int tx = (N + TX - 1) / TX, ty = (N + TY - 1) / TY;
+    Tile t;
+    t.lin = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    t.x = t.lin % tx;
+    t.y = (t.lin / tx) % ty;
+    t.z = t.lin / (tx * ty);
+    return t;
+}
 
 // Tiles are staged in 2x2x2 blocks (the level-1 cells): halo block b (0..HBN-1) is the level-1
 // block (I, J, K) = first block of the tile - 1 + (b % HBX, b / HBX % HBY, b / (HBX HBY)), read
@@ -45,11 +61,10 @@ struct HBlock {
     bool in;    // block inside the box
 };
 
-__device__ __forceinline__ HBlock halo_block(int N, int b) {
+__device__ __forceinline__ HBlock halo_block(int N, const Tile& tl, int b) {
     const int bi = b % HBX, bj = (b / HBX) % HBY, bk = b / (HBX * HBY);
     const int mb = N >> 1;
-    const int I = (int)blockIdx.x * BTX - 1 + bi, J = (int)blockIdx.y * BTY - 1 + bj,
-              K = (int)blockIdx.z * BTZ - 1 + bk;
+    const int I = tl.x * BTX - 1 + bi, J = tl.y * BTY - 1 + bj, K = tl.z * BTZ - 1 + bk;
     const int CI = min(max(I, 0), mb - 1), CJ = min(max(J, 0), mb - 1), CK = min(max(K, 0), mb - 1);
     HBlock r;
     r.in = I >= 0 && J >= 0 && K >= 0 && I < mb && J < mb && K < mb;
@@ -166,9 +181,10 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
     __shared__ __align__(16) uint8_t sm[HN];
     __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
+    const Tile tl = this_tile(N);
     const float a = (float)st->s[S_ALPHA], b = (float)st->s[S_BETA];
     const int t = (int)threadIdx.x, n3 = N * N * N;
-    const HBlock ob = halo_block(N, own_hb(t));
+    const HBlock ob = halo_block(N, tl, own_hb(t));
     float2 po[4], ph[4];
     if (UPD) {
 #pragma unroll
@@ -182,7 +198,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
-        hb[it] = halo_block(N, min(t + it * TILE_THREADS, HBN - 1));
+        hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
 #pragma unroll
         for (int rw = 0; rw < 4; ++rw) {
             const int o = hb[it].i0 + row_off(N, rw);
@@ -237,7 +253,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
         }
     }
     const double t2 = block_sum(s, sh);
-    if (threadIdx.x == 0) part[tile_linear()] = t2;
+    if (threadIdx.x == 0) part[tl.lin] = t2;
 }
 
 // alpha = rz / p.q (the finalize the reference folds into k_pre, :652-659)
@@ -262,12 +278,13 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
     __shared__ __align__(16) float sr[HN];
     __shared__ __align__(16) uint8_t sm[HN];
     if (cg_inactive(st, k)) return;
+    const Tile tl = this_tile(N);
     const float a = (float)st->s[S_ALPHA];
     const int t = (int)threadIdx.x, n3 = N * N * N;
     const int bi = t % BTX, bj = (t / BTX) % BTY, bk = t / (BTX * BTY);
     const int mb = L.m;
-    const int I = (int)blockIdx.x * BTX + bi, J = (int)blockIdx.y * BTY + bj, K = (int)blockIdx.z * BTZ + bk;
-    const HBlock ob = halo_block(N, own_hb(t));
+    const int I = tl.x * BTX + bi, J = tl.y * BTY + bj, K = tl.z * BTZ + bk;
+    const HBlock ob = halo_block(N, tl, own_hb(t));
     const size_t cell = (size_t)ob.lc;
     const bool red = ob.in && do_red && ((I + J + K) & 1) == 0;
     float4 dg0 = make_float4(0.f, 0.f, 0.f, 0.f), dg1 = dg0;
@@ -281,7 +298,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
-        hb[it] = halo_block(N, min(t + it * TILE_THREADS, HBN - 1));
+        hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
 #pragma unroll
         for (int rw = 0; rw < 4; ++rw) {
             const int o = hb[it].i0 + row_off(N, rw);
@@ -368,6 +385,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
     __shared__ __align__(16) uint8_t sm[HN];
     __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
+    const Tile tl = this_tile(N);
     const int t = (int)threadIdx.x, n3 = N * N * N;
     HBlock hb[HB_IT];
     float4 xa[HB_IT], xb[HB_IT];
@@ -375,7 +393,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
-        hb[it] = halo_block(N, min(t + it * TILE_THREADS, HBN - 1));
+        hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
         const float4* xp = reinterpret_cast<const float4*>(L.x + (size_t)hb[it].lc * 8);
         xa[it] = xp[0];
         xb[it] = xp[1];
@@ -413,7 +431,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
     }
     __syncthreads();
     // the thread's own block; a block outside the box is staged as mask 0, value 0 (terms 0)
-    const HBlock ob = halo_block(N, own_hb(t));
+    const HBlock ob = halo_block(N, tl, own_hb(t));
     float kz[8];
     const bool fast = wave_interior(sm, ob.h0);
     if (fast) {
@@ -447,7 +465,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
     const double b = block_max(mx, sh);
     const double c = block_sum(rr, sh);
     if (threadIdx.x == 0) {
-        const int bl = tile_linear();
+        const int bl = tl.lin;
         part[bl] = a;
         part[nparts + bl] = b;
         part[2 * nparts + bl] = c;
@@ -574,13 +592,13 @@ void Pressure::precond(int k, hipStream_t s) {
     float* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
     float* r_out = R[k & 1];
     if (k > 0)
-        hipLaunchKernelGGL(k_pre_resid_fwd<true>, tgrid, dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
+        hipLaunchKernelGGL(k_pre_resid_fwd<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
                            S > 0 ? 1 : 0, st, k);
     else
-        hipLaunchKernelGGL(k_pre_resid_fwd<false>, tgrid, dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
+        hipLaunchKernelGGL(k_pre_resid_fwd<false>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
                            S > 0 ? 1 : 0, st, k);
     vcycle_levels(k, s);
-    hipLaunchKernelGGL(k_inv_post, tgrid, dim3(TILE_THREADS), 0, s, N, lv[0], r_out, fm, z, P[0], part, ntiles, st,
+    hipLaunchKernelGGL(k_inv_post, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lv[0], r_out, fm, z, P[0], part, ntiles, st,
                        k);
 }
 
@@ -594,10 +612,10 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s) {
     if (k > 0) {
         // p_{k-1} = z_{k-1} + beta p_{k-2} (k >= 2); p_0 = z_0 was written at k = 0
         if (k >= 2)
-            hipLaunchKernelGGL(k_upd_matvec<true>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[(k - 2) & 1],
+            hipLaunchKernelGGL(k_upd_matvec<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, z, P[(k - 2) & 1],
                                P[(k - 1) & 1], phi, fm, q, part_pq, st, k);
         else
-            hipLaunchKernelGGL(k_upd_matvec<false>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[0], P[0], phi, fm, q,
+            hipLaunchKernelGGL(k_upd_matvec<false>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, z, P[0], P[0], phi, fm, q,
                                part_pq, st, k);
         hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, ntiles, st, k);
     }
@@ -628,15 +646,15 @@ bool Pressure::launch_single(int which, float* scratch, hipStream_t s) {
     const size_t n3 = (size_t)N * N * N;
     switch (which) {
         case 0:
-            hipLaunchKernelGGL(k_upd_matvec<true>, tgrid, dim3(TILE_THREADS), 0, s, N, z, P[0], scratch,
+            hipLaunchKernelGGL(k_upd_matvec<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, z, P[0], scratch,
                                scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2);
             return true;
         case 1:
-            hipLaunchKernelGGL(k_inv_post, tgrid, dim3(TILE_THREADS), 0, s, N, lv[0], R[0], fm, z, P[0], part, ntiles,
+            hipLaunchKernelGGL(k_inv_post, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lv[0], R[0], fm, z, P[0], part, ntiles,
                                st, 2);
             return true;
         case 5:
-            hipLaunchKernelGGL(k_pre_resid_fwd<true>, tgrid, dim3(TILE_THREADS), 0, s, N, R[0], scratch, q, fm, lv[0],
+            hipLaunchKernelGGL(k_pre_resid_fwd<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, R[0], scratch, q, fm, lv[0],
                                S > 0 ? 1 : 0, st, 2);
             return true;
         default: return launch_single_mg(which, s);

@@ -317,6 +317,11 @@ __device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, i
     F.x[o * 8 + c] = d <= F.dmin ? 0.0f : (r - a) / d;
 }
 
+// item of this thread in a 1D launch, blocks renumbered per XCD (neighbouring rows in one L2)
+__device__ __forceinline__ size_t xcd_item() {
+    return (size_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+}
+
 // half-size colour launch: item t -> (c, K, J, I) with (I + J + K) & 1 == color (m even)
 __device__ __forceinline__ bool half_item(size_t t, int m, int color, int& c, int& K, int& J, int& I) {
     const int mh = m >> 1;
@@ -334,14 +339,14 @@ __device__ __forceinline__ bool half_item(size_t t, int m, int color, int& c, in
 __global__ void __launch_bounds__(256) k_smooth_half(Lvl L, int color, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     int c, K, J, I;
-    if (!half_item(blockIdx.x * (size_t)blockDim.x + threadIdx.x, L.m, color, c, K, J, I)) return;
+    if (!half_item(xcd_item(), L.m, color, c, K, J, I)) return;
     smooth_item<true>(L, c, K, J, I, false);
 }
 
 __global__ void __launch_bounds__(256) k_restrict(Lvl F, Lvl C, int do_red, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     const int m = C.m;
-    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t t = xcd_item();
     if (t >= (size_t)m * m * m * 8) return;
     const int c = (int)(t & 7);
     const size_t cell = t >> 3;
@@ -351,7 +356,7 @@ __global__ void __launch_bounds__(256) k_restrict(Lvl F, Lvl C, int do_red, cons
 __global__ void __launch_bounds__(256) k_prolong_black(Lvl F, Lvl C, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     int c, K, J, I;
-    if (!half_item(blockIdx.x * (size_t)blockDim.x + threadIdx.x, F.m, 1, c, K, J, I)) return;
+    if (!half_item(xcd_item(), F.m, 1, c, K, J, I)) return;
     prolong_black_item(F, C, c, K, J, I);
 }
 

@@ -66,6 +66,16 @@ __device__ __forceinline__ size_t idx3(int N, int z, int y, int x) {
     return ((size_t)z * N + y) * N + x;
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b runs on XCD b % 8), and each
+// XCD has its own L2.  xcd_block renumbers them so that every XCD works through one contiguous
+// range of logical blocks: neighbouring blocks, which share halo and neighbour cache lines, then
+// run on the same XCD at about the same time.  A bijection of [0, nb) for any nb.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb / NUM_XCD, r = nb % NUM_XCD, x = b % NUM_XCD, k = b / NUM_XCD;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 __device__ __forceinline__ int block_linear() {
     return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
