@@ -64,6 +64,20 @@ def step_bytes_per_cell(n: int) -> float:
     return adv + frc + fixed + n * it
 
 
+def pmc_traffic(kernel: str, res: int):
+    """HBM bytes per launch of `kernel` from the committed PMC summary of the same grid size
+    (profiles/pmc_traffic_<res>_fire.md: FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size),
+    or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{res}_fire.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 9 and cols[0] == kernel:
+            return int(cols[6])
+    return None
+
+
 def fire_config(seed: int):
     """Fire scene settings: buoyancy beta 1.5 per K of excess temperature, confinement 0.3,
     combustion at 2.0 /s above 0.3 K excess, pressure to ||r||/||b|| <= 1e-5."""
@@ -190,7 +204,7 @@ def main():
     kms = kern[kname]["us"] / 1000.0
     achieved = kbytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(kname, N), "kernel": kname,
                 "kernel_ms": round(kms, 4), "bytes_per_launch": int(kbytes)}
     sb = step_bytes_per_cell(n_it) * N ** 3
     step_roof = {"bytes_per_step": int(sb), "achieved": round(sb / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
