@@ -117,6 +117,36 @@ int vfx_time_kernel(vfx_ctx* ctx, int which, int reps, void* stream, float* ms_a
 /* Message of the last failed call on this thread. */
 int vfx_last_error(char* buf, int n);
 
+/* ---------------------------------------------------------------- FP64 run of the same solver
+ * The same entry points for FP64 state: fields, multigrid levels and every constant in double
+ * (the parameter buffer stays the float layout above).  This is the "FP64 run of the same
+ * solver" the FP32 path is measured against; it has no counterpart in the reference, which is
+ * FP32 only.  Each function behaves as its FP32 namesake above; contexts of the two precisions
+ * are distinct types. */
+typedef struct vfx_ctx_f64 vfx_ctx_f64;
+int vfx_p_size_f64(void);
+int vfx_last_error_f64(char* buf, int n);
+int vfx_ctx_create_f64(int res, vfx_ctx_f64** out);
+int vfx_ctx_destroy_f64(vfx_ctx_f64* ctx);
+int vfx_step_f64(vfx_ctx_f64* ctx, double* data, const double* solid, const float* prm_host, void* stream);
+int vfx_step_stage_f64(vfx_ctx_f64* ctx, int stage, double* data, const double* solid, const float* prm_host,
+                       void* stream);
+int vfx_project_f64(vfx_ctx_f64* ctx, double* data, const double* solid, const float* prm_host, void* stream);
+int vfx_advect_f64(vfx_ctx_f64* ctx, const double* field, int C, const double* vel, double* out, float dt,
+                   void* stream);
+int vfx_div_abs_f64(vfx_ctx_f64* ctx, const double* data, const double* solid, double* out, void* stream);
+int vfx_pressure_iters_f64(vfx_ctx_f64* ctx, int* out);
+int vfx_pressure_stats_f64(vfx_ctx_f64* ctx, double* out, int n);
+int vfx_set_option_f64(vfx_ctx_f64* ctx, int key, double value);
+int vfx_get_phi_f64(vfx_ctx_f64* ctx, double* dst, void* stream);
+int vfx_set_phi_f64(vfx_ctx_f64* ctx, const double* src, void* stream);
+int vfx_get_advected_f64(vfx_ctx_f64* ctx, double* dst, double* mass_ratio, void* stream);
+int vfx_kmul_f64(vfx_ctx_f64* ctx, const double* solid, const double* p, double* out, void* stream);
+int vfx_precond_f64(vfx_ctx_f64* ctx, const double* solid, const double* r, double* z, void* stream);
+int vfx_get_ops_f64(vfx_ctx_f64* ctx, const double* solid, int level, double* diag, double* off, int* nlev, int* m,
+                    void* stream);
+int vfx_time_kernel_f64(vfx_ctx_f64* ctx, int which, int reps, void* stream, float* ms_avg);
+
 #ifdef __cplusplus
 }
 #endif

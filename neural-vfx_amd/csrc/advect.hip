@@ -11,39 +11,39 @@
 
 namespace vfx {
 
-__device__ __forceinline__ float clampf_(float g, float lo, float hi) {
+__device__ __forceinline__ real clampf_(real g, real lo, real hi) {
     return g < lo ? lo : (g > hi ? hi : g);
 }
 
 // base_idx / frac (kernels.py:67-74): clamp to [0, n-1], floor, keep one cell of room.  The
 // clamp uses fmaxf/fminf so that a NaN coordinate also lands on cell 0 (equal to the reference's
 // clamp for every other input).
-__device__ __forceinline__ int base_idx(float g, int n) {
-    int i = (int)floorf(fminf(fmaxf(g, 0.0f), (float)(n - 1)));
+__device__ __forceinline__ int base_idx(real g, int n) {
+    int i = (int)floor(fmin(fmax(g, RL(0.0)), (real)(n - 1)));
     return i < n - 2 ? i : n - 2;
 }
 
 struct Corner {
     int o;      // linear index of the (z0, y0, x0) corner
-    float fx, fy, fz;
+    real fx, fy, fz;
 };
 
-__device__ __forceinline__ Corner corner(int N, float px, float py, float pz) {
+__device__ __forceinline__ Corner corner(int N, real px, real py, real pz) {
     const int x0 = base_idx(px, N), y0 = base_idx(py, N), z0 = base_idx(pz, N);
     Corner c;
     c.o = (z0 * N + y0) * N + x0;
-    c.fx = clampf_(px, 0.0f, (float)(N - 1)) - (float)x0;
-    c.fy = clampf_(py, 0.0f, (float)(N - 1)) - (float)y0;
-    c.fz = clampf_(pz, 0.0f, (float)(N - 1)) - (float)z0;
+    c.fx = clampf_(px, RL(0.0), (real)(N - 1)) - (real)x0;
+    c.fy = clampf_(py, RL(0.0), (real)(N - 1)) - (real)y0;
+    c.fz = clampf_(pz, RL(0.0), (real)(N - 1)) - (real)z0;
     return c;
 }
 
 // the 8 corner values of a trilinear sample, in (z, y, x) bit order 000 .. 111
 struct Oct {
-    float v[8];
+    real v[8];
 };
 
-__device__ __forceinline__ Oct gather(const float* __restrict__ f, int N, int o) {
+__device__ __forceinline__ Oct gather(const real* __restrict__ f, int N, int o) {
     const int sy = N, sz = N * N;
     Oct g;
     g.v[0] = f[o];
@@ -58,18 +58,18 @@ __device__ __forceinline__ Oct gather(const float* __restrict__ f, int N, int o)
 }
 
 // tri (kernels.py:77-85): x first, then y, then z, in the reference's operation order.
-__device__ __forceinline__ float lerp3(const Oct& g, const Corner& c) {
-    const float a00 = g.v[0] * (1.0f - c.fx) + g.v[1] * c.fx;
-    const float a01 = g.v[2] * (1.0f - c.fx) + g.v[3] * c.fx;
-    const float a10 = g.v[4] * (1.0f - c.fx) + g.v[5] * c.fx;
-    const float a11 = g.v[6] * (1.0f - c.fx) + g.v[7] * c.fx;
-    const float a0 = a00 * (1.0f - c.fy) + a01 * c.fy;
-    const float a1 = a10 * (1.0f - c.fy) + a11 * c.fy;
-    return a0 * (1.0f - c.fz) + a1 * c.fz;
+__device__ __forceinline__ real lerp3(const Oct& g, const Corner& c) {
+    const real a00 = g.v[0] * (RL(1.0) - c.fx) + g.v[1] * c.fx;
+    const real a01 = g.v[2] * (RL(1.0) - c.fx) + g.v[3] * c.fx;
+    const real a10 = g.v[4] * (RL(1.0) - c.fx) + g.v[5] * c.fx;
+    const real a11 = g.v[6] * (RL(1.0) - c.fx) + g.v[7] * c.fx;
+    const real a0 = a00 * (RL(1.0) - c.fy) + a01 * c.fy;
+    const real a1 = a10 * (RL(1.0) - c.fy) + a11 * c.fy;
+    return a0 * (RL(1.0) - c.fz) + a1 * c.fz;
 }
 
 // velocity at p (sample_vel, kernels.py:100-112)
-__device__ __forceinline__ void sample_vel(const float* __restrict__ vel, int N, int n3, const Corner& c, float* v) {
+__device__ __forceinline__ void sample_vel(const real* __restrict__ vel, int N, int n3, const Corner& c, real* v) {
     const Oct gx = gather(vel, N, c.o), gy = gather(vel + n3, N, c.o), gz = gather(vel + 2 * n3, N, c.o);
     v[0] = lerp3(gx, c);
     v[1] = lerp3(gy, c);
@@ -78,29 +78,29 @@ __device__ __forceinline__ void sample_vel(const float* __restrict__ vel, int N,
 
 // trace (kernels.py:115-120): RK2 midpoint over signed s (cells per m/s).  Two traces (back and
 // forward) run side by side so their gathers overlap.
-__device__ __forceinline__ void trace2(const float* __restrict__ vel, int N, int n3, float px, float py, float pz,
-                                       float sa, float sb, float* qa, float* qb) {
+__device__ __forceinline__ void trace2(const real* __restrict__ vel, int N, int n3, real px, real py, real pz,
+                                       real sa, real sb, real* qa, real* qb) {
     const Corner c0 = corner(N, px, py, pz);
-    float v1[3];
+    real v1[3];
     sample_vel(vel, N, n3, c0, v1);
-    const float ha = 0.5f * sa, hb = 0.5f * sb;
+    const real ha = RL(0.5) * sa, hb = RL(0.5) * sb;
     const Corner ca = corner(N, px + ha * v1[0], py + ha * v1[1], pz + ha * v1[2]);
     const Corner cb = corner(N, px + hb * v1[0], py + hb * v1[1], pz + hb * v1[2]);
-    float va[3], vb[3];
+    real va[3], vb[3];
     sample_vel(vel, N, n3, ca, va);
     sample_vel(vel, N, n3, cb, vb);
     qa[0] = px + sa * va[0]; qa[1] = py + sa * va[1]; qa[2] = pz + sa * va[2];
     qb[0] = px + sb * vb[0]; qb[1] = py + sb * vb[1]; qb[2] = pz + sb * vb[2];
 }
 
-__device__ __forceinline__ void trace1(const float* __restrict__ vel, int N, int n3, float px, float py, float pz,
-                                       float s, float* q) {
+__device__ __forceinline__ void trace1(const real* __restrict__ vel, int N, int n3, real px, real py, real pz,
+                                       real s, real* q) {
     const Corner c0 = corner(N, px, py, pz);
-    float v1[3];
+    real v1[3];
     sample_vel(vel, N, n3, c0, v1);
-    const float h = 0.5f * s;
+    const real h = RL(0.5) * s;
     const Corner c1 = corner(N, px + h * v1[0], py + h * v1[1], pz + h * v1[2]);
-    float v2[3];
+    real v2[3];
     sample_vel(vel, N, n3, c1, v2);
     q[0] = px + s * v2[0];
     q[1] = py + s * v2[1];
@@ -109,16 +109,16 @@ __device__ __forceinline__ void trace1(const float* __restrict__ vel, int N, int
 
 // Pass 1: semi-Lagrangian backward value of C channels -> hat.
 template <int C>
-__global__ void __launch_bounds__(256) k_adv_sl(int N, const float* __restrict__ src, const float* __restrict__ vel,
-                                                const float* __restrict__ prm, float inv_dx, float* __restrict__ hat) {
+__global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ src, const real* __restrict__ vel,
+                                                const float* __restrict__ prm, real inv_dx, real* __restrict__ hat) {
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z;
     if (x >= N || y >= N) return;
-    const float s = prm[P_DT] * inv_dx;
-    float q[3];
-    trace1(vel, N, n3, (float)x, (float)y, (float)z, -s, q);
+    const real s = (real)prm[P_DT] * inv_dx;
+    real q[3];
+    trace1(vel, N, n3, (real)x, (real)y, (real)z, -s, q);
     const Corner c = corner(N, q[0], q[1], q[2]);
     Oct g[C];
 #pragma unroll
@@ -131,9 +131,9 @@ __global__ void __launch_bounds__(256) k_adv_sl(int N, const float* __restrict__
 // Pass 2: MacCormack correction with the 8-corner clamp; optionally the block partial sums of
 // fluid density before (src[0]) and after (out[0]) advection (solver.py:46-65), in double.
 template <int C>
-__global__ void __launch_bounds__(256) k_adv_mc(int N, const float* __restrict__ src, const float* __restrict__ vel,
-                                                const float* __restrict__ prm, float inv_dx,
-                                                const float* __restrict__ hat, float* __restrict__ out,
+__global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ src, const real* __restrict__ vel,
+                                                const float* __restrict__ prm, real inv_dx,
+                                                const real* __restrict__ hat, real* __restrict__ out,
                                                 const uint8_t* __restrict__ fm, double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
     const int n3 = N * N * N;
@@ -143,14 +143,14 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const float* __restrict__
     const bool live = x < N && y < N;
     double sa = 0.0, sb = 0.0;
     if (live) {
-        const float s = prm[P_DT] * inv_dx;
-        float q[3], r[3];
-        trace2(vel, N, n3, (float)x, (float)y, (float)z, -s, s, q, r);
+        const real s = (real)prm[P_DT] * inv_dx;
+        real q[3], r[3];
+        trace2(vel, N, n3, (real)x, (real)y, (real)z, -s, s, q, r);
         const int qo = (base_idx(q[2], N) * N + base_idx(q[1], N)) * N + base_idx(q[0], N);
         const Corner cr = corner(N, r[0], r[1], r[2]);
         const int i = (z * N + y) * N + x;
         Oct gs[C], gh[C];
-        float h[C], s0[C];
+        real h[C], s0[C];
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
             gs[ch] = gather(src + ch * n3, N, qo);
@@ -161,12 +161,12 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const float* __restrict__
         const int fmi = part != nullptr ? (int)fm[i] : 0;
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
-            const float bar = lerp3(gh[ch], cr);
-            float val = h[ch] + 0.5f * (s0[ch] - bar);
-            float lo = gs[ch].v[0], hi = lo;
+            const real bar = lerp3(gh[ch], cr);
+            real val = h[ch] + RL(0.5) * (s0[ch] - bar);
+            real lo = gs[ch].v[0], hi = lo;
 #pragma unroll
             for (int k = 1; k < 8; ++k) {
-                const float v = gs[ch].v[k];
+                const real v = gs[ch].v[k];
                 lo = v < lo ? v : lo;
                 hi = v > hi ? v : hi;
             }
@@ -222,8 +222,8 @@ static dim3 cell_grid(const Geom& g) { return dim3(g.grid.x, g.grid.y, (unsigned
 int advect_parts(const Geom& g) { return (int)(g.grid.x * g.grid.y) * g.N; }
 
 template <int C>
-static void launch_pass_t(int pass, const Geom& g, const float* src, const float* vel, const float* prm, float inv_dx,
-                          float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
+static void launch_pass_t(int pass, const Geom& g, const real* src, const real* vel, const float* prm, real inv_dx,
+                          real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s) {
     if (pass == 0)
         hipLaunchKernelGGL(k_adv_sl<C>, cell_grid(g), g.block, 0, s, g.N, src, vel, prm, inv_dx, hat);
     else
@@ -231,8 +231,8 @@ static void launch_pass_t(int pass, const Geom& g, const float* src, const float
                            advect_parts(g));
 }
 
-void launch_adv_pass(int pass, const Geom& g, int C, const float* src, const float* vel, const float* prm,
-                     float inv_dx, float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
+void launch_adv_pass(int pass, const Geom& g, int C, const real* src, const real* vel, const float* prm,
+                     real inv_dx, real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s) {
     switch (C) {
         case 1: launch_pass_t<1>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
         case 2: launch_pass_t<2>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
@@ -243,8 +243,8 @@ void launch_adv_pass(int pass, const Geom& g, int C, const float* src, const flo
     }
 }
 
-void launch_advect(const Geom& g, int C, const float* src, const float* vel, const float* prm, float inv_dx,
-                   float* hat, float* out, const uint8_t* fm, double* part, hipStream_t s) {
+void launch_advect(const Geom& g, int C, const real* src, const real* vel, const float* prm, real inv_dx,
+                   real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s) {
     launch_adv_pass(0, g, C, src, vel, prm, inv_dx, hat, out, fm, part, s);
     launch_adv_pass(1, g, C, src, vel, prm, inv_dx, hat, out, fm, part, s);
 }

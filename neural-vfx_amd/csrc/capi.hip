@@ -9,6 +9,10 @@
 #include "../../include/vfxsim3d.h"
 #include "sim3d_kernels.hpp"
 
+#if VFX_F64
+#define vfx_ctx vfx_ctx_f64   // the context of the FP64 build (include/vfxsim3d.h)
+#endif
+
 namespace vfx {
 
 static thread_local std::string g_err;
@@ -58,11 +62,11 @@ constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag
 struct vfx_ctx {
     int N = 0;
     Geom g;
-    float inv_dx = 0.0f, inv2dx = 0.0f;
+    real inv_dx = RL(0.0), inv2dx = RL(0.0);
     // device buffers
     void* pool = nullptr;
-    float *hat = nullptr, *adv = nullptr, *omega = nullptr, *omega_s = nullptr, *force = nullptr;
-    float *tmp_a = nullptr, *tmp_b = nullptr;
+    real *hat = nullptr, *adv = nullptr, *omega = nullptr, *omega_s = nullptr, *force = nullptr;
+    real *tmp_a = nullptr, *tmp_b = nullptr;
     float* prm = nullptr;      // parameters of the step
     float* prm_adv = nullptr;  // parameters of the standalone advect()
     double* mass_part = nullptr;
@@ -123,7 +127,7 @@ static int validate_prm(const float* prm) {
     return VFX_OK;
 }
 
-static int stage_advect(vfx_ctx* c, float* data, const float* solid, hipStream_t s) {
+static int stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s) {
     c->P.setup(solid, s);
     CHECK_LAUNCH("obstacle setup");
     launch_advect(c->g, 6, data, data + c->g.n3, c->prm, c->inv_dx, c->hat, c->adv, c->P.fm, c->mass_part, s);
@@ -132,7 +136,7 @@ static int stage_advect(vfx_ctx* c, float* data, const float* solid, hipStream_t
     return VFX_OK;
 }
 
-static int stage_forces(vfx_ctx* c, float* data, hipStream_t s) {
+static int stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
     launch_sources_diffuse(c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, data, s);
     launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega, c->omega_s, c->force, c->tmp_a, c->tmp_b,
                        s);
@@ -167,9 +171,9 @@ static int run_cg(vfx_ctx* c, hipStream_t s, int* iters) {
     return VFX_OK;
 }
 
-static int stage_project(vfx_ctx* c, float* data, hipStream_t s) {
+static int stage_project(vfx_ctx* c, real* data, hipStream_t s) {
     Pressure& P = c->P;
-    float* vel = data + c->g.n3;
+    real* vel = data + c->g.n3;
     P.rhs(vel, c->prm, s);
     CHECK_LAUNCH("pressure rhs");
     int upper = 0;
@@ -194,7 +198,7 @@ static int stage_project(vfx_ctx* c, float* data, hipStream_t s) {
             c->last_restarts += 1;
             // restart from the true residual: warm start from the current phi
             float saved = c->prm_last[P_WARM];
-            c->prm_last[P_WARM] = 1.0f;
+            c->prm_last[P_WARM] = RL(1.0);
             rc = upload(c, c->prm_last, c->prm, s);
             c->prm_last[P_WARM] = saved;
             if (rc) return rc;
@@ -220,16 +224,16 @@ static int check_ctx(vfx_ctx* c) {
 
 extern "C" {
 
-int vfx_p_size(void) { return P_SIZE; }
+int VFX_FN(vfx_p_size)(void) { return P_SIZE; }
 
-int vfx_last_error(char* buf, int n) {
+int VFX_FN(vfx_last_error)(char* buf, int n) {
     if (buf && n > 0) {
         std::snprintf(buf, (size_t)n, "%s", g_err.c_str());
     }
     return VFX_OK;
 }
 
-int vfx_ctx_create(int res, vfx_ctx** out) {
+int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     if (out == nullptr) return arg_error("out is NULL");
     *out = nullptr;
     if (res < 2 || res % 2 != 0) return arg_error("even grid size >= 2 required");
@@ -241,8 +245,8 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     c->g = make_geom(N, 1);   // one cell per thread: every load of a thread is in flight at once
     const double dx = 1.0 / (double)N;   // DOMAIN_SIZE / res (fields.py:25)
     const double inv = 1.0 / dx;
-    c->inv_dx = (float)inv;
-    c->inv2dx = (float)(0.5 * inv);
+    c->inv_dx = (real)inv;
+    c->inv2dx = (real)(0.5 * inv);
     Pressure& P = c->P;
     P.init_levels(N);
     P.geom = c->g;
@@ -253,10 +257,10 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     // doubles, then the mask
     const size_t nf = n3 * (6 + 6 + 3 + 3 + 3 + 4 + 4) + 2 * (size_t)P_SIZE + n3 * 7 + P.level_floats();
     const size_t nd = mass_part_doubles(c->g) + (size_t)P.nparts * (3 + 1) + sizeof(DevState) / sizeof(double) + 8;
-    const size_t bytes = nf * 4 + 64 + nd * 8 + 64 + n3 + 64;
+    const size_t bytes = nf * sizeof(real) + 64 + nd * 8 + 64 + n3 + 64;
     if (!hip_ok(hipMalloc(&c->pool, bytes), "device allocation")) { delete c; return VFX_E_HIP; }
     if (!hip_ok(hipMemset(c->pool, 0, bytes), "device clear")) { hipFree(c->pool); delete c; return VFX_E_HIP; }
-    float* f = (float*)c->pool;
+    real* f = (real*)c->pool;
     c->hat = f; f += 6 * n3;
     c->adv = f; f += 6 * n3;
     c->omega = f; f += 3 * n3;
@@ -264,8 +268,8 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
     c->force = f; f += 3 * n3;
     c->tmp_a = f; f += 4 * n3;
     c->tmp_b = f; f += 4 * n3;
-    c->prm = f; f += P_SIZE;
-    c->prm_adv = f; f += P_SIZE;
+    c->prm = reinterpret_cast<float*>(f); f += P_SIZE;
+    c->prm_adv = reinterpret_cast<float*>(f); f += P_SIZE;
     P.R[0] = f; f += n3;
     P.R[1] = f; f += n3;
     P.P[0] = f; f += n3;
@@ -294,12 +298,12 @@ int vfx_ctx_create(int res, vfx_ctx** out) {
         ok = hip_ok(hipEventCreateWithFlags(&c->stop_ev[i], hipEventDisableTiming), "event");
     ok = ok && hip_ok(hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming), "event");
     std::memset(c->prm_last, 0, sizeof(c->prm_last));
-    if (!ok) { vfx_ctx_destroy(c); return VFX_E_HIP; }
+    if (!ok) { VFX_FN(vfx_ctx_destroy)(c); return VFX_E_HIP; }
     *out = c;
     return VFX_OK;
 }
 
-int vfx_ctx_destroy(vfx_ctx* c) {
+int VFX_FN(vfx_ctx_destroy)(vfx_ctx* c) {
     if (!c) return VFX_OK;
     hipDeviceSynchronize();
     for (int i = 0; i < HOST_RING; ++i) if (c->ring_ev[i]) hipEventDestroy(c->ring_ev[i]);
@@ -312,7 +316,7 @@ int vfx_ctx_destroy(vfx_ctx* c) {
     return VFX_OK;
 }
 
-int vfx_set_option(vfx_ctx* c, int key, double value) {
+int VFX_FN(vfx_set_option)(vfx_ctx* c, int key, double value) {
     if (int rc = check_ctx(c)) return rc;
     switch (key) {
         case VFX_OPT_TRUE_RESIDUAL: c->opt_true_res = value != 0.0; return VFX_OK;
@@ -338,7 +342,7 @@ int vfx_set_option(vfx_ctx* c, int key, double value) {
     }
 }
 
-int vfx_step_stage(vfx_ctx* c, int stage, float* data, const float* solid, const float* prm_host, void* stream) {
+int VFX_FN(vfx_step_stage)(vfx_ctx* c, int stage, real* data, const real* solid, const float* prm_host, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!data || !solid) return arg_error("null field pointer");
     hipStream_t s = S(stream);
@@ -358,7 +362,7 @@ int vfx_step_stage(vfx_ctx* c, int stage, float* data, const float* solid, const
     }
 }
 
-int vfx_step(vfx_ctx* c, float* data, const float* solid, const float* prm_host, void* stream) {
+int VFX_FN(vfx_step)(vfx_ctx* c, real* data, const real* solid, const float* prm_host, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!data || !solid) return arg_error("null field pointer");
     if (int rc = validate_prm(prm_host)) return rc;
@@ -370,7 +374,7 @@ int vfx_step(vfx_ctx* c, float* data, const float* solid, const float* prm_host,
     return stage_project(c, data, s);
 }
 
-int vfx_project(vfx_ctx* c, float* data, const float* solid, const float* prm_host, void* stream) {
+int VFX_FN(vfx_project)(vfx_ctx* c, real* data, const real* solid, const float* prm_host, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!data || !solid) return arg_error("null field pointer");
     if (int rc = validate_prm(prm_host)) return rc;
@@ -382,7 +386,7 @@ int vfx_project(vfx_ctx* c, float* data, const float* solid, const float* prm_ho
     return stage_project(c, data, s);
 }
 
-int vfx_advect(vfx_ctx* c, const float* field, int C, const float* vel, float* out, float dt, void* stream) {
+int VFX_FN(vfx_advect)(vfx_ctx* c, const real* field, int C, const real* vel, real* out, float dt, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!field || !vel || !out) return arg_error("null pointer");
     if (C < 1) return arg_error("channel count must be >= 1");
@@ -401,7 +405,7 @@ int vfx_advect(vfx_ctx* c, const float* field, int C, const float* vel, float* o
     return VFX_OK;
 }
 
-int vfx_div_abs(vfx_ctx* c, const float* data, const float* solid, float* out, void* stream) {
+int VFX_FN(vfx_div_abs)(vfx_ctx* c, const real* data, const real* solid, real* out, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!data || !solid || !out) return arg_error("null pointer");
     launch_div_abs(c->g, data + c->g.n3, solid, out, S(stream));
@@ -409,7 +413,7 @@ int vfx_div_abs(vfx_ctx* c, const float* data, const float* solid, float* out, v
     return VFX_OK;
 }
 
-int vfx_pressure_iters(vfx_ctx* c, int* out) {
+int VFX_FN(vfx_pressure_iters)(vfx_ctx* c, int* out) {
     if (int rc = check_ctx(c)) return rc;
     if (!out) return arg_error("null pointer");
     if (!c->projected) { *out = 0; return VFX_OK; }
@@ -420,7 +424,7 @@ int vfx_pressure_iters(vfx_ctx* c, int* out) {
     return VFX_OK;
 }
 
-int vfx_pressure_stats(vfx_ctx* c, double* out, int n) {
+int VFX_FN(vfx_pressure_stats)(vfx_ctx* c, double* out, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (!out || n < 1) return arg_error("null pointer");
     double v[7] = {0, 0, 0, 0, 0, -1, 0};
@@ -440,24 +444,24 @@ int vfx_pressure_stats(vfx_ctx* c, double* out, int n) {
     return VFX_OK;
 }
 
-int vfx_get_phi(vfx_ctx* c, float* dst, void* stream) {
+int VFX_FN(vfx_get_phi)(vfx_ctx* c, real* dst, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!dst) return arg_error("null pointer");
-    CHECK_HIP(hipMemcpyAsync(dst, c->P.phi, c->g.n3 * 4, hipMemcpyDeviceToDevice, S(stream)), "phi copy");
+    CHECK_HIP(hipMemcpyAsync(dst, c->P.phi, c->g.n3 * sizeof(real), hipMemcpyDeviceToDevice, S(stream)), "phi copy");
     return VFX_OK;
 }
 
-int vfx_set_phi(vfx_ctx* c, const float* src, void* stream) {
+int VFX_FN(vfx_set_phi)(vfx_ctx* c, const real* src, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!src) return arg_error("null pointer");
-    CHECK_HIP(hipMemcpyAsync(c->P.phi, src, c->g.n3 * 4, hipMemcpyDeviceToDevice, S(stream)), "phi copy");
+    CHECK_HIP(hipMemcpyAsync(c->P.phi, src, c->g.n3 * sizeof(real), hipMemcpyDeviceToDevice, S(stream)), "phi copy");
     return VFX_OK;
 }
 
-int vfx_get_advected(vfx_ctx* c, float* dst, double* mass_ratio, void* stream) {
+int VFX_FN(vfx_get_advected)(vfx_ctx* c, real* dst, double* mass_ratio, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     hipStream_t s = S(stream);
-    if (dst) CHECK_HIP(hipMemcpyAsync(dst, c->adv, 6 * c->g.n3 * 4, hipMemcpyDeviceToDevice, s), "adv copy");
+    if (dst) CHECK_HIP(hipMemcpyAsync(dst, c->adv, 6 * c->g.n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "adv copy");
     if (mass_ratio) {
         DevState hs;
         CHECK_HIP(hipMemcpyAsync(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "state read");
@@ -467,7 +471,7 @@ int vfx_get_advected(vfx_ctx* c, float* dst, double* mass_ratio, void* stream) {
     return VFX_OK;
 }
 
-int vfx_kmul(vfx_ctx* c, const float* solid, const float* p, float* out, void* stream) {
+int VFX_FN(vfx_kmul)(vfx_ctx* c, const real* solid, const real* p, real* out, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!solid || !p || !out) return arg_error("null pointer");
     hipStream_t s = S(stream);
@@ -477,21 +481,21 @@ int vfx_kmul(vfx_ctx* c, const float* solid, const float* p, float* out, void* s
     return VFX_OK;
 }
 
-int vfx_precond(vfx_ctx* c, const float* solid, const float* r, float* z, void* stream) {
+int VFX_FN(vfx_precond)(vfx_ctx* c, const real* solid, const real* r, real* z, void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!solid || !r || !z) return arg_error("null pointer");
     hipStream_t s = S(stream);
     const size_t n3 = c->g.n3;
     c->P.setup(solid, s);
-    CHECK_HIP(hipMemcpyAsync(c->P.R[0], r, n3 * 4, hipMemcpyDeviceToDevice, s), "r copy");
+    CHECK_HIP(hipMemcpyAsync(c->P.R[0], r, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "r copy");
     hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
     c->P.precond(0, s);
     CHECK_LAUNCH("precond");
-    CHECK_HIP(hipMemcpyAsync(z, c->P.z, n3 * 4, hipMemcpyDeviceToDevice, s), "z copy");
+    CHECK_HIP(hipMemcpyAsync(z, c->P.z, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "z copy");
     return VFX_OK;
 }
 
-int vfx_time_kernel(vfx_ctx* c, int which, int reps, void* stream, float* ms_avg) {
+int VFX_FN(vfx_time_kernel)(vfx_ctx* c, int which, int reps, void* stream, float* ms_avg) {
     if (int rc = check_ctx(c)) return rc;
     if (!ms_avg || reps < 1) return arg_error("bad arguments");
     hipStream_t s = S(stream);
@@ -540,7 +544,7 @@ int vfx_time_kernel(vfx_ctx* c, int which, int reps, void* stream, float* ms_avg
     return VFX_OK;
 }
 
-int vfx_get_ops(vfx_ctx* c, const float* solid, int level, float* diag, float* off, int* nlev, int* m,
+int VFX_FN(vfx_get_ops)(vfx_ctx* c, const real* solid, int level, real* diag, real* off, int* nlev, int* m,
                 void* stream) {
     if (int rc = check_ctx(c)) return rc;
     if (!solid) return arg_error("null pointer");
@@ -552,8 +556,8 @@ int vfx_get_ops(vfx_ctx* c, const float* solid, int level, float* diag, float* o
     c->P.setup(solid, s);
     CHECK_LAUNCH("setup");
     const size_t m3 = (size_t)L.m * L.m * L.m;
-    if (diag) CHECK_HIP(hipMemcpyAsync(diag, L.diag, m3 * 8 * 4, hipMemcpyDeviceToDevice, s), "diag copy");
-    if (off) CHECK_HIP(hipMemcpyAsync(off, L.off, m3 * 24 * 4, hipMemcpyDeviceToDevice, s), "off copy");
+    if (diag) CHECK_HIP(hipMemcpyAsync(diag, L.diag, m3 * 8 * sizeof(real), hipMemcpyDeviceToDevice, s), "diag copy");
+    if (off) CHECK_HIP(hipMemcpyAsync(off, L.off, m3 * 24 * sizeof(real), hipMemcpyDeviceToDevice, s), "off copy");
     return VFX_OK;
 }
 

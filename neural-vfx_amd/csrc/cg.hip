@@ -49,7 +49,7 @@ __device__ __forceinline__ Tile this_tile(int N) {
 
 // Tiles are staged in 2x2x2 blocks (the level-1 cells): halo block b (0..HBN-1) is the level-1
 // block (I, J, K) = first block of the tile - 1 + (b % HBX, b / HBX % HBY, b / (HBX HBY)), read
-// as four rows of two cells (float2 values, two mask bytes).  N is even, so a block is wholly
+// as four rows of two cells (real2 values, two mask bytes).  N is even, so a block is wholly
 // inside or outside the box; one outside is read at the clamped block (field values, scaled by
 // in = 0) with its mask at n3, the zero bytes past the end of the mask array.  Staging loads are
 // unconditional, so all of a thread's loads are in flight together; only the LDS stores of the
@@ -83,11 +83,11 @@ __device__ __forceinline__ int own_hb(int t) {
 __device__ __forceinline__ int row_off(int N, int rw) { return (rw & 1) * N + (rw >> 1) * N * N; }
 __device__ __forceinline__ int row_h(int rw) { return (rw & 1) * HX + (rw >> 1) * HX * HY; }
 
-__device__ __forceinline__ float2 ld2(const float* __restrict__ f, int i) {
-    return *reinterpret_cast<const float2*>(f + i);
+__device__ __forceinline__ real2 ld2(const real* __restrict__ f, int i) {
+    return *reinterpret_cast<const real2*>(f + i);
 }
-__device__ __forceinline__ void st2(float* __restrict__ f, int i, float a, float b) {
-    *reinterpret_cast<float2*>(f + i) = make_float2(a, b);
+__device__ __forceinline__ void st2(real* __restrict__ f, int i, real a, real b) {
+    *reinterpret_cast<real2*>(f + i) = make_real2(a, b);
 }
 __device__ __forceinline__ int ldm2(const uint8_t* __restrict__ fm, int i) {
     return *reinterpret_cast<const uint16_t*>(fm + i);
@@ -107,14 +107,14 @@ __device__ __forceinline__ bool wave_interior(const uint8_t* sm, int h0) {
 
 // K p at halo cell h (0 at a solid cell)
 template <bool INT>
-__device__ __forceinline__ float kq(const float* P, const uint8_t* M, int h) {
+__device__ __forceinline__ real kq(const real* P, const uint8_t* M, int h) {
     if (INT) return kmul_tile_interior(P, h);
-    return M[h] ? kmul_tile(P, M, h) : 0.0f;
+    return M[h] ? kmul_tile(P, M, h) : RL(0.0);
 }
 
 // z0 + T^T x1 at a cell of parity (dz, dy, dx) from hx[c] = HS x_c: hsign(c) x_c = +-(HS x_c)
 // exactly, so the sum equals the reference's c-ordered sum of hsign * x
-__device__ __forceinline__ float inv_sum(float acc, const float* hx, int dz, int dy, int dx) {
+__device__ __forceinline__ real inv_sum(real acc, const real* hx, int dz, int dy, int dx) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int e = (c & 1) * dx + ((c >> 1) & 1) * dy + ((c >> 2) & 1) * dz;
@@ -125,25 +125,25 @@ __device__ __forceinline__ float inv_sum(float acc, const float* hx, int dz, int
 
 // k_rhs (:602-627): r = -d(u*) - K phi0 (phi = 0 without warm start); block max |u*| over fluid
 // and block sum of b^2 (b = -d(u*)) for the relative-residual test.
-__global__ void __launch_bounds__(256) k_rhs(int N, int zc, const float* __restrict__ vel,
-                                             const uint8_t* __restrict__ fm, float* __restrict__ r,
-                                             float* __restrict__ phi, const float* __restrict__ prm,
+__global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restrict__ vel,
+                                             const uint8_t* __restrict__ fm, real* __restrict__ r,
+                                             real* __restrict__ phi, const float* __restrict__ prm,
                                              double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
     const int n3 = N * N * N;
-    const bool warm = prm[P_WARM] > 0.5f;
+    const bool warm = prm[P_WARM] > RL(0.5);
     double mx = 0.0, bb = 0.0;
     CELL_LOOP_BEGIN
-        if (!warm) phi[i] = 0.0f;
+        if (!warm) phi[i] = RL(0.0);
         const int m = fm[i];
-        if (m == 0) { r[i] = 0.0f; continue; }
-        const float b = -div2_xyz(vel, n3, m, N, x, y, cz, i);
-        float rv = b;
+        if (m == 0) { r[i] = RL(0.0); continue; }
+        const real b = -div2_xyz(vel, n3, m, N, x, y, cz, i);
+        real rv = b;
         if (warm) rv -= kmul_xyz(phi, fm, N, x, y, cz, i, m);
         r[i] = rv;
         bb += (double)b * (double)b;
-        const float ux = vel[i], uy = vel[n3 + i], uz = vel[2 * n3 + i];
-        mx = fmax(mx, (double)sqrtf(ux * ux + uy * uy + uz * uz));
+        const real ux = vel[i], uy = vel[n3 + i], uz = vel[2 * n3 + i];
+        mx = fmax(mx, (double)sqrt(ux * ux + uy * uy + uz * uz));
     CELL_LOOP_END
     const double a = block_max(mx, sh);
     const double b2 = block_sum(bb, sh);
@@ -172,20 +172,20 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
 // k_upd_p of iteration k-1 (:712-728, when UPD) + k_matvec_dot of iteration k (:629-646):
 // p = z + beta p_old, phi += alpha p_old; q = K p; block partial p.q (double).
 template <bool UPD>
-__global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float* __restrict__ z,
-                                                             const float* __restrict__ p_in, float* __restrict__ p_out,
-                                                             float* __restrict__ phi, const uint8_t* __restrict__ fm,
-                                                             float* __restrict__ q, double* __restrict__ part,
+__global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const real* __restrict__ z,
+                                                             const real* __restrict__ p_in, real* __restrict__ p_out,
+                                                             real* __restrict__ phi, const uint8_t* __restrict__ fm,
+                                                             real* __restrict__ q, double* __restrict__ part,
                                                              const DevState* __restrict__ st, int k) {
-    __shared__ __align__(16) float sp[HN];
+    __shared__ __align__(16) real sp[HN];
     __shared__ __align__(16) uint8_t sm[HN];
     __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
-    const float a = (float)st->s[S_ALPHA], b = (float)st->s[S_BETA];
+    const real a = (real)st->s[S_ALPHA], b = (real)st->s[S_BETA];
     const int t = (int)threadIdx.x, n3 = N * N * N;
     const HBlock ob = halo_block(N, tl, own_hb(t));
-    float2 po[4], ph[4];
+    real2 po[4], ph[4];
     if (UPD) {
 #pragma unroll
         for (int rw = 0; rw < 4; ++rw) {
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
         }
     }
     HBlock hb[HB_IT];
-    float2 hp[HB_IT][4], hz[HB_IT][4];
+    real2 hp[HB_IT][4], hz[HB_IT][4];
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
@@ -211,11 +211,11 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
         if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
-            const float in = hb[it].in ? 1.0f : 0.0f;
+            const real in = hb[it].in ? RL(1.0) : RL(0.0);
 #pragma unroll
             for (int rw = 0; rw < 4; ++rw) {
-                const float2 pv = hp[it][rw];
-                float v0 = pv.x, v1 = pv.y;
+                const real2 pv = hp[it][rw];
+                real v0 = pv.x, v1 = pv.y;
                 if (UPD) {
                     v0 = hz[it][rw].x + b * pv.x;
                     v1 = hz[it][rw].y + b * pv.y;
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
     }
     __syncthreads();
     // the thread's own block; a block outside the box is staged as mask 0, value 0 (terms 0)
-    float qv[8];
+    real qv[8];
     if (wave_interior(sm, ob.h0)) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) qv[c] = kq<true>(sp, sm, ob.h0 + row_h(c >> 1) + (c & 1));
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const float*
 #pragma unroll
     for (int rw = 0; rw < 4; ++rw) {
         const int h = ob.h0 + row_h(rw);
-        const float pn0 = sp[h], pn1 = sp[h + 1];
+        const real pn0 = sp[h], pn1 = sp[h + 1];
         s += (double)qv[2 * rw] * (double)pn0;
         s += (double)qv[2 * rw + 1] * (double)pn1;
         if (ob.in) {
@@ -270,16 +270,16 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_pq(const double* __restrict
 // 2x2x2 block of the tile: rr = r - K z0 on its 8 cells, the level-1 components and the
 // zero-guess red sweep.
 template <bool UPDR>
-__global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const float* __restrict__ r_in,
-                                                                float* __restrict__ r_out, const float* __restrict__ q,
+__global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const real* __restrict__ r_in,
+                                                                real* __restrict__ r_out, const real* __restrict__ q,
                                                                 const uint8_t* __restrict__ fm, Lvl L, int do_red,
                                                                 const DevState* __restrict__ st, int k) {
-    __shared__ __align__(16) float sz[HN];
-    __shared__ __align__(16) float sr[HN];
+    __shared__ __align__(16) real sz[HN];
+    __shared__ __align__(16) real sr[HN];
     __shared__ __align__(16) uint8_t sm[HN];
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
-    const float a = (float)st->s[S_ALPHA];
+    const real a = (real)st->s[S_ALPHA];
     const int t = (int)threadIdx.x, n3 = N * N * N;
     const int bi = t % BTX, bj = (t / BTX) % BTY, bk = t / (BTX * BTY);
     const int mb = L.m;
@@ -287,14 +287,14 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
     const HBlock ob = halo_block(N, tl, own_hb(t));
     const size_t cell = (size_t)ob.lc;
     const bool red = ob.in && do_red && ((I + J + K) & 1) == 0;
-    float4 dg0 = make_float4(0.f, 0.f, 0.f, 0.f), dg1 = dg0;
+    real4 dg0 = make_real4(RL(0.), RL(0.), RL(0.), RL(0.)), dg1 = dg0;
     if (red) {
-        const float4* dp = reinterpret_cast<const float4*>(L.diag + cell * 8);
+        const real4* dp = reinterpret_cast<const real4*>(L.diag + cell * 8);
         dg0 = dp[0];
         dg1 = dp[1];
     }
     HBlock hb[HB_IT];
-    float2 hr[HB_IT][4], hq[HB_IT][4];
+    real2 hr[HB_IT][4], hq[HB_IT][4];
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
@@ -311,21 +311,21 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
         if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
-            const float in = hb[it].in ? 1.0f : 0.0f;
+            const real in = hb[it].in ? RL(1.0) : RL(0.0);
 #pragma unroll
             for (int rw = 0; rw < 4; ++rw) {
-                float rv[2] = {hr[it][rw].x, hr[it][rw].y};
+                real rv[2] = {hr[it][rw].x, hr[it][rw].y};
                 if (UPDR) {
                     rv[0] = rv[0] - a * hq[it][rw].x;
                     rv[1] = rv[1] - a * hq[it][rw].y;
                 }
                 const int h = hb[it].h0 + row_h(rw);
-                float zv[2];
+                real zv[2];
 #pragma unroll
                 for (int dx = 0; dx < 2; ++dx) {
                     const int m = (hm[it][rw] >> (8 * dx)) & 255;
                     rv[dx] = rv[dx] * in;
-                    zv[dx] = m ? OMEGA_JAC * rv[dx] / kdiag(m) : 0.0f;
+                    zv[dx] = m ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0);
                 }
                 st2(sz, h, zv[0], zv[1]);
                 st2(sr, h, rv[0], rv[1]);
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
     }
     __syncthreads();
     if (!ob.in) return;
-    float kz[8];
+    real kz[8];
     if (wave_interior(sm, ob.h0)) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) kz[c] = kq<true>(sz, sm, ob.h0 + row_h(c >> 1) + (c & 1));
@@ -343,32 +343,32 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
 #pragma unroll
         for (int c = 0; c < 8; ++c) kz[c] = kq<false>(sz, sm, ob.h0 + row_h(c >> 1) + (c & 1));
     }
-    float rr[8];
+    real rr[8];
 #pragma unroll
     for (int rw = 0; rw < 4; ++rw) {
         const int h = ob.h0 + row_h(rw);
-        const float r0 = sr[h], r1 = sr[h + 1];
+        const real r0 = sr[h], r1 = sr[h + 1];
         if (UPDR) st2(r_out, ob.i0 + row_off(N, rw), r0, r1);
-        rr[2 * rw] = sm[h] ? r0 - kz[2 * rw] : 0.0f;
-        rr[2 * rw + 1] = sm[h + 1] ? r1 - kz[2 * rw + 1] : 0.0f;
+        rr[2 * rw] = sm[h] ? r0 - kz[2 * rw] : RL(0.0);
+        rr[2 * rw + 1] = sm[h + 1] ? r1 - kz[2 * rw + 1] : RL(0.0);
     }
-    const float dg[8] = {dg0.x, dg0.y, dg0.z, dg0.w, dg1.x, dg1.y, dg1.z, dg1.w};
-    float out[8], xo[8];
+    const real dg[8] = {dg0.x, dg0.y, dg0.z, dg0.w, dg1.x, dg1.y, dg1.z, dg1.w};
+    real out[8], xo[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        float acc = 0.0f;
+        real acc = RL(0.0);
 #pragma unroll
         for (int s = 0; s < 8; ++s) acc += hsign(c, s >> 2, (s >> 1) & 1, s & 1) * rr[s];
         out[c] = acc;
-        xo[c] = dg[c] <= L.dmin ? 0.0f : acc / dg[c];
+        xo[c] = dg[c] <= L.dmin ? RL(0.0) : acc / dg[c];
     }
-    float4* rp = reinterpret_cast<float4*>(L.r + cell * 8);
-    rp[0] = make_float4(out[0], out[1], out[2], out[3]);
-    rp[1] = make_float4(out[4], out[5], out[6], out[7]);
+    real4* rp = reinterpret_cast<real4*>(L.r + cell * 8);
+    rp[0] = make_real4(out[0], out[1], out[2], out[3]);
+    rp[1] = make_real4(out[4], out[5], out[6], out[7]);
     if (red) {
-        float4* xp4 = reinterpret_cast<float4*>(L.x + cell * 8);
-        xp4[0] = make_float4(xo[0], xo[1], xo[2], xo[3]);
-        xp4[1] = make_float4(xo[4], xo[5], xo[6], xo[7]);
+        real4* xp4 = reinterpret_cast<real4*>(L.x + cell * 8);
+        xp4[0] = make_real4(xo[0], xo[1], xo[2], xo[3]);
+        xp4[1] = make_real4(xo[4], xo[5], xo[6], xo[7]);
     }
 }
 
@@ -376,25 +376,25 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const flo
 // the tile + halo, one 2x2x2 block at a time from its 8 level-1 values, then
 // z = zt + omega D^-1 (r - K zt) on the tile and the block partials r.z, max|r|, r.r.  At k = 0
 // z is also written to p_init (the first search direction, k_upd_p init).
-__global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const float* __restrict__ r,
-                                                           const uint8_t* __restrict__ fm, float* __restrict__ z,
-                                                           float* __restrict__ p_init, double* __restrict__ part,
+__global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const real* __restrict__ r,
+                                                           const uint8_t* __restrict__ fm, real* __restrict__ z,
+                                                           real* __restrict__ p_init, double* __restrict__ part,
                                                            int nparts, const DevState* __restrict__ st, int k) {
-    __shared__ __align__(16) float szt[HN];
-    __shared__ __align__(16) float sr[HN];
+    __shared__ __align__(16) real szt[HN];
+    __shared__ __align__(16) real sr[HN];
     __shared__ __align__(16) uint8_t sm[HN];
     __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
     const int t = (int)threadIdx.x, n3 = N * N * N;
     HBlock hb[HB_IT];
-    float4 xa[HB_IT], xb[HB_IT];
-    float2 hr[HB_IT][4];
+    real4 xa[HB_IT], xb[HB_IT];
+    real2 hr[HB_IT][4];
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
         hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
-        const float4* xp = reinterpret_cast<const float4*>(L.x + (size_t)hb[it].lc * 8);
+        const real4* xp = reinterpret_cast<const real4*>(L.x + (size_t)hb[it].lc * 8);
         xa[it] = xp[0];
         xb[it] = xp[1];
 #pragma unroll
@@ -408,19 +408,19 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
         if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
-            const float in = hb[it].in ? 1.0f : 0.0f;
-            const float hx[8] = {HS * (xa[it].x * in), HS * (xa[it].y * in), HS * (xa[it].z * in),
+            const real in = hb[it].in ? RL(1.0) : RL(0.0);
+            const real hx[8] = {HS * (xa[it].x * in), HS * (xa[it].y * in), HS * (xa[it].z * in),
                                  HS * (xa[it].w * in), HS * (xb[it].x * in), HS * (xb[it].y * in),
                                  HS * (xb[it].z * in), HS * (xb[it].w * in)};
 #pragma unroll
             for (int rw = 0; rw < 4; ++rw) {
-                const float rv[2] = {hr[it][rw].x * in, hr[it][rw].y * in};
-                float zt[2];
+                const real rv[2] = {hr[it][rw].x * in, hr[it][rw].y * in};
+                real zt[2];
 #pragma unroll
                 for (int dx = 0; dx < 2; ++dx) {
                     const int m = (hm[it][rw] >> (8 * dx)) & 255;
-                    const float acc = inv_sum(m ? OMEGA_JAC * rv[dx] / kdiag(m) : 0.0f, hx, rw >> 1, rw & 1, dx);
-                    zt[dx] = m ? acc : 0.0f;
+                    const real acc = inv_sum(m ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0), hx, rw >> 1, rw & 1, dx);
+                    zt[dx] = m ? acc : RL(0.0);
                 }
                 const int h = hb[it].h0 + row_h(rw);
                 st2(szt, h, zt[0], zt[1]);
@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
     __syncthreads();
     // the thread's own block; a block outside the box is staged as mask 0, value 0 (terms 0)
     const HBlock ob = halo_block(N, tl, own_hb(t));
-    float kz[8];
+    real kz[8];
     const bool fast = wave_interior(sm, ob.h0);
     if (fast) {
 #pragma unroll
@@ -445,12 +445,12 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const f
 #pragma unroll
     for (int rw = 0; rw < 4; ++rw) {
         const int h = ob.h0 + row_h(rw);
-        float zv[2];
+        real zv[2];
 #pragma unroll
         for (int dx = 0; dx < 2; ++dx) {
             const int m = sm[h + dx];
-            const float rv = sr[h + dx];
-            zv[dx] = m ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : 0.0f;
+            const real rv = sr[h + dx];
+            zv[dx] = m ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : RL(0.0);
             s += (double)rv * (double)zv[dx];
             mx = fmax(mx, fabs((double)rv));
             rr += (double)rv * (double)rv;
@@ -509,26 +509,26 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict
 
 // The phi update of the stopping iteration J (k_upd_p :723-725 of the last loop body):
 // phi += alpha_J p_{J-1}, where p_{J-1} is in p_buf[(J - 1) & 1].
-__global__ void __launch_bounds__(256) k_phi_final(int N, int zc, float* __restrict__ phi,
-                                                   const float* __restrict__ p0, const float* __restrict__ p1,
+__global__ void __launch_bounds__(256) k_phi_final(int N, int zc, real* __restrict__ phi,
+                                                   const real* __restrict__ p0, const real* __restrict__ p1,
                                                    const DevState* __restrict__ st) {
     const int J = st->stop_at;
     if (J < 1 || J == INT_MAX) return;
-    const float a = (float)st->s[S_ALPHA];
-    const float* __restrict__ p = ((J - 1) & 1) ? p1 : p0;
+    const real a = (real)st->s[S_ALPHA];
+    const real* __restrict__ p = ((J - 1) & 1) ? p1 : p0;
     CELL_LOOP_BEGIN
         phi[i] = phi[i] + a * p[i];
     CELL_LOOP_END
 }
 
 // k_apply (:166-175): u -= g(phi) on fluid, u = 0 in solids
-__global__ void __launch_bounds__(256) k_apply(int N, int zc, float* __restrict__ vel,
-                                               const uint8_t* __restrict__ fm, const float* __restrict__ phi) {
+__global__ void __launch_bounds__(256) k_apply(int N, int zc, real* __restrict__ vel,
+                                               const uint8_t* __restrict__ fm, const real* __restrict__ phi) {
     const int n3 = N * N * N;
     CELL_LOOP_BEGIN
         const int m = fm[i];
         if (m == 0) {
-            vel[i] = 0.0f; vel[n3 + i] = 0.0f; vel[2 * n3 + i] = 0.0f;
+            vel[i] = RL(0.0); vel[n3 + i] = RL(0.0); vel[2 * n3 + i] = RL(0.0);
             continue;
         }
 #pragma unroll
@@ -538,8 +538,8 @@ __global__ void __launch_bounds__(256) k_apply(int N, int zc, float* __restrict_
 }
 
 // True residual ||b - K phi||^2 with b = -d(u*) (fresh matvec, FP64 accumulation)
-__global__ void __launch_bounds__(256) k_true_res(int N, int zc, const float* __restrict__ vel,
-                                                  const uint8_t* __restrict__ fm, const float* __restrict__ phi,
+__global__ void __launch_bounds__(256) k_true_res(int N, int zc, const real* __restrict__ vel,
+                                                  const uint8_t* __restrict__ fm, const real* __restrict__ phi,
                                                   double* __restrict__ part) {
     __shared__ double sh[8];
     const int n3 = N * N * N;
@@ -562,21 +562,21 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_true(const double* __restri
 }
 
 // |div u| dx with the mask computed from solid (k_div_abs :156-163 + k_mask)
-__global__ void __launch_bounds__(256) k_div_abs(int N, int zc, const float* __restrict__ vel,
-                                                 const float* __restrict__ solid, float* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_div_abs(int N, int zc, const real* __restrict__ vel,
+                                                 const real* __restrict__ solid, real* __restrict__ out) {
     const int n3 = N * N * N;
     CELL_LOOP_BEGIN
         const int m = mask_from_solid(solid, N, cz, y, x);
-        out[i] = m == 0 ? 0.0f : 0.5f * fabsf(div2_xyz(vel, n3, m, N, x, y, cz, i));
+        out[i] = m == 0 ? RL(0.0) : RL(0.5) * fabs(div2_xyz(vel, n3, m, N, x, y, cz, i));
     CELL_LOOP_END
 }
 
 // K p at fluid cells (parity tests of the operator)
-__global__ void __launch_bounds__(256) k_kmul(int N, int zc, const float* __restrict__ p,
-                                              const uint8_t* __restrict__ fm, float* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_kmul(int N, int zc, const real* __restrict__ p,
+                                              const uint8_t* __restrict__ fm, real* __restrict__ out) {
     CELL_LOOP_BEGIN
         const int m = fm[i];
-        out[i] = m ? kmul_xyz(p, fm, N, x, y, cz, i, m) : 0.0f;
+        out[i] = m ? kmul_xyz(p, fm, N, x, y, cz, i, m) : RL(0.0);
     CELL_LOOP_END
 }
 
@@ -586,11 +586,11 @@ void Pressure::init_tiles() {
     ntiles = (int)(tgrid.x * tgrid.y * tgrid.z);
 }
 
-float* Pressure::r_buf(int k) const { return R[k & 1]; }
+real* Pressure::r_buf(int k) const { return R[k & 1]; }
 
 void Pressure::precond(int k, hipStream_t s) {
-    float* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
-    float* r_out = R[k & 1];
+    real* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
+    real* r_out = R[k & 1];
     if (k > 0)
         hipLaunchKernelGGL(k_pre_resid_fwd<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
                            S > 0 ? 1 : 0, st, k);
@@ -602,7 +602,7 @@ void Pressure::precond(int k, hipStream_t s) {
                        k);
 }
 
-void Pressure::rhs(const float* vel, const float* prm, hipStream_t s) {
+void Pressure::rhs(const real* vel, const float* prm, hipStream_t s) {
     const Geom& g = geom;
     hipLaunchKernelGGL(k_rhs, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, prm, part, nblocks(g));
     hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), prm, st);
@@ -628,12 +628,12 @@ void Pressure::finish(hipStream_t s) {
     hipLaunchKernelGGL(k_phi_final, g.grid, g.block, 0, s, N, g.zc, phi, P[0], P[1], st);
 }
 
-void Pressure::apply(float* vel, hipStream_t s) {
+void Pressure::apply(real* vel, hipStream_t s) {
     const Geom& g = geom;
     hipLaunchKernelGGL(k_apply, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi);
 }
 
-void Pressure::true_residual(const float* vel, hipStream_t s) {
+void Pressure::true_residual(const real* vel, hipStream_t s) {
     const Geom& g = geom;
     hipLaunchKernelGGL(k_true_res, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi, part);
     hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), st);
@@ -642,7 +642,7 @@ void Pressure::true_residual(const float* vel, hipStream_t s) {
 // One kernel of the projection on the current buffers, for timing (ids relative to
 // VFX_K_MATVEC), in its steady-state form (iteration k >= 2).  What a kernel would write into
 // phi, p, q or r goes to `scratch` (3 N^3 floats) instead, so the solver state is untouched.
-bool Pressure::launch_single(int which, float* scratch, hipStream_t s) {
+bool Pressure::launch_single(int which, real* scratch, hipStream_t s) {
     const size_t n3 = (size_t)N * N * N;
     switch (which) {
         case 0:
@@ -661,11 +661,11 @@ bool Pressure::launch_single(int which, float* scratch, hipStream_t s) {
     }
 }
 
-void launch_div_abs(const Geom& g, const float* vel, const float* solid, float* out, hipStream_t s) {
+void launch_div_abs(const Geom& g, const real* vel, const real* solid, real* out, hipStream_t s) {
     hipLaunchKernelGGL(k_div_abs, g.grid, g.block, 0, s, g.N, g.zc, vel, solid, out);
 }
 
-void launch_kmul(const Geom& g, const float* p, const uint8_t* fm, float* out, hipStream_t s) {
+void launch_kmul(const Geom& g, const real* p, const uint8_t* fm, real* out, hipStream_t s) {
     hipLaunchKernelGGL(k_kmul, g.grid, g.block, 0, s, g.N, g.zc, p, fm, out);
 }
 

@@ -8,7 +8,7 @@
 
 namespace vfx {
 
-__device__ __forceinline__ float clampf_(float g, float lo, float hi) {
+__device__ __forceinline__ real clampf_(real g, real lo, real hi) {
     return g < lo ? lo : (g > hi ? hi : g);
 }
 
@@ -31,60 +31,60 @@ __device__ __forceinline__ uint32_t philox_u32(uint32_t seed, uint32_t em, uint3
 
 // sources (kernels.py:182-250) with the ignition ramp width and per-cell emitter noise as
 // parameters (reference behaviour: ramp 100 K, noise off).
-__global__ void __launch_bounds__(256) k_sources(int N, int zc, const float* __restrict__ adv,
+__global__ void __launch_bounds__(256) k_sources(int N, int zc, const real* __restrict__ adv,
                                                  const uint8_t* __restrict__ fm, const float* __restrict__ prm,
-                                                 const DevState* __restrict__ st, float* __restrict__ out) {
+                                                 const DevState* __restrict__ st, real* __restrict__ out) {
     const size_t n3 = (size_t)N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
-    const float amb = prm[P_AMB], dt = prm[P_DT], dx = prm[P_DX];
+    const real amb = (real)prm[P_AMB], dt = (real)prm[P_DT], dx = (real)prm[P_DX];
     const int n_em = (int)prm[P_N_EM], n_imp = (int)prm[P_N_IMP];
-    const float ramp = prm[P_IGN_RAMP], nlo = prm[P_NOISE_LO], nhi = prm[P_NOISE_HI];
-    const bool noisy = !(nlo == 1.0f && nhi == 1.0f);
+    const real ramp = (real)prm[P_IGN_RAMP], nlo = (real)prm[P_NOISE_LO], nhi = (real)prm[P_NOISE_HI];
+    const bool noisy = !(nlo == RL(1.0) && nhi == RL(1.0));
     const uint32_t seed = (uint32_t)prm[P_SEED], stepno = (uint32_t)prm[P_STEPNO];
-    const float ratio = (float)st->s[S_MASS_RATIO];
-    const float tmax = prm[P_TMAX];
-    const float e_burn = expf(-prm[P_BURN] * dt);
-    const float e_cool = expf(-prm[P_COOL] * dt);
-    const float e_diss = expf(-prm[P_DDISS] * dt);
+    const real ratio = (real)st->s[S_MASS_RATIO];
+    const real tmax = (real)prm[P_TMAX];
+    const real e_burn = exp(-(real)prm[P_BURN] * dt);
+    const real e_cool = exp(-(real)prm[P_COOL] * dt);
+    const real e_diss = exp(-(real)prm[P_DDISS] * dt);
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const size_t i = idx3(N, z, y, x);
         if (fm[i] == 0) {
-            out[i] = 0.0f; out[n3 + i] = 0.0f; out[2 * n3 + i] = 0.0f; out[3 * n3 + i] = 0.0f;
-            out[4 * n3 + i] = amb; out[5 * n3 + i] = 0.0f;
+            out[i] = RL(0.0); out[n3 + i] = RL(0.0); out[2 * n3 + i] = RL(0.0); out[3 * n3 + i] = RL(0.0);
+            out[4 * n3 + i] = amb; out[5 * n3 + i] = RL(0.0);
             continue;
         }
-        const float px = ((float)x + 0.5f) * dx, py = ((float)y + 0.5f) * dx, pz = ((float)z + 0.5f) * dx;
-        float d = adv[i] * ratio;
-        float vx = adv[n3 + i], vy = adv[2 * n3 + i], vz = adv[3 * n3 + i];
-        float T = adv[4 * n3 + i], Fu = adv[5 * n3 + i];
+        const real px = ((real)x + RL(0.5)) * dx, py = ((real)y + RL(0.5)) * dx, pz = ((real)z + RL(0.5)) * dx;
+        real d = adv[i] * ratio;
+        real vx = adv[n3 + i], vy = adv[2 * n3 + i], vz = adv[3 * n3 + i];
+        real T = adv[4 * n3 + i], Fu = adv[5 * n3 + i];
         for (int e = 0; e < n_em; ++e) {
             const float* b = prm + P_EM0 + e * EM_STRIDE;
-            const float rad = b[3];
-            const float ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
-            const float r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
-            if (r2 < 1.0f) {
-                const float w = (1.0f - r2) * (1.0f - r2);
-                float drate = b[4], frate = b[6];
-                const float te = b[5];
-                float nu = 1.0f;
+            const real rad = b[3];
+            const real ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
+            const real r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
+            if (r2 < RL(1.0)) {
+                const real w = (RL(1.0) - r2) * (RL(1.0) - r2);
+                real drate = b[4], frate = b[6];
+                const real te = b[5];
+                real nu = RL(1.0);
                 if (noisy) {
                     const uint32_t u = philox_u32(seed, (uint32_t)e, (uint32_t)i, stepno);
-                    nu = nlo + (nhi - nlo) * ((float)(u >> 8) * (1.0f / 16777216.0f));
+                    nu = nlo + (nhi - nlo) * ((real)(u >> 8) * (RL(1.0) / RL(16777216.0)));
                     drate = drate * nu;
                     frate = frate * nu;
                 }
                 d += drate * w * dt;
                 Fu += frate * w * dt;
-                if (te > 0.0f) {
-                    const float tex = noisy ? (te - amb) * nu : (te - amb);
-                    const float tt = amb + tex * w;
+                if (te > RL(0.0)) {
+                    const real tex = noisy ? (te - amb) * nu : (te - amb);
+                    const real tt = amb + tex * w;
                     T = T > tt ? T : tt;
                 }
-                const float evx = b[7], evy = b[8], evz = b[9];
-                if (evx * evx + evy * evy + evz * evz > 0.0f) {
+                const real evx = b[7], evy = b[8], evz = b[9];
+                if (evx * evx + evy * evy + evz * evz > RL(0.0)) {
                     vx = vx + (evx - vx) * w;
                     vy = vy + (evy - vy) * w;
                     vz = vz + (evz - vz) * w;
@@ -93,30 +93,30 @@ __global__ void __launch_bounds__(256) k_sources(int N, int zc, const float* __r
         }
         for (int m = 0; m < n_imp; ++m) {
             const float* b = prm + P_IMP0 + m * IMP_STRIDE;
-            const float rad = b[3];
-            const float ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
-            const float r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
-            if (r2 < 9.0f) {
-                const float gk = expf(-r2);
+            const real rad = b[3];
+            const real ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
+            const real r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
+            if (r2 < RL(9.0)) {
+                const real gk = exp(-r2);
                 vx = vx + gk * b[4];
                 vy = vy + gk * b[5];
                 vz = vz + gk * b[6];
             }
         }
-        const float ign = clampf_((T - prm[P_IGN]) / ramp, 0.0f, 1.0f);
-        const float burned = Fu * (1.0f - e_burn) * ign;
+        const real ign = clampf_((T - (real)prm[P_IGN]) / ramp, RL(0.0), RL(1.0));
+        const real burned = Fu * (RL(1.0) - e_burn) * ign;
         Fu -= burned;
-        const float room = clampf_((tmax - T) / (tmax - amb), 0.0f, 1.0f);
+        const real room = clampf_((tmax - T) / (tmax - amb), RL(0.0), RL(1.0));
         T += prm[P_BHEAT] * burned * room;
         d += prm[P_SMOKE] * burned;
         T = amb + (T - amb) * e_cool;
         d = d * e_diss;
-        out[i] = d > 0.0f ? d : 0.0f;
+        out[i] = d > RL(0.0) ? d : RL(0.0);
         out[n3 + i] = vx;
         out[2 * n3 + i] = vy;
         out[3 * n3 + i] = vz;
         out[4 * n3 + i] = T < tmax ? T : tmax;
-        out[5 * n3 + i] = Fu > 0.0f ? Fu : 0.0f;
+        out[5 * n3 + i] = Fu > RL(0.0) ? Fu : RL(0.0);
     }
 }
 
@@ -140,33 +140,33 @@ __device__ __forceinline__ Nb6 nb6(int N, int x, int y, int z, int i) {
 
 // diffuse_heat_fuel (kernels.py:268-286): channels 0..3 copied, T and F one explicit substep with
 // zero flux at walls and solids (a missing or solid neighbour contributes the cell's own value).
-__global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const float* __restrict__ src,
+__global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const real* __restrict__ src,
                                                  const uint8_t* __restrict__ fm, const float* __restrict__ prm,
-                                                 float nsub, float* __restrict__ dst) {
+                                                 real nsub, real* __restrict__ dst) {
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
-    const float dx = prm[P_DX];
-    float lam = prm[P_KAPPA] * prm[P_DT] / (nsub * dx * dx);
+    const real dx = (real)prm[P_DX];
+    real lam = (real)prm[P_KAPPA] * (real)prm[P_DT] / (nsub * dx * dx);
     lam = lam < LAM_CAP ? lam : LAM_CAP;
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const int i = (z * N + y) * N + x;
         const Nb6 n = nb6(N, x, y, z, i);
-        const float c0 = src[i], c1 = src[n3 + i], c2 = src[2 * n3 + i], c3 = src[3 * n3 + i];
+        const real c0 = src[i], c1 = src[n3 + i], c2 = src[2 * n3 + i], c3 = src[3 * n3 + i];
         const int m0 = fm[i], mxm = fm[n.xm], mxp = fm[n.xp], mym = fm[n.ym], myp = fm[n.yp], mzm = fm[n.zm],
                   mzp = fm[n.zp];
         const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
         const bool syp = !n.oyp || myp == 0, szm = !n.ozm || mzm == 0, szp = !n.ozp || mzp == 0;
-        float out[2];
+        real out[2];
 #pragma unroll
         for (int cc = 0; cc < 2; ++cc) {
-            const float* f = src + (4 + cc) * n3;
-            const float v = f[i], a0 = f[n.xm], a1 = f[n.xp], a2 = f[n.ym], a3 = f[n.yp], a4 = f[n.zm], a5 = f[n.zp];
-            const float s = (sxm ? v : a0) + (sxp ? v : a1) + (sym ? v : a2) + (syp ? v : a3) + (szm ? v : a4) +
+            const real* f = src + (4 + cc) * n3;
+            const real v = f[i], a0 = f[n.xm], a1 = f[n.xp], a2 = f[n.ym], a3 = f[n.yp], a4 = f[n.zm], a5 = f[n.zp];
+            const real s = (sxm ? v : a0) + (sxp ? v : a1) + (sym ? v : a2) + (syp ? v : a3) + (szm ? v : a4) +
                             (szp ? v : a5);
-            out[cc] = (m0 == 0 || lam <= 0.0f) ? v : v + lam * (s - 6.0f * v);
+            out[cc] = (m0 == 0 || lam <= RL(0.0)) ? v : v + lam * (s - RL(6.0) * v);
         }
         dst[i] = c0;
         dst[n3 + i] = c1;
@@ -178,16 +178,16 @@ __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const float* __r
 }
 
 // curl (kernels.py:297-310): walls and solids count as zero velocity.
-__global__ void __launch_bounds__(256) k_curl(int N, int zc, const float* __restrict__ data,
-                                              const uint8_t* __restrict__ fm, float inv2dx,
-                                              float* __restrict__ w) {
+__global__ void __launch_bounds__(256) k_curl(int N, int zc, const real* __restrict__ data,
+                                              const uint8_t* __restrict__ fm, real inv2dx,
+                                              real* __restrict__ w) {
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
-    const float* vx = data + n3;
-    const float* vy = data + 2 * n3;
-    const float* vz = data + 3 * n3;
+    const real* vx = data + n3;
+    const real* vy = data + 2 * n3;
+    const real* vz = data + 3 * n3;
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const int i = (z * N + y) * N + x;
@@ -195,15 +195,15 @@ __global__ void __launch_bounds__(256) k_curl(int N, int zc, const float* __rest
         const int mxm = fm[n.xm], mxp = fm[n.xp], mym = fm[n.ym], myp = fm[n.yp], mzm = fm[n.zm], mzp = fm[n.zp];
         const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
         const bool syp = !n.oyp || myp == 0, szm = !n.ozm || mzm == 0, szp = !n.ozp || mzp == 0;
-        const float vz_yp = vz[n.yp], vz_ym = vz[n.ym], vy_zp = vy[n.zp], vy_zm = vy[n.zm];
-        const float vx_zp = vx[n.zp], vx_zm = vx[n.zm], vz_xp = vz[n.xp], vz_xm = vz[n.xm];
-        const float vy_xp = vy[n.xp], vy_xm = vy[n.xm], vx_yp = vx[n.yp], vx_ym = vx[n.ym];
-        const float dwdy = (syp ? 0.0f : vz_yp) - (sym ? 0.0f : vz_ym);
-        const float dvdz = (szp ? 0.0f : vy_zp) - (szm ? 0.0f : vy_zm);
-        const float dudz = (szp ? 0.0f : vx_zp) - (szm ? 0.0f : vx_zm);
-        const float dwdx = (sxp ? 0.0f : vz_xp) - (sxm ? 0.0f : vz_xm);
-        const float dvdx = (sxp ? 0.0f : vy_xp) - (sxm ? 0.0f : vy_xm);
-        const float dudy = (syp ? 0.0f : vx_yp) - (sym ? 0.0f : vx_ym);
+        const real vz_yp = vz[n.yp], vz_ym = vz[n.ym], vy_zp = vy[n.zp], vy_zm = vy[n.zm];
+        const real vx_zp = vx[n.zp], vx_zm = vx[n.zm], vz_xp = vz[n.xp], vz_xm = vz[n.xm];
+        const real vy_xp = vy[n.xp], vy_xm = vy[n.xm], vx_yp = vx[n.yp], vx_ym = vx[n.ym];
+        const real dwdy = (syp ? RL(0.0) : vz_yp) - (sym ? RL(0.0) : vz_ym);
+        const real dvdz = (szp ? RL(0.0) : vy_zp) - (szm ? RL(0.0) : vy_zm);
+        const real dudz = (szp ? RL(0.0) : vx_zp) - (szm ? RL(0.0) : vx_zm);
+        const real dwdx = (sxp ? RL(0.0) : vz_xp) - (sxm ? RL(0.0) : vz_xm);
+        const real dvdx = (sxp ? RL(0.0) : vy_xp) - (sxm ? RL(0.0) : vy_xm);
+        const real dudy = (syp ? RL(0.0) : vx_yp) - (sym ? RL(0.0) : vx_ym);
         w[i] = (dwdy - dvdz) * inv2dx;
         w[n3 + i] = (dudz - dwdx) * inv2dx;
         w[2 * n3 + i] = (dvdx - dudy) * inv2dx;
@@ -211,8 +211,8 @@ __global__ void __launch_bounds__(256) k_curl(int N, int zc, const float* __rest
 }
 
 // sep_x (kernels.py:392-406): out[0:3] = [1,2,1]_x (w a), out[3] = [1,2,1]_x w.
-__global__ void __launch_bounds__(256) k_sep_x(int N, int zc, const float* __restrict__ a,
-                                               const uint8_t* __restrict__ fm, float* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_sep_x(int N, int zc, const real* __restrict__ a,
+                                               const uint8_t* __restrict__ fm, real* __restrict__ out) {
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -222,25 +222,25 @@ __global__ void __launch_bounds__(256) k_sep_x(int N, int zc, const float* __res
         const int i = (z * N + y) * N + x;
         const int idx[3] = {x > 0 ? i - 1 : i, i, x + 1 < N ? i + 1 : i};
         const bool inb[3] = {x > 0, true, x + 1 < N};
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
+        real s0 = RL(0.0), s1 = RL(0.0), s2 = RL(0.0), ws = RL(0.0);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int j = idx[k];
             const int mk = fm[j];
-            const float a0 = a[j], a1 = a[n3 + j], a2 = a[2 * n3 + j];
+            const real a0 = a[j], a1 = a[n3 + j], a2 = a[2 * n3 + j];
             const bool fl = inb[k] && mk != 0;
-            const float q = k == 1 ? 2.0f : 1.0f;
-            s0 = s0 + (fl ? q * a0 : 0.0f);
-            s1 = s1 + (fl ? q * a1 : 0.0f);
-            s2 = s2 + (fl ? q * a2 : 0.0f);
-            ws = ws + (fl ? q : 0.0f);
+            const real q = k == 1 ? RL(2.0) : RL(1.0);
+            s0 = s0 + (fl ? q * a0 : RL(0.0));
+            s1 = s1 + (fl ? q * a1 : RL(0.0));
+            s2 = s2 + (fl ? q * a2 : RL(0.0));
+            ws = ws + (fl ? q : RL(0.0));
         }
         out[i] = s0; out[n3 + i] = s1; out[2 * n3 + i] = s2; out[3 * n3 + i] = ws;
     }
 }
 
 // sep_y (kernels.py:409-419)
-__global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const float* __restrict__ a, float* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const real* __restrict__ a, real* __restrict__ out) {
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const float* __res
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const int i = (z * N + y) * N + x;
         const int im = y > 0 ? i - N : i, ip = y + 1 < N ? i + N : i;
-        float c0[4], cm[4], cp[4];
+        real c0[4], cm[4], cp[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             c0[c] = a[c * n3 + i];
@@ -258,9 +258,9 @@ __global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const float* __res
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            float v = 2.0f * c0[c];
-            v = v + (y > 0 ? cm[c] : 0.0f);
-            v = v + (y + 1 < N ? cp[c] : 0.0f);
+            real v = RL(2.0) * c0[c];
+            v = v + (y > 0 ? cm[c] : RL(0.0));
+            v = v + (y + 1 < N ? cp[c] : RL(0.0));
             out[c * n3 + i] = v;
         }
     }
@@ -268,39 +268,39 @@ __global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const float* __res
 
 // sep_z_ratio + sep_z_store / sep_z_apply (kernels.py:422-457).  mode 0: out (3ch) = ratio;
 // mode 1: fluid cells of the state get velocity += dt * ratio.
-__global__ void __launch_bounds__(256) k_sep_z(int N, int zc, const float* __restrict__ a,
+__global__ void __launch_bounds__(256) k_sep_z(int N, int zc, const real* __restrict__ a,
                                                const uint8_t* __restrict__ fm, const float* __restrict__ prm,
-                                               int mode, float* __restrict__ out) {
+                                               int mode, real* __restrict__ out) {
     const int n3 = N * N * N;
     const int sz = N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
-    const float dt = prm[P_DT];
+    const real dt = (real)prm[P_DT];
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const int i = (z * N + y) * N + x;
         const int idx[3] = {z > 0 ? i - sz : i, i, z + 1 < N ? i + sz : i};
         const bool inb[3] = {z > 0, true, z + 1 < N};
-        float v[3][4];
+        real v[3][4];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
             for (int c = 0; c < 4; ++c) v[k][c] = a[c * n3 + idx[k]];
         const int m0 = mode == 1 ? (int)fm[i] : 1;
-        float o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
+        real o1 = RL(0.0), o2 = RL(0.0), o3 = RL(0.0);
         if (mode == 1) { o1 = out[n3 + i]; o2 = out[2 * n3 + i]; o3 = out[3 * n3 + i]; }
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
+        real s0 = RL(0.0), s1 = RL(0.0), s2 = RL(0.0), ws = RL(0.0);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float q = k == 1 ? 2.0f : 1.0f;
-            s0 = s0 + (inb[k] ? q * v[k][0] : 0.0f);
-            s1 = s1 + (inb[k] ? q * v[k][1] : 0.0f);
-            s2 = s2 + (inb[k] ? q * v[k][2] : 0.0f);
-            ws = ws + (inb[k] ? q * v[k][3] : 0.0f);
+            const real q = k == 1 ? RL(2.0) : RL(1.0);
+            s0 = s0 + (inb[k] ? q * v[k][0] : RL(0.0));
+            s1 = s1 + (inb[k] ? q * v[k][1] : RL(0.0));
+            s2 = s2 + (inb[k] ? q * v[k][2] : RL(0.0));
+            ws = ws + (inb[k] ? q * v[k][3] : RL(0.0));
         }
-        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
-        if (ws > 0.0f) { r0 = s0 / ws; r1 = s1 / ws; r2 = s2 / ws; }
+        real r0 = RL(0.0), r1 = RL(0.0), r2 = RL(0.0);
+        if (ws > RL(0.0)) { r0 = s0 / ws; r1 = s1 / ws; r2 = s2 / ws; }
         if (mode == 0) {
             out[i] = r0; out[n3 + i] = r1; out[2 * n3 + i] = r2;
         } else if (m0 != 0) {
@@ -311,53 +311,54 @@ __global__ void __launch_bounds__(256) k_sep_z(int N, int zc, const float* __res
     }
 }
 
-__device__ __forceinline__ float wmag(const float* __restrict__ o, size_t n3, size_t i) {
-    const float a = o[i], b = o[n3 + i], c = o[2 * n3 + i];
-    return sqrtf(a * a + b * b + c * c);
+__device__ __forceinline__ real wmag(const real* __restrict__ o, size_t n3, size_t i) {
+    const real a = o[i], b = o[n3 + i], c = o[2 * n3 + i];
+    return sqrt(a * a + b * b + c * c);
 }
 
 // body_forces (kernels.py:346-387): buoyancy + vorticity confinement + wind marker force.
-__global__ void __launch_bounds__(256) k_body_forces(int N, int zc, const float* __restrict__ f,
-                                                     const uint8_t* __restrict__ fm, const float* __restrict__ om,
-                                                     const float* __restrict__ prm, float* __restrict__ force) {
+__global__ void __launch_bounds__(256) k_body_forces(int N, int zc, const real* __restrict__ f,
+                                                     const uint8_t* __restrict__ fm, const real* __restrict__ om,
+                                                     const float* __restrict__ prm, real* __restrict__ force) {
     const size_t n3 = (size_t)N * N * N;
     const size_t sz = (size_t)N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
-    const float amb = prm[P_AMB], alpha = prm[P_ALPHA], beta = prm[P_BETA], eps = prm[P_EPS];
-    const float cfloor = prm[P_CFLOOR], dxp = prm[P_DX];
-    const float wdx = prm[P_WIND_X], wdy = prm[P_WIND_Y], wdz = prm[P_WIND_Z], wmark = prm[P_WMARK];
-    const bool wind = wdx * wdx + wdy * wdy + wdz * wdz > 0.0f;
+    const real amb = (real)prm[P_AMB], alpha = (real)prm[P_ALPHA], beta = (real)prm[P_BETA], eps = (real)prm[P_EPS];
+    const real cfloor = (real)prm[P_CFLOOR], dxp = (real)prm[P_DX];
+    const real wdx = (real)prm[P_WIND_X], wdy = (real)prm[P_WIND_Y], wdz = (real)prm[P_WIND_Z],
+               wmark = (real)prm[P_WMARK];
+    const bool wind = wdx * wdx + wdy * wdy + wdz * wdz > RL(0.0);
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const size_t i = idx3(N, z, y, x);
         if (fm[i] == 0) {
-            force[i] = 0.0f; force[n3 + i] = 0.0f; force[2 * n3 + i] = 0.0f;
+            force[i] = RL(0.0); force[n3 + i] = RL(0.0); force[2 * n3 + i] = RL(0.0);
             continue;
         }
-        const float d = f[i], T = f[4 * n3 + i];
-        float fx = 0.0f, fy = -alpha * d + beta * (T - amb), fz = 0.0f;
-        if (eps != 0.0f) {
+        const real d = f[i], T = f[4 * n3 + i];
+        real fx = RL(0.0), fy = -alpha * d + beta * (T - amb), fz = RL(0.0);
+        if (eps != RL(0.0)) {
             const int xr = min(x + 1, N - 1), xl = max(x - 1, 0);
             const int yt = min(y + 1, N - 1), yb = max(y - 1, 0);
             const int zf = min(z + 1, N - 1), zb = max(z - 1, 0);
             const size_t rowb = (size_t)z * sz + (size_t)y * N;
-            const float gx = (wmag(om, n3, rowb + xr) - wmag(om, n3, rowb + xl)) / (float)max(xr - xl, 1);
-            const float gy = (wmag(om, n3, (size_t)z * sz + (size_t)yt * N + x) -
-                              wmag(om, n3, (size_t)z * sz + (size_t)yb * N + x)) / (float)max(yt - yb, 1);
-            const float gz = (wmag(om, n3, (size_t)zf * sz + (size_t)y * N + x) -
-                              wmag(om, n3, (size_t)zb * sz + (size_t)y * N + x)) / (float)max(zf - zb, 1);
-            const float wx = om[i], wy = om[n3 + i], wz = om[2 * n3 + i];
-            const float fl = cfloor * sqrtf(wx * wx + wy * wy + wz * wz);
-            const float mag = sqrtf(gx * gx + gy * gy + gz * gz + fl * fl) + 1.0e-20f;
-            const float k = eps * dxp / mag;
+            const real gx = (wmag(om, n3, rowb + xr) - wmag(om, n3, rowb + xl)) / (real)max(xr - xl, 1);
+            const real gy = (wmag(om, n3, (size_t)z * sz + (size_t)yt * N + x) -
+                              wmag(om, n3, (size_t)z * sz + (size_t)yb * N + x)) / (real)max(yt - yb, 1);
+            const real gz = (wmag(om, n3, (size_t)zf * sz + (size_t)y * N + x) -
+                              wmag(om, n3, (size_t)zb * sz + (size_t)y * N + x)) / (real)max(zf - zb, 1);
+            const real wx = om[i], wy = om[n3 + i], wz = om[2 * n3 + i];
+            const real fl = cfloor * sqrt(wx * wx + wy * wy + wz * wz);
+            const real mag = sqrt(gx * gx + gy * gy + gz * gz + fl * fl) + RL(1.0e-20);
+            const real k = eps * dxp / mag;
             fx += k * (gy * wz - gz * wy);
             fy += k * (gz * wx - gx * wz);
             fz += k * (gx * wy - gy * wx);
         }
         if (wind) {
-            const float mk = clampf_(d + f[5 * n3 + i] + (T - amb) / wmark, 0.0f, 1.0f);
+            const real mk = clampf_(d + f[5 * n3 + i] + (T - amb) / wmark, RL(0.0), RL(1.0));
             fx += wdx * mk;
             fy += wdy * mk;
             fz += wdz * mk;
@@ -366,30 +367,30 @@ __global__ void __launch_bounds__(256) k_body_forces(int N, int zc, const float*
     }
 }
 
-void launch_sources_diffuse(const Geom& g, const float* adv, const uint8_t* fm, const float* prm,
-                            const DevState* st, float* hat, float* data, hipStream_t s) {
+void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, const float* prm,
+                            const DevState* st, real* hat, real* data, hipStream_t s) {
     // n_diff = max(1, ceil(N / 256)) substeps, ping-pong ending in data (solver.py:107-109,436-442)
     const int nd = g.N > 256 ? (g.N + 255) / 256 : 1;
-    float* b0 = nd % 2 == 0 ? data : hat;
-    float* b1 = nd % 2 == 0 ? hat : data;
+    real* b0 = nd % 2 == 0 ? data : hat;
+    real* b1 = nd % 2 == 0 ? hat : data;
     hipLaunchKernelGGL(k_sources, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, st, b0);
     for (int k = 0; k < nd; ++k) {
-        float* a = k % 2 == 0 ? b0 : b1;
-        float* b = k % 2 == 0 ? b1 : b0;
-        hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, a, fm, prm, (float)nd, b);
+        real* a = k % 2 == 0 ? b0 : b1;
+        real* b = k % 2 == 0 ? b1 : b0;
+        hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, a, fm, prm, (real)nd, b);
     }
 }
 
-void launch_sources_pass(int pass, const Geom& g, const float* src, const uint8_t* fm, const float* prm,
-                         const DevState* st, float* dst, hipStream_t s) {
+void launch_sources_pass(int pass, const Geom& g, const real* src, const uint8_t* fm, const float* prm,
+                         const DevState* st, real* dst, hipStream_t s) {
     if (pass == 0)
         hipLaunchKernelGGL(k_sources, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, st, dst);
     else
-        hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, 1.0f, dst);
+        hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, RL(1.0), dst);
 }
 
-void launch_vort_forces(const Geom& g, float* data, const uint8_t* fm, const float* prm, float inv2dx,
-                        float* omega, float* omega_s, float* force, float* tmp_a, float* tmp_b, hipStream_t s) {
+void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx,
+                        real* omega, real* omega_s, real* force, real* tmp_a, real* tmp_b, hipStream_t s) {
     hipLaunchKernelGGL(k_curl, g.grid, g.block, 0, s, g.N, g.zc, data, fm, inv2dx, omega);
     hipLaunchKernelGGL(k_sep_x, g.grid, g.block, 0, s, g.N, g.zc, omega, fm, tmp_a);
     hipLaunchKernelGGL(k_sep_y, g.grid, g.block, 0, s, g.N, g.zc, tmp_a, tmp_b);

@@ -22,15 +22,15 @@ namespace vfx {
 // ---------------------------------------------------------------- setup (solid changes only)
 // k_solid_changed (pressure.py:146-153) against the cached face mask: the operators depend on
 // solid only through (solid > 0.5), which the mask's fluid bit records.
-__global__ void __launch_bounds__(256) k_solid_check(size_t n3, const float* __restrict__ solid,
+__global__ void __launch_bounds__(256) k_solid_check(size_t n3, const real* __restrict__ solid,
                                                      const uint8_t* __restrict__ fm, DevState* __restrict__ st) {
     bool diff = false;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x)
-        diff |= (solid[i] > 0.5f) != (fm[i] == 0);
+        diff |= (solid[i] > RL(0.5)) != (fm[i] == 0);
     if (__any(diff) && (threadIdx.x & 63) == 0) st->changed = 1;
 }
 
-__global__ void __launch_bounds__(256) k_mask(int N, const float* __restrict__ solid, uint8_t* __restrict__ fm,
+__global__ void __launch_bounds__(256) k_mask(int N, const real* __restrict__ solid, uint8_t* __restrict__ fm,
                                               const DevState* __restrict__ st) {
     if (!st->changed) return;
     const size_t n3 = (size_t)N * N * N;
@@ -41,9 +41,9 @@ __global__ void __launch_bounds__(256) k_mask(int N, const float* __restrict__ s
 }
 
 // tv / gt / axcell (pressure.py:179-217)
-__device__ __forceinline__ float tv(int c, int BK, int BJ, int BI, int z, int y, int x) {
+__device__ __forceinline__ real tv(int c, int BK, int BJ, int BI, int z, int y, int x) {
     if ((z >> 1) == BK && (y >> 1) == BJ && (x >> 1) == BI) return hsign(c, z & 1, y & 1, x & 1);
-    return 0.0f;
+    return RL(0.0);
 }
 
 __device__ __forceinline__ int mask_at(const uint8_t* __restrict__ fm, int N, int z, int y, int x) {
@@ -53,12 +53,12 @@ __device__ __forceinline__ int mask_at(const uint8_t* __restrict__ fm, int N, in
 
 // ALL: every cell fluid with every face open (the neighbourhood of a standard cell)
 template <bool ALL>
-__device__ __forceinline__ float gt(const uint8_t* __restrict__ fm, int N, int c, int BK, int BJ, int BI, int z,
+__device__ __forceinline__ real gt(const uint8_t* __restrict__ fm, int N, int c, int BK, int BJ, int BI, int z,
                                     int y, int x, int ax) {
     const int m = ALL ? (FLUID | 63) : mask_at(fm, N, z, y, x);
     const int ex = ax == 0, ey = ax == 1, ez = ax == 2;
-    const float t0 = tv(c, BK, BJ, BI, z, y, x);
-    float g = 0.0f;
+    const real t0 = tv(c, BK, BJ, BI, z, y, x);
+    real g = RL(0.0);
     if ((m >> (2 * ax)) & 1) g += tv(c, BK, BJ, BI, z + ez, y + ey, x + ex) - t0;
     if ((m >> (2 * ax + 1)) & 1) g += t0 - tv(c, BK, BJ, BI, z - ez, y - ey, x - ex);
     return g;
@@ -73,22 +73,22 @@ __device__ __forceinline__ void axcell(int ax, int oz, int oy, int ox, int a, in
 // k_probe1 (pressure.py:220-246): exact component-diagonal of T K T^T for (block, c).  ALL
 // evaluates it for a block in an all-fluid neighbourhood (the standard-cell table).
 template <bool ALL>
-__device__ void probe_item(const uint8_t* __restrict__ fm, int N, int m, int c, int K, int J, int I, float& dsum,
-                           float* off3) {
+__device__ void probe_item(const uint8_t* __restrict__ fm, int N, int m, int c, int K, int J, int I, real& dsum,
+                           real* off3) {
     const int oz = 2 * K, oy = 2 * J, ox = 2 * I;
-    dsum = 0.0f;
+    dsum = RL(0.0);
     for (int ax = 0; ax < 3; ++ax)
         for (int b = 0; b < 2; ++b)
             for (int cc = 0; cc < 2; ++cc)
                 for (int a = 0; a < 4; ++a) {
                     int z, y, x;
                     axcell(ax, oz, oy, ox, a - 1, b, cc, z, y, x);
-                    const float v = gt<ALL>(fm, N, c, K, J, I, z, y, x, ax);
+                    const real v = gt<ALL>(fm, N, c, K, J, I, z, y, x, ax);
                     dsum += v * v;
                 }
     for (int ax = 0; ax < 3; ++ax) {
         const int nK = K + (ax == 2), nJ = J + (ax == 1), nI = I + (ax == 0);
-        float s = 0.0f;
+        real s = RL(0.0);
         if (ALL || (nK < m && nJ < m && nI < m)) {
             for (int b = 0; b < 2; ++b)
                 for (int cc = 0; cc < 2; ++cc)
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_probe(int N, const uint8_t* __restrict_
         const int c = (int)(t & 7);
         const size_t cell = t >> 3;
         const int I = (int)(cell % m), J = (int)((cell / m) % m), K = (int)(cell / ((size_t)m * m));
-        float d, o3[3];
+        real d, o3[3];
         probe_item<false>(fm, N, m, c, K, J, I, d, o3);
         L.diag[cell * 8 + c] = d;
         for (int ax = 0; ax < 3; ++ax) L.off[cell * 24 + ax * 8 + c] = o3[ax];
@@ -128,10 +128,10 @@ __global__ void __launch_bounds__(256) k_galerkin(Lvl F, Lvl C, const DevState* 
         const size_t cell = t >> 3;
         const int I = (int)(cell % m), J = (int)((cell / m) % m), K = (int)(cell / ((size_t)m * m));
         const int oz = 2 * K, oy = 2 * J, ox = 2 * I;
-        float d = 0.0f;
+        real d = RL(0.0);
         for (int k = 0; k < 8; ++k) d += F.diag[lcell(mf, oz + (k >> 2), oy + ((k >> 1) & 1), ox + (k & 1)) * 8 + c];
         for (int ax = 0; ax < 3; ++ax) {
-            float oi = 0.0f, oo = 0.0f;
+            real oi = RL(0.0), oo = RL(0.0);
             for (int b = 0; b < 2; ++b)
                 for (int cc = 0; cc < 2; ++cc) {
                     int z, y, x;
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_galerkin(Lvl F, Lvl C, const DevState* 
                     axcell(ax, oz, oy, ox, 1, b, cc, z, y, x);
                     oo += F.off[lcell(mf, z, y, x) * 24 + ax * 8 + c];
                 }
-            d += 2.0f * oi;
+            d += RL(2.0) * oi;
             C.off[cell * 24 + ax * 8 + c] = oo * GALERKIN_SCALE;
         }
         C.diag[cell * 8 + c] = d * GALERKIN_SCALE;
@@ -158,29 +158,27 @@ __global__ void __launch_bounds__(64) k_std_tab(CoarseLevels all, int N, const D
     if (!st->changed) return;
     const int c = (int)threadIdx.x;
     if (c >= 8) return;
-    float d, o3[3];
+    real d, o3[3];
     probe_item<true>(nullptr, N, all.lv[0].m, c, 1, 1, 1, d, o3);
     all.lv[0].tab[c] = d;
     for (int ax = 0; ax < 3; ++ax) all.lv[0].tab[8 + ax * 8 + c] = o3[ax];
     for (int l = 0; l + 1 < all.n; ++l) {
-        const float* ft = all.lv[l].tab;
-        float* ct = all.lv[l + 1].tab;
-        float dd = 0.0f;
+        const real* ft = all.lv[l].tab;
+        real* ct = all.lv[l + 1].tab;
+        real dd = RL(0.0);
         for (int k = 0; k < 8; ++k) dd += ft[c];
         for (int ax = 0; ax < 3; ++ax) {
-            float oi = 0.0f, oo = 0.0f;
+            real oi = RL(0.0), oo = RL(0.0);
             for (int b = 0; b < 4; ++b) {
                 oi += ft[8 + ax * 8 + c];
                 oo += ft[8 + ax * 8 + c];
             }
-            dd += 2.0f * oi;
+            dd += RL(2.0) * oi;
             ct[8 + ax * 8 + c] = oo * GALERKIN_SCALE;
         }
         ct[c] = dd * GALERKIN_SCALE;
     }
 }
-
-__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
 
 // sf = 1 where the cell has all 6 neighbours, its diag and links and its -ax neighbours' links
 // equal tab bit for bit, and every neighbour has diag > dmin
@@ -216,7 +214,7 @@ __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c,
     const int m = F.m;
     size_t co[8], nb[8][6];
     bool e[8][6];
-    float r[8], x0[8], xn[8][6];
+    real r[8], x0[8], xn[8][6];
     int f = 1;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -242,36 +240,36 @@ __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c,
         for (int j = 0; j < 6; ++j) xn[k][j] = F.x[nb[k][j] * 8 + c];
     }
     const Tab tb = STD ? load_tab(F, c) : Tab{};
-    float res[8];
+    real res[8];
     if (STD && __all(f != 0)) {
-        const float cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
+        const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
 #pragma unroll
         for (int k = 0; k < 8; ++k) res[k] = r[k] - tb.d * x0[k] - offsum_v(cf, xn[k], e[k]);
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            float cf[6];
+            real cf[6];
 #pragma unroll
             for (int j = 0; j < 6; ++j) cf[j] = F.off[((j & 1) ? nb[k][j] : co[k]) * 24 + (j >> 1) * 8 + c];
             res[k] = r[k] - F.diag[co[k] * 8 + c] * x0[k] - offsum_v(cf, xn[k], e[k]);
         }
     }
-    float acc = 0.0f;
+    real acc = RL(0.0);
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc += res[k];
-    acc = 0.125f * acc;
+    acc = RL(0.125) * acc;
     const size_t oc = lcell(C.m, K, J, I) * 8 + c;
     C.r[oc] = acc;
     if (do_red && ((I + J + K) & 1) == 0) {
-        const float d = C.diag[oc];
-        C.x[oc] = d <= C.dmin ? 0.0f : acc / d;
+        const real d = C.diag[oc];
+        C.x[oc] = d <= C.dmin ? RL(0.0) : acc / d;
     }
 }
 
 // constant prolongation (k_prolong :319-325)
 __device__ __forceinline__ void prolong_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I) {
     const size_t o = lcell(F.m, K, J, I) * 8 + c;
-    const float x = F.x[o], xc = C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
+    const real x = F.x[o], xc = C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
     if (F.diag[o] <= F.dmin) return;
     F.x[o] = x + xc;
 }
@@ -286,8 +284,8 @@ __device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, i
     const int nI[6] = {I + 1, I - 1, I, I, I, I}, nJ[6] = {J, J, J + 1, J - 1, J, J},
               nK[6] = {K, K, K, K, K + 1, K - 1};
     const int f = F.sf[o];
-    const float r = F.r[o * 8 + c];
-    float xn[6], xcn[6];
+    const real r = F.r[o * 8 + c];
+    real xn[6], xcn[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         // coordinates of the clamped neighbour (the cell itself where it is missing)
@@ -296,7 +294,7 @@ __device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, i
         xcn[k] = C.x[lcell(C.m, kK >> 1, kJ >> 1, kI >> 1) * 8 + c];
     }
     const Tab tb = load_tab(F, c);
-    float d, cf[6], xpv[6];
+    real d, cf[6], xpv[6];
     if (__all(f != 0)) {
         d = tb.d;
 #pragma unroll
@@ -313,8 +311,8 @@ __device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, i
             xpv[k] = F.diag[n.c[k] * 8 + c] <= F.dmin ? xn[k] : xn[k] + xcn[k];
         }
     }
-    const float a = offsum_v(cf, xpv, n.e);
-    F.x[o * 8 + c] = d <= F.dmin ? 0.0f : (r - a) / d;
+    const real a = offsum_v(cf, xpv, n.e);
+    F.x[o * 8 + c] = d <= F.dmin ? RL(0.0) : (r - a) / d;
 }
 
 // item of this thread in a 1D launch, blocks renumbered per XCD (neighbouring rows in one L2)
@@ -397,8 +395,8 @@ __global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevels cl, const De
     const Lvl& B = cl.lv[n - 1];
     if (B.m == 1) {
         if (threadIdx.x < 8) {
-            const float d = B.diag[threadIdx.x];
-            B.x[threadIdx.x] = d <= B.dmin ? 0.0f : B.r[threadIdx.x] / d;
+            const real d = B.diag[threadIdx.x];
+            B.x[threadIdx.x] = d <= B.dmin ? RL(0.0) : B.r[threadIdx.x] / d;
         }
         __syncthreads();
     } else {
@@ -436,7 +434,7 @@ void Pressure::init_levels(int N_) {
     for (;;) {
         Lvl L{};
         L.m = m;
-        L.dmin = (float)(1.0e-3 * 6.0 * std::pow(0.25, level - 1));
+        L.dmin = (real)(1.0e-3 * 6.0 * std::pow(0.25, level - 1));
         lv[nlev++] = L;
         if (m % 2 == 1 || m == 1) break;
         m /= 2;
@@ -460,7 +458,7 @@ size_t Pressure::level_floats() const {
     return t;
 }
 
-void Pressure::bind_levels(float* base) {
+void Pressure::bind_levels(real* base) {
     for (int l = 0; l < nlev; ++l) {
         const size_t m3 = (size_t)lv[l].m * lv[l].m * lv[l].m;
         lv[l].x = base; base += 8 * m3;
@@ -476,7 +474,7 @@ __global__ void k_set_changed(DevState* st) {
     if (threadIdx.x == 0) st->changed = 1;
 }
 
-void Pressure::setup(const float* solid, hipStream_t s) {
+void Pressure::setup(const real* solid, hipStream_t s) {
     const size_t n3 = (size_t)N * N * N;
     if (force_setup) {
         hipLaunchKernelGGL(k_set_changed, dim3(1), dim3(64), 0, s, st);

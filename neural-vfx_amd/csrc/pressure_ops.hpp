@@ -37,29 +37,29 @@ __device__ __forceinline__ Nb neighbours(int N, int i, int co, int st) {
     return n;
 }
 
-__device__ __forceinline__ float bitf(int m, int b) { return (float)((m >> b) & 1); }
+__device__ __forceinline__ real bitf(int m, int b) { return (real)((m >> b) & 1); }
 
 // g_ax (pressure.py:54-65) at cell i (coordinate co along ax)
-__device__ __forceinline__ float g_ax_xyz(const float* __restrict__ p, int m, int N, int i, int co, int ax) {
+__device__ __forceinline__ real g_ax_xyz(const real* __restrict__ p, int m, int N, int i, int co, int ax) {
     const Nb n = neighbours(N, i, co, axstride(N, ax));
-    const float c = p[i], p1 = p[n.p1], m1 = p[n.m1];
-    return (0.0f + bitf(m, 2 * ax) * (p1 - c)) + bitf(m, 2 * ax + 1) * (c - m1);
+    const real c = p[i], p1 = p[n.p1], m1 = p[n.m1];
+    return (RL(0.0) + bitf(m, 2 * ax) * (p1 - c)) + bitf(m, 2 * ax + 1) * (c - m1);
 }
 
 // kmul (pressure.py:68-80): (K p)_i at a fluid cell (x, y, z) with mask m = fm[i]
-__device__ __forceinline__ float kmul_xyz(const float* __restrict__ p, const uint8_t* __restrict__ fm, int N, int x,
+__device__ __forceinline__ real kmul_xyz(const real* __restrict__ p, const uint8_t* __restrict__ fm, int N, int x,
                                           int y, int z, int i, int m) {
-    float d = 0.0f;
+    real d = RL(0.0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const int co = ax == 0 ? x : (ax == 1 ? y : z);
         const Nb n = neighbours(N, i, co, axstride(N, ax));
         const int mp = fm[n.p1], mm = fm[n.m1];
-        const float c = p[i], p1 = p[n.p1], p2 = p[n.p2], m1 = p[n.m1], m2 = p[n.m2];
-        const float op = bitf(m, 2 * ax), om = bitf(m, 2 * ax + 1);
-        const float gi = (0.0f + op * (p1 - c)) + om * (c - m1);
-        const float gp = (0.0f + bitf(mp, 2 * ax) * (p2 - p1)) + bitf(mp, 2 * ax + 1) * (p1 - c);
-        const float gm = (0.0f + bitf(mm, 2 * ax) * (c - m1)) + bitf(mm, 2 * ax + 1) * (m1 - m2);
+        const real c = p[i], p1 = p[n.p1], p2 = p[n.p2], m1 = p[n.m1], m2 = p[n.m2];
+        const real op = bitf(m, 2 * ax), om = bitf(m, 2 * ax + 1);
+        const real gi = (RL(0.0) + op * (p1 - c)) + om * (c - m1);
+        const real gp = (RL(0.0) + bitf(mp, 2 * ax) * (p2 - p1)) + bitf(mp, 2 * ax + 1) * (p1 - c);
+        const real gm = (RL(0.0) + bitf(mm, 2 * ax) * (c - m1)) + bitf(mm, 2 * ax + 1) * (m1 - m2);
         d = d + op * (gi + gp);
         d = d - om * (gi + gm);
     }
@@ -67,41 +67,41 @@ __device__ __forceinline__ float kmul_xyz(const float* __restrict__ p, const uin
 }
 
 // kdiag (pressure.py:83-91): sum over axes of (oL - oR)^2 + oL + oR, which is 2 for an axis with
-// any open face and 0 otherwise (small integers, exact in float)
-__device__ __forceinline__ float kdiag(int m) { return (float)(2 * __popc((m | (m >> 1)) & 0x15)); }
+// any open face and 0 otherwise (small integers, exact in real)
+__device__ __forceinline__ real kdiag(int m) { return (real)(2 * __popc((m | (m >> 1)) & 0x15)); }
 
 // div2 (pressure.py:94-106) of the velocity channels vel[0..2] at cell (x, y, z), mask m
-__device__ __forceinline__ float div2_xyz(const float* __restrict__ vel, int n3, int m, int N, int x, int y, int z,
+__device__ __forceinline__ real div2_xyz(const real* __restrict__ vel, int n3, int m, int N, int x, int y, int z,
                                           int i) {
-    float d = 0.0f;
+    real d = RL(0.0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const float* va = vel + ax * n3;
+        const real* va = vel + ax * n3;
         const int co = ax == 0 ? x : (ax == 1 ? y : z);
         const Nb n = neighbours(N, i, co, axstride(N, ax));
-        const float u = va[i], up = va[n.p1], um = va[n.m1];
+        const real u = va[i], up = va[n.p1], um = va[n.m1];
         d = d + bitf(m, 2 * ax) * (u + up);
         d = d - bitf(m, 2 * ax + 1) * (u + um);
     }
     return d;
 }
 
-__device__ __forceinline__ bool fluid_at(const float* __restrict__ solid, int N, int z, int y, int x) {
-    return x >= 0 && y >= 0 && z >= 0 && x < N && y < N && z < N && solid[((z * N) + y) * N + x] <= 0.5f;
+__device__ __forceinline__ bool fluid_at(const real* __restrict__ solid, int N, int z, int y, int x) {
+    return x >= 0 && y >= 0 && z >= 0 && x < N && y < N && z < N && solid[((z * N) + y) * N + x] <= RL(0.5);
 }
 
 // face mask of one cell from `solid` (k_mask pressure.py:115-143) plus the INTERIOR bit
-__device__ __forceinline__ int mask_from_solid(const float* __restrict__ solid, int N, int z, int y, int x) {
+__device__ __forceinline__ int mask_from_solid(const real* __restrict__ solid, int N, int z, int y, int x) {
     const int i = (z * N + y) * N + x;
-    if (solid[i] > 0.5f) return 0;
+    if (solid[i] > RL(0.5)) return 0;
     const int sz = N * N;
     int m = FLUID;
-    if (x + 1 < N && solid[i + 1] <= 0.5f) m |= 1;
-    if (x > 0 && solid[i - 1] <= 0.5f) m |= 2;
-    if (y + 1 < N && solid[i + N] <= 0.5f) m |= 4;
-    if (y > 0 && solid[i - N] <= 0.5f) m |= 8;
-    if (z + 1 < N && solid[i + sz] <= 0.5f) m |= 16;
-    if (z > 0 && solid[i - sz] <= 0.5f) m |= 32;
+    if (x + 1 < N && solid[i + 1] <= RL(0.5)) m |= 1;
+    if (x > 0 && solid[i - 1] <= RL(0.5)) m |= 2;
+    if (y + 1 < N && solid[i + N] <= RL(0.5)) m |= 4;
+    if (y > 0 && solid[i - N] <= RL(0.5)) m |= 8;
+    if (z + 1 < N && solid[i + sz] <= RL(0.5)) m |= 16;
+    if (z > 0 && solid[i - sz] <= RL(0.5)) m |= 32;
     if (m == (FLUID | 63) && fluid_at(solid, N, z, y, x + 2) && fluid_at(solid, N, z, y, x - 2) &&
         fluid_at(solid, N, z, y + 2, x) && fluid_at(solid, N, z, y - 2, x) && fluid_at(solid, N, z + 2, y, x) &&
         fluid_at(solid, N, z - 2, y, x))
@@ -110,7 +110,7 @@ __device__ __forceinline__ int mask_from_solid(const float* __restrict__ solid, 
 }
 
 // 2x2x2 Hadamard sign of component c at in-block offset (dz, dy, dx) (hsign, pressure.py:179-184)
-__device__ __forceinline__ float hsign(int c, int dz, int dy, int dx) {
+__device__ __forceinline__ real hsign(int c, int dz, int dy, int dx) {
     const int e = (c & 1) * dx + ((c >> 1) & 1) * dy + ((c >> 2) & 1) * dz;
     return (e & 1) ? -HS : HS;
 }
@@ -122,29 +122,29 @@ __device__ __forceinline__ bool cg_inactive(const DevState* st, int k) { return 
 // ---------------------------------------------------------------- LDS tiles of the fine grid
 // A workgroup owns TX x TY x TZ fine cells (whole 2x2x2 blocks) and stages them with a 2-cell
 // halo, the reach of K = d d^T.  Cells outside the box are staged with mask 0 and value 0.
-constexpr int TX = 32, TY = 8, TZ = 8;
+constexpr int TX = 32, TY = 8, TZ = VFX_F64 ? 4 : 8;   // FP64 tiles: half as deep (LDS)
 constexpr int HX = TX + 4, HY = TY + 4, HZ = TZ + 4;
 constexpr int HN = HX * HY * HZ;
-constexpr int TILE_THREADS = 256;
+constexpr int TILE_THREADS = (TX / 2) * (TY / 2) * (TZ / 2);   // one thread per own 2x2x2 block
 constexpr int BTX = TX / 2, BTY = TY / 2, BTZ = TZ / 2;     // blocks per tile
 constexpr int HBX = BTX + 2, HBY = BTY + 2, HBZ = BTZ + 2;  // blocks with a 1-block halo
 constexpr int HBN = HBX * HBY * HBZ;
 
 // kmul (pressure.py:68-80) on an LDS tile: open-face tests become 0/1 multipliers (x * 1 == x,
 // d + (+-0) == d), so the result is bit-identical to the branching form for finite values.
-__device__ __forceinline__ float kmul_tile(const float* P, const uint8_t* M, int h) {
-    float d = 0.0f;
+__device__ __forceinline__ real kmul_tile(const real* P, const uint8_t* M, int h) {
+    real d = RL(0.0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const int st = ax == 0 ? 1 : (ax == 1 ? HX : HX * HY);
         const int m0 = M[h], mp = M[h + st], mm = M[h - st];
-        const float op = (float)((m0 >> (2 * ax)) & 1), om = (float)((m0 >> (2 * ax + 1)) & 1);
-        const float opp = (float)((mp >> (2 * ax)) & 1), omp = (float)((mp >> (2 * ax + 1)) & 1);
-        const float opm = (float)((mm >> (2 * ax)) & 1), omm = (float)((mm >> (2 * ax + 1)) & 1);
-        const float m2 = P[h - 2 * st], m1 = P[h - st], c = P[h], p1 = P[h + st], p2 = P[h + 2 * st];
-        const float gi = (0.0f + op * (p1 - c)) + om * (c - m1);
-        const float gp = (0.0f + opp * (p2 - p1)) + omp * (p1 - c);
-        const float gm = (0.0f + opm * (c - m1)) + omm * (m1 - m2);
+        const real op = (real)((m0 >> (2 * ax)) & 1), om = (real)((m0 >> (2 * ax + 1)) & 1);
+        const real opp = (real)((mp >> (2 * ax)) & 1), omp = (real)((mp >> (2 * ax + 1)) & 1);
+        const real opm = (real)((mm >> (2 * ax)) & 1), omm = (real)((mm >> (2 * ax + 1)) & 1);
+        const real m2 = P[h - 2 * st], m1 = P[h - st], c = P[h], p1 = P[h + st], p2 = P[h + 2 * st];
+        const real gi = (RL(0.0) + op * (p1 - c)) + om * (c - m1);
+        const real gp = (RL(0.0) + opp * (p2 - p1)) + omp * (p1 - c);
+        const real gm = (RL(0.0) + opm * (c - m1)) + omm * (m1 - m2);
         d = d + op * (gi + gp);
         d = d - om * (gi + gm);
     }
@@ -153,15 +153,15 @@ __device__ __forceinline__ float kmul_tile(const float* P, const uint8_t* M, int
 
 // kmul_tile at an INTERIOR cell: every open-face multiplier is 1 (x * 1 == x), so the same
 // operations without the mask
-__device__ __forceinline__ float kmul_tile_interior(const float* P, int h) {
-    float d = 0.0f;
+__device__ __forceinline__ real kmul_tile_interior(const real* P, int h) {
+    real d = RL(0.0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const int st = ax == 0 ? 1 : (ax == 1 ? HX : HX * HY);
-        const float m2 = P[h - 2 * st], m1 = P[h - st], c = P[h], p1 = P[h + st], p2 = P[h + 2 * st];
-        const float gi = (0.0f + (p1 - c)) + (c - m1);
-        const float gp = (0.0f + (p2 - p1)) + (p1 - c);
-        const float gm = (0.0f + (c - m1)) + (m1 - m2);
+        const real m2 = P[h - 2 * st], m1 = P[h - st], c = P[h], p1 = P[h + st], p2 = P[h + 2 * st];
+        const real gi = (RL(0.0) + (p1 - c)) + (c - m1);
+        const real gp = (RL(0.0) + (p2 - p1)) + (p1 - c);
+        const real gm = (RL(0.0) + (c - m1)) + (m1 - m2);
         d = d + (gi + gp);
         d = d - (gi + gm);
     }
@@ -192,7 +192,7 @@ __device__ __forceinline__ LNb level_neighbours(int m, size_t o, int K, int J, i
 
 // coupling coefficient of link k (0..5) of component c: +ax links are stored at the cell, -ax
 // links at the -ax neighbour
-__device__ __forceinline__ float link_coef(const Lvl& L, const LNb& n, size_t o, int k, int c) {
+__device__ __forceinline__ real link_coef(const Lvl& L, const LNb& n, size_t o, int k, int c) {
     const int ax = k >> 1;
     return L.off[((k & 1) ? n.c[k] : o) * 24 + ax * 8 + c];
 }
@@ -204,7 +204,7 @@ __device__ __forceinline__ float link_coef(const Lvl& L, const LNb& n, size_t o,
 // coefficients come from tab (a 128-byte table) instead of the level arrays; the arithmetic is
 // unchanged, so the result is the same as the general path's.
 struct Tab {
-    float d, o[3];   // diag and the x, y, z link coefficients of component c
+    real d, o[3];   // diag and the x, y, z link coefficients of component c
 };
 
 __device__ __forceinline__ Tab load_tab(const Lvl& L, int c) {
@@ -217,30 +217,30 @@ __device__ __forceinline__ Tab load_tab(const Lvl& L, int c) {
 }
 
 // offsum (pressure.py:271-287) from the link coefficients cf and neighbour values xn
-__device__ __forceinline__ float offsum_v(const float* cf, const float* xn, const bool* e) {
-    float a = 0.0f;
+__device__ __forceinline__ real offsum_v(const real* cf, const real* xn, const bool* e) {
+    real a = RL(0.0);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) a = a + (e[k] ? cf[k] * xn[k] : 0.0f);
+    for (int k = 0; k < 6; ++k) a = a + (e[k] ? cf[k] * xn[k] : RL(0.0));
     return a;
 }
 
 // one GS update (k_smooth pressure.py:290-303); returns the new value.  STD: waves of standard
 // cells use the table (the single-workgroup coarse levels have few standard cells and skip it).
 template <bool STD>
-__device__ __forceinline__ float smooth_value(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
+__device__ __forceinline__ real smooth_value(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
     const size_t o = lcell(L.m, K, J, I);
-    const float r = L.r[o * 8 + c];
+    const real r = L.r[o * 8 + c];
     if (zero_init) {
-        const float d = L.diag[o * 8 + c];
-        return d <= L.dmin ? 0.0f : r / d;
+        const real d = L.diag[o * 8 + c];
+        return d <= L.dmin ? RL(0.0) : r / d;
     }
     const LNb n = level_neighbours(L.m, o, K, J, I);
     const int f = STD ? L.sf[o] : 0;
-    float xn[6];
+    real xn[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) xn[k] = L.x[n.c[k] * 8 + c];
     const Tab tb = STD ? load_tab(L, c) : Tab{};
-    float d, cf[6];
+    real d, cf[6];
     if (STD && __all(f != 0)) {
         d = tb.d;
 #pragma unroll
@@ -250,13 +250,13 @@ __device__ __forceinline__ float smooth_value(const Lvl& L, int c, int K, int J,
 #pragma unroll
         for (int k = 0; k < 6; ++k) cf[k] = link_coef(L, n, o, k, c);
     }
-    const float acc = r - offsum_v(cf, xn, n.e);
-    return d <= L.dmin ? 0.0f : acc / d;
+    const real acc = r - offsum_v(cf, xn, n.e);
+    return d <= L.dmin ? RL(0.0) : acc / d;
 }
 
 template <bool STD>
-__device__ __forceinline__ float smooth_item(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
-    const float v = smooth_value<STD>(L, c, K, J, I, zero_init);
+__device__ __forceinline__ real smooth_item(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
+    const real v = smooth_value<STD>(L, c, K, J, I, zero_init);
     L.x[lcell(L.m, K, J, I) * 8 + c] = v;
     return v;
 }

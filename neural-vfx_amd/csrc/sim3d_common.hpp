@@ -1,7 +1,7 @@
 // Shared definitions of the MI355X (gfx950) sim3d library: parameter-buffer slots, device
 // scalar slots, launch geometry, the collocated-grid helpers and block reductions.
 //
-// Layout of every field: (C, D, H, W) float32, index [c][z][y][x], x fastest (the reference's
+// Layout of every field: (C, D, H, W) `real`, index [c][z][y][x], x fastest (the reference's
 // FieldState3D, vfx/core/fields3d.py:1-10).  Masks are uint8 open-face bit sets.
 #pragma once
 
@@ -10,7 +10,39 @@
 
 #include <string>
 
+// Field precision of this build.  The sources are compiled twice: VFX_F64 = 0 is the product
+// path (FP32 fields and levels, FP64 reductions); VFX_F64 = 1 is an FP64 run of the same solver
+// (every field, level and constant in double) whose C entry points carry the suffix _f64 and
+// whose namespace is vfx_f64, so both builds link into one library.
+#ifndef VFX_F64
+#define VFX_F64 0
+#endif
+#if VFX_F64
+#define vfx vfx_f64
+#define VFX_FN(name) name##_f64
+#define RL(x) (x)
+#else
+#define VFX_FN(name) name
+#define RL(x) (x##f)
+#endif
+
 namespace vfx {
+
+#if VFX_F64
+using real = double;
+using real2 = double2;
+using real4 = double4;
+__host__ __device__ inline real2 make_real2(real a, real b) { return make_double2(a, b); }
+__host__ __device__ inline real4 make_real4(real a, real b, real c, real d) { return make_double4(a, b, c, d); }
+__device__ __forceinline__ bool same_bits(real a, real b) { return __double_as_longlong(a) == __double_as_longlong(b); }
+#else
+using real = float;
+using real2 = float2;
+using real4 = float4;
+__host__ __device__ inline real2 make_real2(real a, real b) { return make_float2(a, b); }
+__host__ __device__ inline real4 make_real4(real a, real b, real c, real d) { return make_float4(a, b, c, d); }
+__device__ __forceinline__ bool same_bits(real a, real b) { return __float_as_uint(a) == __float_as_uint(b); }
+#endif
 
 // ---------------------------------------------------------------- parameter buffer (floats)
 // Slot numbers of the reference (vfx/sim3d/kernels.py:11-42) plus extension slots in the
@@ -28,7 +60,7 @@ enum : int {
 };
 
 // ---------------------------------------------------------------- device scalar state
-// doubles written by the finalize kernels (the reference keeps the same quantities as float
+// doubles written by the finalize kernels (the reference keeps the same quantities as real
 // slots, kernels.py:44-53); the CG inner products, alpha and beta are FP64 here.
 enum : int {
     S_MASS_RATIO = 0, S_MAXU = 1, S_RZ = 2, S_ALPHA = 3, S_BETA = 4, S_MAXR = 5, S_THR = 6,
@@ -44,10 +76,10 @@ struct DevState {
 };
 
 constexpr int FLUID = 64;          // face-mask fluid bit (pressure.py:32-33)
-constexpr float LAM_CAP = 0.16f;   // kernels.py:63
-constexpr float HS = 0.35355339059327373f;   // 1/sqrt(8), pressure.py:30
-constexpr float OMEGA_JAC = 0.8f;  // pressure.py:785
-constexpr float GALERKIN_SCALE = 1.0f / 16.0f;   // pressure.py:783
+constexpr real LAM_CAP = (real)0.16f;   // kernels.py:63 (the reference's float constant)
+constexpr real HS = (real)0.35355339059327373;   // 1/sqrt(8), pressure.py:30
+constexpr real OMEGA_JAC = (real)0.8f;  // pressure.py:785 (the reference's float constant)
+constexpr real GALERKIN_SCALE = (real)0.0625;   // 1/16, pressure.py:783
 
 // ---------------------------------------------------------------- launch geometry
 // Cell kernels use 256-thread blocks of BX x (256/BX) threads over (x, y) and walk ZC z-planes

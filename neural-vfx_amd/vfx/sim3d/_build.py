@@ -60,11 +60,14 @@ def build(verbose: bool = False) -> str:
     hipcc = _hipcc()
     objs = []
     cmds = []
+    # every source twice: the FP32 product path and the FP64 run of the same solver (VFX_F64=1:
+    # double fields, namespace vfx_f64, entry points suffixed _f64)
     for src in SOURCES:
-        obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
-        objs.append(obj)
-        cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-    with cf.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+        for f64 in (0, 1):
+            obj = os.path.join(BUILD_DIR, src.replace(".hip", "_f64.o" if f64 else ".o"))
+            objs.append(obj)
+            cmds.append([hipcc, *HIPCC_FLAGS, f"-DVFX_F64={f64}", "-c", os.path.join(CSRC, src), "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         outs = list(ex.map(_run, cmds))
     if verbose:
         for c, o in zip(cmds, outs):

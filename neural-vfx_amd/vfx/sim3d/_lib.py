@@ -43,6 +43,9 @@ SIGNATURES = {
     "vfx_time_kernel": [_P, _I, _I, _P, ctypes.POINTER(ctypes.c_float)],
     "vfx_last_error": [ctypes.c_char_p, _I],
 }
+# the FP64 run of the same solver: every entry point again with the suffix _f64 (double fields;
+# the argument kinds are the same, pointers are untyped here)
+SIGNATURES.update({name + "_f64": args for name, args in list(SIGNATURES.items())})
 
 _lib = None
 
@@ -68,17 +71,28 @@ def load() -> ctypes.CDLL:
     return lib
 
 
-def last_error() -> str:
+def last_error(f64: bool = False) -> str:
     buf = ctypes.create_string_buffer(512)
-    load().vfx_last_error(buf, 512)
+    (load().vfx_last_error_f64 if f64 else load().vfx_last_error)(buf, 512)
     return buf.value.decode(errors="replace")
 
 
-def check(rc: int, what: str = "") -> None:
+class Api:
+    """The entry points of one precision under their FP32 names (vfx_step is vfx_step_f64 for
+    the FP64 build), so callers are written once."""
+
+    def __init__(self, lib: ctypes.CDLL, f64: bool):
+        self.f64 = f64
+        for name in SIGNATURES:
+            if not name.endswith("_f64"):
+                setattr(self, name, getattr(lib, name + ("_f64" if f64 else "")))
+
+
+def check(rc: int, what: str = "", f64: bool = False) -> None:
     """Map a status code to the reference's exception types (solver.py:311-315, :347-348)."""
     if rc == VFX_OK:
         return
-    msg = f"{what}: {last_error()}" if what else last_error()
+    msg = f"{what}: {last_error(f64)}" if what else last_error(f64)
     if rc == VFX_E_ARG:
         raise AssertionError(msg)
     if rc == VFX_E_VALUE:

@@ -86,9 +86,15 @@ def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
 class Solver3D:
     HOST_RING = 4
 
-    def __init__(self, res: int, config: SolverConfig3D | None = None, device="cuda", scalar_res: int | None = None):
+    def __init__(self, res: int, config: SolverConfig3D | None = None, device="cuda", scalar_res: int | None = None,
+                 dtype: torch.dtype = torch.float32):
+        """`dtype` torch.float64 selects the FP64 run of the same solver (double state, levels and
+        constants); the reference is FP32 only, so float32 is the reference-compatible default."""
         if scalar_res is not None:
             raise NotImplementedError("scalar_res (the reference's opt-in finer scalar grid) is not built")
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError(f"dtype must be torch.float32 or torch.float64, got {dtype}")
+        self.dtype = dtype
         self.res = int(res)
         self.config = config if config is not None else SolverConfig3D()
         self.scalar_res = None
@@ -98,24 +104,29 @@ class Solver3D:
             self.device = f"cuda:{torch.cuda.current_device()}"
         self.dx = DOMAIN_SIZE / self.res
         self._lib = _lib.load()
+        self._f64 = dtype == torch.float64
+        self._api = _lib.Api(self._lib, self._f64)
         self._ctx = ctypes.c_void_p()
         with torch.cuda.device(torch.device(self.device)):
-            _lib.check(self._lib.vfx_ctx_create(self.res, ctypes.byref(self._ctx)), "Solver3D")
+            self._chk(self._api.vfx_ctx_create(self.res, ctypes.byref(self._ctx)), "Solver3D")
         self._h = np.zeros(P_SIZE, np.float32)
         self._apply_options()
 
     def __del__(self):
         ctx = getattr(self, "_ctx", None)
-        if ctx is not None and ctx.value and getattr(self, "_lib", None) is not None:
-            self._lib.vfx_ctx_destroy(ctx)
+        if ctx is not None and ctx.value and getattr(self, "_api", None) is not None:
+            self._api.vfx_ctx_destroy(ctx)
             self._ctx = None
 
+    def _chk(self, rc: int, what: str = "") -> None:
+        _lib.check(rc, what, self._f64)
+
     def _apply_options(self):
-        _lib.check(self._lib.vfx_set_option(self._ctx, _lib.OPT_RESTARTS, float(self.config.pressure_restarts)))
+        self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_RESTARTS, float(self.config.pressure_restarts)))
 
     def set_option(self, key: int, value: float) -> None:
         """Library option (vfxsim3d.h VFX_OPT_*): true-residual report, restarts, coarse size."""
-        _lib.check(self._lib.vfx_set_option(self._ctx, int(key), float(value)))
+        self._chk(self._api.vfx_set_option(self._ctx, int(key), float(value)))
 
     @property
     def version(self) -> str:
@@ -124,8 +135,8 @@ class Solver3D:
     # ------------------------------------------------------------------ public API
     def reset(self, obstacles=()) -> FieldState3D:
         N = self.res
-        solid = rasterize_obstacles_3d(list(obstacles), N, device=self.device)
-        data = torch.zeros(NUM_CHANNELS_3D, N, N, N, device=self.device, dtype=torch.float32)
+        solid = rasterize_obstacles_3d(list(obstacles), N, device=self.device).to(self.dtype)
+        data = torch.zeros(NUM_CHANNELS_3D, N, N, N, device=self.device, dtype=self.dtype)
         data[4] = AMBIENT_TEMP
         meta = FieldMeta3D.for_res(N)
         meta.solver_version = self.version
@@ -136,7 +147,7 @@ class Solver3D:
         self._structure()
         self._apply_options()
         h = pack_params(self.config, self.res, forces, dt, state.meta.step, self._h)
-        _lib.check(self._lib.vfx_step(self._ctx, _ptr(state.data), _ptr(state.solid),
+        self._chk(self._api.vfx_step(self._ctx, _ptr(state.data), _ptr(state.solid),
                                       h.ctypes.data_as(ctypes.c_void_p), self._stream()), "step")
         state.meta.step += 1
         state.meta.dt = float(dt)
@@ -156,7 +167,7 @@ class Solver3D:
         ev[0].record(ts)
         hp = h.ctypes.data_as(ctypes.c_void_p)
         for k, stage in enumerate((_lib.STAGE_ADVECT, _lib.STAGE_FORCES, _lib.STAGE_PROJECT)):
-            _lib.check(self._lib.vfx_step_stage(self._ctx, stage, _ptr(state.data), _ptr(state.solid),
+            self._chk(self._api.vfx_step_stage(self._ctx, stage, _ptr(state.data), _ptr(state.solid),
                                                 hp if stage == _lib.STAGE_ADVECT else None, self._stream()),
                        "step_profiled")
             ev[k + 1].record(ts)
@@ -180,7 +191,7 @@ class Solver3D:
         self._structure()
         self._apply_options()
         h = pack_params(self.config, self.res, Forces3D(), 0.0, state.meta.step, self._h)
-        _lib.check(self._lib.vfx_project(self._ctx, _ptr(state.data), _ptr(state.solid),
+        self._chk(self._api.vfx_project(self._ctx, _ptr(state.data), _ptr(state.solid),
                                          h.ctypes.data_as(ctypes.c_void_p), self._stream()), "project")
         return state
 
@@ -188,8 +199,8 @@ class Solver3D:
         """(D,H,W) |div u| dx / (max|u| + 1e-8), fluid cells only (solver.py:274-287)."""
         self._check(state)
         N = self.res
-        out = torch.zeros(N, N, N, device=self.device, dtype=torch.float32)
-        _lib.check(self._lib.vfx_div_abs(self._ctx, _ptr(state.data), _ptr(state.solid), _ptr(out),
+        out = torch.zeros(N, N, N, device=self.device, dtype=self.dtype)
+        self._chk(self._api.vfx_div_abs(self._ctx, _ptr(state.data), _ptr(state.solid), _ptr(out),
                                          self._stream()), "divergence")
         fluid = state.solid < 0.5
         speed = state.velocity.norm(dim=0)
@@ -203,24 +214,24 @@ class Solver3D:
         N = self.res
         assert f.shape[-3:] == (N, N, N) and velocity.shape == (3, N, N, N)
         assert f.is_cuda and velocity.is_cuda
-        f = f.contiguous().float()
-        v = velocity.contiguous().float()
+        f = f.contiguous().to(self.dtype)
+        v = velocity.contiguous().to(self.dtype)
         out = torch.empty_like(f)
-        _lib.check(self._lib.vfx_advect(self._ctx, _ptr(f), int(f.shape[0]), _ptr(v), _ptr(out), float(dt),
+        self._chk(self._api.vfx_advect(self._ctx, _ptr(f), int(f.shape[0]), _ptr(v), _ptr(out), float(dt),
                                         self._stream()), "advect")
         return out[0] if squeeze else out
 
     def pressure_iters(self) -> int:
         """PCG iterations of the last projection (synchronizes)."""
         n = ctypes.c_int()
-        _lib.check(self._lib.vfx_pressure_iters(self._ctx, ctypes.byref(n)))
+        self._chk(self._api.vfx_pressure_iters(self._ctx, ctypes.byref(n)))
         return int(n.value)
 
     def pressure_stats(self) -> dict:
         """Diagnostics of the last projection (synchronizes): iterations, max|u*|, ||b||, the
         recurrence ||r||, max|r|, the true ||b - K phi|| (if enabled) and restarts."""
         v = (ctypes.c_double * 7)()
-        _lib.check(self._lib.vfx_pressure_stats(self._ctx, v, 7))
+        self._chk(self._api.vfx_pressure_stats(self._ctx, v, 7))
         keys = ("iters", "max_u", "b_norm", "r_norm", "max_r", "true_r_norm", "restarts")
         d = dict(zip(keys, list(v)))
         d["iters"] = int(d["iters"])
@@ -232,8 +243,8 @@ class Solver3D:
     # ------------------------------------------------------------------ internals
     def _check(self, state: FieldState3D):
         N = self.res
-        assert state.data.shape == (NUM_CHANNELS_3D, N, N, N) and state.data.dtype == torch.float32
-        assert state.solid.shape == (N, N, N) and state.solid.dtype == torch.float32
+        assert state.data.shape == (NUM_CHANNELS_3D, N, N, N) and state.data.dtype == self.dtype
+        assert state.solid.shape == (N, N, N) and state.solid.dtype == self.dtype
         assert state.data.is_cuda and state.data.is_contiguous() and state.solid.is_contiguous()
 
     def _structure(self) -> tuple:
