@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import dataclasses
 import json
 import math
 import os
@@ -126,6 +127,8 @@ def main():
     ap.add_argument("--profiled", type=int, default=5, help="extra step_profiled steps for the stage split")
     ap.add_argument("--roofline-kernel", default="", help="kernel for the roofline (default: largest step share)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mg", default="fp32", choices=("fp32", "fp16", "bf16"),
+                    help="multigrid level storage (the reference stores FP32)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: one 256 MiB device copy (the scale reference), "
                          "2 steps, then one launch of every timed kernel; prints no bench line")
@@ -142,7 +145,7 @@ def main():
     torch.cuda.set_device(dev)
     N = args.res
     dt = 1.0 / 60.0
-    cfg = fire_config(args.seed)
+    cfg = dataclasses.replace(fire_config(args.seed), mg_storage=args.mg)
     forces = fire_forces()
     s = Solver3D(N, cfg, device="cuda:0")
     s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
@@ -216,7 +219,7 @@ def main():
         cpu = cpu_baseline(st.data, st.solid, cfg, forces, dt, N, st.meta.step)
 
     out = {
-        "metric": f"{N}^3 fire step ms, plain MGPCG to rel-res 1e-5",
+        "metric": f"{N}^3 fire step ms, plain MGPCG to rel-res 1e-5" + ("" if args.mg == "fp32" else f" ({args.mg} levels)"),
         "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 3), "steps_per_s": round(1000.0 / ms, 2), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",

@@ -39,6 +39,8 @@ extern "C" {
 #define VFX_OPT_COARSE_M 3       /* multigrid levels with m <= this run in one workgroup */
 #define VFX_OPT_STD_STENCIL 4    /* 0: always read the level operators (no standard-cell table);
                                     results are identical either way, default 1 */
+#define VFX_OPT_MG_STORAGE 5     /* multigrid level storage: 0 FP32 (default), 1 FP16, 2 BF16; FP32
+                                    arithmetic either way, CG vectors FP32, dots and alpha/beta FP64 */
 
 typedef struct vfx_ctx vfx_ctx;
 
@@ -89,7 +91,7 @@ int vfx_get_advected(vfx_ctx* ctx, float* dst, double* mass_ratio, void* stream)
 /* Operator access for parity tests: rebuild masks and multigrid operators for `solid`, then
  * out = K p (vfx_kmul), z = M^-1 r (vfx_precond, the V(1,1) preconditioner), or the level-l
  * operators in component-fastest layout diag [m^3][8], off [m^3][24] (vfx_get_ops; returns
- * the level count in *nlev and the level size in *m). */
+ * the level count in *nlev and the level size in *m; FP32 level storage only). */
 int vfx_kmul(vfx_ctx* ctx, const float* solid, const float* p, float* out, void* stream);
 int vfx_precond(vfx_ctx* ctx, const float* solid, const float* r, float* z, void* stream);
 int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float* off, int* nlev, int* m,

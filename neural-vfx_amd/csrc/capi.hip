@@ -330,6 +330,22 @@ int VFX_FN(vfx_set_option)(vfx_ctx* c, int key, double value) {
                 if (c->P.lv[l].m <= cm) { c->P.S = l; break; }
             return VFX_OK;
         }
+        case VFX_OPT_MG_STORAGE: {
+            const int kind = (int)value;
+            if (kind != MG_REAL && kind != MG_FP16 && kind != MG_BF16) {
+                set_error("multigrid storage must be 0 (FP32), 1 (FP16) or 2 (BF16)");
+                return VFX_E_VALUE;
+            }
+            if (VFX_F64 && kind != MG_REAL) {
+                set_error("the FP64 build stores its multigrid levels in FP64");
+                return VFX_E_VALUE;
+            }
+            if (kind != c->P.mg_kind) {
+                c->P.mg_kind = kind;
+                c->P.force_setup = true;   // the operators are rebuilt in the new storage
+            }
+            return VFX_OK;
+        }
         case VFX_OPT_STD_STENCIL: {
             const bool on = value != 0.0;
             if (on != c->P.use_std) {
@@ -550,6 +566,10 @@ int VFX_FN(vfx_get_ops)(vfx_ctx* c, const real* solid, int level, real* diag, re
     if (!solid) return arg_error("null pointer");
     if (nlev) *nlev = c->P.nlev;
     if (level < 0 || level >= c->P.nlev) return arg_error("level out of range");
+    if (c->P.mg_kind != MG_REAL) {
+        set_error("operators are exported in FP32 level storage only");
+        return VFX_E_VALUE;
+    }
     const Lvl& L = c->P.lv[level];
     if (m) *m = L.m;
     hipStream_t s = S(stream);

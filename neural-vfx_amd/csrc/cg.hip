@@ -123,8 +123,12 @@ __device__ __forceinline__ real inv_sum(real acc, const real* hx, int dz, int dy
     return acc;
 }
 
-// k_rhs (:602-627): r = -d(u*) - K phi0 (phi = 0 without warm start); block max |u*| over fluid
-// and block sum of b^2 (b = -d(u*)) for the relative-residual test.
+// 2^-e with 2^e <= v < 2^(e+1): brings v into [1, 2) exactly (1 for v = 0 or non-finite)
+__device__ __forceinline__ double pow2_scale(double v) { return (v > 0.0 && v < 1e300) ? ldexp(1.0, -ilogb(v)) : 1.0; }
+
+// k_rhs (:602-627): r = -d(u*) - K phi0 (phi = 0 without warm start); block max |u*| over fluid,
+// block sum of b^2 (b = -d(u*)) for the relative-residual test and block max |r| (the scale of
+// the first V-cycle's right-hand side with 16-bit level storage).
 __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restrict__ vel,
                                              const uint8_t* __restrict__ fm, real* __restrict__ r,
                                              real* __restrict__ phi, const float* __restrict__ prm,
@@ -132,7 +136,7 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restri
     __shared__ double sh[8];
     const int n3 = N * N * N;
     const bool warm = prm[P_WARM] > RL(0.5);
-    double mx = 0.0, bb = 0.0;
+    double mx = 0.0, bb = 0.0, mr = 0.0;
     CELL_LOOP_BEGIN
         if (!warm) phi[i] = RL(0.0);
         const int m = fm[i];
@@ -142,14 +146,17 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restri
         if (warm) rv -= kmul_xyz(phi, fm, N, x, y, cz, i, m);
         r[i] = rv;
         bb += (double)b * (double)b;
+        mr = fmax(mr, fabs((double)rv));
         const real ux = vel[i], uy = vel[n3 + i], uz = vel[2 * n3 + i];
         mx = fmax(mx, (double)sqrt(ux * ux + uy * uy + uz * uz));
     CELL_LOOP_END
     const double a = block_max(mx, sh);
     const double b2 = block_sum(bb, sh);
+    const double c = block_max(mr, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         part[block_linear()] = a;
         part[nparts + block_linear()] = b2;
+        part[2 * nparts + block_linear()] = c;
     }
 }
 
@@ -159,8 +166,10 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
     __shared__ double sh[FIN_THREADS / 64];
     const double mx = reduce_parts_max(part, nparts, sh);
     const double bb = reduce_parts_sum(part + nparts, nparts, sh);
+    const double mr = reduce_parts_max(part + 2 * nparts, nparts, sh);
     if (threadIdx.x == 0) {
         const float tol = prm[P_TOL] > 1.0e-6f ? prm[P_TOL] : 1.0e-6f;
+        st->s[S_LSCALE] = pow2_scale(mr);
         st->s[S_MAXU] = mx;
         st->s[S_THR] = (double)(2.0f * tol * (float)mx);
         st->s[S_BB] = bb;
@@ -269,10 +278,10 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_pq(const double* __restrict
 // written to r_out) and z0 = omega D^-1 r are staged for the tile + halo; then thread t owns one
 // 2x2x2 block of the tile: rr = r - K z0 on its 8 cells, the level-1 components and the
 // zero-guess red sweep.
-template <bool UPDR>
+template <bool UPDR, class LT>
 __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const real* __restrict__ r_in,
                                                                 real* __restrict__ r_out, const real* __restrict__ q,
-                                                                const uint8_t* __restrict__ fm, Lvl L, int do_red,
+                                                                const uint8_t* __restrict__ fm, LvlT<LT> L, int do_red,
                                                                 const DevState* __restrict__ st, int k) {
     __shared__ __align__(16) real sz[HN];
     __shared__ __align__(16) real sr[HN];
@@ -287,12 +296,9 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
     const HBlock ob = halo_block(N, tl, own_hb(t));
     const size_t cell = (size_t)ob.lc;
     const bool red = ob.in && do_red && ((I + J + K) & 1) == 0;
-    real4 dg0 = make_real4(RL(0.), RL(0.), RL(0.), RL(0.)), dg1 = dg0;
-    if (red) {
-        const real4* dp = reinterpret_cast<const real4*>(L.diag + cell * 8);
-        dg0 = dp[0];
-        dg1 = dp[1];
-    }
+    real dg[8] = {RL(0.0), RL(0.0), RL(0.0), RL(0.0), RL(0.0), RL(0.0), RL(0.0), RL(0.0)};
+    if (red) ld8(L.diag + cell * 8, dg);
+    const real sc = level_scale<LT>(st);   // 1 for `real` storage
     HBlock hb[HB_IT];
     real2 hr[HB_IT][4], hq[HB_IT][4];
     int hm[HB_IT][4];
@@ -352,31 +358,27 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
         rr[2 * rw] = sm[h] ? r0 - kz[2 * rw] : RL(0.0);
         rr[2 * rw + 1] = sm[h + 1] ? r1 - kz[2 * rw + 1] : RL(0.0);
     }
-    const real dg[8] = {dg0.x, dg0.y, dg0.z, dg0.w, dg1.x, dg1.y, dg1.z, dg1.w};
     real out[8], xo[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         real acc = RL(0.0);
 #pragma unroll
         for (int s = 0; s < 8; ++s) acc += hsign(c, s >> 2, (s >> 1) & 1, s & 1) * rr[s];
-        out[c] = acc;
-        xo[c] = dg[c] <= L.dmin ? RL(0.0) : acc / dg[c];
+        out[c] = acc * sc;
+        // the red sweep reads the level-1 right-hand side as stored
+        const real rs = as_stored<LT>(out[c]);
+        xo[c] = dg[c] <= L.dmin ? RL(0.0) : rs / dg[c];
     }
-    real4* rp = reinterpret_cast<real4*>(L.r + cell * 8);
-    rp[0] = make_real4(out[0], out[1], out[2], out[3]);
-    rp[1] = make_real4(out[4], out[5], out[6], out[7]);
-    if (red) {
-        real4* xp4 = reinterpret_cast<real4*>(L.x + cell * 8);
-        xp4[0] = make_real4(xo[0], xo[1], xo[2], xo[3]);
-        xp4[1] = make_real4(xo[4], xo[5], xo[6], xo[7]);
-    }
+    st8(L.r + cell * 8, out);
+    if (red) st8(L.x + cell * 8, xo);
 }
 
 // k_inv_transform (:352-365) + k_post_rz (:687-710): zt = z0 + T^T x1 (0 in solids) staged for
 // the tile + halo, one 2x2x2 block at a time from its 8 level-1 values, then
 // z = zt + omega D^-1 (r - K zt) on the tile and the block partials r.z, max|r|, r.r.  At k = 0
 // z is also written to p_init (the first search direction, k_upd_p init).
-__global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const real* __restrict__ r,
+template <class LT>
+__global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, const real* __restrict__ r,
                                                            const uint8_t* __restrict__ fm, real* __restrict__ z,
                                                            real* __restrict__ p_init, double* __restrict__ part,
                                                            int nparts, const DevState* __restrict__ st, int k) {
@@ -387,16 +389,15 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const r
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
     const int t = (int)threadIdx.x, n3 = N * N * N;
+    const real isc = RL(1.0) / level_scale<LT>(st);   // undoes the scale of the level-1 right-hand side
     HBlock hb[HB_IT];
-    real4 xa[HB_IT], xb[HB_IT];
+    real xv[HB_IT][8];
     real2 hr[HB_IT][4];
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
         hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
-        const real4* xp = reinterpret_cast<const real4*>(L.x + (size_t)hb[it].lc * 8);
-        xa[it] = xp[0];
-        xb[it] = xp[1];
+        ld8(L.x + (size_t)hb[it].lc * 8, xv[it]);
 #pragma unroll
         for (int rw = 0; rw < 4; ++rw) {
             const int o = hb[it].i0 + row_off(N, rw);
@@ -409,9 +410,9 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, Lvl L, const r
     for (int it = 0; it < HB_IT; ++it) {
         if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
             const real in = hb[it].in ? RL(1.0) : RL(0.0);
-            const real hx[8] = {HS * (xa[it].x * in), HS * (xa[it].y * in), HS * (xa[it].z * in),
-                                 HS * (xa[it].w * in), HS * (xb[it].x * in), HS * (xb[it].y * in),
-                                 HS * (xb[it].z * in), HS * (xb[it].w * in)};
+            real hx[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) hx[c] = HS * ((xv[it][c] * isc) * in);
 #pragma unroll
             for (int rw = 0; rw < 4; ++rw) {
                 const real rv[2] = {hr[it][rw].x * in, hr[it][rw].y * in};
@@ -499,6 +500,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict
     st->it = k;
     st->s[S_RZ] = rz_new;
     st->s[S_MAXR] = mr;
+    st->s[S_LSCALE] = pow2_scale(mr);
     st->s[S_RR] = rr;
     st->s[S_BEST] = best;
     const double rel_tol = (double)prm[P_REL_TOL];
@@ -591,15 +593,22 @@ real* Pressure::r_buf(int k) const { return R[k & 1]; }
 void Pressure::precond(int k, hipStream_t s) {
     real* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
     real* r_out = R[k & 1];
-    if (k > 0)
-        hipLaunchKernelGGL(k_pre_resid_fwd<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
-                           S > 0 ? 1 : 0, st, k);
-    else
-        hipLaunchKernelGGL(k_pre_resid_fwd<false>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q, fm, lv[0],
-                           S > 0 ? 1 : 0, st, k);
+    with_mg_type(mg_kind, [&](auto tag) {
+        using LT = decltype(tag);
+        const LvlT<LT> L1 = lvl_as<LT>(lv[0]);
+        if (k > 0)
+            hipLaunchKernelGGL((k_pre_resid_fwd<true, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q,
+                               fm, L1, S > 0 ? 1 : 0, st, k);
+        else
+            hipLaunchKernelGGL((k_pre_resid_fwd<false, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q,
+                               fm, L1, S > 0 ? 1 : 0, st, k);
+    });
     vcycle_levels(k, s);
-    hipLaunchKernelGGL(k_inv_post, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lv[0], r_out, fm, z, P[0], part, ntiles, st,
-                       k);
+    with_mg_type(mg_kind, [&](auto tag) {
+        using LT = decltype(tag);
+        hipLaunchKernelGGL(k_inv_post<LT>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lvl_as<LT>(lv[0]), r_out, fm, z,
+                           P[0], part, ntiles, st, k);
+    });
 }
 
 void Pressure::rhs(const real* vel, const float* prm, hipStream_t s) {
@@ -650,12 +659,18 @@ bool Pressure::launch_single(int which, real* scratch, hipStream_t s) {
                                scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2);
             return true;
         case 1:
-            hipLaunchKernelGGL(k_inv_post, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lv[0], R[0], fm, z, P[0], part, ntiles,
-                               st, 2);
+            with_mg_type(mg_kind, [&](auto tag) {
+                using LT = decltype(tag);
+                hipLaunchKernelGGL(k_inv_post<LT>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lvl_as<LT>(lv[0]), R[0],
+                                   fm, z, P[0], part, ntiles, st, 2);
+            });
             return true;
         case 5:
-            hipLaunchKernelGGL(k_pre_resid_fwd<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, R[0], scratch, q, fm, lv[0],
-                               S > 0 ? 1 : 0, st, 2);
+            with_mg_type(mg_kind, [&](auto tag) {
+                using LT = decltype(tag);
+                hipLaunchKernelGGL((k_pre_resid_fwd<true, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, R[0],
+                                   scratch, q, fm, lvl_as<LT>(lv[0]), S > 0 ? 1 : 0, st, 2);
+            });
             return true;
         default: return launch_single_mg(which, s);
     }

@@ -102,7 +102,8 @@ __device__ void probe_item(const uint8_t* __restrict__ fm, int N, int m, int c, 
     }
 }
 
-__global__ void __launch_bounds__(256) k_probe(int N, const uint8_t* __restrict__ fm, Lvl L,
+template <class LT>
+__global__ void __launch_bounds__(256) k_probe(int N, const uint8_t* __restrict__ fm, LvlT<LT> L,
                                                const DevState* __restrict__ st) {
     if (!st->changed) return;
     const int m = L.m;
@@ -113,13 +114,14 @@ __global__ void __launch_bounds__(256) k_probe(int N, const uint8_t* __restrict_
         const int I = (int)(cell % m), J = (int)((cell / m) % m), K = (int)(cell / ((size_t)m * m));
         real d, o3[3];
         probe_item<false>(fm, N, m, c, K, J, I, d, o3);
-        L.diag[cell * 8 + c] = d;
-        for (int ax = 0; ax < 3; ++ax) L.off[cell * 24 + ax * 8 + c] = o3[ax];
+        stl(L.diag + cell * 8 + c, d);
+        for (int ax = 0; ax < 3; ++ax) stl(L.off + cell * 24 + ax * 8 + c, o3[ax]);
     }
 }
 
 // k_galerkin (pressure.py:249-268)
-__global__ void __launch_bounds__(256) k_galerkin(Lvl F, Lvl C, const DevState* __restrict__ st) {
+template <class LT>
+__global__ void __launch_bounds__(256) k_galerkin(LvlT<LT> F, LvlT<LT> C, const DevState* __restrict__ st) {
     if (!st->changed) return;
     const int m = C.m, mf = F.m;
     const size_t items = (size_t)m * m * m * 8;
@@ -129,21 +131,21 @@ __global__ void __launch_bounds__(256) k_galerkin(Lvl F, Lvl C, const DevState* 
         const int I = (int)(cell % m), J = (int)((cell / m) % m), K = (int)(cell / ((size_t)m * m));
         const int oz = 2 * K, oy = 2 * J, ox = 2 * I;
         real d = RL(0.0);
-        for (int k = 0; k < 8; ++k) d += F.diag[lcell(mf, oz + (k >> 2), oy + ((k >> 1) & 1), ox + (k & 1)) * 8 + c];
+        for (int k = 0; k < 8; ++k) d += ldl(F.diag + lcell(mf, oz + (k >> 2), oy + ((k >> 1) & 1), ox + (k & 1)) * 8 + c);
         for (int ax = 0; ax < 3; ++ax) {
             real oi = RL(0.0), oo = RL(0.0);
             for (int b = 0; b < 2; ++b)
                 for (int cc = 0; cc < 2; ++cc) {
                     int z, y, x;
                     axcell(ax, oz, oy, ox, 0, b, cc, z, y, x);
-                    oi += F.off[lcell(mf, z, y, x) * 24 + ax * 8 + c];
+                    oi += ldl(F.off + lcell(mf, z, y, x) * 24 + ax * 8 + c);
                     axcell(ax, oz, oy, ox, 1, b, cc, z, y, x);
-                    oo += F.off[lcell(mf, z, y, x) * 24 + ax * 8 + c];
+                    oo += ldl(F.off + lcell(mf, z, y, x) * 24 + ax * 8 + c);
                 }
             d += RL(2.0) * oi;
-            C.off[cell * 24 + ax * 8 + c] = oo * GALERKIN_SCALE;
+            stl(C.off + cell * 24 + ax * 8 + c, oo * GALERKIN_SCALE);
         }
-        C.diag[cell * 8 + c] = d * GALERKIN_SCALE;
+        stl(C.diag + cell * 8 + c, d * GALERKIN_SCALE);
     }
 }
 
@@ -153,15 +155,16 @@ __global__ void k_clear_changed(DevState* st) {
 
 // Standard-cell tables of every level (thread c: component c): level 1 from the probe of a block
 // in an all-fluid neighbourhood, each coarser level from k_galerkin's sums over standard fine
-// values, in k_galerkin's order.
-__global__ void __launch_bounds__(64) k_std_tab(CoarseLevels all, int N, const DevState* __restrict__ st) {
+// values, in k_galerkin's order; every value as stored (rounded to LT), like the level arrays.
+template <class LT>
+__global__ void __launch_bounds__(64) k_std_tab(CoarseLevelsT<LT> all, int N, const DevState* __restrict__ st) {
     if (!st->changed) return;
     const int c = (int)threadIdx.x;
     if (c >= 8) return;
     real d, o3[3];
     probe_item<true>(nullptr, N, all.lv[0].m, c, 1, 1, 1, d, o3);
-    all.lv[0].tab[c] = d;
-    for (int ax = 0; ax < 3; ++ax) all.lv[0].tab[8 + ax * 8 + c] = o3[ax];
+    all.lv[0].tab[c] = as_stored<LT>(d);
+    for (int ax = 0; ax < 3; ++ax) all.lv[0].tab[8 + ax * 8 + c] = as_stored<LT>(o3[ax]);
     for (int l = 0; l + 1 < all.n; ++l) {
         const real* ft = all.lv[l].tab;
         real* ct = all.lv[l + 1].tab;
@@ -174,15 +177,16 @@ __global__ void __launch_bounds__(64) k_std_tab(CoarseLevels all, int N, const D
                 oo += ft[8 + ax * 8 + c];
             }
             dd += RL(2.0) * oi;
-            ct[8 + ax * 8 + c] = oo * GALERKIN_SCALE;
+            ct[8 + ax * 8 + c] = as_stored<LT>(oo * GALERKIN_SCALE);
         }
-        ct[c] = dd * GALERKIN_SCALE;
+        ct[c] = as_stored<LT>(dd * GALERKIN_SCALE);
     }
 }
 
 // sf = 1 where the cell has all 6 neighbours, its diag and links and its -ax neighbours' links
 // equal tab bit for bit, and every neighbour has diag > dmin
-__global__ void __launch_bounds__(256) k_std_flags(Lvl L, int use_std, const DevState* __restrict__ st) {
+template <class LT>
+__global__ void __launch_bounds__(256) k_std_flags(LvlT<LT> L, int use_std, const DevState* __restrict__ st) {
     if (!st->changed) return;
     const int m = L.m;
     const size_t m3 = (size_t)m * m * m;
@@ -193,12 +197,12 @@ __global__ void __launch_bounds__(256) k_std_flags(Lvl L, int use_std, const Dev
             const size_t sy = (size_t)m, sz = (size_t)m * m;
             const size_t nb[6] = {o + 1, o - 1, o + sy, o - sy, o + sz, o - sz};
             for (int c = 0; c < 8; ++c) {
-                s = s && same_bits(L.diag[o * 8 + c], L.tab[c]);
+                s = s && same_bits(ldl(L.diag + o * 8 + c), L.tab[c]);
                 for (int ax = 0; ax < 3; ++ax) {
-                    s = s && same_bits(L.off[o * 24 + ax * 8 + c], L.tab[8 + ax * 8 + c]);
-                    s = s && same_bits(L.off[nb[2 * ax + 1] * 24 + ax * 8 + c], L.tab[8 + ax * 8 + c]);
+                    s = s && same_bits(ldl(L.off + o * 24 + ax * 8 + c), L.tab[8 + ax * 8 + c]);
+                    s = s && same_bits(ldl(L.off + nb[2 * ax + 1] * 24 + ax * 8 + c), L.tab[8 + ax * 8 + c]);
                 }
-                for (int k = 0; k < 6; ++k) s = s && L.diag[nb[k] * 8 + c] > L.dmin;
+                for (int k = 0; k < 6; ++k) s = s && ldl(L.diag + nb[k] * 8 + c) > L.dmin;
             }
         }
         L.sf[o] = s ? 1 : 0;
@@ -209,8 +213,9 @@ __global__ void __launch_bounds__(256) k_std_flags(Lvl L, int use_std, const Dev
 // residual restriction (k_resrestrict :306-316) + optional zero-guess red sweep at the coarse
 // level (k_restrict_red :444-459).  A child's neighbour inside the 2x2x2 block is another child
 // (the same address, loaded once); one outside is clamped to the child itself where missing.
-template <bool STD>
-__device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I, bool do_red) {
+template <bool STD, class LT>
+__device__ __forceinline__ void restrict_item(const LvlT<LT>& F, const LvlT<LT>& C, int c, int K, int J, int I,
+                                              bool do_red) {
     const int m = F.m;
     size_t co[8], nb[8][6];
     bool e[8][6];
@@ -234,10 +239,10 @@ __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c,
         nb[k][4] = e[k][4] ? lcell(m, z + 1, y, x) : co[k];
         nb[k][5] = e[k][5] ? lcell(m, z - 1, y, x) : co[k];
         if (STD) f &= (int)F.sf[co[k]];
-        r[k] = F.r[co[k] * 8 + c];
-        x0[k] = F.x[co[k] * 8 + c];
+        r[k] = ldl(F.r + co[k] * 8 + c);
+        x0[k] = ldl(F.x + co[k] * 8 + c);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) xn[k][j] = F.x[nb[k][j] * 8 + c];
+        for (int j = 0; j < 6; ++j) xn[k][j] = ldl(F.x + nb[k][j] * 8 + c);
     }
     const Tab tb = STD ? load_tab(F, c) : Tab{};
     real res[8];
@@ -250,8 +255,8 @@ __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c,
         for (int k = 0; k < 8; ++k) {
             real cf[6];
 #pragma unroll
-            for (int j = 0; j < 6; ++j) cf[j] = F.off[((j & 1) ? nb[k][j] : co[k]) * 24 + (j >> 1) * 8 + c];
-            res[k] = r[k] - F.diag[co[k] * 8 + c] * x0[k] - offsum_v(cf, xn[k], e[k]);
+            for (int j = 0; j < 6; ++j) cf[j] = ldl(F.off + ((j & 1) ? nb[k][j] : co[k]) * 24 + (j >> 1) * 8 + c);
+            res[k] = r[k] - ldl(F.diag + co[k] * 8 + c) * x0[k] - offsum_v(cf, xn[k], e[k]);
         }
     }
     real acc = RL(0.0);
@@ -259,39 +264,42 @@ __device__ __forceinline__ void restrict_item(const Lvl& F, const Lvl& C, int c,
     for (int k = 0; k < 8; ++k) acc += res[k];
     acc = RL(0.125) * acc;
     const size_t oc = lcell(C.m, K, J, I) * 8 + c;
-    C.r[oc] = acc;
+    stl(C.r + oc, acc);
     if (do_red && ((I + J + K) & 1) == 0) {
-        const real d = C.diag[oc];
-        C.x[oc] = d <= C.dmin ? RL(0.0) : acc / d;
+        // the coarse red sweep reads the right-hand side as stored
+        const real d = ldl(C.diag + oc), ra = as_stored<LT>(acc);
+        stl(C.x + oc, d <= C.dmin ? RL(0.0) : ra / d);
     }
 }
 
 // constant prolongation (k_prolong :319-325)
-__device__ __forceinline__ void prolong_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I) {
+template <class LT>
+__device__ __forceinline__ void prolong_item(const LvlT<LT>& F, const LvlT<LT>& C, int c, int K, int J, int I) {
     const size_t o = lcell(F.m, K, J, I) * 8 + c;
-    const real x = F.x[o], xc = C.x[lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c];
-    if (F.diag[o] <= F.dmin) return;
-    F.x[o] = x + xc;
+    const real x = ldl(F.x + o), xc = ldl(C.x + lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c);
+    if (ldl(F.diag + o) <= F.dmin) return;
+    stl(F.x + o, x + xc);
 }
 
 // prolongation fused into the black post-sweep: equal to prolonging every cell, then sweeping
 // black (a GS update ignores the cell's own old value; red cells are rewritten by the red sweep
 // that follows).  Same rounding as the unfused pair (pressure.py:396-423 vs :902-931).
-__device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, int c, int K, int J, int I) {
+template <class LT>
+__device__ __forceinline__ void prolong_black_item(const LvlT<LT>& F, const LvlT<LT>& C, int c, int K, int J, int I) {
     const int m = F.m;
     const size_t o = lcell(m, K, J, I);
     const LNb n = level_neighbours(m, o, K, J, I);
     const int nI[6] = {I + 1, I - 1, I, I, I, I}, nJ[6] = {J, J, J + 1, J - 1, J, J},
               nK[6] = {K, K, K, K, K + 1, K - 1};
     const int f = F.sf[o];
-    const real r = F.r[o * 8 + c];
+    const real r = ldl(F.r + o * 8 + c);
     real xn[6], xcn[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         // coordinates of the clamped neighbour (the cell itself where it is missing)
         const int kI = n.e[k] ? nI[k] : I, kJ = n.e[k] ? nJ[k] : J, kK = n.e[k] ? nK[k] : K;
-        xn[k] = F.x[n.c[k] * 8 + c];
-        xcn[k] = C.x[lcell(C.m, kK >> 1, kJ >> 1, kI >> 1) * 8 + c];
+        xn[k] = ldl(F.x + n.c[k] * 8 + c);
+        xcn[k] = ldl(C.x + lcell(C.m, kK >> 1, kJ >> 1, kI >> 1) * 8 + c);
     }
     const Tab tb = load_tab(F, c);
     real d, cf[6], xpv[6];
@@ -300,19 +308,19 @@ __device__ __forceinline__ void prolong_black_item(const Lvl& F, const Lvl& C, i
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             cf[k] = tb.o[k >> 1];
-            xpv[k] = xn[k] + xcn[k];   // every neighbour of a standard cell has diag > dmin
+            xpv[k] = as_stored<LT>(xn[k] + xcn[k]);   // a standard cell's neighbours have diag > dmin
         }
     } else {
-        d = F.diag[o * 8 + c];
+        d = ldl(F.diag + o * 8 + c);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             cf[k] = link_coef(F, n, o, k, c);
             // x of the neighbour after the constant prolongation (which skips diag <= dmin)
-            xpv[k] = F.diag[n.c[k] * 8 + c] <= F.dmin ? xn[k] : xn[k] + xcn[k];
+            xpv[k] = ldl(F.diag + n.c[k] * 8 + c) <= F.dmin ? xn[k] : as_stored<LT>(xn[k] + xcn[k]);
         }
     }
     const real a = offsum_v(cf, xpv, n.e);
-    F.x[o * 8 + c] = d <= F.dmin ? RL(0.0) : (r - a) / d;
+    stl(F.x + o * 8 + c, d <= F.dmin ? RL(0.0) : (r - a) / d);
 }
 
 // item of this thread in a 1D launch, blocks renumbered per XCD (neighbouring rows in one L2)
@@ -334,14 +342,17 @@ __device__ __forceinline__ bool half_item(size_t t, int m, int color, int& c, in
     return true;
 }
 
-__global__ void __launch_bounds__(256) k_smooth_half(Lvl L, int color, const DevState* __restrict__ st, int k) {
+template <class LT>
+__global__ void __launch_bounds__(256) k_smooth_half(LvlT<LT> L, int color, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     int c, K, J, I;
     if (!half_item(xcd_item(), L.m, color, c, K, J, I)) return;
     smooth_item<true>(L, c, K, J, I, false);
 }
 
-__global__ void __launch_bounds__(256) k_restrict(Lvl F, Lvl C, int do_red, const DevState* __restrict__ st, int k) {
+template <class LT>
+__global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do_red, const DevState* __restrict__ st,
+                                                  int k) {
     if (cg_inactive(st, k)) return;
     const int m = C.m;
     const size_t t = xcd_item();
@@ -351,7 +362,8 @@ __global__ void __launch_bounds__(256) k_restrict(Lvl F, Lvl C, int do_red, cons
     restrict_item<true>(F, C, c, (int)(cell / ((size_t)m * m)), (int)((cell / m) % m), (int)(cell % m), do_red != 0);
 }
 
-__global__ void __launch_bounds__(256) k_prolong_black(Lvl F, Lvl C, const DevState* __restrict__ st, int k) {
+template <class LT>
+__global__ void __launch_bounds__(256) k_prolong_black(LvlT<LT> F, LvlT<LT> C, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     int c, K, J, I;
     if (!half_item(xcd_item(), F.m, 1, c, K, J, I)) return;
@@ -363,7 +375,8 @@ __global__ void __launch_bounds__(256) k_prolong_black(Lvl F, Lvl C, const DevSt
 // levels lv[0..n-1], starting with the zero-guess red sweep at lv[0].
 constexpr int CB_THREADS = 1024;
 
-__device__ void cb_smooth(const Lvl& L, int color, bool zero_init) {
+template <class LT>
+__device__ void cb_smooth(const LvlT<LT>& L, int color, bool zero_init) {
     const int m = L.m;
     const size_t items = (size_t)m * m * m * 8;
     for (size_t t = threadIdx.x; t < items; t += CB_THREADS) {
@@ -375,14 +388,15 @@ __device__ void cb_smooth(const Lvl& L, int color, bool zero_init) {
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevels cl, const DevState* __restrict__ st, int k) {
+template <class LT>
+__global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevelsT<LT> cl, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     const int n = cl.n;
     for (int j = 0; j + 1 < n; ++j) {
         cb_smooth(cl.lv[j], 0, true);
         cb_smooth(cl.lv[j], 1, false);
-        const Lvl& F = cl.lv[j];
-        const Lvl& C = cl.lv[j + 1];
+        const LvlT<LT>& F = cl.lv[j];
+        const LvlT<LT>& C = cl.lv[j + 1];
         const size_t items = (size_t)C.m * C.m * C.m * 8;
         for (size_t t = threadIdx.x; t < items; t += CB_THREADS) {
             const int c = (int)(t & 7);
@@ -392,11 +406,11 @@ __global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevels cl, const De
         }
         __syncthreads();
     }
-    const Lvl& B = cl.lv[n - 1];
+    const LvlT<LT>& B = cl.lv[n - 1];
     if (B.m == 1) {
         if (threadIdx.x < 8) {
-            const real d = B.diag[threadIdx.x];
-            B.x[threadIdx.x] = d <= B.dmin ? RL(0.0) : B.r[threadIdx.x] / d;
+            const real d = ldl(B.diag + threadIdx.x);
+            stl(B.x + threadIdx.x, d <= B.dmin ? RL(0.0) : ldl(B.r + threadIdx.x) / d);
         }
         __syncthreads();
     } else {
@@ -406,8 +420,8 @@ __global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevels cl, const De
         }
     }
     for (int j = n - 2; j >= 0; --j) {
-        const Lvl& F = cl.lv[j];
-        const Lvl& C = cl.lv[j + 1];
+        const LvlT<LT>& F = cl.lv[j];
+        const LvlT<LT>& C = cl.lv[j + 1];
         const size_t items = (size_t)F.m * F.m * F.m * 8;
         for (size_t t = threadIdx.x; t < items; t += CB_THREADS) {
             const int c = (int)(t & 7);
@@ -482,20 +496,25 @@ void Pressure::setup(const real* solid, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_solid_check, dim3(grid_for(n3, 4096)), dim3(256), 0, s, n3, solid, fm, st);
     hipLaunchKernelGGL(k_mask, dim3(grid_for(n3, 4096)), dim3(256), 0, s, N, solid, fm, st);
-    const size_t m3 = (size_t)lv[0].m * lv[0].m * lv[0].m;
-    hipLaunchKernelGGL(k_probe, dim3(grid_for(m3 * 8, 4096)), dim3(256), 0, s, N, fm, lv[0], st);
-    for (int l = 0; l + 1 < nlev; ++l) {
-        const size_t mc3 = (size_t)lv[l + 1].m * lv[l + 1].m * lv[l + 1].m;
-        hipLaunchKernelGGL(k_galerkin, dim3(grid_for(mc3 * 8, 4096)), dim3(256), 0, s, lv[l], lv[l + 1], st);
-    }
-    CoarseLevels all{};
-    all.n = nlev;
-    for (int l = 0; l < nlev; ++l) all.lv[l] = lv[l];
-    hipLaunchKernelGGL(k_std_tab, dim3(1), dim3(64), 0, s, all, N, st);
-    for (int l = 0; l < nlev; ++l) {
-        const size_t ml3 = (size_t)lv[l].m * lv[l].m * lv[l].m;
-        hipLaunchKernelGGL(k_std_flags, dim3(grid_for(ml3, 4096)), dim3(256), 0, s, lv[l], use_std ? 1 : 0, st);
-    }
+    with_mg_type(mg_kind, [&](auto tag) {
+        using LT = decltype(tag);
+        const size_t m3 = (size_t)lv[0].m * lv[0].m * lv[0].m;
+        hipLaunchKernelGGL(k_probe<LT>, dim3(grid_for(m3 * 8, 4096)), dim3(256), 0, s, N, fm, lvl_as<LT>(lv[0]), st);
+        for (int l = 0; l + 1 < nlev; ++l) {
+            const size_t mc3 = (size_t)lv[l + 1].m * lv[l + 1].m * lv[l + 1].m;
+            hipLaunchKernelGGL(k_galerkin<LT>, dim3(grid_for(mc3 * 8, 4096)), dim3(256), 0, s, lvl_as<LT>(lv[l]),
+                               lvl_as<LT>(lv[l + 1]), st);
+        }
+        CoarseLevelsT<LT> all{};
+        all.n = nlev;
+        for (int l = 0; l < nlev; ++l) all.lv[l] = lvl_as<LT>(lv[l]);
+        hipLaunchKernelGGL(k_std_tab<LT>, dim3(1), dim3(64), 0, s, all, N, st);
+        for (int l = 0; l < nlev; ++l) {
+            const size_t ml3 = (size_t)lv[l].m * lv[l].m * lv[l].m;
+            hipLaunchKernelGGL(k_std_flags<LT>, dim3(grid_for(ml3, 4096)), dim3(256), 0, s, lvl_as<LT>(lv[l]),
+                               use_std ? 1 : 0, st);
+        }
+    });
     hipLaunchKernelGGL(k_clear_changed, dim3(1), dim3(64), 0, s, st);
 }
 
@@ -507,55 +526,63 @@ namespace vfx {
 // k_inv_post after): black sweep + restriction on every level above the coarse kernel, the
 // single-workgroup coarse cycle, then prolongation + black sweep and the red sweep on the way up.
 void Pressure::vcycle_levels(int k, hipStream_t s) {
-    for (int l = 0; l < S; ++l) {
-        const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
-        hipLaunchKernelGGL(k_smooth_half, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s, lv[l], 1, st, k);
-        const size_t items = (size_t)lv[l + 1].m * lv[l + 1].m * lv[l + 1].m * 8;
-        hipLaunchKernelGGL(k_restrict, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, lv[l], lv[l + 1],
-                           (l + 1 < S) ? 1 : 0, st, k);
-    }
-    CoarseLevels cl{};
-    cl.n = nlev - S;
-    for (int j = 0; j < cl.n; ++j) cl.lv[j] = lv[S + j];
-    hipLaunchKernelGGL(k_coarse, dim3(1), dim3(CB_THREADS), 0, s, cl, st, k);
-    for (int l = S - 1; l >= 0; --l) {
-        const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
-        hipLaunchKernelGGL(k_prolong_black, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s, lv[l], lv[l + 1],
-                           st, k);
-        hipLaunchKernelGGL(k_smooth_half, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s, lv[l], 0, st, k);
-    }
+    with_mg_type(mg_kind, [&](auto tag) {
+        using LT = decltype(tag);
+        for (int l = 0; l < S; ++l) {
+            const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
+            hipLaunchKernelGGL(k_smooth_half<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
+                               lvl_as<LT>(lv[l]), 1, st, k);
+            const size_t items = (size_t)lv[l + 1].m * lv[l + 1].m * lv[l + 1].m * 8;
+            hipLaunchKernelGGL(k_restrict<LT>, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s,
+                               lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), (l + 1 < S) ? 1 : 0, st, k);
+        }
+        CoarseLevelsT<LT> cl{};
+        cl.n = nlev - S;
+        for (int j = 0; j < cl.n; ++j) cl.lv[j] = lvl_as<LT>(lv[S + j]);
+        hipLaunchKernelGGL(k_coarse<LT>, dim3(1), dim3(CB_THREADS), 0, s, cl, st, k);
+        for (int l = S - 1; l >= 0; --l) {
+            const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
+            hipLaunchKernelGGL(k_prolong_black<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
+                               lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), st, k);
+            hipLaunchKernelGGL(k_smooth_half<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
+                               lvl_as<LT>(lv[l]), 0, st, k);
+        }
+    });
 }
 
 // Multigrid kernels for timing (ids relative to VFX_K_MATVEC).
 bool Pressure::launch_single_mg(int which, hipStream_t s) {
     const size_t m0 = (size_t)lv[0].m;
     const size_t half0 = m0 * m0 * (m0 / 2) * 8;
-    switch (which) {
-        case 2:
-            if (S < 1) return false;
-            hipLaunchKernelGGL(k_smooth_half, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s, lv[0], 1, st, 0);
-            return true;
-        case 3: {
-            if (S < 1) return false;
-            const size_t items = (size_t)lv[1].m * lv[1].m * lv[1].m * 8;
-            hipLaunchKernelGGL(k_restrict, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, lv[0], lv[1],
-                               S > 1 ? 1 : 0, st, 0);
-            return true;
+    if (which != 10 && (which < 2 || which > 4)) return false;
+    if (which != 10 && S < 1) return false;
+    with_mg_type(mg_kind, [&](auto tag) {
+        using LT = decltype(tag);
+        switch (which) {
+            case 2:
+                hipLaunchKernelGGL(k_smooth_half<LT>, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s,
+                                   lvl_as<LT>(lv[0]), 1, st, 0);
+                break;
+            case 3: {
+                const size_t items = (size_t)lv[1].m * lv[1].m * lv[1].m * 8;
+                hipLaunchKernelGGL(k_restrict<LT>, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s,
+                                   lvl_as<LT>(lv[0]), lvl_as<LT>(lv[1]), S > 1 ? 1 : 0, st, 0);
+                break;
+            }
+            case 4:
+                hipLaunchKernelGGL(k_prolong_black<LT>, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s,
+                                   lvl_as<LT>(lv[0]), lvl_as<LT>(lv[1]), st, 0);
+                break;
+            default: {
+                CoarseLevelsT<LT> cl{};
+                cl.n = nlev - S;
+                for (int j = 0; j < cl.n; ++j) cl.lv[j] = lvl_as<LT>(lv[S + j]);
+                hipLaunchKernelGGL(k_coarse<LT>, dim3(1), dim3(CB_THREADS), 0, s, cl, st, 0);
+                break;
+            }
         }
-        case 4:
-            if (S < 1) return false;
-            hipLaunchKernelGGL(k_prolong_black, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s, lv[0], lv[1],
-                               st, 0);
-            return true;
-        case 10: {
-            CoarseLevels cl{};
-            cl.n = nlev - S;
-            for (int j = 0; j < cl.n; ++j) cl.lv[j] = lv[S + j];
-            hipLaunchKernelGGL(k_coarse, dim3(1), dim3(CB_THREADS), 0, s, cl, st, 0);
-            return true;
-        }
-        default: return false;
-    }
+    });
+    return true;
 }
 
 }  // namespace vfx

@@ -192,9 +192,10 @@ __device__ __forceinline__ LNb level_neighbours(int m, size_t o, int K, int J, i
 
 // coupling coefficient of link k (0..5) of component c: +ax links are stored at the cell, -ax
 // links at the -ax neighbour
-__device__ __forceinline__ real link_coef(const Lvl& L, const LNb& n, size_t o, int k, int c) {
+template <class LT>
+__device__ __forceinline__ real link_coef(const LvlT<LT>& L, const LNb& n, size_t o, int k, int c) {
     const int ax = k >> 1;
-    return L.off[((k & 1) ? n.c[k] : o) * 24 + ax * 8 + c];
+    return ldl(L.off + ((k & 1) ? n.c[k] : o) * 24 + ax * 8 + c);
 }
 
 // Standard cells.  Away from walls and solids every cell of a level has the same diag and link
@@ -207,7 +208,8 @@ struct Tab {
     real d, o[3];   // diag and the x, y, z link coefficients of component c
 };
 
-__device__ __forceinline__ Tab load_tab(const Lvl& L, int c) {
+template <class LT>
+__device__ __forceinline__ Tab load_tab(const LvlT<LT>& L, int c) {
     Tab t;
     t.d = L.tab[c];
     t.o[0] = L.tab[8 + c];
@@ -226,19 +228,19 @@ __device__ __forceinline__ real offsum_v(const real* cf, const real* xn, const b
 
 // one GS update (k_smooth pressure.py:290-303); returns the new value.  STD: waves of standard
 // cells use the table (the single-workgroup coarse levels have few standard cells and skip it).
-template <bool STD>
-__device__ __forceinline__ real smooth_value(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
+template <bool STD, class LT>
+__device__ __forceinline__ real smooth_value(const LvlT<LT>& L, int c, int K, int J, int I, bool zero_init) {
     const size_t o = lcell(L.m, K, J, I);
-    const real r = L.r[o * 8 + c];
+    const real r = ldl(L.r + o * 8 + c);
     if (zero_init) {
-        const real d = L.diag[o * 8 + c];
+        const real d = ldl(L.diag + o * 8 + c);
         return d <= L.dmin ? RL(0.0) : r / d;
     }
     const LNb n = level_neighbours(L.m, o, K, J, I);
     const int f = STD ? L.sf[o] : 0;
     real xn[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xn[k] = L.x[n.c[k] * 8 + c];
+    for (int k = 0; k < 6; ++k) xn[k] = ldl(L.x + n.c[k] * 8 + c);
     const Tab tb = STD ? load_tab(L, c) : Tab{};
     real d, cf[6];
     if (STD && __all(f != 0)) {
@@ -246,7 +248,7 @@ __device__ __forceinline__ real smooth_value(const Lvl& L, int c, int K, int J, 
 #pragma unroll
         for (int k = 0; k < 6; ++k) cf[k] = tb.o[k >> 1];
     } else {
-        d = L.diag[o * 8 + c];
+        d = ldl(L.diag + o * 8 + c);
 #pragma unroll
         for (int k = 0; k < 6; ++k) cf[k] = link_coef(L, n, o, k, c);
     }
@@ -254,10 +256,10 @@ __device__ __forceinline__ real smooth_value(const Lvl& L, int c, int K, int J, 
     return d <= L.dmin ? RL(0.0) : acc / d;
 }
 
-template <bool STD>
-__device__ __forceinline__ real smooth_item(const Lvl& L, int c, int K, int J, int I, bool zero_init) {
+template <bool STD, class LT>
+__device__ __forceinline__ real smooth_item(const LvlT<LT>& L, int c, int K, int J, int I, bool zero_init) {
     const real v = smooth_value<STD>(L, c, K, J, I, zero_init);
-    L.x[lcell(L.m, K, J, I) * 8 + c] = v;
+    stl(L.x + lcell(L.m, K, J, I) * 8 + c, v);
     return v;
 }
 

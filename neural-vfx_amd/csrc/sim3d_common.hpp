@@ -5,6 +5,8 @@
 // FieldState3D, vfx/core/fields3d.py:1-10).  Masks are uint8 open-face bit sets.
 #pragma once
 
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -64,7 +66,7 @@ enum : int {
 // slots, kernels.py:44-53); the CG inner products, alpha and beta are FP64 here.
 enum : int {
     S_MASS_RATIO = 0, S_MAXU = 1, S_RZ = 2, S_ALPHA = 3, S_BETA = 4, S_MAXR = 5, S_THR = 6,
-    S_BEST = 7, S_BB = 8, S_RR = 9, S_TRUE_RR = 10, S_COUNT = 16,
+    S_BEST = 7, S_BB = 8, S_RR = 9, S_TRUE_RR = 10, S_LSCALE = 11, S_COUNT = 16,
 };
 
 struct DevState {

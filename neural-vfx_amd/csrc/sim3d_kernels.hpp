@@ -7,24 +7,113 @@
 
 namespace vfx {
 
-// One multigrid level: m^3 blocks x 8 Hadamard components (component-fastest storage).
-struct Lvl {
+// One multigrid level: m^3 blocks x 8 Hadamard components (component-fastest storage).  LT is
+// the storage type of the level arrays: `real`, or a 16-bit type (FP16 / BF16) whose values are
+// converted to `real` on load and rounded to nearest on store; all arithmetic is in `real`.
+template <class LT>
+struct LvlT {
     int m;
     real dmin;   // diag <= dmin -> x = 0 (pressure.py:803)
-    real* x;     // [m^3][8]
-    real* r;     // [m^3][8]
-    real* diag;  // [m^3][8]
-    real* off;   // [m^3][24]: ax*8 + c couples to the +ax neighbour
-    real* tab;   // [32]: diag[8], off[24] of a cell in an all-fluid neighbourhood
+    LT* x;       // [m^3][8]
+    LT* r;       // [m^3][8]
+    LT* diag;    // [m^3][8]
+    LT* off;     // [m^3][24]: ax*8 + c couples to the +ax neighbour
+    real* tab;   // [32]: diag[8], off[24] of a cell in an all-fluid neighbourhood (as stored)
     uint8_t* sf;  // [m^3]: 1 where a cell's whole stencil equals tab (see k_std_flags)
 };
+using Lvl = LvlT<real>;
+
+// the same level arrays viewed with storage type LT (each array has room for m^3 x n reals)
+template <class LT>
+inline LvlT<LT> lvl_as(const Lvl& L) {
+    return LvlT<LT>{L.m, L.dmin, reinterpret_cast<LT*>(L.x), reinterpret_cast<LT*>(L.r),
+                    reinterpret_cast<LT*>(L.diag), reinterpret_cast<LT*>(L.off), L.tab, L.sf};
+}
 
 constexpr int MAX_LEVELS = 16;
 
-struct CoarseLevels {
+template <class LT>
+struct CoarseLevelsT {
     int n;
-    Lvl lv[MAX_LEVELS];
+    LvlT<LT> lv[MAX_LEVELS];
 };
+using CoarseLevels = CoarseLevelsT<real>;
+
+// storage kinds of the multigrid levels (VFX_OPT_MG_STORAGE)
+enum : int { MG_REAL = 0, MG_FP16 = 1, MG_BF16 = 2 };
+
+// element load / store of level arrays: convert to `real`, round to nearest on store
+__device__ __forceinline__ real ldl(const real* p) { return *p; }
+__device__ __forceinline__ void stl(real* p, real v) { *p = v; }
+#if !VFX_F64
+__device__ __forceinline__ real ldl(const __half* p) { return __half2float(*p); }
+__device__ __forceinline__ void stl(__half* p, real v) { *p = __float2half_rn(v); }
+__device__ __forceinline__ real ldl(const __hip_bfloat16* p) { return __bfloat162float(*p); }
+__device__ __forceinline__ void stl(__hip_bfloat16* p, real v) { *p = __float2bfloat16(v); }
+#endif
+
+// v rounded to the storage type LT
+template <class LT>
+__device__ __forceinline__ real as_stored(real v) {
+    LT t;
+    stl(&t, v);
+    return ldl(&t);
+}
+
+// the 8 components of one level cell (p = array + cell * 8): two real4 or one 16-byte load
+template <class LT>
+__device__ __forceinline__ void ld8(const LT* p, real* v) {
+    if constexpr (sizeof(LT) == sizeof(real)) {
+        const real4 a = reinterpret_cast<const real4*>(p)[0], b = reinterpret_cast<const real4*>(p)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+        union { uint4 u; LT h[8]; } w;
+        w.u = *reinterpret_cast<const uint4*>(p);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = ldl(&w.h[c]);
+    }
+}
+template <class LT>
+__device__ __forceinline__ void st8(LT* p, const real* v) {
+    if constexpr (sizeof(LT) == sizeof(real)) {
+        reinterpret_cast<real4*>(p)[0] = make_real4(v[0], v[1], v[2], v[3]);
+        reinterpret_cast<real4*>(p)[1] = make_real4(v[4], v[5], v[6], v[7]);
+    } else {
+        union { uint4 u; LT h[8]; } w;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) stl(&w.h[c], v[c]);
+        *reinterpret_cast<uint4*>(p) = w.u;
+    }
+}
+
+// call f(LT{}) with the level storage type of `kind` (the FP64 build stores levels in double)
+template <class Fn>
+void with_mg_type(int kind, Fn&& f) {
+#if VFX_F64
+    (void)kind;
+    f(real{});
+#else
+    switch (kind) {
+        case MG_FP16: f(__half{}); break;
+        case MG_BF16: f(__hip_bfloat16{}); break;
+        default: f(real{}); break;
+    }
+#endif
+}
+
+// scale of the level-1 right-hand side of a V-cycle with 16-bit storage: a power of two that
+// brings the current residual's magnitude near 1 (exact in `real`; FP16 keeps no subnormal
+// residuals), 1 for `real` storage
+template <class LT>
+__device__ __forceinline__ real level_scale(const DevState* st) {
+    if constexpr (sizeof(LT) == sizeof(real)) {
+        return RL(1.0);
+    } else {
+        const double v = st->s[S_LSCALE];
+        return v > 0.0 ? (real)v : RL(1.0);
+    }
+}
 
 // ---------------------------------------------------------------- advection / forces
 void launch_advect(const Geom& g, int C, const real* src, const real* vel, const float* prm, real inv_dx,
@@ -52,6 +141,7 @@ struct Pressure {
     int S = 0;            // first level handled by the single-workgroup coarse kernel
     int coarse_m = 8;     // levels with m <= coarse_m run inside that kernel
     bool use_std = true;  // standard-cell flags on (off: every cell reads the level arrays)
+    int mg_kind = MG_REAL;   // storage of the multigrid levels (MG_REAL / MG_FP16 / MG_BF16)
     bool force_setup = false;   // rebuild the operators at the next setup
     Lvl lv[MAX_LEVELS];
     Geom geom;

@@ -14,7 +14,8 @@ from ._build import LIB_PATH
 
 VFX_OK, VFX_E_ARG, VFX_E_VALUE, VFX_E_HIP = 0, 1, 2, 3
 STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
-OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL = 1, 2, 3, 4
+OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL, OPT_MG_STORAGE = 1, 2, 3, 4, 5
+MG_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
 # kernel ids of vfx_time_kernel (include/vfxsim3d.h VFX_K_*); "" = unused id
 KERNELS = ("adv_sl", "adv_mc", "sources", "diffuse", "upd_matvec", "inv_post", "smooth0", "restrict0", "prolong0",
            "pre_resid_fwd", "", "", "", "", "coarse")

@@ -58,6 +58,9 @@ class SolverConfig3D:
     noise_seed: int = 0
     # global emitter noise amplitude: injection scaled by U[1 - a, 1 + a] per cell and step
     emitter_noise: float = 0.0
+    # storage of the multigrid levels: "fp32" (the reference), "fp16" or "bf16" (FP32 arithmetic;
+    # CG vectors stay FP32, dots and alpha/beta FP64)
+    mg_storage: str = "fp32"
 
 
 def fused_config(**kw) -> SolverConfig3D:
@@ -90,6 +93,8 @@ def solver_version_tag(cfg: SolverConfig3D, scalar_res: int | None = None) -> st
         tag += f"+relres{cfg.pressure_rel_tol:g}"
     if cfg.emitter_noise > 0:
         tag += f"+noise{cfg.emitter_noise:g}s{cfg.noise_seed}"
+    if cfg.mg_storage != "fp32":
+        tag += f"+mg{cfg.mg_storage}"
     return tag
 
 

@@ -123,6 +123,10 @@ class Solver3D:
 
     def _apply_options(self):
         self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_RESTARTS, float(self.config.pressure_restarts)))
+        kind = _lib.MG_STORAGE.get(self.config.mg_storage)
+        if kind is None:
+            raise ValueError(f"mg_storage must be one of {sorted(_lib.MG_STORAGE)}, got {self.config.mg_storage!r}")
+        self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_MG_STORAGE, float(kind)))
 
     def set_option(self, key: int, value: float) -> None:
         """Library option (vfxsim3d.h VFX_OPT_*): true-residual report, restarts, coarse size."""

@@ -100,25 +100,32 @@ PLUME = Forces3D(emitters=[Emitter3D(pos=(0.5, 0.1, 0.5), radius=0.08, density_r
                                      temperature=294.0, fuel_rate=0.0, velocity=(0.0, 0.0, 0.0))])
 
 
-def test_fp32_against_fp64_run(cuda, hiplib):
-    """The FP32 path against the FP64 run of the same 64^3 plume (40 steps): both converge to
-    true relative residual 1e-5 every step; the FP32 fields stay close to the FP64 ones."""
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_fp32_against_fp64_run(cuda, hiplib, storage):
+    """The FP32 path (multigrid levels stored in `storage`) against the FP64 run of the same 64^3
+    plume past a sphere (40 steps): both converge to true relative residual 1e-5 every step; the
+    FP32 fields stay close to the FP64 ones."""
+    import dataclasses
     N, steps, dt = 64, 40, 1.0 / 60.0
-    cfg = _plume_config()
     obst = [Obstacle3D(kind="sphere", pos=(0.5, 0.5, 0.5), size=(0.15, 0.15, 0.15))]
     runs = {}
     for dtype in (torch.float32, torch.float64):
+        cfg = _plume_config()
+        if dtype == torch.float32:
+            cfg = dataclasses.replace(cfg, mg_storage=storage)
         s = Solver3D(N, cfg, dtype=dtype)
         s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
         st = s.reset(obst)
-        worst = 0.0
+        worst, iters = 0.0, 0
         for i in range(steps):
             s.step(st, PLUME, dt)
             ps = s.pressure_stats()
             worst = max(worst, ps["true_rel_res"])
+            iters += ps["iters"]
         div = s.divergence(st)
         runs[dtype] = (st.data.double().cpu().numpy(), float(div.max()), worst)
-        _say(f"  {dtype}: worst true rel-res {worst:.2e}, max normalised |div| {float(div.max()):.3e}")
+        _say(f"  {dtype} (levels {cfg.mg_storage}): worst true rel-res {worst:.2e}, {iters / steps:.2f} PCG "
+             f"iterations per step, max normalised |div| {float(div.max()):.3e}")
         assert worst <= 1e-5
     a, b = runs[torch.float32][0], runs[torch.float64][0]
     for c, name in enumerate(("density", "vx", "vy", "vz", "T", "fuel")):
@@ -130,3 +137,29 @@ def test_fp32_against_fp64_run(cuda, hiplib):
         el2 = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
         _say(f"  FP32 vs FP64 {name}: max {emax:.3e}  L2 {el2:.3e}")
         assert el2 <= 1e-2, name
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16", 2e-3), ("bf16", 2e-2)])
+def test_mg16_precond(cuda, hiplib, storage, tol):
+    """A V-cycle with 16-bit level storage stays close to the FP32 one, and its standard-stencil
+    path is bit-identical to reading every coefficient from the (16-bit) level arrays."""
+    N = 64
+    _say(f"precond with {storage} levels N={N}")
+    solid = sphere_solid(N)
+    fm = oracle.mask(solid)
+    r = smooth_noise(N, 1, 9)[0] * (fm > 0)
+    sd = torch.from_numpy(solid).to(DEV)
+    rd = torch.from_numpy(np.ascontiguousarray(r)).to(DEV)
+    outs = {}
+    for kind, std in (("fp32", 1), (storage, 1), (storage, 0)):
+        s = Solver3D(N)
+        s.set_option(_lib.OPT_MG_STORAGE, _lib.MG_STORAGE[kind])
+        s.set_option(_lib.OPT_STD_STENCIL, std)
+        z = torch.empty(N, N, N, device=DEV)
+        _lib.check(hiplib.vfx_precond(s._ctx, _ptr(sd), _ptr(rd), _ptr(z), _stream()))
+        torch.cuda.synchronize()
+        outs[(kind, std)] = z.cpu()
+    assert torch.equal(outs[(storage, 1)], outs[(storage, 0)])
+    e = rel_err(outs[(storage, 1)].numpy(), outs[("fp32", 1)].numpy())
+    _say(f"  max rel difference to FP32 levels {e:.3e}")
+    assert e <= tol
