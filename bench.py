@@ -98,6 +98,74 @@ def fire_forces():
                                         fuel_rate=1.0, velocity=(0.0, 0.0, 0.0))])
 
 
+def plume_config(mg: str = "fp32"):
+    """Smoke plume past a sphere: buoyancy beta 1 per K, confinement 0.3, emitter noise
+    U[0.8, 1.2], pressure to ||r||/||b|| <= 1e-5 (true residual judged, up to 2 restarts)."""
+    from vfx.sim3d import SolverConfig3D
+    return SolverConfig3D(buoyancy_alpha=0.0, buoyancy_beta=1.0, vorticity_eps=0.3, pressure_rel_tol=1e-5,
+                          pressure_max_iters=200, pressure_restarts=2, emitter_noise=0.2, noise_seed=0,
+                          mg_storage=mg)
+
+
+def plume_scene():
+    """Emitter sphere r 0.08 m at (0.5, 0.1, 0.5) injecting density 1 per step and a 1 K excess
+    temperature target (both x U[0.8, 1.2] per cell); solid sphere r 0.15 m at the centre."""
+    from vfx.core.fields3d import Emitter3D, Forces3D, Obstacle3D
+    forces = Forces3D(emitters=[Emitter3D(pos=(0.5, 0.1, 0.5), radius=0.08, density_rate=60.0, temperature=294.0,
+                                          fuel_rate=0.0, velocity=(0.0, 0.0, 0.0))])
+    return forces, [Obstacle3D(kind="sphere", pos=(0.5, 0.5, 0.5), size=(0.15, 0.15, 0.15))]
+
+
+def precision_report(N: int, steps: int):
+    """Pressure, divergence and density of the FP32 solver (multigrid levels in FP32, FP16, BF16)
+    against the FP64 run of the same solver on the same plume, after `steps` steps."""
+    import numpy as np
+    import torch
+    from vfx.sim3d import Solver3D, _lib
+    forces, obst = plume_scene()
+    dt = 1.0 / 60.0
+    res = {}
+    for name, dtype, mg in (("fp64", torch.float64, "fp32"), ("fp32", torch.float32, "fp32"),
+                            ("fp16_levels", torch.float32, "fp16"), ("bf16_levels", torch.float32, "bf16")):
+        print(f"# {name}: {steps} steps at {N}^3", file=sys.stderr, flush=True)
+        s = Solver3D(N, plume_config(mg), dtype=dtype)
+        s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
+        st = s.reset(obst)
+        iters, worst = [], 0.0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            s.step(st, forces, dt)
+            ps = s.pressure_stats()
+            iters.append(ps["iters"])
+            worst = max(worst, ps["true_rel_res"])
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / steps
+        phi = torch.empty(N, N, N, device=st.data.device, dtype=dtype)
+        s._chk(s._api.vfx_get_phi(s._ctx, ctypes.c_void_p(phi.data_ptr()), s._stream()))
+        div = s.divergence(st)
+        torch.cuda.synchronize()
+        res[name] = {"density": st.data[0].double().cpu().numpy(), "pressure": phi.double().cpu().numpy(),
+                     "divergence": div.double().cpu().numpy(), "iters_mean": round(float(np.mean(iters)), 2),
+                     "iters_max": int(max(iters)), "worst_true_rel_res": float(f"{worst:.3g}"),
+                     "max_div": float(f"{float(div.max()):.3g}"), "ms_per_step_with_sync": round(ms, 2)}
+        del s, st
+        torch.cuda.empty_cache()
+    ref = res["fp64"]
+    out = {"report": f"{N}^3 smoke plume past a sphere, {steps} steps: FP32 solver vs FP64 run of the same solver",
+           "res": N, "steps": steps, "fp64": {k: v for k, v in ref.items() if not isinstance(v, np.ndarray)}}
+    for name in ("fp32", "fp16_levels", "bf16_levels"):
+        r = res[name]
+        row = {k: v for k, v in r.items() if not isinstance(v, np.ndarray)}
+        for f in ("pressure", "divergence", "density"):
+            a, b = r[f], ref[f]
+            scale = max(float(np.abs(b).max()), 1e-300)
+            row[f"{f}_max_err"] = float(f"{float(np.abs(a - b).max()) / scale:.3g}")
+            row[f"{f}_l2_err"] = float(f"{float(np.linalg.norm(a - b)) / max(float(np.linalg.norm(b)), 1e-300):.3g}")
+        out[name] = row
+    print(json.dumps(out))
+
+
 def cpu_baseline(st_data, st_solid, cfg, forces, dt, res, step):
     """The oracle's FP32 step on the host cores, one step of the same state (a bounded sample)."""
     import numpy as np
@@ -129,6 +197,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mg", default="fp32", choices=("fp32", "fp16", "bf16"),
                     help="multigrid level storage (the reference stores FP32)")
+    ap.add_argument("--precision-report", type=int, default=0, metavar="STEPS",
+                    help="instead of the bench line: FP32 (FP32/FP16/BF16 levels) against the FP64 run of the "
+                         "smoke plume past a sphere after STEPS steps at --res")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: one 256 MiB device copy (the scale reference), "
                          "2 steps, then one launch of every timed kernel; prints no bench line")
@@ -141,6 +212,9 @@ def main():
     import torch
     from vfx.sim3d import Solver3D, _lib
 
+    if args.precision_report > 0:
+        precision_report(args.res, args.precision_report)
+        return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     N = args.res
