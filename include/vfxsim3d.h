@@ -119,6 +119,35 @@ int vfx_time_kernel(vfx_ctx* ctx, int which, int reps, void* stream, float* ms_a
 /* Message of the last failed call on this thread. */
 int vfx_last_error(char* buf, int n);
 
+/* ---------------------------------------------------------------- learned preconditioner
+ * A 3-level 3D U-Net (16 / 32 / 64 channels, 3^3 convolutions, ReLU, average-pool down,
+ * nearest-neighbour up, skip concatenation) used as the preconditioner of flexible CG.  BF16
+ * weights and activations, FP32 accumulation on MFMA, FP32 master weights trained with Adam.
+ * No counterpart in the reference (its solver is MG-preconditioned CG only).  One network object
+ * serves one grid size (res % 4 == 0).  Parameters are flat FP32, per convolution in execution
+ * order: weight [cout][cin][3][3][3] (torch conv3d layout), then bias [cout]. */
+typedef struct vfx_net vfx_net;
+int vfx_net_create(int res, vfx_net** out);
+int vfx_net_destroy(vfx_net* net);
+int vfx_net_param_count(vfx_net* net, int* count);
+int vfx_net_get_params(vfx_net* net, float* host);   /* host array of param_count floats */
+int vfx_net_set_params(vfx_net* net, const float* host);   /* also resets the Adam state */
+/* raw network on a device input (2, res, res, res): channel 0 (residual), channel 1 (fluid 0/1)
+ * -> out (res^3), device pointers */
+int vfx_net_forward(vfx_net* net, const float* in, float* out, void* stream);
+/* after vfx_net_forward: d loss / d out (res^3, device) -> grad (param_count floats, device; may
+ * be NULL to keep it inside the network) */
+int vfx_net_backward(vfx_net* net, const float* grad_out, float* grad, void* stream);
+/* one Adam step with grad (device, or NULL for the last backward's), bias-corrected */
+int vfx_net_adam(vfx_net* net, const float* grad, float lr, float beta1, float beta2, float eps, void* stream);
+/* forward + energy-norm loss e.K e / phi.b (e = net(b) - phi, K of `solid` as the solver builds
+ * it) + backward + Adam, on one (b, phi) pair of the solver's grid size; *loss is written on the
+ * host (synchronises) when not NULL */
+int vfx_net_train_step(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* b, const float* phi, float lr,
+                       double* loss, void* stream);
+/* the preconditioner z = M(r): rms(r) * net(r / rms(r), fluid), 0 in solids */
+int vfx_net_apply(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* r, float* z, void* stream);
+
 /* ---------------------------------------------------------------- FP64 run of the same solver
  * The same entry points for FP64 state: fields, multigrid levels and every constant in double
  * (the parameter buffer stays the float layout above).  This is the "FP64 run of the same

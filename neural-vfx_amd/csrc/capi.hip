@@ -8,6 +8,12 @@
 
 #include "../../include/vfxsim3d.h"
 #include "sim3d_kernels.hpp"
+#if !VFX_F64
+#include "unet.hpp"
+struct vfx_net {
+    vfx::Net net;
+};
+#endif
 
 #if VFX_F64
 #define vfx_ctx vfx_ctx_f64   // the context of the FP64 build (include/vfxsim3d.h)
@@ -580,5 +586,34 @@ int VFX_FN(vfx_get_ops)(vfx_ctx* c, const real* solid, int level, real* diag, re
     if (off) CHECK_HIP(hipMemcpyAsync(off, L.off, m3 * 24 * sizeof(real), hipMemcpyDeviceToDevice, s), "off copy");
     return VFX_OK;
 }
+
+#if !VFX_F64
+int vfx_net_train_step(vfx_net* n, vfx_ctx* c, const real* solid, const real* b, const real* phi, float lr,
+                       double* loss, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!n || !solid || !b || !phi) return arg_error("null pointer");
+    if (n->net.N != c->N) return arg_error("network and solver grid sizes differ");
+    hipStream_t s = S(stream);
+    c->P.setup(solid, s);
+    net_train_step(n->net, c->g, b, phi, c->P.fm, lr, s);
+    CHECK_LAUNCH("network training step");
+    if (loss) {
+        CHECK_HIP(hipMemcpyAsync(loss, n->net.scal + 4, sizeof(double), hipMemcpyDeviceToHost, s), "loss copy");
+        CHECK_HIP(hipStreamSynchronize(s), "loss copy");
+    }
+    return VFX_OK;
+}
+
+int vfx_net_apply(vfx_net* n, vfx_ctx* c, const real* solid, const real* r, real* z, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!n || !solid || !r || !z) return arg_error("null pointer");
+    if (n->net.N != c->N) return arg_error("network and solver grid sizes differ");
+    hipStream_t s = S(stream);
+    c->P.setup(solid, s);
+    n->net.apply(r, c->P.fm, z, s);
+    CHECK_LAUNCH("network preconditioner");
+    return VFX_OK;
+}
+#endif
 
 }  // extern "C"

@@ -1,0 +1,64 @@
+// Host side of the learned pressure preconditioner: a 3-level 3D U-Net (16 / 32 / 64 channels,
+// 3^3 convolutions, ReLU, average-pool down, nearest up, skip concatenation), BF16 weights and
+// activations with FP32 accumulation, trained with Adam.  FP32 build only.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sim3d_kernels.hpp"
+
+namespace vfx {
+
+typedef __bf16 bf16_t;
+
+constexpr int NET_LAYERS = 11;
+
+struct NetLayer {
+    int cin, cout, cinp, coutp, relu;
+    size_t w_off, b_off;   // offsets of weight [cout][cin][27] and bias [cout] in the flat parameters
+    int kp, kpt;           // K of the forward and of the transposed GEMM (multiples of 32)
+    bf16_t* wf = nullptr;  // [coutp][kp]
+    bf16_t* wb = nullptr;  // [cinp][kpt]
+};
+
+struct Net {
+    int N = 0;             // fine grid size (multiple of 4)
+    NetLayer L[NET_LAYERS];
+    size_t nparam = 0;
+    void* pool = nullptr;
+    // FP32 parameters, Adam moments, gradient
+    float *p = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
+    // activations (channels-last BF16), levels 0 (N^3), 1 ((N/2)^3), 2 ((N/4)^3)
+    bf16_t *X, *A0, *C0, *D0a, *D0b;
+    bf16_t *P0, *A1, *C1, *D1a, *D1b;
+    bf16_t *P1, *A2, *B2;
+    float* out = nullptr;  // N^3 network output (FP32)
+    // gradient scratch per level
+    bf16_t *gA0, *gB0, *gA1, *gB1, *gC1, *gA2, *gB2, *gC2;
+    float* wpart = nullptr;   // weight-gradient partials
+    float* bpart = nullptr;   // bias-gradient partials
+    double* dpart = nullptr;  // FP64 dot partials
+    double* scal = nullptr;   // device scalars (scale, dots, loss)
+    float* work = nullptr;    // N^3 FP32 scratch (K e, dL/dz)
+    float* work2 = nullptr;
+    int adam_t = 0;
+
+    int create(int N);
+    void destroy();
+    void pack_weights(hipStream_t s);
+    // raw network: X (prepared by the caller) -> out
+    void forward(hipStream_t s);
+    // d loss / d out (FP32, N^3, in `work`) -> g (flat parameter gradient)
+    void backward(hipStream_t s);
+    void adam(float lr, float b1, float b2, float eps, hipStream_t s);
+    // preconditioner: z = rms(r) * net(r / rms(r), fluid) in fluid cells, 0 in solids
+    void apply(const real* r, const uint8_t* fm, real* z, hipStream_t s);
+};
+
+// one Adam step of the network on a (b, phi) pair in the energy norm of K (mask fm, geometry g of
+// a solver of the same size); the loss e.Ke / phi.b lands in n.scal[4]
+void net_train_step(Net& n, const Geom& g, const real* b, const real* phi, const uint8_t* fm, float lr, hipStream_t s);
+
+}  // namespace vfx

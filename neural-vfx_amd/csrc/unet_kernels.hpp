@@ -1,0 +1,296 @@
+// BF16 MFMA kernels of the learned pressure preconditioner (a 3-level 3D U-Net).
+//
+// Activations are channels-last (voxel-major, channels fastest) BF16 with the channel count
+// padded to a multiple of 16; weights are FP32 masters with BF16 copies in GEMM layout
+// [Cout_pad][K_pad], k = tap * Cin_pad + ci (tap = kz * 9 + ky * 3 + kx, the cross-correlation
+// of torch's conv3d with padding 1), K_pad = 27 * Cin_pad rounded up to 32.  Every convolution
+// is an implicit GEMM on v_mfma_f32_16x16x32_bf16 (M = voxels, N = output channels, K = taps x
+// input channels) with FP32 accumulation.  Rounding points (where a value is stored as BF16) are
+// explicit so that a CPU reference can reproduce them: conv outputs after bias + ReLU, pooled
+// values, gradients with respect to stored activations.
+#pragma once
+
+#include "sim3d_common.hpp"
+
+namespace vfx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16_t;
+
+constexpr int NET_THREADS = 256;   // 4 waves
+constexpr int WG_CHUNK = 32;       // voxels per MFMA K-step of the weight gradient
+
+__device__ __forceinline__ float bf(bf16_t v) { return (float)v; }
+__device__ __forceinline__ bf16_t tobf(float v) { return (bf16_t)v; }   // round to nearest even
+
+// tap t (0..26) -> offsets (dz, dy, dx) in {-1, 0, 1}
+__device__ __forceinline__ void tap_off(int t, int& dz, int& dy, int& dx) {
+    dz = t / 9 - 1;
+    dy = (t / 3) % 3 - 1;
+    dx = t % 3 - 1;
+}
+
+// Forward convolution 3^3, zero padding: out[v][co] = b[co] + sum_{t, ci} in[v + off(t)][ci] w[co][t][ci]
+// One wave computes 16 consecutive voxels x CT*16 output channels; 4 waves per workgroup.
+// Output: BF16 (channel stride ocs, offset ooff, ReLU if relu) or, if out32, FP32 channel 0.
+template <int CIN, int CT>
+__global__ void __launch_bounds__(NET_THREADS)
+k_conv3(int N, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, int kp, const float* __restrict__ bias,
+        int cout, int relu, bf16_t* __restrict__ out, int ocs, int ooff, float* __restrict__ out32) {
+    const int n3 = N * N * N;
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int v0 = ((int)blockIdx.x * (NET_THREADS / 64) + wave) * 16;
+    const int row = lane & 15, grp = lane >> 4;
+    const int v = v0 + row;
+    const bool vin = v < n3;
+    const int vc = vin ? v : 0;
+    const int x = vc % N, y = (vc / N) % N, z = vc / (N * N);
+    floatx4 acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int kk = 0; kk < kp; kk += 32) {
+        const int k = kk + grp * 8;
+        const int t = k / CIN, ci = k % CIN;
+        bf16x8 a = {};
+        if (t < 27 && vin) {
+            int dz, dy, dx;
+            tap_off(t, dz, dy, dx);
+            const int xx = x + dx, yy = y + dy, zz = z + dz;
+            if (xx >= 0 && yy >= 0 && zz >= 0 && xx < N && yy < N && zz < N)
+                a = *reinterpret_cast<const bf16x8*>(in + (size_t)((zz * N + yy) * N + xx) * CIN + ci);
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(w + (size_t)(c * 16 + row) * kp + k);
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[c], 0, 0, 0);
+        }
+    }
+    // lane holds rows (voxels) grp*4 + i, column (channel) row of each 16-wide tile
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int co = c * 16 + row;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int vo = v0 + grp * 4 + i;
+            if (vo >= n3 || co >= cout) continue;
+            float val = acc[c][i] + (bias ? bias[co] : 0.0f);
+            if (relu) val = fmaxf(val, 0.0f);
+            if (out32) {
+                if (co == 0) out32[vo] = val;
+            } else {
+                out[(size_t)vo * ocs + ooff + co] = tobf(val);
+            }
+        }
+    }
+}
+
+// Weight gradient of a 3^3 convolution for one tap t = blockIdx.y:
+//   part[g][t][co][ci] = sum over the voxels of group g = blockIdx.x of gpre[v][co] in[v + off(t)][ci]
+// The group's voxels go through LDS in chunks of 32, transposed so that both MFMA operands read 8
+// consecutive voxels of one channel (K = voxels).  Waves split the (co, ci) 16x16 tiles.
+template <int CINP, int COUTP>
+__global__ void __launch_bounds__(NET_THREADS)
+k_conv3_wgrad(int N, const bf16_t* __restrict__ in, const bf16_t* __restrict__ gpre, int vg, float* __restrict__ part) {
+    constexpr int TI = COUTP / 16, TJ = CINP / 16, T = TI * TJ, TW = (T + 3) / 4;
+    __shared__ __align__(16) bf16_t sg[COUTP][WG_CHUNK];
+    __shared__ __align__(16) bf16_t sx[CINP][WG_CHUNK];
+    const int n3 = N * N * N;
+    const int g = (int)blockIdx.x, t = (int)blockIdx.y;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6, row = lane & 15, grp = lane >> 4;
+    int dz, dy, dx;
+    tap_off(t, dz, dy, dx);
+    const int vbeg = g * vg, vend = min(vbeg + vg, n3);
+    floatx4 acc[TW];
+#pragma unroll
+    for (int i = 0; i < TW; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int v0 = vbeg; v0 < vend; v0 += WG_CHUNK) {
+        for (int idx = tid; idx < WG_CHUNK * COUTP; idx += NET_THREADS) {
+            const int vx = idx / COUTP, co = idx % COUTP, v = v0 + vx;
+            sg[co][vx] = v < vend ? gpre[(size_t)v * COUTP + co] : tobf(0.0f);
+        }
+        for (int idx = tid; idx < WG_CHUNK * CINP; idx += NET_THREADS) {
+            const int vx = idx / CINP, ci = idx % CINP, v = v0 + vx;
+            bf16_t val = tobf(0.0f);
+            if (v < vend) {
+                const int x = v % N + dx, y = (v / N) % N + dy, z = v / (N * N) + dz;
+                if (x >= 0 && y >= 0 && z >= 0 && x < N && y < N && z < N)
+                    val = in[(size_t)((z * N + y) * N + x) * CINP + ci];
+            }
+            sx[ci][vx] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TW; ++q) {
+            const int ti = wave + 4 * q;
+            if (ti < T) {
+                const int i = ti / TJ, j = ti % TJ;
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sg[i * 16 + row][grp * 8]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[j * 16 + row][grp * 8]);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[q], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < TW; ++q) {
+        const int ti = wave + 4 * q;
+        if (ti >= T) continue;
+        const int i = ti / TJ, j = ti % TJ;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = i * 16 + grp * 4 + r, ci = j * 16 + row;
+            part[(((size_t)g * 27 + t) * COUTP + co) * CINP + ci] = acc[q][r];
+        }
+    }
+}
+
+// gw[co][ci][t] (the FP32 parameter layout) = sum over groups (in order) of the partials
+__global__ void __launch_bounds__(NET_THREADS)
+k_wgrad_reduce(const float* __restrict__ part, int ng, int cinp, int coutp, int cin, int cout, float* __restrict__ gw) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= cout * cin * 27) return;
+    const int t = i % 27, ci = (i / 27) % cin, co = i / (27 * cin);
+    float s = 0.0f;
+    for (int g = 0; g < ng; ++g) s += part[(((size_t)g * 27 + t) * coutp + co) * cinp + ci];
+    gw[i] = s;
+}
+
+// bias gradient: block partials of sum_v gpre[v][co], then an ordered sum
+__global__ void __launch_bounds__(NET_THREADS)
+k_bias_part(int n3, const bf16_t* __restrict__ gpre, int coutp, int cout, int vpb, float* __restrict__ part) {
+    __shared__ float sh[NET_THREADS];
+    const int b = (int)blockIdx.x, tid = (int)threadIdx.x;
+    const int v0 = b * vpb, v1 = min(v0 + vpb, n3);
+    for (int co = 0; co < cout; ++co) {
+        float s = 0.0f;
+        for (int v = v0 + tid; v < v1; v += NET_THREADS) s += bf(gpre[(size_t)v * coutp + co]);
+        sh[tid] = s;
+        __syncthreads();
+        for (int w = NET_THREADS / 2; w > 0; w >>= 1) {
+            if (tid < w) sh[tid] += sh[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) part[(size_t)b * cout + co] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_bias_reduce(const float* __restrict__ part, int nb, int cout, float* __restrict__ gb) {
+    const int co = (int)threadIdx.x;
+    if (co >= cout) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * cout + co];
+    gb[co] = s;
+}
+
+// GEMM weights from the FP32 parameters w[co][ci][t]:
+//   fwd[co][t * cinp + ci] (Cout_pad rows, K_pad columns) and, for the input gradient (the
+//   transposed convolution), bwd[ci][t' * coutp + co] = w[co][ci][26 - t'].  Padding is zero.
+__global__ void __launch_bounds__(NET_THREADS)
+k_pack_weights(const float* __restrict__ w, int cin, int cout, int cinp, int coutp, int kp, int kpt,
+               bf16_t* __restrict__ fwd, bf16_t* __restrict__ bwd) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < coutp * kp) {
+        const int co = i / kp, k = i % kp, t = k / cinp, ci = k % cinp;
+        const float v = (co < cout && ci < cin && t < 27) ? w[((size_t)co * cin + ci) * 27 + t] : 0.0f;
+        fwd[i] = tobf(v);
+    }
+    if (bwd && i < cinp * kpt) {
+        const int ci = i / kpt, k = i % kpt, t = k / coutp, co = k % coutp;
+        const float v = (co < cout && ci < cin && t < 27) ? w[((size_t)co * cin + ci) * 27 + (26 - t)] : 0.0f;
+        bwd[i] = tobf(v);
+    }
+}
+
+// 2x2x2 average pool of channels [off, off + c) of src (stride cs) at fine size n -> dst (c channels)
+__global__ void __launch_bounds__(NET_THREADS)
+k_pool(int n, const bf16_t* __restrict__ src, int cs, int off, int c, bf16_t* __restrict__ dst) {
+    const int m = n / 2;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)m * m * m * c) return;
+    const int ch = (int)(i % c);
+    const size_t p = i / c;
+    const int X = (int)(p % m), Y = (int)((p / m) % m), Z = (int)(p / ((size_t)m * m));
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int x = 2 * X + (k & 1), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k >> 2);
+        s += bf(src[(size_t)((z * n + y) * n + x) * cs + off + ch]);
+    }
+    dst[i] = tobf(0.125f * s);
+}
+
+// nearest 2x upsample of src (c channels, coarse size m) into channels [off, off + c) of dst (stride cs)
+__global__ void __launch_bounds__(NET_THREADS)
+k_upsample(int m, const bf16_t* __restrict__ src, int c, bf16_t* __restrict__ dst, int cs, int off) {
+    const int n = 2 * m;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * n * n * c) return;
+    const int ch = (int)(i % c);
+    const size_t v = i / c;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((size_t)n * n));
+    dst[v * cs + off + ch] = src[(size_t)(((z / 2) * m + y / 2) * m + x / 2) * c + ch];
+}
+
+// gradient of the upsample: out[p][ch] = sum over the 8 children of g[child][ch] (g stride gs)
+__global__ void __launch_bounds__(NET_THREADS)
+k_upsample_bwd(int m, const bf16_t* __restrict__ g, int gs, int c, bf16_t* __restrict__ out) {
+    const int n = 2 * m;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)m * m * m * c) return;
+    const int ch = (int)(i % c);
+    const size_t p = i / c;
+    const int X = (int)(p % m), Y = (int)((p / m) % m), Z = (int)(p / ((size_t)m * m));
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int x = 2 * X + (k & 1), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k >> 2);
+        s += bf(g[(size_t)((z * n + y) * n + x) * gs + ch]);
+    }
+    out[i] = tobf(s);
+}
+
+// gradient at the pre-activation of a ReLU layer whose output y (stride ys, offset yo) is read
+// by the next layer (gradient g, c channels): gpre = y > 0 ? g : 0
+__global__ void __launch_bounds__(NET_THREADS)
+k_relu_bwd(size_t nv, const bf16_t* __restrict__ g, const bf16_t* __restrict__ y, int ys, int yo, int c,
+           bf16_t* __restrict__ gpre) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= nv * c) return;
+    const size_t v = i / c;
+    const int ch = (int)(i % c);
+    gpre[i] = bf(y[v * ys + yo + ch]) > 0.0f ? g[i] : tobf(0.0f);
+}
+
+// gradient of a skip activation y = cat[:, off:off + c] (stride cs) that also feeds the pool:
+//   g = gcat[v][off + ch] + gpool[parent][ch] / 8 (rounded to BF16), then the ReLU mask of y
+__global__ void __launch_bounds__(NET_THREADS)
+k_skip_bwd(int n, const bf16_t* __restrict__ gcat, const bf16_t* __restrict__ ycat, int cs, int off,
+           const bf16_t* __restrict__ gpool, int c, bf16_t* __restrict__ gpre) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * n * n * c) return;
+    const int m = n / 2;
+    const int ch = (int)(i % c);
+    const size_t v = i / c;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((size_t)n * n));
+    const size_t p = (size_t)((z / 2) * m + y / 2) * m + x / 2;
+    const float s = bf(gcat[v * cs + off + ch]) + 0.125f * bf(gpool[p * c + ch]);
+    gpre[i] = bf(ycat[v * cs + off + ch]) > 0.0f ? tobf(s) : tobf(0.0f);
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 parameters, bias-corrected with step t
+__global__ void __launch_bounds__(NET_THREADS)
+k_adam(int n, float* __restrict__ p, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
+       float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+}  // namespace vfx

@@ -47,6 +47,20 @@ SIGNATURES = {
 # the FP64 run of the same solver: every entry point again with the suffix _f64 (double fields;
 # the argument kinds are the same, pointers are untyped here)
 SIGNATURES.update({name + "_f64": args for name, args in list(SIGNATURES.items())})
+# the learned preconditioner (FP32 build only)
+_F = ctypes.c_float
+SIGNATURES.update({
+    "vfx_net_create": [_I, ctypes.POINTER(_P)],
+    "vfx_net_destroy": [_P],
+    "vfx_net_param_count": [_P, ctypes.POINTER(_I)],
+    "vfx_net_get_params": [_P, _P],
+    "vfx_net_set_params": [_P, _P],
+    "vfx_net_forward": [_P, _P, _P, _P],
+    "vfx_net_backward": [_P, _P, _P, _P],
+    "vfx_net_adam": [_P, _P, _F, _F, _F, _F, _P],
+    "vfx_net_train_step": [_P, _P, _P, _P, _P, _F, ctypes.POINTER(ctypes.c_double), _P],
+    "vfx_net_apply": [_P, _P, _P, _P, _P, _P],
+})
 
 _lib = None
 
@@ -85,7 +99,7 @@ class Api:
     def __init__(self, lib: ctypes.CDLL, f64: bool):
         self.f64 = f64
         for name in SIGNATURES:
-            if not name.endswith("_f64"):
+            if not name.endswith("_f64") and not name.startswith("vfx_net_"):
                 setattr(self, name, getattr(lib, name + ("_f64" if f64 else "")))
 
 

@@ -1,0 +1,137 @@
+"""Learned pressure preconditioner: a 3-level 3D U-Net on the library's BF16 MFMA kernels.
+
+The network maps a residual field to a pressure correction: z = rms(r) * net(r / rms(r), fluid),
+zero in solids.  It is trained on (divergence, pressure) pairs that the solver itself produces,
+in the energy norm of the solver's operator K (loss = e.K e / phi.K phi with e = net(b) - phi),
+with Adam on FP32 master weights; the convolutions run in BF16 with FP32 accumulation
+(libvfxsim3d.so, vfx_net_*).  Flexible CG uses it as its preconditioner, with plain MGPCG as the
+fallback (Solver3D with pressure_mode="learned").
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+# (cin, cout, relu) of the 11 convolutions in execution order: encoder 2->16->16 | pool |
+# 16->32->32 | pool | 32->64->64 | up, concat (64 + 32) | 96->32->32 | up, concat (32 + 16) |
+# 48->16->16 -> 1
+LAYERS = ((2, 16, True), (16, 16, True), (16, 32, True), (32, 32, True), (32, 64, True), (64, 64, True),
+          (96, 32, True), (32, 32, True), (48, 16, True), (16, 16, True), (16, 1, False))
+
+
+def param_count() -> int:
+    return sum(co * ci * 27 + co for ci, co, _ in LAYERS)
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights for the ReLU layers (std sqrt(2 / fan_in)), a small last layer, zero
+    biases; flat FP32 in the library's order (per layer: weight [cout][cin][3][3][3], bias)."""
+    g = torch.Generator().manual_seed(int(seed))
+    out = []
+    for i, (ci, co, relu) in enumerate(LAYERS):
+        std = math.sqrt(2.0 / (ci * 27)) if relu else 0.1 * math.sqrt(1.0 / (ci * 27))
+        out.append((torch.randn(co * ci * 27, generator=g, dtype=torch.float64) * std).numpy())
+        out.append(np.zeros(co))
+    return np.concatenate(out).astype(np.float32)
+
+
+def split_params(flat: np.ndarray):
+    """flat parameters -> [(weight (cout, cin, 3, 3, 3), bias (cout,)), ...]"""
+    res, k = [], 0
+    for ci, co, _ in LAYERS:
+        w = flat[k:k + co * ci * 27].reshape(co, ci, 3, 3, 3)
+        k += co * ci * 27
+        b = flat[k:k + co]
+        k += co
+        res.append((w, b))
+    return res
+
+
+class UNet:
+    """One network instance for grids of size res (res % 4 == 0)."""
+
+    def __init__(self, res: int, params: np.ndarray | None = None, seed: int = 0):
+        self.res = int(res)
+        self._lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        _lib.check(self._lib.vfx_net_create(self.res, ctypes.byref(self._h)), "UNet")
+        n = ctypes.c_int()
+        _lib.check(self._lib.vfx_net_param_count(self._h, ctypes.byref(n)))
+        self.nparam = int(n.value)
+        assert self.nparam == param_count()
+        self.set_params(init_params(seed) if params is None else params)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and getattr(self, "_lib", None) is not None:
+            self._lib.vfx_net_destroy(h)
+            self._h = None
+
+    # ---------------------------------------------------------------- parameters
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        _lib.check(self._lib.vfx_net_get_params(self._h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (self.nparam,)
+        _lib.check(self._lib.vfx_net_set_params(self._h, p.ctypes.data_as(ctypes.c_void_p)))
+
+    def save(self, path: str) -> None:
+        np.savez(path, params=self.params(), layers=np.array(LAYERS, dtype=np.int64))
+
+    @classmethod
+    def load(cls, path: str, res: int) -> "UNet":
+        with np.load(path, allow_pickle=False) as f:
+            return cls(res, params=f["params"])
+
+    # ---------------------------------------------------------------- raw network (tests)
+    @staticmethod
+    def _stream():
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def forward(self, inp: torch.Tensor) -> torch.Tensor:
+        """inp (2, res, res, res) float32 on the GPU -> (res, res, res) float32."""
+        N = self.res
+        assert inp.shape == (2, N, N, N) and inp.dtype == torch.float32 and inp.is_cuda
+        inp = inp.contiguous()
+        out = torch.empty(N, N, N, device=inp.device)
+        _lib.check(self._lib.vfx_net_forward(self._h, ctypes.c_void_p(inp.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                             self._stream()), "UNet.forward")
+        return out
+
+    def backward(self, grad_out: torch.Tensor) -> torch.Tensor:
+        """After forward(): d loss / d out -> flat parameter gradient (float32, GPU)."""
+        g = grad_out.contiguous().float()
+        grad = torch.empty(self.nparam, device=g.device)
+        _lib.check(self._lib.vfx_net_backward(self._h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
+                                              self._stream()), "UNet.backward")
+        return grad
+
+    def adam(self, grad: torch.Tensor | None, lr: float, beta1=0.9, beta2=0.999, eps=1e-8) -> None:
+        gp = ctypes.c_void_p(grad.contiguous().data_ptr()) if grad is not None else None
+        _lib.check(self._lib.vfx_net_adam(self._h, gp, float(lr), float(beta1), float(beta2), float(eps),
+                                          self._stream()), "UNet.adam")
+
+    # ---------------------------------------------------------------- with a solver
+    def train_step(self, solver, solid: torch.Tensor, b: torch.Tensor, phi: torch.Tensor, lr: float) -> float:
+        """One Adam step on the pair (b, phi) of `solver`'s grid; returns the energy-norm loss."""
+        loss = ctypes.c_double()
+        _lib.check(self._lib.vfx_net_train_step(self._h, solver._ctx, ctypes.c_void_p(solid.data_ptr()),
+                                                ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(phi.data_ptr()),
+                                                float(lr), ctypes.byref(loss), self._stream()), "UNet.train_step")
+        return float(loss.value)
+
+    def apply(self, solver, solid: torch.Tensor, r: torch.Tensor) -> torch.Tensor:
+        """The preconditioner z = M(r) on `solver`'s grid."""
+        z = torch.empty_like(r)
+        _lib.check(self._lib.vfx_net_apply(self._h, solver._ctx, ctypes.c_void_p(solid.data_ptr()),
+                                           ctypes.c_void_p(r.contiguous().data_ptr()), ctypes.c_void_p(z.data_ptr()),
+                                           self._stream()), "UNet.apply")
+        return z
