@@ -41,6 +41,12 @@ extern "C" {
                                     results are identical either way, default 1 */
 #define VFX_OPT_MG_STORAGE 5     /* multigrid level storage: 0 FP32 (default), 1 FP16, 2 BF16; FP32
                                     arithmetic either way, CG vectors FP32, dots and alpha/beta FP64 */
+#define VFX_OPT_FCG_MG 6         /* without a network: 1 runs the projection as flexible CG with the MG
+                                    V-cycle as preconditioner (the learned path's solver, checked against
+                                    PCG), 2 as flexible CG without a preconditioner (plain CG); 0 off */
+#define VFX_OPT_LEARNED_MAXIT 7  /* iteration budget of flexible CG before the MGPCG fallback (100) */
+#define VFX_OPT_LEARNED_SWEEPS 8 /* damped-Jacobi corrections z += 0.8 D^-1 (r - K z) after each network
+                                    application (0..8, default 2) */
 
 typedef struct vfx_ctx vfx_ctx;
 
@@ -76,7 +82,8 @@ int vfx_pressure_iters(vfx_ctx* ctx, int* out);
 
 /* Diagnostics of the last projection (waits for it): out[0] iterations (all restarts),
  * out[1] max|u*|, out[2] ||b||_2, out[3] recurrence ||r||_2, out[4] max|r|,
- * out[5] true ||b - K phi||_2 (when VFX_OPT_TRUE_RESIDUAL is on, else -1), out[6] restarts. */
+ * out[5] true ||b - K phi||_2 (when VFX_OPT_TRUE_RESIDUAL is on, else -1), out[6] restarts,
+ * out[7] flexible-CG iterations (learned path), out[8] 1 if it fell back to MGPCG. */
 int vfx_pressure_stats(vfx_ctx* ctx, double* out, int n);
 
 int vfx_set_option(vfx_ctx* ctx, int key, double value);
@@ -145,6 +152,15 @@ int vfx_net_adam(vfx_net* net, const float* grad, float lr, float beta1, float b
  * host (synchronises) when not NULL */
 int vfx_net_train_step(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* b, const float* phi, float lr,
                        double* loss, void* stream);
+/* attach a network (or NULL to detach) to a solver context of the same grid size: its
+ * projections then run flexible CG (Polak-Ribiere form) with the network as the preconditioner,
+ * falling back to plain MGPCG from the current pressure when ||r|| falls less than 2x over 3
+ * iterations, turns non-finite or VFX_OPT_LEARNED_MAXIT runs out; the true residual judges as for
+ * MGPCG.  The context does not own the network. */
+int vfx_ctx_set_net(vfx_ctx* ctx, vfx_net* net);
+/* ||r_k||_2 of every flexible-CG iteration of the last projection (convergence history): up to n
+ * values into out (host), the count into *count */
+int vfx_fcg_history(vfx_ctx* ctx, double* out, int n, int* count);
 /* the preconditioner z = M(r): rms(r) * net(r / rms(r), fluid), 0 in solids */
 int vfx_net_apply(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* r, float* z, void* stream);
 

@@ -1,12 +1,16 @@
 // C ABI of the sim3d library (include/vfxsim3d.h): context, buffers, parameter upload ring and
 // the host-side step / projection sequencing (Solver3D, vfx/sim3d/solver.py:79-459).
+#include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/vfxsim3d.h"
+#include "pressure_ops.hpp"
 #include "sim3d_kernels.hpp"
 #if !VFX_F64
 #include "unet.hpp"
@@ -63,6 +67,7 @@ __global__ void k_reset_cg(DevState* st) {
 using namespace vfx;
 
 constexpr int HOST_RING = 4;   // pinned parameter slots (solver.py:81, :331-359)
+constexpr int FCG_BLOCKS = 1024;   // blocks of the flexible-CG reductions (4 partial arrays)
 constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag
 
 struct vfx_ctx {
@@ -91,6 +96,16 @@ struct vfx_ctx {
     // options
     bool opt_true_res = false;
     int opt_restarts = 0;
+#if !VFX_F64
+    vfx_net* net = nullptr;     // learned preconditioner of flexible CG (not owned)
+    int fcg_mode = 0;           // flexible CG without a network: 1 MG V-cycle, 2 identity (plain CG)
+    int learned_maxit = 100;
+    int learned_sweeps = 2;     // damped-Jacobi corrections after each network application
+    int last_learned = 0;       // flexible-CG iterations of the last projection
+    bool last_fallback = false; // the last projection fell back to MGPCG
+    double last_fcg_rn = 0.0;   // ||r|| at which flexible CG converged
+    std::vector<double> fcg_hist;   // ||r_k|| of the last flexible-CG run
+#endif
     // diagnostics of the last projection
     int last_iters = 0, last_restarts = 0;
     bool last_true_valid = false;
@@ -177,15 +192,194 @@ static int run_cg(vfx_ctx* c, hipStream_t s, int* iters) {
     return VFX_OK;
 }
 
+#if !VFX_F64
+// ---------------------------------------------------------------- flexible CG
+// r.z, z.r_prev, r.r and max|r| block partials (FP64)
+__global__ void __launch_bounds__(256) k_fcg_dots(size_t n, const real* __restrict__ r, const real* __restrict__ z,
+                                                  const real* __restrict__ rp, double* __restrict__ part, int nb) {
+    __shared__ double sh[4];
+    double a = 0.0, b = 0.0, c = 0.0, m = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double ri = r[i], zi = z[i];
+        a += ri * zi;
+        b += zi * (double)rp[i];
+        c += ri * ri;
+        m = fmax(m, fabs(ri));
+    }
+    const double a1 = block_sum(a, sh), b1 = block_sum(b, sh), c1 = block_sum(c, sh), m1 = block_max(m, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a1;
+        part[nb + blockIdx.x] = b1;
+        part[2 * nb + blockIdx.x] = c1;
+        part[3 * nb + blockIdx.x] = m1;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fcg_dot(size_t n, const real* __restrict__ a, const real* __restrict__ b,
+                                                 double* __restrict__ part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// ordered sums (max for slot 3) of nk partial arrays of nb blocks -> out[0..nk)
+__global__ void __launch_bounds__(FIN_THREADS) k_fcg_fin(const double* __restrict__ part, int nb, int nk,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[FIN_THREADS / 64];
+    for (int j = 0; j < nk; ++j) {
+        const double v = j == 3 ? reduce_parts_max(part + (size_t)j * nb, nb, sh) : reduce_parts_sum(part + (size_t)j * nb, nb, sh);
+        if (threadIdx.x == 0) out[j] = v;
+        __syncthreads();
+    }
+}
+
+// z += omega D^-1 (r - t), t = K z (a damped-Jacobi correction of the network's output)
+__global__ void __launch_bounds__(256) k_fcg_jacobi(size_t n, const real* __restrict__ r, const real* __restrict__ t,
+                                                    const uint8_t* __restrict__ fm, real* __restrict__ z) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = fm[i];
+    if (m) z[i] = z[i] + OMEGA_JAC * (r[i] - t[i]) / kdiag(m);
+}
+
+__global__ void __launch_bounds__(256) k_fcg_p(size_t n, const real* __restrict__ z, real beta, real* __restrict__ p) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void __launch_bounds__(256) k_fcg_update(size_t n, real alpha, const real* __restrict__ p,
+                                                    const real* __restrict__ q, const real* __restrict__ r,
+                                                    real* __restrict__ phi, real* __restrict__ r_new) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    phi[i] = phi[i] + alpha * p[i];
+    r_new[i] = r[i] - alpha * q[i];
+}
+
+static int fcg_reduce(vfx_ctx* c, int nk, double* host, hipStream_t s) {
+    double* part = c->P.part;
+    double* out = c->P.part + (size_t)4 * FCG_BLOCKS;
+    hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(FIN_THREADS), 0, s, part, FCG_BLOCKS, nk, out);
+    CHECK_HIP(hipMemcpyAsync(host, out, sizeof(double) * nk, hipMemcpyDeviceToHost, s), "fcg scalars");
+    CHECK_HIP(hipStreamSynchronize(s), "fcg scalars");
+    return VFX_OK;
+}
+
+// Flexible CG (Polak-Ribiere form: beta = z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1}, Golub & Ye 1999)
+// preconditioned by the network (or, as a check, by the MG V-cycle), FP64 scalars.  Falls back to
+// MGPCG from the current phi when ||r|| falls less than 2x over 3 iterations, turns non-finite or
+// the learned iteration budget runs out.  Returns with phi (and r in P.R[0] semantics undefined).
+static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) {
+    Pressure& P = c->P;
+    const size_t n3 = c->g.n3;
+    real* r = c->hat;
+    real* rp = c->hat + n3;
+    real* p = c->hat + 2 * n3;
+    real* q = c->hat + 3 * n3;
+    real* z = c->hat + 4 * n3;
+    const unsigned nb = (unsigned)((n3 + 255) / 256);
+    const double rel_tol = (double)c->prm_last[P_REL_TOL];
+    const float tol = c->prm_last[P_TOL] > 1.0e-6f ? c->prm_last[P_TOL] : 1.0e-6f;
+    const int maxit = std::min(c->learned_maxit, (int)c->prm_last[P_MAXIT]);
+    DevState hs;
+    CHECK_HIP(hipMemcpyAsync(r, P.R[0], n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "fcg r0");
+    CHECK_HIP(hipMemcpyAsync(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "fcg state");
+    CHECK_HIP(hipStreamSynchronize(s), "fcg state");
+    const double bnorm = std::sqrt(hs.s[S_BB]);
+    const double thr = hs.s[S_THR];
+    double rz_prev = 0.0;
+    std::vector<double>& hist = c->fcg_hist;
+    hist.clear();
+    *converged = false;
+    c->last_learned = 0;
+    for (int k = 0; k < maxit; ++k) {
+        // z = M(r)
+        if (c->fcg_mode == 2) {
+            CHECK_HIP(hipMemcpyAsync(z, r, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "fcg identity");
+        } else if (c->fcg_mode == 1) {
+            CHECK_HIP(hipMemcpyAsync(P.R[0], r, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "fcg mg input");
+            hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
+            P.precond(0, s);
+            CHECK_HIP(hipMemcpyAsync(z, P.z, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "fcg mg output");
+        } else {
+            c->net->net.apply(r, P.fm, z, s);
+            for (int sw = 0; sw < c->learned_sweeps; ++sw) {
+                launch_kmul(c->g, z, P.fm, q, s);
+                hipLaunchKernelGGL(k_fcg_jacobi, dim3(nb), dim3(256), 0, s, n3, r, q, P.fm, z);
+            }
+        }
+        hipLaunchKernelGGL(k_fcg_dots, dim3(FCG_BLOCKS), dim3(256), 0, s, n3, r, z, rp, P.part, FCG_BLOCKS);
+        double d[4];
+        if (int rc = fcg_reduce(c, 4, d, s)) return rc;
+        const double rz = d[0], zrp = d[1], rr = d[2], mr = d[3];
+        const double rn = std::sqrt(rr);
+        hist.push_back(rn);
+        c->last_learned = k;
+        const bool conv = rel_tol > 0.0 ? rn <= rel_tol * bnorm : mr <= thr;
+        if (conv) {
+            *converged = true;
+            c->last_fcg_rn = rn;
+            return VFX_OK;
+        }
+        if (!std::isfinite(rz) || !std::isfinite(rn) || rz <= 0.0) return VFX_OK;   // fallback
+        if (k >= 3 && rn > 0.5 * hist[k - 3]) return VFX_OK;                           // stalled: fallback
+        const double beta = k == 0 ? 0.0 : (rz - zrp) / rz_prev;
+        rz_prev = rz;
+        hipLaunchKernelGGL(k_fcg_p, dim3(nb), dim3(256), 0, s, n3, z, (real)beta, p);
+        launch_kmul(c->g, p, P.fm, q, s);
+        hipLaunchKernelGGL(k_fcg_dot, dim3(FCG_BLOCKS), dim3(256), 0, s, n3, p, q, P.part);
+        double pq;
+        if (int rc = fcg_reduce(c, 1, &pq, s)) return rc;
+        if (!(pq > 0.0) || !std::isfinite(pq)) return VFX_OK;   // fallback
+        const double alpha = rz / pq;
+        hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(256), 0, s, n3, (real)alpha, p, q, r, P.phi, rp);
+        std::swap(r, rp);   // r <- r_new, rp <- r_old
+        CHECK_LAUNCH("flexible cg");
+        c->last_learned = k + 1;
+    }
+    return VFX_OK;
+}
+#endif
+
 static int stage_project(vfx_ctx* c, real* data, hipStream_t s) {
     Pressure& P = c->P;
     real* vel = data + c->g.n3;
     P.rhs(vel, c->prm, s);
     CHECK_LAUNCH("pressure rhs");
     int upper = 0;
-    int rc = run_cg(c, s, &upper);
-    if (rc) return rc;
-    P.finish(s);
+    int rc = VFX_OK;
+    bool need_pcg = true;
+#if !VFX_F64
+    c->last_learned = 0;
+    c->last_fallback = false;
+    if (c->net || c->fcg_mode) {
+        bool conv = false;
+        rc = run_fcg(c, vel, s, &conv);
+        if (rc) return rc;
+        need_pcg = !conv;
+        c->last_fallback = !conv;
+        if (!conv) {
+            // plain MGPCG from the current phi
+            float saved = c->prm_last[P_WARM];
+            c->prm_last[P_WARM] = 1.0f;
+            rc = upload(c, c->prm_last, c->prm, s);
+            c->prm_last[P_WARM] = saved;
+            if (rc) return rc;
+            P.rhs(vel, c->prm, s);
+            CHECK_LAUNCH("fallback rhs");
+        } else {
+            hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);   // no MGPCG iterations
+        }
+    }
+#endif
+    if (need_pcg) {
+        rc = run_cg(c, s, &upper);
+        if (rc) return rc;
+        P.finish(s);
+    }
     c->last_restarts = 0;
     c->last_true_valid = false;
     int total = 0;
@@ -257,7 +451,7 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     P.init_levels(N);
     P.geom = c->g;
     P.init_tiles();
-    P.nparts = nblocks(c->g) > P.ntiles ? nblocks(c->g) : P.ntiles;
+    P.nparts = std::max({nblocks(c->g), P.ntiles, 2 * FCG_BLOCKS});   // flexible CG uses 4 * FCG_BLOCKS + 4
     const size_t n3 = c->g.n3;
     // one pool: floats first (every array a multiple of 8 floats, so 32-byte aligned), then
     // doubles, then the mask
@@ -336,6 +530,20 @@ int VFX_FN(vfx_set_option)(vfx_ctx* c, int key, double value) {
                 if (c->P.lv[l].m <= cm) { c->P.S = l; break; }
             return VFX_OK;
         }
+#if !VFX_F64
+        case VFX_OPT_FCG_MG:
+            if (value < 0 || value > 2) return arg_error("flexible-CG mode must be 0, 1 or 2");
+            c->fcg_mode = (int)value;
+            return VFX_OK;
+        case VFX_OPT_LEARNED_SWEEPS:
+            if (value < 0 || value > 8) return arg_error("learned smoothing sweeps must be 0..8");
+            c->learned_sweeps = (int)value;
+            return VFX_OK;
+        case VFX_OPT_LEARNED_MAXIT:
+            if (value < 1) return arg_error("learned iteration budget must be >= 1");
+            c->learned_maxit = (int)value;
+            return VFX_OK;
+#endif
         case VFX_OPT_MG_STORAGE: {
             const int kind = (int)value;
             if (kind != MG_REAL && kind != MG_FP16 && kind != MG_BF16) {
@@ -449,7 +657,7 @@ int VFX_FN(vfx_pressure_iters)(vfx_ctx* c, int* out) {
 int VFX_FN(vfx_pressure_stats)(vfx_ctx* c, double* out, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (!out || n < 1) return arg_error("null pointer");
-    double v[7] = {0, 0, 0, 0, 0, -1, 0};
+    double v[9] = {0, 0, 0, 0, 0, -1, 0, 0, 0};
     if (c->projected) {
         CHECK_HIP(hipEventSynchronize(c->done_ev), "projection wait");
         DevState hs;
@@ -461,8 +669,13 @@ int VFX_FN(vfx_pressure_stats)(vfx_ctx* c, double* out, int n) {
         v[4] = hs.s[S_MAXR];
         v[5] = c->last_true_valid ? std::sqrt(hs.s[S_TRUE_RR]) : -1.0;
         v[6] = c->last_restarts;
+#if !VFX_F64
+        if ((c->net || c->fcg_mode) && !c->last_fallback && c->last_restarts == 0) v[3] = c->last_fcg_rn;
+        v[7] = c->last_learned;
+        v[8] = c->last_fallback ? 1.0 : 0.0;
+#endif
     }
-    for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
+    for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
     return VFX_OK;
 }
 
@@ -588,6 +801,20 @@ int VFX_FN(vfx_get_ops)(vfx_ctx* c, const real* solid, int level, real* diag, re
 }
 
 #if !VFX_F64
+int vfx_fcg_history(vfx_ctx* c, double* out, int n, int* count) {
+    if (int rc = check_ctx(c)) return rc;
+    if (count) *count = (int)c->fcg_hist.size();
+    for (int i = 0; out && i < n && i < (int)c->fcg_hist.size(); ++i) out[i] = c->fcg_hist[i];
+    return VFX_OK;
+}
+
+int vfx_ctx_set_net(vfx_ctx* c, vfx_net* n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n && n->net.N != c->N) return arg_error("network and solver grid sizes differ");
+    c->net = n;
+    return VFX_OK;
+}
+
 int vfx_net_train_step(vfx_net* n, vfx_ctx* c, const real* solid, const real* b, const real* phi, float lr,
                        double* loss, void* stream) {
     if (int rc = check_ctx(c)) return rc;

@@ -15,6 +15,7 @@ from ._build import LIB_PATH
 VFX_OK, VFX_E_ARG, VFX_E_VALUE, VFX_E_HIP = 0, 1, 2, 3
 STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
 OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL, OPT_MG_STORAGE = 1, 2, 3, 4, 5
+OPT_FCG_MG, OPT_LEARNED_MAXIT, OPT_LEARNED_SWEEPS = 6, 7, 8
 MG_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
 # kernel ids of vfx_time_kernel (include/vfxsim3d.h VFX_K_*); "" = unused id
 KERNELS = ("adv_sl", "adv_mc", "sources", "diffuse", "upd_matvec", "inv_post", "smooth0", "restrict0", "prolong0",
@@ -48,6 +49,7 @@ SIGNATURES = {
 # the argument kinds are the same, pointers are untyped here)
 SIGNATURES.update({name + "_f64": args for name, args in list(SIGNATURES.items())})
 # the learned preconditioner (FP32 build only)
+FP32_ONLY = ("vfx_ctx_set_net", "vfx_fcg_history")
 _F = ctypes.c_float
 SIGNATURES.update({
     "vfx_net_create": [_I, ctypes.POINTER(_P)],
@@ -60,6 +62,8 @@ SIGNATURES.update({
     "vfx_net_adam": [_P, _P, _F, _F, _F, _F, _P],
     "vfx_net_train_step": [_P, _P, _P, _P, _P, _F, ctypes.POINTER(ctypes.c_double), _P],
     "vfx_net_apply": [_P, _P, _P, _P, _P, _P],
+    "vfx_ctx_set_net": [_P, _P],
+    "vfx_fcg_history": [_P, _P, _I, ctypes.POINTER(_I)],
 })
 
 _lib = None
@@ -99,7 +103,7 @@ class Api:
     def __init__(self, lib: ctypes.CDLL, f64: bool):
         self.f64 = f64
         for name in SIGNATURES:
-            if not name.endswith("_f64") and not name.startswith("vfx_net_"):
+            if not name.endswith("_f64") and not name.startswith("vfx_net_") and name not in FP32_ONLY:
                 setattr(self, name, getattr(lib, name + ("_f64" if f64 else "")))
 
 

@@ -135,3 +135,98 @@ class UNet:
                                            ctypes.c_void_p(r.contiguous().data_ptr()), ctypes.c_void_p(z.data_ptr()),
                                            self._stream()), "UNet.apply")
         return z
+
+
+# ---------------------------------------------------------------------------- data and training
+def random_scene(seed: int):
+    """A seeded training / evaluation scene: 1-3 spherical emitters at uniform positions in the
+    lower half, 0-2 sphere obstacles of radius U[0.05, 0.2] m, buoyancy beta U[0.5, 2] per K.
+    Returns (Forces3D, obstacles, beta)."""
+    from vfx.core.fields3d import Emitter3D, Forces3D, Obstacle3D
+    g = np.random.default_rng(int(seed))
+    ems = []
+    for _ in range(int(g.integers(1, 4))):
+        pos = (float(g.uniform(0.15, 0.85)), float(g.uniform(0.05, 0.45)), float(g.uniform(0.15, 0.85)))
+        ems.append(Emitter3D(pos=pos, radius=float(g.uniform(0.05, 0.1)), density_rate=60.0,
+                             temperature=float(293.0 + g.uniform(1.0, 5.0)), fuel_rate=0.0,
+                             velocity=(0.0, float(g.uniform(0.0, 1.0)), 0.0)))
+    obst = []
+    for _ in range(int(g.integers(0, 3))):
+        r = float(g.uniform(0.05, 0.2))
+        obst.append(Obstacle3D(kind="sphere", pos=(float(g.uniform(0.25, 0.75)), float(g.uniform(0.45, 0.8)),
+                                                   float(g.uniform(0.25, 0.75))), size=(r, r, r)))
+    return Forces3D(emitters=ems), obst, float(g.uniform(0.5, 2.0))
+
+
+def scene_config(beta: float, seed: int, rel_tol: float = 1e-5):
+    from .config import SolverConfig3D
+    return SolverConfig3D(buoyancy_alpha=0.0, buoyancy_beta=beta, vorticity_eps=0.3, pressure_rel_tol=rel_tol,
+                          pressure_max_iters=200, pressure_restarts=2, emitter_noise=0.2, noise_seed=int(seed))
+
+
+def generate_pairs(res: int, seeds, steps: int, every: int = 1, warmup: int = 3):
+    """(solid, b, phi) training pairs from the plain solver: phi is the converged pressure of a
+    step (relative residual 1e-6) of a random scene, b = K phi (the right-hand side phi solves
+    exactly).  Returns a list of GPU tensor triples."""
+    from .solver import Solver3D
+    pairs = []
+    for sd in seeds:
+        forces, obst, beta = random_scene(sd)
+        s = Solver3D(res, scene_config(beta, sd, rel_tol=1e-6))
+        st = s.reset(obst)
+        for i in range(steps):
+            s.step(st, forces, 1.0 / 60.0)
+            if i >= warmup and (i - warmup) % every == 0:
+                phi = torch.empty(res, res, res, device=st.data.device)
+                s._chk(s._api.vfx_get_phi(s._ctx, ctypes.c_void_p(phi.data_ptr()), s._stream()))
+                b = torch.empty_like(phi)
+                s._chk(s._api.vfx_kmul(s._ctx, ctypes.c_void_p(st.solid.data_ptr()), ctypes.c_void_p(phi.data_ptr()),
+                                       ctypes.c_void_p(b.data_ptr()), s._stream()))
+                pairs.append((st.solid.clone(), b, phi))
+        torch.cuda.synchronize()
+    return pairs
+
+
+def noise_pairs(res: int, solids, count: int, seed: int = 0):
+    """(solid, K v, v) pairs with v multi-scale random noise in the fluid cells (octaves 4^3 to
+    res^3 with equal weight): right-hand sides whose content reaches the high frequencies that
+    the residuals of later CG iterations carry.  Exactly consistent, like generate_pairs."""
+    import torch.nn.functional as F
+    from .solver import Solver3D
+    s = Solver3D(res)
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    out = []
+    for i in range(int(count)):
+        solid = solids[i % len(solids)]
+        v = torch.zeros(1, 1, res, res, res)
+        size = 4
+        while size <= res:
+            z = torch.randn(1, 1, size, size, size, generator=g)
+            v += F.interpolate(z, size=(res,) * 3, mode="trilinear", align_corners=False)
+            size *= 2
+        v = (v[0, 0].to(solid.device) * (solid < 0.5)).contiguous()
+        b = torch.empty_like(v)
+        s._chk(s._api.vfx_kmul(s._ctx, ctypes.c_void_p(solid.data_ptr()), ctypes.c_void_p(v.data_ptr()),
+                               ctypes.c_void_p(b.data_ptr()), s._stream()))
+        out.append((solid, b, v))
+    torch.cuda.synchronize()
+    return out
+
+
+def train(net: UNet, pairs, iters: int, lr: float = 1e-3, seed: int = 0, lr_decay: float = 0.3, log=None):
+    """Adam on the energy-norm loss over `pairs` (shuffled each epoch); the learning rate decays
+    by lr_decay at 60 % and 85 % of the iterations.  Returns the loss per iteration."""
+    from .solver import Solver3D
+    solver = Solver3D(net.res)
+    g = np.random.default_rng(int(seed))
+    order = []
+    losses = []
+    for it in range(int(iters)):
+        if not order:
+            order = list(g.permutation(len(pairs)))
+        solid, b, phi = pairs[order.pop()]
+        rate = lr * (lr_decay if it >= 0.6 * iters else 1.0) * (lr_decay if it >= 0.85 * iters else 1.0)
+        losses.append(net.train_step(solver, solid, b, phi, rate))
+        if log is not None and (it % 100 == 0 or it == iters - 1):
+            log(f"train {it}: loss {float(np.mean(losses[-50:])):.4f}")
+    return losses

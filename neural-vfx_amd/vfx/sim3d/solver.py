@@ -110,6 +110,7 @@ class Solver3D:
         with torch.cuda.device(torch.device(self.device)):
             self._chk(self._api.vfx_ctx_create(self.res, ctypes.byref(self._ctx)), "Solver3D")
         self._h = np.zeros(P_SIZE, np.float32)
+        self._net = None
         self._apply_options()
 
     def __del__(self):
@@ -131,6 +132,17 @@ class Solver3D:
     def set_option(self, key: int, value: float) -> None:
         """Library option (vfxsim3d.h VFX_OPT_*): true-residual report, restarts, coarse size."""
         self._chk(self._api.vfx_set_option(self._ctx, int(key), float(value)))
+
+    def set_preconditioner(self, net) -> None:
+        """Attach a learned preconditioner (vfx.sim3d.learned.UNet of this grid size) or None: the
+        projection then runs flexible CG with it, falling back to plain MGPCG when the residual
+        reduction stalls (pressure_stats()["fallback"])."""
+        if self._f64:
+            raise ValueError("the learned preconditioner runs in the FP32 solver")
+        if net is not None and net.res != self.res:
+            raise ValueError(f"network grid {net.res} != solver grid {self.res}")
+        self._chk(self._lib.vfx_ctx_set_net(self._ctx, net._h if net is not None else None))
+        self._net = net
 
     @property
     def version(self) -> str:
@@ -234,15 +246,23 @@ class Solver3D:
     def pressure_stats(self) -> dict:
         """Diagnostics of the last projection (synchronizes): iterations, max|u*|, ||b||, the
         recurrence ||r||, max|r|, the true ||b - K phi|| (if enabled) and restarts."""
-        v = (ctypes.c_double * 7)()
-        self._chk(self._api.vfx_pressure_stats(self._ctx, v, 7))
-        keys = ("iters", "max_u", "b_norm", "r_norm", "max_r", "true_r_norm", "restarts")
+        v = (ctypes.c_double * 9)()
+        self._chk(self._api.vfx_pressure_stats(self._ctx, v, 9))
+        keys = ("iters", "max_u", "b_norm", "r_norm", "max_r", "true_r_norm", "restarts", "learned_iters", "fallback")
         d = dict(zip(keys, list(v)))
-        d["iters"] = int(d["iters"])
-        d["restarts"] = int(d["restarts"])
+        for k in ("iters", "restarts", "learned_iters", "fallback"):
+            d[k] = int(d[k])
         d["rel_res"] = d["r_norm"] / d["b_norm"] if d["b_norm"] > 0 else 0.0
         d["true_rel_res"] = d["true_r_norm"] / d["b_norm"] if d["b_norm"] > 0 and d["true_r_norm"] >= 0 else -1.0
         return d
+
+    def fcg_history(self) -> list:
+        """||r_k|| of each flexible-CG iteration of the last projection (learned path)."""
+        n = ctypes.c_int()
+        self._chk(self._lib.vfx_fcg_history(self._ctx, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_double * max(n.value, 1))()
+        self._chk(self._lib.vfx_fcg_history(self._ctx, buf, n.value, ctypes.byref(n)))
+        return list(buf)[:n.value]
 
     # ------------------------------------------------------------------ internals
     def _check(self, state: FieldState3D):
@@ -255,8 +275,12 @@ class Solver3D:
         c = self.config
         if c.pressure_mode == "pcg":
             return ()
+        if c.pressure_mode == "learned":
+            if getattr(self, "_net", None) is None and hasattr(self, "_ctx"):
+                raise RuntimeError("pressure_mode='learned' needs Solver3D.set_preconditioner(net)")
+            return ()
         if c.pressure_mode != "fused":
-            raise ValueError(f"pressure_mode must be 'pcg' or 'fused', got {c.pressure_mode!r}")
+            raise ValueError(f"pressure_mode must be 'pcg', 'learned' or 'fused', got {c.pressure_mode!r}")
         raise NotImplementedError("pressure_mode='fused' (the reference's opt-in stationary MG) is not built")
 
     def _stream(self) -> ctypes.c_void_p:
