@@ -166,6 +166,90 @@ def precision_report(N: int, steps: int):
     print(json.dumps(out))
 
 
+BATCH_SIZES = (48, 64, 96, 128)
+
+
+def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: bool = True):
+    """Batched frames: nsims independent simulations with N drawn by seed from 48/64/96/128, seeded
+    scenes (1-3 emitters, 0-2 sphere obstacles, beta U[0.5, 2]), MGPCG to rel-res 1e-5 with the
+    adaptive iteration budget.  Aggregate cell-steps/s and per-simulation latency of graph frames
+    (after the autotuner), beside the same frames enqueued directly and the simulations stepped
+    one by one with Solver3D.step."""
+    import numpy as np
+    import torch
+    from vfx.sim3d import BatchSolver3D
+    from vfx.sim3d.learned import random_scene, scene_config
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.choice(BATCH_SIZES, nsims)]
+    scenes = [random_scene(1000 * seed + i) for i in range(nsims)]
+    cfgs = [dataclasses.replace(scene_config(beta, 1000 * seed + i), pressure_restarts=0)
+            for i, (_, _, beta) in enumerate(scenes)]
+    forces = [f for f, _, _ in scenes]
+    dt = 1.0 / 60.0
+    cells = float(sum(n ** 3 for n in sizes))
+    batch = BatchSolver3D(sizes, cfgs, streams=4, graph=True)
+    states = batch.reset([o for _, o, _ in scenes])
+    print(f"# batch: {nsims} sims, sizes {sorted(sizes)}, {cells / 1e6:.1f} M cells", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        batch.step(states, forces, dt)
+        batch.wait()
+    rep = None
+    if tune:
+        t0 = time.perf_counter()
+        rep = batch.tune(states, forces, dt, reps=2)
+        rep["seconds"] = round(time.perf_counter() - t0, 2)
+        print(f"# tuned: {rep}", file=sys.stderr, flush=True)
+        for _ in range(3):   # budgets settle, graph captured again
+            batch.step(states, forces, dt)
+            batch.wait()
+
+    def run(graph: bool):
+        lat, fms, its, unconv = [], [], [], 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            batch.step(states, forces, dt, graph=graph)
+            info = batch.wait()
+            lat.extend(info["latency_ms"].tolist())
+            fms.append(info["frame_ms"])
+            its.extend(info["iters"].tolist())
+            unconv += int((~info["converged"]).sum())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return {"cell_steps_per_s": cells * frames / wall, "wall_ms_per_frame": 1000.0 * wall / frames,
+                "gpu_ms_per_frame": float(np.median(fms)), "latency_p50_ms": float(np.percentile(lat, 50)),
+                "latency_p99_ms": float(np.percentile(lat, 99)), "iters_mean": float(np.mean(its)),
+                "unconverged": unconv}
+
+    g = run(True)
+    print(f"# graph frames: {g}", file=sys.stderr, flush=True)
+    e = run(False)
+    print(f"# direct frames: {e}", file=sys.stderr, flush=True)
+    # one by one with the host-polled Solver3D.step
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat = []
+    for _ in range(frames):
+        tf = time.perf_counter()
+        for s, st, f in zip(batch.solvers, states, forces):
+            s.step(st, f, dt)
+            torch.cuda.synchronize()
+            lat.append(1000.0 * (time.perf_counter() - tf))
+    wall = time.perf_counter() - t0
+    seq = {"cell_steps_per_s": cells * frames / wall, "wall_ms_per_frame": 1000.0 * wall / frames,
+           "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99))}
+    print(f"# one by one: {seq}", file=sys.stderr, flush=True)
+    r3 = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+    out = {"metric": f"aggregate cell-steps/s, {nsims} sims N in {{48,64,96,128}}^3 (seed {seed}), MGPCG to "
+                     f"rel-res 1e-5, graph frames",
+           "value": round(g["cell_steps_per_s"], 1), "unit": "cell-steps/s", "n_gpus": 1, "frames": frames,
+           "warmup": warmup, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"batched frames: {nsims} sims, sizes {sorted(sizes)}", "cells": int(cells)},
+           "graph": r3(g), "direct": r3(e), "one_by_one": r3(seq),
+           "tune": rep, "vs_one_by_one": round(g["cell_steps_per_s"] / seq["cell_steps_per_s"], 2)}
+    print(json.dumps(out))
+
+
 def cpu_baseline(st_data, st_solid, cfg, forces, dt, res, step):
     """The oracle's FP32 step on the host cores, one step of the same state (a bounded sample)."""
     import numpy as np
@@ -200,6 +284,10 @@ def main():
     ap.add_argument("--precision-report", type=int, default=0, metavar="STEPS",
                     help="instead of the bench line: FP32 (FP32/FP16/BF16 levels) against the FP64 run of the "
                          "smoke plume past a sphere after STEPS steps at --res")
+    ap.add_argument("--batch", type=int, default=0, metavar="FRAMES",
+                    help="instead of the bench line: batched frames of 32 simulations of sizes 48-128 "
+                         "(--warmup frames, autotuner, FRAMES timed frames)")
+    ap.add_argument("--no-tune", action="store_true", help="--batch without the autotuner")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: one 256 MiB device copy (the scale reference), "
                          "2 steps, then one launch of every timed kernel; prints no bench line")
@@ -214,6 +302,10 @@ def main():
 
     if args.precision_report > 0:
         precision_report(args.res, args.precision_report)
+        return
+    if args.batch > 0:
+        torch.cuda.set_device(torch.device("cuda:0"))
+        batch_report(args.batch, args.warmup, args.seed, tune=not args.no_tune)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

@@ -47,6 +47,11 @@ extern "C" {
 #define VFX_OPT_LEARNED_MAXIT 7  /* iteration budget of flexible CG before the MGPCG fallback (100) */
 #define VFX_OPT_LEARNED_SWEEPS 8 /* damped-Jacobi corrections z += 0.8 D^-1 (r - K z) after each network
                                     application (0..8, default 2) */
+#define VFX_OPT_CG_BUDGET 9      /* 0 (default): the host polls the device's PCG stop flag each
+                                    iteration; B > 0: exactly min(B, P_MAXIT) iterations are enqueued
+                                    and the ones after the stop test are no-ops on the device */
+#define VFX_OPT_CELLS_PER_THREAD 10  /* z-cells per thread of the cell kernels: 1 (default), 2, 4, 8;
+                                    results equal up to FP64 reduction order */
 
 typedef struct vfx_ctx vfx_ctx;
 
@@ -170,6 +175,41 @@ int vfx_net_apply(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* r
  * solver" the FP32 path is measured against; it has no counterpart in the reference, which is
  * FP32 only.  Each function behaves as its FP32 namesake above; contexts of the two precisions
  * are distinct types. */
+/* ---- batched frames (FP32): many independent simulations of any sizes as one frame ----
+ * No reference counterpart: the reference steps one Solver3D per call (solver.py:146-155) and
+ * graph-captures one solver's step (solver.py:388-459).  Each frame enqueues every simulation's
+ * step (vfx_step's stages) on one of `streams` HIP streams forked from and joined back into the
+ * caller's stream, directly or as one captured hipGraph replayed while the pointers, budgets and
+ * launch shapes stay the same.  Projections are MGPCG with a fixed per-simulation iteration
+ * budget (the device stops at the budget or at convergence; no host round trip inside a frame;
+ * no restarts); vfx_batch_wait grows a budget that was used up.  The contexts stay usable on
+ * their own between frames; a context may belong to one batch at a time. */
+typedef struct vfx_batch vfx_batch;
+#define VFX_TUNE_MAX_SIZES 8
+#define VFX_TUNE_REPORT 46   /* doubles of the autotuner report (layout at vfx_batch_tune) */
+int vfx_batch_create(vfx_ctx* const* ctxs, int n, int streams, vfx_batch** out);
+int vfx_batch_destroy(vfx_batch* batch);
+/* PCG iteration budget of simulation `sim` (-1: all); default 10 */
+int vfx_batch_set_budget(vfx_batch* batch, int sim, int budget);
+int vfx_batch_get_budget(vfx_batch* batch, int* budget);   /* n ints */
+/* One frame: simulation i steps data[i] (6,N_i,N_i,N_i), solid[i], parameters prm_host[i * vfx_p_size()
+ * ...]; graph 1 replays (capturing when needed) the frame's hipGraph, 0 enqueues it directly.
+ * Returns once enqueued. */
+int vfx_batch_step(vfx_batch* batch, float* const* data, const float* const* solid, const float* prm_host,
+                   int graph, void* stream);
+/* Waits for the last frame.  latency_ms[i]: frame start to the end of simulation i's step (GPU
+ * wall clock); iters[i]: PCG iterations; converged[i]: the projection met its tolerance within
+ * the budget; frame_ms: frame start to the last simulation's end.  Any output may be NULL. */
+int vfx_batch_wait(vfx_batch* batch, float* latency_ms, int* iters, int* converged, float* frame_ms);
+/* Autotuner: per grid size, VFX_OPT_CELLS_PER_THREAD in {1,2,4} x VFX_OPT_COARSE_M in {4,8,16} by the
+ * median of `reps` timed steps of the size's first simulation; then the stream count in {1,2,4,8}
+ * by the median frame time of `reps` graph frames.  Every simulation is restored to its state
+ * before the call (data and pressure).  report (VFX_TUNE_REPORT doubles, may be NULL): [0] number
+ * of sizes; per size k at 1 + 5k: N, cells per thread, coarse size, best ms, ms of the setting
+ * before the call; at 41: chosen streams, then frame ms at 1, 2, 4, 8 streams. */
+int vfx_batch_tune(vfx_batch* batch, float* const* data, const float* const* solid, const float* prm_host, int reps,
+                   void* stream, double* report, int report_len);
+
 typedef struct vfx_ctx_f64 vfx_ctx_f64;
 int vfx_p_size_f64(void);
 int vfx_last_error_f64(char* buf, int n);

@@ -9,19 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/vfxsim3d.h"
-#include "pressure_ops.hpp"
-#include "sim3d_kernels.hpp"
-#if !VFX_F64
-#include "unet.hpp"
-struct vfx_net {
-    vfx::Net net;
-};
-#endif
-
-#if VFX_F64
-#define vfx_ctx vfx_ctx_f64   // the context of the FP64 build (include/vfxsim3d.h)
-#endif
+#include "ctx.hpp"
 
 namespace vfx {
 
@@ -66,51 +54,6 @@ __global__ void k_reset_cg(DevState* st) {
 
 using namespace vfx;
 
-constexpr int HOST_RING = 4;   // pinned parameter slots (solver.py:81, :331-359)
-constexpr int FCG_BLOCKS = 1024;   // blocks of the flexible-CG reductions (4 partial arrays)
-constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag
-
-struct vfx_ctx {
-    int N = 0;
-    Geom g;
-    real inv_dx = RL(0.0), inv2dx = RL(0.0);
-    // device buffers
-    void* pool = nullptr;
-    real *hat = nullptr, *adv = nullptr, *omega = nullptr, *omega_s = nullptr, *force = nullptr;
-    real *tmp_a = nullptr, *tmp_b = nullptr;
-    float* prm = nullptr;      // parameters of the step
-    float* prm_adv = nullptr;  // parameters of the standalone advect()
-    double* mass_part = nullptr;
-    DevState* st = nullptr;
-    Pressure P;
-    // host side
-    float* ring = nullptr;   // HOST_RING * P_SIZE pinned floats
-    hipEvent_t ring_ev[HOST_RING] = {};
-    bool ring_used[HOST_RING] = {};
-    int ring_i = 0;
-    float prm_last[P_SIZE];
-    int* stop_pin = nullptr;  // STOP_RING pinned ints
-    hipEvent_t stop_ev[STOP_RING] = {};
-    hipEvent_t done_ev = nullptr;
-    bool projected = false;
-    // options
-    bool opt_true_res = false;
-    int opt_restarts = 0;
-#if !VFX_F64
-    vfx_net* net = nullptr;     // learned preconditioner of flexible CG (not owned)
-    int fcg_mode = 0;           // flexible CG without a network: 1 MG V-cycle, 2 identity (plain CG)
-    int learned_maxit = 100;
-    int learned_sweeps = 2;     // damped-Jacobi corrections after each network application
-    int last_learned = 0;       // flexible-CG iterations of the last projection
-    bool last_fallback = false; // the last projection fell back to MGPCG
-    double last_fcg_rn = 0.0;   // ||r|| at which flexible CG converged
-    std::vector<double> fcg_hist;   // ||r_k|| of the last flexible-CG run
-#endif
-    // diagnostics of the last projection
-    int last_iters = 0, last_restarts = 0;
-    bool last_true_valid = false;
-};
-
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 #define CHECK_HIP(expr, what)                                    \
@@ -122,12 +65,12 @@ static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); 
         if (!launch_ok(what)) return VFX_E_HIP;                  \
     } while (0)
 
-static int arg_error(const std::string& m) {
+int vfx::arg_error(const std::string& m) {
     set_error(m);
     return VFX_E_ARG;
 }
 
-static int upload(vfx_ctx* c, const float* host, float* dst, hipStream_t s) {
+int vfx::upload(vfx_ctx* c, const float* host, float* dst, hipStream_t s) {
     const int i = c->ring_i;
     c->ring_i = (i + 1) % HOST_RING;
     if (c->ring_used[i]) CHECK_HIP(hipEventSynchronize(c->ring_ev[i]), "parameter ring wait");
@@ -139,7 +82,7 @@ static int upload(vfx_ctx* c, const float* host, float* dst, hipStream_t s) {
     return VFX_OK;
 }
 
-static int validate_prm(const float* prm) {
+int vfx::validate_prm(const float* prm) {
     if (prm == nullptr) return arg_error("prm_host is NULL");
     if (prm[P_N_EM] < 0 || prm[P_N_EM] > MAX_EMITTERS || prm[P_N_IMP] < 0 || prm[P_N_IMP] > MAX_IMPULSES) {
         set_error("at most 32 emitters / 32 impulses");
@@ -148,7 +91,7 @@ static int validate_prm(const float* prm) {
     return VFX_OK;
 }
 
-static int stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s) {
+int vfx::stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s) {
     c->P.setup(solid, s);
     CHECK_LAUNCH("obstacle setup");
     launch_advect(c->g, 6, data, data + c->g.n3, c->prm, c->inv_dx, c->hat, c->adv, c->P.fm, c->mass_part, s);
@@ -157,7 +100,7 @@ static int stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s
     return VFX_OK;
 }
 
-static int stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
+int vfx::stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
     launch_sources_diffuse(c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, data, s);
     launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega, c->omega_s, c->force, c->tmp_a, c->tmp_b,
                        s);
@@ -174,6 +117,15 @@ static int run_cg(vfx_ctx* c, hipStream_t s, int* iters) {
     const int maxit = (int)c->prm_last[P_MAXIT];
     P.iteration(0, c->prm, s);
     CHECK_LAUNCH("pcg init");
+    if (c->cg_budget > 0) {
+        // fixed budget: the iterations after the device's stop test are no-ops on the device, and
+        // the device stops by itself at P_MAXIT, so the host never waits
+        const int last = std::min(c->cg_budget, std::max(maxit, 1));
+        for (int k = 1; k <= last; ++k) P.iteration(k, c->prm, s);
+        CHECK_LAUNCH("pcg iterations");
+        *iters = last;
+        return VFX_OK;
+    }
     CHECK_HIP(hipMemcpyAsync(&c->stop_pin[0], &c->st->stop_at, sizeof(int), hipMemcpyDeviceToHost, s), "stop read");
     CHECK_HIP(hipEventRecord(c->stop_ev[0], s), "stop event");
     int k = 1;
@@ -344,7 +296,7 @@ static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) 
 }
 #endif
 
-static int stage_project(vfx_ctx* c, real* data, hipStream_t s) {
+int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
     Pressure& P = c->P;
     real* vel = data + c->g.n3;
     P.rhs(vel, c->prm, s);
@@ -355,6 +307,10 @@ static int stage_project(vfx_ctx* c, real* data, hipStream_t s) {
 #if !VFX_F64
     c->last_learned = 0;
     c->last_fallback = false;
+    if (in_frame && (c->net || c->fcg_mode)) {
+        set_error("a batched frame runs MGPCG only (the flexible-CG loop waits on the host)");
+        return VFX_E_VALUE;
+    }
     if (c->net || c->fcg_mode) {
         bool conv = false;
         rc = run_fcg(c, vel, s, &conv);
@@ -389,7 +345,7 @@ static int stage_project(vfx_ctx* c, real* data, hipStream_t s) {
             P.true_residual(vel, s);
             CHECK_LAUNCH("true residual");
             c->last_true_valid = true;
-            if (!(c->opt_restarts > c->last_restarts && rel_tol > 0.0)) break;
+            if (in_frame || !(c->opt_restarts > c->last_restarts && rel_tol > 0.0)) break;
             DevState hs;
             CHECK_HIP(hipMemcpyAsync(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "state read");
             CHECK_HIP(hipStreamSynchronize(s), "state sync");
@@ -412,7 +368,7 @@ static int stage_project(vfx_ctx* c, real* data, hipStream_t s) {
     c->last_iters = total;   // plus DevState::it of the final run, read lazily
     P.apply(vel, s);
     CHECK_LAUNCH("pressure apply");
-    CHECK_HIP(hipEventRecord(c->done_ev, s), "done event");
+    if (!in_frame) CHECK_HIP(hipEventRecord(c->done_ev, s), "done event");
     c->projected = true;
     return VFX_OK;
 }
@@ -544,6 +500,17 @@ int VFX_FN(vfx_set_option)(vfx_ctx* c, int key, double value) {
             c->learned_maxit = (int)value;
             return VFX_OK;
 #endif
+        case VFX_OPT_CG_BUDGET:
+            if (value < 0 || value > 1000) return arg_error("CG budget must be 0 (host-polled) or 1..1000");
+            c->cg_budget = (int)value;
+            return VFX_OK;
+        case VFX_OPT_CELLS_PER_THREAD: {
+            const int zc = (int)value;
+            if (zc != 1 && zc != 2 && zc != 4 && zc != 8) return arg_error("cells per thread must be 1, 2, 4 or 8");
+            c->g = make_geom(c->N, zc);   // fewer blocks than zc = 1: every partial buffer still fits
+            c->P.geom = c->g;
+            return VFX_OK;
+        }
         case VFX_OPT_MG_STORAGE: {
             const int kind = (int)value;
             if (kind != MG_REAL && kind != MG_FP16 && kind != MG_BF16) {

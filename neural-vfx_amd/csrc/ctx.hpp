@@ -1,0 +1,83 @@
+// The per-simulation context of the library (vfx_ctx of include/vfxsim3d.h) and the step stages
+// that the single-simulation entry points (capi.hip) and the batched frame (batch.hip) share.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/vfxsim3d.h"
+#include "pressure_ops.hpp"
+#include "sim3d_kernels.hpp"
+#if !VFX_F64
+#include "unet.hpp"
+struct vfx_net {
+    vfx::Net net;
+};
+#endif
+
+#if VFX_F64
+#define vfx_ctx vfx_ctx_f64   // the context of the FP64 build (include/vfxsim3d.h)
+#define vfx_batch vfx_batch_f64
+#endif
+
+using namespace vfx;
+
+constexpr int HOST_RING = 4;   // pinned parameter slots (solver.py:81, :331-359)
+constexpr int FCG_BLOCKS = 1024;   // blocks of the flexible-CG reductions (4 partial arrays)
+constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag
+
+struct vfx_ctx {
+    int N = 0;
+    Geom g;
+    real inv_dx = RL(0.0), inv2dx = RL(0.0);
+    // device buffers
+    void* pool = nullptr;
+    real *hat = nullptr, *adv = nullptr, *omega = nullptr, *omega_s = nullptr, *force = nullptr;
+    real *tmp_a = nullptr, *tmp_b = nullptr;
+    float* prm = nullptr;      // parameters of the step
+    float* prm_adv = nullptr;  // parameters of the standalone advect()
+    double* mass_part = nullptr;
+    DevState* st = nullptr;
+    Pressure P;
+    // host side
+    float* ring = nullptr;   // HOST_RING * P_SIZE pinned floats
+    hipEvent_t ring_ev[HOST_RING] = {};
+    bool ring_used[HOST_RING] = {};
+    int ring_i = 0;
+    float prm_last[P_SIZE];
+    int* stop_pin = nullptr;  // STOP_RING pinned ints
+    hipEvent_t stop_ev[STOP_RING] = {};
+    hipEvent_t done_ev = nullptr;
+    bool projected = false;
+    // options
+    bool opt_true_res = false;
+    int opt_restarts = 0;
+    int cg_budget = 0;   // > 0: exactly this many PCG iterations are enqueued, with no host polling
+#if !VFX_F64
+    vfx_net* net = nullptr;     // learned preconditioner of flexible CG (not owned)
+    int fcg_mode = 0;           // flexible CG without a network: 1 MG V-cycle, 2 identity (plain CG)
+    int learned_maxit = 100;
+    int learned_sweeps = 2;     // damped-Jacobi corrections after each network application
+    int last_learned = 0;       // flexible-CG iterations of the last projection
+    bool last_fallback = false; // the last projection fell back to MGPCG
+    double last_fcg_rn = 0.0;   // ||r|| at which flexible CG converged
+    std::vector<double> fcg_hist;   // ||r_k|| of the last flexible-CG run
+#endif
+    // diagnostics of the last projection
+    int last_iters = 0, last_restarts = 0;
+    bool last_true_valid = false;
+};
+
+namespace vfx {
+
+int arg_error(const std::string& m);
+int upload(vfx_ctx* c, const float* host, float* dst, hipStream_t s);
+int validate_prm(const float* prm);
+// the three stages of Solver3D.step (solver.py:146-155) on stream s, parameters already in c->prm
+int stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s);
+int stage_forces(vfx_ctx* c, real* data, hipStream_t s);
+// in_frame: part of a batched frame (fixed CG budget, no host synchronisation, no restarts, the
+// caller records done_ev)
+int stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame = false);
+
+}  // namespace vfx

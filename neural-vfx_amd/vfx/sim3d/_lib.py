@@ -16,6 +16,8 @@ VFX_OK, VFX_E_ARG, VFX_E_VALUE, VFX_E_HIP = 0, 1, 2, 3
 STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
 OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL, OPT_MG_STORAGE = 1, 2, 3, 4, 5
 OPT_FCG_MG, OPT_LEARNED_MAXIT, OPT_LEARNED_SWEEPS = 6, 7, 8
+OPT_CG_BUDGET, OPT_CELLS_PER_THREAD = 9, 10
+TUNE_MAX_SIZES, TUNE_REPORT = 8, 46
 MG_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
 # kernel ids of vfx_time_kernel (include/vfxsim3d.h VFX_K_*); "" = unused id
 KERNELS = ("adv_sl", "adv_mc", "sources", "diffuse", "upd_matvec", "inv_post", "smooth0", "restrict0", "prolong0",
@@ -48,8 +50,9 @@ SIGNATURES = {
 # the FP64 run of the same solver: every entry point again with the suffix _f64 (double fields;
 # the argument kinds are the same, pointers are untyped here)
 SIGNATURES.update({name + "_f64": args for name, args in list(SIGNATURES.items())})
-# the learned preconditioner (FP32 build only)
-FP32_ONLY = ("vfx_ctx_set_net", "vfx_fcg_history")
+# the learned preconditioner and the batched frames (FP32 build only)
+FP32_ONLY = ("vfx_ctx_set_net", "vfx_fcg_history", "vfx_batch_create", "vfx_batch_destroy", "vfx_batch_set_budget",
+             "vfx_batch_get_budget", "vfx_batch_step", "vfx_batch_wait", "vfx_batch_tune")
 _F = ctypes.c_float
 SIGNATURES.update({
     "vfx_net_create": [_I, ctypes.POINTER(_P)],
@@ -64,6 +67,13 @@ SIGNATURES.update({
     "vfx_net_apply": [_P, _P, _P, _P, _P, _P],
     "vfx_ctx_set_net": [_P, _P],
     "vfx_fcg_history": [_P, _P, _I, ctypes.POINTER(_I)],
+    "vfx_batch_create": [_P, _I, _I, ctypes.POINTER(_P)],
+    "vfx_batch_destroy": [_P],
+    "vfx_batch_set_budget": [_P, _I, _I],
+    "vfx_batch_get_budget": [_P, _P],
+    "vfx_batch_step": [_P, _P, _P, _P, _I, _P],
+    "vfx_batch_wait": [_P, _P, _P, _P, ctypes.POINTER(_F)],
+    "vfx_batch_tune": [_P, _P, _P, _P, _I, _P, _P, _I],
 })
 
 _lib = None
