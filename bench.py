@@ -166,6 +166,108 @@ def precision_report(N: int, steps: int):
     print(json.dumps(out))
 
 
+def net_flops(N: int):
+    """(algorithmic, executed) FLOPs of one U-Net application at N^3: 2 * 27 * cin * cout per output
+    voxel of each convolution, with the nominal channel counts and with the MFMA-padded ones."""
+    from vfx.sim3d.learned import LAYERS
+    level = (0, 0, 1, 1, 2, 2, 1, 1, 0, 0, 0)
+    pad = lambda c: max(16, (c + 15) // 16 * 16)
+    alg = exe = 0.0
+    for (ci, co, _), lv in zip(LAYERS, level):
+        vox = (N >> lv) ** 3
+        alg += 2.0 * 27 * ci * co * vox
+        exe += 2.0 * 27 * pad(ci) * pad(co) * vox
+    return alg, exe
+
+
+def learned_report(train_res: int, eval_res, iters: int):
+    """Learned flexible CG against plain MGPCG: the U-Net trained at train_res on solver pairs
+    (seeded scenes 20-27) plus multi-scale noise pairs, then both solvers on the same
+    pre-projection state of the held-out scene 99 at each eval_res (cold start, ||r||/||b|| <=
+    1e-5, true residual judged): pressure ms, iterations, fallback; the network's application ms
+    and its MFMA rate."""
+    import numpy as np
+    import torch
+    from vfx.core.fields3d import FieldState3D
+    from vfx.sim3d import Solver3D, _lib
+    from vfx.sim3d.learned import UNet, generate_pairs, noise_pairs, random_scene, scene_config, train
+    from vfx.sim3d.solver import pack_params
+    say = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    pairs = generate_pairs(train_res, seeds=range(20, 28), steps=14, every=1)
+    pairs += noise_pairs(train_res, [p[0] for p in pairs[::11]], count=len(pairs), seed=5)
+    net = UNet(train_res, seed=1)
+    losses = train(net, pairs, iters=iters, lr=2e-3, seed=0, log=say)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    say(f"trained at {train_res}^3 on {len(pairs)} pairs, {iters} its in {t_train:.1f} s")
+    params = net.params()
+    del net, pairs
+    torch.cuda.empty_cache()
+    out = {"report": "learned flexible CG (U-Net + 2 Jacobi corrections, MGPCG fallback) vs plain MGPCG, "
+                     "held-out scene 99, cold start, rel-res 1e-5",
+           "train": {"res": train_res, "pairs": "8 scenes x 11 steps + as many noise pairs", "iters": iters,
+                     "seconds": round(t_train, 1), "loss_first50": round(float(np.mean(losses[:50])), 4),
+                     "loss_last50": round(float(np.mean(losses[-50:])), 4)}}
+    forces, obst, beta = random_scene(99)
+    for N in eval_res:
+        cfg = dataclasses.replace(scene_config(beta, 99), pressure_warm_start=False, pressure_rel_tol=1e-5,
+                                  pressure_max_iters=400, pressure_restarts=2)
+        s = Solver3D(N, cfg)
+        st = s.reset(obst)
+        for _ in range(6):
+            s.step(st, forces, 1.0 / 60.0)
+        h = pack_params(cfg, N, forces, 1.0 / 60.0, st.meta.step)
+        for stage in (_lib.STAGE_ADVECT, _lib.STAGE_FORCES):
+            s._chk(s._api.vfx_step_stage(s._ctx, stage, ctypes.c_void_p(st.data.data_ptr()),
+                                          ctypes.c_void_p(st.solid.data_ptr()),
+                                          h.ctypes.data_as(ctypes.c_void_p) if stage == 0 else None, s._stream()))
+        torch.cuda.synchronize()
+        netN = UNet(N, params=params)
+        row = {}
+        for name in ("mgpcg", "learned"):
+            s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
+            s.set_preconditioner(netN if name == "learned" else None)
+            times = []
+            for r in range(3):
+                work = FieldState3D(data=st.data.clone(), solid=st.solid, meta=st.meta)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                s.project(work)
+                ps = s.pressure_stats()   # synchronizes
+                times.append(1000.0 * (time.perf_counter() - t1))
+            row[name] = {"pressure_ms": round(float(np.median(times)), 3), "mgpcg_iters": ps["iters"],
+                         "learned_iters": ps["learned_iters"], "fallback": ps["fallback"],
+                         "true_rel_res": float(f"{ps['true_rel_res']:.3g}")}
+            if name == "learned":
+                row[name]["history_first"] = [float(f"{x / max(ps['b_norm'], 1e-300):.2e}")
+                                              for x in s.fcg_history()[:8]]
+        s.set_preconditioner(None)
+        # the network's application alone
+        r = torch.randn(N, N, N, device="cuda") * (st.solid < 0.5)
+        for _ in range(2):
+            netN.apply(s, st.solid, r)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 10
+        ev[0].record()
+        for _ in range(reps):
+            netN.apply(s, st.solid, r)
+        ev[1].record()
+        ev[1].synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / reps
+        alg, exe = net_flops(N)
+        row["net_apply_ms"] = round(ms, 3)
+        row["net_tflops_alg"] = round(alg / 1e12, 4)
+        row["net_mfma_tflops_per_s"] = round(exe / ms / 1e9, 1)
+        row["net_mfma_frac_of_bf16_dense_peak"] = round(exe / ms / 1e9 / 2500.0, 4)
+        row["speedup_learned_vs_mgpcg"] = round(row["mgpcg"]["pressure_ms"] / row["learned"]["pressure_ms"], 3)
+        say(f"{N}^3: {row}")
+        out[str(N)] = row
+        del s, st, netN, r
+        torch.cuda.empty_cache()
+    print(json.dumps(out))
+
+
 BATCH_SIZES = (48, 64, 96, 128)
 
 
@@ -204,7 +306,7 @@ def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: boo
             batch.wait()
 
     def run(graph: bool):
-        lat, fms, its, unconv = [], [], [], 0
+        lat, fms, its, unconv, misses = [], [], [], 0, []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(frames):
@@ -214,12 +316,14 @@ def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: boo
             fms.append(info["frame_ms"])
             its.extend(info["iters"].tolist())
             unconv += int((~info["converged"]).sum())
+            for i in np.nonzero(~info["converged"])[0]:
+                misses.append((int(i), sizes[int(i)], int(info["iters"][i]), float(f"{info['rel_res'][i]:.3g}")))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         return {"cell_steps_per_s": cells * frames / wall, "wall_ms_per_frame": 1000.0 * wall / frames,
                 "gpu_ms_per_frame": float(np.median(fms)), "latency_p50_ms": float(np.percentile(lat, 50)),
                 "latency_p99_ms": float(np.percentile(lat, 99)), "iters_mean": float(np.mean(its)),
-                "unconverged": unconv}
+                "unconverged": unconv, "unconverged_first": misses[:6], "budgets": batch.budgets.tolist()}
 
     g = run(True)
     print(f"# graph frames: {g}", file=sys.stderr, flush=True)
@@ -288,6 +392,10 @@ def main():
                     help="instead of the bench line: batched frames of 32 simulations of sizes 48-128 "
                          "(--warmup frames, autotuner, FRAMES timed frames)")
     ap.add_argument("--no-tune", action="store_true", help="--batch without the autotuner")
+    ap.add_argument("--learned", type=int, default=0, metavar="ITERS",
+                    help="instead of the bench line: train the U-Net ITERS Adam steps at --train-res, then "
+                         "learned flexible CG vs MGPCG at 64^3, 128^3 and 256^3")
+    ap.add_argument("--train-res", type=int, default=64)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: one 256 MiB device copy (the scale reference), "
                          "2 steps, then one launch of every timed kernel; prints no bench line")
@@ -302,6 +410,10 @@ def main():
 
     if args.precision_report > 0:
         precision_report(args.res, args.precision_report)
+        return
+    if args.learned > 0:
+        torch.cuda.set_device(torch.device("cuda:0"))
+        learned_report(args.train_res, (64, 128, 256), args.learned)
         return
     if args.batch > 0:
         torch.cuda.set_device(torch.device("cuda:0"))

@@ -199,8 +199,10 @@ int vfx_batch_step(vfx_batch* batch, float* const* data, const float* const* sol
                    int graph, void* stream);
 /* Waits for the last frame.  latency_ms[i]: frame start to the end of simulation i's step (GPU
  * wall clock); iters[i]: PCG iterations; converged[i]: the projection met its tolerance within
- * the budget; frame_ms: frame start to the last simulation's end.  Any output may be NULL. */
-int vfx_batch_wait(vfx_batch* batch, float* latency_ms, int* iters, int* converged, float* frame_ms);
+ * the budget; rel_res[i]: the recurrence ||r|| / ||b|| at the stop; frame_ms: frame start to the
+ * last simulation's end.  Any output may be NULL. */
+int vfx_batch_wait(vfx_batch* batch, float* latency_ms, int* iters, int* converged, double* rel_res,
+                   float* frame_ms);
 /* Autotuner: per grid size, VFX_OPT_CELLS_PER_THREAD in {1,2,4} x VFX_OPT_COARSE_M in {4,8,16} by the
  * median of `reps` timed steps of the size's first simulation; then the stream count in {1,2,4,8}
  * by the median frame time of `reps` graph frames.  Every simulation is restored to its state

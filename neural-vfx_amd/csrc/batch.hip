@@ -248,7 +248,7 @@ int frame(vfx_batch* b, float* const* data, const float* const* solid, const flo
 }
 
 // wait for the last frame; per-simulation results; grow the budgets that were too small
-int wait_frame(vfx_batch* b, float* latency_ms, int* iters, int* converged, float* frame_ms) {
+int wait_frame(vfx_batch* b, float* latency_ms, int* iters, int* converged, double* rel_res, float* frame_ms) {
     if (!b->frames) return arg_error("no frame was enqueued");
     if (!hip_ok(hipEventSynchronize(b->done), "frame wait")) return VFX_E_HIP;
     const unsigned long long t0 = b->pin_ts[b->n];
@@ -263,6 +263,7 @@ int wait_frame(vfx_batch* b, float* latency_ms, int* iters, int* converged, floa
         const bool conv = rt > 0.0 ? std::sqrt(o[2]) <= rt * std::sqrt(o[3]) : o[4] <= o[5];
         if (iters) iters[i] = it;
         if (converged) converged[i] = conv ? 1 : 0;
+        if (rel_res) rel_res[i] = o[3] > 0.0 ? std::sqrt(o[2] / o[3]) : 0.0;
         const int cap = std::max(1, (int)b->user_maxit[i]);
         int want = b->budget[i];
         if (!conv && it >= b->budget[i]) want = b->budget[i] + 2 * BUDGET_MARGIN;   // ran out
@@ -480,9 +481,9 @@ int vfx_batch_step(vfx_batch* b, float* const* data, const float* const* solid, 
     return frame(b, data, solid, prm_host, graph, reinterpret_cast<hipStream_t>(stream));
 }
 
-int vfx_batch_wait(vfx_batch* b, float* latency_ms, int* iters, int* converged, float* frame_ms) {
+int vfx_batch_wait(vfx_batch* b, float* latency_ms, int* iters, int* converged, double* rel_res, float* frame_ms) {
     if (!b) return arg_error("null batch");
-    return wait_frame(b, latency_ms, iters, converged, frame_ms);
+    return wait_frame(b, latency_ms, iters, converged, rel_res, frame_ms);
 }
 
 int vfx_batch_tune(vfx_batch* b, float* const* data, const float* const* solid, const float* prm_host, int reps,
@@ -569,7 +570,7 @@ int vfx_batch_tune(vfx_batch* b, float* const* data, const float* const* solid, 
                 if (int rc = snap.restore(b, i, data, main)) return rc;
             if (int rc = frame(b, data, solid, prm_host, 1, main)) return rc;
             float fm = 0.0f;
-            if (int rc = wait_frame(b, nullptr, nullptr, nullptr, &fm)) return rc;
+            if (int rc = wait_frame(b, nullptr, nullptr, nullptr, nullptr, &fm)) return rc;
             if (r > 0) t.push_back(fm);
         }
         const float ms = median(t);

@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256) k_fcg_jacobi(size_t n, const real* __rest
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int m = fm[i];
-    if (m) z[i] = z[i] + OMEGA_JAC * (r[i] - t[i]) / kdiag(m);
+    if (kopen(m)) z[i] = z[i] + OMEGA_JAC * (r[i] - t[i]) / kdiag(m);
 }
 
 __global__ void __launch_bounds__(256) k_fcg_p(size_t n, const real* __restrict__ z, real beta, real* __restrict__ p) {

@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
                 for (int dx = 0; dx < 2; ++dx) {
                     const int m = (hm[it][rw] >> (8 * dx)) & 255;
                     rv[dx] = rv[dx] * in;
-                    zv[dx] = m ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0);
+                    zv[dx] = kopen(m) ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0);
                 }
                 st2(sz, h, zv[0], zv[1]);
                 st2(sr, h, rv[0], rv[1]);
@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
 #pragma unroll
                 for (int dx = 0; dx < 2; ++dx) {
                     const int m = (hm[it][rw] >> (8 * dx)) & 255;
-                    const real acc = inv_sum(m ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0), hx, rw >> 1, rw & 1, dx);
+                    const real acc = inv_sum(kopen(m) ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0), hx, rw >> 1, rw & 1, dx);
                     zt[dx] = m ? acc : RL(0.0);
                 }
                 const int h = hb[it].h0 + row_h(rw);
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
         for (int dx = 0; dx < 2; ++dx) {
             const int m = sm[h + dx];
             const real rv = sr[h + dx];
-            zv[dx] = m ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : RL(0.0);
+            zv[dx] = kopen(m) ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : RL(0.0);
             s += (double)rv * (double)zv[dx];
             mx = fmax(mx, fabs((double)rv));
             rr += (double)rv * (double)rv;

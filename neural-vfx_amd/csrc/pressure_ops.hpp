@@ -70,6 +70,12 @@ __device__ __forceinline__ real kmul_xyz(const real* __restrict__ p, const uint8
 // any open face and 0 otherwise (small integers, exact in real)
 __device__ __forceinline__ real kdiag(int m) { return (real)(2 * __popc((m | (m >> 1)) & 0x15)); }
 
+// A fluid cell with at least one open face.  A fluid cell sealed in by solids on all six sides has
+// diag(K) = 0 and a zero row and column of K (its right-hand side is 0 too): its Jacobi terms are
+// 0 here.  The reference divides 0 / 0 there (pressure.py:671, :702), and the NaN stops its PCG at
+// iteration 0 with the velocity left unprojected.
+__device__ __forceinline__ bool kopen(int m) { return (m & 63) != 0; }
+
 // div2 (pressure.py:94-106) of the velocity channels vel[0..2] at cell (x, y, z), mask m
 __device__ __forceinline__ real div2_xyz(const real* __restrict__ vel, int n3, int m, int N, int x, int y, int z,
                                           int i) {

@@ -72,7 +72,7 @@ SIGNATURES.update({
     "vfx_batch_set_budget": [_P, _I, _I],
     "vfx_batch_get_budget": [_P, _P],
     "vfx_batch_step": [_P, _P, _P, _P, _I, _P],
-    "vfx_batch_wait": [_P, _P, _P, _P, ctypes.POINTER(_F)],
+    "vfx_batch_wait": [_P, _P, _P, _P, _P, ctypes.POINTER(_F)],
     "vfx_batch_tune": [_P, _P, _P, _P, _I, _P, _P, _I],
 })
 

@@ -90,16 +90,20 @@ class BatchSolver3D:
 
     def wait(self) -> dict:
         """Wait for the last frame: per-simulation latency (ms from the frame's start to the end of
-        the simulation's step), PCG iterations, convergence flags, and the frame's GPU time."""
+        the simulation's step), PCG iterations, convergence flags, recurrence ||r||/||b||, and the
+        frame's GPU time."""
         lat = np.zeros(self.n, np.float32)
         its = np.zeros(self.n, np.int32)
         conv = np.zeros(self.n, np.int32)
+        rr = np.zeros(self.n, np.float64)
         fm = ctypes.c_float()
         _lib.check(self._lib.vfx_batch_wait(self._h, lat.ctypes.data_as(ctypes.c_void_p),
                                             its.ctypes.data_as(ctypes.c_void_p), conv.ctypes.data_as(ctypes.c_void_p),
-                                            ctypes.byref(fm)), "BatchSolver3D.wait")
+                                            rr.ctypes.data_as(ctypes.c_void_p), ctypes.byref(fm)),
+                   "BatchSolver3D.wait")
         self._pending = False
-        return {"latency_ms": lat, "iters": its, "converged": conv.astype(bool), "frame_ms": float(fm.value)}
+        return {"latency_ms": lat, "iters": its, "converged": conv.astype(bool), "rel_res": rr,
+                "frame_ms": float(fm.value)}
 
     @property
     def budgets(self) -> np.ndarray:

@@ -801,7 +801,9 @@ static void F(precond)(int N, const int* fm, F(levels_t) * L, const R* r, R* z0,
     const R omega = (R)OMEGA_JAC;
     size_t n3 = (size_t)N * N * N;
 #pragma omp parallel for schedule(static)
-    for (size_t i = 0; i < n3; ++i) z0[i] = fm[i] ? omega * r[i] / F(kdiag)(fm[i]) : (R)0;
+    /* (fm & 63): a fluid cell sealed in by solids has diag(K) = 0 and a zero row and column of K;
+     * its Jacobi terms are 0 (the reference divides 0 / 0 there, pressure.py:671, :702) */
+    for (size_t i = 0; i < n3; ++i) z0[i] = (fm[i] & 63) ? omega * r[i] / F(kdiag)(fm[i]) : (R)0;
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < N; ++z)
         for (int y = 0; y < N; ++y)
@@ -821,7 +823,7 @@ static void F(precond)(int N, const int* fm, F(levels_t) * L, const R* r, R* z0,
             for (int x = 0; x < N; ++x) {
                 size_t i = IDX(z, y, x);
                 R rv = r[i], zv = 0;
-                if (fm[i]) zv = zt[i] + omega * (rv - F(kmul)(zt, fm, N, z, y, x)) / F(kdiag)(fm[i]);
+                if (fm[i] & 63) zv = zt[i] + omega * (rv - F(kmul)(zt, fm, N, z, y, x)) / F(kdiag)(fm[i]);
                 zout[i] = zv;
                 s += (double)rv * (double)zv;
                 double a = fabs((double)rv);

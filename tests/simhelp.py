@@ -38,6 +38,15 @@ def sphere_solid(res, cx=0.5, cy=0.45, cz=0.5, r=0.14) -> np.ndarray:
     return ((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2 <= r * r).astype(np.float32)
 
 
+def sealed_cell_solid(res, c, r=0.2) -> np.ndarray:
+    """A centred sphere obstacle plus the six neighbours of cell c, which seal c in (a fluid cell
+    with no open face: diag(K) = 0)."""
+    solid = sphere_solid(res, cy=0.5, r=r).copy()
+    for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+        solid[c[0] + d[0], c[1] + d[1], c[2] + d[2]] = 1.0
+    return solid
+
+
 def random_state(res: int, seed: int, solid: np.ndarray | None = None, vel_amp: float = 1.0) -> np.ndarray:
     """(6,N,N,N) float32: smooth density / velocity / temperature / fuel, zeroed in solids."""
     d = np.abs(smooth_noise(res, 1, seed, amp=0.5))[0]

@@ -17,7 +17,7 @@ import pytest
 import torch
 
 import oracle
-from simhelp import rel_err, random_state, smooth_noise, soa_from_aos, sphere_solid, to_dev
+from simhelp import rel_err, random_state, sealed_cell_solid, smooth_noise, soa_from_aos, sphere_solid, to_dev
 from vfx.core.fields3d import Emitter3D, Forces3D, Impulse3D
 from vfx.sim3d import Solver3D, SolverConfig3D, runtime_config
 from vfx.sim3d import _lib
@@ -167,6 +167,41 @@ def test_precond_parity(cuda, hiplib, N, coarse_m):
     e = rel_err(z.cpu().numpy(), zref)
     _say(f"  max rel err {e:.3e}")
     assert e <= 1e-6
+
+
+def test_sealed_fluid_cell_parity(cuda, hiplib):
+    """A fluid cell sealed in by solids (diag(K) = 0): the preconditioner equals the oracle's, with
+    0 in that cell, and the projection converges in the oracle's iterations."""
+    N, c = 32, (9, 20, 11)
+    _say(f"sealed fluid cell N={N} at {c}")
+    solid = sealed_cell_solid(N, c)
+    fm = oracle.mask(solid)
+    assert fm[c] == 64
+    r = smooth_noise(N, 1, 52)[0] * (fm > 0)
+    s = Solver3D(N)
+    z = torch.empty(N, N, N, device=DEV)
+    sd, rd = to_dev(solid), to_dev(r)
+    _lib.check(hiplib.vfx_precond(s._ctx, _ptr(sd), _ptr(rd), _ptr(z), _stream()))
+    torch.cuda.synchronize()
+    zg = z.cpu().numpy()
+    zref, _ = oracle.precond_apply(fm, r)
+    e = rel_err(zg, zref)
+    _say(f"  precond max rel err {e:.3e}, z at the cell {zg[c]}")
+    assert np.isfinite(zg).all() and zg[c] == 0.0 and e <= 1e-6
+    data = _divergent_state(N, solid, seed=5)
+    cfg = SolverConfig3D(pressure_rel_tol=1e-5, pressure_max_iters=200, pressure_warm_start=False)
+    s2 = Solver3D(N, cfg)
+    s2.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
+    st = s2.reset()
+    st.data.copy_(to_dev(data))
+    st.solid.copy_(to_dev(solid))
+    s2.project(st)
+    stats = s2.pressure_stats()
+    ref = data.copy()
+    it_ref, _ = oracle.project(ref, solid, pack_params(cfg, N, Forces3D(), 0.0), np.zeros((N,) * 3, np.float32))
+    _say(f"  iters gpu {stats['iters']} oracle {it_ref}; true rel-res {stats['true_rel_res']:.2e}")
+    assert it_ref > 0 and abs(stats["iters"] - it_ref) <= 1 and stats["true_rel_res"] <= 1e-5
+    assert rel_err(st.data.cpu().numpy()[1:4], ref[1:4]) <= 1e-4
 
 
 @pytest.mark.parametrize("N", [64, 128])

@@ -19,7 +19,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import oracle
-from simhelp import smooth_noise, sphere_solid
+from simhelp import sealed_cell_solid, smooth_noise, sphere_solid
 from vfx.core.fields3d import Forces3D
 from vfx.sim3d import SolverConfig3D
 from vfx.sim3d.solver import pack_params
@@ -196,11 +196,13 @@ def _exact_projection(solid, u):
     b = -oracle.div2(fm, u, prec="f64").ravel()
     idx = np.flatnonzero(fl)
     Kf = K[idx][:, idx].tolil()
-    # pin one cell per connected component (here: one) to fix the constant
-    Kf[0, :] = 0
-    Kf[0, 0] = 1.0
     bf = b[idx].copy()
-    bf[0] = 0.0
+    # pin one cell of the main component to fix the constant, and every sealed cell (a zero row
+    # and column of K: a component of its own)
+    for j in [0] + list(np.flatnonzero(Kf.diagonal() == 0)):
+        Kf[j, :] = 0
+        Kf[j, j] = 1.0
+        bf[j] = 0.0
     phi = np.zeros(N ** 3)
     phi[idx] = spla.spsolve(Kf.tocsr(), bf)
     return oracle.apply(fm, phi.reshape(N, N, N), u, prec="f64"), phi
@@ -220,6 +222,43 @@ def test_pcg_equals_exact_solve(N, r):
     np.testing.assert_allclose(data[1:4], ref, atol=1e-9 * np.abs(u).max())
     fm = oracle.mask(solid)
     assert np.abs(oracle.div2(fm, data[1:4], prec="f64")).max() < 1e-9
+
+
+def test_sealed_fluid_cell():
+    """A fluid cell sealed in by solids on all six sides has diag(K) = 0 and a zero row and column
+    of K.  Its Jacobi terms are 0, so the preconditioner stays finite, symmetric and positive on
+    the rest (the reference divides 0 / 0 there, pressure.py:671 and :702, and its PCG stops at
+    iteration 0 on the NaN), and the projection equals the exact solve with the sealed cell left
+    alone."""
+    N, c = 12, (3, 4, 3)
+    solid = sealed_cell_solid(N, c)
+    fm = oracle.mask(solid)
+    assert fm[c] == 64 and oracle.kmul(fm, np.ones((N,) * 3), prec="f64")[c] == 0.0
+    fl = (fm != 0).astype(np.float64)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((N,) * 3) * fl
+    y = rng.standard_normal((N,) * 3) * fl
+    Mx, _ = oracle.precond_apply(fm, x, prec="f64")
+    My, _ = oracle.precond_apply(fm, y, prec="f64")
+    assert np.isfinite(Mx).all() and Mx[c] == 0.0
+    assert (x * Mx).sum() > 0 and (y * My).sum() > 0
+    assert abs((x * My).sum() - (Mx * y).sum()) <= 0.05 * abs((x * My).sum())
+    u = (smooth_noise(N, 3, 4, amp=2.0) + np.array([0.5, 0.2, -0.4], np.float32)[:, None, None, None])
+    u = u.astype(np.float64) * (solid < 0.5)
+    ref, _ = _exact_projection(solid, u)
+    data = np.zeros((6, N, N, N))
+    data[1:4] = u
+    cfg = SolverConfig3D(pressure_rel_tol=1e-12, pressure_max_iters=500, pressure_warm_start=False)
+    it, st = oracle.project(data, solid, pack_params(cfg, N, Forces3D(), 0.0), np.zeros((N,) * 3), prec="f64")
+    assert 0 < it < 100 and st["r_norm"] <= 1e-12 * st["b_norm"]
+    np.testing.assert_allclose(data[1:4], ref, atol=1e-9 * np.abs(u).max())
+    assert np.array_equal(data[1:4, c[0], c[1], c[2]], u[:, c[0], c[1], c[2]])   # g = 0 there
+    # the FP32 solver converges too
+    d32 = np.zeros((6, N, N, N), np.float32)
+    d32[1:4] = u
+    cfg32 = SolverConfig3D(pressure_rel_tol=1e-5, pressure_max_iters=200, pressure_warm_start=False)
+    it32, st32 = oracle.project(d32, solid, pack_params(cfg32, N, Forces3D(), 0.0), np.zeros((N,) * 3, np.float32))
+    assert 0 < it32 < 40 and st32["r_norm"] <= 1e-5 * st32["b_norm"] and np.isfinite(d32).all()
 
 
 def test_pure_gradient_flow_is_removed():
