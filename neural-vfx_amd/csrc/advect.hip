@@ -76,13 +76,20 @@ __device__ __forceinline__ void sample_vel(const real* __restrict__ vel, int N, 
     v[2] = lerp3(gz, c);
 }
 
-// trace (kernels.py:115-120): RK2 midpoint over signed s (cells per m/s).  Two traces (back and
-// forward) run side by side so their gathers overlap.
-__device__ __forceinline__ void trace2(const real* __restrict__ vel, int N, int n3, real px, real py, real pz,
+// velocity at the centre of cell i: the trilinear sample there has weights exactly 0 and 1, so
+// it is the cell's own value (up to the sign of a zero, which no traced coordinate keeps)
+__device__ __forceinline__ void cell_vel(const real* __restrict__ vel, int n3, int i, real* v) {
+    v[0] = vel[i];
+    v[1] = vel[n3 + i];
+    v[2] = vel[2 * n3 + i];
+}
+
+// trace (kernels.py:115-120): RK2 midpoint over signed s (cells per m/s) from the centre of
+// cell i = (px, py, pz).  Two traces (back and forward) run side by side so their gathers overlap.
+__device__ __forceinline__ void trace2(const real* __restrict__ vel, int N, int n3, int i, real px, real py, real pz,
                                        real sa, real sb, real* qa, real* qb) {
-    const Corner c0 = corner(N, px, py, pz);
     real v1[3];
-    sample_vel(vel, N, n3, c0, v1);
+    cell_vel(vel, n3, i, v1);
     const real ha = RL(0.5) * sa, hb = RL(0.5) * sb;
     const Corner ca = corner(N, px + ha * v1[0], py + ha * v1[1], pz + ha * v1[2]);
     const Corner cb = corner(N, px + hb * v1[0], py + hb * v1[1], pz + hb * v1[2]);
@@ -93,11 +100,10 @@ __device__ __forceinline__ void trace2(const real* __restrict__ vel, int N, int 
     qb[0] = px + sb * vb[0]; qb[1] = py + sb * vb[1]; qb[2] = pz + sb * vb[2];
 }
 
-__device__ __forceinline__ void trace1(const real* __restrict__ vel, int N, int n3, real px, real py, real pz,
+__device__ __forceinline__ void trace1(const real* __restrict__ vel, int N, int n3, int i, real px, real py, real pz,
                                        real s, real* q) {
-    const Corner c0 = corner(N, px, py, pz);
     real v1[3];
-    sample_vel(vel, N, n3, c0, v1);
+    cell_vel(vel, n3, i, v1);
     const real h = RL(0.5) * s;
     const Corner c1 = corner(N, px + h * v1[0], py + h * v1[1], pz + h * v1[2]);
     real v2[3];
@@ -117,13 +123,13 @@ __global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ 
     const int z = blockIdx.z;
     if (x >= N || y >= N) return;
     const real s = (real)prm[P_DT] * inv_dx;
+    const int i = (z * N + y) * N + x;
     real q[3];
-    trace1(vel, N, n3, (real)x, (real)y, (real)z, -s, q);
+    trace1(vel, N, n3, i, (real)x, (real)y, (real)z, -s, q);
     const Corner c = corner(N, q[0], q[1], q[2]);
     Oct g[C];
 #pragma unroll
     for (int ch = 0; ch < C; ++ch) g[ch] = gather(src + ch * n3, N, c.o);
-    const int i = (z * N + y) * N + x;
 #pragma unroll
     for (int ch = 0; ch < C; ++ch) hat[ch * n3 + i] = lerp3(g[ch], c);
 }
@@ -144,11 +150,11 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
     double sa = 0.0, sb = 0.0;
     if (live) {
         const real s = (real)prm[P_DT] * inv_dx;
+        const int i = (z * N + y) * N + x;
         real q[3], r[3];
-        trace2(vel, N, n3, (real)x, (real)y, (real)z, -s, s, q, r);
+        trace2(vel, N, n3, i, (real)x, (real)y, (real)z, -s, s, q, r);
         const int qo = (base_idx(q[2], N) * N + base_idx(q[1], N)) * N + base_idx(q[0], N);
         const Corner cr = corner(N, r[0], r[1], r[2]);
-        const int i = (z * N + y) * N + x;
         Oct gs[C], gh[C];
         real h[C], s0[C];
 #pragma unroll

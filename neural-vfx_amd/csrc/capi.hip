@@ -102,8 +102,7 @@ int vfx::stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s) 
 
 int vfx::stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
     launch_sources_diffuse(c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, data, s);
-    launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega, c->omega_s, c->force, c->tmp_a, c->tmp_b,
-                       s);
+    launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega, c->omega_s, c->force, s);
     CHECK_LAUNCH("forces");
     return VFX_OK;
 }
@@ -411,7 +410,7 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     const size_t n3 = c->g.n3;
     // one pool: floats first (every array a multiple of 8 floats, so 32-byte aligned), then
     // doubles, then the mask
-    const size_t nf = n3 * (6 + 6 + 3 + 3 + 3 + 4 + 4) + 2 * (size_t)P_SIZE + n3 * 7 + P.level_floats();
+    const size_t nf = n3 * (6 + 6 + 3 + 3 + 3) + 2 * (size_t)P_SIZE + n3 * 7 + P.level_floats();
     const size_t nd = mass_part_doubles(c->g) + (size_t)P.nparts * (3 + 1) + sizeof(DevState) / sizeof(double) + 8;
     const size_t bytes = nf * sizeof(real) + 64 + nd * 8 + 64 + n3 + 64;
     if (!hip_ok(hipMalloc(&c->pool, bytes), "device allocation")) { delete c; return VFX_E_HIP; }
@@ -422,8 +421,6 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     c->omega = f; f += 3 * n3;
     c->omega_s = f; f += 3 * n3;
     c->force = f; f += 3 * n3;
-    c->tmp_a = f; f += 4 * n3;
-    c->tmp_b = f; f += 4 * n3;
     c->prm = reinterpret_cast<float*>(f); f += P_SIZE;
     c->prm_adv = reinterpret_cast<float*>(f); f += P_SIZE;
     P.R[0] = f; f += n3;

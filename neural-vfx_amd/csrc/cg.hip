@@ -57,7 +57,9 @@ __device__ __forceinline__ Tile this_tile(int N) {
 struct HBlock {
     int h0;     // halo index of the block's (0, 0, 0) cell
     int i0;     // fine index of the (clamped) block's (0, 0, 0) cell
-    int lc;     // level-1 cell of the clamped block
+    int lc;     // level-1 cell of the clamped block (natural order: operator coefficients)
+    int lx;     // its storage index in the level-1 x and r arrays (lsto)
+    int o0;     // index of its (0, 0, 0) cell in the tile's own region (TX x TY x TZ), -1 in the halo
     bool in;    // block inside the box
 };
 
@@ -71,6 +73,9 @@ __device__ __forceinline__ HBlock halo_block(int N, const Tile& tl, int b) {
     r.h0 = ((2 * bk) * HY + 2 * bj) * HX + 2 * bi;
     r.i0 = ((2 * CK) * N + 2 * CJ) * N + 2 * CI;
     r.lc = (CK * mb + CJ) * mb + CI;
+    r.lx = (int)lsto(mb, (size_t)r.lc, (CI + CJ + CK) & 1);
+    const bool own = bi >= 1 && bi <= BTX && bj >= 1 && bj <= BTY && bk >= 1 && bk <= BTZ;
+    r.o0 = own ? ((2 * (bk - 1)) * TY + 2 * (bj - 1)) * TX + 2 * (bi - 1) : -1;
     return r;
 }
 
@@ -82,6 +87,8 @@ __device__ __forceinline__ int own_hb(int t) {
 // row rw = dy + 2 dz of a block: offset of its dx = 0 cell in the fine grid and in the halo
 __device__ __forceinline__ int row_off(int N, int rw) { return (rw & 1) * N + (rw >> 1) * N * N; }
 __device__ __forceinline__ int row_h(int rw) { return (rw & 1) * HX + (rw >> 1) * HX * HY; }
+__device__ __forceinline__ int row_o(int rw) { return (rw & 1) * TX + (rw >> 1) * TX * TY; }
+constexpr int ON = TX * TY * TZ;   // cells of a tile's own region
 
 __device__ __forceinline__ real2 ld2(const real* __restrict__ f, int i) {
     return *reinterpret_cast<const real2*>(f + i);
@@ -284,7 +291,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
                                                                 const uint8_t* __restrict__ fm, LvlT<LT> L, int do_red,
                                                                 const DevState* __restrict__ st, int k) {
     __shared__ __align__(16) real sz[HN];
-    __shared__ __align__(16) real sr[HN];
+    __shared__ __align__(16) real sr[ON];   // r of the own region only (the halo needs z0 and the mask)
     __shared__ __align__(16) uint8_t sm[HN];
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
@@ -334,7 +341,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
                     zv[dx] = kopen(m) ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0);
                 }
                 st2(sz, h, zv[0], zv[1]);
-                st2(sr, h, rv[0], rv[1]);
+                if (hb[it].o0 >= 0) st2(sr, hb[it].o0 + row_o(rw), rv[0], rv[1]);
                 *reinterpret_cast<uint16_t*>(sm + h) = (uint16_t)hm[it][rw];
             }
         }
@@ -352,8 +359,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
     real rr[8];
 #pragma unroll
     for (int rw = 0; rw < 4; ++rw) {
-        const int h = ob.h0 + row_h(rw);
-        const real r0 = sr[h], r1 = sr[h + 1];
+        const int h = ob.h0 + row_h(rw), ho = ob.o0 + row_o(rw);
+        const real r0 = sr[ho], r1 = sr[ho + 1];
         if (UPDR) st2(r_out, ob.i0 + row_off(N, rw), r0, r1);
         rr[2 * rw] = sm[h] ? r0 - kz[2 * rw] : RL(0.0);
         rr[2 * rw + 1] = sm[h + 1] ? r1 - kz[2 * rw + 1] : RL(0.0);
@@ -369,8 +376,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
         const real rs = as_stored<LT>(out[c]);
         xo[c] = dg[c] <= L.dmin ? RL(0.0) : rs / dg[c];
     }
-    st8(L.r + cell * 8, out);
-    if (red) st8(L.x + cell * 8, xo);
+    st8(L.r + (size_t)ob.lx * 8, out);
+    if (red) st8(L.x + (size_t)ob.lx * 8, xo);
 }
 
 // k_inv_transform (:352-365) + k_post_rz (:687-710): zt = z0 + T^T x1 (0 in solids) staged for
@@ -383,7 +390,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
                                                            real* __restrict__ p_init, double* __restrict__ part,
                                                            int nparts, const DevState* __restrict__ st, int k) {
     __shared__ __align__(16) real szt[HN];
-    __shared__ __align__(16) real sr[HN];
+    __shared__ __align__(16) real sr[ON];   // r of the own region only
     __shared__ __align__(16) uint8_t sm[HN];
     __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
@@ -397,7 +404,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
         hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
-        ld8(L.x + (size_t)hb[it].lc * 8, xv[it]);
+        ld8(L.x + (size_t)hb[it].lx * 8, xv[it]);
 #pragma unroll
         for (int rw = 0; rw < 4; ++rw) {
             const int o = hb[it].i0 + row_off(N, rw);
@@ -425,7 +432,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
                 }
                 const int h = hb[it].h0 + row_h(rw);
                 st2(szt, h, zt[0], zt[1]);
-                st2(sr, h, rv[0], rv[1]);
+                if (hb[it].o0 >= 0) st2(sr, hb[it].o0 + row_o(rw), rv[0], rv[1]);
                 *reinterpret_cast<uint16_t*>(sm + h) = (uint16_t)hm[it][rw];
             }
         }
@@ -445,12 +452,12 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
     double s = 0.0, mx = 0.0, rr = 0.0;
 #pragma unroll
     for (int rw = 0; rw < 4; ++rw) {
-        const int h = ob.h0 + row_h(rw);
+        const int h = ob.h0 + row_h(rw), ho = ob.o0 + row_o(rw);
         real zv[2];
 #pragma unroll
         for (int dx = 0; dx < 2; ++dx) {
             const int m = sm[h + dx];
-            const real rv = sr[h + dx];
+            const real rv = sr[ho + dx];
             zv[dx] = kopen(m) ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : RL(0.0);
             s += (double)rv * (double)zv[dx];
             mx = fmax(mx, fabs((double)rv));

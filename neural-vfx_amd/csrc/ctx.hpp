@@ -33,7 +33,6 @@ struct vfx_ctx {
     // device buffers
     void* pool = nullptr;
     real *hat = nullptr, *adv = nullptr, *omega = nullptr, *omega_s = nullptr, *force = nullptr;
-    real *tmp_a = nullptr, *tmp_b = nullptr;
     float* prm = nullptr;      // parameters of the step
     float* prm_adv = nullptr;  // parameters of the standalone advect()
     double* mass_part = nullptr;

@@ -1,8 +1,8 @@
 // Sources, heat/fuel diffusion, vorticity confinement, buoyancy, wind and the smoothed force.
 //
 // Restates vfx/sim3d/kernels.py:181-457 (sources, diffuse_heat_fuel, curl, body_forces, the
-// separable fluid-renormalised [1,2,1]^3 smoothing sep_x / sep_y / sep_z_store / sep_z_apply)
-// in the launch order of solver.py:432-455.  "Solid" is read from the uint8 face mask
+// separable fluid-renormalised [1,2,1]^3 smoothing sep_x / sep_y / sep_z_store / sep_z_apply,
+// here one kernel for the three passes) in the launch order of solver.py:432-455.  "Solid" is read from the uint8 face mask
 // (mask == 0 <=> solid > 0.5, pressure.py:115-143); cells outside the box count as solid.
 #include "sim3d_kernels.hpp"
 
@@ -210,105 +210,144 @@ __global__ void __launch_bounds__(256) k_curl(int N, int zc, const real* __restr
     }
 }
 
-// sep_x (kernels.py:392-406): out[0:3] = [1,2,1]_x (w a), out[3] = [1,2,1]_x w.
-__global__ void __launch_bounds__(256) k_sep_x(int N, int zc, const real* __restrict__ a,
-                                               const uint8_t* __restrict__ fm, real* __restrict__ out) {
+// The fluid-renormalised [1,2,1]^3 smoothing B(w a) / B(w) of three channels in ONE pass: sep_x,
+// sep_y and sep_z_store / sep_z_apply (kernels.py:392-457) composed.  A 256-thread workgroup owns
+// a 64 x 8 column of cells (two rows per thread) and marches up z through SM_ZT planes.  Each
+// input plane is staged in LDS with a one-cell x / y halo, the x pass runs on the 10 staged rows,
+// the y pass per thread, and the y-pass values of three consecutive planes stay in registers
+// for the z pass.  Every pass keeps the reference's operation order, so the result is the three
+// separate passes' result, while a and the mask are read once and no intermediate is written.
+// MODE 0: out (3 channels) = the smoothed field; MODE 1: fluid cells of the state get
+// velocity += dt * the smoothed field.  Tiles are numbered through xcd_block, so the tiles that
+// share halo rows run on one XCD.
+constexpr int SM_TX = 64, SM_TY = 8, SM_ZT = 16, SM_RY = SM_TY / 4;   // blockDim (64, 4)
+constexpr int SM_HX = SM_TX + 2, SM_HY = SM_TY + 2;
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__ a, const uint8_t* __restrict__ fm,
+                                                 const float* __restrict__ prm, real* __restrict__ out) {
+    __shared__ real sa[3][SM_HY][SM_HX];
+    __shared__ real sw[SM_HY][SM_HX];
+    __shared__ real sx[4][SM_HY][SM_TX];
     const int n3 = N * N * N;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= N || y >= N) return;
-    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
-    for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const int i = (z * N + y) * N + x;
-        const int idx[3] = {x > 0 ? i - 1 : i, i, x + 1 < N ? i + 1 : i};
-        const bool inb[3] = {x > 0, true, x + 1 < N};
-        real s0 = RL(0.0), s1 = RL(0.0), s2 = RL(0.0), ws = RL(0.0);
+    const int gx = (N + SM_TX - 1) / SM_TX, gy = (N + SM_TY - 1) / SM_TY;
+    const int nb = (int)gridDim.x;
+    const int t = xcd_block((int)blockIdx.x, nb);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SM_TX + tx;
+    const int x0 = bx * SM_TX, y0 = by * SM_TY, z0 = bz * SM_ZT;
+    const int z1 = min(z0 + SM_ZT, N);
+    const int x = x0 + tx;
+    const real dt = MODE == 1 ? (real)prm[P_DT] : RL(0.0);
+    real ym[SM_RY][4], yc[SM_RY][4], yp[SM_RY][4];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int j = idx[k];
-            const int mk = fm[j];
-            const real a0 = a[j], a1 = a[n3 + j], a2 = a[2 * n3 + j];
-            const bool fl = inb[k] && mk != 0;
-            const real q = k == 1 ? RL(2.0) : RL(1.0);
-            s0 = s0 + (fl ? q * a0 : RL(0.0));
-            s1 = s1 + (fl ? q * a1 : RL(0.0));
-            s2 = s2 + (fl ? q * a2 : RL(0.0));
-            ws = ws + (fl ? q : RL(0.0));
+    for (int j = 0; j < SM_RY; ++j)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ym[j][c] = yc[j][c] = yp[j][c] = RL(0.0);
+    // the staged plane: SM_ST slots per thread, loaded one plane ahead into registers so the loads
+    // of plane zi + 1 are in flight while plane zi is computed
+    constexpr int SM_ST = (SM_HY * SM_HX + 255) / 256;
+    real pv[SM_ST][3];
+    real pw[SM_ST];
+    auto fetch = [&](int zi) {
+        const bool zin = zi >= 0 && zi < N;
+#pragma unroll
+        for (int j = 0; j < SM_ST; ++j) {
+            const int k = tid + 256 * j;
+            const int r = k / SM_HX, c = k - r * SM_HX;
+            const int py = y0 - 1 + r, px = x0 - 1 + c;
+            const bool in = k < SM_HY * SM_HX && zin && px >= 0 && px < N && py >= 0 && py < N;
+            const int gi = in ? (zi * N + py) * N + px : 0;   // clamped address: always loaded
+            pv[j][0] = a[gi];
+            pv[j][1] = a[n3 + gi];
+            pv[j][2] = a[2 * n3 + gi];
+            const int m = fm[gi];
+            pw[j] = (in && m != 0) ? RL(1.0) : RL(0.0);
         }
-        out[i] = s0; out[n3 + i] = s1; out[2 * n3 + i] = s2; out[3 * n3 + i] = ws;
+    };
+    fetch(z0 - 1);
+    for (int zi = z0 - 1; zi <= z1; ++zi) {
+        // stage plane zi: 3 channels and the fluid weight (0 outside the box)
+#pragma unroll
+        for (int j = 0; j < SM_ST; ++j) {
+            const int k = tid + 256 * j;
+            if (k < SM_HY * SM_HX) {
+                const int r = k / SM_HX, c = k - r * SM_HX;
+                sa[0][r][c] = pv[j][0];
+                sa[1][r][c] = pv[j][1];
+                sa[2][r][c] = pv[j][2];
+                sw[r][c] = pw[j];
+            }
+        }
+        if (zi < z1) fetch(zi + 1);
+        __syncthreads();
+        // x pass (sep_x): s = sum_k q_k w a over x-1, x, x+1 in that order
+        for (int k = tid; k < SM_HY * SM_TX; k += 256) {
+            const int r = k / SM_TX, c = k - r * SM_TX;
+            real s0 = RL(0.0), s1 = RL(0.0), s2 = RL(0.0), ws = RL(0.0);
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                const bool fl = sw[r][c + kk] != RL(0.0);
+                const real q = kk == 1 ? RL(2.0) : RL(1.0);
+                s0 = s0 + (fl ? q * sa[0][r][c + kk] : RL(0.0));
+                s1 = s1 + (fl ? q * sa[1][r][c + kk] : RL(0.0));
+                s2 = s2 + (fl ? q * sa[2][r][c + kk] : RL(0.0));
+                ws = ws + (fl ? q : RL(0.0));
+            }
+            sx[0][r][c] = s0;
+            sx[1][r][c] = s1;
+            sx[2][r][c] = s2;
+            sx[3][r][c] = ws;
+        }
+        __syncthreads();
+        // y pass (sep_y) of this thread's rows at plane zi, into the register ring
+#pragma unroll
+        for (int j = 0; j < SM_RY; ++j) {
+            const int ly = ty + 4 * j, y = y0 + ly;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                real v = RL(2.0) * sx[c][ly + 1][tx];
+                v = v + (y > 0 ? sx[c][ly][tx] : RL(0.0));
+                v = v + (y + 1 < N ? sx[c][ly + 2][tx] : RL(0.0));
+                ym[j][c] = yc[j][c];
+                yc[j][c] = yp[j][c];
+                yp[j][c] = v;
+            }
+        }
+        // z pass (sep_z_ratio) of plane zo = zi - 1 from planes zo - 1, zo, zo + 1
+        const int zo = zi - 1;
+        if (zo >= z0 && x < N) {
+#pragma unroll
+            for (int j = 0; j < SM_RY; ++j) {
+                const int y = y0 + ty + 4 * j;
+                if (y >= N) continue;
+                const bool inb0 = zo > 0, inb2 = zo + 1 < N;
+                real sv[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    real u = RL(0.0);
+                    u = u + (inb0 ? ym[j][c] : RL(0.0));
+                    u = u + RL(2.0) * yc[j][c];
+                    u = u + (inb2 ? yp[j][c] : RL(0.0));
+                    sv[c] = u;
+                }
+                real r0 = RL(0.0), r1 = RL(0.0), r2 = RL(0.0);
+                if (sv[3] > RL(0.0)) { r0 = sv[0] / sv[3]; r1 = sv[1] / sv[3]; r2 = sv[2] / sv[3]; }
+                const int i = (zo * N + y) * N + x;
+                if (MODE == 0) {
+                    out[i] = r0; out[n3 + i] = r1; out[2 * n3 + i] = r2;
+                } else if (fm[i] != 0) {
+                    out[n3 + i] = out[n3 + i] + dt * r0;
+                    out[2 * n3 + i] = out[2 * n3 + i] + dt * r1;
+                    out[3 * n3 + i] = out[3 * n3 + i] + dt * r2;
+                }
+            }
+        }
     }
 }
 
-// sep_y (kernels.py:409-419)
-__global__ void __launch_bounds__(256) k_sep_y(int N, int zc, const real* __restrict__ a, real* __restrict__ out) {
-    const int n3 = N * N * N;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= N || y >= N) return;
-    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
-    for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const int i = (z * N + y) * N + x;
-        const int im = y > 0 ? i - N : i, ip = y + 1 < N ? i + N : i;
-        real c0[4], cm[4], cp[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            c0[c] = a[c * n3 + i];
-            cm[c] = a[c * n3 + im];
-            cp[c] = a[c * n3 + ip];
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            real v = RL(2.0) * c0[c];
-            v = v + (y > 0 ? cm[c] : RL(0.0));
-            v = v + (y + 1 < N ? cp[c] : RL(0.0));
-            out[c * n3 + i] = v;
-        }
-    }
-}
-
-// sep_z_ratio + sep_z_store / sep_z_apply (kernels.py:422-457).  mode 0: out (3ch) = ratio;
-// mode 1: fluid cells of the state get velocity += dt * ratio.
-__global__ void __launch_bounds__(256) k_sep_z(int N, int zc, const real* __restrict__ a,
-                                               const uint8_t* __restrict__ fm, const float* __restrict__ prm,
-                                               int mode, real* __restrict__ out) {
-    const int n3 = N * N * N;
-    const int sz = N * N;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= N || y >= N) return;
-    const real dt = (real)prm[P_DT];
-    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
-    for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const int i = (z * N + y) * N + x;
-        const int idx[3] = {z > 0 ? i - sz : i, i, z + 1 < N ? i + sz : i};
-        const bool inb[3] = {z > 0, true, z + 1 < N};
-        real v[3][4];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) v[k][c] = a[c * n3 + idx[k]];
-        const int m0 = mode == 1 ? (int)fm[i] : 1;
-        real o1 = RL(0.0), o2 = RL(0.0), o3 = RL(0.0);
-        if (mode == 1) { o1 = out[n3 + i]; o2 = out[2 * n3 + i]; o3 = out[3 * n3 + i]; }
-        real s0 = RL(0.0), s1 = RL(0.0), s2 = RL(0.0), ws = RL(0.0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const real q = k == 1 ? RL(2.0) : RL(1.0);
-            s0 = s0 + (inb[k] ? q * v[k][0] : RL(0.0));
-            s1 = s1 + (inb[k] ? q * v[k][1] : RL(0.0));
-            s2 = s2 + (inb[k] ? q * v[k][2] : RL(0.0));
-            ws = ws + (inb[k] ? q * v[k][3] : RL(0.0));
-        }
-        real r0 = RL(0.0), r1 = RL(0.0), r2 = RL(0.0);
-        if (ws > RL(0.0)) { r0 = s0 / ws; r1 = s1 / ws; r2 = s2 / ws; }
-        if (mode == 0) {
-            out[i] = r0; out[n3 + i] = r1; out[2 * n3 + i] = r2;
-        } else if (m0 != 0) {
-            out[n3 + i] = o1 + dt * r0;
-            out[2 * n3 + i] = o2 + dt * r1;
-            out[3 * n3 + i] = o3 + dt * r2;
-        }
-    }
+static int smooth3_blocks(int N) {
+    return ((N + SM_TX - 1) / SM_TX) * ((N + SM_TY - 1) / SM_TY) * ((N + SM_ZT - 1) / SM_ZT);
 }
 
 __device__ __forceinline__ real wmag(const real* __restrict__ o, size_t n3, size_t i) {
@@ -390,15 +429,12 @@ void launch_sources_pass(int pass, const Geom& g, const real* src, const uint8_t
 }
 
 void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx,
-                        real* omega, real* omega_s, real* force, real* tmp_a, real* tmp_b, hipStream_t s) {
+                        real* omega, real* omega_s, real* force, hipStream_t s) {
+    const dim3 sb(SM_TX, 4), sg(smooth3_blocks(g.N));
     hipLaunchKernelGGL(k_curl, g.grid, g.block, 0, s, g.N, g.zc, data, fm, inv2dx, omega);
-    hipLaunchKernelGGL(k_sep_x, g.grid, g.block, 0, s, g.N, g.zc, omega, fm, tmp_a);
-    hipLaunchKernelGGL(k_sep_y, g.grid, g.block, 0, s, g.N, g.zc, tmp_a, tmp_b);
-    hipLaunchKernelGGL(k_sep_z, g.grid, g.block, 0, s, g.N, g.zc, tmp_b, fm, prm, 0, omega_s);
+    hipLaunchKernelGGL(k_smooth3<0>, sg, sb, 0, s, g.N, omega, fm, prm, omega_s);
     hipLaunchKernelGGL(k_body_forces, g.grid, g.block, 0, s, g.N, g.zc, data, fm, omega_s, prm, force);
-    hipLaunchKernelGGL(k_sep_x, g.grid, g.block, 0, s, g.N, g.zc, force, fm, tmp_a);
-    hipLaunchKernelGGL(k_sep_y, g.grid, g.block, 0, s, g.N, g.zc, tmp_a, tmp_b);
-    hipLaunchKernelGGL(k_sep_z, g.grid, g.block, 0, s, g.N, g.zc, tmp_b, fm, prm, 1, data);
+    hipLaunchKernelGGL(k_smooth3<1>, sg, sb, 0, s, g.N, force, fm, prm, data);
 }
 
 }  // namespace vfx

@@ -217,7 +217,7 @@ template <bool STD, class LT>
 __device__ __forceinline__ void restrict_item(const LvlT<LT>& F, const LvlT<LT>& C, int c, int K, int J, int I,
                                               bool do_red) {
     const int m = F.m;
-    size_t co[8], nb[8][6];
+    uint32_t co[8], nb[8][6];
     bool e[8][6];
     real r[8], x0[8], xn[8][6];
     int f = 1;
@@ -239,10 +239,11 @@ __device__ __forceinline__ void restrict_item(const LvlT<LT>& F, const LvlT<LT>&
         nb[k][4] = e[k][4] ? lcell(m, z + 1, y, x) : co[k];
         nb[k][5] = e[k][5] ? lcell(m, z - 1, y, x) : co[k];
         if (STD) f &= (int)F.sf[co[k]];
-        r[k] = ldl(F.r + co[k] * 8 + c);
-        x0[k] = ldl(F.x + co[k] * 8 + c);
+        const int col = (x + y + z) & 1;
+        r[k] = ldl(F.r + lsto(m, co[k], col) * 8 + c);
+        x0[k] = ldl(F.x + lsto(m, co[k], col) * 8 + c);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) xn[k][j] = ldl(F.x + nb[k][j] * 8 + c);
+        for (int j = 0; j < 6; ++j) xn[k][j] = ldl(F.x + lsto(m, nb[k][j], e[k][j] ? col ^ 1 : col) * 8 + c);
     }
     const Tab tb = STD ? load_tab(F, c) : Tab{};
     real res[8];
@@ -263,22 +264,22 @@ __device__ __forceinline__ void restrict_item(const LvlT<LT>& F, const LvlT<LT>&
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc += res[k];
     acc = RL(0.125) * acc;
-    const size_t oc = lcell(C.m, K, J, I) * 8 + c;
-    stl(C.r + oc, acc);
+    const uint32_t oc = lcell(C.m, K, J, I) * 8 + c, xc = lsto(C.m, K, J, I) * 8 + c;
+    stl(C.r + xc, acc);
     if (do_red && ((I + J + K) & 1) == 0) {
         // the coarse red sweep reads the right-hand side as stored
         const real d = ldl(C.diag + oc), ra = as_stored<LT>(acc);
-        stl(C.x + oc, d <= C.dmin ? RL(0.0) : ra / d);
+        stl(C.x + xc, d <= C.dmin ? RL(0.0) : ra / d);
     }
 }
 
 // constant prolongation (k_prolong :319-325)
 template <class LT>
 __device__ __forceinline__ void prolong_item(const LvlT<LT>& F, const LvlT<LT>& C, int c, int K, int J, int I) {
-    const size_t o = lcell(F.m, K, J, I) * 8 + c;
-    const real x = ldl(F.x + o), xc = ldl(C.x + lcell(C.m, K >> 1, J >> 1, I >> 1) * 8 + c);
+    const uint32_t o = lcell(F.m, K, J, I) * 8 + c, ox = lsto(F.m, K, J, I) * 8 + c;
+    const real x = ldl(F.x + ox), xc = ldl(C.x + lsto(C.m, K >> 1, J >> 1, I >> 1) * 8 + c);
     if (ldl(F.diag + o) <= F.dmin) return;
-    stl(F.x + o, x + xc);
+    stl(F.x + ox, x + xc);
 }
 
 // prolongation fused into the black post-sweep: equal to prolonging every cell, then sweeping
@@ -287,19 +288,20 @@ __device__ __forceinline__ void prolong_item(const LvlT<LT>& F, const LvlT<LT>& 
 template <class LT>
 __device__ __forceinline__ void prolong_black_item(const LvlT<LT>& F, const LvlT<LT>& C, int c, int K, int J, int I) {
     const int m = F.m;
-    const size_t o = lcell(m, K, J, I);
+    const uint32_t o = lcell(m, K, J, I);
     const LNb n = level_neighbours(m, o, K, J, I);
     const int nI[6] = {I + 1, I - 1, I, I, I, I}, nJ[6] = {J, J, J + 1, J - 1, J, J},
               nK[6] = {K, K, K, K, K + 1, K - 1};
     const int f = F.sf[o];
-    const real r = ldl(F.r + o * 8 + c);
+    const int col = (I + J + K) & 1;
+    const real r = ldl(F.r + lsto(m, o, col) * 8 + c);
     real xn[6], xcn[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         // coordinates of the clamped neighbour (the cell itself where it is missing)
         const int kI = n.e[k] ? nI[k] : I, kJ = n.e[k] ? nJ[k] : J, kK = n.e[k] ? nK[k] : K;
-        xn[k] = ldl(F.x + n.c[k] * 8 + c);
-        xcn[k] = ldl(C.x + lcell(C.m, kK >> 1, kJ >> 1, kI >> 1) * 8 + c);
+        xn[k] = ldl(F.x + lsto(m, n.c[k], n.e[k] ? col ^ 1 : col) * 8 + c);
+        xcn[k] = ldl(C.x + lsto(C.m, kK >> 1, kJ >> 1, kI >> 1) * 8 + c);
     }
     const Tab tb = load_tab(F, c);
     real d, cf[6], xpv[6];
@@ -320,7 +322,7 @@ __device__ __forceinline__ void prolong_black_item(const LvlT<LT>& F, const LvlT
         }
     }
     const real a = offsum_v(cf, xpv, n.e);
-    stl(F.x + o * 8 + c, d <= F.dmin ? RL(0.0) : (r - a) / d);
+    stl(F.x + lsto(m, o, col) * 8 + c, d <= F.dmin ? RL(0.0) : (r - a) / d);
 }
 
 // item of this thread in a 1D launch, blocks renumbered per XCD (neighbouring rows in one L2)
@@ -328,18 +330,29 @@ __device__ __forceinline__ size_t xcd_item() {
     return (size_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
 }
 
-// half-size colour launch: item t -> (c, K, J, I) with (I + J + K) & 1 == color (m even)
+// half-size colour launch: item t -> (c, K, J, I) with (I + J + K) & 1 == color (m even).  Item
+// counts stay below 2^32 (8 m^3 / 2 <= 2^30 for N <= 1024), so the decomposition is 32-bit
+// unsigned arithmetic (a 64-bit division is a long emulated sequence per lane).
 __device__ __forceinline__ bool half_item(size_t t, int m, int color, int& c, int& K, int& J, int& I) {
-    const int mh = m >> 1;
-    const size_t items = (size_t)m * m * mh * 8;
-    if (t >= items) return false;
-    c = (int)(t & 7);
-    const size_t h = t >> 3;
-    const int Ih = (int)(h % mh);
-    J = (int)((h / mh) % m);
-    K = (int)(h / ((size_t)mh * m));
-    I = 2 * Ih + ((color + J + K) & 1);
+    const uint32_t um = (uint32_t)m, mh = um >> 1;
+    if (t >= (size_t)um * um * mh * 8) return false;
+    const uint32_t tt = (uint32_t)t;
+    c = (int)(tt & 7u);
+    const uint32_t h = tt >> 3, q = h / mh;
+    const uint32_t Ih = h - q * mh;
+    K = (int)(q / um);
+    J = (int)(q - (uint32_t)K * um);
+    I = (int)(2u * Ih) + ((color + J + K) & 1);
     return true;
+}
+
+// full-size launch: item t -> (c, K, J, I) (32-bit, as half_item)
+__device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K, int& J, int& I) {
+    c = (int)(t & 7u);
+    const uint32_t cell = t >> 3, q = cell / m;
+    I = (int)(cell - q * m);
+    K = (int)(q / m);
+    J = (int)(q - (uint32_t)K * m);
 }
 
 template <class LT>
@@ -357,9 +370,9 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
     const int m = C.m;
     const size_t t = xcd_item();
     if (t >= (size_t)m * m * m * 8) return;
-    const int c = (int)(t & 7);
-    const size_t cell = t >> 3;
-    restrict_item<true>(F, C, c, (int)(cell / ((size_t)m * m)), (int)((cell / m) % m), (int)(cell % m), do_red != 0);
+    int c, K, J, I;
+    full_item((uint32_t)t, (uint32_t)m, c, K, J, I);
+    restrict_item<true>(F, C, c, K, J, I, do_red != 0);
 }
 
 template <class LT>
@@ -377,36 +390,33 @@ constexpr int CB_THREADS = 1024;
 
 template <class LT>
 __device__ void cb_smooth(const LvlT<LT>& L, int color, bool zero_init) {
-    const int m = L.m;
-    const size_t items = (size_t)m * m * m * 8;
-    for (size_t t = threadIdx.x; t < items; t += CB_THREADS) {
-        const int c = (int)(t & 7);
-        const size_t cell = t >> 3;
-        const int I = (int)(cell % m), J = (int)((cell / m) % m), K = (int)(cell / ((size_t)m * m));
+    const uint32_t m = (uint32_t)L.m;
+    const uint32_t items = m * m * m * 8;
+    for (uint32_t t = threadIdx.x; t < items; t += CB_THREADS) {
+        int c, K, J, I;
+        full_item(t, m, c, K, J, I);
         if (((I + J + K) & 1) == color) smooth_item<false>(L, c, K, J, I, zero_init);
     }
     __syncthreads();
 }
 
-template <class LT>
-__global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevelsT<LT> cl, const DevState* __restrict__ st, int k) {
-    if (cg_inactive(st, k)) return;
-    const int n = cl.n;
+// the V-cycle over levels lvl(0 .. n-1); lvl(j) gives level j (in global memory or in LDS)
+template <class LT, class Get>
+__device__ __forceinline__ void coarse_vcycle(int n, const Get& lvl) {
     for (int j = 0; j + 1 < n; ++j) {
-        cb_smooth(cl.lv[j], 0, true);
-        cb_smooth(cl.lv[j], 1, false);
-        const LvlT<LT>& F = cl.lv[j];
-        const LvlT<LT>& C = cl.lv[j + 1];
-        const size_t items = (size_t)C.m * C.m * C.m * 8;
-        for (size_t t = threadIdx.x; t < items; t += CB_THREADS) {
-            const int c = (int)(t & 7);
-            const size_t cell = t >> 3;
-            restrict_item<false>(F, C, c, (int)(cell / ((size_t)C.m * C.m)), (int)((cell / C.m) % C.m),
-                                 (int)(cell % C.m), false);
+        const auto& F = lvl(j);
+        const auto& C = lvl(j + 1);
+        cb_smooth(F, 0, true);
+        cb_smooth(F, 1, false);
+        const uint32_t items = (uint32_t)C.m * C.m * C.m * 8;
+        for (uint32_t t = threadIdx.x; t < items; t += CB_THREADS) {
+            int c, K, J, I;
+            full_item(t, (uint32_t)C.m, c, K, J, I);
+            restrict_item<false>(F, C, c, K, J, I, false);
         }
         __syncthreads();
     }
-    const LvlT<LT>& B = cl.lv[n - 1];
+    const auto& B = lvl(n - 1);
     if (B.m == 1) {
         if (threadIdx.x < 8) {
             const real d = ldl(B.diag + threadIdx.x);
@@ -420,18 +430,81 @@ __global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevelsT<LT> cl, con
         }
     }
     for (int j = n - 2; j >= 0; --j) {
-        const LvlT<LT>& F = cl.lv[j];
-        const LvlT<LT>& C = cl.lv[j + 1];
-        const size_t items = (size_t)F.m * F.m * F.m * 8;
-        for (size_t t = threadIdx.x; t < items; t += CB_THREADS) {
-            const int c = (int)(t & 7);
-            const size_t cell = t >> 3;
-            prolong_item(F, C, c, (int)(cell / ((size_t)F.m * F.m)), (int)((cell / F.m) % F.m), (int)(cell % F.m));
+        const auto& F = lvl(j);
+        const auto& C = lvl(j + 1);
+        const uint32_t items = (uint32_t)F.m * F.m * F.m * 8;
+        for (uint32_t t = threadIdx.x; t < items; t += CB_THREADS) {
+            int c, K, J, I;
+            full_item(t, (uint32_t)F.m, c, K, J, I);
+            prolong_item(F, C, c, K, J, I);
         }
         __syncthreads();
         cb_smooth(F, 1, false);
         cb_smooth(F, 0, false);
     }
+}
+
+template <class LT>
+__global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevelsT<LT> cl, const DevState* __restrict__ st, int k) {
+    if (cg_inactive(st, k)) return;
+    coarse_vcycle<LT>(cl.n, [&](int j) -> const LvlT<LT>& { return cl.lv[j]; });
+}
+
+// The same V-cycle with every coarse level resident in LDS (FP32 / FP64 level storage): x, r,
+// diag and off of each level in the global layout, 48 reals per cell.  The operators and the
+// right-hand side of the top level are read once, and only the top level's x is written back
+// (the next finer level prolongs it); every sweep then waits on LDS instead of the L2.
+__host__ __device__ inline size_t coarse_lds_reals(const CoarseLevelsT<real>& cl, int upto) {
+    size_t off = 0;
+    for (int i = 0; i < upto; ++i) off += (size_t)48 * cl.lv[i].m * cl.lv[i].m * cl.lv[i].m;
+    return off;
+}
+
+__global__ void __launch_bounds__(CB_THREADS) k_coarse_lds(CoarseLevelsT<real> cl, const DevState* __restrict__ st,
+                                                           int k) {
+    extern __shared__ __align__(16) real cb_lds[];
+    if (cg_inactive(st, k)) return;
+    auto lvl = [&](int j) {
+        LvlT<real> L = cl.lv[j];
+        const size_t u = (size_t)L.m * L.m * L.m * 8;
+        L.x = cb_lds + coarse_lds_reals(cl, j);
+        L.r = L.x + u;
+        L.diag = L.r + u;
+        L.off = L.diag + u;
+        return L;
+    };
+    for (int j = 0; j < cl.n; ++j) {
+        const LvlT<real> G = cl.lv[j], L = lvl(j);
+        const uint32_t u = (uint32_t)G.m * G.m * G.m * 8;
+        for (uint32_t t = threadIdx.x; t < u; t += CB_THREADS) {
+            L.diag[t] = G.diag[t];
+            if (j == 0) L.r[t] = G.r[t];
+        }
+        for (uint32_t t = threadIdx.x; t < 3 * u; t += CB_THREADS) L.off[t] = G.off[t];
+    }
+    __syncthreads();
+    coarse_vcycle<real>(cl.n, lvl);
+    const LvlT<real> L0 = lvl(0);
+    const uint32_t u0 = (uint32_t)L0.m * L0.m * L0.m * 8;
+    for (uint32_t t = threadIdx.x; t < u0; t += CB_THREADS) cl.lv[0].x[t] = L0.x[t];
+}
+
+constexpr size_t COARSE_LDS_MAX = 64 * 1024;   // bytes of dynamic LDS the resident kernel may use
+
+// the coarse V-cycle of levels S.. on stream s: LDS-resident when the storage is `real` and fits
+template <class LT>
+static void launch_coarse(const Pressure& P, hipStream_t s, int k) {
+    CoarseLevelsT<LT> cl{};
+    cl.n = P.nlev - P.S;
+    for (int j = 0; j < cl.n; ++j) cl.lv[j] = lvl_as<LT>(P.lv[P.S + j]);
+    if constexpr (std::is_same<LT, real>::value) {
+        const size_t bytes = coarse_lds_reals(cl, cl.n) * sizeof(real);
+        if (bytes <= COARSE_LDS_MAX) {
+            hipLaunchKernelGGL(k_coarse_lds, dim3(1), dim3(CB_THREADS), bytes, s, cl, P.st, k);
+            return;
+        }
+    }
+    hipLaunchKernelGGL(k_coarse<LT>, dim3(1), dim3(CB_THREADS), 0, s, cl, P.st, k);
 }
 
 // ================================================================= host side
@@ -536,10 +609,7 @@ void Pressure::vcycle_levels(int k, hipStream_t s) {
             hipLaunchKernelGGL(k_restrict<LT>, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s,
                                lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), (l + 1 < S) ? 1 : 0, st, k);
         }
-        CoarseLevelsT<LT> cl{};
-        cl.n = nlev - S;
-        for (int j = 0; j < cl.n; ++j) cl.lv[j] = lvl_as<LT>(lv[S + j]);
-        hipLaunchKernelGGL(k_coarse<LT>, dim3(1), dim3(CB_THREADS), 0, s, cl, st, k);
+        launch_coarse<LT>(*this, s, k);
         for (int l = S - 1; l >= 0; --l) {
             const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
             hipLaunchKernelGGL(k_prolong_black<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
@@ -573,13 +643,7 @@ bool Pressure::launch_single_mg(int which, hipStream_t s) {
                 hipLaunchKernelGGL(k_prolong_black<LT>, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s,
                                    lvl_as<LT>(lv[0]), lvl_as<LT>(lv[1]), st, 0);
                 break;
-            default: {
-                CoarseLevelsT<LT> cl{};
-                cl.n = nlev - S;
-                for (int j = 0; j < cl.n; ++j) cl.lv[j] = lvl_as<LT>(lv[S + j]);
-                hipLaunchKernelGGL(k_coarse<LT>, dim3(1), dim3(CB_THREADS), 0, s, cl, st, 0);
-                break;
-            }
+            default: launch_coarse<LT>(*this, s, 0); break;
         }
     });
     return true;

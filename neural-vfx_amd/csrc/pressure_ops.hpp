@@ -121,7 +121,25 @@ __device__ __forceinline__ real hsign(int c, int dz, int dy, int dx) {
     return (e & 1) ? -HS : HS;
 }
 
-__device__ __forceinline__ size_t lcell(int m, int K, int J, int I) { return ((size_t)K * m + J) * m + I; }
+// Level cell indices are 32-bit: for N <= 1024 the largest level offset, 24 m^3 with m = 512, is
+// below 2^32 (the context rejects larger grids).
+__device__ __forceinline__ uint32_t lcell(int m, int K, int J, int I) {
+    return ((uint32_t)K * (uint32_t)m + (uint32_t)J) * (uint32_t)m + (uint32_t)I;
+}
+
+// Storage of the level unknowns x and r (8 components per cell).  For even m they are split by
+// colour: cell o = lcell(K, J, I) of colour (I + J + K) & 1 is stored at colour * m^3/2 + o / 2,
+// so one colour's cells form one contiguous half (cells I and I + 1 of a row share o / 2 and have
+// opposite colours) and a red-black sweep reads and writes only the lines it uses.  Odd m (the
+// coarsest level of some grids) keeps the natural order.  Operator coefficients (diag, off, sf)
+// stay in the natural order.
+__device__ __forceinline__ uint32_t lsto(int m, uint32_t o, int color) {
+    const uint32_t um = (uint32_t)m;
+    return (m & 1) ? o : (uint32_t)color * ((um * um * um) >> 1) + (o >> 1);
+}
+__device__ __forceinline__ uint32_t lsto(int m, int K, int J, int I) {
+    return lsto(m, lcell(m, K, J, I), (I + J + K) & 1);
+}
 
 __device__ __forceinline__ bool cg_inactive(const DevState* st, int k) { return st->stop_at < k; }
 
@@ -180,13 +198,13 @@ __device__ __forceinline__ int tile_h(int lx, int ly, int lz) { return ((lz + 2)
 // The six neighbour cells of a level cell, clamped to the cell itself where missing, with the
 // existence flags (the tests of offsum, pressure.py:275-286).
 struct LNb {
-    size_t c[6];   // +I, -I, +J, -J, +K, -K
+    uint32_t c[6];   // +I, -I, +J, -J, +K, -K
     bool e[6];
 };
 
-__device__ __forceinline__ LNb level_neighbours(int m, size_t o, int K, int J, int I) {
+__device__ __forceinline__ LNb level_neighbours(int m, uint32_t o, int K, int J, int I) {
     LNb n;
-    const size_t sy = (size_t)m, sz = (size_t)m * m;
+    const uint32_t sy = (uint32_t)m, sz = (uint32_t)m * (uint32_t)m;
     n.e[0] = I + 1 < m; n.c[0] = n.e[0] ? o + 1 : o;
     n.e[1] = I > 0;     n.c[1] = n.e[1] ? o - 1 : o;
     n.e[2] = J + 1 < m; n.c[2] = n.e[2] ? o + sy : o;
@@ -199,7 +217,7 @@ __device__ __forceinline__ LNb level_neighbours(int m, size_t o, int K, int J, i
 // coupling coefficient of link k (0..5) of component c: +ax links are stored at the cell, -ax
 // links at the -ax neighbour
 template <class LT>
-__device__ __forceinline__ real link_coef(const LvlT<LT>& L, const LNb& n, size_t o, int k, int c) {
+__device__ __forceinline__ real link_coef(const LvlT<LT>& L, const LNb& n, uint32_t o, int k, int c) {
     const int ax = k >> 1;
     return ldl(L.off + ((k & 1) ? n.c[k] : o) * 24 + ax * 8 + c);
 }
@@ -236,8 +254,9 @@ __device__ __forceinline__ real offsum_v(const real* cf, const real* xn, const b
 // cells use the table (the single-workgroup coarse levels have few standard cells and skip it).
 template <bool STD, class LT>
 __device__ __forceinline__ real smooth_value(const LvlT<LT>& L, int c, int K, int J, int I, bool zero_init) {
-    const size_t o = lcell(L.m, K, J, I);
-    const real r = ldl(L.r + o * 8 + c);
+    const uint32_t o = lcell(L.m, K, J, I);
+    const int col = (I + J + K) & 1;
+    const real r = ldl(L.r + lsto(L.m, o, col) * 8 + c);
     if (zero_init) {
         const real d = ldl(L.diag + o * 8 + c);
         return d <= L.dmin ? RL(0.0) : r / d;
@@ -246,7 +265,7 @@ __device__ __forceinline__ real smooth_value(const LvlT<LT>& L, int c, int K, in
     const int f = STD ? L.sf[o] : 0;
     real xn[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xn[k] = ldl(L.x + n.c[k] * 8 + c);
+    for (int k = 0; k < 6; ++k) xn[k] = ldl(L.x + lsto(L.m, n.c[k], n.e[k] ? col ^ 1 : col) * 8 + c);
     const Tab tb = STD ? load_tab(L, c) : Tab{};
     real d, cf[6];
     if (STD && __all(f != 0)) {
@@ -265,7 +284,7 @@ __device__ __forceinline__ real smooth_value(const LvlT<LT>& L, int c, int K, in
 template <bool STD, class LT>
 __device__ __forceinline__ real smooth_item(const LvlT<LT>& L, int c, int K, int J, int I, bool zero_init) {
     const real v = smooth_value<STD>(L, c, K, J, I, zero_init);
-    stl(L.x + lcell(L.m, K, J, I) * 8 + c, v);
+    stl(L.x + lsto(L.m, K, J, I) * 8 + c, v);
     return v;
 }
 

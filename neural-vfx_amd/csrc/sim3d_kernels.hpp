@@ -127,7 +127,7 @@ size_t mass_part_doubles(const Geom& g);   // partial-sum slots of the advection
 void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, const float* prm,
                             const DevState* st, real* hat, real* data, hipStream_t s);
 void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx,
-                        real* omega, real* omega_s, real* force, real* tmp_a, real* tmp_b, hipStream_t s);
+                        real* omega, real* omega_s, real* force, hipStream_t s);
 
 // ---------------------------------------------------------------- projection
 void launch_div_abs(const Geom& g, const real* vel, const real* solid, real* out, hipStream_t s);
@@ -139,7 +139,7 @@ struct Pressure {
     int N = 0;
     int nlev = 0;
     int S = 0;            // first level handled by the single-workgroup coarse kernel
-    int coarse_m = 8;     // levels with m <= coarse_m run inside that kernel
+    int coarse_m = 4;     // levels with m <= coarse_m run inside that kernel
     bool use_std = true;  // standard-cell flags on (off: every cell reads the level arrays)
     int mg_kind = MG_REAL;   // storage of the multigrid levels (MG_REAL / MG_FP16 / MG_BF16)
     bool force_setup = false;   // rebuild the operators at the next setup
