@@ -43,17 +43,21 @@ struct Oct {
     real v[8];
 };
 
+// The two x corners of a row are adjacent (x0 <= N - 2, so the pair never leaves the row): one
+// 2-element load at element alignment, which gfx950 issues as a single load instruction.
+typedef real real2_e __attribute__((ext_vector_type(2), aligned(sizeof(real))));
+
 __device__ __forceinline__ Oct gather(const real* __restrict__ f, int N, int o) {
     const int sy = N, sz = N * N;
+    const real2_e a = *reinterpret_cast<const real2_e*>(f + o);
+    const real2_e b = *reinterpret_cast<const real2_e*>(f + o + sy);
+    const real2_e c = *reinterpret_cast<const real2_e*>(f + o + sz);
+    const real2_e d = *reinterpret_cast<const real2_e*>(f + o + sz + sy);
     Oct g;
-    g.v[0] = f[o];
-    g.v[1] = f[o + 1];
-    g.v[2] = f[o + sy];
-    g.v[3] = f[o + sy + 1];
-    g.v[4] = f[o + sz];
-    g.v[5] = f[o + sz + 1];
-    g.v[6] = f[o + sz + sy];
-    g.v[7] = f[o + sz + sy + 1];
+    g.v[0] = a.x; g.v[1] = a.y;
+    g.v[2] = b.x; g.v[3] = b.y;
+    g.v[4] = c.x; g.v[5] = c.y;
+    g.v[6] = d.x; g.v[7] = d.y;
     return g;
 }
 

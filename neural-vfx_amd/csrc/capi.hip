@@ -410,7 +410,7 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     const size_t n3 = c->g.n3;
     // one pool: floats first (every array a multiple of 8 floats, so 32-byte aligned), then
     // doubles, then the mask
-    const size_t nf = n3 * (6 + 6 + 3 + 3 + 3) + 2 * (size_t)P_SIZE + n3 * 7 + P.level_floats();
+    const size_t nf = n3 * (6 + 6 + 3 + 4 + 3) + 2 * (size_t)P_SIZE + n3 * 7 + P.level_floats();
     const size_t nd = mass_part_doubles(c->g) + (size_t)P.nparts * (3 + 1) + sizeof(DevState) / sizeof(double) + 8;
     const size_t bytes = nf * sizeof(real) + 64 + nd * 8 + 64 + n3 + 64;
     if (!hip_ok(hipMalloc(&c->pool, bytes), "device allocation")) { delete c; return VFX_E_HIP; }
@@ -419,7 +419,7 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     c->hat = f; f += 6 * n3;
     c->adv = f; f += 6 * n3;
     c->omega = f; f += 3 * n3;
-    c->omega_s = f; f += 3 * n3;
+    c->omega_s = f; f += 4 * n3;   // smoothed vorticity and its magnitude
     c->force = f; f += 3 * n3;
     c->prm = reinterpret_cast<float*>(f); f += P_SIZE;
     c->prm_adv = reinterpret_cast<float*>(f); f += P_SIZE;

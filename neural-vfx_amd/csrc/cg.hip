@@ -171,9 +171,17 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restri
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restrict__ part, int nparts,
                                                  const float* __restrict__ prm, DevState* __restrict__ st) {
     __shared__ double sh[FIN_THREADS / 64];
-    const double mx = reduce_parts_max(part, nparts, sh);
-    const double bb = reduce_parts_sum(part + nparts, nparts, sh);
-    const double mr = reduce_parts_max(part + 2 * nparts, nparts, sh);
+    // the three reductions in one pass (each in reduce_parts' order): a third of the load rounds
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll 8
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        v0 = fmax(v0, part[i]);
+        v1 += part[nparts + i];
+        v2 = fmax(v2, part[2 * nparts + i]);
+    }
+    const double mx = block_max(v0, sh);
+    const double bb = block_sum(v1, sh);
+    const double mr = block_max(v2, sh);
     if (threadIdx.x == 0) {
         const float tol = prm[P_TOL] > 1.0e-6f ? prm[P_TOL] : 1.0e-6f;
         st->s[S_LSCALE] = pow2_scale(mr);

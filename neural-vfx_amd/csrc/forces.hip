@@ -217,8 +217,9 @@ __global__ void __launch_bounds__(256) k_curl(int N, int zc, const real* __restr
 // the y pass per thread, and the y-pass values of three consecutive planes stay in registers
 // for the z pass.  Every pass keeps the reference's operation order, so the result is the three
 // separate passes' result, while a and the mask are read once and no intermediate is written.
-// MODE 0: out (3 channels) = the smoothed field; MODE 1: fluid cells of the state get
-// velocity += dt * the smoothed field.  Tiles are numbered through xcd_block, so the tiles that
+// MODE 0: out (4 channels) = the smoothed field and its magnitude sqrt(a^2 + b^2 + c^2) (body_forces'
+// |omega_s| at every cell, evaluated once); MODE 1: fluid cells of the state get velocity += dt *
+// the smoothed field.  Tiles are numbered through xcd_block, so the tiles that
 // share halo rows run on one XCD.
 constexpr int SM_TX = 64, SM_TY = 8, SM_ZT = 16, SM_RY = SM_TY / 4;   // blockDim (64, 4)
 constexpr int SM_HX = SM_TX + 2, SM_HY = SM_TY + 2;
@@ -336,6 +337,7 @@ __global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__
                 const int i = (zo * N + y) * N + x;
                 if (MODE == 0) {
                     out[i] = r0; out[n3 + i] = r1; out[2 * n3 + i] = r2;
+                    out[3 * n3 + i] = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
                 } else if (fm[i] != 0) {
                     out[n3 + i] = out[n3 + i] + dt * r0;
                     out[2 * n3 + i] = out[2 * n3 + i] + dt * r1;
@@ -350,10 +352,9 @@ static int smooth3_blocks(int N) {
     return ((N + SM_TX - 1) / SM_TX) * ((N + SM_TY - 1) / SM_TY) * ((N + SM_ZT - 1) / SM_ZT);
 }
 
-__device__ __forceinline__ real wmag(const real* __restrict__ o, size_t n3, size_t i) {
-    const real a = o[i], b = o[n3 + i], c = o[2 * n3 + i];
-    return sqrt(a * a + b * b + c * c);
-}
+// |omega_s| at cell i: channel 3 of the smoothed vorticity (k_smooth3<0> stores
+// sqrt(a * a + b * b + c * c) of the values it writes, the expression of kernels.py:362-365)
+__device__ __forceinline__ real wmag(const real* __restrict__ o, size_t n3, size_t i) { return o[3 * n3 + i]; }
 
 // body_forces (kernels.py:346-387): buoyancy + vorticity confinement + wind marker force.
 __global__ void __launch_bounds__(256) k_body_forces(int N, int zc, const real* __restrict__ f,
