@@ -11,9 +11,9 @@
 
 namespace vfx {
 
-__device__ __forceinline__ real clampf_(real g, real lo, real hi) {
-    return g < lo ? lo : (g > hi ? hi : g);
-}
+// clamp (kernels.py:67-74): min / max instructions (equal to the branching form for every
+// non-NaN g, including the sign of a zero result used only through subtraction of an integer)
+__device__ __forceinline__ real clampf_(real g, real lo, real hi) { return fmin(fmax(g, lo), hi); }
 
 // base_idx / frac (kernels.py:67-74): clamp to [0, n-1], floor, keep one cell of room.  The
 // clamp uses fmaxf/fminf so that a NaN coordinate also lands on cell 0 (equal to the reference's
@@ -89,21 +89,7 @@ __device__ __forceinline__ void cell_vel(const real* __restrict__ vel, int n3, i
 }
 
 // trace (kernels.py:115-120): RK2 midpoint over signed s (cells per m/s) from the centre of
-// cell i = (px, py, pz).  Two traces (back and forward) run side by side so their gathers overlap.
-__device__ __forceinline__ void trace2(const real* __restrict__ vel, int N, int n3, int i, real px, real py, real pz,
-                                       real sa, real sb, real* qa, real* qb) {
-    real v1[3];
-    cell_vel(vel, n3, i, v1);
-    const real ha = RL(0.5) * sa, hb = RL(0.5) * sb;
-    const Corner ca = corner(N, px + ha * v1[0], py + ha * v1[1], pz + ha * v1[2]);
-    const Corner cb = corner(N, px + hb * v1[0], py + hb * v1[1], pz + hb * v1[2]);
-    real va[3], vb[3];
-    sample_vel(vel, N, n3, ca, va);
-    sample_vel(vel, N, n3, cb, vb);
-    qa[0] = px + sa * va[0]; qa[1] = py + sa * va[1]; qa[2] = pz + sa * va[2];
-    qb[0] = px + sb * vb[0]; qb[1] = py + sb * vb[1]; qb[2] = pz + sb * vb[2];
-}
-
+// cell i = (px, py, pz).
 __device__ __forceinline__ void trace1(const real* __restrict__ vel, int N, int n3, int i, real px, real py, real pz,
                                        real s, real* q) {
     real v1[3];
@@ -117,10 +103,12 @@ __device__ __forceinline__ void trace1(const real* __restrict__ vel, int N, int 
     q[2] = pz + s * v2[2];
 }
 
-// Pass 1: semi-Lagrangian backward value of C channels -> hat.
+// Pass 1: semi-Lagrangian backward value of C channels -> hat, and the base corner of the
+// backward trace -> qidx (pass 2's clamp reads the same corners; the trace is not repeated).
 template <int C>
 __global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ src, const real* __restrict__ vel,
-                                                const float* __restrict__ prm, real inv_dx, real* __restrict__ hat) {
+                                                const float* __restrict__ prm, real inv_dx, real* __restrict__ hat,
+                                                int* __restrict__ qidx) {
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -131,6 +119,7 @@ __global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ 
     real q[3];
     trace1(vel, N, n3, i, (real)x, (real)y, (real)z, -s, q);
     const Corner c = corner(N, q[0], q[1], q[2]);
+    qidx[i] = c.o;
     Oct g[C];
 #pragma unroll
     for (int ch = 0; ch < C; ++ch) g[ch] = gather(src + ch * n3, N, c.o);
@@ -138,13 +127,16 @@ __global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ 
     for (int ch = 0; ch < C; ++ch) hat[ch * n3 + i] = lerp3(g[ch], c);
 }
 
-// Pass 2: MacCormack correction with the 8-corner clamp; optionally the block partial sums of
-// fluid density before (src[0]) and after (out[0]) advection (solver.py:46-65), in double.
+// Pass 2: MacCormack correction with the 8-corner clamp around the backward trace's base corner
+// (qidx, from pass 1; clamped into the box so that any content stays in bounds); optionally the
+// block partial sums of fluid density before (src[0]) and after (out[0]) advection
+// (solver.py:46-65), in double.
 template <int C>
 __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ src, const real* __restrict__ vel,
                                                 const float* __restrict__ prm, real inv_dx,
-                                                const real* __restrict__ hat, real* __restrict__ out,
-                                                const uint8_t* __restrict__ fm, double* __restrict__ part, int nparts) {
+                                                const real* __restrict__ hat, const int* __restrict__ qidx,
+                                                real* __restrict__ out, const uint8_t* __restrict__ fm,
+                                                double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,37 +147,48 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
     if (live) {
         const real s = (real)prm[P_DT] * inv_dx;
         const int i = (z * N + y) * N + x;
-        real q[3], r[3];
-        trace2(vel, N, n3, i, (real)x, (real)y, (real)z, -s, s, q, r);
-        const int qo = (base_idx(q[2], N) * N + base_idx(q[1], N)) * N + base_idx(q[0], N);
+        const int qo = min(max(qidx[i], 0), n3 - N * N - N - 2);
+        real r[3];
+        trace1(vel, N, n3, i, (real)x, (real)y, (real)z, s, r);
         const Corner cr = corner(N, r[0], r[1], r[2]);
-        Oct gs[C], gh[C];
-        real h[C], s0[C];
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            gs[ch] = gather(src + ch * n3, N, qo);
-            gh[ch] = gather(hat + ch * n3, N, cr.o);
-            h[ch] = hat[ch * n3 + i];
-            s0[ch] = src[ch * n3 + i];
-        }
         const int fmi = part != nullptr ? (int)fm[i] : 0;
+        // channels in groups of MC_GROUP: the loads of one group are in flight together, and a
+        // group's registers are free before the next group loads (more waves per SIMD)
+        constexpr int MC_GROUP = C > 3 ? 3 : C;
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            const real bar = lerp3(gh[ch], cr);
-            real val = h[ch] + RL(0.5) * (s0[ch] - bar);
-            real lo = gs[ch].v[0], hi = lo;
+        for (int c0 = 0; c0 < C; c0 += MC_GROUP) {
+        Oct gs[MC_GROUP], gh[MC_GROUP];
+        real h[MC_GROUP], s0[MC_GROUP];
 #pragma unroll
-            for (int k = 1; k < 8; ++k) {
-                const real v = gs[ch].v[k];
-                lo = v < lo ? v : lo;
-                hi = v > hi ? v : hi;
+        for (int k = 0; k < MC_GROUP; ++k) {
+            const int ch = c0 + k < C ? c0 + k : C - 1;
+            gs[k] = gather(src + ch * n3, N, qo);
+            gh[k] = gather(hat + ch * n3, N, cr.o);
+            h[k] = hat[ch * n3 + i];
+            s0[k] = src[ch * n3 + i];
+        }
+#pragma unroll
+        for (int k = 0; k < MC_GROUP; ++k) {
+            const int ch = c0 + k;
+            if (ch >= C) break;
+            const real bar = lerp3(gh[k], cr);
+            real val = h[k] + RL(0.5) * (s0[k] - bar);
+            // wp.min / wp.max over the 8 source values (kernels.py:171-177); lo and hi are only
+            // compared with val, so the sign of a zero bound does not matter
+            real lo = gs[k].v[0], hi = lo;
+#pragma unroll
+            for (int j = 1; j < 8; ++j) {
+                lo = fmin(lo, gs[k].v[j]);
+                hi = fmax(hi, gs[k].v[j]);
             }
-            if (val < lo || val > hi) val = h[ch];
+            if (val < lo || val > hi) val = h[k];
             out[ch * n3 + i] = val;
             if (ch == 0 && fmi != 0) {
                 sa = (double)s0[0];
                 sb = (double)val;
             }
+        }
+        __builtin_amdgcn_sched_barrier(0);
         }
     }
     if (part != nullptr) {
@@ -233,30 +236,30 @@ int advect_parts(const Geom& g) { return (int)(g.grid.x * g.grid.y) * g.N; }
 
 template <int C>
 static void launch_pass_t(int pass, const Geom& g, const real* src, const real* vel, const float* prm, real inv_dx,
-                          real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s) {
+                          real* hat, int* qidx, real* out, const uint8_t* fm, double* part, hipStream_t s) {
     if (pass == 0)
-        hipLaunchKernelGGL(k_adv_sl<C>, cell_grid(g), g.block, 0, s, g.N, src, vel, prm, inv_dx, hat);
+        hipLaunchKernelGGL(k_adv_sl<C>, cell_grid(g), g.block, 0, s, g.N, src, vel, prm, inv_dx, hat, qidx);
     else
-        hipLaunchKernelGGL(k_adv_mc<C>, cell_grid(g), g.block, 0, s, g.N, src, vel, prm, inv_dx, hat, out, fm, part,
-                           advect_parts(g));
+        hipLaunchKernelGGL(k_adv_mc<C>, cell_grid(g), g.block, 0, s, g.N, src, vel, prm, inv_dx, hat, qidx, out, fm,
+                           part, advect_parts(g));
 }
 
 void launch_adv_pass(int pass, const Geom& g, int C, const real* src, const real* vel, const float* prm,
-                     real inv_dx, real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s) {
+                     real inv_dx, real* hat, int* qidx, real* out, const uint8_t* fm, double* part, hipStream_t s) {
     switch (C) {
-        case 1: launch_pass_t<1>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
-        case 2: launch_pass_t<2>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
-        case 3: launch_pass_t<3>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
-        case 4: launch_pass_t<4>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
-        case 5: launch_pass_t<5>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
-        default: launch_pass_t<6>(pass, g, src, vel, prm, inv_dx, hat, out, fm, part, s); break;
+        case 1: launch_pass_t<1>(pass, g, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s); break;
+        case 2: launch_pass_t<2>(pass, g, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s); break;
+        case 3: launch_pass_t<3>(pass, g, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s); break;
+        case 4: launch_pass_t<4>(pass, g, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s); break;
+        case 5: launch_pass_t<5>(pass, g, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s); break;
+        default: launch_pass_t<6>(pass, g, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s); break;
     }
 }
 
 void launch_advect(const Geom& g, int C, const real* src, const real* vel, const float* prm, real inv_dx,
-                   real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s) {
-    launch_adv_pass(0, g, C, src, vel, prm, inv_dx, hat, out, fm, part, s);
-    launch_adv_pass(1, g, C, src, vel, prm, inv_dx, hat, out, fm, part, s);
+                   real* hat, int* qidx, real* out, const uint8_t* fm, double* part, hipStream_t s) {
+    launch_adv_pass(0, g, C, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s);
+    launch_adv_pass(1, g, C, src, vel, prm, inv_dx, hat, qidx, out, fm, part, s);
 }
 
 // part holds 2 * advect_parts(g) partials; part2 (after them) 2 * MASS_PART2 partials.

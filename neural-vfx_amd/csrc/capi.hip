@@ -94,7 +94,8 @@ int vfx::validate_prm(const float* prm) {
 int vfx::stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s) {
     c->P.setup(solid, s);
     CHECK_LAUNCH("obstacle setup");
-    launch_advect(c->g, 6, data, data + c->g.n3, c->prm, c->inv_dx, c->hat, c->adv, c->P.fm, c->mass_part, s);
+    launch_advect(c->g, 6, data, data + c->g.n3, c->prm, c->inv_dx, c->hat, c->qidx(), c->adv, c->P.fm,
+                  c->mass_part, s);
     launch_mass_fin(c->g, c->mass_part, c->st, s);
     CHECK_LAUNCH("advection");
     return VFX_OK;
@@ -592,8 +593,8 @@ int VFX_FN(vfx_advect)(vfx_ctx* c, const real* field, int C, const real* vel, re
     const size_t n3 = c->g.n3;
     for (int c0 = 0; c0 < C; c0 += 6) {
         const int cc = C - c0 < 6 ? C - c0 : 6;
-        launch_advect(c->g, cc, field + c0 * n3, vel, c->prm_adv, c->inv_dx, c->hat, out + c0 * n3, nullptr, nullptr,
-                      s);
+        launch_advect(c->g, cc, field + c0 * n3, vel, c->prm_adv, c->inv_dx, c->hat, c->qidx(), out + c0 * n3,
+                      nullptr, nullptr, s);
     }
     CHECK_LAUNCH("advect");
     return VFX_OK;
@@ -711,18 +712,22 @@ int VFX_FN(vfx_time_kernel)(vfx_ctx* c, int which, int reps, void* stream, float
         const size_t n3 = c->g.n3;
         switch (which) {
             case VFX_K_ADV_SL:
-                launch_adv_pass(0, c->g, 6, c->adv, c->adv + n3, c->prm, c->inv_dx, c->hat, nullptr, nullptr,
-                                nullptr, s);
+                launch_adv_pass(0, c->g, 6, c->adv, c->adv + n3, c->prm, c->inv_dx, c->hat, c->qidx(), nullptr,
+                                nullptr, nullptr, s);
                 return true;
             case VFX_K_ADV_MC:
-                launch_adv_pass(1, c->g, 6, c->adv, c->adv + n3, c->prm, c->inv_dx, c->hat, c->omega, c->P.fm,
-                                c->mass_part, s);
+                launch_adv_pass(1, c->g, 6, c->adv, c->adv + n3, c->prm, c->inv_dx, c->hat, c->qidx(), c->omega,
+                                c->P.fm, c->mass_part, s);
                 return true;
             case VFX_K_SOURCES: launch_sources_pass(0, c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, s); return true;
             case VFX_K_DIFFUSE: launch_sources_pass(1, c->g, c->hat, c->P.fm, c->prm, c->st, c->omega, s); return true;
             default: return which >= VFX_K_MATVEC && c->P.launch_single(which - VFX_K_MATVEC, c->hat, s);
         }
     };
+    // pass 2 of the advection reads pass 1's trace corners: pass 1 runs first on the same input
+    if (which == VFX_K_ADV_MC)
+        launch_adv_pass(0, c->g, 6, c->adv, c->adv + c->g.n3, c->prm, c->inv_dx, c->hat, c->qidx(), nullptr, nullptr,
+                        nullptr, s);
     if (!one()) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);

@@ -33,6 +33,8 @@ struct vfx_ctx {
     // device buffers
     void* pool = nullptr;
     real *hat = nullptr, *adv = nullptr, *omega = nullptr, *omega_s = nullptr, *force = nullptr;
+    // the advection's trace corners (N^3 ints) live in `force`, which is free until the forces stage
+    int* qidx() const { return reinterpret_cast<int*>(force); }
     float* prm = nullptr;      // parameters of the step
     float* prm_adv = nullptr;  // parameters of the standalone advect()
     double* mass_part = nullptr;

@@ -116,10 +116,11 @@ __device__ __forceinline__ real level_scale(const DevState* st) {
 }
 
 // ---------------------------------------------------------------- advection / forces
+// qidx: N^3 ints of scratch (pass 1 writes the backward trace's base corner, pass 2 reads it)
 void launch_advect(const Geom& g, int C, const real* src, const real* vel, const float* prm, real inv_dx,
-                   real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s);
+                   real* hat, int* qidx, real* out, const uint8_t* fm, double* part, hipStream_t s);
 void launch_adv_pass(int pass, const Geom& g, int C, const real* src, const real* vel, const float* prm,
-                     real inv_dx, real* hat, real* out, const uint8_t* fm, double* part, hipStream_t s);
+                     real inv_dx, real* hat, int* qidx, real* out, const uint8_t* fm, double* part, hipStream_t s);
 void launch_sources_pass(int pass, const Geom& g, const real* src, const uint8_t* fm, const float* prm,
                          const DevState* st, real* dst, hipStream_t s);
 void launch_mass_fin(const Geom& g, double* part, DevState* st, hipStream_t s);
