@@ -32,8 +32,8 @@ COPY_PROBE_BYTES = 256 << 20   # device copy of known size that scales the PMC b
 # 32-float table, so the level kernels read a 1-byte flag per level-1 cell (1/8 byte per unknown)
 # instead of diag/off.  The CG kernels are the steady-state forms (iteration k >= 2).
 KERNEL_BYTES_PER_CELL = {
-    "adv_sl": 24 + 24,                  # data (6 ch, velocity 3 of them) -> hat (6 ch)
-    "adv_mc": 24 + 24 + 1 + 24,         # data + hat + mask -> adv
+    "adv_sl": 24 + 24 + 4,              # data (6 ch, velocity 3 of them) -> hat (6 ch), trace corner
+    "adv_mc": 24 + 24 + 4 + 1 + 24,     # data + hat + trace corner + mask -> adv
     "sources": 24 + 1 + 24,             # adv + mask -> hat
     "diffuse": 24 + 1 + 24,             # hat + mask -> data
     "upd_matvec": 4 * 3 + 1 + 4 * 3,    # z, p_old, phi + mask -> p, phi, q
@@ -51,18 +51,15 @@ def launches_per_step(name: str, n: int) -> float:
             "pre_resid_fwd": n, "smooth0": 2 * n, "restrict0": n, "prolong0": n, "coarse": n}.get(name, 0)
 
 
-def step_bytes_per_cell(n: int) -> float:
-    """Algorithmic bytes per cell of one whole step with n CG iterations: advection, sources +
-    diffusion, the vorticity/force chain, the projection's fixed part (rhs, phi final, apply) and
-    n iterations of the fused CG kernels plus the level-1 multigrid (coarser levels: 1/7 more)."""
-    adv = KERNEL_BYTES_PER_CELL["adv_sl"] + KERNEL_BYTES_PER_CELL["adv_mc"]
-    frc = KERNEL_BYTES_PER_CELL["sources"] + KERNEL_BYTES_PER_CELL["diffuse"] + 37
-    fixed = 21 + 8 + 29
-    mg = (2 * KERNEL_BYTES_PER_CELL["smooth0"] + KERNEL_BYTES_PER_CELL["restrict0"]
-          + KERNEL_BYTES_PER_CELL["prolong0"]) * 8 / 7
-    it = (KERNEL_BYTES_PER_CELL["upd_matvec"] + KERNEL_BYTES_PER_CELL["inv_post"]
-          + KERNEL_BYTES_PER_CELL["pre_resid_fwd"] + mg)
-    return adv + frc + fixed + n * it
+def step_bytes_per_cell(n: int, s: int = 4) -> float:
+    """Implementation-independent algorithmic bytes per cell of one whole step with n CG iterations
+    and multigrid level storage of s bytes per value: advection 49 (read 24 + mask 1, write 24),
+    sources + diffusion 49, the vorticity / force chain 37, the projection's fixed cost 58 (rhs 21,
+    apply 29, p init 8), one preconditioner application B_prec = (5 + s) + 15.1 s + (9 + s) =
+    14 + 17.1 s, and per iteration 9 (matvec + dot) + 32 (r, phi, p updates) + B_prec:
+    193 + B_prec + n (41 + B_prec)."""
+    b_prec = 14 + 17.1 * s
+    return 193 + b_prec + n * (41 + b_prec)
 
 
 def pmc_traffic(kernel: str, res: int):
