@@ -146,7 +146,7 @@ __device__ __forceinline__ bool cg_inactive(const DevState* st, int k) { return 
 // ---------------------------------------------------------------- LDS tiles of the fine grid
 // A workgroup owns TX x TY x TZ fine cells (whole 2x2x2 blocks) and stages them with a 2-cell
 // halo, the reach of K = d d^T.  Cells outside the box are staged with mask 0 and value 0.
-constexpr int TX = 32, TY = 8, TZ = VFX_F64 ? 4 : 8;   // FP64 tiles: half as deep (LDS)
+constexpr int TX = 32, TY = 8, TZ = VFX_F64 ? 4 : 16;   // FP64 tiles: a quarter as deep (LDS)
 constexpr int HX = TX + 4, HY = TY + 4, HZ = TZ + 4;
 constexpr int HN = HX * HY * HZ;
 constexpr int TILE_THREADS = (TX / 2) * (TY / 2) * (TZ / 2);   // one thread per own 2x2x2 block
