@@ -93,6 +93,7 @@ struct vfx_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<float*> g_data;
     std::vector<const float*> g_solid;
+    std::vector<char> g_warm;   // warm-start flags the capture was made with (they pick kernel variants)
     bool dirty = true;
     bool frames = false;   // a frame has been enqueued
 };
@@ -210,6 +211,8 @@ int capture(vfx_batch* b, float* const* data, const float* const* solid) {
     }
     b->g_data.assign(data, data + b->n);
     b->g_solid.assign(solid, solid + b->n);
+    b->g_warm.resize(b->n);
+    for (int i = 0; i < b->n; ++i) b->g_warm[i] = b->sims[i]->prm_last[P_WARM] > 0.5f;
     b->dirty = false;
     return VFX_OK;
 }
@@ -221,8 +224,9 @@ int frame(vfx_batch* b, float* const* data, const float* const* solid, const flo
     for (vfx_ctx* c : b->sims) setup_pending = setup_pending || c->P.force_setup;
     if (use_graph && !setup_pending) {
         // a forced operator rebuild is a one-off: it runs eagerly and stays out of the graph
-        const bool same = !b->dirty && b->exec && std::equal(data, data + b->n, b->g_data.begin()) &&
-                          std::equal(solid, solid + b->n, b->g_solid.begin());
+        bool same = !b->dirty && b->exec && std::equal(data, data + b->n, b->g_data.begin()) &&
+                    std::equal(solid, solid + b->n, b->g_solid.begin());
+        for (int i = 0; same && i < b->n; ++i) same = b->g_warm[i] == (b->sims[i]->prm_last[P_WARM] > 0.5f);
         if (!same)
             if (int rc = capture(b, data, solid)) return rc;
         if (!hip_ok(hipGraphLaunch(b->exec, main), "frame launch")) return VFX_E_HIP;

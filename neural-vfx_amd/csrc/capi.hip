@@ -299,7 +299,7 @@ static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) 
 int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
     Pressure& P = c->P;
     real* vel = data + c->g.n3;
-    P.rhs(vel, c->prm, s);
+    P.rhs(vel, c->prm, c->prm_last[P_WARM] > 0.5f, s);
     CHECK_LAUNCH("pressure rhs");
     int upper = 0;
     int rc = VFX_OK;
@@ -324,7 +324,7 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
             rc = upload(c, c->prm_last, c->prm, s);
             c->prm_last[P_WARM] = saved;
             if (rc) return rc;
-            P.rhs(vel, c->prm, s);
+            P.rhs(vel, c->prm, true, s);
             CHECK_LAUNCH("fallback rhs");
         } else {
             hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);   // no MGPCG iterations
@@ -358,7 +358,7 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
             rc = upload(c, c->prm_last, c->prm, s);
             c->prm_last[P_WARM] = saved;
             if (rc) return rc;
-            P.rhs(vel, c->prm, s);
+            P.rhs(vel, c->prm, true, s);
             CHECK_LAUNCH("pressure restart rhs");
             rc = run_cg(c, s, &upper);
             if (rc) return rc;

@@ -133,29 +133,31 @@ __device__ __forceinline__ real inv_sum(real acc, const real* hx, int dz, int dy
 // 2^-e with 2^e <= v < 2^(e+1): brings v into [1, 2) exactly (1 for v = 0 or non-finite)
 __device__ __forceinline__ double pow2_scale(double v) { return (v > 0.0 && v < 1e300) ? ldexp(1.0, -ilogb(v)) : 1.0; }
 
-// k_rhs (:602-627): r = -d(u*) - K phi0 (phi = 0 without warm start); block max |u*| over fluid,
+// k_rhs (:602-627): r = -d(u*) - K phi0 (WARM; otherwise phi = 0); block max |u*| over fluid,
 // block sum of b^2 (b = -d(u*)) for the relative-residual test and block max |r| (the scale of
-// the first V-cycle's right-hand side with 16-bit level storage).
+// the first V-cycle's right-hand side with 16-bit level storage).  Every load is issued
+// unconditionally (clamped neighbours; a solid cell's terms are computed and discarded), so they
+// are all in flight together.
+template <bool WARM>
 __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restrict__ vel,
                                              const uint8_t* __restrict__ fm, real* __restrict__ r,
-                                             real* __restrict__ phi, const float* __restrict__ prm,
-                                             double* __restrict__ part, int nparts) {
+                                             real* __restrict__ phi, double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
     const int n3 = N * N * N;
-    const bool warm = prm[P_WARM] > RL(0.5);
     double mx = 0.0, bb = 0.0, mr = 0.0;
     CELL_LOOP_BEGIN
-        if (!warm) phi[i] = RL(0.0);
         const int m = fm[i];
-        if (m == 0) { r[i] = RL(0.0); continue; }
         const real b = -div2_xyz(vel, n3, m, N, x, y, cz, i);
         real rv = b;
-        if (warm) rv -= kmul_xyz(phi, fm, N, x, y, cz, i, m);
-        r[i] = rv;
-        bb += (double)b * (double)b;
-        mr = fmax(mr, fabs((double)rv));
+        if (WARM) rv -= kmul_xyz(phi, fm, N, x, y, cz, i, m);
         const real ux = vel[i], uy = vel[n3 + i], uz = vel[2 * n3 + i];
-        mx = fmax(mx, (double)sqrt(ux * ux + uy * uy + uz * uz));
+        const double sp = (double)sqrt(ux * ux + uy * uy + uz * uz);
+        const bool fl = m != 0;
+        if (!WARM) phi[i] = RL(0.0);
+        r[i] = fl ? rv : RL(0.0);
+        bb += fl ? (double)b * (double)b : 0.0;
+        mr = fl ? fmax(mr, fabs((double)rv)) : mr;
+        mx = fl ? fmax(mx, sp) : mx;
     CELL_LOOP_END
     const double a = block_max(mx, sh);
     const double b2 = block_sum(bb, sh);
@@ -562,10 +564,9 @@ __global__ void __launch_bounds__(256) k_true_res(int N, int zc, const real* __r
     const int n3 = N * N * N;
     double s = 0.0;
     CELL_LOOP_BEGIN
-        const int m = fm[i];
-        if (m == 0) continue;
+        const int m = fm[i];   // a solid cell's terms are computed (clamped loads) and discarded
         const double v = -(double)div2_xyz(vel, n3, m, N, x, y, cz, i) - (double)kmul_xyz(phi, fm, N, x, y, cz, i, m);
-        s += v * v;
+        s += m != 0 ? v * v : 0.0;
     CELL_LOOP_END
     const double t = block_sum(s, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = t;
@@ -626,9 +627,12 @@ void Pressure::precond(int k, hipStream_t s) {
     });
 }
 
-void Pressure::rhs(const real* vel, const float* prm, hipStream_t s) {
+void Pressure::rhs(const real* vel, const float* prm, bool warm, hipStream_t s) {
     const Geom& g = geom;
-    hipLaunchKernelGGL(k_rhs, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, prm, part, nblocks(g));
+    if (warm)
+        hipLaunchKernelGGL(k_rhs<true>, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, part, nblocks(g));
+    else
+        hipLaunchKernelGGL(k_rhs<false>, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, part, nblocks(g));
     hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), prm, st);
 }
 

@@ -163,7 +163,8 @@ struct Pressure {
     size_t level_floats() const;
     void bind_levels(real* base);
     void setup(const real* solid, hipStream_t s);
-    void rhs(const real* vel, const float* prm, hipStream_t s);
+    // warm: the host's copy of the P_WARM parameter (the kernel variant is chosen on the host)
+    void rhs(const real* vel, const float* prm, bool warm, hipStream_t s);
     void precond(int k, hipStream_t s);
     void iteration(int k, const float* prm, hipStream_t s);
     void finish(hipStream_t s);
