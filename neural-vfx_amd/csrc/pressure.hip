@@ -363,16 +363,36 @@ __global__ void __launch_bounds__(256) k_smooth_half(LvlT<LT> L, int color, cons
     smooth_item<true>(L, c, K, J, I, false);
 }
 
+// Restriction of the level below C.  For m >= 32 the coarse cells go by columns: workgroup b
+// (XCD-renumbered) owns 32 cells of one row (8 components each) and consecutive workgroups walk
+// up K, then J, then along I, so the fine planes two coarse planes share are read again while
+// they are still in the L2.  Smaller levels use the natural order.
 template <class LT>
 __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do_red, const DevState* __restrict__ st,
                                                   int k) {
     if (cg_inactive(st, k)) return;
     const int m = C.m;
-    const size_t t = xcd_item();
-    if (t >= (size_t)m * m * m * 8) return;
     int c, K, J, I;
-    full_item((uint32_t)t, (uint32_t)m, c, K, J, I);
+    if (m >= 32) {
+        const uint32_t um = (uint32_t)m, lb = (uint32_t)xcd_block((int)blockIdx.x, (int)gridDim.x);
+        const uint32_t col = lb / um, seg = col / um;
+        K = (int)(lb - col * um);
+        J = (int)(col - seg * um);
+        I = (int)(seg * 32u + (threadIdx.x >> 3));
+        c = (int)(threadIdx.x & 7u);
+        if (I >= m) return;
+    } else {
+        const size_t t = xcd_item();
+        if (t >= (size_t)m * m * m * 8) return;
+        full_item((uint32_t)t, (uint32_t)m, c, K, J, I);
+    }
     restrict_item<true>(F, C, c, K, J, I, do_red != 0);
+}
+
+// workgroups of a k_restrict launch onto coarse size m
+static unsigned restrict_blocks(int m) {
+    if (m >= 32) return (unsigned)m * (unsigned)m * (unsigned)((m + 31) / 32);
+    return (unsigned)(((size_t)m * m * m * 8 + 255) / 256);
 }
 
 template <class LT>
@@ -605,8 +625,7 @@ void Pressure::vcycle_levels(int k, hipStream_t s) {
             const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
             hipLaunchKernelGGL(k_smooth_half<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
                                lvl_as<LT>(lv[l]), 1, st, k);
-            const size_t items = (size_t)lv[l + 1].m * lv[l + 1].m * lv[l + 1].m * 8;
-            hipLaunchKernelGGL(k_restrict<LT>, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[l + 1].m)), dim3(256), 0, s,
                                lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), (l + 1 < S) ? 1 : 0, st, k);
         }
         launch_coarse<LT>(*this, s, k);
@@ -634,8 +653,7 @@ bool Pressure::launch_single_mg(int which, hipStream_t s) {
                                    lvl_as<LT>(lv[0]), 1, st, 0);
                 break;
             case 3: {
-                const size_t items = (size_t)lv[1].m * lv[1].m * lv[1].m * 8;
-                hipLaunchKernelGGL(k_restrict<LT>, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s,
+                hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[1].m)), dim3(256), 0, s,
                                    lvl_as<LT>(lv[0]), lvl_as<LT>(lv[1]), S > 1 ? 1 : 0, st, 0);
                 break;
             }
