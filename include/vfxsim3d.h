@@ -52,6 +52,12 @@ extern "C" {
                                     and the ones after the stop test are no-ops on the device */
 #define VFX_OPT_CELLS_PER_THREAD 10  /* z-cells per thread of the cell kernels: 1 (default), 2, 4, 8;
                                     results equal up to FP64 reduction order */
+#define VFX_OPT_PRESSURE_MODE 11 /* 0 MGPCG (default); 1 the fused projection (pressure_fused.py):
+                                    sweeps + V-cycles as a stationary iteration, adaptive tail while
+                                    P_TOL > 0 (up to P_MAXIT cycles); FP32 / FP64 level storage */
+#define VFX_OPT_FUSED_CYCLES 12  /* fused: V-cycles always run (pressure_cycles, default 2) */
+#define VFX_OPT_FUSED_SMOOTH 13  /* fused: exact-K sweeps before / between / after (pressure_smooth, 4) */
+#define VFX_OPT_FUSED_SMOOTHER 14 /* fused: 0 Chebyshev weights on [0.5, 2] (default), 1 Jacobi 0.8 */
 
 typedef struct vfx_ctx vfx_ctx;
 

@@ -299,6 +299,19 @@ static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) 
 int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
     Pressure& P = c->P;
     real* vel = data + c->g.n3;
+    if (c->fused) {
+        c->last_iters = 0;
+        c->last_restarts = 0;
+        c->last_true_valid = false;
+#if !VFX_F64
+        c->last_learned = 0;
+        c->last_fallback = false;
+#endif
+        if (int rc = run_fused(c, vel, s, in_frame)) return rc;
+        if (!in_frame) CHECK_HIP(hipEventRecord(c->done_ev, s), "done event");
+        c->projected = true;
+        return VFX_OK;
+    }
     P.rhs(vel, c->prm, c->prm_last[P_WARM] > 0.5f, s);
     CHECK_LAUNCH("pressure rhs");
     int upper = 0;
@@ -498,6 +511,22 @@ int VFX_FN(vfx_set_option)(vfx_ctx* c, int key, double value) {
             c->learned_maxit = (int)value;
             return VFX_OK;
 #endif
+        case VFX_OPT_PRESSURE_MODE:
+            if (value != 0.0 && value != 1.0) return arg_error("pressure mode must be 0 (pcg) or 1 (fused)");
+            c->fused = value != 0.0;
+            return VFX_OK;
+        case VFX_OPT_FUSED_CYCLES:
+            if (value < 0 || value > 1000) return arg_error("fused cycles must be 0..1000");
+            c->fused_cycles = (int)value;
+            return VFX_OK;
+        case VFX_OPT_FUSED_SMOOTH:
+            if (value < 1 || value > 64) return arg_error("fused sweeps must be 1..64");
+            c->fused_smooth = (int)value;
+            return VFX_OK;
+        case VFX_OPT_FUSED_SMOOTHER:
+            if (value != 0.0 && value != 1.0) return arg_error("fused smoother must be 0 (chebyshev) or 1 (jacobi)");
+            c->fused_jacobi = (int)value;
+            return VFX_OK;
         case VFX_OPT_CG_BUDGET:
             if (value < 0 || value > 1000) return arg_error("CG budget must be 0 (host-polled) or 1..1000");
             c->cg_budget = (int)value;

@@ -54,6 +54,9 @@ struct vfx_ctx {
     bool opt_true_res = false;
     int opt_restarts = 0;
     int cg_budget = 0;   // > 0: exactly this many PCG iterations are enqueued, with no host polling
+    // pressure_mode "fused" (stationary multigrid, no CG): cycles, sweeps, smoother (0 Chebyshev, 1 Jacobi)
+    bool fused = false;
+    int fused_cycles = 2, fused_smooth = 4, fused_jacobi = 0;
 #if !VFX_F64
     vfx_net* net = nullptr;     // learned preconditioner of flexible CG (not owned)
     int fcg_mode = 0;           // flexible CG without a network: 1 MG V-cycle, 2 identity (plain CG)
@@ -80,5 +83,7 @@ int stage_forces(vfx_ctx* c, real* data, hipStream_t s);
 // in_frame: part of a batched frame (fixed CG budget, no host synchronisation, no restarts, the
 // caller records done_ev)
 int stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame = false);
+// the fused projection of the velocity vel (fused.hip); phi is the context's warm start
+int run_fused(vfx_ctx* c, real* vel, hipStream_t s, bool in_frame);
 
 }  // namespace vfx

@@ -17,6 +17,7 @@ STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
 OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL, OPT_MG_STORAGE = 1, 2, 3, 4, 5
 OPT_FCG_MG, OPT_LEARNED_MAXIT, OPT_LEARNED_SWEEPS = 6, 7, 8
 OPT_CG_BUDGET, OPT_CELLS_PER_THREAD = 9, 10
+OPT_PRESSURE_MODE, OPT_FUSED_CYCLES, OPT_FUSED_SMOOTH, OPT_FUSED_SMOOTHER = 11, 12, 13, 14
 TUNE_MAX_SIZES, TUNE_REPORT = 8, 46
 MG_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
 # kernel ids of vfx_time_kernel (include/vfxsim3d.h VFX_K_*); "" = unused id

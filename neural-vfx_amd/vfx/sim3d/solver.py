@@ -128,6 +128,14 @@ class Solver3D:
         if kind is None:
             raise ValueError(f"mg_storage must be one of {sorted(_lib.MG_STORAGE)}, got {self.config.mg_storage!r}")
         self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_MG_STORAGE, float(kind)))
+        c = self.config
+        fused = c.pressure_mode == "fused"
+        self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_PRESSURE_MODE, 1.0 if fused else 0.0))
+        if fused:
+            self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_FUSED_CYCLES, float(max(0, int(c.pressure_cycles)))))
+            self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_FUSED_SMOOTH, float(max(1, int(c.pressure_smooth)))))
+            self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_FUSED_SMOOTHER,
+                                               1.0 if c.pressure_smoother == "jacobi" else 0.0))
 
     def set_option(self, key: int, value: float) -> None:
         """Library option (vfxsim3d.h VFX_OPT_*): true-residual report, restarts, coarse size."""
@@ -238,7 +246,7 @@ class Solver3D:
         return out[0] if squeeze else out
 
     def pressure_iters(self) -> int:
-        """PCG iterations of the last projection (synchronizes)."""
+        """PCG iterations (default) or V-cycles (fused) of the last projection (synchronizes)."""
         n = ctypes.c_int()
         self._chk(self._api.vfx_pressure_iters(self._ctx, ctypes.byref(n)))
         return int(n.value)
@@ -281,7 +289,12 @@ class Solver3D:
             return ()
         if c.pressure_mode != "fused":
             raise ValueError(f"pressure_mode must be 'pcg', 'learned' or 'fused', got {c.pressure_mode!r}")
-        raise NotImplementedError("pressure_mode='fused' (the reference's opt-in stationary MG) is not built")
+        if getattr(self, "_net", None) is not None:
+            raise ValueError("pressure_mode='fused' runs without the learned preconditioner")
+        if c.pressure_smoother not in ("chebyshev", "jacobi"):
+            raise ValueError(f"pressure_smoother must be 'chebyshev' or 'jacobi', got {c.pressure_smoother!r}")
+        return ("fused", int(c.pressure_cycles), int(c.pressure_smooth), str(c.pressure_smoother),
+                c.pressure_tol > 0, int(c.pressure_max_iters))
 
     def _stream(self) -> ctypes.c_void_p:
         return ctypes.c_void_p(torch.cuda.current_stream(torch.device(self.device)).cuda_stream)

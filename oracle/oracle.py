@@ -25,6 +25,7 @@ Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -248,6 +249,29 @@ def project(data, solid, prm, phi, prec="f32"):
     it = _fn("or_project", prec, ctypes.c_int)(ctypes.c_int(solid.shape[-1]), _p(data), _p(solid), _p(prm), _p(phi),
                                                _p(stats))
     return int(it), {"b_norm": stats[0], "r_norm": stats[1], "max_u": stats[2]}
+
+
+def project_fused(data, solid, prm, phi, cycles, smooth, smoother="chebyshev", prec="f32"):
+    """The fused (stationary multigrid) projection of data[1:4] in place, phi updated in place
+    (pressure_fused.py:398-489).  Returns (cycles run, {"max_u", "max_r"})."""
+    assert data.dtype == _dt(prec) and data.flags.c_contiguous
+    assert phi.dtype == _dt(prec) and phi.flags.c_contiguous
+    solid = _a(solid, _dt(prec))
+    prm = _a(prm, np.float32)
+    stats = np.zeros(2, np.float64)
+    it = _fn("or_project_fused", prec, ctypes.c_int)(ctypes.c_int(solid.shape[-1]), _p(data), _p(solid), _p(prm),
+                                                     _p(phi), ctypes.c_int(int(cycles)), ctypes.c_int(int(smooth)),
+                                                     ctypes.c_int(1 if smoother == "jacobi" else 0), _p(stats))
+    return int(it), {"max_u": stats[0], "max_r": stats[1]}
+
+
+def fused_weights(smooth, smoother="chebyshev"):
+    """Sweep weights of the fused projection (chebyshev_weights, pressure_fused.py:348-352)."""
+    n = max(1, int(smooth))
+    if smoother == "jacobi":
+        return [0.8] * n
+    lo, hi = 0.5, 2.0
+    return [1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * math.cos(math.pi * (2 * i + 1) / (2 * n))) for i in range(n)]
 
 
 def stage_advect(data, solid, prm, inv_dx, prec="f32"):

@@ -998,6 +998,134 @@ int F(or_project)(int N, R* data, const R* solid, const float* prm, R* phi, doub
     return it;
 }
 
+/* The fused projection (FusedPressure3D.record, pressure_fused.py:398-489, with its default
+ * kernel boundaries FUSE_TOP = False / FUSE_MIN_M = inf, :491-499): no CG.  `smooth` weighted
+ * Jacobi sweeps on the exact K (k_sweep :41-57; weights: Chebyshev :348-352 or 0.8), then
+ * `cycles` x (V-cycle correction phi += T^T V(b - K phi), k_resid_phi / k_fwd_red / V(1,1) /
+ * k_inv_transform, + `smooth` sweeps); with tol > 0 (adaptive) more cycles while
+ * 0.5 max|b - K phi| >= tol max|u*| and fewer than P_MAXIT cycles have run (k_test :335-347,
+ * the tail :463-487); then u -= g(phi) (k_apply_copy :60-72).  A fluid cell sealed in by solids
+ * (kdiag = 0) keeps its phi in a sweep (the reference divides 0 / 0 there).  Returns the cycles
+ * run; stats[0] = max|u*|, stats[1] = max|b - K phi| at the last test (-1 without the tail). */
+static void F(fused_weights)(int smooth, int jacobi, float* w) {
+    const int n = smooth < 1 ? 1 : smooth;
+    const double lo = 2.0 / 4.0, hi = 2.0;
+    for (int i = 0; i < n; ++i)
+        w[i] = jacobi ? 0.8f
+                      : (float)(1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * cos(3.14159265358979323846 * (2 * i + 1) / (2.0 * n))));
+}
+
+static void F(fused_sweep)(int N, const int* fm, const R* vel, const R* src, R* dst, R w, int cold) {
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                size_t i = IDX(z, y, x);
+                int m = fm[i];
+                if (m == 0) { dst[i] = 0; continue; }
+                if ((m & 63) == 0) { dst[i] = cold ? (R)0 : src[i]; continue; }
+                R b = -F(div2)(vel, fm, N, z, y, x);
+                if (cold) dst[i] = w * b / F(kdiag)(m);
+                else dst[i] = src[i] + w * (b - F(kmul)(src, fm, N, z, y, x)) / F(kdiag)(m);
+            }
+}
+
+/* rr = b - K phi (0 in solids); returns max|rr| (as R) */
+static R F(fused_resid)(int N, const int* fm, const R* vel, const R* phi, R* rr) {
+    R mx = 0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                size_t i = IDX(z, y, x);
+                R v = 0;
+                if (fm[i] != 0) v = -F(div2)(vel, fm, N, z, y, x) - F(kmul)(phi, fm, N, z, y, x);
+                rr[i] = v;
+                R a = v < 0 ? -v : v;
+                mx = a > mx ? a : mx;
+            }
+    return mx;
+}
+
+int F(or_project_fused)(int N, R* data, const R* solid, const float* prm, R* phi, int cycles, int smooth, int jacobi,
+                        double* stats) {
+    size_t n3 = (size_t)N * N * N;
+    int* fm = (int*)calloc(n3, sizeof(int));
+    F(or_mask)(N, solid, fm);
+    F(levels_t) L;
+    F(levels_init)(&L, N);
+    F(levels_build)(&L, N, fm);
+    R* vel = data + n3;
+    R* alt = (R*)calloc(n3, sizeof(R));
+    R* rr = (R*)calloc(n3, sizeof(R));
+    const int warm = prm[P_WARM] > 0.5f;
+    const int ns = smooth < 1 ? 1 : smooth;
+    float w[64];
+    F(fused_weights)(ns > 64 ? 64 : ns, jacobi, w);
+    R* cur = phi;
+    int first = 1;
+#define OTHER(p) ((p) == phi ? alt : phi)
+#define SWEEPS()                                                                        \
+    do {                                                                                \
+        for (int k_ = 0; k_ < ns && k_ < 64; ++k_) {                                    \
+            R* d_ = OTHER(cur);                                                         \
+            F(fused_sweep)(N, fm, vel, cur, d_, (R)w[k_], first && !warm);              \
+            cur = d_;                                                                   \
+            first = 0;                                                                  \
+        }                                                                               \
+    } while (0)
+    SWEEPS();
+    for (int c = 0; c < cycles; ++c) {
+        F(fused_resid)(N, fm, vel, cur, rr);
+        F(fwd_transform)(N, rr, L.r[0]);
+        F(mg_vcycle)(&L);
+        R* d = OTHER(cur);
+        F(inv_transform)(N, L.x[0], fm, cur, d);
+        cur = d;
+        SWEEPS();
+    }
+    int it = cycles;
+    double rlast = -1.0, umax = 0.0;
+    const float tol = prm[P_TOL];
+    const int maxit = (int)prm[P_MAXIT];
+    if (tol > 0.0f && maxit > cycles) {
+        R um = 0;
+        for (size_t i = 0; i < n3; ++i) {
+            if (fm[i] == 0) continue;
+            R ux = vel[i], uy = vel[n3 + i], uz = vel[2 * n3 + i];
+            R sp = SQRTR(ux * ux + uy * uy + uz * uz);
+            um = sp > um ? sp : um;
+        }
+        umax = (double)um;
+        R rmax = F(fused_resid)(N, fm, vel, cur, rr);
+        rlast = (double)rmax;
+        int go = it < maxit && (float)0.5f * (float)rmax >= tol * (float)um && um > 0;
+        R* loop = cur;
+        while (go) {
+            F(fwd_transform)(N, rr, L.r[0]);
+            F(mg_vcycle)(&L);
+            F(inv_transform)(N, L.x[0], fm, loop, loop);
+            cur = loop;
+            first = 0;
+            SWEEPS();
+            if (cur != loop) memcpy(loop, cur, n3 * sizeof(R));
+            cur = loop;
+            rmax = F(fused_resid)(N, fm, vel, cur, rr);
+            rlast = (double)rmax;
+            it += 1;
+            go = it < maxit && (float)0.5f * (float)rmax >= tol * (float)um && um > 0;
+        }
+    }
+#undef SWEEPS
+#undef OTHER
+    F(or_apply)(N, fm, cur, vel);
+    if (cur != phi) memcpy(phi, cur, n3 * sizeof(R));
+    if (stats) { stats[0] = umax; stats[1] = rlast; }
+    free(alt); free(rr); free(fm);
+    F(levels_free)(&L);
+    return it;
+}
+
 /* ---------------------------------------------------------------- stages of Solver3D.step
  * _record_advect solver.py:422-430: MacCormack of all 6 channels by the state's own velocity,
  * then the mass ratio.  data -> adv (6ch); returns the ratio. */

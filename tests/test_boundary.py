@@ -102,5 +102,6 @@ def test_unbuilt_modes_fail_loudly():
         sim.Solver3D(32, scalar_res=64)
     with pytest.raises(ValueError):
         sim.Solver3D(32, sim.SolverConfig3D(pressure_mode="bogus"))
-    with pytest.raises(NotImplementedError):
-        sim.Solver3D(32, sim.fused_config())
+    # the fused mode is built; its smoother name is validated before any device work
+    with pytest.raises(ValueError):
+        sim.Solver3D(32, sim.fused_config(pressure_smoother="bogus"))

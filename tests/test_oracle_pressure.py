@@ -296,3 +296,90 @@ def test_reference_stop_test_and_warm_start():
     it2, _ = oracle.project(data2, solid, prm, phi)
     assert it2 == 0
     np.testing.assert_allclose(data2[1:4], data[1:4], atol=1e-5 * np.abs(u).max())
+
+
+# ---------------------------------------------------------------- the fused projection
+def test_fused_weights_are_chebyshev():
+    """The sweep weights are the reciprocals of the degree-n Chebyshev nodes on [0.5, 2]
+    (numpy's chebroots), so the sweep polynomial prod(1 - w lambda) has the minimax value
+    1 / T_n(5/3) on that interval (pressure_fused.py:348-352)."""
+    for n in (1, 2, 4, 6):
+        w = np.array(oracle.fused_weights(n))
+        t = np.polynomial.chebyshev.chebroots([0] * n + [1])
+        nodes = np.sort(1.25 + 0.75 * t)
+        np.testing.assert_allclose(np.sort(1.0 / w), nodes, rtol=1e-12)
+        lam = np.linspace(0.5, 2.0, 20001)
+        p = np.prod(1.0 - w[:, None] * lam[None, :], axis=0)
+        tn = np.cosh(n * np.arccosh(5.0 / 3.0))
+        assert abs(np.abs(p).max() - 1.0 / tn) < 1e-6
+    assert oracle.fused_weights(3, "jacobi") == [0.8, 0.8, 0.8]
+
+
+def _kdiag_np(fm):
+    s = np.zeros(fm.shape)
+    for ax in range(3):
+        oR = (fm >> (2 * ax)) & 1
+        oL = (fm >> (2 * ax + 1)) & 1
+        s += (oL - oR) ** 2 + oL + oR
+    return s
+
+
+def test_fused_single_sweep_is_weighted_jacobi():
+    """cycles = 0, one sweep from a warm start: phi_1 = phi_0 + w D^-1 (b - K phi_0) with
+    b = -d(u*), then u -= g(phi_1) (assembled here from the oracle's operators)."""
+    N = 12
+    solid = sphere_solid(N, r=0.25)
+    fm = oracle.mask(solid)
+    u = (smooth_noise(N, 3, 6, amp=1.0) * (solid < 0.5)).astype(np.float64)
+    phi0 = smooth_noise(N, 1, 7)[0].astype(np.float64) * (fm != 0)
+    cfg = SolverConfig3D(pressure_tol=0.0, pressure_warm_start=True)
+    data = np.zeros((6, N, N, N))
+    data[1:4] = u
+    phi = phi0.copy()
+    it, _ = oracle.project_fused(data, solid, pack_params(cfg, N, Forces3D(), 0.0), phi, 0, 1, prec="f64")
+    assert it == 0
+    w = oracle.fused_weights(1)[0]
+    b = -oracle.div2(fm, u, prec="f64")
+    w32 = float(np.float32(w))
+    ref = np.where(fm != 0, phi0 + w32 * (b - oracle.kmul(fm, phi0, prec="f64")) / np.maximum(_kdiag_np(fm), 1), 0.0)
+    np.testing.assert_allclose(phi, ref, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(data[1:4], oracle.apply(fm, ref, u, prec="f64"), rtol=1e-13, atol=1e-14)
+
+
+def test_fused_adaptive_converges_to_exact_solve():
+    """With the adaptive tail (tolerance 1e-10, at most 200 cycles) the stationary multigrid
+    iteration reaches the exact projection; with a fixed 2 cycles it is already close."""
+    N = 12
+    solid = sphere_solid(N, r=0.25)
+    u = (smooth_noise(N, 3, 2, amp=2.0) + np.array([0.8, -0.3, 0.4], np.float32)[:, None, None, None])
+    u = u.astype(np.float64) * (solid < 0.5)
+    ref, _ = _exact_projection(solid, u)
+    errs = {}
+    for name, tol, maxit in (("fixed", 0.0, 24), ("adaptive", 1e-10, 200)):
+        cfg = SolverConfig3D(pressure_tol=tol, pressure_max_iters=maxit, pressure_warm_start=False)
+        data = np.zeros((6, N, N, N))
+        data[1:4] = u
+        it, st = oracle.project_fused(data, solid, pack_params(cfg, N, Forces3D(), 0.0), np.zeros((N,) * 3), 2, 4,
+                                      prec="f64")
+        errs[name] = (it, float(np.abs(data[1:4] - ref).max() / np.abs(u).max()), st)
+    assert errs["fixed"][0] == 2 and errs["fixed"][1] < 0.05
+    it, e, st = errs["adaptive"]
+    assert 2 < it < 200 and 0.5 * st["max_r"] < 1e-10 * st["max_u"]
+    assert e < 1e-8
+
+
+def test_fused_sealed_cell_stays_finite():
+    """A sealed fluid cell (kdiag = 0) keeps its phi in the sweeps: the fused projection stays
+    finite and matches the exact solve on the rest."""
+    N, c = 12, (3, 4, 3)
+    solid = sealed_cell_solid(N, c)
+    u = (smooth_noise(N, 3, 4, amp=2.0) + np.array([0.5, 0.2, -0.4], np.float32)[:, None, None, None])
+    u = u.astype(np.float64) * (solid < 0.5)
+    ref, _ = _exact_projection(solid, u)
+    cfg = SolverConfig3D(pressure_tol=1e-10, pressure_max_iters=300, pressure_warm_start=False)
+    data = np.zeros((6, N, N, N))
+    data[1:4] = u
+    phi = np.zeros((N,) * 3)
+    it, _ = oracle.project_fused(data, solid, pack_params(cfg, N, Forces3D(), 0.0), phi, 2, 4, prec="f64")
+    assert np.isfinite(data).all() and np.isfinite(phi).all()
+    assert np.abs(data[1:4] - ref).max() < 1e-7 * np.abs(u).max()
