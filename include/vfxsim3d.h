@@ -31,6 +31,7 @@ extern "C" {
 #define VFX_STAGE_ADVECT 0   /* obstacle setup + MacCormack advection + mass ratio: solver.py:422-430 */
 #define VFX_STAGE_FORCES 1   /* sources, diffusion, vorticity confinement, forces: solver.py:432-455 */
 #define VFX_STAGE_PROJECT 2  /* pressure projection: pressure.py:940-950 */
+#define VFX_STAGE_SCALARS 3  /* the finer scalar grid's stage (solver.py:241-259); a no-op without one */
 
 /* options of vfx_set_option */
 #define VFX_OPT_TRUE_RESIDUAL 1  /* 1: compute ||b - K phi|| / ||b|| after each projection */
@@ -103,7 +104,8 @@ int vfx_set_option(vfx_ctx* ctx, int key, double value);
 int vfx_get_phi(vfx_ctx* ctx, float* dst, void* stream);
 int vfx_set_phi(vfx_ctx* ctx, const float* src, void* stream);
 
-/* Copy the advected state of the last advection stage (6,N,N,N) and the mass ratio. */
+/* Copy the advected state of the last advection stage (6,N,N,N) and the mass ratio.  With a finer
+ * scalar grid attached only the velocity channels 1..3 are advected on this grid. */
 int vfx_get_advected(vfx_ctx* ctx, float* dst, double* mass_ratio, void* stream);
 
 /* Operator access for parity tests: rebuild masks and multigrid operators for `solid`, then
@@ -128,11 +130,30 @@ int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float*
 #define VFX_K_PRE 9        /* r -= alpha q, pre-Jacobi, residual, Hadamard transform, red sweep
                               (k_pre + k_resid + k_fwd_red, pressure.py:648-685,426-441) */
 #define VFX_K_COARSE 14    /* single-workgroup coarse V-cycle (k_coarse_block, pressure.py:550-591) */
+#define VFX_K_FADV_SL 15   /* fine scalar grid: semi-Lagrangian pass (advect_sl_fine, hires.py:29-48) */
+#define VFX_K_FADV_MC 16   /* fine scalar grid: MacCormack pass + mass partials (advect_mc_fine, hires.py:51-89) */
+#define VFX_K_FSOURCES 17  /* fine scalar grid: sources (sources_fine, hires.py:92-139) */
+#define VFX_K_FDIFFUSE 18  /* fine scalar grid: one diffusion substep (diffuse_fine, hires.py:151-170) */
+#define VFX_K_FRESTRICT 19 /* fine -> coarse box restriction (restrict_scalars, hires.py:179-219) */
 
 /* Average device time (ms) of `reps` back-to-back launches of one kernel on the solver's current
  * buffers, bracketed by HIP events on `stream` (measurement support for the benchmark; call it
  * after a step so the buffers hold live data).  Leaves the solver state as it found it. */
 int vfx_time_kernel(vfx_ctx* ctx, int which, int reps, void* stream, float* ms_avg);
+
+/* ---------------------------------------------------------------- finer scalar grid
+ * Solver3D(res, ..., scalar_res=S) (solver.py:86-125, :187-264; hires.py).  The caller owns the fine
+ * fields: hi (3, S, S, S) float32 density, temperature (K), fuel, and hi_solid (S, S, S) float32,
+ * 1 = obstacle, both covering the simulation's world box.  Once attached, vfx_step carries the
+ * scalars on the fine grid by the res^3 velocity and box-restricts them into data's density,
+ * temperature and fuel channels before the forces read them.  S >= res (VFX_E_VALUE otherwise, the
+ * reference's ValueError, solver.py:219-220), S <= 1024.  Allocates the fine scratch
+ * (7 S^3 floats + S^3 bytes); S = 0 detaches and frees it.  hi_solid is read every step. */
+int vfx_hires_attach(vfx_ctx* ctx, int S, float* hi, const float* hi_solid);
+
+/* Solver3D.init_hires_from_state (solver.py:204-215): hi = the density / temperature / fuel of
+ * `data` resized trilinearly to S^3 (half-pixel centres), 0 / ambient / 0 where hi_solid >= 0.5. */
+int vfx_hires_init(vfx_ctx* ctx, const float* data, float ambient, void* stream);
 
 /* Message of the last failed call on this thread. */
 int vfx_last_error(char* buf, int n);
@@ -241,6 +262,8 @@ int vfx_precond_f64(vfx_ctx_f64* ctx, const double* solid, const double* r, doub
 int vfx_get_ops_f64(vfx_ctx_f64* ctx, const double* solid, int level, double* diag, double* off, int* nlev, int* m,
                     void* stream);
 int vfx_time_kernel_f64(vfx_ctx_f64* ctx, int which, int reps, void* stream, float* ms_avg);
+int vfx_hires_attach_f64(vfx_ctx_f64* ctx, int S, double* hi, const double* hi_solid);
+int vfx_hires_init_f64(vfx_ctx_f64* ctx, const double* data, float ambient, void* stream);
 
 #ifdef __cplusplus
 }

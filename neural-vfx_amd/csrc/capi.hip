@@ -94,15 +94,30 @@ int vfx::validate_prm(const float* prm) {
 int vfx::stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s) {
     c->P.setup(solid, s);
     CHECK_LAUNCH("obstacle setup");
-    launch_advect(c->g, 6, data, data + c->g.n3, c->prm, c->inv_dx, c->hat, c->qidx(), c->adv, c->P.fm,
-                  c->mass_part, s);
-    launch_mass_fin(c->g, c->mass_part, c->st, s);
+    const size_t n3 = c->g.n3;
+    if (c->hr.S > 0) {
+        // with the finer scalar grid only the velocity is advected here: the coarse density,
+        // temperature and fuel after the step are the restriction of the fine ones, and nothing
+        // reads their advected values (solver.py:443-445)
+        launch_advect(c->g, 3, data + n3, data + n3, c->prm, c->inv_dx, c->hat + n3, c->qidx(), c->adv + n3,
+                      nullptr, nullptr, s);
+    } else {
+        launch_advect(c->g, 6, data, data + n3, c->prm, c->inv_dx, c->hat, c->qidx(), c->adv, c->P.fm,
+                      c->mass_part, s);
+        launch_mass_fin(c->g, c->mass_part, c->st, s);
+    }
     CHECK_LAUNCH("advection");
     return VFX_OK;
 }
 
 int vfx::stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
-    launch_sources_diffuse(c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, data, s);
+    if (c->hr.S > 0) {
+        // the coarse scalars the forces read = the fine scalars, box-restricted (solver.py:443-445)
+        launch_sources_vel(c->g, c->adv, c->P.fm, c->prm, data, s);
+        launch_restrict(c, data, s);
+    } else {
+        launch_sources_diffuse(c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, data, s);
+    }
     launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega, c->omega_s, c->force, s);
     CHECK_LAUNCH("forces");
     return VFX_OK;
@@ -479,6 +494,7 @@ int VFX_FN(vfx_ctx_destroy)(vfx_ctx* c) {
     if (c->ring) hipHostFree(c->ring);
     if (c->stop_pin) hipHostFree(c->stop_pin);
     if (c->pool) hipFree(c->pool);
+    hires_free(c);
     delete c;
     return VFX_OK;
 }
@@ -577,6 +593,7 @@ int VFX_FN(vfx_step_stage)(vfx_ctx* c, int stage, real* data, const real* solid,
     }
     switch (stage) {
         case VFX_STAGE_ADVECT: return stage_advect(c, data, solid, s);
+        case VFX_STAGE_SCALARS: return stage_scalars(c, data, s);
         case VFX_STAGE_FORCES: return stage_forces(c, data, s);
         case VFX_STAGE_PROJECT:
             c->P.setup(solid, s);
@@ -594,6 +611,7 @@ int VFX_FN(vfx_step)(vfx_ctx* c, real* data, const real* solid, const float* prm
     std::memcpy(c->prm_last, prm_host, sizeof(c->prm_last));
     if (int rc = upload(c, prm_host, c->prm, s)) return rc;
     if (int rc = stage_advect(c, data, solid, s)) return rc;
+    if (int rc = stage_scalars(c, data, s)) return rc;
     if (int rc = stage_forces(c, data, s)) return rc;
     return stage_project(c, data, s);
 }
@@ -750,10 +768,13 @@ int VFX_FN(vfx_time_kernel)(vfx_ctx* c, int which, int reps, void* stream, float
                 return true;
             case VFX_K_SOURCES: launch_sources_pass(0, c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, s); return true;
             case VFX_K_DIFFUSE: launch_sources_pass(1, c->g, c->hat, c->P.fm, c->prm, c->st, c->omega, s); return true;
-            default: return which >= VFX_K_MATVEC && c->P.launch_single(which - VFX_K_MATVEC, c->hat, s);
+            default:
+                if (which >= VFX_K_FADV_SL) return hires_launch_single(c, which, s);
+                return which >= VFX_K_MATVEC && c->P.launch_single(which - VFX_K_MATVEC, c->hat, s);
         }
     };
     // pass 2 of the advection reads pass 1's trace corners: pass 1 runs first on the same input
+    if (which == VFX_K_FADV_MC) hires_launch_single(c, VFX_K_FADV_SL, s);
     if (which == VFX_K_ADV_MC)
         launch_adv_pass(0, c->g, 6, c->adv, c->adv + c->g.n3, c->prm, c->inv_dx, c->hat, c->qidx(), nullptr, nullptr,
                         nullptr, s);

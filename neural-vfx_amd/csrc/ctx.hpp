@@ -26,6 +26,25 @@ constexpr int HOST_RING = 4;   // pinned parameter slots (solver.py:81, :331-359
 constexpr int FCG_BLOCKS = 1024;   // blocks of the flexible-CG reductions (4 partial arrays)
 constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag
 
+// The finer scalar grid (Solver3D(res, scalar_res=S), hires.py; solver.py:217-264): the caller's
+// fine fields and the library's fine scratch.
+struct HiresGrid {
+    int S = 0;                     // 0: off
+    real* hi = nullptr;            // (3, S, S, S) density, temperature, fuel (caller's)
+    const real* hsolid = nullptr;  // (S, S, S), 1 = obstacle (caller's)
+    void* pool = nullptr;
+    real *hat = nullptr, *adv = nullptr;   // 3 S^3 each
+    int* qidx = nullptr;           // S^3 backward-trace base corners
+    uint8_t* ffm = nullptr;        // S^3 fluid flags (hsolid <= 0.5), written by the first pass
+    double* part = nullptr;        // mass partials
+    Geom g;                        // fine launch geometry
+    real a = RL(0.0), b = RL(0.0); // fine index -> coarse index: p_c = a p_f + b
+    real dxf = RL(0.0);            // fine cell size
+    real ratio = RL(0.0);          // fine cells per coarse cell, S / res
+    int span = 0;                  // fine cells a coarse cell can overlap per axis
+    int nd = 1;                    // diffusion substeps on the fine grid
+};
+
 struct vfx_ctx {
     int N = 0;
     Geom g;
@@ -67,6 +86,7 @@ struct vfx_ctx {
     double last_fcg_rn = 0.0;   // ||r|| at which flexible CG converged
     std::vector<double> fcg_hist;   // ||r_k|| of the last flexible-CG run
 #endif
+    HiresGrid hr;
     // diagnostics of the last projection
     int last_iters = 0, last_restarts = 0;
     bool last_true_valid = false;
@@ -85,5 +105,13 @@ int stage_forces(vfx_ctx* c, real* data, hipStream_t s);
 int stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame = false);
 // the fused projection of the velocity vel (fused.hip); phi is the context's warm start
 int run_fused(vfx_ctx* c, real* vel, hipStream_t s, bool in_frame);
+// the finer scalar grid (hires.hip): the fine stage (advection, mass ratio, sources, diffusion) by
+// the velocity of `data`, and the restriction of the fine scalars into data's density,
+// temperature and fuel channels
+int stage_scalars(vfx_ctx* c, const real* data, hipStream_t s);
+void launch_restrict(vfx_ctx* c, real* out, hipStream_t s);
+void hires_free(vfx_ctx* c);
+// one launch of fine kernel VFX_K_F* for vfx_time_kernel; false without a fine grid or for another id
+bool hires_launch_single(vfx_ctx* c, int which, hipStream_t s);
 
 }  // namespace vfx

@@ -5,32 +5,14 @@
 // here one kernel for the three passes) in the launch order of solver.py:432-455.  "Solid" is read from the uint8 face mask
 // (mask == 0 <=> solid > 0.5, pressure.py:115-143); cells outside the box count as solid.
 #include "sim3d_kernels.hpp"
+#include "sources_ops.hpp"
 
 namespace vfx {
 
-__device__ __forceinline__ real clampf_(real g, real lo, real hi) {
-    return g < lo ? lo : (g > hi ? hi : g);
-}
-
-// Philox-4x32-10 (Salmon et al., SC'11), first output word; key (seed, emitter), counter
-// (cell, step, 'EMIT', 0).
-__device__ __forceinline__ uint32_t philox_u32(uint32_t seed, uint32_t em, uint32_t cell, uint32_t step) {
-    uint32_t c0 = cell, c1 = step, c2 = 0x454D4954u, c3 = 0u, k0 = seed, k1 = em;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        c0 = hi1 ^ c1 ^ k0;
-        c1 = lo1;
-        c2 = hi0 ^ c3 ^ k1;
-        c3 = lo0;
-    }
-    return c0;
-}
-
-// sources (kernels.py:182-250) with the ignition ramp width and per-cell emitter noise as
-// parameters (reference behaviour: ramp 100 K, noise off).
+// sources (kernels.py:182-250).  VEL_ONLY (with the finer scalar grid): only the velocity
+// channels 1..3 are written; density, temperature and fuel come from the fine grid's restriction,
+// which overwrites them before anything reads them (solver.py:443-445).
+template <bool VEL_ONLY>
 __global__ void __launch_bounds__(256) k_sources(int N, int zc, const real* __restrict__ adv,
                                                  const uint8_t* __restrict__ fm, const float* __restrict__ prm,
                                                  const DevState* __restrict__ st, real* __restrict__ out) {
@@ -38,104 +20,37 @@ __global__ void __launch_bounds__(256) k_sources(int N, int zc, const real* __re
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= N || y >= N) return;
-    const real amb = (real)prm[P_AMB], dt = (real)prm[P_DT], dx = (real)prm[P_DX];
-    const int n_em = (int)prm[P_N_EM], n_imp = (int)prm[P_N_IMP];
-    const real ramp = (real)prm[P_IGN_RAMP], nlo = (real)prm[P_NOISE_LO], nhi = (real)prm[P_NOISE_HI];
-    const bool noisy = !(nlo == RL(1.0) && nhi == RL(1.0));
-    const uint32_t seed = (uint32_t)prm[P_SEED], stepno = (uint32_t)prm[P_STEPNO];
-    const real ratio = (real)st->s[S_MASS_RATIO];
-    const real tmax = (real)prm[P_TMAX];
-    const real e_burn = exp(-(real)prm[P_BURN] * dt);
-    const real e_cool = exp(-(real)prm[P_COOL] * dt);
-    const real e_diss = exp(-(real)prm[P_DDISS] * dt);
+    const SrcConst k(prm);
+    const real dx = (real)prm[P_DX];
+    const real ratio = VEL_ONLY ? RL(1.0) : (real)st->s[S_MASS_RATIO];
     const int z1 = min((int)(blockIdx.z + 1) * zc, N);
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const size_t i = idx3(N, z, y, x);
         if (fm[i] == 0) {
-            out[i] = RL(0.0); out[n3 + i] = RL(0.0); out[2 * n3 + i] = RL(0.0); out[3 * n3 + i] = RL(0.0);
-            out[4 * n3 + i] = amb; out[5 * n3 + i] = RL(0.0);
+            out[n3 + i] = RL(0.0); out[2 * n3 + i] = RL(0.0); out[3 * n3 + i] = RL(0.0);
+            if (!VEL_ONLY) {
+                out[i] = RL(0.0); out[4 * n3 + i] = k.amb; out[5 * n3 + i] = RL(0.0);
+            }
             continue;
         }
         const real px = ((real)x + RL(0.5)) * dx, py = ((real)y + RL(0.5)) * dx, pz = ((real)z + RL(0.5)) * dx;
-        real d = adv[i] * ratio;
+        real d = RL(0.0), T = RL(0.0), Fu = RL(0.0);
+        if (!VEL_ONLY) {
+            d = adv[i] * ratio;
+            T = adv[4 * n3 + i];
+            Fu = adv[5 * n3 + i];
+        }
         real vx = adv[n3 + i], vy = adv[2 * n3 + i], vz = adv[3 * n3 + i];
-        real T = adv[4 * n3 + i], Fu = adv[5 * n3 + i];
-        for (int e = 0; e < n_em; ++e) {
-            const float* b = prm + P_EM0 + e * EM_STRIDE;
-            const real rad = b[3];
-            const real ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
-            const real r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
-            if (r2 < RL(1.0)) {
-                const real w = (RL(1.0) - r2) * (RL(1.0) - r2);
-                real drate = b[4], frate = b[6];
-                const real te = b[5];
-                real nu = RL(1.0);
-                if (noisy) {
-                    const uint32_t u = philox_u32(seed, (uint32_t)e, (uint32_t)i, stepno);
-                    nu = nlo + (nhi - nlo) * ((real)(u >> 8) * (RL(1.0) / RL(16777216.0)));
-                    drate = drate * nu;
-                    frate = frate * nu;
-                }
-                d += drate * w * dt;
-                Fu += frate * w * dt;
-                if (te > RL(0.0)) {
-                    const real tex = noisy ? (te - amb) * nu : (te - amb);
-                    const real tt = amb + tex * w;
-                    T = T > tt ? T : tt;
-                }
-                const real evx = b[7], evy = b[8], evz = b[9];
-                if (evx * evx + evy * evy + evz * evz > RL(0.0)) {
-                    vx = vx + (evx - vx) * w;
-                    vy = vy + (evy - vy) * w;
-                    vz = vz + (evz - vz) * w;
-                }
-            }
-        }
-        for (int m = 0; m < n_imp; ++m) {
-            const float* b = prm + P_IMP0 + m * IMP_STRIDE;
-            const real rad = b[3];
-            const real ddx = px - b[0], ddy = py - b[1], ddz = pz - b[2];
-            const real r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
-            if (r2 < RL(9.0)) {
-                const real gk = exp(-r2);
-                vx = vx + gk * b[4];
-                vy = vy + gk * b[5];
-                vz = vz + gk * b[6];
-            }
-        }
-        const real ign = clampf_((T - (real)prm[P_IGN]) / ramp, RL(0.0), RL(1.0));
-        const real burned = Fu * (RL(1.0) - e_burn) * ign;
-        Fu -= burned;
-        const real room = clampf_((tmax - T) / (tmax - amb), RL(0.0), RL(1.0));
-        T += prm[P_BHEAT] * burned * room;
-        d += prm[P_SMOKE] * burned;
-        T = amb + (T - amb) * e_cool;
-        d = d * e_diss;
-        out[i] = d > RL(0.0) ? d : RL(0.0);
+        cell_sources<!VEL_ONLY, true>(prm, k, px, py, pz, (uint32_t)i, d, T, Fu, vx, vy, vz);
         out[n3 + i] = vx;
         out[2 * n3 + i] = vy;
         out[3 * n3 + i] = vz;
-        out[4 * n3 + i] = T < tmax ? T : tmax;
-        out[5 * n3 + i] = Fu > RL(0.0) ? Fu : RL(0.0);
+        if (!VEL_ONLY) {
+            out[i] = d;
+            out[4 * n3 + i] = T;
+            out[5 * n3 + i] = Fu;
+        }
     }
-}
-
-// Neighbour loads go to in-bounds addresses (clamped to the cell itself outside the box) and are
-// issued unconditionally; the solid / range tests only select values.  A load guarded by a test
-// would wait for the test's own load and serialise the memory traffic of every cell.
-struct Nb6 {
-    int xm, xp, ym, yp, zm, zp;   // clamped neighbour indices
-    bool oxm, oxp, oym, oyp, ozm, ozp;   // neighbour inside the box
-};
-
-__device__ __forceinline__ Nb6 nb6(int N, int x, int y, int z, int i) {
-    Nb6 n;
-    const int sz = N * N;
-    n.oxm = x > 0; n.oxp = x + 1 < N; n.oym = y > 0; n.oyp = y + 1 < N; n.ozm = z > 0; n.ozp = z + 1 < N;
-    n.xm = n.oxm ? i - 1 : i; n.xp = n.oxp ? i + 1 : i;
-    n.ym = n.oym ? i - N : i; n.yp = n.oyp ? i + N : i;
-    n.zm = n.ozm ? i - sz : i; n.zp = n.ozp ? i + sz : i;
-    return n;
 }
 
 // diffuse_heat_fuel (kernels.py:268-286): channels 0..3 copied, T and F one explicit substep with
@@ -413,7 +328,7 @@ void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, c
     const int nd = g.N > 256 ? (g.N + 255) / 256 : 1;
     real* b0 = nd % 2 == 0 ? data : hat;
     real* b1 = nd % 2 == 0 ? hat : data;
-    hipLaunchKernelGGL(k_sources, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, st, b0);
+    hipLaunchKernelGGL(k_sources<false>, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, st, b0);
     for (int k = 0; k < nd; ++k) {
         real* a = k % 2 == 0 ? b0 : b1;
         real* b = k % 2 == 0 ? b1 : b0;
@@ -421,10 +336,15 @@ void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, c
     }
 }
 
+void launch_sources_vel(const Geom& g, const real* adv, const uint8_t* fm, const float* prm, real* data,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_sources<true>, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, nullptr, data);
+}
+
 void launch_sources_pass(int pass, const Geom& g, const real* src, const uint8_t* fm, const float* prm,
                          const DevState* st, real* dst, hipStream_t s) {
     if (pass == 0)
-        hipLaunchKernelGGL(k_sources, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, st, dst);
+        hipLaunchKernelGGL(k_sources<false>, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, st, dst);
     else
         hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, RL(1.0), dst);
 }

@@ -66,7 +66,9 @@ enum : int {
 // slots, kernels.py:44-53); the CG inner products, alpha and beta are FP64 here.
 enum : int {
     S_MASS_RATIO = 0, S_MAXU = 1, S_RZ = 2, S_ALPHA = 3, S_BETA = 4, S_MAXR = 5, S_THR = 6,
-    S_BEST = 7, S_BB = 8, S_RR = 9, S_TRUE_RR = 10, S_LSCALE = 11, S_COUNT = 16,
+    S_BEST = 7, S_BB = 8, S_RR = 9, S_TRUE_RR = 10, S_LSCALE = 11,
+    S_MASS_RATIO_F = 12,   // mass ratio of the finer scalar grid (hires.py, solver.py:250-253)
+    S_COUNT = 16,
 };
 
 struct DevState {
@@ -98,6 +100,24 @@ inline int nblocks(const Geom& g) { return (int)(g.grid.x * g.grid.y * g.grid.z)
 
 __device__ __forceinline__ size_t idx3(int N, int z, int y, int x) {
     return ((size_t)z * N + y) * N + x;
+}
+
+// Neighbour loads go to in-bounds addresses (clamped to the cell itself outside the box) and are
+// issued unconditionally; the solid / range tests only select values.  A load guarded by a test
+// would wait for the test's own load and serialise the memory traffic of every cell.
+struct Nb6 {
+    int xm, xp, ym, yp, zm, zp;   // clamped neighbour indices
+    bool oxm, oxp, oym, oyp, ozm, ozp;   // neighbour inside the box
+};
+
+__device__ __forceinline__ Nb6 nb6(int N, int x, int y, int z, int i) {
+    Nb6 n;
+    const int sz = N * N;
+    n.oxm = x > 0; n.oxp = x + 1 < N; n.oym = y > 0; n.oyp = y + 1 < N; n.ozm = z > 0; n.ozp = z + 1 < N;
+    n.xm = n.oxm ? i - 1 : i; n.xp = n.oxp ? i + 1 : i;
+    n.ym = n.oym ? i - N : i; n.yp = n.oyp ? i + N : i;
+    n.zm = n.ozm ? i - sz : i; n.zp = n.ozp ? i + sz : i;
+    return n;
 }
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b runs on XCD b % 8), and each

@@ -123,10 +123,16 @@ void launch_adv_pass(int pass, const Geom& g, int C, const real* src, const real
                      real inv_dx, real* hat, int* qidx, real* out, const uint8_t* fm, double* part, hipStream_t s);
 void launch_sources_pass(int pass, const Geom& g, const real* src, const uint8_t* fm, const float* prm,
                          const DevState* st, real* dst, hipStream_t s);
-void launch_mass_fin(const Geom& g, double* part, DevState* st, hipStream_t s);
-size_t mass_part_doubles(const Geom& g);   // partial-sum slots of the advection mass sums
+// ratio -> st->s[slot] (S_MASS_RATIO of the simulation grid, S_MASS_RATIO_F of the fine scalar grid)
+void launch_mass_fin(const Geom& g, double* part, DevState* st, hipStream_t s, int slot = S_MASS_RATIO);
+size_t mass_part_doubles(const Geom& g);
+int advect_parts(const Geom& g);           // blocks of the advection launch (one mass partial each)   // partial-sum slots of the advection mass sums
 void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, const float* prm,
                             const DevState* st, real* hat, real* data, hipStream_t s);
+// sources of the velocity channels only, adv -> data (with the finer scalar grid, whose restriction
+// supplies density, temperature and fuel; no diffusion, which leaves the velocity as it is)
+void launch_sources_vel(const Geom& g, const real* adv, const uint8_t* fm, const float* prm, real* data,
+                        hipStream_t s);
 void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx,
                         real* omega, real* omega_s, real* force, hipStream_t s);
 

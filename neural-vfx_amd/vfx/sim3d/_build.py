@@ -16,7 +16,8 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 BUILD_DIR = os.path.join(PKG_ROOT, "_build")
 LIB_DIR = os.path.join(PKG_ROOT, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libvfxsim3d.so")
-SOURCES = ("advect.hip", "forces.hip", "pressure.hip", "cg.hip", "capi.hip", "unet.hip", "batch.hip", "fused.hip")
+SOURCES = ("advect.hip", "forces.hip", "pressure.hip", "cg.hip", "capi.hip", "unet.hip", "batch.hip", "fused.hip",
+           "hires.hip")
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [

@@ -13,16 +13,17 @@ import torch  # noqa: F401  (loads torch's HIP runtime before the library binds 
 from ._build import LIB_PATH
 
 VFX_OK, VFX_E_ARG, VFX_E_VALUE, VFX_E_HIP = 0, 1, 2, 3
-STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT = 0, 1, 2
+STAGE_ADVECT, STAGE_FORCES, STAGE_PROJECT, STAGE_SCALARS = 0, 1, 2, 3
 OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL, OPT_MG_STORAGE = 1, 2, 3, 4, 5
 OPT_FCG_MG, OPT_LEARNED_MAXIT, OPT_LEARNED_SWEEPS = 6, 7, 8
 OPT_CG_BUDGET, OPT_CELLS_PER_THREAD = 9, 10
 OPT_PRESSURE_MODE, OPT_FUSED_CYCLES, OPT_FUSED_SMOOTH, OPT_FUSED_SMOOTHER = 11, 12, 13, 14
 TUNE_MAX_SIZES, TUNE_REPORT = 8, 46
+HIRES_MAX = 1024   # largest scalar_res (vfx_hires_attach)
 MG_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
 # kernel ids of vfx_time_kernel (include/vfxsim3d.h VFX_K_*); "" = unused id
 KERNELS = ("adv_sl", "adv_mc", "sources", "diffuse", "upd_matvec", "inv_post", "smooth0", "restrict0", "prolong0",
-           "pre_resid_fwd", "", "", "", "", "coarse")
+           "pre_resid_fwd", "", "", "", "", "coarse", "fadv_sl", "fadv_mc", "fsources", "fdiffuse", "frestrict")
 
 # every entry point of include/vfxsim3d.h with its argument types
 _P = ctypes.c_void_p
@@ -46,6 +47,8 @@ SIGNATURES = {
     "vfx_precond": [_P, _P, _P, _P, _P],
     "vfx_get_ops": [_P, _P, _I, _P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I), _P],
     "vfx_time_kernel": [_P, _I, _I, _P, ctypes.POINTER(ctypes.c_float)],
+    "vfx_hires_attach": [_P, _I, _P, _P],
+    "vfx_hires_init": [_P, _P, ctypes.c_float, _P],
     "vfx_last_error": [ctypes.c_char_p, _I],
 }
 # the FP64 run of the same solver: every entry point again with the suffix _f64 (double fields;

@@ -4,6 +4,8 @@ over the HIP library libvfxsim3d.so.
 Step order (the reference's, solver.py:3-8):
   1. advect density, velocity, temperature, fuel by u^n (MacCormack, RK2 traces, clamped)
   2. density mass correction (global fluid-density ratio, fixed-order FP64 reduction)
+     [scalar_res: density, temperature and fuel are advected, corrected, sourced and diffused on
+     the finer grid instead, and box-restricted into the state before step 4]
   3. emitters, impulses, combustion, cooling, dissipation; heat + fuel diffusion
   4. buoyancy, vorticity confinement on the smoothed vorticity, wind; the [1,2,1]^3-smoothed
      force is applied
@@ -22,7 +24,7 @@ import numpy as np
 import torch
 
 from vfx.core.fields import AMBIENT_TEMP, DOMAIN_SIZE
-from vfx.core.fields3d import NUM_CHANNELS_3D, FieldMeta3D, FieldState3D, Forces3D, rasterize_obstacles_3d
+from vfx.core.fields3d import NUM_CHANNELS_3D, FieldMeta3D, FieldState3D, Forces3D, Volume, rasterize_obstacles_3d
 
 from . import _lib
 from .config import SolverConfig3D, solver_version_tag
@@ -88,16 +90,20 @@ class Solver3D:
 
     def __init__(self, res: int, config: SolverConfig3D | None = None, device="cuda", scalar_res: int | None = None,
                  dtype: torch.dtype = torch.float32):
-        """`dtype` torch.float64 selects the FP64 run of the same solver (double state, levels and
+        """scalar_res (opt-in, S >= res): density, temperature and fuel are also carried on a finer
+        S^3 grid by the res^3 velocity and box-restricted back each step (hires.py).
+        `dtype` torch.float64 selects the FP64 run of the same solver (double state, levels and
         constants); the reference is FP32 only, so float32 is the reference-compatible default."""
-        if scalar_res is not None:
-            raise NotImplementedError("scalar_res (the reference's opt-in finer scalar grid) is not built")
         if dtype not in (torch.float32, torch.float64):
             raise ValueError(f"dtype must be torch.float32 or torch.float64, got {dtype}")
         self.dtype = dtype
         self.res = int(res)
         self.config = config if config is not None else SolverConfig3D()
-        self.scalar_res = None
+        self.scalar_res = None if scalar_res is None else int(scalar_res)
+        if self.scalar_res is not None and self.scalar_res < self.res:
+            raise ValueError("scalar_res must be >= res")
+        if self.scalar_res is not None and self.scalar_res > _lib.HIRES_MAX:
+            raise ValueError(f"scalar_res must be <= {_lib.HIRES_MAX}")
         self._structure()
         self.device = str(device)
         if self.device == "cuda":
@@ -112,6 +118,8 @@ class Solver3D:
         self._h = np.zeros(P_SIZE, np.float32)
         self._net = None
         self._apply_options()
+        if self.scalar_res is not None:
+            self._init_hires()
 
     def __del__(self):
         ctx = getattr(self, "_ctx", None)
@@ -164,6 +172,10 @@ class Solver3D:
         data[4] = AMBIENT_TEMP
         meta = FieldMeta3D.for_res(N)
         meta.solver_version = self.version
+        if self.scalar_res is not None:
+            self.set_hires_obstacles(obstacles)
+            self.hi.zero_()
+            self.hi[1] = AMBIENT_TEMP
         return FieldState3D(data=data, solid=solid, meta=meta)
 
     def step(self, state: FieldState3D, forces: Forces3D, dt: float) -> FieldState3D:
@@ -179,18 +191,22 @@ class Solver3D:
         return state
 
     def step_profiled(self, state: FieldState3D, forces: Forces3D, dt: float) -> dict:
-        """Same step as step(), as three stages with device events between them
-        (solver.py:157-184).  Returns ms per stage {advection, forces, projection, total}."""
+        """Same step as step(), as three stages with device events between them (four with
+        scalar_res: the fine-scalar stage is timed as `scalars`) (solver.py:157-184).  Returns ms
+        per stage {advection, [scalars,] forces, projection, total}."""
         self._check(state)
         self._structure()
         self._apply_options()
         h = pack_params(self.config, self.res, forces, dt, state.meta.step, self._h)
+        stages = [(_lib.STAGE_ADVECT, "advection"), (_lib.STAGE_FORCES, "forces"), (_lib.STAGE_PROJECT, "projection")]
+        if self.scalar_res is not None:
+            stages.insert(1, (_lib.STAGE_SCALARS, "scalars"))
         ts = torch.cuda.current_stream(torch.device(self.device))
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
         torch.cuda._sleep(3_000_000)
         ev[0].record(ts)
         hp = h.ctypes.data_as(ctypes.c_void_p)
-        for k, stage in enumerate((_lib.STAGE_ADVECT, _lib.STAGE_FORCES, _lib.STAGE_PROJECT)):
+        for k, (stage, _) in enumerate(stages):
             self._chk(self._api.vfx_step_stage(self._ctx, stage, _ptr(state.data), _ptr(state.solid),
                                                 hp if stage == _lib.STAGE_ADVECT else None, self._stream()),
                        "step_profiled")
@@ -199,16 +215,45 @@ class Solver3D:
         state.meta.step += 1
         state.meta.dt = float(dt)
         state.meta.solver_version = self.version
-        t = [ev[i].elapsed_time(ev[i + 1]) for i in range(3)]
-        out = dict(zip(("advection", "forces", "projection"), t))
+        t = [ev[i].elapsed_time(ev[i + 1]) for i in range(len(stages))]
+        out = dict(zip((name for _, name in stages), t))
         out["total"] = sum(t)
         return out
 
-    def hires_volume(self):
-        return None
+    # ------------------------------------------------------------------ finer scalar grid (opt-in)
+    def hires_volume(self) -> Volume | None:
+        """The fine density + temperature (views of the solver's buffer, no copy; valid until the
+        next step), or None without scalar_res (solver.py:187-192)."""
+        if self.scalar_res is None:
+            return None
+        return Volume(density=self.hi[0], temperature=self.hi[1])
 
-    def hires_fields(self):
-        return None
+    def hires_fields(self) -> torch.Tensor | None:
+        """(3, S, S, S) density, temperature, fuel (the live buffer), or None."""
+        return None if self.scalar_res is None else self.hi
+
+    def set_hires_obstacles(self, obstacles=()) -> None:
+        """Rasterise obstacles at scalar_res (reset() does this).  Call it whenever state.solid is
+        changed by hand, with the same obstacles: the fine grid reads hi_solid, not state.solid."""
+        if self.scalar_res is not None:
+            self.hi_solid.copy_(rasterize_obstacles_3d(list(obstacles), self.scalar_res, device=self.device))
+
+    def init_hires_from_state(self, state: FieldState3D) -> None:
+        """Fill the fine scalars from the coarse ones (trilinear, cell centred; 0 / ambient in fine
+        solids), on the device.  For states built by hand; reset() already starts both grids empty."""
+        if self.scalar_res is None:
+            return
+        self._check(state)
+        self._chk(self._api.vfx_hires_init(self._ctx, _ptr(state.data), float(AMBIENT_TEMP), self._stream()),
+                  "init_hires_from_state")
+
+    def _init_hires(self):
+        S = self.scalar_res
+        self.dxf = DOMAIN_SIZE / S
+        self.hi = torch.zeros(3, S, S, S, device=self.device, dtype=self.dtype)
+        self.hi[1] = AMBIENT_TEMP
+        self.hi_solid = torch.zeros(S, S, S, device=self.device, dtype=self.dtype)
+        self._chk(self._api.vfx_hires_attach(self._ctx, S, _ptr(self.hi), _ptr(self.hi_solid)), "scalar_res")
 
     def project(self, state: FieldState3D) -> FieldState3D:
         self._check(state)

@@ -21,6 +21,15 @@ Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
                   explicit prolongation matrices
   precond         symmetric M^-1, positive definite, error reduction per application
   project         exact FP64 sparse solve (scipy) of the same system; d(u) = 0 to tolerance
+  project_fused   Chebyshev weights closed form, one sweep by hand, convergence to the exact solve
+  hires_advect    S == N equals advect bit for bit; constant field; integer fine-cell shift by a
+                  uniform velocity; linear field under a uniform velocity (exact for trilinear)
+  hires_sources   S == N equals the scalar channels of sources; emitter footprint closed form at
+                  fine centres; solid reset
+  hires_diffuse   S == N equals diffuse's T / F; conservation; single-spike stencil
+  hires_restrict  integer ratio = numpy block mean; any ratio = an independent float64 overlap
+                  matrix; mass conservation; coarse solids
+  hires_stage     the composition of the above (advect, mass ratio, sources, diffusion)
 """
 from __future__ import annotations
 
@@ -284,20 +293,96 @@ def stage_advect(data, solid, prm, inv_dx, prec="f32"):
     return adv, float(ratio)
 
 
-def stage_forces(adv, solid, prm, ratio, inv_dx, prec="f32"):
+def stage_forces(adv, solid, prm, ratio, inv_dx, prec="f32", hi=None):
+    """hi: the fine scalars (3, S, S, S) after the fine stage, restricted into the result's
+    density, temperature and fuel after the diffusion (scalar_res)."""
     dt = _dt(prec)
     adv, solid = _a(adv, dt), _a(solid, dt)
     prm = _a(prm, np.float32)
     data = np.empty_like(adv)
     r = np.float32(ratio) if prec == "f32" else float(ratio)
-    _fn("or_stage_forces", prec)(ctypes.c_int(solid.shape[-1]), _p(adv), _p(solid), _p(prm), _real(prec)(r),
-                                 _real(prec)(inv_dx), _p(data))
+    if hi is None:
+        _fn("or_stage_forces", prec)(ctypes.c_int(solid.shape[-1]), _p(adv), _p(solid), _p(prm), _real(prec)(r),
+                                     _real(prec)(inv_dx), _p(data))
+    else:
+        hi = _a(hi, dt)
+        _fn("or_stage_forces_hires", prec)(ctypes.c_int(solid.shape[-1]), _p(adv), _p(solid), _p(prm),
+                                           _real(prec)(r), _real(prec)(inv_dx), ctypes.c_int(hi.shape[-1]), _p(hi),
+                                           _p(data))
     return data
 
 
-def step(data, solid, prm, phi, inv_dx, prec="f32"):
-    """One whole step (advect, forces, project); returns (new data, iters)."""
+# ---------------------------------------------------------------- finer scalar grid (hires.py)
+def hires_map(N, S):
+    """f2c constants a, b (fine index -> coarse index, p_c = a p_f + b) as float32."""
+    return np.float32(N / S), np.float32(0.5 * N / S - 0.5)
+
+
+def hires_advect(src, vel, s, prec="f32"):
+    """MacCormack of fine (3, S^3) scalars by the coarse velocity (3, N^3), s = dt / dx_coarse."""
+    dt = _dt(prec)
+    src, vel = _a(src, dt), _a(vel, dt)
+    S, N = src.shape[-1], vel.shape[-1]
+    hat = np.empty_like(src)
+    out = np.empty_like(src)
+    _fn("or_hires_advect", prec)(ctypes.c_int(S), ctypes.c_int(N), _p(src), _p(vel), _real(prec)(s), _p(hat), _p(out))
+    return out
+
+
+def hires_sources(adv, hsolid, prm, ratio, prec="f32"):
+    dt = _dt(prec)
+    adv, hsolid = _a(adv, dt), _a(hsolid, dt)
+    prm = _a(prm, np.float32)
+    S = hsolid.shape[-1]
+    out = np.empty_like(adv)
+    r = np.float32(ratio) if prec == "f32" else float(ratio)
+    dxf = np.float32(1.0 / S) if prec == "f32" else 1.0 / S
+    _fn("or_hires_sources", prec)(ctypes.c_int(S), _p(adv), _p(hsolid), _p(prm), _real(prec)(r), _real(prec)(dxf),
+                                  _p(out))
+    return out
+
+
+def hires_diffuse(src, hsolid, prm, nsub=1.0, prec="f32"):
+    dt = _dt(prec)
+    src, hsolid = _a(src, dt), _a(hsolid, dt)
+    prm = _a(prm, np.float32)
+    S = hsolid.shape[-1]
+    out = np.empty_like(src)
+    dxf = np.float32(1.0 / S) if prec == "f32" else 1.0 / S
+    _fn("or_hires_diffuse", prec)(ctypes.c_int(S), _p(src), _p(hsolid), _p(prm), _real(prec)(nsub), _real(prec)(dxf),
+                                  _p(out))
+    return out
+
+
+def hires_restrict(fine, csolid, prm, data, prec="f32"):
+    """data (6, N^3) copy with channels 0, 4, 5 replaced by the restriction of fine (3, S^3)."""
+    dt = _dt(prec)
+    fine, csolid = _a(fine, dt), _a(csolid, dt)
+    prm = _a(prm, np.float32)
+    out = _a(data, dt).copy()
+    _fn("or_hires_restrict", prec)(ctypes.c_int(csolid.shape[-1]), ctypes.c_int(fine.shape[-1]), _p(fine), _p(csolid),
+                                   _p(prm), _p(out))
+    return out
+
+
+def hires_stage(hi, hsolid, vel, prm, inv_dx, prec="f32"):
+    """The fine stage in place on hi (3, S^3) by the coarse velocity vel (3, N^3); returns the
+    fine mass ratio."""
+    assert hi.dtype == _dt(prec) and hi.flags.c_contiguous
+    dt = _dt(prec)
+    hsolid, vel = _a(hsolid, dt), _a(vel, dt)
+    prm = _a(prm, np.float32)
+    return float(_fn("or_hires_stage", prec, ctypes.c_double)(ctypes.c_int(vel.shape[-1]), ctypes.c_int(hi.shape[-1]),
+                                                             _p(hi), _p(hsolid), _p(vel), _p(prm),
+                                                             _real(prec)(inv_dx)))
+
+
+def step(data, solid, prm, phi, inv_dx, prec="f32", hi=None, hsolid=None):
+    """One whole step (advect, [fine scalars,] forces, project); returns (new data, iters).
+    With hi / hsolid (scalar_res), hi is updated in place."""
     adv, ratio = stage_advect(data, solid, prm, inv_dx, prec)
-    out = stage_forces(adv, solid, prm, ratio, inv_dx, prec)
+    if hi is not None:
+        hires_stage(hi, hsolid, data[1:4], prm, inv_dx, prec)
+    out = stage_forces(adv, solid, prm, ratio, inv_dx, prec, hi=hi)
     it, _ = project(out, solid, prm, phi, prec)
     return out, it

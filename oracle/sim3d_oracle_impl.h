@@ -1126,6 +1126,224 @@ int F(or_project_fused)(int N, R* data, const R* solid, const float* prm, R* phi
     return it;
 }
 
+/* ---------------------------------------------------------------- finer scalar grid
+ * vfx/sim3d/hires.py.  Fine fields are (3, S, S, S): density, temperature, fuel.  In these
+ * functions N is the size of the grid being indexed (IDX), Nc the coarse (velocity) grid's. */
+
+/* f2c constants, from the reference's Python doubles (solver.py:234-239): p_c = a p_f + b */
+static inline void F(hires_map)(int Nc, int S, R* a, R* b) {
+    *a = (R)((double)Nc / (double)S);
+    *b = (R)(0.5 * (double)Nc / (double)S - 0.5);
+}
+
+/* advect_sl_fine + advect_mc_fine (hires.py:29-89): MacCormack of the 3 fine channels, RK2
+ * traces through the coarse velocity vel (3, Nc^3) in coarse index space, s = dt * inv_dx_c. */
+void F(or_hires_advect)(int N, int Nc, const R* src, const R* vel, R s, R* hat, R* out) {
+    size_t n3 = (size_t)N * N * N;
+    R a, b;
+    F(hires_map)(Nc, N, &a, &b);
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                R qc[3];
+                F(trace)(vel, Nc, a * (R)x + b, a * (R)y + b, a * (R)z + b, -s, qc);
+                F(corner_t) c = F(corner)(N, (qc[0] - b) / a, (qc[1] - b) / a, (qc[2] - b) / a);
+                for (int ch = 0; ch < 3; ++ch)
+                    hat[ch * n3 + IDX(z, y, x)] = F(tri)(src + ch * n3, N, c.z0, c.y0, c.x0, c.fz, c.fy, c.fx);
+            }
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                R qc[3], rc[3];
+                R px = a * (R)x + b, py = a * (R)y + b, pz = a * (R)z + b;
+                F(trace)(vel, Nc, px, py, pz, -s, qc);
+                F(trace)(vel, Nc, px, py, pz, s, rc);
+                R q[3] = {(qc[0] - b) / a, (qc[1] - b) / a, (qc[2] - b) / a};
+                R r[3] = {(rc[0] - b) / a, (rc[1] - b) / a, (rc[2] - b) / a};
+                int qx = F(base_idx)(q[0], N), qy = F(base_idx)(q[1], N), qz = F(base_idx)(q[2], N);
+                F(corner_t) cr = F(corner)(N, r[0], r[1], r[2]);
+                for (int ch = 0; ch < 3; ++ch) {
+                    const R* sc = src + ch * n3;
+                    R h = hat[ch * n3 + IDX(z, y, x)];
+                    R bar = F(tri)(hat + ch * n3, N, cr.z0, cr.y0, cr.x0, cr.fz, cr.fy, cr.fx);
+                    R val = h + (R)0.5 * (sc[IDX(z, y, x)] - bar);
+                    R lo = sc[IDX(qz, qy, qx)], hi = lo;
+                    for (int k = 1; k < 8; ++k) {
+                        R v = sc[IDX(qz + (k >> 2), qy + ((k >> 1) & 1), qx + (k & 1))];
+                        lo = v < lo ? v : lo;
+                        hi = v > hi ? v : hi;
+                    }
+                    if (val < lo || val > hi) val = h;
+                    out[ch * n3 + IDX(z, y, x)] = val;
+                }
+            }
+}
+
+/* sources_fine (hires.py:92-139): the scalar terms of sources at fine-cell centres (fine cell
+ * size dxf), fine mass ratio on density, 0 / ambient / 0 in fine solids.  Same extensions as
+ * or_sources (ignition ramp, emitter noise with the fine cell index as counter). */
+void F(or_hires_sources)(int N, const R* adv, const R* hsolid, const float* prm, R mass_ratio, R dxf, R* out) {
+    size_t n3 = (size_t)N * N * N;
+    const R amb = (R)prm[P_AMB], dt = (R)prm[P_DT];
+    const int n_em = (int)prm[P_N_EM];
+    const R ramp = (R)prm[P_IGN_RAMP];
+    const R nlo = (R)prm[P_NOISE_LO], nhi = (R)prm[P_NOISE_HI];
+    const int noisy = !(prm[P_NOISE_LO] == 1.0f && prm[P_NOISE_HI] == 1.0f);
+    const uint32_t seed = (uint32_t)prm[P_SEED], stepno = (uint32_t)prm[P_STEPNO];
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                size_t i = IDX(z, y, x);
+                if (hsolid[i] > (R)0.5) {
+                    out[i] = 0; out[n3 + i] = amb; out[2 * n3 + i] = 0;
+                    continue;
+                }
+                R px = ((R)x + (R)0.5) * dxf, py = ((R)y + (R)0.5) * dxf, pz = ((R)z + (R)0.5) * dxf;
+                R d = adv[i] * mass_ratio, T = adv[n3 + i], Fu = adv[2 * n3 + i];
+                for (int e = 0; e < n_em; ++e) {
+                    const float* bb = prm + P_EM0 + e * EM_STRIDE;
+                    R rad = (R)bb[3];
+                    R ddx = px - (R)bb[0], ddy = py - (R)bb[1], ddz = pz - (R)bb[2];
+                    R r2 = (ddx * ddx + ddy * ddy + ddz * ddz) / (rad * rad);
+                    if (r2 < (R)1) {
+                        R w = ((R)1 - r2) * ((R)1 - r2);
+                        R nu = (R)1;
+                        if (noisy) {
+                            uint32_t u = or_philox_u32(seed, (uint32_t)e, (uint32_t)i, stepno);
+                            nu = nlo + (nhi - nlo) * ((R)(u >> 8) * (R)(1.0 / 16777216.0));
+                        }
+                        R drate = (R)bb[4], frate = (R)bb[6], te = (R)bb[5];
+                        if (noisy) { drate = drate * nu; frate = frate * nu; }
+                        d += drate * w * dt;
+                        Fu += frate * w * dt;
+                        if (te > (R)0) {
+                            R tex = noisy ? (te - amb) * nu : (te - amb);
+                            R tt = amb + tex * w;
+                            T = T > tt ? T : tt;
+                        }
+                    }
+                }
+                R ign = F(clampr)((T - (R)prm[P_IGN]) / ramp, (R)0, (R)1);
+                R burned = Fu * ((R)1 - EXPR(-(R)prm[P_BURN] * dt)) * ign;
+                Fu -= burned;
+                R tmax = (R)prm[P_TMAX];
+                R room = F(clampr)((tmax - T) / (tmax - amb), (R)0, (R)1);
+                T += (R)prm[P_BHEAT] * burned * room;
+                d += (R)prm[P_SMOKE] * burned;
+                T = amb + (T - amb) * EXPR(-(R)prm[P_COOL] * dt);
+                d = d * EXPR(-(R)prm[P_DDISS] * dt);
+                out[i] = d > (R)0 ? d : (R)0;
+                out[n3 + i] = T < tmax ? T : tmax;
+                out[2 * n3 + i] = Fu > (R)0 ? Fu : (R)0;
+            }
+}
+
+/* diffuse_fine (hires.py:142-170): T and F one explicit substep on the fine grid, a missing or
+ * solid neighbour contributing the cell's own value (nbv); density copied. */
+void F(or_hires_diffuse)(int N, const R* src, const R* hsolid, const float* prm, R nsub, R dxf, R* dst) {
+    size_t n3 = (size_t)N * N * N;
+    R lam = (R)prm[P_KAPPA] * (R)prm[P_DT] / (nsub * dxf * dxf);
+    lam = lam < (R)LAM_CAP ? lam : (R)LAM_CAP;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                size_t i = IDX(z, y, x);
+                dst[i] = src[i];
+                for (int c = 1; c <= 2; ++c) {
+                    const R* f = src + c * n3;
+                    R v = f[i];
+                    if (hsolid[i] > (R)0.5 || lam <= (R)0) {
+                        dst[c * n3 + i] = v;
+                    } else {
+#define NB(zz, yy, xx) (F(solid_at)(hsolid, N, zz, yy, xx) ? v : f[IDX(zz, yy, xx)])
+                        R sum = NB(z, y, x - 1) + NB(z, y, x + 1) + NB(z, y - 1, x) + NB(z, y + 1, x)
+                                + NB(z - 1, y, x) + NB(z + 1, y, x);
+#undef NB
+                        dst[c * n3 + i] = v + lam * (sum - (R)6 * v);
+                    }
+                }
+            }
+}
+
+/* ovl (hires.py:173-176) */
+static inline R F(ovl)(int i, int j, R ratio) {
+    R hi_a = (R)(j + 1), hi_b = (R)(i + 1) * ratio, lo_a = (R)j, lo_b = (R)i * ratio;
+    R len = (hi_a < hi_b ? hi_a : hi_b) - (lo_a > lo_b ? lo_a : lo_b);
+    return len > (R)0 ? len : (R)0;
+}
+
+/* restrict_scalars (hires.py:179-219): coarse density / temperature / fuel (channels 0, 4, 5 of
+ * the (6, N^3) `data`) = overlap-weighted mean of the fine cells, ratio = S / N, span =
+ * ceil(S / N) + 1; 0 / ambient / 0 in coarse solids. */
+void F(or_hires_restrict)(int N, int S, const R* fine, const R* csolid, const float* prm, R* data) {
+    size_t n3 = (size_t)N * N * N, fn3 = (size_t)S * S * S;
+    const R ratio = (R)((double)S / (double)N);
+    const int span = (S + N - 1) / N + 1;
+    const R amb = (R)prm[P_AMB];
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                size_t i = IDX(z, y, x);
+                if (csolid[i] > (R)0.5) {
+                    data[i] = 0; data[4 * n3 + i] = amb; data[5 * n3 + i] = 0;
+                    continue;
+                }
+                int z0 = (int)FLOORR((R)z * ratio), y0 = (int)FLOORR((R)y * ratio), x0 = (int)FLOORR((R)x * ratio);
+                R sw = 0, sd = 0, st = 0, sf = 0;
+                for (int k = 0; k < span; ++k) {
+                    int zz = z0 + k;
+                    R wz = zz < S ? F(ovl)(z, zz, ratio) : (R)0;
+                    for (int j = 0; j < span; ++j) {
+                        int yy = y0 + j;
+                        R wy = yy < S ? F(ovl)(y, yy, ratio) : (R)0;
+                        for (int ii = 0; ii < span; ++ii) {
+                            int xx = x0 + ii;
+                            R wx = xx < S ? F(ovl)(x, xx, ratio) : (R)0;
+                            R w = wz * wy * wx;
+                            if (w > (R)0) {
+                                size_t o = ((size_t)zz * S + yy) * S + xx;
+                                sw += w;
+                                sd += w * fine[o];
+                                st += w * fine[fn3 + o];
+                                sf += w * fine[2 * fn3 + o];
+                            }
+                        }
+                    }
+                }
+                if (sw > (R)0) {
+                    data[i] = sd / sw; data[4 * n3 + i] = st / sw; data[5 * n3 + i] = sf / sw;
+                } else {
+                    data[i] = 0; data[4 * n3 + i] = amb; data[5 * n3 + i] = 0;
+                }
+            }
+}
+
+/* _record_hires (solver.py:241-259): the whole fine stage in place on hi (3, S^3), by the coarse
+ * velocity vel (3, Nc^3) at the start of the step.  Returns the fine mass ratio. */
+double F(or_hires_stage)(int Nc, int S, R* hi, const R* hsolid, const R* vel, const float* prm, R inv_dx) {
+    size_t f3 = (size_t)S * S * S;
+    R* hat = (R*)malloc(3 * f3 * sizeof(R));
+    R* adv = (R*)malloc(3 * f3 * sizeof(R));
+    R s = (R)prm[P_DT] * inv_dx;
+    R dxf = (R)(1.0 / (double)S);
+    int nd = S > 256 ? (S + 255) / 256 : 1;
+    F(or_hires_advect)(S, Nc, hi, vel, s, hat, adv);
+    double ratio = F(or_mass_ratio)(S, hi, adv, hsolid);
+    R* bufs[2];
+    bufs[0] = nd % 2 == 0 ? hi : hat;
+    bufs[1] = nd % 2 == 0 ? hat : hi;
+    F(or_hires_sources)(S, adv, hsolid, prm, (R)ratio, dxf, bufs[0]);
+    for (int k = 0; k < nd; ++k) F(or_hires_diffuse)(S, bufs[k % 2], hsolid, prm, (R)nd, dxf, bufs[(k + 1) % 2]);
+    free(hat);
+    free(adv);
+    return ratio;
+}
+
 /* ---------------------------------------------------------------- stages of Solver3D.step
  * _record_advect solver.py:422-430: MacCormack of all 6 channels by the state's own velocity,
  * then the mass ratio.  data -> adv (6ch); returns the ratio. */
@@ -1139,8 +1357,10 @@ double F(or_stage_advect)(int N, const R* data, const R* solid, const float* prm
 }
 
 /* _record_forces solver.py:432-455: sources, n_diff diffusion substeps, curl, smoothed
- * vorticity, body forces, smoothed force applied.  adv (6ch) -> data (6ch). */
-void F(or_stage_forces)(int N, const R* adv, const R* solid, const float* prm, R mass_ratio, R inv_dx, R* data) {
+ * vorticity, body forces, smoothed force applied.  adv (6ch) -> data (6ch).  With a fine grid
+ * (hi = (3, S^3) non-NULL) the fine scalars are restricted into data after the diffusion. */
+void F(or_stage_forces_hires)(int N, const R* adv, const R* solid, const float* prm, R mass_ratio, R inv_dx, int S,
+                              const R* hi, R* data) {
     size_t n3 = (size_t)N * N * N;
     int nd = N > 256 ? (N + 255) / 256 : 1;
     R* hat = (R*)malloc(6 * n3 * sizeof(R));
@@ -1149,6 +1369,8 @@ void F(or_stage_forces)(int N, const R* adv, const R* solid, const float* prm, R
     bufs[1] = nd % 2 == 0 ? hat : data;
     F(or_sources)(N, adv, solid, prm, mass_ratio, bufs[0]);
     for (int k = 0; k < nd; ++k) F(or_diffuse)(N, bufs[k % 2], solid, prm, (R)nd, bufs[(k + 1) % 2]);
+    /* with scalar_res, the coarse scalars the forces read = the fine ones restricted (solver.py:443-445) */
+    if (hi) F(or_hires_restrict)(N, S, hi, solid, prm, data);
     R* omega = (R*)malloc(3 * n3 * sizeof(R));
     R* omega_s = (R*)malloc(3 * n3 * sizeof(R));
     R* force = (R*)malloc(3 * n3 * sizeof(R));
@@ -1160,6 +1382,10 @@ void F(or_stage_forces)(int N, const R* adv, const R* solid, const float* prm, R
     F(or_body_forces)(N, data, solid, omega_s, prm, force);
     F(or_smooth3)(N, force, solid, ta, tb, 1, (R)prm[P_DT], data);
     free(hat); free(omega); free(omega_s); free(force); free(ta); free(tb);
+}
+
+void F(or_stage_forces)(int N, const R* adv, const R* solid, const float* prm, R mass_ratio, R inv_dx, R* data) {
+    F(or_stage_forces_hires)(N, adv, solid, prm, mass_ratio, inv_dx, 0, NULL, data);
 }
 
 #undef F

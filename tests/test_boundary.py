@@ -97,9 +97,13 @@ def test_rasterize_matches_reference_rule():
         rasterize_obstacles_3d([Obstacle3D("cone", (0, 0, 0), (1, 1, 1))], 8, device="cpu")
 
 
-def test_unbuilt_modes_fail_loudly():
-    with pytest.raises(NotImplementedError):
-        sim.Solver3D(32, scalar_res=64)
+def test_bad_modes_fail_loudly():
+    # scalar_res below res is the reference's ValueError (solver.py:219-220), raised before any
+    # device work
+    with pytest.raises(ValueError):
+        sim.Solver3D(32, scalar_res=16)
+    with pytest.raises(ValueError):
+        sim.Solver3D(32, scalar_res=2048)
     with pytest.raises(ValueError):
         sim.Solver3D(32, sim.SolverConfig3D(pressure_mode="bogus"))
     # the fused mode is built; its smoother name is validated before any device work
