@@ -44,11 +44,25 @@ KERNEL_BYTES_PER_CELL = {
     "prolong0": 2 + 4 + 0.5 + 0.125 + 2,   # black: r, x (all), level-2 x + flags -> x (half)
 }
 
+# Algorithmic bytes per launch of the finer scalar grid's kernels (S^3 fine cells, N^3 coarse
+# cells, FP32, 3 fine channels): every distinct element read once and written once, the coarse
+# velocity (12 bytes per coarse cell) read by both advection passes, fine solid (4 bytes) read by
+# the first pass which writes the 1-byte fluid flag the later passes read.
+def fine_kernel_bytes(name: str, N: int, S: int) -> float:
+    f, c = float(S) ** 3, float(N) ** 3
+    return {"fadv_sl": f * (12 + 4 + 12 + 4 + 1) + 12 * c,      # hi, solid -> hat, trace corner, flag
+            "fadv_mc": f * (12 + 12 + 4 + 1 + 12) + 12 * c,     # hi, hat, corner, flag -> adv
+            "fsources": f * (12 + 1 + 12),                      # adv, flag -> hat / hi
+            "fdiffuse": f * (12 + 1 + 12),                      # one substep
+            "frestrict": f * 12 + c * (1 + 12)}.get(name, 0.0)  # hi, coarse mask -> 3 coarse channels
+
+
 # launches per step of each timed kernel for n CG iterations (the per-step share picks the
 # dominant kernel)
 def launches_per_step(name: str, n: int) -> float:
     return {"adv_sl": 1, "adv_mc": 1, "sources": 1, "diffuse": 1, "upd_matvec": max(n - 1, 0), "inv_post": n,
-            "pre_resid_fwd": n, "smooth0": 2 * n, "restrict0": n, "prolong0": n, "coarse": n}.get(name, 0)
+            "pre_resid_fwd": n, "smooth0": 2 * n, "restrict0": n, "prolong0": n, "coarse": n,
+            "fadv_sl": 1, "fadv_mc": 1, "fsources": 1, "fdiffuse": 1, "frestrict": 1}.get(name, 0)
 
 
 def step_bytes_per_cell(n: int, s: int = 4) -> float:
@@ -62,11 +76,12 @@ def step_bytes_per_cell(n: int, s: int = 4) -> float:
     return 193 + b_prec + n * (41 + b_prec)
 
 
-def pmc_traffic(kernel: str, res: int):
+def pmc_traffic(kernel: str, res: int, scalar_res=None):
     """HBM bytes per launch of `kernel` from the committed PMC summary of the same grid size
-    (profiles/pmc_traffic_<res>_fire.md: FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size),
-    or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{res}_fire.md")
+    (profiles/pmc_traffic_<res>_fire[_scalars<S>].md: FETCH_SIZE + WRITE_SIZE, scaled by a copy of
+    known size), or None when there is none."""
+    tag = "" if scalar_res is None else f"_scalars{scalar_res}"
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{res}_fire{tag}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -383,6 +398,9 @@ def main():
     ap.add_argument("--pressure", default="pcg", choices=("pcg", "fused"),
                     help="projection: MGPCG (the bench line) or the fused stationary multigrid of "
                          "fused_config() (2 V-cycles + 4 Chebyshev sweeps, adaptive to normalised residual 5e-5)")
+    ap.add_argument("--scalar-res", type=int, default=0, metavar="S",
+                    help="with the finer scalar grid (Solver3D(res, scalar_res=S)): density, temperature and "
+                         "fuel carried on S^3 and restricted each step; the fine kernels join the table")
     ap.add_argument("--mg", default="fp32", choices=("fp32", "fp16", "bf16"),
                     help="multigrid level storage (the reference stores FP32)")
     ap.add_argument("--precision-report", type=int, default=0, metavar="STEPS",
@@ -397,7 +415,7 @@ def main():
                          "learned flexible CG vs MGPCG at 64^3, 128^3 and 256^3")
     ap.add_argument("--train-res", type=int, default=64)
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="short run for counter collection: one 256 MiB device copy (the scale reference), "
+                    help="short run for counter collection: one 256 MiB device copy first (the scale reference), "
                          "2 steps, then one launch of every timed kernel; prints no bench line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -428,15 +446,18 @@ def main():
     if fused:
         cfg = dataclasses.replace(cfg, pressure_mode="fused", pressure_tol=5e-5, pressure_max_iters=24)
     forces = fire_forces()
-    s = Solver3D(N, cfg, device="cuda:0")
-    s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
-    st = s.reset()
+    S = args.scalar_res or None
     if args.pmc_probe:
+        # the scale reference is the run's first device copy (the solver's set-up copies come later)
         src = torch.ones(COPY_PROBE_BYTES // 4, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    s = Solver3D(N, cfg, device="cuda:0", scalar_res=S)
+    s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
+    st = s.reset()
+    if args.pmc_probe:
         for i in range(2):
             s.step(st, forces, dt)
         torch.cuda.synchronize()
@@ -478,40 +499,50 @@ def main():
         if s._lib.vfx_time_kernel(s._ctx, _lib.KERNELS.index(name), 20, s._stream(), ctypes.byref(t)) != 0:
             continue
         row = {"us": round(t.value * 1000.0, 1), "per_step_us": round(t.value * 1000.0 * launches_per_step(name, n_it), 1)}
-        if name in KERNEL_BYTES_PER_CELL:
-            gbs = KERNEL_BYTES_PER_CELL[name] * N ** 3 / (t.value * 1e-3) / 1e9
+        nbytes = KERNEL_BYTES_PER_CELL[name] * N ** 3 if name in KERNEL_BYTES_PER_CELL else \
+            fine_kernel_bytes(name, N, S or N)
+        if S is None and name.startswith("f"):
+            continue
+        if S is not None and name in ("adv_sl", "adv_mc", "sources", "diffuse"):
+            continue   # with the fine grid the step runs these on the velocity only (3 channels)
+        if nbytes:
+            gbs = nbytes / (t.value * 1e-3) / 1e9
             row.update(gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 3))
         kern[name] = row
-    kname = args.roofline_kernel or max((k for k in kern if k in KERNEL_BYTES_PER_CELL),
-                                        key=lambda k: kern[k]["per_step_us"])
-    kbytes = KERNEL_BYTES_PER_CELL[kname] * N ** 3
+    byte_of = lambda k: KERNEL_BYTES_PER_CELL[k] * N ** 3 if k in KERNEL_BYTES_PER_CELL else fine_kernel_bytes(k, N, S)
+    kname = args.roofline_kernel or max((k for k in kern if "gbs" in kern[k]), key=lambda k: kern[k]["per_step_us"])
+    kbytes = byte_of(kname)
     kms = kern[kname]["us"] / 1000.0
     achieved = kbytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(kname, N), "kernel": kname,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(kname, N, S), "kernel": kname,
                 "kernel_ms": round(kms, 4), "bytes_per_launch": int(kbytes)}
     sb = step_bytes_per_cell(n_it) * N ** 3
     step_roof = {"bytes_per_step": int(sb), "achieved": round(sb / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
                  "frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and S is None:
         print("# CPU baseline (oracle, one step)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(st.data, st.solid, cfg, forces, dt, N, st.meta.step)
 
-    if fused:
-        step_roof = None   # the byte formula is the MGPCG step's
+    if fused or S is not None:
+        step_roof = None   # the byte formula is the plain MGPCG step's
     solve = ("fused projection (2 V-cycles + 4 Chebyshev sweeps, adaptive to normalised residual 5e-5)" if fused
              else "plain MGPCG to rel-res 1e-5")
     out = {
-        "metric": f"{N}^3 fire step ms, {solve}" + ("" if args.mg == "fp32" else f" ({args.mg} levels)"),
+        "metric": f"{N}^3 fire step ms, {solve}" + ("" if args.mg == "fp32" else f" ({args.mg} levels)")
+                  + ("" if S is None else f", scalars on {S}^3"),
         "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 3), "steps_per_s": round(1000.0 / ms, 2), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{N}^3 fire (fuel disc, combustion, buoyancy, confinement), "
                                + ("fused projection" if fused else "MGPCG rel-res 1e-5 (true residual judged)")
-                               + f", dt 1/60, seed {args.seed}", "res": N},
+                               + f", dt 1/60, seed {args.seed}"
+                               + ("" if S is None else f", density / temperature / fuel on a {S}^3 grid"),
+                   "res": N, "scalar_res": S},
         "pressure_ms": med("projection"), "advection_ms": med("advection"), "forces_ms": med("forces"),
+        "scalars_ms": med("scalars") if S is not None else None,
         "profiled_total_ms": med("total"), "pcg_iters": iters, "rel_res": [float(f"{r:.3g}") for r in rel],
         "true_rel_res": [float(f"{r:.3g}") for r in true_rel],
         "roofline": roofline, "step_roofline": step_roof, "cpu_baseline": cpu, "kernels": kern,

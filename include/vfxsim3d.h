@@ -10,8 +10,8 @@
  *     (vfx/sim3d/kernels.py:11-42, filled like Solver3D._upload, solver.py:331-359);
  *   - `stream` is a hipStream_t (torch.cuda.current_stream().cuda_stream), 0 = null stream.
  * Every function returns 0 on success or a VFX_E_* code; vfx_last_error() gives the message.
- * The library allocates device memory only in vfx_ctx_create; calls on one context must not
- * overlap (one context per Solver3D, as the reference keeps one solver per grid).
+ * The library allocates device memory only in vfx_ctx_create and vfx_hires_attach; calls on one
+ * context must not overlap (one context per Solver3D, as the reference keeps one solver per grid).
  */
 #ifndef VFXSIM3D_H
 #define VFXSIM3D_H

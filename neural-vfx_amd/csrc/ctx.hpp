@@ -39,10 +39,12 @@ struct HiresGrid {
     double* part = nullptr;        // mass partials
     Geom g;                        // fine launch geometry
     real a = RL(0.0), b = RL(0.0); // fine index -> coarse index: p_c = a p_f + b
+    real ia = RL(0.0);             // S / res (exactly 1 / a when pow2)
     real dxf = RL(0.0);            // fine cell size
     real ratio = RL(0.0);          // fine cells per coarse cell, S / res
     int span = 0;                  // fine cells a coarse cell can overlap per axis
     int nd = 1;                    // diffusion substeps on the fine grid
+    bool pow2 = false;             // S / res is a power of two (a = 2^-k)
 };
 
 struct vfx_ctx {

@@ -23,13 +23,23 @@ namespace vfx {
 
 constexpr int HIRES_MAX = 1024;   // S^3 indices stay 32-bit
 
+// coarse index -> fine index, (p_c - b) / a (hires.py:36, :61-62).  POW2: a = res / S is a power of
+// two, so the quotient is exactly (p_c - b) * ia with ia = 1 / a = S / res, and the multiply
+// replaces the division.
+template <bool POW2>
+__device__ __forceinline__ real to_fine(real pc, real a, real ia, real b) {
+    if constexpr (POW2) return (pc - b) * ia;
+    else return (pc - b) / a;
+}
+
 // Pass 1 (advect_sl_fine): the backward trace through the coarse velocity from the fine cell's
 // centre, mapped to fine index space; the semi-Lagrangian value of the 3 channels -> hat, the
 // base corner -> qidx (pass 2's clamp), and the fine fluid flag (solid <= 0.5) -> ffm for the
 // later passes.
+template <bool POW2>
 __global__ void __launch_bounds__(256) k_fadv_sl(int S, int N, const real* __restrict__ src,
                                                  const real* __restrict__ vel, const real* __restrict__ hsolid,
-                                                 const float* __restrict__ prm, real inv_dx, real a, real b,
+                                                 const float* __restrict__ prm, real inv_dx, real a, real ia, real b,
                                                  real* __restrict__ hat, int* __restrict__ qidx,
                                                  uint8_t* __restrict__ ffm) {
     const size_t n3 = (size_t)S * S * S;
@@ -42,7 +52,8 @@ __global__ void __launch_bounds__(256) k_fadv_sl(int S, int N, const real* __res
     const real so = hsolid[i];
     real qc[3];
     trace_pt(vel, N, N * N * N, a * (real)x + b, a * (real)y + b, a * (real)z + b, -s, qc);
-    const Corner c = corner(S, (qc[0] - b) / a, (qc[1] - b) / a, (qc[2] - b) / a);
+    const Corner c = corner(S, to_fine<POW2>(qc[0], a, ia, b), to_fine<POW2>(qc[1], a, ia, b),
+                            to_fine<POW2>(qc[2], a, ia, b));
     Oct g[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) g[ch] = gather(src + ch * n3, S, c.o);
@@ -56,9 +67,10 @@ __global__ void __launch_bounds__(256) k_fadv_sl(int S, int N, const real* __res
 // trace's base corner (qidx, clamped into the box so that any content stays in bounds), and the
 // block partial sums of fine fluid density before (src) and after (out) the advection, in double
 // (the fine grid's own _make_mass_kernels pair, solver.py:250-253).
+template <bool POW2>
 __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __restrict__ src,
                                                  const real* __restrict__ vel, const float* __restrict__ prm,
-                                                 real inv_dx, real a, real b, const real* __restrict__ hat,
+                                                 real inv_dx, real a, real ia, real b, const real* __restrict__ hat,
                                                  const int* __restrict__ qidx, const uint8_t* __restrict__ ffm,
                                                  real* __restrict__ out, double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
@@ -74,7 +86,8 @@ __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __res
         const int fl = ffm[i];
         real rc[3];
         trace_pt(vel, N, N * N * N, a * (real)x + b, a * (real)y + b, a * (real)z + b, s, rc);
-        const Corner cr = corner(S, (rc[0] - b) / a, (rc[1] - b) / a, (rc[2] - b) / a);
+        const Corner cr = corner(S, to_fine<POW2>(rc[0], a, ia, b), to_fine<POW2>(rc[1], a, ia, b),
+                                    to_fine<POW2>(rc[2], a, ia, b));
         Oct gs[3], gh[3];
         real h[3], s0[3];
 #pragma unroll
@@ -324,12 +337,13 @@ static void fine_pass(vfx_ctx* c, int pass, const real* src, const real* vel, re
     const Geom& g = h.g;
     switch (pass) {
         case 0:
-            hipLaunchKernelGGL(k_fadv_sl, fine_grid(g), g.block, 0, s, h.S, c->N, src, vel, h.hsolid, c->prm,
-                               c->inv_dx, h.a, h.b, dst, h.qidx, h.ffm);
+            hipLaunchKernelGGL(h.pow2 ? k_fadv_sl<true> : k_fadv_sl<false>, fine_grid(g), g.block, 0, s, h.S, c->N,
+                               src, vel, h.hsolid, c->prm, c->inv_dx, h.a, h.ia, h.b, dst, h.qidx, h.ffm);
             break;
         case 1:
-            hipLaunchKernelGGL(k_fadv_mc, fine_grid(g), g.block, 0, s, h.S, c->N, src, vel, c->prm, c->inv_dx, h.a,
-                               h.b, h.hat, h.qidx, h.ffm, dst, h.part, advect_parts(g));
+            hipLaunchKernelGGL(h.pow2 ? k_fadv_mc<true> : k_fadv_mc<false>, fine_grid(g), g.block, 0, s, h.S, c->N,
+                               src, vel, c->prm, c->inv_dx, h.a, h.ia, h.b, h.hat, h.qidx, h.ffm, dst, h.part,
+                               advect_parts(g));
             break;
         case 2:
             hipLaunchKernelGGL(k_fsources, fine_grid(g), g.block, 0, s, h.S, src, h.ffm, c->prm, c->st, h.dxf, dst);
@@ -431,6 +445,8 @@ int VFX_FN(vfx_hires_attach)(vfx_ctx* c, int S, real* hi, const real* hsolid) {
         h.b = (real)(0.5 * (double)N / (double)S - 0.5);
         h.ratio = (real)((double)S / (double)N);
         h.span = (S + N - 1) / N + 1;
+        h.pow2 = S % N == 0 && ((S / N) & (S / N - 1)) == 0;   // a = 2^-k
+        h.ia = (real)((double)S / (double)N);
         h.nd = std::max(1, (S + 255) / 256);
     }
     h.hi = hi;
