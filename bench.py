@@ -283,6 +283,82 @@ def learned_report(train_res: int, eval_res, iters: int):
 BATCH_SIZES = (48, 64, 96, 128)
 
 
+def pair_report(steps: int, warmup: int, lr_res: int = 64, hr_res: int = 256):
+    """Paired runs (vfx/datagen3d/sim.py run_pair): the free LR runtime run and the HR run guided to
+    follow it, on the reference's plume-past-a-sphere scene with sandbox dissipation.  Each phase of
+    a paired step is bracketed by device events on the stream: the guidance velocity nudge, the HR
+    step, the LR step, the scalar nudge, the tracking error.  Also the guided / unguided tracking."""
+    import numpy as np
+    import torch
+    from vfx.core.fields3d import SIM_DT, Emitter3D, Forces3D, Obstacle3D
+    from vfx.datagen3d import DEFAULT_GUIDE, Guide3D, tracking_error
+    from vfx.datagen3d import sim as pair
+
+    def plume(k):
+        return Forces3D(emitters=[Emitter3D(pos=(0.42, 0.1, 0.5), radius=0.06, density_rate=2.0, temperature=900.0,
+                                            velocity=(0.0, 1.2, 0.0))], wind=(0.4, 0.0, 0.0))
+    sc = pair.PairScenario(plume, num_steps=warmup + steps, sim=dict(density_dissipation=1.2, temperature_cooling=5.0),
+                           obstacles=[Obstacle3D("sphere", (0.5, 0.55, 0.5), (0.12, 0.12, 0.12))])
+    ls, hs = pair.solver(lr_res, True), pair.solver(hr_res, False)
+    pair.configure(ls, sc.sim, True)
+    pair.configure(hs, sc.sim, False)
+    lr = pair.fresh_state(ls, ("lr", lr_res), sc.obstacles)
+    hr = pair.fresh_state(hs, ("hr", hr_res), sc.obstacles)
+    g = Guide3D(DEFAULT_GUIDE, hr.solid, lr_res)
+    names = ("guide_velocity", "hr_step", "lr_step", "guide_scalars", "tracking")
+    ts = torch.cuda.current_stream()
+    rows, tracks = [], []
+    t_wall = 0.0
+    for k in range(warmup + steps):
+        f = sc.forces(k)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev[0].record(ts)
+        g.velocity(hr, lr)
+        ev[1].record(ts)
+        hs.step(hr, f, SIM_DT)
+        ev[2].record(ts)
+        ls.step(lr, f, SIM_DT)
+        ev[3].record(ts)
+        g.scalars(hr, lr)
+        ev[4].record(ts)
+        tr = tracking_error(hr.data, lr.data)
+        ev[5].record(ts)
+        ev[5].synchronize()
+        if k >= warmup:
+            t_wall += time.perf_counter() - t0
+            rows.append([ev[i].elapsed_time(ev[i + 1]) for i in range(5)])
+            tracks.append(tr.cpu().numpy())
+    med = {n: round(statistics.median(r[i] for r in rows), 4) for i, n in enumerate(names)}
+    ms = 1000.0 * t_wall / steps
+    # the guidance kernels' compulsory HBM bytes per call at the HR size: velocity reads the HR
+    # velocity for the pool and reads / writes it for the update (3 * (4 + 8) bytes per cell) plus the
+    # fluid flag; scalars the same for density and temperature (2 * 12) plus the flag
+    h3 = hr_res ** 3
+    vb, sb = 37.0 * h3, 25.0 * h3
+    vel_gbs = vb / (med["guide_velocity"] * 1e-3) / 1e9
+    sc_gbs = sb / (med["guide_scalars"] * 1e-3) / 1e9
+    tr = np.array(tracks)
+    unguided = []
+    pair.free_cache()
+    for s_, _, _, t in pair.run_pair(sc, None, hr_res=hr_res, lr_res=lr_res, steps=min(warmup + steps, 60)):
+        if s_ >= 10:
+            unguided.append(t.cpu().numpy())
+    ung = np.array(unguided)
+    out = {"metric": f"paired run step ms ({lr_res}^3 runtime + {hr_res}^3 guided)", "value": round(ms, 3),
+           "unit": "ms", "higher_is_better": False, "steps": steps, "warmup": warmup,
+           "phases_ms": med, "guide_ms": round(med["guide_velocity"] + med["guide_scalars"], 4),
+           "guide_velocity_gbs": round(vel_gbs, 1), "guide_scalars_gbs": round(sc_gbs, 1),
+           "guide_bytes": {"velocity": int(vb), "scalars": int(sb)},
+           "tracking_guided_worst": [round(float(v), 4) for v in tr.max(axis=0)],
+           "tracking_guided_mean": [round(float(v), 4) for v in tr.mean(axis=0)],
+           "tracking_unguided_worst": [round(float(v), 4) for v in ung.max(axis=0)],
+           "config": {"workload": "plume past a sphere (emitter r 0.06 at (0.42, 0.1, 0.5), 900 K, 1.2 m/s up; "
+                                  "wind 0.4; density dissipation 1.2, cooling 5), DEFAULT_GUIDE"}}
+    print(json.dumps(out))
+
+
 def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: bool = True):
     """Batched frames: nsims independent simulations with N drawn by seed from 48/64/96/128, seeded
     scenes (1-3 emitters, 0-2 sphere obstacles, beta U[0.5, 2]), MGPCG to rel-res 1e-5 with the
@@ -410,6 +486,8 @@ def main():
                     help="instead of the bench line: batched frames of 32 simulations of sizes 48-128 "
                          "(--warmup frames, autotuner, FRAMES timed frames)")
     ap.add_argument("--no-tune", action="store_true", help="--batch without the autotuner")
+    ap.add_argument("--pair", type=int, default=0, metavar="STEPS",
+                    help="paired runs (datagen3d run_pair): 64^3 runtime + 256^3 guided, phase split and tracking")
     ap.add_argument("--learned", type=int, default=0, metavar="ITERS",
                     help="instead of the bench line: train the U-Net ITERS Adam steps at --train-res, then "
                          "learned flexible CG vs MGPCG at 64^3, 128^3 and 256^3")
@@ -436,6 +514,10 @@ def main():
     if args.batch > 0:
         torch.cuda.set_device(torch.device("cuda:0"))
         batch_report(args.batch, args.warmup, args.seed, tune=not args.no_tune)
+        return
+    if args.pair > 0:
+        torch.cuda.set_device(torch.device("cuda:0"))
+        pair_report(args.pair, args.warmup)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

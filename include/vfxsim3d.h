@@ -155,6 +155,32 @@ int vfx_hires_attach(vfx_ctx* ctx, int S, float* hi, const float* hi_solid);
  * `data` resized trilinearly to S^3 (half-pixel centres), 0 / ambient / 0 where hi_solid >= 0.5. */
 int vfx_hires_init(vfx_ctx* ctx, const float* data, float ambient, void* stream);
 
+/* ---------------------------------------------------------------- paired-run guidance
+ * vfx/datagen3d/guide3d.py (Guide3D, tracking_error), the guidance between the 64^3 runtime run and
+ * the 256^3 run of vfx/datagen3d/sim.py:52-79.  FP32 only.  Fields are (6, n, n, n) float32 states. */
+typedef struct vfx_guide vfx_guide;
+
+/* Guide3D.__init__ (guide3d.py:91-100): hr_res = f lr_res for an integer f, lr_res a multiple of 16
+ * in [16, 128]; cfg = 7 floats k_vel, k_scalar, k_fuel, cutoff, eps_density, eps_temp, log_clamp
+ * (GuideConfig3D, guide3d.py:33-41); hr_solid (hr^3 float32) is read once, as the reference keeps
+ * its fluid mask.  Allocates the guide's buffers; synchronises `stream` once. */
+int vfx_guide_create(int hr_res, int lr_res, const float* cfg, const float* hr_solid, void* stream, vfx_guide** out);
+int vfx_guide_destroy(vfx_guide* guide);
+
+/* Guide3D.velocity (guide3d.py:105-111): before the HR step, in place on hr_data. */
+int vfx_guide_velocity(vfx_guide* guide, float* hr_data, const float* lr_data, void* stream);
+
+/* Guide3D.scalars (guide3d.py:113-130): after the HR step, in place on hr_data. */
+int vfx_guide_scalars(vfx_guide* guide, float* hr_data, const float* lr_data, void* stream);
+
+/* lowpass (guide3d.py:60-68) of C <= 3 channels (C, lr, lr, lr) into out (may not alias x). */
+int vfx_guide_lowpass(vfx_guide* guide, const float* x, int C, float* out, void* stream);
+
+/* tracking_error (guide3d.py:133-145): out3 (device, 3 floats) = relative L2 at res^3 of hr vs lr
+ * (density, temperature - ambient, velocity); scratch holds 12 res^3 floats (device). */
+int vfx_tracking_error(const float* hr_data, int hr_res, const float* lr_data, int lr_res, int res, float* scratch,
+                       float* out3, void* stream);
+
 /* Message of the last failed call on this thread. */
 int vfx_last_error(char* buf, int n);
 

@@ -13,13 +13,17 @@ import torch
 
 from .fields import AMBIENT_TEMP, DOMAIN_SIZE, SIM_DT
 
+LR_RES_3D: int = 64                 # the runtime (low-resolution) grid of the paired runs
+UPSCALE_3D: int = 4
+HR_RES_3D: int = LR_RES_3D * UPSCALE_3D   # 256
+
 CHANNELS_3D: Tuple[str, ...] = ("density", "vel_x", "vel_y", "vel_z", "temperature", "fuel")
 CH3 = {name: i for i, name in enumerate(CHANNELS_3D)}
 NUM_CHANNELS_3D = len(CHANNELS_3D)
 
 Vec3 = Tuple[float, float, float]
 
-__all__ = ["AMBIENT_TEMP", "DOMAIN_SIZE", "SIM_DT", "CHANNELS_3D", "CH3", "NUM_CHANNELS_3D", "FieldMeta3D",
+__all__ = ["AMBIENT_TEMP", "DOMAIN_SIZE", "SIM_DT", "LR_RES_3D", "UPSCALE_3D", "HR_RES_3D", "CHANNELS_3D", "CH3", "NUM_CHANNELS_3D", "FieldMeta3D",
            "FieldState3D", "Volume", "Emitter3D", "Impulse3D", "Forces3D", "Obstacle3D", "rasterize_obstacles_3d"]
 
 
