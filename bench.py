@@ -359,6 +359,50 @@ def pair_report(steps: int, warmup: int, lr_res: int = 64, hr_res: int = 256):
     print(json.dumps(out))
 
 
+def render_report(reps: int):
+    """The volume renderer (vfx/render3d VolumeRenderer) at 2560 x 1440 on the solver's own 256^3
+    fire state (and 64^3, 128^3 states of the same scene), as the reference's feasibility table
+    (render3d README: full draw, view-only redraw with reuse_volume, prep / view split).  Times are
+    device events around each draw, median of `reps` draws after 3 warm-up draws."""
+    import torch
+    from vfx.core.fields3d import Camera, Volume
+    from vfx.render3d import RenderParams3D, VolumeRenderer
+    from vfx.sim3d import Solver3D
+
+    rows = []
+    cam = Camera(position=(0.5, 0.55, -1.4), target=(0.5, 0.45, 0.5))
+    for N in (64, 128, 256):
+        s = Solver3D(N, fire_config(0))
+        st = s.reset()
+        f = fire_forces()
+        for _ in range(60):
+            s.step(st, f, 1.0 / 60.0)
+        vol = Volume(density=st.ch("density"), temperature=st.ch("temperature"))
+        for scale, steps in ((0.5, 192.0), (1.0, 64.0), (1.0, 192.0)):
+            r = VolumeRenderer((1440, 2560), device="cuda:0", march_scale=scale,
+                               params=RenderParams3D(steps=steps), profile=True)
+            full, prep, view, reuse = [], [], [], []
+            for k in range(3 + reps):
+                r.draw(vol, cam)
+                if k >= 3:
+                    t = r.last_timings()
+                    full.append(t["total"])
+                    prep.append(t["prep"])
+                    view.append(t["view"])
+                r.draw(vol, cam, reuse_volume=True)
+                if k >= 3:
+                    reuse.append(r.last_timings()["total"])
+            med = lambda v: round(statistics.median(v), 3)
+            rows.append({"volume": N, "march_scale": scale, "steps_per_m": steps, "march_px": f"{r.mw}x{r.mh}",
+                         "full_ms": med(full), "prep_ms": med(prep), "view_ms": med(view), "reuse_ms": med(reuse)})
+            del r
+        del s, st
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": "volume render ms at 2560x1440 (full draw / view-only redraw)", "rows": rows,
+                      "config": {"workload": "the solver's fire state after 60 steps, default RenderParams3D "
+                                             "(light_res 80), camera (0.5, 0.55, -1.4) -> (0.5, 0.45, 0.5)"}}))
+
+
 def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: bool = True):
     """Batched frames: nsims independent simulations with N drawn by seed from 48/64/96/128, seeded
     scenes (1-3 emitters, 0-2 sphere obstacles, beta U[0.5, 2]), MGPCG to rel-res 1e-5 with the
@@ -486,6 +530,8 @@ def main():
                     help="instead of the bench line: batched frames of 32 simulations of sizes 48-128 "
                          "(--warmup frames, autotuner, FRAMES timed frames)")
     ap.add_argument("--no-tune", action="store_true", help="--batch without the autotuner")
+    ap.add_argument("--render", type=int, default=0, metavar="REPS",
+                    help="volume renderer at 2560x1440 on 64^3 / 128^3 / 256^3 solver states (render3d)")
     ap.add_argument("--pair", type=int, default=0, metavar="STEPS",
                     help="paired runs (datagen3d run_pair): 64^3 runtime + 256^3 guided, phase split and tracking")
     ap.add_argument("--learned", type=int, default=0, metavar="ITERS",
@@ -518,6 +564,10 @@ def main():
     if args.pair > 0:
         torch.cuda.set_device(torch.device("cuda:0"))
         pair_report(args.pair, args.warmup)
+        return
+    if args.render > 0:
+        torch.cuda.set_device(torch.device("cuda:0"))
+        render_report(args.render)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

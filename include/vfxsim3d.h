@@ -181,6 +181,37 @@ int vfx_guide_lowpass(vfx_guide* guide, const float* x, int C, float* out, void*
 int vfx_tracking_error(const float* hr_data, int hr_res, const float* lr_data, int lr_res, int res, float* scratch,
                        float* out3, void* stream);
 
+/* ---------------------------------------------------------------- volume renderer
+ * vfx/render3d/renderer.py VolumeRenderer (default path: colour only, no temporal resolve, no detail
+ * field).  FP32 only.  params: VFX_RENDER_PARAMS floats, RenderParams3D's fields in declaration order
+ * (renderer.py:31-66) with the 3-tuples flattened: steps, light_res, k_abs, k_shadow, k_solid, k_amb,
+ * k_emit, emit_min, emit_scale, soot, fire_light, phase_g, rim, ms_a, ms_b, ms_c, exposure, contrast,
+ * bloom, bloom_thresh, t_lo, t_hi, t_ref, d_eps, light_dir[3], key_col[3], amb_col[3],
+ * smoke_albedo[3], bg_top[3], bg_bot[3], floor_col[3], grid_col[3], obst_col[3], glow_coarse,
+ * floor.  lut: the 256 x 3 float blackbody table (vfx/render/blackbody.py). */
+#define VFX_RENDER_PARAMS 53
+typedef struct vfx_render vfx_render;
+
+/* VolumeRenderer.__init__ / set_output (renderer.py:96-156): march resolution
+ * max(8, round(out * march_scale)); allocates the renderer's buffers. */
+int vfx_render_create(int out_h, int out_w, float march_scale, const float* params, const float* lut,
+                      vfx_render** out);
+int vfx_render_destroy(vfx_render* r);
+int vfx_render_set_output(vfx_render* r, int out_h, int out_w, float march_scale);
+/* set_params (renderer.py:168-174): a new light_res reallocates the light buffers. */
+int vfx_render_set_params(vfx_render* r, const float* params, const float* lut);
+
+/* VolumeRenderer.draw (renderer.py:281-386): dens / temp (D, H, W) float32 device, solid (sD, sH, sW)
+ * float32 device or NULL (upload_solid != 0 when it changed since the last draw), cam = 16 host floats
+ * (camera_basis, renderer.py:69-82), reuse_volume != 0 re-renders the last prepared volume, out =
+ * (out_h, out_w, 3) uint8 device.  ms_prep / ms_view (host, optional): device time of the volume prep
+ * and of the view part (synchronises when given). */
+int vfx_render_draw(vfx_render* r, const float* dens, const float* temp, int D, int H, int W, const float* solid,
+                    int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
+                    void* stream, float* ms_prep, float* ms_view);
+/* The march buffer of the last draw, (mh, mw, 4) float: premultiplied radiance, transmittance. */
+int vfx_render_get_hdr(vfx_render* r, float* dst, int* mh, int* mw, void* stream);
+
 /* Message of the last failed call on this thread. */
 int vfx_last_error(char* buf, int n);
 

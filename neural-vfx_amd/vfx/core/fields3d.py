@@ -23,8 +23,9 @@ NUM_CHANNELS_3D = len(CHANNELS_3D)
 
 Vec3 = Tuple[float, float, float]
 
-__all__ = ["AMBIENT_TEMP", "DOMAIN_SIZE", "SIM_DT", "LR_RES_3D", "UPSCALE_3D", "HR_RES_3D", "CHANNELS_3D", "CH3", "NUM_CHANNELS_3D", "FieldMeta3D",
-           "FieldState3D", "Volume", "Emitter3D", "Impulse3D", "Forces3D", "Obstacle3D", "rasterize_obstacles_3d"]
+__all__ = ["AMBIENT_TEMP", "DOMAIN_SIZE", "SIM_DT", "LR_RES_3D", "UPSCALE_3D", "HR_RES_3D", "CHANNELS_3D", "CH3",
+           "NUM_CHANNELS_3D", "FieldMeta3D", "FieldState3D", "Volume", "Camera", "Emitter3D", "Impulse3D", "Forces3D",
+           "Obstacle3D", "rasterize_obstacles_3d"]
 
 
 @dataclass
@@ -113,3 +114,13 @@ def rasterize_obstacles_3d(obstacles: List[Obstacle3D], res: int, device: str = 
             raise ValueError(o.kind)
         solid[inside] = 1.0
     return solid
+
+
+@dataclass
+class Camera:
+    """Perspective camera in world space; the domain cube spans [0, DOMAIN_SIZE]^3 (the reference's
+    vfx/core/fields3d.py:119-125)."""
+    position: Vec3 = (0.5, 0.55, -1.4)
+    target: Vec3 = (0.5, 0.45, 0.5)
+    up: Vec3 = (0.0, 1.0, 0.0)
+    fov_y_deg: float = 40.0

@@ -54,10 +54,12 @@ SIGNATURES = {
 # the FP64 run of the same solver: every entry point again with the suffix _f64 (double fields;
 # the argument kinds are the same, pointers are untyped here)
 SIGNATURES.update({name + "_f64": args for name, args in list(SIGNATURES.items())})
-# the learned preconditioner, the batched frames and the paired-run guidance (FP32 build only)
+# the learned preconditioner, the batched frames, the paired-run guidance and the renderer (FP32 build only)
 FP32_ONLY = ("vfx_ctx_set_net", "vfx_fcg_history", "vfx_batch_create", "vfx_batch_destroy", "vfx_batch_set_budget",
              "vfx_batch_get_budget", "vfx_batch_step", "vfx_batch_wait", "vfx_batch_tune", "vfx_guide_create",
-             "vfx_guide_destroy", "vfx_guide_velocity", "vfx_guide_scalars", "vfx_guide_lowpass", "vfx_tracking_error")
+             "vfx_guide_destroy", "vfx_guide_velocity", "vfx_guide_scalars", "vfx_guide_lowpass", "vfx_tracking_error",
+             "vfx_render_create", "vfx_render_destroy", "vfx_render_set_output", "vfx_render_set_params",
+             "vfx_render_draw", "vfx_render_get_hdr")
 _F = ctypes.c_float
 SIGNATURES.update({
     "vfx_net_create": [_I, ctypes.POINTER(_P)],
@@ -85,6 +87,13 @@ SIGNATURES.update({
     "vfx_guide_scalars": [_P, _P, _P, _P],
     "vfx_guide_lowpass": [_P, _P, _I, _P, _P],
     "vfx_tracking_error": [_P, _I, _P, _I, _I, _P, _P, _P],
+    "vfx_render_create": [_I, _I, _F, _P, _P, ctypes.POINTER(_P)],
+    "vfx_render_destroy": [_P],
+    "vfx_render_set_output": [_P, _I, _I, _F],
+    "vfx_render_set_params": [_P, _P, _P],
+    "vfx_render_draw": [_P, _P, _P, _I, _I, _I, _P, _I, _I, _I, _I, _P, _I, _P, _P, ctypes.POINTER(_F),
+                        ctypes.POINTER(_F)],
+    "vfx_render_get_hdr": [_P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I), _P],
 })
 GUIDE_CFG = ("k_vel", "k_scalar", "k_fuel", "cutoff", "eps_density", "eps_temp", "log_clamp")   # vfx_guide_create cfg
 

@@ -1,0 +1,438 @@
+/* CPU restatement of the reference's volume-renderer kernels (vfx/render3d/kernels.py), FP32.
+ *
+ * TEST INFRASTRUCTURE ONLY: the oracle the HIP renderer (neural-vfx_amd/csrc/render.hip) is checked
+ * against; the product never links it.  Driven by oracle/render_ref.py, which does the fp16
+ * rounding of the stored grids and the reference's torch steps (blurs, pools, resizes).
+ *
+ * Textures are sampled with the normalized-coordinate convention of the reference's linear-filter,
+ * clamp-address 3D textures (texel centres at (i + 1/2) / n), in FP32 with exact weights.
+ * `prm` is the renderer's packed parameter array (RenderParams3D fields in order, tuples flat).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#define BRICKS 32
+#define GLOW 32
+#define LUT_N 256
+
+typedef struct { float x, y, z; } v3;
+
+static inline v3 V(float a, float b, float c) { v3 r = {a, b, c}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 vcw(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float vlen(v3 a) { return sqrtf(vdot(a, a)); }
+static inline v3 vnorm(v3 a) { float l = vlen(a); return l > 0.f ? V(a.x / l, a.y / l, a.z / l) : V(0.f, 0.f, 0.f); }
+static inline float clampf_(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static inline float fracf_(float x) { return x - floorf(x); }
+static inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static inline float smooth(float e0, float e1, float x) {
+    float t = clampf_((x - e0) / (e1 - e0), 0.f, 1.f);
+    return t * t * (3.f - 2.f * t);
+}
+
+typedef struct {
+    float steps, k_abs, k_shadow, k_solid, k_amb, k_emit, emit_min, emit_scale, soot, fire_light, phase_g, rim;
+    float ms_a, ms_b, ms_c, exposure, contrast, bloom, bloom_thresh, t_lo, t_hi, d_eps, glow_coarse, floor_;
+    int L;
+    v3 light_dir, key_col, amb_col, smoke_albedo, bg_top, bg_bot, floor_col, grid_col, obst_col;
+} shade_t;
+
+/* the packed parameter layout: see RenderParams3D.pack (kernels.py:19-52 for the meanings) */
+static shade_t unpack(const float* v) {
+    shade_t s;
+    int i = 0;
+    s.steps = v[i++];
+    s.L = (int)v[i++];
+    s.k_abs = v[i++]; s.k_shadow = v[i++]; s.k_solid = v[i++]; s.k_amb = v[i++]; s.k_emit = v[i++];
+    s.emit_min = v[i++]; s.emit_scale = v[i++]; s.soot = v[i++]; s.fire_light = v[i++]; s.phase_g = v[i++];
+    s.rim = v[i++]; s.ms_a = v[i++]; s.ms_b = v[i++]; s.ms_c = v[i++]; s.exposure = v[i++]; s.contrast = v[i++];
+    s.bloom = v[i++]; s.bloom_thresh = v[i++]; s.t_lo = v[i++]; s.t_hi = v[i++];
+    i++; /* t_ref */
+    s.d_eps = v[i++];
+    {
+        double lx = v[i], ly = v[i + 1], lz = v[i + 2], ll = sqrt(lx * lx + ly * ly + lz * lz);
+        s.light_dir = V((float)(lx / ll), (float)(ly / ll), (float)(lz / ll));
+        i += 3;
+    }
+#define COL(c) do { c = V(v[i], v[i + 1], v[i + 2]); i += 3; } while (0)
+    COL(s.key_col); COL(s.amb_col); COL(s.smoke_albedo); COL(s.bg_top); COL(s.bg_bot); COL(s.floor_col);
+    COL(s.grid_col); COL(s.obst_col);
+#undef COL
+    s.glow_coarse = v[i++];
+    s.floor_ = v[i++];
+    return s;
+}
+
+/* kernels.py:70-77 */
+static void box_hit(v3 o, v3 invd, v3 lo, v3 hi, float* tn, float* tf) {
+    v3 t0 = vcw(vsub(lo, o), invd), t1 = vcw(vsub(hi, o), invd);
+    float nx = fminf(t0.x, t1.x), ny = fminf(t0.y, t1.y), nz = fminf(t0.z, t1.z);
+    float xx = fmaxf(t0.x, t1.x), xy = fmaxf(t0.y, t1.y), xz = fmaxf(t0.z, t1.z);
+    *tn = fmaxf(fmaxf(nx, ny), nz);
+    *tf = fminf(fminf(xx, xy), xz);
+}
+
+static v3 safe_inv(v3 d) {
+    const float e = 1.0e-8f;
+    float x = fabsf(d.x) < e ? (d.x < 0.f ? -e : e) : d.x;
+    float y = fabsf(d.y) < e ? (d.y < 0.f ? -e : e) : d.y;
+    float z = fabsf(d.z) < e ? (d.z < 0.f ? -e : e) : d.z;
+    return V(1.f / x, 1.f / y, 1.f / z);
+}
+
+static float hg(float c, float g) {
+    float d = 1.f + g * g - 2.f * g * c;
+    return (1.f - g * g) / (d * sqrtf(d));
+}
+
+static float ign(float x, float y) {
+    float f = 52.9829189f * fracf_(0.06711056f * x + 0.00583715f * y);
+    return fracf_(f);
+}
+
+static v3 lut_lookup(const float* lut, float T, float lo, float hi) {
+    float f = clampf_((T - lo) / (hi - lo), 0.f, 1.f) * (float)(LUT_N - 1);
+    int i0 = (int)f < LUT_N - 2 ? (int)f : LUT_N - 2;
+    float t = f - (float)i0;
+    const float* a = lut + 3 * i0;
+    return V(a[0] * (1.f - t) + a[3] * t, a[1] * (1.f - t) + a[4] * t, a[2] * (1.f - t) + a[5] * t);
+}
+
+static v3 emission(const float* lut, const shade_t* p, float d, float T) {
+    float e = p->emit_min + (1.f - p->emit_min) * (1.f - expf(-p->k_emit * d));
+    return vmul(lut_lookup(lut, T, p->t_lo, p->t_hi), e * p->emit_scale);
+}
+
+/* normalized coordinate -> texel pair and weight (clamp addressing) */
+static void tw(float u, int n, int* i0, int* i1, float* t) {
+    float f = clampf_(u, -1.f, 2.f) * (float)n - 0.5f;
+    float fl = floorf(f);
+    int i = (int)fl;
+    *t = f - fl;
+    *i0 = clampi(i, 0, n - 1);
+    *i1 = clampi(i + 1, 0, n - 1);
+}
+
+/* trilinear sample of channel c of a (D, H, W, C) float grid */
+static float tex(const float* g, int D, int H, int W, int C, int c, v3 q) {
+    int x0, x1, y0, y1, z0, z1;
+    float tx, ty, tz;
+    tw(q.x, W, &x0, &x1, &tx);
+    tw(q.y, H, &y0, &y1, &ty);
+    tw(q.z, D, &z0, &z1, &tz);
+#define AT(z, y, x) g[(((size_t)(z) * H + (y)) * W + (x)) * C + c]
+    float a00 = AT(z0, y0, x0) * (1.f - tx) + AT(z0, y0, x1) * tx;
+    float a01 = AT(z0, y1, x0) * (1.f - tx) + AT(z0, y1, x1) * tx;
+    float a10 = AT(z1, y0, x0) * (1.f - tx) + AT(z1, y0, x1) * tx;
+    float a11 = AT(z1, y1, x0) * (1.f - tx) + AT(z1, y1, x1) * tx;
+#undef AT
+    return (a00 * (1.f - ty) + a01 * ty) * (1.f - tz) + (a10 * (1.f - ty) + a11 * ty) * tz;
+}
+
+static void mark(uint8_t* occ, int z, int y, int x, int D, int H, int W) {
+    int bz0 = (z - 1 > 0 ? z - 1 : 0) * BRICKS / D, bz1 = (z + 1 < D - 1 ? z + 1 : D - 1) * BRICKS / D;
+    int by0 = (y - 1 > 0 ? y - 1 : 0) * BRICKS / H, by1 = (y + 1 < H - 1 ? y + 1 : H - 1) * BRICKS / H;
+    int bx0 = (x - 1 > 0 ? x - 1 : 0) * BRICKS / W, bx1 = (x + 1 < W - 1 ? x + 1 : W - 1) * BRICKS / W;
+    for (int bz = bz0; bz <= bz1; ++bz)
+        for (int by = by0; by <= by1; ++by)
+            for (int bx = bx0; bx <= bx1; ++bx) occ[(bz * BRICKS + by) * BRICKS + bx] = 1;
+}
+
+/* occupancy of pack_volume (kernels.py:124-147) / pack_solid (:175-194): mode 0 density above d_eps
+ * or temperature above t_hot; mode 1 solid > 0 */
+void or_render_occ(const float* a, const float* b, int D, int H, int W, int mode, float d_eps, float t_hot,
+                   uint8_t* occ) {
+    for (int i = 0; i < BRICKS * BRICKS * BRICKS; ++i) occ[i] = 0;
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                size_t i = ((size_t)z * H + y) * W + x;
+                int on = mode == 0 ? (a[i] > d_eps || b[i] > t_hot) : (a[i] > 0.f);
+                if (on) mark(occ, z, y, x, D, H, W);
+            }
+}
+
+/* light_volume (kernels.py:203-254): vol (D, H, W, 2) density / temperature, solid (sD, sH, sW)
+ * values in [0, 1]; light (L^3, 2), glow_src (L^3, 3) */
+void or_render_light(const float* vol, int D, int H, int W, const float* solid, int sD, int sH, int sW,
+                     int has_solid, const float* lut, const float* prm, float* light, float* glow_src) {
+    shade_t p = unpack(prm);
+    int L = p.L;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < L; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                size_t i = ((size_t)z * L + y) * L + x;
+                float inv = 1.f / (float)L;
+                v3 pos = V(((float)x + 0.5f) * inv, ((float)y + 0.5f) * inv, ((float)z + 0.5f) * inv);
+                float hd = tex(vol, D, H, W, 2, 0, pos), hT = tex(vol, D, H, W, 2, 1, pos);
+                v3 ld = p.light_dir;
+                float tn, tf;
+                box_hit(pos, safe_inv(ld), V(0.f, 0.f, 0.f), V(1.f, 1.f, 1.f), &tn, &tf);
+                float ds = 1.5f * inv, od = 0.f;
+                float t = (0.25f + 0.75f * ign((float)x + 7.f * (float)z, (float)y + 13.f * (float)z)) * ds;
+                while (t < tf) {
+                    v3 q = vadd(pos, vmul(ld, t));
+                    od += p.k_shadow * tex(vol, D, H, W, 2, 0, q) * ds;
+                    if (has_solid) od += p.k_solid * tex(solid, sD, sH, sW, 1, 0, q) * ds;
+                    if (od > 12.f) break;
+                    t += ds;
+                }
+                float oda = 0.f, da = 3.f * inv;
+                for (int k = 0; k < 6; ++k) {
+                    float qy = pos.y + ((float)k + 0.5f) * da;
+                    if (qy < 1.f) oda += p.k_amb * tex(vol, D, H, W, 2, 0, V(pos.x, qy, pos.z)) * da;
+                }
+                light[2 * i] = expf(-od);
+                light[2 * i + 1] = expf(-oda);
+                v3 e = V(0.f, 0.f, 0.f);
+                if (hT > p.t_lo) e = emission(lut, &p, hd, hT);
+                glow_src[3 * i] = e.x;
+                glow_src[3 * i + 1] = e.y;
+                glow_src[3 * i + 2] = e.z;
+            }
+}
+
+static void light_at(const float* light, int L, v3 ld, v3 q, float* l0, float* l1) {
+    int inside = q.x >= 0.f && q.x <= 1.f && q.y >= 0.f && q.y <= 1.f && q.z >= 0.f && q.z <= 1.f;
+    if (inside) {
+        *l0 = tex(light, L, L, L, 2, 0, q);
+        *l1 = tex(light, L, L, L, 2, 1, q);
+        return;
+    }
+    float tn, tf;
+    box_hit(q, safe_inv(ld), V(0.f, 0.f, 0.f), V(1.f, 1.f, 1.f), &tn, &tf);
+    if (tn <= tf && tf > 0.f) {
+        v3 e = vadd(q, vmul(ld, fmaxf(tn, 0.f) + 1.0e-3f));
+        *l0 = tex(light, L, L, L, 2, 0, e);
+        *l1 = 1.f;
+        return;
+    }
+    *l0 = 1.f;
+    *l1 = 1.f;
+}
+
+static v3 glow_at(const float* glow, v3 q) {
+    v3 c = V(clampf_(q.x, 0.f, 1.f), clampf_(q.y, 0.f, 1.f), clampf_(q.z, 0.f, 1.f));
+    v3 g = V(tex(glow, GLOW, GLOW, GLOW, 3, 0, c), tex(glow, GLOW, GLOW, GLOW, 3, 1, c), tex(glow, GLOW, GLOW, GLOW, 3, 2, c));
+    float dout = vlen(vsub(q, c));
+    return vmul(g, 1.f / (1.f + 60.f * dout * dout));
+}
+
+/* march (kernels.py:288-489, colour only): hdr (mh, mw, 4); skip = 0 disables empty-brick skipping */
+void or_render_march(const float* vol, int D, int H, int W, const float* light, const float* glow,
+                     const float* solid, int sD, int sH, int sW, const uint8_t* occ, const uint8_t* occ_s,
+                     int has_solid, float solid_res, const float* lut, const float* prm, const float* cam, int h,
+                     int w, int skip, float* out) {
+    shade_t p = unpack(prm);
+    int L = p.L;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int r = 0; r < h; ++r)
+        for (int c = 0; c < w; ++c) {
+            v3 o = V(cam[0], cam[1], cam[2]), fw = V(cam[3], cam[4], cam[5]), rt = V(cam[6], cam[7], cam[8]);
+            v3 up = V(cam[9], cam[10], cam[11]);
+            float th = cam[12], aspect = cam[13];
+            float sx = (2.f * ((float)c + 0.5f) / (float)w - 1.f) * th * aspect;
+            float sy = (1.f - 2.f * ((float)r + 0.5f) / (float)h) * th;
+            v3 d = vnorm(vadd(vadd(fw, vmul(rt, sx)), vmul(up, sy)));
+            v3 invd = safe_inv(d), ld = p.light_dir;
+            v3 rad = V(0.f, 0.f, 0.f);
+            float T = 1.f, tn, tf;
+            box_hit(o, invd, V(0.f, 0.f, 0.f), V(1.f, 1.f, 1.f), &tn, &tf);
+            float t_near = fmaxf(tn, 0.f), t_far = tf, ds = 1.f / p.steps;
+            float cos_t = vdot(d, ld);
+            float ph0 = hg(cos_t, p.phase_g) * p.rim + hg(cos_t, -0.25f) * (1.f - p.rim);
+            float ph1 = hg(cos_t, p.phase_g * p.ms_c) * p.rim + hg(cos_t, -0.25f * p.ms_c) * (1.f - p.rim);
+            float ph2 = hg(cos_t, p.phase_g * p.ms_c * p.ms_c) * p.rim + (1.f - p.rim);
+            float a1 = p.ms_a, a2 = p.ms_a * p.ms_a, b1 = p.ms_b, b2 = p.ms_b * p.ms_b;
+            if (t_near < t_far) {
+                float jit = fracf_(ign((float)c, (float)r) + cam[14]);
+                float t0 = t_near + jit * ds, t = t0, t_prev = t_near;
+                while (t < t_far) {
+                    v3 q = vadd(o, vmul(d, t));
+                    int bx = clampi((int)(q.x * (float)BRICKS), 0, BRICKS - 1);
+                    int by = clampi((int)(q.y * (float)BRICKS), 0, BRICKS - 1);
+                    int bz = clampi((int)(q.z * (float)BRICKS), 0, BRICKS - 1);
+                    int bi = (bz * BRICKS + by) * BRICKS + bx;
+                    int o_d = skip ? occ[bi] : 1, o_s = has_solid ? (skip ? occ_s[bi] : 1) : 0;
+                    if (o_d == 0 && o_s == 0) {
+                        float inv_b = 1.f / (float)BRICKS, bn, bf;
+                        v3 lo = vmul(V((float)bx, (float)by, (float)bz), inv_b);
+                        box_hit(o, invd, lo, vadd(lo, V(inv_b, inv_b, inv_b)), &bn, &bf);
+                        float k = ceilf((bf - t0) / ds + 1.0e-4f);
+                        t_prev = t;
+                        t = fmaxf(t0 + k * ds, t + ds);
+                        continue;
+                    }
+                    if (o_s != 0 && tex(solid, sD, sH, sW, 1, 0, q) > 0.5f) {
+                        float a = t_prev, b = t;
+                        for (int it = 0; it < 5; ++it) {
+                            float m = 0.5f * (a + b);
+                            if (tex(solid, sD, sH, sW, 1, 0, vadd(o, vmul(d, m))) > 0.5f) b = m;
+                            else a = m;
+                        }
+                        v3 ph = vadd(o, vmul(d, b));
+                        float e = 1.f / solid_res;
+#define S(v) tex(solid, sD, sH, sW, 1, 0, v)
+                        float gx = S(vadd(ph, V(e, 0.f, 0.f))) - S(vsub(ph, V(e, 0.f, 0.f)));
+                        float gy = S(vadd(ph, V(0.f, e, 0.f))) - S(vsub(ph, V(0.f, e, 0.f)));
+                        float gz = S(vadd(ph, V(0.f, 0.f, e))) - S(vsub(ph, V(0.f, 0.f, e)));
+#undef S
+                        v3 n = V(-gx, -gy, -gz);
+                        if (vlen(n) < 1.0e-6f) n = V(-d.x, -d.y, -d.z);
+                        n = vnorm(n);
+                        v3 lq = vadd(ph, vmul(n, 0.03f));
+                        float lv0 = tex(light, L, L, L, 2, 0, lq), lv1 = tex(light, L, L, L, 2, 1, lq);
+                        v3 gl = glow_at(glow, lq);
+                        float ndl = fmaxf(vdot(n, ld), 0.f);
+                        v3 hv = vnorm(vsub(ld, d));
+                        float spec = powf(fmaxf(vdot(n, hv), 0.f), 40.f) * 0.25f;
+                        float hemi = 0.45f + 0.55f * (0.5f + 0.5f * n.y);
+                        float fres = powf(1.f - fmaxf(vdot(n, V(-d.x, -d.y, -d.z)), 0.f), 4.f) * 0.08f;
+                        v3 col = vcw(p.obst_col, vadd(vadd(vmul(p.key_col, ndl * lv0), vmul(p.amb_col, hemi * lv1)),
+                                                      vmul(gl, 2.f * p.fire_light)));
+                        col = vadd(vadd(col, vmul(p.key_col, spec * lv0)), vmul(p.amb_col, fres));
+                        rad = vadd(rad, vmul(col, T));
+                        T = 0.f;
+                        break;
+                    }
+                    if (o_d != 0) {
+                        float dn = tex(vol, D, H, W, 2, 0, q), Tk = tex(vol, D, H, W, 2, 1, q);
+                        int hot = Tk > p.t_lo;
+                        if (dn > p.d_eps || hot) {
+                            float sigma = p.k_abs * fmaxf(dn, 0.f);
+                            float al = 1.f - expf(-sigma * ds);
+                            v3 col = V(0.f, 0.f, 0.f);
+                            if (al > 1.0e-5f) {
+                                float lv0 = tex(light, L, L, L, 2, 0, q), lv1 = tex(light, L, L, L, 2, 1, q);
+                                v3 gl = glow_at(glow, q);
+                                float shd = smooth(p.t_lo, p.t_lo + 500.f, Tk);
+                                v3 alb = vmul(p.smoke_albedo, 1.f - p.soot * shd);
+                                float tl = fmaxf(lv0, 1.0e-6f);
+                                float key = tl * ph0 + a1 * powf(tl, b1) * ph1 + a2 * powf(tl, b2) * ph2;
+                                v3 lit = vadd(vadd(vmul(p.key_col, key), vmul(p.amb_col, lv1)), vmul(gl, p.fire_light));
+                                col = vmul(vcw(alb, lit), al);
+                            }
+                            if (hot) col = vadd(col, vmul(emission(lut, &p, dn, Tk), ds));
+                            rad = vadd(rad, vmul(col, T));
+                            T = T * (1.f - al);
+                            if (T < 0.004f) {
+                                T = 0.f;
+                                break;
+                            }
+                        }
+                    }
+                    t_prev = t;
+                    t = t + ds;
+                }
+            }
+            if (T > 0.f && d.y < -1.0e-4f && o.y > 0.f) {
+                float tfl = -o.y / d.y;
+                v3 pf = vadd(o, vmul(d, tfl));
+                float cx = pf.x - 0.5f, cz = pf.z - 0.5f;
+                float rr = sqrtf(cx * cx + cz * cz);
+                float fade = (1.f - smooth(0.35f, 1.9f, rr)) * p.floor_;
+                if (fade > 0.f) {
+                    float pix = tfl * 2.f * th / (float)h, gs = 0.125f;
+                    float gxl = fabsf(pf.x / gs - roundf(pf.x / gs)) * gs;
+                    float gzl = fabsf(pf.z / gs - roundf(pf.z / gs)) * gs;
+                    float line = 1.f - smooth(0.f, 1.2f * pix + 0.0015f, fminf(gxl, gzl));
+                    int inside = pf.x >= 0.f && pf.x <= 1.f && pf.z >= 0.f && pf.z <= 1.f;
+                    v3 lq = V(pf.x, 0.004f, pf.z);
+                    float l0, l1;
+                    light_at(light, L, ld, lq, &l0, &l1);
+                    v3 gl = glow_at(glow, lq);
+                    v3 base = vadd(p.floor_col, vmul(p.grid_col, line * (inside ? 1.f : 0.45f)));
+                    v3 lit = vadd(vadd(vmul(p.key_col, ld.y * l0), vmul(p.amb_col, 0.6f + 0.4f * l1)),
+                                  vmul(gl, 2.5f * p.fire_light));
+                    v3 fc = vcw(base, lit);
+                    rad = vadd(rad, vmul(fc, T * fade));
+                    T = T * (1.f - fade);
+                }
+            }
+            float* px = out + ((size_t)r * w + c) * 4;
+            px[0] = rad.x; px[1] = rad.y; px[2] = rad.z; px[3] = T;
+        }
+}
+
+static float aces(float x) {
+    return clampf_((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f), 0.f, 1.f);
+}
+
+static uint8_t to_srgb8(float x, float dither) {
+    float s = x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+    return (uint8_t)clampi((int)(s * 255.f + 0.5f + dither), 0, 255);
+}
+
+/* composite (kernels.py:571-600) with the Catmull-Rom upsample (:512-549) and bilinear bloom */
+void or_render_composite(const float* hdr, int mh, int mw, const float* bloom, int bh, int bw, const float* prm,
+                         int Ho, int Wo, uint8_t* out) {
+    shade_t p = unpack(prm);
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < Ho; ++r)
+        for (int c = 0; c < Wo; ++c) {
+            float u = ((float)c + 0.5f) / (float)Wo, v = ((float)r + 0.5f) / (float)Ho;
+            float px[4];
+            if (mh == Ho && mw == Wo) {
+                for (int k = 0; k < 4; ++k) px[k] = hdr[((size_t)r * mw + c) * 4 + k];
+            } else {
+                float fx = u * (float)mw - 0.5f, fy = v * (float)mh - 0.5f;
+                float x0f = floorf(fx), y0f = floorf(fy);
+                float wx[4], wy[4], t;
+                t = fx - x0f;
+                wx[0] = -0.5f * t * t * t + t * t - 0.5f * t; wx[1] = 1.5f * t * t * t - 2.5f * t * t + 1.f;
+                wx[2] = -1.5f * t * t * t + 2.f * t * t + 0.5f * t; wx[3] = 0.5f * t * t * t - 0.5f * t * t;
+                t = fy - y0f;
+                wy[0] = -0.5f * t * t * t + t * t - 0.5f * t; wy[1] = 1.5f * t * t * t - 2.5f * t * t + 1.f;
+                wy[2] = -1.5f * t * t * t + 2.f * t * t + 0.5f * t; wy[3] = 0.5f * t * t * t - 0.5f * t * t;
+                int xi = (int)x0f, yi = (int)y0f;
+                float acc[4] = {0.f, 0.f, 0.f, 0.f}, lo[4] = {1e9f, 1e9f, 1e9f, 1e9f}, hi[4] = {-1e9f, -1e9f, -1e9f, -1e9f};
+                for (int j = 0; j < 4; ++j) {
+                    int yy = clampi(yi - 1 + j, 0, mh - 1);
+                    float row[4] = {0.f, 0.f, 0.f, 0.f};
+                    for (int i = 0; i < 4; ++i) {
+                        int xx = clampi(xi - 1 + i, 0, mw - 1);
+                        const float* s = hdr + ((size_t)yy * mw + xx) * 4;
+                        for (int k = 0; k < 4; ++k) {
+                            row[k] = row[k] + s[k] * wx[i];
+                            if ((i == 1 || i == 2) && (j == 1 || j == 2)) {
+                                lo[k] = fminf(lo[k], s[k]);
+                                hi[k] = fmaxf(hi[k], s[k]);
+                            }
+                        }
+                    }
+                    for (int k = 0; k < 4; ++k) acc[k] = acc[k] + row[k] * wy[j];
+                }
+                for (int k = 0; k < 4; ++k) px[k] = clampf_(acc[k], lo[k], hi[k]);
+            }
+            float vy = 1.f - v;
+            float vig = 1.f - 0.6f * ((u - 0.5f) * (u - 0.5f) + (vy - 0.6f) * (vy - 0.6f));
+            v3 bg = vmul(vadd(vmul(p.bg_bot, 1.f - vy), vmul(p.bg_top, vy)), vig);
+            /* bilinear bloom */
+            float fx = u * (float)bw - 0.5f, fy = v * (float)bh - 0.5f;
+            float x0f = floorf(fx), y0f = floorf(fy), tx = fx - x0f, ty = fy - y0f;
+            int x0 = clampi((int)x0f, 0, bw - 1), y0 = clampi((int)y0f, 0, bh - 1);
+            int x1 = clampi((int)x0f + 1, 0, bw - 1), y1 = clampi((int)y0f + 1, 0, bh - 1);
+            float bl[3];
+            for (int k = 0; k < 3; ++k) {
+                float a0 = bloom[((size_t)y0 * bw + x0) * 3 + k] * (1.f - tx) + bloom[((size_t)y0 * bw + x1) * 3 + k] * tx;
+                float a1 = bloom[((size_t)y1 * bw + x0) * 3 + k] * (1.f - tx) + bloom[((size_t)y1 * bw + x1) * 3 + k] * tx;
+                bl[k] = a0 * (1.f - ty) + a1 * ty;
+            }
+            v3 col = vadd(vadd(V(px[0], px[1], px[2]), vmul(bg, px[3])), vmul(V(bl[0], bl[1], bl[2]), p.bloom));
+            col = vmul(col, p.exposure);
+            if (p.contrast != 1.f)
+                col = V(0.18f * powf(fmaxf(col.x, 0.f) / 0.18f, p.contrast), 0.18f * powf(fmaxf(col.y, 0.f) / 0.18f, p.contrast),
+                        0.18f * powf(fmaxf(col.z, 0.f) / 0.18f, p.contrast));
+            float hh = sinf((float)r * 12.9898f + (float)c * 78.233f) * 43758.5453f;
+            float dither = (hh - floorf(hh)) - 0.5f;
+            uint8_t* o = out + ((size_t)r * Wo + c) * 3;
+            o[0] = to_srgb8(aces(col.x), dither);
+            o[1] = to_srgb8(aces(col.y), dither);
+            o[2] = to_srgb8(aces(col.z), dither);
+        }
+}

@@ -1,0 +1,162 @@
+"""CPU restatement of the reference's volume renderer (vfx/render3d/renderer.py VolumeRenderer.draw,
+default path) around the C kernels of oracle/render_oracle.c.
+
+TEST INFRASTRUCTURE ONLY: the oracle of the HIP renderer (neural-vfx_amd/csrc/render.hip).
+
+The reference's Warp kernels (pack, light volume, march, composite) are restated in C; its torch
+steps (the solid and glow blurs, the glow pyramid, the bloom levels, renderer.py:227-229, :408-416,
+:452-457) run here as the same torch calls on the CPU.  Stored grids are rounded to FP16 where the
+reference stores FP16 (packed volume, light volume, glow source, glow volume).
+
+Parity status (tests/test_oracle_render.py): empty-brick skipping against the unskipped march; a
+uniform non-scattering slab's transmittance against Beer-Lambert; an optically thin hot slab's
+emission against the blackbody table times the path length; an opaque sphere's silhouette against
+the ray-sphere intersection; the light volume's transmittance and sky occlusion of a uniform medium
+against their closed forms; the background of an empty frame against its gradient.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+import oracle as _sim_oracle
+
+BRICKS = 32
+
+
+def _lib():
+    return _sim_oracle._load()
+
+
+def _p(x):
+    return x.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(x):
+    return np.ascontiguousarray(x, dtype=np.float32)
+
+
+def _h(x):
+    """round to FP16 and back (the reference's half storage)"""
+    return _f32(np.asarray(x, np.float32).astype(np.float16).astype(np.float32))
+
+
+def gauss1d(sigma: float) -> torch.Tensor:
+    r = max(1, int(math.ceil(2.5 * sigma)))
+    t = torch.arange(-r, r + 1, dtype=torch.float32)
+    k = torch.exp(-0.5 * (t / sigma) ** 2)
+    return k / k.sum()
+
+
+def blur3(x: torch.Tensor, k: torch.Tensor) -> torch.Tensor:
+    """renderer.py:_blur3: separable replicate-padded blur of (1, C, D, H, W), z then y then x."""
+    C, r = x.shape[1], k.numel() // 2
+    for dim in range(3):
+        shp = [1, 1, 1]
+        shp[dim] = -1
+        kk = k.view(1, 1, *shp).expand(C, 1, *shp)
+        pad = [0] * 6
+        pad[2 * (2 - dim)] = pad[2 * (2 - dim) + 1] = r
+        x = F.conv3d(F.pad(x, pad, mode="replicate"), kk, groups=C)
+    return x
+
+
+def blur2(x: torch.Tensor, k: torch.Tensor) -> torch.Tensor:
+    C, r = x.shape[1], k.numel() // 2
+    kx = k.view(1, 1, 1, -1).expand(C, 1, 1, -1)
+    ky = k.view(1, 1, -1, 1).expand(C, 1, -1, 1)
+    x = F.conv2d(F.pad(x, (r, r, 0, 0), mode="replicate"), kx, groups=C)
+    return F.conv2d(F.pad(x, (0, 0, r, r), mode="replicate"), ky, groups=C)
+
+
+def occupancy(a, b, mode, d_eps=0.0, t_hot=0.0):
+    a, b = _f32(a), _f32(b)
+    D, H, W = a.shape
+    occ = np.zeros(BRICKS ** 3, np.uint8)
+    _lib().or_render_occ(_p(a), _p(b), D, H, W, int(mode), ctypes.c_float(d_eps), ctypes.c_float(t_hot), _p(occ))
+    return occ
+
+
+def prep(dens, temp, solid, prm, lut):
+    """Volume prep: packed (D, H, W, 2), occupancy, solid (values / 255) and its occupancy, light
+    (L^3, 2), glow (32^3, 3)."""
+    prm = _f32(prm)
+    lut = _f32(lut)
+    L = int(prm[1])
+    d_eps, t_lo = float(prm[23]), float(prm[20])
+    vol = _h(np.stack([np.maximum(dens, 0.0), temp], -1))
+    occ = occupancy(dens, temp, 0, d_eps, t_lo)
+    if solid is not None:
+        s = blur3(torch.from_numpy(_f32(solid))[None, None], gauss1d(1.0))[0, 0].numpy()
+        su8 = np.clip((s * 255.0 + 0.5).astype(np.int64), 0, 255).astype(np.float32) / 255.0
+        su8 = _f32(su8)
+        occ_s = occupancy(s, s, 1)
+    else:
+        su8 = np.zeros((2, 2, 2), np.float32)
+        occ_s = np.zeros(BRICKS ** 3, np.uint8)
+    D, H, W = dens.shape
+    sD, sH, sW = su8.shape
+    light = np.empty((L ** 3, 2), np.float32)
+    gsrc = np.empty((L ** 3, 3), np.float32)
+    _lib().or_render_light(_p(vol), D, H, W, _p(su8), sD, sH, sW, int(solid is not None), _p(lut), _p(prm),
+                           _p(light), _p(gsrc))
+    light = _h(light)
+    gsrc = _h(gsrc)
+    # glow pyramid (renderer.py:408-416)
+    g = torch.from_numpy(gsrc.reshape(L, L, L, 3)).permute(3, 0, 1, 2)[None]
+    g32 = F.adaptive_avg_pool3d(g, 32)
+    g3 = gauss1d(1.2)
+    g32b = blur3(g32, g3)
+    g8 = blur3(F.avg_pool3d(g32, 4), g3)
+    glow = g32b + float(prm[51]) * F.interpolate(g8, size=(32, 32, 32), mode="trilinear", align_corners=False)
+    glow = _h(glow[0].permute(1, 2, 3, 0).numpy())
+    return dict(vol=vol, occ=occ, solid=su8, occ_s=occ_s, has_solid=solid is not None, light=light, glow=glow,
+                solid_res=float(max(sD, sH, sW)) if solid is not None else 1.0)
+
+
+def march(pr, prm, lut, cam, mh, mw, skip=True):
+    prm, lut, cam = _f32(prm), _f32(lut), _f32(cam)
+    D, H, W, _ = pr["vol"].shape
+    sD, sH, sW = pr["solid"].shape
+    out = np.empty((mh, mw, 4), np.float32)
+    _lib().or_render_march(_p(pr["vol"]), D, H, W, _p(pr["light"]), _p(pr["glow"]), _p(pr["solid"]), sD, sH, sW,
+                           _p(pr["occ"]), _p(pr["occ_s"]), int(pr["has_solid"]), ctypes.c_float(pr["solid_res"]),
+                           _p(lut), _p(prm), _p(cam), mh, mw, int(skip), _p(out))
+    return out
+
+
+def bloom(hdr, thresh):
+    """renderer.py:452-457: (bh, bw, 3)"""
+    mh, mw = hdr.shape[:2]
+    bh, bw = max(2, mh // 4), max(2, mw // 4)
+    x = torch.from_numpy(_f32(hdr[..., :3])).permute(2, 0, 1)[None]
+    x = F.relu(x - thresh)
+    g2 = gauss1d(2.0)
+    b1 = blur2(F.adaptive_avg_pool2d(x, (bh, bw)), g2)
+    b2 = blur2(F.avg_pool2d(b1, 4, ceil_mode=True), g2)
+    b = b1 + 1.5 * F.interpolate(b2, size=b1.shape[-2:], mode="bilinear", align_corners=False)
+    return _f32(b[0].permute(1, 2, 0).numpy())
+
+
+def composite(hdr, bl, prm, out_h, out_w):
+    prm = _f32(prm)
+    hdr, bl = _f32(hdr), _f32(bl)
+    out = np.empty((out_h, out_w, 3), np.uint8)
+    _lib().or_render_composite(_p(hdr), hdr.shape[0], hdr.shape[1], _p(bl), bl.shape[0], bl.shape[1], _p(prm),
+                               out_h, out_w, _p(out))
+    return out
+
+
+def render(dens, temp, solid, cam, prm, lut, out_h, out_w, march_scale=1.0):
+    """The whole draw: (uint8 image (out_h, out_w, 3), hdr (mh, mw, 4))."""
+    mh = max(8, int(round(out_h * march_scale)))
+    mw = max(8, int(round(out_w * march_scale)))
+    pr = prep(np.asarray(dens, np.float32), np.asarray(temp, np.float32), solid, prm, lut)
+    hdr = march(pr, prm, lut, cam, mh, mw)
+    img = composite(hdr, bloom(hdr, float(_f32(prm)[19])), prm, out_h, out_w)
+    return img, hdr

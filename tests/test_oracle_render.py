@@ -1,0 +1,183 @@
+"""Pins of the renderer oracle (oracle/render_ref.py + render_oracle.c; reference vfx/render3d) by
+closed forms of the physics and the geometry:
+
+  * empty-brick skipping changes nothing beyond single-sample threshold flips (against the march
+    that samples every step);
+  * a uniform non-scattering medium attenuates a ray by Beer-Lambert, exp(-k_abs d L) over the
+    ray's chord L through the unit cube (to one step);
+  * an optically thin hot medium emits the blackbody table's colour times the chord length;
+  * an opaque sphere hides exactly the rays that pass within its radius (to the smoothing width);
+  * the light volume of a uniform medium lit from straight above holds exp(-k_shadow d (1 - y))
+    and the sky occlusion exp(-k_amb d * the in-cube length of its 6 upward steps);
+  * the blackbody table: the fitted colour-matching functions against the CIE 1931 tabulated values,
+    warmer temperatures bluer and brighter.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import render_ref as R
+from simhelp import smooth_noise
+from vfx.core.fields3d import Camera
+from vfx.render.blackbody import blackbody_lut, cie_xyz_bar
+from vfx.render3d.renderer import RenderParams3D, camera_basis
+
+CAM = Camera(position=(0.5, 0.5, -1.6), target=(0.5, 0.5, 0.5), fov_y_deg=30.0)
+
+
+def _rays(cam16, mh, mw):
+    """ray origins / directions of the march's pixel grid, float64"""
+    o = cam16[0:3].astype(np.float64)
+    fw, rt, up = cam16[3:6].astype(np.float64), cam16[6:9].astype(np.float64), cam16[9:12].astype(np.float64)
+    th, asp = float(cam16[12]), float(cam16[13])
+    c = (np.arange(mw) + 0.5) / mw
+    r = (np.arange(mh) + 0.5) / mh
+    sx = (2.0 * c - 1.0) * th * asp
+    sy = (1.0 - 2.0 * r) * th
+    d = fw[None, None] + rt[None, None] * sx[None, :, None] + up[None, None] * sy[:, None, None]
+    return o, d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def _chord(o, d):
+    inv = 1.0 / np.where(np.abs(d) < 1e-12, 1e-12, d)
+    t0 = (0.0 - o) * inv
+    t1 = (1.0 - o) * inv
+    tn = np.max(np.minimum(t0, t1), axis=-1)
+    tf = np.min(np.maximum(t0, t1), axis=-1)
+    return np.maximum(tf - np.maximum(tn, 0.0), 0.0)
+
+
+def _params(**kw):
+    p = RenderParams3D(light_res=16, **kw)
+    return p, p.pack(), blackbody_lut(p.t_lo, p.t_hi, 256, t_ref=p.t_ref)
+
+
+def test_brick_skipping_changes_nothing():
+    N = 32
+    p, prm, lut = _params()
+    f = smooth_noise(N, 2, 7, amp=1.0)
+    dens = np.clip(f[0], 0.0, None) * np.linspace(0, 1, N)[:, None, None] ** 4
+    temp = 293.0 + 1500.0 * np.clip(f[1], 0.0, None) * (dens > 0.05)
+    solid = np.zeros((N, N, N), np.float32)
+    solid[20:26, 8:14, 10:16] = 1.0
+    pr = R.prep(dens.astype(np.float32), temp.astype(np.float32), solid, prm, lut)
+    cam = camera_basis(CAM, 1.25)
+    a = R.march(pr, prm, lut, cam, 40, 50, skip=True)
+    b = R.march(pr, prm, lut, cam, 40, 50, skip=False)
+    assert (pr["occ"] == 0).mean() > 0.3        # some bricks are empty and skipped
+    # after a skip the samples sit on the exact lattice t0 + k ds, while the unskipped march adds
+    # ds step by step: positions differ by rounding, which can move one sample across a threshold
+    # (d_eps, the early-termination limit) on a few rays
+    e = np.abs(a - b).max(-1) / np.abs(b).max()
+    assert (e > 1e-4).mean() < 0.005 and e.max() < 0.01
+
+
+@pytest.mark.parametrize("d0", [0.01, 0.05])
+def test_beer_lambert_transmittance(d0):
+    N = 32      # volumes below the 32^3 brick grid leave the last brick of each axis unmarked (as the reference)
+    p, prm, lut = _params(smoke_albedo=(0.0, 0.0, 0.0), floor=0.0)
+    dens = np.full((N, N, N), d0, np.float32)
+    temp = np.full((N, N, N), 293.0, np.float32)
+    pr = R.prep(dens, temp, None, prm, lut)
+    cam = camera_basis(CAM, 1.0)
+    mh = mw = 24
+    hdr = R.march(pr, prm, lut, cam, mh, mw)
+    o, d = _rays(cam, mh, mw)
+    L = _chord(o, d)
+    dd = float(np.float16(d0))
+    sigma = p.k_abs * dd
+    ds = 1.0 / p.steps
+    hit = L > 2 * ds
+    assert hit.sum() > 100
+    lo, hi = np.exp(-sigma * (L + ds)), np.exp(-sigma * np.maximum(L - ds, 0.0))
+    T = hdr[..., 3]
+    assert np.all(T[hit] >= lo[hit] * (1 - 1e-4)) and np.all(T[hit] <= hi[hit] * (1 + 1e-4))
+    assert np.all(T[~hit] >= lo[~hit] * (1 - 1e-4))
+    assert np.abs(hdr[..., :3]).max() == 0.0
+
+
+def test_thin_hot_medium_emits_blackbody_times_chord():
+    N = 32
+    p, prm, lut = _params(k_abs=0.0, floor=0.0)
+    Tk = 1500.0
+    dens = np.full((N, N, N), 0.0, np.float32)
+    temp = np.full((N, N, N), Tk, np.float32)
+    pr = R.prep(dens, temp, None, prm, lut)
+    cam = camera_basis(CAM, 1.0)
+    hdr = R.march(pr, prm, lut, cam, 20, 20)
+    o, d = _rays(cam, 20, 20)
+    L = _chord(o, d)
+    f = (float(np.float16(Tk)) - p.t_lo) / (p.t_hi - p.t_lo) * 255.0
+    i0 = int(f)
+    col = lut[i0] * (1 - (f - i0)) + lut[i0 + 1] * (f - i0)
+    e = col * (p.emit_min * p.emit_scale)          # density 0: emissivity emit_min
+    ds = 1.0 / p.steps
+    hit = L > 4 * ds
+    for c in range(3):
+        got = hdr[..., c][hit]
+        assert np.all(np.abs(got - e[c] * L[hit]) <= e[c] * ds * 1.01 + 1e-5)
+    assert np.all(hdr[..., 3] == 1.0)
+
+
+def test_opaque_sphere_silhouette():
+    N = 48
+    p, prm, lut = _params(floor=0.0)
+    c0, rad = np.array([0.5, 0.5, 0.5]), 0.2
+    g = (np.arange(N) + 0.5) / N
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    solid = (((xx - c0[0]) ** 2 + (yy - c0[1]) ** 2 + (zz - c0[2]) ** 2) <= rad ** 2).astype(np.float32)
+    z = np.zeros((N, N, N), np.float32)
+    pr = R.prep(z, z + 293.0, solid, prm, lut)
+    cam = camera_basis(CAM, 1.0)
+    hdr = R.march(pr, prm, lut, cam, 32, 32)
+    o, d = _rays(cam, 32, 32)
+    v = c0 - o
+    dist = np.linalg.norm(v[None, None] - (d @ v)[..., None] * d, axis=-1)
+    tol = 2.0 / N                      # the binary mask is smoothed by a Gaussian of one voxel
+    inside, outside = dist < rad - tol, dist > rad + tol
+    assert inside.sum() > 50 and outside.sum() > 50
+    assert np.all(hdr[..., 3][inside] == 0.0)
+    assert np.all(hdr[..., 3][outside] == 1.0)
+    assert np.all(hdr[..., :3][inside].sum(-1) > 0)      # lit obstacle colour
+
+
+def test_light_volume_uniform_medium_from_above():
+    N, d0 = 16, 0.02
+    p, prm, lut = _params(light_dir=(0.0, 1.0, 0.0))
+    dens = np.full((N, N, N), d0, np.float32)
+    temp = np.full((N, N, N), 293.0, np.float32)
+    pr = R.prep(dens, temp, None, prm, lut)
+    L = p.light_res
+    light = pr["light"].reshape(L, L, L, 2)
+    dd = float(np.float16(d0))
+    y = (np.arange(L) + 0.5) / L
+    ds = 1.5 / L
+    lo = np.exp(-p.k_shadow * dd * (1.0 - y + ds))
+    hi = np.exp(-p.k_shadow * dd * np.maximum(1.0 - y - ds, 0.0))
+    tr = light[..., 0]
+    for j in range(L):
+        assert np.all(tr[:, j] >= lo[j] * (1 - 2e-3)) and np.all(tr[:, j] <= hi[j] * (1 + 2e-3)), j
+    # sky occlusion: 6 steps of 3 / L straight up, the ones below y = 1 count
+    da = 3.0 / L
+    for j in range(L):
+        n_in = sum(1 for k in range(6) if y[j] + (k + 0.5) * da < 1.0)
+        want = math.exp(-p.k_amb * dd * da * n_in)
+        assert np.allclose(light[:, j, :, 1], want, rtol=2e-3), j
+
+
+def test_blackbody_table():
+    # the fitted colour-matching functions against CIE 1931 2-degree tabulated values
+    cie = {450.0: (0.3362, 0.0380, 1.7721), 555.0: (0.5121, 1.0000, 0.0057), 600.0: (1.0622, 0.6310, 0.0008)}
+    for lam, want in cie.items():
+        got = cie_xyz_bar(np.array([lam]))[0]
+        assert np.all(np.abs(got - np.array(want)) <= 0.08 * max(want) + 0.01), (lam, got, want)
+    lut = blackbody_lut(700.0, 3000.0, 256, t_ref=1400.0)
+    assert np.all(lut[0] == 0)                       # the onset fade starts at 0
+    ratio_b = lut[1:, 2] / np.maximum(lut[1:, 0], 1e-9)
+    assert np.all(np.diff(ratio_b[20:]) >= -1e-6)    # bluer with temperature
+    lum = lut @ np.array([0.2126, 0.7152, 0.0722])
+    assert np.all(np.diff(lum) >= 0)                 # brighter with temperature
+    assert np.allclose(lut.max(axis=1)[1:], np.max(lut[1:], axis=1))
