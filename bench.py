@@ -362,8 +362,9 @@ def pair_report(steps: int, warmup: int, lr_res: int = 64, hr_res: int = 256):
 def render_report(reps: int):
     """The volume renderer (vfx/render3d VolumeRenderer) at 2560 x 1440 on the solver's own 256^3
     fire state (and 64^3, 128^3 states of the same scene), as the reference's feasibility table
-    (render3d README: full draw, view-only redraw with reuse_volume, prep / view split).  Times are
-    device events around each draw, median of `reps` draws after 3 warm-up draws."""
+    (render3d README: full draw, view-only redraw with reuse_volume, prep / view split), and the
+    full draw with return_aux (alpha, depth, motion from the state's velocity and a moved camera).
+    Times are device events around each draw, median of `reps` draws after 3 warm-up draws."""
     import torch
     from vfx.core.fields3d import Camera, Volume
     from vfx.render3d import RenderParams3D, VolumeRenderer
@@ -371,6 +372,7 @@ def render_report(reps: int):
 
     rows = []
     cam = Camera(position=(0.5, 0.55, -1.4), target=(0.5, 0.45, 0.5))
+    prev = Camera(position=(0.49, 0.55, -1.4), target=(0.5, 0.45, 0.5))
     for N in (64, 128, 256):
         s = Solver3D(N, fire_config(0))
         st = s.reset()
@@ -381,7 +383,7 @@ def render_report(reps: int):
         for scale, steps in ((0.5, 192.0), (1.0, 64.0), (1.0, 192.0)):
             r = VolumeRenderer((1440, 2560), device="cuda:0", march_scale=scale,
                                params=RenderParams3D(steps=steps), profile=True)
-            full, prep, view, reuse = [], [], [], []
+            full, prep, view, reuse, aux = [], [], [], [], []
             for k in range(3 + reps):
                 r.draw(vol, cam)
                 if k >= 3:
@@ -392,9 +394,13 @@ def render_report(reps: int):
                 r.draw(vol, cam, reuse_volume=True)
                 if k >= 3:
                     reuse.append(r.last_timings()["total"])
+                r.draw(vol, cam, return_aux=True, velocity=st.velocity, prev_camera=prev)
+                if k >= 3:
+                    aux.append(r.last_timings()["total"])
             med = lambda v: round(statistics.median(v), 3)
             rows.append({"volume": N, "march_scale": scale, "steps_per_m": steps, "march_px": f"{r.mw}x{r.mh}",
-                         "full_ms": med(full), "prep_ms": med(prep), "view_ms": med(view), "reuse_ms": med(reuse)})
+                         "full_ms": med(full), "prep_ms": med(prep), "view_ms": med(view), "reuse_ms": med(reuse),
+                         "aux_full_ms": med(aux)})
             del r
         del s, st
         torch.cuda.empty_cache()

@@ -182,7 +182,7 @@ int vfx_tracking_error(const float* hr_data, int hr_res, const float* lr_data, i
                        float* out3, void* stream);
 
 /* ---------------------------------------------------------------- volume renderer
- * vfx/render3d/renderer.py VolumeRenderer (default path: colour only, no temporal resolve, no detail
+ * vfx/render3d/renderer.py VolumeRenderer (default path and return_aux; no temporal resolve, no detail
  * field).  FP32 only.  params: VFX_RENDER_PARAMS floats, RenderParams3D's fields in declaration order
  * (renderer.py:31-66) with the 3-tuples flattened: steps, light_res, k_abs, k_shadow, k_solid, k_amb,
  * k_emit, emit_min, emit_scale, soot, fire_light, phase_g, rim, ms_a, ms_b, ms_c, exposure, contrast,
@@ -209,6 +209,17 @@ int vfx_render_set_params(vfx_render* r, const float* params, const float* lut);
 int vfx_render_draw(vfx_render* r, const float* dens, const float* temp, int D, int H, int W, const float* solid,
                     int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
                     void* stream, float* ms_prep, float* ms_view);
+/* VolumeRenderer.draw(return_aux=True) (renderer.py:295-393; kernels.py:288-489 march_aux, :603-626
+ * aux_resolve): the same frame as vfx_render_draw, bit for bit, plus aux_out = (4, out_h, out_w) half
+ * device: alpha (1 - the medium's transmittance), opacity-weighted depth along the ray, and motion
+ * (x, y) in output pixels (current minus previous projection of the opacity-weighted mean point moved
+ * back by vel * vel_dt); depth and motion are 0 where alpha < 1e-3.  vel: (3, uD, uH, uW) float32
+ * device in m/s, or NULL for no medium motion (upload_vel != 0 when it changed since the last aux
+ * draw); cam_prev: 16 host floats, the previous sim step's camera (pass cam when there is none). */
+int vfx_render_draw_aux(vfx_render* r, const float* dens, const float* temp, int D, int H, int W, const float* solid,
+                        int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
+                        const float* vel, int uD, int uH, int uW, int upload_vel, const float* cam_prev, float vel_dt,
+                        uint16_t* aux_out, void* stream, float* ms_prep, float* ms_view);
 /* The march buffer of the last draw, (mh, mw, 4) float: premultiplied radiance, transmittance. */
 int vfx_render_get_hdr(vfx_render* r, float* dst, int* mh, int* mw, void* stream);
 

@@ -175,15 +175,15 @@ __device__ __forceinline__ float tex_u8(const uint8_t* __restrict__ v, const Tex
     return tri_(g, q, [&](int z, int y, int x) { return (float)v[((size_t)z * g.H + y) * g.W + x] * (1.f / 255.f); });
 }
 
-// rgb of a (n^3, 4) half grid: one 8-byte load per texel, the three channels interpolated alike
-__device__ __forceinline__ float3 tex_h4(const __half* __restrict__ v, int n, float3 q) {
-    const Tw wx = tw(q.x, n), wy = tw(q.y, n), wz = tw(q.z, n);
+// rgb of a (D, H, W, 4) half grid: one 8-byte load per texel, the three channels interpolated alike
+__device__ __forceinline__ float3 tex_h4(const __half* __restrict__ v, const Tex3& g, float3 q) {
+    const Tw wx = tw(q.x, g.W), wy = tw(q.y, g.H), wz = tw(q.z, g.D);
     const uint2* p = reinterpret_cast<const uint2*>(v);
     float c[8][3];
     const int zs[2] = {wz.i0, wz.i1}, ys[2] = {wy.i0, wy.i1}, xs[2] = {wx.i0, wx.i1};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const uint2 u = p[((size_t)zs[k >> 2] * n + ys[(k >> 1) & 1]) * n + xs[k & 1]];
+        const uint2 u = p[((size_t)zs[k >> 2] * g.H + ys[(k >> 1) & 1]) * g.W + xs[k & 1]];
         const float2 rg = __half22float2(*reinterpret_cast<const __half2*>(&u.x));
         const float2 ba = __half22float2(*reinterpret_cast<const __half2*>(&u.y));
         c[k][0] = rg.x;
@@ -396,7 +396,7 @@ __device__ __forceinline__ float2 light_at(const __half2* __restrict__ light, co
 // glow lookup, fading with the squared distance outside the cube (kernels.py:270-275)
 __device__ __forceinline__ float3 glow_at(const __half* __restrict__ glow, float3 q) {
     const float3 c = f3(clampf(q.x, 0.f, 1.f), clampf(q.y, 0.f, 1.f), clampf(q.z, 0.f, 1.f));
-    const float3 g = tex_h4(glow, GLOW, c);
+    const float3 g = tex_h4(glow, Tex3{GLOW, GLOW, GLOW}, c);
     const float dout = len3(sub(q, c));
     return mul(g, 1.f / (1.f + 60.f * dout * dout));
 }
@@ -416,8 +416,32 @@ struct MarchArgs {
     const float* lut;
 };
 
-// march (kernels.py:288-489, the colour-only variant)
-__global__ void __launch_bounds__(256) k_march(MarchArgs a, Shade p, Cam cam, int h, int w, float4* __restrict__ out) {
+// aux inputs of the march (return_aux): velocity grid (D, H, W, 4) half in m/s, the previous
+// camera, the sim step, the output size, and the premultiplied aux buffer (mh, mw)
+struct AuxArgs {
+    const __half* vel;
+    Tex3 gvel;
+    int has_vel;
+    Cam cam_prev;
+    float vel_dt, out_w, out_h;
+    float4* aux;
+};
+
+// world point -> continuous output-pixel coordinates (x right, y down) of a camera (kernels.py:278-285)
+__device__ __forceinline__ float2 project_px(float3 q, const Cam& cam, float out_w, float out_h) {
+    const float3 rel = sub(q, f3(cam.v[0], cam.v[1], cam.v[2]));
+    const float z = fmaxf(dot3(rel, f3(cam.v[3], cam.v[4], cam.v[5])), 1.0e-4f);
+    const float sx = dot3(rel, f3(cam.v[6], cam.v[7], cam.v[8])) / (z * cam.v[12] * cam.v[13]);
+    const float sy = dot3(rel, f3(cam.v[9], cam.v[10], cam.v[11])) / (z * cam.v[12]);
+    return make_float2((sx + 1.f) * 0.5f * out_w, (1.f - sy) * 0.5f * out_h);
+}
+
+// march (kernels.py:288-489).  AUX also accumulates, from the same samples, the medium-only
+// transmittance and the opacity-weighted (T a) mean distance and velocity, and writes the
+// premultiplied (alpha, alpha depth, alpha motion_x, alpha motion_y); the colour code is the same.
+template <bool AUX>
+__global__ void __launch_bounds__(256) k_march(MarchArgs a, Shade p, Cam cam, int h, int w, float4* __restrict__ out,
+                                               AuxArgs x) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y * blockDim.y + threadIdx.y;
     if (c >= w || r >= h) return;
@@ -441,6 +465,8 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs a, Shade p, Cam cam, in
     const float ph1 = hg(cos_t, p.phase_g * p.ms_c) * p.rim + hg(cos_t, -0.25f * p.ms_c) * (1.f - p.rim);
     const float ph2 = hg(cos_t, p.phase_g * p.ms_c * p.ms_c) * p.rim + (1.f - p.rim);
     const float a1 = p.ms_a, a2 = p.ms_a * p.ms_a, b1 = p.ms_b, b2 = p.ms_b * p.ms_b;
+    float Tm = 1.f, aw = 0.f, ad = 0.f;   // AUX: medium transmittance, sum of weights, sum w t
+    float3 av = f3(0.f, 0.f, 0.f);        // AUX: sum w v
     if (t_near < t_far) {
         const float jit = fracf(ign((float)c, (float)r) + cam.v[14]);
         const float t0 = t_near + jit * ds;
@@ -513,12 +539,25 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs a, Shade p, Cam cam, in
                         col = mul(cw(alb, lit), al);
                     }
                     if (hot) col = add(col, mul(emission(a.lut, p, dn, Tk), ds));
+                    if (AUX) {
+                        const float wgt = T * al;
+                        if (wgt > 0.f) {
+                            aw += wgt;
+                            ad += wgt * t;
+                            if (x.has_vel != 0) {
+                                const float3 v = tex_h4(x.vel, x.gvel, q);
+                                av = add(av, mul(v, wgt));
+                            }
+                        }
+                    }
                     rad = add(rad, mul(col, T));
                     T = T * (1.f - al);
                     if (T < 0.004f) {
                         T = 0.f;
+                        if (AUX) Tm = 0.f;
                         break;
                     }
+                    if (AUX) Tm = T;
                 }
             }
             t_prev = t;
@@ -551,6 +590,69 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs a, Shade p, Cam cam, in
         }
     }
     out[(size_t)r * w + c] = make_float4(rad.x, rad.y, rad.z, T);
+    if (AUX) {
+        const float alpha = 1.f - Tm;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (aw > 1.0e-6f && alpha > 0.f) {
+            // the opacity-weighted mean point and velocity; that point was at qm - v dt one sim step
+            // ago, seen through the previous camera; the current projection is the pixel centre
+            const float dep = ad / aw;
+            const float3 qm = add(o, mul(d, dep));
+            const float3 qp = sub(qm, mul(av, x.vel_dt / aw));
+            const float2 cur = make_float2(((float)c + 0.5f) * x.out_w / (float)w, ((float)r + 0.5f) * x.out_h / (float)h);
+            const float2 pp = project_px(qp, x.cam_prev, x.out_w, x.out_h);
+            v = make_float4(alpha, dep * alpha, (cur.x - pp.x) * alpha, (cur.y - pp.y) * alpha);
+        }
+        x.aux[(size_t)r * w + c] = v;
+    }
+}
+
+// (D, H, W) velocity components (3, D, H, W) float -> (D, H, W, 4) half, 4th channel 0
+__global__ void __launch_bounds__(256) k_pack_velocity(const float* __restrict__ vel, size_t n, __half* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[4 * i + 0] = __float2half_rn(vel[i]);
+    out[4 * i + 1] = __float2half_rn(vel[n + i]);
+    out[4 * i + 2] = __float2half_rn(vel[2 * n + i]);
+    out[4 * i + 3] = __float2half_rn(0.f);
+}
+
+// aux_resolve (kernels.py:603-626): march-resolution premultiplied aux -> (4, Ho, Wo) half alpha,
+// depth, motion_x, motion_y (bilinear upsample, then un-premultiplied; 0 where alpha < a_min)
+__global__ void __launch_bounds__(256) k_aux_resolve(const float4* __restrict__ aux, int mh, int mw, int Ho, int Wo,
+                                                     float a_min, __half* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y * blockDim.y + threadIdx.y;
+    if (c >= Wo || r >= Ho) return;
+    float4 v;
+    if (mh == Ho && mw == Wo) {
+        v = aux[(size_t)r * mw + c];
+    } else {
+        const float u = ((float)c + 0.5f) / (float)Wo, vv = ((float)r + 0.5f) / (float)Ho;
+        const float fx = u * (float)mw - 0.5f, fy = vv * (float)mh - 0.5f;
+        const float x0f = floorf(fx), y0f = floorf(fy), tx = fx - x0f, ty = fy - y0f;
+        const int x0 = min(max((int)x0f, 0), mw - 1), y0 = min(max((int)y0f, 0), mh - 1);
+        const int x1 = min(max((int)x0f + 1, 0), mw - 1), y1 = min(max((int)y0f + 1, 0), mh - 1);
+        const float4 p00 = aux[(size_t)y0 * mw + x0], p01 = aux[(size_t)y0 * mw + x1];
+        const float4 p10 = aux[(size_t)y1 * mw + x0], p11 = aux[(size_t)y1 * mw + x1];
+        auto l = [&](float a0, float a1, float b0, float b1) {
+            return (a0 * (1.f - tx) + a1 * tx) * (1.f - ty) + (b0 * (1.f - tx) + b1 * tx) * ty;
+        };
+        v = make_float4(l(p00.x, p01.x, p10.x, p11.x), l(p00.y, p01.y, p10.y, p11.y), l(p00.z, p01.z, p10.z, p11.z),
+                        l(p00.w, p01.w, p10.w, p11.w));
+    }
+    const float al = v.x;
+    float dep = 0.f, mx = 0.f, my = 0.f;
+    if (al >= a_min) {
+        dep = v.y / al;
+        mx = v.z / al;
+        my = v.w / al;
+    }
+    const size_t n = (size_t)Ho * Wo, i = (size_t)r * Wo + c;
+    out[i] = __float2half_rn(al);
+    out[n + i] = __float2half_rn(dep);
+    out[2 * n + i] = __float2half_rn(mx);
+    out[3 * n + i] = __float2half_rn(my);
 }
 
 // ---------------------------------------------------------------- bloom and composite
@@ -737,6 +839,11 @@ struct vfx_render {
     float4* hdr = nullptr;
     float* bwork = nullptr;      // 2 (3 bh bw) + 2 (3 ph pw) floats
     float* bloom = nullptr;      // bh bw 3
+    // aux (return_aux): velocity grid and the march-resolution premultiplied aux buffer
+    int uD = 0, uH = 0, uW = 0;
+    __half* vel = nullptr;       // uD uH uW 4
+    float4* aux = nullptr;       // aux_m float4
+    size_t aux_m = 0;
 };
 
 static bool ralloc(void** p, size_t bytes) {
@@ -797,7 +904,7 @@ static int alloc_output(vfx_render* r, int out_h, int out_w, float scale) {
 static void free_all(vfx_render* r) {
     hipDeviceSynchronize();
     void* ps[] = {r->lut, r->packed, r->occ, r->sblur, r->solid, r->light, r->glow_src, r->gwork, r->glow, r->hdr,
-                  r->bwork, r->bloom};
+                  r->bwork, r->bloom, r->vel, r->aux};
     for (void* p : ps)
         if (p) hipFree(p);
 }
@@ -856,22 +963,64 @@ int vfx_render_set_params(vfx_render* r, const float* params, const float* lut) 
     return VFX_OK;
 }
 
-// Volume.draw (renderer.py:281-386), colour only.  dens / temp: (D, H, W) float32 device; solid:
-// (sD, sH, sW) float32 device or NULL; upload_solid: the solid changed since the last draw (the
-// caller keys it like the reference, renderer.py:224-226); cam: 16 host floats (camera_basis);
-// reuse_volume: skip the volume prep and re-render the last prepared volume; out: (out_h, out_w, 3)
-// uint8 device.  ms_prep / ms_view (optional, host): device times of the two parts (synchronises).
-int vfx_render_draw(vfx_render* r, const float* dens, const float* temp, int D, int H, int W, const float* solid,
-                    int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
-                    void* stream, float* ms_prep, float* ms_view) {
+}  // extern "C"
+
+// the aux request of one draw (return_aux): velocity (3, uD, uH, uW) float device or NULL, re-packed
+// when upload_vel; the previous camera (16 host floats); the sim step; the (4, out_h, out_w) half output
+struct AuxReq {
+    const float* vel;
+    int uD, uH, uW, upload_vel;
+    const float* cam_prev;
+    float vel_dt;
+    __half* out;
+};
+
+static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D, int H, int W, const float* solid,
+                     int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
+                     hipStream_t s, float* ms_prep, float* ms_view, const AuxReq* aq) {
     if (!r || !cam || !out) return arg_error("null pointer");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    AuxArgs x{};
+    if (aq) {
+        if (!aq->out || !aq->cam_prev) return arg_error("null aux pointer");
+        const size_t m = (size_t)r->mh * r->mw;
+        if (r->aux_m != m) {
+            if (!ralloc((void**)&r->aux, m * sizeof(float4))) return VFX_E_HIP;
+            r->aux_m = m;
+        }
+        if (aq->vel) {
+            if (aq->uD < 2 || aq->uH < 2 || aq->uW < 2) return arg_error("velocity must be (3, D, H, W) with sizes >= 2");
+            const bool shape_new = aq->uD != r->uD || aq->uH != r->uH || aq->uW != r->uW;
+            const size_t un = (size_t)aq->uD * aq->uH * aq->uW;
+            if (shape_new) {
+                if (!ralloc((void**)&r->vel, un * 4 * sizeof(__half))) return VFX_E_HIP;
+                r->uD = aq->uD; r->uH = aq->uH; r->uW = aq->uW;
+            }
+            if (shape_new || aq->upload_vel)
+                hipLaunchKernelGGL(k_pack_velocity, dim3(nblk(un)), dim3(256), 0, s, aq->vel, un, r->vel);
+        }
+        x.vel = r->vel;
+        x.gvel = Tex3{r->uD, r->uH, r->uW};
+        x.has_vel = aq->vel ? 1 : 0;
+        for (int i = 0; i < 16; ++i) x.cam_prev.v[i] = aq->cam_prev[i];
+        x.vel_dt = aq->vel_dt;
+        x.out_w = (float)r->out_w;
+        x.out_h = (float)r->out_h;
+        x.aux = r->aux;
+    }
     const bool has_solid = solid != nullptr;
-    hipEvent_t ev[3] = {};
+    // timing events, destroyed on every return
+    struct Events {
+        hipEvent_t e[3] = {};
+        ~Events() {
+            for (auto& x : e)
+                if (x) hipEventDestroy(x);
+        }
+    } evs;
+    hipEvent_t* ev = evs.e;
     const bool timed = ms_prep || ms_view;
     if (timed) {
-        for (auto& e : ev)
-            if (!hip_ok(hipEventCreate(&e), "render events")) return VFX_E_HIP;
+        for (int i = 0; i < 3; ++i)
+            if (!hip_ok(hipEventCreate(&ev[i]), "render events")) return VFX_E_HIP;
         hipEventRecord(ev[0], s);
     }
     uint8_t* occ_v = r->occ;
@@ -950,7 +1099,12 @@ int vfx_render_draw(vfx_render* r, const float* dens, const float* temp, int D, 
     a.solid_res = (float)std::max(r->sD, std::max(r->sH, r->sW));
     a.lut = r->lut;
     const dim3 blk(64, 4);
-    hipLaunchKernelGGL(k_march, dim3((r->mw + 63) / 64, (r->mh + 3) / 4), blk, 0, s, a, r->sh, c, r->mh, r->mw, r->hdr);
+    if (aq)
+        hipLaunchKernelGGL(k_march<true>, dim3((r->mw + 63) / 64, (r->mh + 3) / 4), blk, 0, s, a, r->sh, c, r->mh,
+                           r->mw, r->hdr, x);
+    else
+        hipLaunchKernelGGL(k_march<false>, dim3((r->mw + 63) / 64, (r->mh + 3) / 4), blk, 0, s, a, r->sh, c, r->mh,
+                           r->mw, r->hdr, x);
     const size_t b = (size_t)r->bh * r->bw, q = (size_t)r->ph * r->pw;
     float* x0 = r->bwork;
     float* x1 = x0 + 3 * b;
@@ -967,6 +1121,9 @@ int vfx_render_draw(vfx_render* r, const float* dens, const float* temp, int D, 
     hipLaunchKernelGGL(k_bloom_combine, dim3(nblk(b)), dim3(256), 0, s, x0, y0, r->bh, r->bw, r->ph, r->pw, r->bloom);
     hipLaunchKernelGGL(k_composite, dim3((r->out_w + 63) / 64, (r->out_h + 3) / 4), blk, 0, s, r->hdr, r->mh, r->mw,
                        r->bloom, r->bh, r->bw, r->sh, r->out_h, r->out_w, out);
+    if (aq)
+        hipLaunchKernelGGL(k_aux_resolve, dim3((r->out_w + 63) / 64, (r->out_h + 3) / 4), blk, 0, s, r->aux, r->mh,
+                           r->mw, r->out_h, r->out_w, 1.0e-3f, aq->out);
     if (!launch_ok("render")) return VFX_E_HIP;
     if (timed) {
         hipEventRecord(ev[2], s);
@@ -976,9 +1133,36 @@ int vfx_render_draw(vfx_render* r, const float* dens, const float* temp, int D, 
         hipEventElapsedTime(&a1, ev[1], ev[2]);
         if (ms_prep) *ms_prep = a0;
         if (ms_view) *ms_view = a1;
-        for (auto& e : ev) hipEventDestroy(e);
     }
     return VFX_OK;
+}
+
+extern "C" {
+
+// Volume.draw (renderer.py:281-386), colour only.  dens / temp: (D, H, W) float32 device; solid:
+// (sD, sH, sW) float32 device or NULL; upload_solid: the solid changed since the last draw (the
+// caller keys it like the reference, renderer.py:224-226); cam: 16 host floats (camera_basis);
+// reuse_volume: skip the volume prep and re-render the last prepared volume; out: (out_h, out_w, 3)
+// uint8 device.  ms_prep / ms_view (optional, host): device times of the two parts (synchronises).
+int vfx_render_draw(vfx_render* r, const float* dens, const float* temp, int D, int H, int W, const float* solid,
+                    int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
+                    void* stream, float* ms_prep, float* ms_view) {
+    return draw_impl(r, dens, temp, D, H, W, solid, sD, sH, sW, upload_solid, cam, reuse_volume, out,
+                     reinterpret_cast<hipStream_t>(stream), ms_prep, ms_view, nullptr);
+}
+
+// Volume.draw(return_aux=True) (renderer.py:295-393, kernels.py:288-489, :603-626): the same frame
+// as vfx_render_draw, plus aux_out (4, out_h, out_w) half device = alpha, depth, motion_x, motion_y.
+// vel: (3, uD, uH, uW) float32 device in m/s or NULL (zero motion from the medium); upload_vel: the
+// velocity changed since the last aux draw; cam_prev: 16 host floats, the previous sim step's camera
+// (the current one when the caller has none); vel_dt: the sim step in seconds.
+int vfx_render_draw_aux(vfx_render* r, const float* dens, const float* temp, int D, int H, int W, const float* solid,
+                        int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
+                        const float* vel, int uD, int uH, int uW, int upload_vel, const float* cam_prev, float vel_dt,
+                        uint16_t* aux_out, void* stream, float* ms_prep, float* ms_view) {
+    const AuxReq aq{vel, uD, uH, uW, upload_vel, cam_prev, vel_dt, reinterpret_cast<__half*>(aux_out)};
+    return draw_impl(r, dens, temp, D, H, W, solid, sD, sH, sW, upload_solid, cam, reuse_volume, out,
+                     reinterpret_cast<hipStream_t>(stream), ms_prep, ms_view, &aq);
 }
 
 // The march buffer of the last draw: (mh, mw, 4) float premultiplied radiance + transmittance,

@@ -12,8 +12,11 @@ Per draw, on torch's current stream, no host synchronisation:
     empty-brick skipping, early termination, opaque obstacles at the solid's 0.5 iso-surface, floor
     plane); bloom from the HDR march buffer (2 levels); composite at output resolution (upsample,
     background, ACES tone map, sRGB, dither) -> uint8.
+  aux (return_aux=True): the same march also accumulates the medium's opacity, the opacity-weighted
+    depth and, from a velocity field and the previous camera, per-pixel motion; resolved to
+    (4, out_h, out_w) FP16 at output resolution.  The frame is the colour-only draw's, bit for bit.
 
-The reference's opt-in variants (return_aux, temporal, detail) are not part of this renderer.
+The reference's other opt-in variants (temporal, detail) are not part of this renderer.
 """
 from __future__ import annotations
 
@@ -25,6 +28,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from vfx.core.fields import SIM_DT
 from vfx.core.fields3d import Camera, Volume
 from vfx.render.blackbody import blackbody_lut
 from vfx.sim3d import _lib
@@ -119,6 +123,9 @@ class VolumeRenderer:
                        "VolumeRenderer")
         self._solid_key = None
         self._last_vol = None
+        self._vel_key = None
+        self._aux_out = None
+        self.aux: dict[str, torch.Tensor] = {}
         self._alloc_out()
 
     def __del__(self):
@@ -140,6 +147,7 @@ class VolumeRenderer:
         self.mh = max(8, int(round(self.out_h * self.march_scale)))
         self.mw = max(8, int(round(self.out_w * self.march_scale)))
         self.out = torch.empty(self.out_h, self.out_w, 3, dtype=torch.uint8, device=self.device)
+        self._aux_out = None
 
     # ------------------------------------------------------------------ configuration
     def set_output(self, out_size, march_scale: float | None = None) -> None:
@@ -159,13 +167,27 @@ class VolumeRenderer:
         _lib.check(self.lib.vfx_render_set_params(self._h, self._params_ptr(), self._lut_ptr()), "set_params")
 
     # ------------------------------------------------------------------ main entry
+    def _ensure_aux(self) -> None:
+        if self._aux_out is None:
+            self._aux_out = torch.zeros(4, self.out_h, self.out_w, dtype=torch.float16, device=self.device)
+            self.aux = {"alpha": self._aux_out[0], "depth": self._aux_out[1], "motion": self._aux_out[2:4]}
+
     @torch.no_grad()
     def draw(self, volume: Volume, camera: Camera, solid: torch.Tensor | None = None,
-             reuse_volume: bool = False) -> torch.Tensor:
+             reuse_volume: bool = False, *, return_aux: bool = False, velocity: torch.Tensor | None = None,
+             prev_camera: Camera | None = None):
         """Render to (out_h, out_w, 3) uint8 cuda, row 0 = top.  The result is an internal buffer
         reused by the next draw().  reuse_volume=True skips the volume prep and re-renders the volume
         this renderer prepared last with the new camera; it is only valid when the previous draw()
-        was given the same volume and solid (renderer.py:281-308)."""
+        was given the same volume and solid (renderer.py:281-308).
+
+        return_aux=True -> (frame, aux), aux = {"alpha" (h, w), "depth" (h, w), "motion" (2, h, w)}
+        FP16 views of an internal buffer reused by the next draw (renderer.py:295-393): alpha = 1 -
+        the medium's transmittance (obstacles and floor excluded), depth = opacity-weighted mean
+        distance along the ray, motion = this pixel minus where the opacity-weighted mean point was
+        one sim step ago (moved back by `velocity` (3, D, H, W) m/s, any resolution, seen through
+        `prev_camera`), in output pixels.  Without velocity the medium is taken as still; without
+        prev_camera the camera as unmoved.  Both are ignored when return_aux=False."""
         has_solid = solid is not None
         if reuse_volume and (self._last_vol is None or self._last_vol != has_solid):
             reuse_volume = False
@@ -188,14 +210,35 @@ class VolumeRenderer:
         cb = camera_basis(camera, self.mw / self.mh)
         tp, tv = ctypes.c_float(), ctypes.c_float()
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        _lib.check(self.lib.vfx_render_draw(self._h, _ptr(dens), _ptr(temp), D, H, W, _ptr(s_t), sD, sH, sW, upload,
-                                            cb.ctypes.data_as(ctypes.c_void_p), int(reuse_volume), _ptr(self.out),
-                                            stream, ctypes.byref(tp) if self.profile else None,
-                                            ctypes.byref(tv) if self.profile else None), "draw")
+        times = (ctypes.byref(tp) if self.profile else None, ctypes.byref(tv) if self.profile else None)
+        if not return_aux:
+            _lib.check(self.lib.vfx_render_draw(self._h, _ptr(dens), _ptr(temp), D, H, W, _ptr(s_t), sD, sH, sW,
+                                                upload, cb.ctypes.data_as(ctypes.c_void_p), int(reuse_volume),
+                                                _ptr(self.out), stream, *times), "draw")
+        else:
+            self._ensure_aux()
+            v_t = None
+            uD = uH = uW = 0
+            up_v = 0
+            if velocity is not None:
+                v_t = velocity.float().contiguous()
+                assert v_t.dim() == 4 and v_t.shape[0] == 3 and v_t.is_cuda, "velocity must be (3, D, H, W) cuda"
+                _, uD, uH, uW = v_t.shape
+                vkey = (velocity.data_ptr(), tuple(velocity.shape))
+                up_v = int(not reuse_volume or vkey != self._vel_key)     # same contract as reuse_volume
+                self._vel_key = vkey
+            cp = camera_basis(prev_camera, self.mw / self.mh) if prev_camera is not None else cb
+            _lib.check(self.lib.vfx_render_draw_aux(self._h, _ptr(dens), _ptr(temp), D, H, W, _ptr(s_t), sD, sH, sW,
+                                                    upload, cb.ctypes.data_as(ctypes.c_void_p), int(reuse_volume),
+                                                    _ptr(self.out), _ptr(v_t), uD, uH, uW, up_v,
+                                                    cp.ctypes.data_as(ctypes.c_void_p), ctypes.c_float(SIM_DT),
+                                                    _ptr(self._aux_out), stream, *times), "draw")
         if not reuse_volume:
             self._last_vol = has_solid
         if self.profile:
             self.timings = {"prep": tp.value, "view": tv.value, "total": tp.value + tv.value}
+        if return_aux:
+            return self.out, self.aux
         return self.out
 
     def hdr(self) -> torch.Tensor:

@@ -223,11 +223,25 @@ static v3 glow_at(const float* glow, v3 q) {
     return vmul(g, 1.f / (1.f + 60.f * dout * dout));
 }
 
-/* march (kernels.py:288-489, colour only): hdr (mh, mw, 4); skip = 0 disables empty-brick skipping */
+/* world point -> continuous output-pixel coordinates of a camera basis (kernels.py:278-285) */
+static void project_px(v3 q, const float* cam, float out_w, float out_h, float* px, float* py) {
+    v3 rel = vsub(q, V(cam[0], cam[1], cam[2]));
+    float z = fmaxf(vdot(rel, V(cam[3], cam[4], cam[5])), 1.0e-4f);
+    float sx = vdot(rel, V(cam[6], cam[7], cam[8])) / (z * cam[12] * cam[13]);
+    float sy = vdot(rel, V(cam[9], cam[10], cam[11])) / (z * cam[12]);
+    *px = (sx + 1.f) * 0.5f * out_w;
+    *py = (1.f - sy) * 0.5f * out_h;
+}
+
+/* march (kernels.py:288-489): hdr (mh, mw, 4); skip = 0 disables empty-brick skipping.  aux != NULL
+ * is the march_aux variant: aux (mh, mw, 4) premultiplied alpha, alpha depth, alpha motion (x, y)
+ * from vel (uD, uH, uW, 4) (NULL: no medium motion), the previous camera cam_prev, vel_dt and the
+ * output size; the colour result is the same. */
 void or_render_march(const float* vol, int D, int H, int W, const float* light, const float* glow,
                      const float* solid, int sD, int sH, int sW, const uint8_t* occ, const uint8_t* occ_s,
                      int has_solid, float solid_res, const float* lut, const float* prm, const float* cam, int h,
-                     int w, int skip, float* out) {
+                     int w, int skip, float* out, const float* vel, int uD, int uH, int uW, const float* cam_prev,
+                     float vel_dt, float out_w, float out_h, float* aux) {
     shade_t p = unpack(prm);
     int L = p.L;
 #pragma omp parallel for schedule(dynamic, 4)
@@ -249,6 +263,8 @@ void or_render_march(const float* vol, int D, int H, int W, const float* light, 
             float ph1 = hg(cos_t, p.phase_g * p.ms_c) * p.rim + hg(cos_t, -0.25f * p.ms_c) * (1.f - p.rim);
             float ph2 = hg(cos_t, p.phase_g * p.ms_c * p.ms_c) * p.rim + (1.f - p.rim);
             float a1 = p.ms_a, a2 = p.ms_a * p.ms_a, b1 = p.ms_b, b2 = p.ms_b * p.ms_b;
+            float Tm = 1.f, aw = 0.f, ad = 0.f;
+            v3 av = V(0.f, 0.f, 0.f);
             if (t_near < t_far) {
                 float jit = fracf_(ign((float)c, (float)r) + cam[14]);
                 float t0 = t_near + jit * ds, t = t0, t_prev = t_near;
@@ -318,12 +334,26 @@ void or_render_march(const float* vol, int D, int H, int W, const float* light, 
                                 col = vmul(vcw(alb, lit), al);
                             }
                             if (hot) col = vadd(col, vmul(emission(lut, &p, dn, Tk), ds));
+                            if (aux) {
+                                float wgt = T * al;
+                                if (wgt > 0.f) {
+                                    aw += wgt;
+                                    ad += wgt * t;
+                                    if (vel) {
+                                        v3 v = V(tex(vel, uD, uH, uW, 4, 0, q), tex(vel, uD, uH, uW, 4, 1, q),
+                                                 tex(vel, uD, uH, uW, 4, 2, q));
+                                        av = vadd(av, vmul(v, wgt));
+                                    }
+                                }
+                            }
                             rad = vadd(rad, vmul(col, T));
                             T = T * (1.f - al);
                             if (T < 0.004f) {
                                 T = 0.f;
+                                Tm = 0.f;
                                 break;
                             }
+                            Tm = T;
                         }
                     }
                     t_prev = t;
@@ -356,6 +386,23 @@ void or_render_march(const float* vol, int D, int H, int W, const float* light, 
             }
             float* px = out + ((size_t)r * w + c) * 4;
             px[0] = rad.x; px[1] = rad.y; px[2] = rad.z; px[3] = T;
+            if (aux) {
+                float* ax = aux + ((size_t)r * w + c) * 4;
+                float alpha = 1.f - Tm;
+                ax[0] = ax[1] = ax[2] = ax[3] = 0.f;
+                if (aw > 1.0e-6f && alpha > 0.f) {
+                    float dep = ad / aw;
+                    v3 qm = vadd(o, vmul(d, dep));
+                    v3 qp = vsub(qm, vmul(av, vel_dt / aw));
+                    float cx = ((float)c + 0.5f) * out_w / (float)w, cy = ((float)r + 0.5f) * out_h / (float)h;
+                    float ppx, ppy;
+                    project_px(qp, cam_prev, out_w, out_h, &ppx, &ppy);
+                    ax[0] = alpha;
+                    ax[1] = dep * alpha;
+                    ax[2] = (cx - ppx) * alpha;
+                    ax[3] = (cy - ppy) * alpha;
+                }
+            }
         }
 }
 

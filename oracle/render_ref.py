@@ -12,7 +12,11 @@ Parity status (tests/test_oracle_render.py): empty-brick skipping against the un
 uniform non-scattering slab's transmittance against Beer-Lambert; an optically thin hot slab's
 emission against the blackbody table times the path length; an opaque sphere's silhouette against
 the ray-sphere intersection; the light volume's transmittance and sky occlusion of a uniform medium
-against their closed forms; the background of an empty frame against its gradient.
+against their closed forms; the background of an empty frame against its gradient.  Aux (return_aux):
+a uniform slab's alpha against 1 - Beer-Lambert and its depth against the closed-form weighted mean
+distance; zero motion for a still medium and an unmoved camera; uniform-velocity and moved-camera
+motion against the projection of the displaced mean point; the colour unchanged by the aux variant;
+the resolve against float64 bilinear weights.
 """
 from __future__ import annotations
 
@@ -119,15 +123,65 @@ def prep(dens, temp, solid, prm, lut):
                 solid_res=float(max(sD, sH, sW)) if solid is not None else 1.0)
 
 
-def march(pr, prm, lut, cam, mh, mw, skip=True):
+def march(pr, prm, lut, cam, mh, mw, skip=True, aux=None):
+    """hdr (mh, mw, 4).  aux = dict(vel (3, D, H, W) or None, cam_prev (16,), vel_dt, out_h, out_w)
+    selects the march_aux variant: returns (hdr, aux (mh, mw, 4) premultiplied)."""
     prm, lut, cam = _f32(prm), _f32(lut), _f32(cam)
     D, H, W, _ = pr["vol"].shape
     sD, sH, sW = pr["solid"].shape
     out = np.empty((mh, mw, 4), np.float32)
+    vel = cam_prev = abuf = None
+    uD = uH = uW = 0
+    vel_dt = out_w = out_h = 0.0
+    if aux is not None:
+        abuf = np.empty((mh, mw, 4), np.float32)
+        cam_prev = _f32(aux["cam_prev"])
+        vel_dt, out_h, out_w = float(aux["vel_dt"]), float(aux["out_h"]), float(aux["out_w"])
+        if aux.get("vel") is not None:
+            v = np.asarray(aux["vel"], np.float32)
+            _, uD, uH, uW = v.shape
+            vel = np.zeros((uD, uH, uW, 4), np.float32)     # the half (D, H, W, 4) texture, 4th channel 0
+            vel[..., :3] = _h(np.moveaxis(v, 0, -1))
+    nul = ctypes.c_void_p(None)
     _lib().or_render_march(_p(pr["vol"]), D, H, W, _p(pr["light"]), _p(pr["glow"]), _p(pr["solid"]), sD, sH, sW,
                            _p(pr["occ"]), _p(pr["occ_s"]), int(pr["has_solid"]), ctypes.c_float(pr["solid_res"]),
-                           _p(lut), _p(prm), _p(cam), mh, mw, int(skip), _p(out))
-    return out
+                           _p(lut), _p(prm), _p(cam), mh, mw, int(skip), _p(out),
+                           _p(vel) if vel is not None else nul, uD, uH, uW,
+                           _p(cam_prev) if cam_prev is not None else nul, ctypes.c_float(vel_dt),
+                           ctypes.c_float(out_w), ctypes.c_float(out_h), _p(abuf) if abuf is not None else nul)
+    return out if aux is None else (out, abuf)
+
+
+def aux_resolve(abuf, out_h, out_w, a_min=1.0e-3):
+    """kernels.py:603-626: premultiplied (mh, mw, 4) -> (4, out_h, out_w) FP16 alpha, depth, motion_x,
+    motion_y (bilinear upsample with clamped taps, then divided by alpha; 0 where alpha < a_min)."""
+    a = _f32(abuf)
+    mh, mw = a.shape[:2]
+    if (mh, mw) == (out_h, out_w):
+        x = a
+    else:
+        u = ((np.arange(out_w, dtype=np.float32) + np.float32(0.5)) / np.float32(out_w))
+        v = ((np.arange(out_h, dtype=np.float32) + np.float32(0.5)) / np.float32(out_h))
+        fx = u * np.float32(mw) - np.float32(0.5)
+        fy = v * np.float32(mh) - np.float32(0.5)
+        x0f, y0f = np.floor(fx), np.floor(fy)
+        tx, ty = (fx - x0f)[None, :, None], (fy - y0f)[:, None, None]
+        x0 = np.clip(x0f.astype(np.int64), 0, mw - 1)
+        x1 = np.clip(x0f.astype(np.int64) + 1, 0, mw - 1)
+        y0 = np.clip(y0f.astype(np.int64), 0, mh - 1)
+        y1 = np.clip(y0f.astype(np.int64) + 1, 0, mh - 1)
+        one = np.float32(1.0)
+        r0 = a[y0][:, x0] * (one - tx) + a[y0][:, x1] * tx
+        r1 = a[y1][:, x0] * (one - tx) + a[y1][:, x1] * tx
+        x = r0 * (one - ty) + r1 * ty
+    al = x[..., 0]
+    ok = al >= np.float32(a_min)
+    safe = np.where(ok, al, np.float32(1.0))
+    out = np.zeros((4, out_h, out_w), np.float32)
+    out[0] = al
+    for k in range(1, 4):
+        out[k] = np.where(ok, x[..., k] / safe, np.float32(0.0))
+    return out.astype(np.float16)
 
 
 def bloom(hdr, thresh):
@@ -152,11 +206,18 @@ def composite(hdr, bl, prm, out_h, out_w):
     return out
 
 
-def render(dens, temp, solid, cam, prm, lut, out_h, out_w, march_scale=1.0):
-    """The whole draw: (uint8 image (out_h, out_w, 3), hdr (mh, mw, 4))."""
+def render(dens, temp, solid, cam, prm, lut, out_h, out_w, march_scale=1.0, aux=None):
+    """The whole draw: (uint8 image (out_h, out_w, 3), hdr (mh, mw, 4)); with aux = dict(vel,
+    cam_prev, vel_dt) (return_aux) also the resolved (4, out_h, out_w) FP16 aux and the
+    premultiplied march-resolution aux buffer."""
     mh = max(8, int(round(out_h * march_scale)))
     mw = max(8, int(round(out_w * march_scale)))
     pr = prep(np.asarray(dens, np.float32), np.asarray(temp, np.float32), solid, prm, lut)
-    hdr = march(pr, prm, lut, cam, mh, mw)
+    if aux is None:
+        hdr = march(pr, prm, lut, cam, mh, mw)
+    else:
+        hdr, abuf = march(pr, prm, lut, cam, mh, mw, aux=dict(aux, out_h=out_h, out_w=out_w))
     img = composite(hdr, bloom(hdr, float(_f32(prm)[19])), prm, out_h, out_w)
-    return img, hdr
+    if aux is None:
+        return img, hdr
+    return img, hdr, aux_resolve(abuf, out_h, out_w), abuf

@@ -6,7 +6,9 @@ Bars: the march buffer (premultiplied radiance, transmittance) agrees to 1e-4 of
 99.5 % of the pixels (the rest may be single samples moved across a threshold, the early-termination
 limit, d_eps or the obstacle iso-level, by rounding in the prep's blurs); the uint8 frame agrees to
 +-2 levels everywhere (the dither hash takes sin of arguments up to 2e5, where the GPU's and the
-host's sinf round differently).
+host's sinf round differently).  Aux (return_aux): the frame equals the colour-only draw's bit for
+bit; the FP16 alpha, depth and motion agree to 2e-3 relative + 1e-3 absolute (one FP16 rounding
+step and the sampling differences above) on 99.5 % of the output pixels.
 """
 from __future__ import annotations
 
@@ -119,3 +121,65 @@ def test_solver_volume_renders(cuda, hiplib):
     _say(f"  frame mean {a.mean().item():.1f}, max {a.max().item():.0f}")
     assert img.shape == (90, 160, 3) and img.dtype == torch.uint8
     assert a.max() > a.mean() + 20
+
+
+def _vel_field(N, seed=9):
+    """a swirling updraft (m/s)"""
+    f = smooth_noise(N, 3, seed, amp=0.6)
+    g = (np.arange(N) + 0.5) / N
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    v = np.stack([-(zz - 0.5) * 2.0, 1.5 * np.exp(-((xx - 0.5) ** 2 + (zz - 0.5) ** 2) / 0.03), (xx - 0.5) * 2.0])
+    return (v + f).astype(np.float32)
+
+
+def _compare_aux(got, ref, label):
+    got = got.float().cpu().numpy()
+    ref = ref.astype(np.float32)
+    bad = np.abs(got - ref) > 2e-3 * np.abs(ref) + 1e-3
+    frac = bad.any(0).mean()
+    _say(f"  {label}: aux pixels off {frac:.4f}; alpha mean {ref[0].mean():.3f}, depth max {ref[1].max():.2f}, "
+         f"|motion| max {np.abs(ref[2:]).max():.2f} px")
+    assert frac <= 0.005
+
+
+@pytest.mark.parametrize("N,out,scale", [(48, (72, 96), 1.0), (64, (96, 128), 0.5)])
+def test_render_aux_matches_oracle(cuda, hiplib, N, out, scale):
+    _say(f"render aux N={N} out={out} march_scale={scale}")
+    dens, temp, solid = _scene(N)
+    vel = _vel_field(N // 2)                      # velocity at its own (coarser) resolution
+    p = RenderParams3D(light_res=32)
+    lut = blackbody_lut(p.t_lo, p.t_hi, 256, t_ref=p.t_ref)
+    r = VolumeRenderer(out, device=DEV, march_scale=scale, params=p)
+    vol = Volume(density=to_dev(dens), temperature=to_dev(temp))
+    s_d, v_d = to_dev(solid), to_dev(vel)
+    prev = Camera(position=(0.47, 0.56, -1.38), target=(0.5, 0.45, 0.5))
+    for k, (cam, pc, vv) in enumerate([(CAMS[0], prev, v_d), (CAMS[1], None, v_d), (CAMS[0], prev, None)]):
+        plain = r.draw(vol, cam, solid=s_d).clone()
+        img, aux = r.draw(vol, cam, solid=s_d, return_aux=True, velocity=vv, prev_camera=pc)
+        torch.cuda.synchronize()
+        assert torch.equal(img, plain)
+        assert aux["alpha"].shape == out and aux["depth"].shape == out and aux["motion"].shape == (2,) + out
+        assert aux["alpha"].dtype == torch.float16
+        asp = r.mw / r.mh
+        cp = camera_basis(pc if pc is not None else cam, asp)
+        _, _, ref, _ = R.render(dens, temp, solid, camera_basis(cam, asp), p.pack(), lut, out[0], out[1], scale,
+                                aux=dict(vel=vel if vv is not None else None, cam_prev=cp, vel_dt=1.0 / 30.0))
+        got = torch.cat([aux["alpha"][None], aux["depth"][None], aux["motion"]])
+        _compare_aux(got, ref, f"camera {k}")
+
+
+def test_render_aux_reuse_volume(cuda, hiplib):
+    """an aux draw with reuse_volume (velocity kept, new camera) equals the full aux draw"""
+    dens, temp, solid = _scene(48, seed=5)
+    vel = to_dev(_vel_field(48, seed=2))
+    r = VolumeRenderer((60, 80), device=DEV, params=RenderParams3D(light_res=24))
+    vol = Volume(density=to_dev(dens), temperature=to_dev(temp))
+    s = to_dev(solid)
+    r.draw(vol, CAMS[0], solid=s, return_aux=True, velocity=vel, prev_camera=CAMS[0])
+    img_a, aux = r.draw(vol, CAMS[1], solid=s, reuse_volume=True, return_aux=True, velocity=vel, prev_camera=CAMS[0])
+    a_img, a_aux = img_a.clone(), {k: v.clone() for k, v in aux.items()}
+    img_b, aux = r.draw(vol, CAMS[1], solid=s, return_aux=True, velocity=vel, prev_camera=CAMS[0])
+    torch.cuda.synchronize()
+    assert torch.equal(a_img, img_b)
+    for k in aux:
+        assert torch.equal(a_aux[k], aux[k]), k

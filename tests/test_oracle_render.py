@@ -181,3 +181,135 @@ def test_blackbody_table():
     lum = lut @ np.array([0.2126, 0.7152, 0.0722])
     assert np.all(np.diff(lum) >= 0)                 # brighter with temperature
     assert np.allclose(lut.max(axis=1)[1:], np.max(lut[1:], axis=1))
+
+
+# ---------------------------------------------------------------------- aux (return_aux)
+def _slab_aux(d0, vel=None, cam_prev=None, mh=24, mw=24, vel_dt=1.0 / 30.0):
+    N = 32
+    p, prm, lut = _params(smoke_albedo=(0.0, 0.0, 0.0), floor=0.0)
+    dens = np.full((N, N, N), d0, np.float32)
+    temp = np.full((N, N, N), 293.0, np.float32)
+    pr = R.prep(dens, temp, None, prm, lut)
+    cam = camera_basis(CAM, mw / mh)
+    aux = dict(vel=vel, cam_prev=cam if cam_prev is None else cam_prev, vel_dt=vel_dt, out_h=mh, out_w=mw)
+    hdr, abuf = R.march(pr, prm, lut, cam, mh, mw, aux=aux)
+    return p, pr, prm, lut, cam, hdr, abuf
+
+
+def test_aux_leaves_colour_unchanged():
+    N = 32
+    p, prm, lut = _params()
+    f = smooth_noise(N, 3, 11, amp=1.0)
+    dens = np.clip(f[0], 0.0, None).astype(np.float32)
+    temp = (293.0 + 1500.0 * np.clip(f[1], 0.0, None)).astype(np.float32)
+    pr = R.prep(dens, temp, None, prm, lut)
+    cam = camera_basis(CAM, 1.25)
+    a = R.march(pr, prm, lut, cam, 32, 40)
+    b, _ = R.march(pr, prm, lut, cam, 32, 40, aux=dict(vel=f, cam_prev=cam, vel_dt=0.1, out_h=64, out_w=80))
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("d0", [0.01, 0.05])
+def test_aux_alpha_and_depth_of_a_uniform_slab(d0):
+    """alpha = 1 - exp(-sigma L) (Beer-Lambert over the chord, to one step); the opacity-weighted
+    depth = the mean of the truncated exponential distance: t_near + 1/sigma - L e^(-sigma L) /
+    (1 - e^(-sigma L)), to 0.6 steps: the samples sit at t_near + (jitter + k) ds with the jitter in
+    [0, 1), and the step sum's mean lies half a step before the integral's (thick media) or halfway
+    between the first and last sample (thin media), so the offset stays within half a step"""
+    p, pr, prm, lut, cam, hdr, a = _slab_aux(d0)
+    o, d = _rays(cam, 24, 24)
+    L = _chord(o, d)
+    inv = 1.0 / np.where(np.abs(d) < 1e-12, 1e-12, d)
+    tn = np.maximum(np.max(np.minimum((0.0 - o) * inv, (1.0 - o) * inv), axis=-1), 0.0)
+    sigma = p.k_abs * float(np.float16(d0))
+    ds = 1.0 / p.steps
+    hit = L > 2 * ds
+    assert hit.sum() > 100
+    al = a[..., 0]
+    # no obstacle, no floor: the medium's transmittance is the march's
+    assert np.array_equal(al, 1.0 - hdr[..., 3])
+    lo, hi = 1 - np.exp(-sigma * np.maximum(L - ds, 0.0)), 1 - np.exp(-sigma * (L + ds))
+    assert np.all(al[hit] >= lo[hit] - 1e-5) and np.all(al[hit] <= hi[hit] + 1e-5)
+    dep = a[..., 1][hit] / al[hit]
+    Lh = L[hit]
+    want = tn[hit] + 1.0 / sigma - Lh * np.exp(-sigma * Lh) / (1.0 - np.exp(-sigma * Lh))
+    err = dep - want
+    assert np.abs(err).max() <= 0.6 * ds
+    assert err.min() < -0.2 * ds and err.max() > 0.2 * ds      # the jitter spreads it both ways
+    assert np.all(a[~(L > 0)] == 0.0)                  # rays that miss the cube carry no aux
+    assert np.all(a[..., 2:][hit] ** 2 < (1e-3 * al[hit, None]) ** 2 + 1e-12)   # still medium, same camera
+
+
+def _mean_point_z(cam, a, mh, mw):
+    """camera-space depth of each pixel's opacity-weighted mean point"""
+    o, d = _rays(cam, mh, mw)
+    al = a[..., 0]
+    ok = al > 0.05
+    dep = np.where(ok, a[..., 1] / np.where(ok, al, 1.0), 0.0)
+    q = o[None, None] + d * dep[..., None]
+    return ok, (q - cam[0:3].astype(np.float64)) @ cam[3:6].astype(np.float64)
+
+
+@pytest.mark.parametrize("v", [0.5, -0.75])
+def test_aux_motion_of_a_uniform_velocity(v):
+    """a medium moving at v along the camera's right (up) axis: the mean point moved back by v dt
+    projects v dt out_w / (2 z tan(fov/2) aspect) pixels to the left (v dt out_h / (2 z tan) pixels
+    below), so the motion is +that in x (-that in y: rows grow downward)"""
+    N, mh, mw, dt = 32, 24, 30, 1.0 / 30.0
+    for axis in (0, 1):
+        vel = np.zeros((3, N, N, N), np.float32)
+        vel[axis] = v
+        p, pr, prm, lut, cam, hdr, a = _slab_aux(0.05, vel=vel, mh=mh, mw=mw, vel_dt=dt)
+        ok, z = _mean_point_z(cam, a, mh, mw)
+        assert ok.sum() > 100
+        th, asp = float(cam[12]), float(cam[13])
+        m = a[..., 2 + axis][ok] / a[..., 0][ok]
+        other = a[..., 3 - axis][ok] / a[..., 0][ok]
+        if axis == 0:
+            want = v * dt * mw / (2.0 * z[ok] * th * asp)
+        else:
+            want = -v * dt * mh / (2.0 * z[ok] * th)
+        assert np.allclose(m, want, rtol=2e-3, atol=1e-4)
+        # rt and up are the world x and y axes here, so the other component stays 0 up to the
+        # perspective of the displaced point's depth (unchanged: the motion is parallel to the image)
+        assert np.abs(other).max() < 1e-3
+
+
+def test_aux_motion_of_a_moved_camera():
+    """a still medium and a camera that moved by +delta along its right axis since the last sim
+    step: every point slides delta out_w / (2 z tan(fov/2) aspect) pixels to the left"""
+    mh, mw, delta = 20, 28, 0.02
+    cam = camera_basis(CAM, mw / mh)
+    prev = cam.copy()
+    prev[0:3] = cam[0:3] - delta * cam[6:9]
+    p, pr, prm, lut, cam, hdr, a = _slab_aux(0.05, cam_prev=prev, mh=mh, mw=mw)
+    ok, z = _mean_point_z(cam, a, mh, mw)
+    assert ok.sum() > 100
+    th, asp = float(cam[12]), float(cam[13])
+    mx = a[..., 2][ok] / a[..., 0][ok]
+    my = a[..., 3][ok] / a[..., 0][ok]
+    assert np.allclose(mx, -delta * mw / (2.0 * z[ok] * th * asp), rtol=2e-3, atol=1e-4)
+    assert np.abs(my).max() < 1e-3
+
+
+@pytest.mark.parametrize("shape", [(24, 30), (48, 60), (37, 53)])
+def test_aux_resolve_against_bilinear(shape):
+    """the resolve's upsample is torch's bilinear interpolation (align_corners=False, clamped
+    borders), then the division by alpha where alpha >= 1e-3"""
+    import torch
+    import torch.nn.functional as F
+    rng = np.random.default_rng(5)
+    mh, mw = 24, 30
+    a = rng.random((mh, mw, 4), dtype=np.float32)
+    a[..., 0] = np.where(rng.random((mh, mw)) < 0.3, 0.0, a[..., 0])
+    a[..., 1:] *= a[..., :1] * 3.0
+    got = R.aux_resolve(a, *shape).astype(np.float32)
+    up = F.interpolate(torch.from_numpy(a).permute(2, 0, 1)[None].double(), size=shape, mode="bilinear",
+                       align_corners=False)[0].numpy()
+    al = up[0]
+    ok = al >= 1e-3
+    want = np.zeros((4,) + shape)
+    want[0] = al
+    want[1:] = np.where(ok, up[1:] / np.where(ok, al, 1.0), 0.0)
+    both = ok | (np.abs(al - 1e-3) > 1e-6)
+    assert np.allclose(got[:, both], want[:, both], rtol=2e-3, atol=2e-3)
