@@ -363,7 +363,8 @@ def render_report(reps: int):
     """The volume renderer (vfx/render3d VolumeRenderer) at 2560 x 1440 on the solver's own 256^3
     fire state (and 64^3, 128^3 states of the same scene), as the reference's feasibility table
     (render3d README: full draw, view-only redraw with reuse_volume, prep / view split), and the
-    full draw with return_aux (alpha, depth, motion from the state's velocity and a moved camera).
+    full draw with return_aux (alpha, depth, motion from the state's velocity and a moved camera),
+    and the view-only redraw with the temporal resolve on.
     Times are device events around each draw, median of `reps` draws after 3 warm-up draws."""
     import torch
     from vfx.core.fields3d import Camera, Volume
@@ -397,10 +398,17 @@ def render_report(reps: int):
                 r.draw(vol, cam, return_aux=True, velocity=st.velocity, prev_camera=prev)
                 if k >= 3:
                     aux.append(r.last_timings()["total"])
+            temporal = []
+            r.set_temporal(True)
+            for k in range(3 + reps):
+                r.draw(vol, cam, reuse_volume=k > 0)
+                if k >= 3:
+                    temporal.append(r.last_timings()["total"])
+            r.set_temporal(False)
             med = lambda v: round(statistics.median(v), 3)
             rows.append({"volume": N, "march_scale": scale, "steps_per_m": steps, "march_px": f"{r.mw}x{r.mh}",
                          "full_ms": med(full), "prep_ms": med(prep), "view_ms": med(view), "reuse_ms": med(reuse),
-                         "aux_full_ms": med(aux)})
+                         "aux_full_ms": med(aux), "temporal_reuse_ms": med(temporal)})
             del r
         del s, st
         torch.cuda.empty_cache()

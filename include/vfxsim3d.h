@@ -182,7 +182,7 @@ int vfx_tracking_error(const float* hr_data, int hr_res, const float* lr_data, i
                        float* out3, void* stream);
 
 /* ---------------------------------------------------------------- volume renderer
- * vfx/render3d/renderer.py VolumeRenderer (default path and return_aux; no temporal resolve, no detail
+ * vfx/render3d/renderer.py VolumeRenderer (default path, return_aux and the temporal resolve; no detail
  * field).  FP32 only.  params: VFX_RENDER_PARAMS floats, RenderParams3D's fields in declaration order
  * (renderer.py:31-66) with the 3-tuples flattened: steps, light_res, k_abs, k_shadow, k_solid, k_amb,
  * k_emit, emit_min, emit_scale, soot, fire_light, phase_g, rim, ms_a, ms_b, ms_c, exposure, contrast,
@@ -220,7 +220,18 @@ int vfx_render_draw_aux(vfx_render* r, const float* dens, const float* temp, int
                         int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
                         const float* vel, int uD, int uH, int uW, int upload_vel, const float* cam_prev, float vel_dt,
                         uint16_t* aux_out, void* stream, float* ms_prep, float* ms_view);
-/* The march buffer of the last draw, (mh, mw, 4) float: premultiplied radiance, transmittance. */
+/* Temporal amortization (renderer.py:130-163, :331-346, :485-492; temporal.py:70-150): on != 0 makes
+ * every draw offset its ray-start jitter by a golden-ratio sequence and blend the march into a history
+ * reprojected by camera motion (at the medium's opacity-weighted depth, else the floor, else the
+ * domain centre), with a 3x3 variance clip in (luma-compressed YCoCg, T); bloom and composite run on
+ * the resolved buffer.  alpha: base current weight (0.1), gamma: clip width (1), k_t: gain of the T
+ * change (4), alpha_tick: minimum current weight on draws with a new volume (0.5).  The history is
+ * invalid after a resize and after vfx_render_reset_history (VolumeRenderer.set_temporal /
+ * reset_history, renderer.py:157-163). */
+int vfx_render_set_temporal(vfx_render* r, int on, float alpha, float gamma, float k_t, float alpha_tick);
+int vfx_render_reset_history(vfx_render* r);
+/* The march buffer of the last draw (after the temporal resolve when that is on), (mh, mw, 4) float:
+ * premultiplied radiance, transmittance. */
 int vfx_render_get_hdr(vfx_render* r, float* dst, int* mh, int* mw, void* stream);
 
 /* Message of the last failed call on this thread. */

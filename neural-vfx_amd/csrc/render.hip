@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "ctx.hpp"
@@ -800,6 +801,102 @@ __global__ void __launch_bounds__(256) k_composite(const float4* __restrict__ hd
     o[2] = to_srgb8(aces(col.z), dither);
 }
 
+// ---------------------------------------------------------------- temporal resolve
+// (temporal.py:26-150) The march buffer blended into a history reprojected by camera motion only,
+// in (luma-compressed YCoCg, T) space with a 3x3 variance clip of the current frame.
+
+// (rgb, T) -> (Y, Co, Cg, T) scaled by 1 / (1 + max(Y, 0)) so bright fire does not dominate
+__device__ __forceinline__ float4 to_c4(float4 v) {
+    const float y = 0.25f * v.x + 0.5f * v.y + 0.25f * v.z;
+    const float co = 0.5f * v.x - 0.5f * v.z;
+    const float cg = -0.25f * v.x + 0.5f * v.y - 0.25f * v.z;
+    const float s = 1.f / (1.f + fmaxf(y, 0.f));
+    return make_float4(y * s, co * s, cg * s, v.w);
+}
+
+__device__ __forceinline__ float4 from_c4(float4 c) {
+    const float s = 1.f / fmaxf(1.f - c.x, 1.0e-3f);
+    const float y = c.x * s, co = c.y * s, cg = c.z * s;
+    return make_float4(y + co - cg, y + cg, y - co - cg, c.w);
+}
+
+__device__ __forceinline__ float c4_at(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// cur: this frame's march buffer; aux: its premultiplied (alpha, alpha depth, ...); hist: the last
+// resolved buffer; cam: this draw's camera; tcam: the previous draw's camera with tcam[14] = the
+// minimum current weight of this draw and tcam[15] = history valid; out: the resolved buffer
+__global__ void __launch_bounds__(256) k_temporal(const float4* __restrict__ cur, const float4* __restrict__ aux,
+                                                  const float4* __restrict__ hist, Cam cam, Cam tcam, int H, int W,
+                                                  float alpha_base, float gamma, float k_t, float4* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y * blockDim.y + threadIdx.y;
+    if (c >= W || r >= H) return;
+    const size_t i = (size_t)r * W + c;
+    const float4 now = cur[i];
+    if (tcam.v[15] < 0.5f) {
+        out[i] = now;
+        return;
+    }
+    const float3 o = f3(cam.v[0], cam.v[1], cam.v[2]);
+    const float th = cam.v[12], aspect = cam.v[13];
+    const float sx = (2.f * ((float)c + 0.5f) / (float)W - 1.f) * th * aspect;
+    const float sy = (1.f - 2.f * ((float)r + 0.5f) / (float)H) * th;
+    const float3 d = norm3(add(add(f3(cam.v[3], cam.v[4], cam.v[5]), mul(f3(cam.v[6], cam.v[7], cam.v[8]), sx)),
+                               mul(f3(cam.v[9], cam.v[10], cam.v[11]), sy)));
+    // the reprojected point: the medium's opacity-weighted depth, else the floor plane, else the
+    // distance to the domain centre
+    const float4 ax = aux[i];
+    float dep;
+    if (ax.x > 0.02f) dep = ax.y / ax.x;
+    else if (d.y < -1.0e-4f && o.y > 0.f) dep = -o.y / d.y;
+    else dep = len3(sub(f3(0.5f, 0.5f, 0.5f), o));
+    const float3 rel = sub(add(o, mul(d, dep)), f3(tcam.v[0], tcam.v[1], tcam.v[2]));
+    const float z = dot3(rel, f3(tcam.v[3], tcam.v[4], tcam.v[5]));
+    bool valid = z > 1.0e-3f;
+    float px = 0.f, py = 0.f;
+    if (valid) {
+        const float psx = dot3(rel, f3(tcam.v[6], tcam.v[7], tcam.v[8])) / (z * tcam.v[12] * tcam.v[13]);
+        const float psy = dot3(rel, f3(tcam.v[9], tcam.v[10], tcam.v[11])) / (z * tcam.v[12]);
+        px = (psx + 1.f) * 0.5f * (float)W;
+        py = (1.f - psy) * 0.5f * (float)H;
+        valid = px >= 0.f && px <= (float)W && py >= 0.f && py <= (float)H;
+    }
+    if (!valid) {
+        out[i] = now;   // off screen or behind the previous camera: reset
+        return;
+    }
+    const float4 hp = to_c4(sample_cr(hist, H, W, px / (float)W, py / (float)H));
+    float4 m1 = make_float4(0.f, 0.f, 0.f, 0.f), m2 = m1;
+    float4 lo = make_float4(1.0e9f, 1.0e9f, 1.0e9f, 1.0e9f), hi = make_float4(-1.0e9f, -1.0e9f, -1.0e9f, -1.0e9f);
+    for (int j = -1; j <= 1; ++j)
+        for (int k = -1; k <= 1; ++k) {
+            const float4 sn = to_c4(cur[(size_t)min(max(r + j, 0), H - 1) * W + min(max(c + k, 0), W - 1)]);
+            m1 = make_float4(m1.x + sn.x, m1.y + sn.y, m1.z + sn.z, m1.w + sn.w);
+            m2 = make_float4(m2.x + sn.x * sn.x, m2.y + sn.y * sn.y, m2.z + sn.z * sn.z, m2.w + sn.w * sn.w);
+            lo = make_float4(fminf(lo.x, sn.x), fminf(lo.y, sn.y), fminf(lo.z, sn.z), fminf(lo.w, sn.w));
+            hi = make_float4(fmaxf(hi.x, sn.x), fmaxf(hi.y, sn.y), fmaxf(hi.z, sn.z), fmaxf(hi.w, sn.w));
+        }
+    m1 = make_float4(m1.x / 9.f, m1.y / 9.f, m1.z / 9.f, m1.w / 9.f);
+    m2 = make_float4(m2.x / 9.f, m2.y / 9.f, m2.z / 9.f, m2.w / 9.f);
+    float cl[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float mk = c4_at(m1, k);
+        const float sig = sqrtf(fmaxf(c4_at(m2, k) - mk * mk, 0.f));
+        const float a = fmaxf(c4_at(lo, k), mk - gamma * sig);
+        const float b = fminf(c4_at(hi, k), mk + gamma * sig);
+        cl[k] = clampf(c4_at(hp, k), a, b);
+    }
+    const float4 nc = to_c4(now);
+    // trust the history less where the medium's T changed or its neighbourhood is high-contrast in T
+    const float dT = fabsf(nc.w - hp.w);
+    const float a0 = fmaxf(alpha_base, tcam.v[14]);
+    const float wgt = clampf(a0 + k_t * dT + 0.5f * k_t * (hi.w - lo.w) * dT, a0, 1.f);
+    const float4 res = make_float4(cl[0] * (1.f - wgt) + nc.x * wgt, cl[1] * (1.f - wgt) + nc.y * wgt,
+                                   cl[2] * (1.f - wgt) + nc.z * wgt, cl[3] * (1.f - wgt) + nc.w * wgt);
+    out[i] = from_c4(res);
+}
+
 static int nblk(size_t n) { return (int)((n + 255) / 256); }
 
 static Taps gauss_taps(double sigma) {
@@ -844,6 +941,16 @@ struct vfx_render {
     __half* vel = nullptr;       // uD uH uW 4
     float4* aux = nullptr;       // aux_m float4
     size_t aux_m = 0;
+    // temporal resolve: parameters, frame counter, the previous draw's camera, history buffers
+    bool t_on = false;
+    float t_alpha = 0.1f, t_gamma = 1.f, t_kT = 4.f, t_alpha_tick = 0.5f;
+    long long t_frame = 0;
+    bool t_prev_valid = false;
+    float t_prev[16] = {};
+    float4* hist = nullptr;      // hist_m float4 each: the last resolved buffer and the next one
+    float4* tres = nullptr;
+    size_t hist_m = 0;
+    const float4* view = nullptr;   // the buffer bloom and composite read (hdr, or the resolved one)
 };
 
 static bool ralloc(void** p, size_t bytes) {
@@ -898,13 +1005,15 @@ static int alloc_output(vfx_render* r, int out_h, int out_w, float scale) {
     if (!ralloc((void**)&r->hdr, m * sizeof(float4)) || !ralloc((void**)&r->bwork, (6 * b + 6 * q) * sizeof(float)) ||
         !ralloc((void**)&r->bloom, 3 * b * sizeof(float)))
         return VFX_E_HIP;
+    r->view = r->hdr;
+    r->t_prev_valid = false;   // the history is invalid after a resize
     return VFX_OK;
 }
 
 static void free_all(vfx_render* r) {
     hipDeviceSynchronize();
     void* ps[] = {r->lut, r->packed, r->occ, r->sblur, r->solid, r->light, r->glow_src, r->gwork, r->glow, r->hdr,
-                  r->bwork, r->bloom, r->vel, r->aux};
+                  r->bwork, r->bloom, r->vel, r->aux, r->hist, r->tres};
     for (void* p : ps)
         if (p) hipFree(p);
 }
@@ -979,16 +1088,55 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
                      int sD, int sH, int sW, int upload_solid, const float* cam, int reuse_volume, uint8_t* out,
                      hipStream_t s, float* ms_prep, float* ms_view, const AuxReq* aq) {
     if (!r || !cam || !out) return arg_error("null pointer");
-    AuxArgs x{};
-    if (aq) {
-        if (!aq->out || !aq->cam_prev) return arg_error("null aux pointer");
-        const size_t m = (size_t)r->mh * r->mw;
-        if (r->aux_m != m) {
-            if (!ralloc((void**)&r->aux, m * sizeof(float4))) return VFX_E_HIP;
-            r->aux_m = m;
+    if (aq && (!aq->out || !aq->cam_prev)) return arg_error("null aux pointer");
+    // argument checks before any state changes
+    if (!reuse_volume) {
+        if (!dens || !temp || D < 2 || H < 2 || W < 2) return arg_error("volume must be (D, H, W) with sizes >= 2");
+        if (solid && (sD < 2 || sH < 2 || sW < 2)) return arg_error("solid must be (D, H, W) with sizes >= 2");
+    } else if (r->vD == 0) {
+        return arg_error("reuse_volume without a prepared volume");
+    }
+    if (aq && aq->vel && (aq->uD < 2 || aq->uH < 2 || aq->uW < 2))
+        return arg_error("velocity must be (3, D, H, W) with sizes >= 2");
+    const bool temporal = r->t_on;
+    const bool march_aux = aq || temporal;   // the temporal resolve needs the march's opacity-weighted depth
+    const size_t m = (size_t)r->mh * r->mw;
+    if (march_aux && r->aux_m != m) {
+        if (!ralloc((void**)&r->aux, m * sizeof(float4))) return VFX_E_HIP;
+        r->aux_m = m;
+    }
+    if (temporal && r->hist_m != m) {
+        if (!ralloc((void**)&r->hist, m * sizeof(float4)) || !ralloc((void**)&r->tres, m * sizeof(float4)))
+            return VFX_E_HIP;
+        r->hist_m = m;
+        r->t_prev_valid = false;
+    }
+    Cam c;
+    for (int i = 0; i < 16; ++i) c.v[i] = cam[i];
+    Cam tc{};
+    if (temporal) {
+        // per-frame offset of the ray-start jitter (golden-ratio sequence), and the previous draw's
+        // camera with this draw's minimum current weight and the history-valid flag
+        ++r->t_frame;
+        c.v[14] = (float)std::fmod((double)r->t_frame * 0.6180339887498949, 1.0);
+        if (r->t_prev_valid) {
+            for (int i = 0; i < 16; ++i) tc.v[i] = r->t_prev[i];
+            tc.v[14] = reuse_volume ? 0.f : r->t_alpha_tick;
+            tc.v[15] = 1.f;
         }
+        for (int i = 0; i < 16; ++i) r->t_prev[i] = c.v[i];
+        r->t_prev_valid = true;
+    }
+    AuxArgs x{};
+    if (march_aux) {
+        for (int i = 0; i < 16; ++i) x.cam_prev.v[i] = aq ? aq->cam_prev[i] : c.v[i];
+        x.vel_dt = aq ? aq->vel_dt : 0.f;
+        x.out_w = (float)r->out_w;
+        x.out_h = (float)r->out_h;
+        x.aux = r->aux;
+    }
+    if (aq) {
         if (aq->vel) {
-            if (aq->uD < 2 || aq->uH < 2 || aq->uW < 2) return arg_error("velocity must be (3, D, H, W) with sizes >= 2");
             const bool shape_new = aq->uD != r->uD || aq->uH != r->uH || aq->uW != r->uW;
             const size_t un = (size_t)aq->uD * aq->uH * aq->uW;
             if (shape_new) {
@@ -1001,11 +1149,6 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
         x.vel = r->vel;
         x.gvel = Tex3{r->uD, r->uH, r->uW};
         x.has_vel = aq->vel ? 1 : 0;
-        for (int i = 0; i < 16; ++i) x.cam_prev.v[i] = aq->cam_prev[i];
-        x.vel_dt = aq->vel_dt;
-        x.out_w = (float)r->out_w;
-        x.out_h = (float)r->out_h;
-        x.aux = r->aux;
     }
     const bool has_solid = solid != nullptr;
     // timing events, destroyed on every return
@@ -1027,14 +1170,12 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
     uint8_t* occ_s = r->occ + BRICKS * BRICKS * BRICKS;
     const int B3 = BRICKS * BRICKS * BRICKS;
     if (!reuse_volume) {
-        if (!dens || !temp || D < 2 || H < 2 || W < 2) return arg_error("volume must be (D, H, W) with sizes >= 2");
         const size_t n = (size_t)D * H * W;
         if (D != r->vD || H != r->vH || W != r->vW) {
             if (!ralloc((void**)&r->packed, n * sizeof(__half2))) return VFX_E_HIP;
             r->vD = D; r->vH = H; r->vW = W;
         }
         if (has_solid && (upload_solid || sD != r->sD || sH != r->sH || sW != r->sW || !r->have_solid)) {
-            if (sD < 2 || sH < 2 || sW < 2) return arg_error("solid must be (D, H, W) with sizes >= 2");
             const size_t sn = (size_t)sD * sH * sW;
             if (sD != r->sD || sH != r->sH || sW != r->sW) {
                 if (!ralloc((void**)&r->sblur, 2 * sn * sizeof(float)) || !ralloc((void**)&r->solid, sn))
@@ -1078,13 +1219,9 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
         hipLaunchKernelGGL(k_blur_axis, dim3(nblk(3 * g8)), dim3(256), 0, s, g8t, g8b, 3, G8, G8, G8, 1, k3);
         hipLaunchKernelGGL(k_blur_axis, dim3(nblk(3 * g8)), dim3(256), 0, s, g8b, g8t, 3, G8, G8, G8, 0, k3);
         hipLaunchKernelGGL(k_glow_combine, dim3(nblk(g3)), dim3(256), 0, s, gt, g8t, r->sh.glow_coarse, r->glow);
-    } else if (r->vD == 0) {
-        return arg_error("reuse_volume without a prepared volume");
     }
     if (timed) hipEventRecord(ev[1], s);
     // view
-    Cam c;
-    for (int i = 0; i < 16; ++i) c.v[i] = cam[i];
     MarchArgs a;
     a.vol = r->packed;
     a.gv = Tex3{r->vD, r->vH, r->vW};
@@ -1099,19 +1236,26 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
     a.solid_res = (float)std::max(r->sD, std::max(r->sH, r->sW));
     a.lut = r->lut;
     const dim3 blk(64, 4);
-    if (aq)
+    if (march_aux)
         hipLaunchKernelGGL(k_march<true>, dim3((r->mw + 63) / 64, (r->mh + 3) / 4), blk, 0, s, a, r->sh, c, r->mh,
                            r->mw, r->hdr, x);
     else
         hipLaunchKernelGGL(k_march<false>, dim3((r->mw + 63) / 64, (r->mh + 3) / 4), blk, 0, s, a, r->sh, c, r->mh,
                            r->mw, r->hdr, x);
+    r->view = r->hdr;
+    if (temporal) {
+        hipLaunchKernelGGL(k_temporal, dim3((r->mw + 63) / 64, (r->mh + 3) / 4), blk, 0, s, r->hdr, r->aux, r->hist, c,
+                           tc, r->mh, r->mw, r->t_alpha, r->t_gamma, r->t_kT, r->tres);
+        std::swap(r->hist, r->tres);   // the resolved buffer is the next draw's history
+        r->view = r->hist;
+    }
     const size_t b = (size_t)r->bh * r->bw, q = (size_t)r->ph * r->pw;
     float* x0 = r->bwork;
     float* x1 = x0 + 3 * b;
     float* y0 = x1 + 3 * b;
     float* y1 = y0 + 3 * q;
     const Taps k2 = gauss_taps(2.0);
-    hipLaunchKernelGGL(k_bloom_pool, dim3(nblk(b)), dim3(256), 0, s, r->hdr, r->mh, r->mw, r->sh.bloom_thresh, r->bh,
+    hipLaunchKernelGGL(k_bloom_pool, dim3(nblk(b)), dim3(256), 0, s, r->view, r->mh, r->mw, r->sh.bloom_thresh, r->bh,
                        r->bw, x0);
     hipLaunchKernelGGL(k_blur_axis, dim3(nblk(3 * b)), dim3(256), 0, s, x0, x1, 3, 1, r->bh, r->bw, 0, k2);
     hipLaunchKernelGGL(k_blur_axis, dim3(nblk(3 * b)), dim3(256), 0, s, x1, x0, 3, 1, r->bh, r->bw, 1, k2);
@@ -1119,7 +1263,7 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
     hipLaunchKernelGGL(k_blur_axis, dim3(nblk(3 * q)), dim3(256), 0, s, y0, y1, 3, 1, r->ph, r->pw, 0, k2);
     hipLaunchKernelGGL(k_blur_axis, dim3(nblk(3 * q)), dim3(256), 0, s, y1, y0, 3, 1, r->ph, r->pw, 1, k2);
     hipLaunchKernelGGL(k_bloom_combine, dim3(nblk(b)), dim3(256), 0, s, x0, y0, r->bh, r->bw, r->ph, r->pw, r->bloom);
-    hipLaunchKernelGGL(k_composite, dim3((r->out_w + 63) / 64, (r->out_h + 3) / 4), blk, 0, s, r->hdr, r->mh, r->mw,
+    hipLaunchKernelGGL(k_composite, dim3((r->out_w + 63) / 64, (r->out_h + 3) / 4), blk, 0, s, r->view, r->mh, r->mw,
                        r->bloom, r->bh, r->bw, r->sh, r->out_h, r->out_w, out);
     if (aq)
         hipLaunchKernelGGL(k_aux_resolve, dim3((r->out_w + 63) / 64, (r->out_h + 3) / 4), blk, 0, s, r->aux, r->mh,
@@ -1165,14 +1309,34 @@ int vfx_render_draw_aux(vfx_render* r, const float* dens, const float* temp, int
                      reinterpret_cast<hipStream_t>(stream), ms_prep, ms_view, &aq);
 }
 
-// The march buffer of the last draw: (mh, mw, 4) float premultiplied radiance + transmittance,
-// copied to dst (device); its size in *mh, *mw.
+// Temporal amortization (renderer.py:130-163, temporal.py): on != 0 blends each draw's march into
+// a history reprojected by camera motion (alpha: base current weight, gamma: clip width, k_t: gain
+// of the T change, alpha_tick: current weight on draws with a new volume).  Does not reset the
+// history; vfx_render_reset_history does.
+int vfx_render_set_temporal(vfx_render* r, int on, float alpha, float gamma, float k_t, float alpha_tick) {
+    if (!r) return arg_error("null renderer");
+    r->t_on = on != 0;
+    r->t_alpha = alpha;
+    r->t_gamma = gamma;
+    r->t_kT = k_t;
+    r->t_alpha_tick = alpha_tick;
+    return VFX_OK;
+}
+
+int vfx_render_reset_history(vfx_render* r) {
+    if (!r) return arg_error("null renderer");
+    r->t_prev_valid = false;
+    return VFX_OK;
+}
+
+// The march buffer of the last draw, after the temporal resolve when that is on: (mh, mw, 4)
+// float premultiplied radiance + transmittance, copied to dst (device); its size in *mh, *mw.
 int vfx_render_get_hdr(vfx_render* r, float* dst, int* mh, int* mw, void* stream) {
     if (!r) return arg_error("null renderer");
     if (mh) *mh = r->mh;
     if (mw) *mw = r->mw;
     if (!dst) return VFX_OK;
-    if (!hip_ok(hipMemcpyAsync(dst, r->hdr, (size_t)r->mh * r->mw * sizeof(float4), hipMemcpyDeviceToDevice,
+    if (!hip_ok(hipMemcpyAsync(dst, r->view, (size_t)r->mh * r->mw * sizeof(float4), hipMemcpyDeviceToDevice,
                                reinterpret_cast<hipStream_t>(stream)),
                 "hdr copy"))
         return VFX_E_HIP;

@@ -15,8 +15,11 @@ Per draw, on torch's current stream, no host synchronisation:
   aux (return_aux=True): the same march also accumulates the medium's opacity, the opacity-weighted
     depth and, from a velocity field and the previous camera, per-pixel motion; resolved to
     (4, out_h, out_w) FP16 at output resolution.  The frame is the colour-only draw's, bit for bit.
+  temporal (temporal=True / set_temporal): each draw offsets its ray-start jitter by a golden-ratio
+    sequence and blends the march buffer into a history reprojected by camera motion, with a 3x3
+    variance clip; bloom and composite run on the resolved buffer (temporal.py).
 
-The reference's other opt-in variants (temporal, detail) are not part of this renderer.
+The reference's detail-field variant is not part of this renderer.
 """
 from __future__ import annotations
 
@@ -106,7 +109,7 @@ def _ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
 
 class VolumeRenderer:
     def __init__(self, out_size: tuple[int, int], device: str = "cuda", march_scale: float = 1.0,
-                 params: RenderParams3D | None = None, profile: bool = False):
+                 params: RenderParams3D | None = None, profile: bool = False, temporal: bool = False):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -126,6 +129,11 @@ class VolumeRenderer:
         self._vel_key = None
         self._aux_out = None
         self.aux: dict[str, torch.Tensor] = {}
+        # temporal amortization (renderer.py:130-137): base current weight, clip width, T-change
+        # gain, and the current weight on draws with a new volume
+        self.temporal = bool(temporal)
+        self.t_alpha, self.t_gamma, self.t_kT = 0.1, 1.0, 4.0
+        self.t_alpha_tick = 0.5
         self._alloc_out()
 
     def __del__(self):
@@ -160,6 +168,14 @@ class VolumeRenderer:
 
     def set_march_scale(self, s: float) -> None:
         self.set_output((self.out_h, self.out_w), s)
+
+    def set_temporal(self, on: bool) -> None:
+        """Toggle temporal amortization; turning it on starts from an empty history."""
+        self.temporal = bool(on)
+        self.reset_history()
+
+    def reset_history(self) -> None:
+        _lib.check(self.lib.vfx_render_reset_history(self._h), "reset_history")
 
     def set_params(self, **kw) -> None:
         for k, v in kw.items():
@@ -211,6 +227,9 @@ class VolumeRenderer:
         tp, tv = ctypes.c_float(), ctypes.c_float()
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         times = (ctypes.byref(tp) if self.profile else None, ctypes.byref(tv) if self.profile else None)
+        _lib.check(self.lib.vfx_render_set_temporal(self._h, int(self.temporal), ctypes.c_float(self.t_alpha),
+                                                    ctypes.c_float(self.t_gamma), ctypes.c_float(self.t_kT),
+                                                    ctypes.c_float(self.t_alpha_tick)), "temporal")
         if not return_aux:
             _lib.check(self.lib.vfx_render_draw(self._h, _ptr(dens), _ptr(temp), D, H, W, _ptr(s_t), sD, sH, sW,
                                                 upload, cb.ctypes.data_as(ctypes.c_void_p), int(reuse_volume),
@@ -242,7 +261,8 @@ class VolumeRenderer:
         return self.out
 
     def hdr(self) -> torch.Tensor:
-        """The march buffer of the last draw: (mh, mw, 4) premultiplied radiance + transmittance."""
+        """The march buffer of the last draw (after the temporal resolve when that is on): (mh, mw, 4)
+        premultiplied radiance + transmittance."""
         out = torch.empty(self.mh, self.mw, 4, device=self.device)
         _lib.check(self.lib.vfx_render_get_hdr(self._h, _ptr(out), None, None,
                                                ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)),

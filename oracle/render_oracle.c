@@ -416,6 +416,39 @@ static uint8_t to_srgb8(float x, float dither) {
 }
 
 /* composite (kernels.py:571-600) with the Catmull-Rom upsample (:512-549) and bilinear bloom */
+/* Catmull-Rom sample of a (H, W, 4) buffer at normalized (u, v), clamped to the range of its 2x2
+ * bilinear footprint (kernels.py:512-549) */
+static void cr_sample(const float* a, int H, int W, float u, float v, float px[4]) {
+    float fx = u * (float)W - 0.5f, fy = v * (float)H - 0.5f;
+    float x0f = floorf(fx), y0f = floorf(fy);
+    float wx[4], wy[4], t;
+    t = fx - x0f;
+    wx[0] = -0.5f * t * t * t + t * t - 0.5f * t; wx[1] = 1.5f * t * t * t - 2.5f * t * t + 1.f;
+    wx[2] = -1.5f * t * t * t + 2.f * t * t + 0.5f * t; wx[3] = 0.5f * t * t * t - 0.5f * t * t;
+    t = fy - y0f;
+    wy[0] = -0.5f * t * t * t + t * t - 0.5f * t; wy[1] = 1.5f * t * t * t - 2.5f * t * t + 1.f;
+    wy[2] = -1.5f * t * t * t + 2.f * t * t + 0.5f * t; wy[3] = 0.5f * t * t * t - 0.5f * t * t;
+    int xi = (int)x0f, yi = (int)y0f;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f}, lo[4] = {1e9f, 1e9f, 1e9f, 1e9f}, hi[4] = {-1e9f, -1e9f, -1e9f, -1e9f};
+    for (int j = 0; j < 4; ++j) {
+        int yy = clampi(yi - 1 + j, 0, H - 1);
+        float row[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < 4; ++i) {
+            int xx = clampi(xi - 1 + i, 0, W - 1);
+            const float* s = a + ((size_t)yy * W + xx) * 4;
+            for (int k = 0; k < 4; ++k) {
+                row[k] = row[k] + s[k] * wx[i];
+                if ((i == 1 || i == 2) && (j == 1 || j == 2)) {
+                    lo[k] = fminf(lo[k], s[k]);
+                    hi[k] = fmaxf(hi[k], s[k]);
+                }
+            }
+        }
+        for (int k = 0; k < 4; ++k) acc[k] = acc[k] + row[k] * wy[j];
+    }
+    for (int k = 0; k < 4; ++k) px[k] = clampf_(acc[k], lo[k], hi[k]);
+}
+
 void or_render_composite(const float* hdr, int mh, int mw, const float* bloom, int bh, int bw, const float* prm,
                          int Ho, int Wo, uint8_t* out) {
     shade_t p = unpack(prm);
@@ -427,34 +460,7 @@ void or_render_composite(const float* hdr, int mh, int mw, const float* bloom, i
             if (mh == Ho && mw == Wo) {
                 for (int k = 0; k < 4; ++k) px[k] = hdr[((size_t)r * mw + c) * 4 + k];
             } else {
-                float fx = u * (float)mw - 0.5f, fy = v * (float)mh - 0.5f;
-                float x0f = floorf(fx), y0f = floorf(fy);
-                float wx[4], wy[4], t;
-                t = fx - x0f;
-                wx[0] = -0.5f * t * t * t + t * t - 0.5f * t; wx[1] = 1.5f * t * t * t - 2.5f * t * t + 1.f;
-                wx[2] = -1.5f * t * t * t + 2.f * t * t + 0.5f * t; wx[3] = 0.5f * t * t * t - 0.5f * t * t;
-                t = fy - y0f;
-                wy[0] = -0.5f * t * t * t + t * t - 0.5f * t; wy[1] = 1.5f * t * t * t - 2.5f * t * t + 1.f;
-                wy[2] = -1.5f * t * t * t + 2.f * t * t + 0.5f * t; wy[3] = 0.5f * t * t * t - 0.5f * t * t;
-                int xi = (int)x0f, yi = (int)y0f;
-                float acc[4] = {0.f, 0.f, 0.f, 0.f}, lo[4] = {1e9f, 1e9f, 1e9f, 1e9f}, hi[4] = {-1e9f, -1e9f, -1e9f, -1e9f};
-                for (int j = 0; j < 4; ++j) {
-                    int yy = clampi(yi - 1 + j, 0, mh - 1);
-                    float row[4] = {0.f, 0.f, 0.f, 0.f};
-                    for (int i = 0; i < 4; ++i) {
-                        int xx = clampi(xi - 1 + i, 0, mw - 1);
-                        const float* s = hdr + ((size_t)yy * mw + xx) * 4;
-                        for (int k = 0; k < 4; ++k) {
-                            row[k] = row[k] + s[k] * wx[i];
-                            if ((i == 1 || i == 2) && (j == 1 || j == 2)) {
-                                lo[k] = fminf(lo[k], s[k]);
-                                hi[k] = fmaxf(hi[k], s[k]);
-                            }
-                        }
-                    }
-                    for (int k = 0; k < 4; ++k) acc[k] = acc[k] + row[k] * wy[j];
-                }
-                for (int k = 0; k < 4; ++k) px[k] = clampf_(acc[k], lo[k], hi[k]);
+                cr_sample(hdr, mh, mw, u, v, px);
             }
             float vy = 1.f - v;
             float vig = 1.f - 0.6f * ((u - 0.5f) * (u - 0.5f) + (vy - 0.6f) * (vy - 0.6f));
@@ -481,5 +487,94 @@ void or_render_composite(const float* hdr, int mh, int mw, const float* bloom, i
             o[0] = to_srgb8(aces(col.x), dither);
             o[1] = to_srgb8(aces(col.y), dither);
             o[2] = to_srgb8(aces(col.z), dither);
+        }
+}
+
+/* temporal resolve (temporal.py:26-150): cur / aux / hist (h, w, 4) -> out (h, w, 4); cam: this
+ * draw's camera (16), tcam: the previous draw's camera with [14] = minimum current weight and
+ * [15] = history valid */
+static void to_c4(const float* v, float* o) {
+    float y = 0.25f * v[0] + 0.5f * v[1] + 0.25f * v[2];
+    float co = 0.5f * v[0] - 0.5f * v[2];
+    float cg = -0.25f * v[0] + 0.5f * v[1] - 0.25f * v[2];
+    float s = 1.f / (1.f + fmaxf(y, 0.f));
+    o[0] = y * s; o[1] = co * s; o[2] = cg * s; o[3] = v[3];
+}
+
+void or_render_temporal(const float* cur, const float* aux, const float* hist, int H, int W, const float* cam,
+                        const float* tcam, float alpha_base, float gamma, float k_t, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            const size_t i = (size_t)r * W + c;
+            const float* now = cur + i * 4;
+            float* o4 = out + i * 4;
+            int k;
+            if (tcam[15] < 0.5f) {
+                for (k = 0; k < 4; ++k) o4[k] = now[k];
+                continue;
+            }
+            v3 o = V(cam[0], cam[1], cam[2]);
+            float th = cam[12], aspect = cam[13];
+            float sx = (2.f * ((float)c + 0.5f) / (float)W - 1.f) * th * aspect;
+            float sy = (1.f - 2.f * ((float)r + 0.5f) / (float)H) * th;
+            v3 d = vnorm(vadd(vadd(V(cam[3], cam[4], cam[5]), vmul(V(cam[6], cam[7], cam[8]), sx)),
+                              vmul(V(cam[9], cam[10], cam[11]), sy)));
+            const float* ax = aux + i * 4;
+            float dep;
+            if (ax[0] > 0.02f) dep = ax[1] / ax[0];
+            else if (d.y < -1.0e-4f && o.y > 0.f) dep = -o.y / d.y;
+            else dep = vlen(vsub(V(0.5f, 0.5f, 0.5f), o));
+            v3 rel = vsub(vadd(o, vmul(d, dep)), V(tcam[0], tcam[1], tcam[2]));
+            float z = vdot(rel, V(tcam[3], tcam[4], tcam[5]));
+            int valid = z > 1.0e-3f;
+            float px = 0.f, py = 0.f;
+            if (valid) {
+                float psx = vdot(rel, V(tcam[6], tcam[7], tcam[8])) / (z * tcam[12] * tcam[13]);
+                float psy = vdot(rel, V(tcam[9], tcam[10], tcam[11])) / (z * tcam[12]);
+                px = (psx + 1.f) * 0.5f * (float)W;
+                py = (1.f - psy) * 0.5f * (float)H;
+                valid = px >= 0.f && px <= (float)W && py >= 0.f && py <= (float)H;
+            }
+            if (!valid) {
+                for (k = 0; k < 4; ++k) o4[k] = now[k];
+                continue;
+            }
+            float h4[4], hp[4];
+            cr_sample(hist, H, W, px / (float)W, py / (float)H, h4);
+            to_c4(h4, hp);
+            float m1[4] = {0.f, 0.f, 0.f, 0.f}, m2[4] = {0.f, 0.f, 0.f, 0.f};
+            float lo[4] = {1e9f, 1e9f, 1e9f, 1e9f}, hi[4] = {-1e9f, -1e9f, -1e9f, -1e9f};
+            for (int j = -1; j <= 1; ++j)
+                for (int q = -1; q <= 1; ++q) {
+                    float sn[4];
+                    to_c4(cur + ((size_t)clampi(r + j, 0, H - 1) * W + clampi(c + q, 0, W - 1)) * 4, sn);
+                    for (k = 0; k < 4; ++k) {
+                        m1[k] = m1[k] + sn[k];
+                        m2[k] = m2[k] + sn[k] * sn[k];
+                        lo[k] = fminf(lo[k], sn[k]);
+                        hi[k] = fmaxf(hi[k], sn[k]);
+                    }
+                }
+            float cl[4], nc[4];
+            for (k = 0; k < 4; ++k) {
+                m1[k] = m1[k] / 9.f;
+                m2[k] = m2[k] / 9.f;
+                float sig = sqrtf(fmaxf(m2[k] - m1[k] * m1[k], 0.f));
+                float a = fmaxf(lo[k], m1[k] - gamma * sig), b = fminf(hi[k], m1[k] + gamma * sig);
+                cl[k] = fminf(fmaxf(hp[k], a), b);   /* wp.clamp's min(max()) form: b wins if a > b */
+            }
+            to_c4(now, nc);
+            float dT = fabsf(nc[3] - hp[3]);
+            float a0 = fmaxf(alpha_base, tcam[14]);
+            float w = fminf(fmaxf(a0 + k_t * dT + 0.5f * k_t * (hi[3] - lo[3]) * dT, a0), 1.f);
+            float res[4];
+            for (k = 0; k < 4; ++k) res[k] = cl[k] * (1.f - w) + nc[k] * w;
+            float s = 1.f / fmaxf(1.f - res[0], 1.0e-3f);
+            float y = res[0] * s, co = res[1] * s, cg = res[2] * s;
+            o4[0] = y + co - cg;
+            o4[1] = y + cg;
+            o4[2] = y - co - cg;
+            o4[3] = res[3];
         }
 }

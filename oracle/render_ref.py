@@ -16,7 +16,10 @@ against their closed forms; the background of an empty frame against its gradien
 a uniform slab's alpha against 1 - Beer-Lambert and its depth against the closed-form weighted mean
 distance; zero motion for a still medium and an unmoved camera; uniform-velocity and moved-camera
 motion against the projection of the displaced mean point; the colour unchanged by the aux variant;
-the resolve against float64 bilinear weights.
+the resolve against float64 bilinear weights.  Temporal resolve: the first draw and disoccluded pixels
+pass the march through; a linear transmittance ramp under a camera translation against the closed
+form of the reprojected, Catmull-Rom fetched and blended history (with medium, floor and
+domain-centre depths); the variance clip of a checkerboard against its closed form.
 """
 from __future__ import annotations
 
@@ -204,6 +207,52 @@ def composite(hdr, bl, prm, out_h, out_w):
     _lib().or_render_composite(_p(hdr), hdr.shape[0], hdr.shape[1], _p(bl), bl.shape[0], bl.shape[1], _p(prm),
                                out_h, out_w, _p(out))
     return out
+
+
+GOLDEN = 0.6180339887498949
+
+
+def temporal_resolve(cur, abuf, hist, cam, tcam, alpha=0.1, gamma=1.0, k_t=4.0):
+    """temporal.py:70-150: (h, w, 4) march buffer, its premultiplied aux and the history ->
+    the resolved (h, w, 4); tcam[14] = minimum current weight, tcam[15] = history valid."""
+    cur, abuf = _f32(cur), _f32(abuf)
+    hist = _f32(hist if hist is not None else cur)
+    H, W = cur.shape[:2]
+    out = np.empty((H, W, 4), np.float32)
+    _lib().or_render_temporal(_p(cur), _p(abuf), _p(hist), H, W, _p(_f32(cam)), _p(_f32(tcam)),
+                              ctypes.c_float(alpha), ctypes.c_float(gamma), ctypes.c_float(k_t), _p(out))
+    return out
+
+
+class TemporalRender:
+    """The temporal draw sequence of one renderer (renderer.py:331-346, :465-493): per draw the
+    jitter offset (frame * golden ratio) mod 1, the march with aux, the resolve against the
+    previous draw's camera, then bloom and composite on the resolved buffer."""
+
+    def __init__(self, prm, lut, out_h, out_w, march_scale=1.0, alpha=0.1, gamma=1.0, k_t=4.0, tick=0.5):
+        self.prm, self.lut, self.out_h, self.out_w = _f32(prm), _f32(lut), out_h, out_w
+        self.mh = max(8, int(round(out_h * march_scale)))
+        self.mw = max(8, int(round(out_w * march_scale)))
+        self.alpha, self.gamma, self.k_t, self.tick = alpha, gamma, k_t, tick
+        self.frame, self.prev, self.hist = 0, None, None
+
+    def draw(self, pr, cam, reuse_volume=False):
+        """(uint8 frame, resolved buffer) for a prepared volume (prep) and a camera basis"""
+        self.frame += 1
+        cb = _f32(cam).copy()
+        cb[14] = np.float32((self.frame * GOLDEN) % 1.0)
+        tcam = np.zeros(16, np.float32)
+        if self.prev is not None:
+            tcam[:] = self.prev
+            tcam[14] = 0.0 if reuse_volume else self.tick
+            tcam[15] = 1.0
+        self.prev = cb.copy()
+        hdr, abuf = march(pr, self.prm, self.lut, cb, self.mh, self.mw,
+                          aux=dict(vel=None, cam_prev=cb, vel_dt=0.0, out_h=self.out_h, out_w=self.out_w))
+        res = temporal_resolve(hdr, abuf, self.hist, cb, tcam, self.alpha, self.gamma, self.k_t)
+        self.hist = res
+        img = composite(res, bloom(res, float(self.prm[19])), self.prm, self.out_h, self.out_w)
+        return img, res
 
 
 def render(dens, temp, solid, cam, prm, lut, out_h, out_w, march_scale=1.0, aux=None):

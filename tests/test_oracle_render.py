@@ -313,3 +313,93 @@ def test_aux_resolve_against_bilinear(shape):
     want[1:] = np.where(ok, up[1:] / np.where(ok, al, 1.0), 0.0)
     both = ok | (np.abs(al - 1e-3) > 1e-6)
     assert np.allclose(got[:, both], want[:, both], rtol=2e-3, atol=2e-3)
+
+
+# ---------------------------------------------------------------------- temporal resolve
+def _ramp_bufs(H, W, a=0.2, b=0.02, alpha=1.0, depth=1.8):
+    """rgb 0, T = a + b * column; aux = (alpha, alpha depth, 0, 0)"""
+    cur = np.zeros((H, W, 4), np.float32)
+    cur[..., 3] = a + b * np.arange(W, dtype=np.float32)[None, :]
+    aux = np.zeros((H, W, 4), np.float32)
+    aux[..., 0] = alpha
+    aux[..., 1] = alpha * depth
+    return cur, aux
+
+
+def test_temporal_passes_through_without_history_or_on_disocclusion():
+    H, W = 20, 28
+    cam = camera_basis(CAM, W / H)
+    rng = np.random.default_rng(3)
+    cur = rng.random((H, W, 4), dtype=np.float32)
+    aux = np.zeros((H, W, 4), np.float32)
+    aux[..., 0], aux[..., 1] = 1.0, 1.5
+    hist = rng.random((H, W, 4), dtype=np.float32)
+    t0 = np.zeros(16, np.float32)                                  # history not valid
+    assert np.array_equal(R.temporal_resolve(cur, aux, hist, cam, t0), cur)
+    away = camera_basis(Camera(position=(0.5, 0.5, 3.0), target=(0.5, 0.5, 5.0)), W / H)
+    away[15] = 1.0                                                 # valid, but every point is behind it
+    assert np.array_equal(R.temporal_resolve(cur, aux, hist, cam, away), cur)
+
+
+@pytest.mark.parametrize("case", ["medium", "floor", "centre"])
+def test_temporal_reprojects_a_ramp_under_camera_translation(case):
+    """A previous camera moved by -delta along the right axis sees every point delta W / (2 z
+    tan(fov/2) aspect) pixels further right (z: the point's depth along the view axis, at the
+    medium's mean depth, the floor hit or the distance to the domain centre).  A linear T ramp is
+    reproduced exactly by the Catmull-Rom fetch and stays inside the 3x3 variance clip (mean +-
+    sqrt(2/3) b), so the resolved T = a + b c + (1 - w) b s with w = alpha + k_t |b s| (1 + b)."""
+    H, W, a, b, alpha, k_t = 24, 32, 0.2, 0.02, 0.1, 4.0
+    camera = {"medium": CAM, "floor": Camera(position=(0.5, 0.8, -1.0), target=(0.5, 0.0, 0.5), fov_y_deg=30.0),
+              "centre": Camera(position=(0.5, 0.2, -1.5), target=(0.5, 0.9, 0.5), fov_y_deg=30.0)}[case]
+    cam = camera_basis(camera, W / H)
+    o, d = _rays(cam, H, W)
+    depth = 1.8
+    cur, aux = _ramp_bufs(H, W, a, b, alpha=1.0 if case == "medium" else 0.0, depth=depth)
+    if case == "medium":
+        dep = np.full((H, W), depth)
+    elif case == "floor":
+        assert np.all(d[..., 1] < -1e-4)
+        dep = -o[1] / d[..., 1]
+    else:
+        assert np.all(d[..., 1] > 0)
+        dep = np.full((H, W), np.linalg.norm(np.array([0.5, 0.5, 0.5]) - o))
+    z = dep * (d @ cam[3:6].astype(np.float64))
+    th, asp = float(cam[12]), float(cam[13])
+    delta = 0.6 * 2.0 * z.min() * th * asp / W                    # |s| <= 0.6 px everywhere
+    s = delta * W / (2.0 * z * th * asp)
+    prev = cam.copy()
+    prev[0:3] = cam[0:3] - delta * cam[6:9]
+    prev[14], prev[15] = 0.0, 1.0
+    got = R.temporal_resolve(cur, aux, cur.copy(), cam, prev, alpha=alpha, gamma=1.0, k_t=k_t)
+    col = np.arange(W)[None, :]
+    w = np.clip(alpha + k_t * np.abs(b * s) * (1.0 + b), alpha, 1.0)
+    want = a + b * col + (1.0 - w) * b * s
+    inner = (slice(2, H - 2), slice(3, W - 3))                     # away from the clamped borders
+    assert np.abs(got[..., 3][inner] - want[inner]).max() < 2e-5
+    assert np.abs(got[..., :3]).max() == 0.0
+    assert s[inner].min() > 0.1                                    # the history really moved
+
+
+def test_temporal_variance_clip_of_a_checkerboard():
+    """T = 0.5 +- e in a checkerboard: the 3x3 neighbourhood holds 5 of the centre's value and 4 of
+    the other, mean 0.5 + p e / 9, sigma e sqrt(80) / 9, range [0.5 - e, 0.5 + e]; a history of
+    0.95 is clipped to min(0.5 + e, mean + sigma) and blended with w = alpha + k_t dT (1 + e)"""
+    H, W, e, alpha, k_t = 16, 20, 0.1, 0.1, 0.5
+    cam = camera_basis(CAM, W / H)
+    p = np.where((np.arange(H)[:, None] + np.arange(W)[None, :]) % 2 == 0, 1.0, -1.0)
+    cur = np.zeros((H, W, 4), np.float32)
+    cur[..., 3] = 0.5 + e * p
+    aux = np.zeros((H, W, 4), np.float32)
+    aux[..., 0], aux[..., 1] = 1.0, 1.6
+    hist = np.zeros((H, W, 4), np.float32)
+    hist[..., 3] = 0.95
+    tc = cam.copy()
+    tc[14], tc[15] = 0.0, 1.0
+    got = R.temporal_resolve(cur, aux, hist, cam, tc, alpha=alpha, gamma=1.0, k_t=k_t)[..., 3]
+    m1 = 0.5 + p * e / 9.0
+    clip = np.minimum(0.5 + e, m1 + e * math.sqrt(80.0) / 9.0)
+    dT = np.abs(0.5 + e * p - 0.95)
+    w = np.clip(alpha + k_t * dT + 0.5 * k_t * (2 * e) * dT, alpha, 1.0)
+    want = (1.0 - w) * clip + w * (0.5 + e * p)
+    inner = (slice(1, H - 1), slice(1, W - 1))
+    assert np.abs(got[inner] - want[inner]).max() < 1e-5
