@@ -12,31 +12,6 @@
 
 namespace vfx {
 
-#ifndef ADV_ORDER
-#define ADV_ORDER 0
-#endif
-
-// The cell block of this workgroup: (bx, by, z) and its logical index (x fastest, then y, then z).
-struct AdvBlock {
-    int x, y, z, lin;
-};
-__device__ __forceinline__ AdvBlock adv_block(int N) {
-    AdvBlock b;
-#if ADV_ORDER == 1
-    const int gx = (N + (int)blockDim.x - 1) / (int)blockDim.x, gy = (N + (int)blockDim.y - 1) / (int)blockDim.y;
-    b.lin = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    b.x = b.lin % gx;
-    b.y = (b.lin / gx) % gy;
-    b.z = b.lin / (gx * gy);
-#else
-    b.x = blockIdx.x;
-    b.y = blockIdx.y;
-    b.z = blockIdx.z;
-    b.lin = block_linear();
-#endif
-    return b;
-}
-
 // Pass 1: semi-Lagrangian backward value of C channels -> hat, and the base corner of the
 // backward trace -> qidx (pass 2's clamp reads the same corners; the trace is not repeated).
 template <int C>
@@ -44,10 +19,9 @@ __global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ 
                                                 const float* __restrict__ prm, real inv_dx, real* __restrict__ hat,
                                                 int* __restrict__ qidx) {
     const int n3 = N * N * N;
-    const AdvBlock ab = adv_block(N);
-    const int x = ab.x * blockDim.x + threadIdx.x;
-    const int y = ab.y * blockDim.y + threadIdx.y;
-    const int z = ab.z;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
     if (x >= N || y >= N) return;
     const real s = (real)prm[P_DT] * inv_dx;
     const int i = (z * N + y) * N + x;
@@ -74,10 +48,9 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
                                                 double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
     const int n3 = N * N * N;
-    const AdvBlock ab = adv_block(N);
-    const int x = ab.x * blockDim.x + threadIdx.x;
-    const int y = ab.y * blockDim.y + threadIdx.y;
-    const int z = ab.z;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
     const bool live = x < N && y < N;
     double sa = 0.0, sb = 0.0;
     if (live) {
@@ -131,7 +104,7 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
         const double a = block_sum(sa, sh);
         const double b = block_sum(sb, sh);
         if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const int bl = ab.lin;
+            const int bl = block_linear();
             part[bl] = a;
             part[nparts + bl] = b;
         }
@@ -166,11 +139,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mass_fin(const double* __restri
     if (threadIdx.x == 0) st->s[slot] = (a > 0.0 && b > 1e-30) ? a / b : 1.0;
 }
 
-#if ADV_ORDER == 1
-static dim3 cell_grid(const Geom& g) { return dim3(g.grid.x * g.grid.y * (unsigned)g.N, 1, 1); }
-#else
 static dim3 cell_grid(const Geom& g) { return dim3(g.grid.x, g.grid.y, (unsigned)g.N); }
-#endif
 
 int advect_parts(const Geom& g) { return (int)(g.grid.x * g.grid.y) * g.N; }
 
