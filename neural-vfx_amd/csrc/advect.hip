@@ -7,13 +7,39 @@
 // One cell per thread, channel count a template parameter: every gather of a trace step (3
 // channels x 8 corners) and of the final samples (C channels x 8 corners) is loaded before any
 // of it is used, so each RK2 trace costs three dependent memory rounds, not one per load.
+//
+// Both passes are bound by the address rate of their gathers (one vector memory instruction costs
+// the same for 4 to 16 bytes per lane), so the backward values travel between the passes as
+// channel pairs: plane k of hat holds (channel 2k, channel 2k + 1) of every cell as two adjacent
+// values, and the MacCormack pass reads both channels of a corner pair with one 16-byte load.
 #include "sim3d_kernels.hpp"
 #include "trilinear.hpp"
 
+#ifndef MC_PLANES
+#define MC_PLANES 1
+#endif
+
 namespace vfx {
 
-// Pass 1: semi-Lagrangian backward value of C channels -> hat, and the base corner of the
-// backward trace -> qidx (pass 2's clamp reads the same corners; the trace is not repeated).
+// four values at 2-value alignment (two cells of one channel-pair plane)
+typedef real real4_e __attribute__((ext_vector_type(4), aligned(2 * sizeof(real))));
+
+// the 8 corners of channels 2k and 2k + 1 from plane `pl` (pair layout) at corner o
+__device__ __forceinline__ void gather_pair(const real* __restrict__ pl, int N, int o, Oct& a, Oct& b) {
+    const int sy = N, sz = N * N;
+    const real4_e r0 = *reinterpret_cast<const real4_e*>(pl + 2 * o);
+    const real4_e r1 = *reinterpret_cast<const real4_e*>(pl + 2 * (o + sy));
+    const real4_e r2 = *reinterpret_cast<const real4_e*>(pl + 2 * (o + sz));
+    const real4_e r3 = *reinterpret_cast<const real4_e*>(pl + 2 * (o + sz + sy));
+    a.v[0] = r0.x; a.v[1] = r0.z; b.v[0] = r0.y; b.v[1] = r0.w;
+    a.v[2] = r1.x; a.v[3] = r1.z; b.v[2] = r1.y; b.v[3] = r1.w;
+    a.v[4] = r2.x; a.v[5] = r2.z; b.v[4] = r2.y; b.v[5] = r2.w;
+    a.v[6] = r3.x; a.v[7] = r3.z; b.v[6] = r3.y; b.v[7] = r3.w;
+}
+
+// Pass 1: semi-Lagrangian backward value of C channels -> hat (channel-pair planes, the second
+// value of an odd C's last pair 0), and the base corner of the backward trace -> qidx (pass 2's
+// clamp reads the same corners; the trace is not repeated).  hat holds 2 ceil(C / 2) n3 values.
 template <int C>
 __global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ src, const real* __restrict__ vel,
                                                 const float* __restrict__ prm, real inv_dx, real* __restrict__ hat,
@@ -32,8 +58,17 @@ __global__ void __launch_bounds__(256) k_adv_sl(int N, const real* __restrict__ 
     Oct g[C];
 #pragma unroll
     for (int ch = 0; ch < C; ++ch) g[ch] = gather(src + ch * n3, N, c.o);
+    real v[C + 1];
 #pragma unroll
-    for (int ch = 0; ch < C; ++ch) hat[ch * n3 + i] = lerp3(g[ch], c);
+    for (int ch = 0; ch < C; ++ch) v[ch] = lerp3(g[ch], c);
+    v[C] = RL(0.0);
+#pragma unroll
+    for (int k = 0; 2 * k < C; ++k) {
+        real2_e w;
+        w.x = v[2 * k];
+        w.y = v[2 * k + 1];
+        *reinterpret_cast<real2_e*>(hat + 2 * ((size_t)k * n3 + i)) = w;
+    }
 }
 
 // Pass 2: MacCormack correction with the 8-corner clamp around the backward trace's base corner
@@ -61,24 +96,32 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
         trace1(vel, N, n3, i, (real)x, (real)y, (real)z, s, r);
         const Corner cr = corner(N, r[0], r[1], r[2]);
         const int fmi = part != nullptr ? (int)fm[i] : 0;
-        // channels in groups of MC_GROUP: the loads of one group are in flight together, and a
-        // group's registers are free before the next group loads (more waves per SIMD)
-        constexpr int MC_GROUP = C > 3 ? 3 : C;
+        // channel pairs in groups of MC_PLANES planes: the loads of one group are in flight
+        // together, and a group's registers are free before the next group loads
+        constexpr int NP = (C + 1) / 2;
 #pragma unroll
-        for (int c0 = 0; c0 < C; c0 += MC_GROUP) {
-        Oct gs[MC_GROUP], gh[MC_GROUP];
-        real h[MC_GROUP], s0[MC_GROUP];
+        for (int p0 = 0; p0 < NP; p0 += MC_PLANES) {
+        Oct gs[2 * MC_PLANES], gh[2 * MC_PLANES];
+        real h[2 * MC_PLANES], s0[2 * MC_PLANES];
 #pragma unroll
-        for (int k = 0; k < MC_GROUP; ++k) {
-            const int ch = c0 + k < C ? c0 + k : C - 1;
-            gs[k] = gather(src + ch * n3, N, qo);
-            gh[k] = gather(hat + ch * n3, N, cr.o);
-            h[k] = hat[ch * n3 + i];
-            s0[k] = src[ch * n3 + i];
+        for (int k = 0; k < MC_PLANES; ++k) {
+            const int pl = p0 + k < NP ? p0 + k : NP - 1;
+            const real* hp = hat + 2 * (size_t)pl * n3;
+            gather_pair(hp, N, cr.o, gh[2 * k], gh[2 * k + 1]);
+            const real2_e hv = *reinterpret_cast<const real2_e*>(hp + 2 * i);
+            h[2 * k] = hv.x;
+            h[2 * k + 1] = hv.y;
+            const int c0 = 2 * pl, c1 = 2 * pl + 1 < C ? 2 * pl + 1 : 2 * pl;
+            gs[2 * k] = gather(src + c0 * n3, N, qo);
+            s0[2 * k] = src[c0 * n3 + i];
+            if (2 * pl + 1 < C) {
+                gs[2 * k + 1] = gather(src + c1 * n3, N, qo);
+                s0[2 * k + 1] = src[c1 * n3 + i];
+            }
         }
 #pragma unroll
-        for (int k = 0; k < MC_GROUP; ++k) {
-            const int ch = c0 + k;
+        for (int k = 0; k < 2 * MC_PLANES; ++k) {
+            const int ch = 2 * p0 + k;
             if (ch >= C) break;
             const real bar = lerp3(gh[k], cr);
             real val = h[k] + RL(0.5) * (s0[k] - bar);
