@@ -21,22 +21,6 @@
 
 namespace vfx {
 
-// four values at 2-value alignment (two cells of one channel-pair plane)
-typedef real real4_e __attribute__((ext_vector_type(4), aligned(2 * sizeof(real))));
-
-// the 8 corners of channels 2k and 2k + 1 from plane `pl` (pair layout) at corner o
-__device__ __forceinline__ void gather_pair(const real* __restrict__ pl, int N, int o, Oct& a, Oct& b) {
-    const int sy = N, sz = N * N;
-    const real4_e r0 = *reinterpret_cast<const real4_e*>(pl + 2 * o);
-    const real4_e r1 = *reinterpret_cast<const real4_e*>(pl + 2 * (o + sy));
-    const real4_e r2 = *reinterpret_cast<const real4_e*>(pl + 2 * (o + sz));
-    const real4_e r3 = *reinterpret_cast<const real4_e*>(pl + 2 * (o + sz + sy));
-    a.v[0] = r0.x; a.v[1] = r0.z; b.v[0] = r0.y; b.v[1] = r0.w;
-    a.v[2] = r1.x; a.v[3] = r1.z; b.v[2] = r1.y; b.v[3] = r1.w;
-    a.v[4] = r2.x; a.v[5] = r2.z; b.v[4] = r2.y; b.v[5] = r2.w;
-    a.v[6] = r3.x; a.v[7] = r3.z; b.v[6] = r3.y; b.v[7] = r3.w;
-}
-
 // Pass 1: semi-Lagrangian backward value of C channels -> hat (channel-pair planes, the second
 // value of an odd C's last pair 0), and the base corner of the backward trace -> qidx (pass 2's
 // clamp reads the same corners; the trace is not repeated).  hat holds 2 ceil(C / 2) n3 values.

@@ -33,7 +33,8 @@ __device__ __forceinline__ real to_fine(real pc, real a, real ia, real b) {
 }
 
 // Pass 1 (advect_sl_fine): the backward trace through the coarse velocity from the fine cell's
-// centre, mapped to fine index space; the semi-Lagrangian value of the 3 channels -> hat, the
+// centre, mapped to fine index space; the semi-Lagrangian value of the 3 channels -> hat (channels
+// 0 and 1 as one pair plane, channel 2 after it, for the pair gathers of pass 2), the
 // base corner -> qidx (pass 2's clamp), and the fine fluid flag (solid <= 0.5) -> ffm for the
 // later passes.
 template <bool POW2>
@@ -59,8 +60,12 @@ __global__ void __launch_bounds__(256) k_fadv_sl(int S, int N, const real* __res
     for (int ch = 0; ch < 3; ++ch) g[ch] = gather(src + ch * n3, S, c.o);
     qidx[i] = c.o;
     ffm[i] = so > RL(0.5) ? 0 : 1;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) hat[ch * n3 + i] = lerp3(g[ch], c);
+    // hat: channels 0 and 1 as one pair plane (2 n3 values), channel 2 alone after it
+    real2_e w;
+    w.x = lerp3(g[0], c);
+    w.y = lerp3(g[1], c);
+    *reinterpret_cast<real2_e*>(hat + 2 * (size_t)i) = w;
+    hat[2 * n3 + i] = lerp3(g[2], c);
 }
 
 // Pass 2 (advect_mc_fine): MacCormack correction with the 8-corner clamp around the backward
@@ -90,11 +95,15 @@ __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __res
                                     to_fine<POW2>(rc[2], a, ia, b));
         Oct gs[3], gh[3];
         real h[3], s0[3];
+        gather_pair(hat, S, cr.o, gh[0], gh[1]);
+        gh[2] = gather(hat + 2 * n3, S, cr.o);
+        const real2_e hp = *reinterpret_cast<const real2_e*>(hat + 2 * (size_t)i);
+        h[0] = hp.x;
+        h[1] = hp.y;
+        h[2] = hat[2 * n3 + i];
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             gs[ch] = gather(src + ch * n3, S, qo);
-            gh[ch] = gather(hat + ch * n3, S, cr.o);
-            h[ch] = hat[ch * n3 + i];
             s0[ch] = src[ch * n3 + i];
         }
         real v0 = RL(0.0);

@@ -47,10 +47,12 @@ FIRE = Forces3D(emitters=[Emitter3D(pos=(0.5, 0.12, 0.5), radius=0.09, density_r
                 wind=(0.2, 0.0, 0.1))
 
 
-@pytest.mark.parametrize("N", [32, 48])
-def test_advect_parity(cuda, hiplib, N):
-    _say(f"advect parity N={N}")
-    f = np.concatenate([smooth_noise(N, 2, 11), np.abs(smooth_noise(N, 1, 12))]).astype(np.float32)
+@pytest.mark.parametrize("N,C", [(32, 3), (48, 3), (32, 1), (32, 4), (32, 5), (32, 6)])
+def test_advect_parity(cuda, hiplib, N, C):
+    """every channel count: the backward values travel between the passes as channel pairs, the
+    last pair of an odd count padded"""
+    _say(f"advect parity N={N} C={C}")
+    f = np.concatenate([smooth_noise(N, 2, 11), np.abs(smooth_noise(N, 4, 12))]).astype(np.float32)[:C]
     v = smooth_noise(N, 3, 13, amp=3.0)
     dt = 1.0 / 30.0
     s = Solver3D(N)
