@@ -32,6 +32,68 @@ __device__ __forceinline__ real to_fine(real pc, real a, real ia, real b) {
     else return (pc - b) / a;
 }
 
+// The first sample of a fine cell's trace is at a fixed coarse point (p_c = a p_f + b), so the
+// coarse velocity a workgroup's first samples read is known before the trace: its corner range
+// per axis runs from the first cell's base corner to the last cell's plus one (base_idx is
+// monotonic).  It is staged in LDS with coalesced row loads, and the first sample reads its
+// 8 corners x 3 channels there instead of with 12 gathers.  VT_*: the largest range a <= 1
+// (S >= res) needs for the fine workgroups (make_geom: 64 x 4 or 32 x 8 cells, one plane): at
+// most blockDim + 1 corners per axis.
+constexpr int VT_X = 66, VT_Y = 10, VT_Z = 2, VT_N = VT_X * VT_Y * VT_Z;
+
+struct VelTile {
+    int x0, y0, z0, nx, ny;
+};
+
+// every thread of the workgroup calls this (it ends with a barrier)
+__device__ __forceinline__ VelTile stage_vel(const real* __restrict__ vel, int N, int S, real a, real b,
+                                             real* __restrict__ vt) {
+    const int fx0 = blockIdx.x * blockDim.x, fy0 = blockIdx.y * blockDim.y, fz = blockIdx.z;
+    const int fx1 = min(fx0 + (int)blockDim.x, S) - 1, fy1 = min(fy0 + (int)blockDim.y, S) - 1;
+    VelTile t;
+    t.x0 = base_idx(a * (real)fx0 + b, N);
+    t.y0 = base_idx(a * (real)fy0 + b, N);
+    t.z0 = base_idx(a * (real)fz + b, N);
+    t.nx = base_idx(a * (real)fx1 + b, N) + 2 - t.x0;
+    t.ny = base_idx(a * (real)fy1 + b, N) + 2 - t.y0;
+    const int n3 = N * N * N, per = t.nx * t.ny * VT_Z;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+    for (int k = tid; k < 3 * per; k += nt) {
+        const int ch = k / per, r = k - ch * per;
+        const int lz = r / (t.nx * t.ny), r2 = r - lz * t.nx * t.ny, ly = r2 / t.nx, lx = r2 - ly * t.nx;
+        vt[ch * VT_N + (lz * VT_Y + ly) * VT_X + lx] = vel[ch * n3 + ((t.z0 + lz) * N + t.y0 + ly) * N + t.x0 + lx];
+    }
+    __syncthreads();
+    return t;
+}
+
+// trace_pt (kernels.py:115-120) with the first velocity sample from the staged tile (the same
+// corner values as sample_vel gathers, so the same result)
+__device__ __forceinline__ void trace_pt_tile(const real* __restrict__ vel, int N, int n3, const real* __restrict__ vt,
+                                              const VelTile& t, real px, real py, real pz, real s, real* q) {
+    const Corner c0 = corner(N, px, py, pz);
+    const int x0 = c0.o % N, r = c0.o / N, y0 = r % N, z0 = r / N;
+    const int l = ((z0 - t.z0) * VT_Y + (y0 - t.y0)) * VT_X + (x0 - t.x0);
+    real v1[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const real* p = vt + ch * VT_N + l;
+        Oct g;
+        g.v[0] = p[0]; g.v[1] = p[1];
+        g.v[2] = p[VT_X]; g.v[3] = p[VT_X + 1];
+        g.v[4] = p[VT_X * VT_Y]; g.v[5] = p[VT_X * VT_Y + 1];
+        g.v[6] = p[VT_X * VT_Y + VT_X]; g.v[7] = p[VT_X * VT_Y + VT_X + 1];
+        v1[ch] = lerp3(g, c0);
+    }
+    const real h = RL(0.5) * s;
+    const Corner c1 = corner(N, px + h * v1[0], py + h * v1[1], pz + h * v1[2]);
+    real v2[3];
+    sample_vel(vel, N, n3, c1, v2);
+    q[0] = px + s * v2[0];
+    q[1] = py + s * v2[1];
+    q[2] = pz + s * v2[2];
+}
+
 // Pass 1 (advect_sl_fine): the backward trace through the coarse velocity from the fine cell's
 // centre, mapped to fine index space; the semi-Lagrangian value of the 3 channels -> hat (channels
 // 0 and 1 as one pair plane, channel 2 after it, for the pair gathers of pass 2), the
@@ -43,16 +105,18 @@ __global__ void __launch_bounds__(256) k_fadv_sl(int S, int N, const real* __res
                                                  const float* __restrict__ prm, real inv_dx, real a, real ia, real b,
                                                  real* __restrict__ hat, int* __restrict__ qidx,
                                                  uint8_t* __restrict__ ffm) {
+    __shared__ real vt[3 * VT_N];
     const size_t n3 = (size_t)S * S * S;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z;
+    const VelTile t = stage_vel(vel, N, S, a, b, vt);
     if (x >= S || y >= S) return;
     const int i = (z * S + y) * S + x;
     const real s = (real)prm[P_DT] * inv_dx;
     const real so = hsolid[i];
     real qc[3];
-    trace_pt(vel, N, N * N * N, a * (real)x + b, a * (real)y + b, a * (real)z + b, -s, qc);
+    trace_pt_tile(vel, N, N * N * N, vt, t, a * (real)x + b, a * (real)y + b, a * (real)z + b, -s, qc);
     const Corner c = corner(S, to_fine<POW2>(qc[0], a, ia, b), to_fine<POW2>(qc[1], a, ia, b),
                             to_fine<POW2>(qc[2], a, ia, b));
     Oct g[3];
@@ -79,10 +143,12 @@ __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __res
                                                  const int* __restrict__ qidx, const uint8_t* __restrict__ ffm,
                                                  real* __restrict__ out, double* __restrict__ part, int nparts) {
     __shared__ double sh[8];
+    __shared__ real vt[3 * VT_N];
     const size_t n3 = (size_t)S * S * S;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z;
+    const VelTile t = stage_vel(vel, N, S, a, b, vt);
     double sa = 0.0, sb = 0.0;
     if (x < S && y < S) {
         const int i = (z * S + y) * S + x;
@@ -90,7 +156,7 @@ __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __res
         const int qo = min(max(qidx[i], 0), S * S * S - S * S - S - 2);
         const int fl = ffm[i];
         real rc[3];
-        trace_pt(vel, N, N * N * N, a * (real)x + b, a * (real)y + b, a * (real)z + b, s, rc);
+        trace_pt_tile(vel, N, N * N * N, vt, t, a * (real)x + b, a * (real)y + b, a * (real)z + b, s, rc);
         const Corner cr = corner(S, to_fine<POW2>(rc[0], a, ia, b), to_fine<POW2>(rc[1], a, ia, b),
                                     to_fine<POW2>(rc[2], a, ia, b));
         Oct gs[3], gh[3];
