@@ -495,6 +495,10 @@ int VFX_FN(vfx_hires_attach)(vfx_ctx* c, int S, real* hi, const real* hsolid) {
     if (h.S != S) {
         hires_free(c);
         const Geom g = make_geom(S, 1);
+        // the fine passes stage the coarse velocity of one workgroup's first samples in a fixed
+        // LDS tile, which holds blockDim + 1 corners per axis (S >= res)
+        if ((int)g.block.x + 1 > VT_X || (int)g.block.y + 1 > VT_Y || (int)g.block.z != 1)
+            return arg_error("fine workgroup shape exceeds the velocity tile");
         const size_t n3 = g.n3;
         const size_t nd = mass_part_doubles(g);
         const size_t bytes = nd * sizeof(double) + 6 * n3 * sizeof(real) + n3 * sizeof(int) + n3;
