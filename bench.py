@@ -34,8 +34,8 @@ COPY_PROBE_BYTES = 256 << 20   # device copy of known size that scales the PMC b
 KERNEL_BYTES_PER_CELL = {
     "adv_sl": 24 + 24 + 4,              # data (6 ch, velocity 3 of them) -> hat (6 ch), trace corner
     "adv_mc": 24 + 24 + 4 + 1 + 24,     # data + hat + trace corner + mask -> adv
-    "sources": 24 + 1 + 24,             # adv + mask -> hat
-    "diffuse": 24 + 1 + 24,             # hat + mask -> data
+    "sources": 24 + 1 + 24,             # adv + mask -> data (density, velocity), hat (T, fuel)
+    "diffuse": 8 + 1 + 8,               # hat (T, fuel) + mask -> data (T, fuel)
     "upd_matvec": 4 * 3 + 1 + 4 * 3,    # z, p_old, phi + mask -> p, phi, q
     "inv_post": 4 + 4 + 1 + 4,          # level-1 x, r + mask -> z
     "pre_resid_fwd": 4 + 4 + 1 + 4 + 4 + 2 + 2,   # r, q + mask -> r, level-1 r; red half: diag -> x

@@ -9,13 +9,17 @@
 
 namespace vfx {
 
-// sources (kernels.py:182-250).  VEL_ONLY (with the finer scalar grid): only the velocity
-// channels 1..3 are written; density, temperature and fuel come from the fine grid's restriction,
-// which overwrites them before anything reads them (solver.py:443-445).
+// sources (kernels.py:182-250).  Channels 0..3 (density, velocity) go to `out`, temperature and fuel
+// to `out_tf` (channels 4, 5 of that buffer): the diffusion that follows changes only those two,
+// so density and velocity are written to their final place here instead of being copied through.
+// VEL_ONLY (with the finer scalar grid): only the velocity channels 1..3 are written; density,
+// temperature and fuel come from the fine grid's restriction, which overwrites them before
+// anything reads them (solver.py:443-445).
 template <bool VEL_ONLY>
 __global__ void __launch_bounds__(256) k_sources(int N, int zc, const real* __restrict__ adv,
                                                  const uint8_t* __restrict__ fm, const float* __restrict__ prm,
-                                                 const DevState* __restrict__ st, real* __restrict__ out) {
+                                                 const DevState* __restrict__ st, real* __restrict__ out,
+                                                 real* __restrict__ out_tf) {
     const size_t n3 = (size_t)N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -29,7 +33,7 @@ __global__ void __launch_bounds__(256) k_sources(int N, int zc, const real* __re
         if (fm[i] == 0) {
             out[n3 + i] = RL(0.0); out[2 * n3 + i] = RL(0.0); out[3 * n3 + i] = RL(0.0);
             if (!VEL_ONLY) {
-                out[i] = RL(0.0); out[4 * n3 + i] = k.amb; out[5 * n3 + i] = RL(0.0);
+                out[i] = RL(0.0); out_tf[4 * n3 + i] = k.amb; out_tf[5 * n3 + i] = RL(0.0);
             }
             continue;
         }
@@ -47,14 +51,15 @@ __global__ void __launch_bounds__(256) k_sources(int N, int zc, const real* __re
         out[3 * n3 + i] = vz;
         if (!VEL_ONLY) {
             out[i] = d;
-            out[4 * n3 + i] = T;
-            out[5 * n3 + i] = Fu;
+            out_tf[4 * n3 + i] = T;
+            out_tf[5 * n3 + i] = Fu;
         }
     }
 }
 
-// diffuse_heat_fuel (kernels.py:268-286): channels 0..3 copied, T and F one explicit substep with
-// zero flux at walls and solids (a missing or solid neighbour contributes the cell's own value).
+// diffuse_heat_fuel (kernels.py:268-286): T and F (channels 4, 5) one explicit substep with zero
+// flux at walls and solids (a missing or solid neighbour contributes the cell's own value).
+// Channels 0..3, which the reference copies through, are already in place (k_sources).
 __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const real* __restrict__ src,
                                                  const uint8_t* __restrict__ fm, const float* __restrict__ prm,
                                                  real nsub, real* __restrict__ dst) {
@@ -69,7 +74,6 @@ __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const real* __re
     for (int z = blockIdx.z * zc; z < z1; ++z) {
         const int i = (z * N + y) * N + x;
         const Nb6 n = nb6(N, x, y, z, i);
-        const real c0 = src[i], c1 = src[n3 + i], c2 = src[2 * n3 + i], c3 = src[3 * n3 + i];
         const int m0 = fm[i], mxm = fm[n.xm], mxp = fm[n.xp], mym = fm[n.ym], myp = fm[n.yp], mzm = fm[n.zm],
                   mzp = fm[n.zp];
         const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
@@ -83,10 +87,6 @@ __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const real* __re
                             (szp ? v : a5);
             out[cc] = (m0 == 0 || lam <= RL(0.0)) ? v : v + lam * (s - RL(6.0) * v);
         }
-        dst[i] = c0;
-        dst[n3 + i] = c1;
-        dst[2 * n3 + i] = c2;
-        dst[3 * n3 + i] = c3;
         dst[4 * n3 + i] = out[0];
         dst[5 * n3 + i] = out[1];
     }
@@ -330,7 +330,7 @@ void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, c
     const int nd = g.N > 256 ? (g.N + 255) / 256 : 1;
     real* b0 = nd % 2 == 0 ? data : hat;
     real* b1 = nd % 2 == 0 ? hat : data;
-    hipLaunchKernelGGL(k_sources<false>, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, st, b0);
+    hipLaunchKernelGGL(k_sources<false>, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, st, data, b0);
     for (int k = 0; k < nd; ++k) {
         real* a = k % 2 == 0 ? b0 : b1;
         real* b = k % 2 == 0 ? b1 : b0;
@@ -340,13 +340,13 @@ void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, c
 
 void launch_sources_vel(const Geom& g, const real* adv, const uint8_t* fm, const float* prm, real* data,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_sources<true>, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, nullptr, data);
+    hipLaunchKernelGGL(k_sources<true>, g.grid, g.block, 0, s, g.N, g.zc, adv, fm, prm, nullptr, data, nullptr);
 }
 
 void launch_sources_pass(int pass, const Geom& g, const real* src, const uint8_t* fm, const float* prm,
                          const DevState* st, real* dst, hipStream_t s) {
     if (pass == 0)
-        hipLaunchKernelGGL(k_sources<false>, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, st, dst);
+        hipLaunchKernelGGL(k_sources<false>, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, st, dst, dst);
     else
         hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, RL(1.0), dst);
 }
