@@ -53,7 +53,7 @@ def fine_kernel_bytes(name: str, N: int, S: int) -> float:
     return {"fadv_sl": f * (12 + 4 + 12 + 4 + 1) + 12 * c,      # hi, solid -> hat, trace corner, flag
             "fadv_mc": f * (12 + 12 + 4 + 1 + 12) + 12 * c,     # hi, hat, corner, flag -> adv
             "fsources": f * (12 + 1 + 12),                      # adv, flag -> hat / hi
-            "fdiffuse": f * (12 + 1 + 12),                      # one substep
+            "fdiffuse": f * (8 + 1 + 8),                        # one substep: T, fuel + flag -> T, fuel
             "frestrict": f * 12 + c * (1 + 12)}.get(name, 0.0)  # hi, coarse mask -> 3 coarse channels
 
 

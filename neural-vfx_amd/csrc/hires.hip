@@ -203,9 +203,11 @@ __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __res
 
 // sources_fine: density (times the fine mass ratio), temperature and fuel at the fine cell
 // centre; 0 / ambient / 0 in fine solids.  Loads are issued before the solid test selects.
+// Density goes to out_d (its final place: the diffusion that follows changes only temperature and
+// fuel), temperature and fuel to channels 1, 2 of `out`.
 __global__ void __launch_bounds__(256) k_fsources(int S, const real* __restrict__ adv, const uint8_t* __restrict__ ffm,
                                                   const float* __restrict__ prm, const DevState* __restrict__ st,
-                                                  real dxf, real* __restrict__ out) {
+                                                  real dxf, real* __restrict__ out_d, real* __restrict__ out) {
     const size_t n3 = (size_t)S * S * S;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -216,7 +218,7 @@ __global__ void __launch_bounds__(256) k_fsources(int S, const real* __restrict_
     const real a0 = adv[i], a1 = adv[n3 + i], a2 = adv[2 * n3 + i];
     const SrcConst k(prm);
     if (fl == 0) {
-        out[i] = RL(0.0);
+        out_d[i] = RL(0.0);
         out[n3 + i] = k.amb;
         out[2 * n3 + i] = RL(0.0);
         return;
@@ -224,13 +226,14 @@ __global__ void __launch_bounds__(256) k_fsources(int S, const real* __restrict_
     const real px = ((real)x + RL(0.5)) * dxf, py = ((real)y + RL(0.5)) * dxf, pz = ((real)z + RL(0.5)) * dxf;
     real d = a0 * (real)st->s[S_MASS_RATIO_F], T = a1, Fu = a2, vx = RL(0.0), vy = RL(0.0), vz = RL(0.0);
     cell_sources<true, false>(prm, k, px, py, pz, (uint32_t)i, d, T, Fu, vx, vy, vz);
-    out[i] = d;
+    out_d[i] = d;
     out[n3 + i] = T;
     out[2 * n3 + i] = Fu;
 }
 
 // diffuse_fine: temperature and fuel take one explicit 7-point substep (a missing or solid
-// neighbour contributes the cell's own value, nbv); density is copied.
+// neighbour contributes the cell's own value, nbv); density, which the reference copies through,
+// is already in place (k_fsources).
 __global__ void __launch_bounds__(256) k_fdiffuse(int S, const real* __restrict__ src, const uint8_t* __restrict__ ffm,
                                                   const float* __restrict__ prm, real nsub, real dxf,
                                                   real* __restrict__ dst) {
@@ -243,7 +246,6 @@ __global__ void __launch_bounds__(256) k_fdiffuse(int S, const real* __restrict_
     lam = lam < LAM_CAP ? lam : LAM_CAP;
     const int i = (z * S + y) * S + x;
     const Nb6 n = nb6(S, x, y, z, i);
-    const real c0 = src[i];
     const int m0 = ffm[i], mxm = ffm[n.xm], mxp = ffm[n.xp], mym = ffm[n.ym], myp = ffm[n.yp], mzm = ffm[n.zm],
               mzp = ffm[n.zp];
     const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
@@ -257,7 +259,6 @@ __global__ void __launch_bounds__(256) k_fdiffuse(int S, const real* __restrict_
                          (szp ? v : a5);
         o[cc] = (m0 == 0 || lam <= RL(0.0)) ? v : v + lam * (sum - RL(6.0) * v);
     }
-    dst[i] = c0;
     dst[n3 + i] = o[0];
     dst[2 * n3 + i] = o[1];
 }
@@ -406,8 +407,10 @@ static void restrict_launch(const HiresGrid& h, int N, const uint8_t* fm, const 
 void launch_restrict(vfx_ctx* c, real* out, hipStream_t s) { restrict_launch(c->hr, c->N, c->P.fm, c->prm, out, s); }
 
 // one launch of a fine-stage pass (0 SL, 1 MC, 2 sources, 3 one diffusion substep) from `src`
-// with the coarse velocity `vel` into the pass's usual destination `dst`
-static void fine_pass(vfx_ctx* c, int pass, const real* src, const real* vel, real* dst, hipStream_t s) {
+// with the coarse velocity `vel` into the pass's usual destination `dst` (the sources' density
+// into dst_d when given)
+static void fine_pass(vfx_ctx* c, int pass, const real* src, const real* vel, real* dst, hipStream_t s,
+                      real* dst_d = nullptr) {
     const HiresGrid& h = c->hr;
     const Geom& g = h.g;
     switch (pass) {
@@ -421,7 +424,8 @@ static void fine_pass(vfx_ctx* c, int pass, const real* src, const real* vel, re
                                advect_parts(g));
             break;
         case 2:
-            hipLaunchKernelGGL(k_fsources, fine_grid(g), g.block, 0, s, h.S, src, h.ffm, c->prm, c->st, h.dxf, dst);
+            hipLaunchKernelGGL(k_fsources, fine_grid(g), g.block, 0, s, h.S, src, h.ffm, c->prm, c->st, h.dxf,
+                               dst_d ? dst_d : dst, dst);
             break;
         default:
             hipLaunchKernelGGL(k_fdiffuse, fine_grid(g), g.block, 0, s, h.S, src, h.ffm, c->prm, (real)h.nd, h.dxf,
@@ -441,7 +445,7 @@ int stage_scalars(vfx_ctx* c, const real* data, hipStream_t s) {
     launch_mass_fin(h.g, h.part, c->st, s, S_MASS_RATIO_F);
     real* b0 = h.nd % 2 == 0 ? h.hi : h.hat;
     real* b1 = h.nd % 2 == 0 ? h.hat : h.hi;
-    fine_pass(c, 2, h.adv, nullptr, b0, s);
+    fine_pass(c, 2, h.adv, nullptr, b0, s, h.hi);
     for (int k = 0; k < h.nd; ++k) fine_pass(c, 3, k % 2 == 0 ? b0 : b1, nullptr, k % 2 == 0 ? b1 : b0, s);
     if (!launch_ok("fine scalars")) return VFX_E_HIP;
     return VFX_OK;
