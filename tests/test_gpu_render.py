@@ -236,3 +236,24 @@ def test_render_temporal_off_equals_plain(cuda, hiplib):
     y = b.draw(vol, CAMS[1]).clone()
     torch.cuda.synchronize()
     assert torch.equal(x, y)
+
+
+def test_render_temporal_history_dropped_on_resize(cuda, hiplib):
+    """a new output size drops the history: the next temporal draw passes its march through"""
+    dens, temp, solid = _scene(48, seed=8)
+    p = RenderParams3D(light_res=24)
+    lut = blackbody_lut(p.t_lo, p.t_hi, 256, t_ref=p.t_ref)
+    vol = Volume(density=to_dev(dens), temperature=to_dev(temp))
+    r = VolumeRenderer((60, 80), device=DEV, params=p, temporal=True)
+    r.draw(vol, CAMS[0])
+    r.draw(vol, CAMS[1])
+    r.set_output((48, 64))
+    r.draw(vol, CAMS[0])                      # the renderer's third temporal draw
+    got = r.hdr().cpu().numpy()
+    pr = R.prep(dens, temp, None, p.pack(), lut)
+    cb = camera_basis(CAMS[0], r.mw / r.mh)
+    cb[14] = np.float32((3 * R.GOLDEN) % 1.0)
+    ref = R.march(pr, p.pack(), lut, cb, r.mh, r.mw)
+    e = np.abs(got - ref).max(-1) / np.abs(ref).max()
+    assert got.shape == (48, 64, 4)
+    assert (e > 1e-4).mean() <= 0.005
