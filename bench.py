@@ -500,6 +500,58 @@ def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: boo
     print(json.dumps(out))
 
 
+def config0_report(steps: int):
+    """The 64^3 smoke plume past a sphere (dt 1/60, `steps` steps, seed 0), timed on the GPU and
+    on the host cores in the same job: median ms per step (GPU: HIP events around each step after
+    20 warm-up steps), median pressure ms (step_profiled, a second run from the same start), PCG
+    iterations to rel-res 1e-5; the C oracle (OpenMP) runs the same steps from the same start."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from vfx.sim3d import Solver3D
+    from vfx.sim3d.solver import pack_params
+    N, dt = 64, 1.0 / 60.0
+    cfg = plume_config()
+    forces, obst = plume_scene()
+    med = lambda v: round(float(statistics.median(v)), 3)
+    s = Solver3D(N, cfg)
+    st = s.reset(obst)
+    data0, solid = st.data.cpu().numpy().copy(), st.solid.cpu().numpy()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    gpu_ms, gpu_it = [], []
+    for k in range(steps):
+        ev[0].record()
+        s.step(st, forces, dt)
+        ev[1].record()
+        torch.cuda.synchronize()
+        gpu_ms.append(ev[0].elapsed_time(ev[1]))
+        gpu_it.append(s.pressure_iters())
+    s2 = Solver3D(N, cfg)
+    st2 = s2.reset(obst)
+    prof = [s2.step_profiled(st2, forces, dt) for _ in range(steps)]
+    warm = min(20, steps // 2)
+    cpu_ms, cpu_it = [], []
+    data, phi = data0, np.zeros((N,) * 3, np.float32)
+    for k in range(steps):
+        prm = pack_params(cfg, N, forces, dt, k)
+        t0 = time.perf_counter()
+        data, it = oracle.step(data, solid, prm, phi, np.float32(N))
+        cpu_ms.append(1000.0 * (time.perf_counter() - t0))
+        cpu_it.append(int(it))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    g, c = med(gpu_ms[warm:]), med(cpu_ms)
+    print(json.dumps({"metric": "64^3 plume past a sphere: ms per step, GPU and host cores in the same job",
+                      "steps": steps, "gpu_ms_per_step": g, "gpu_pressure_ms": med(p["projection"] for p in prof[warm:]),
+                      "gpu_iters_median": med(gpu_it), "gpu_iters_max": max(gpu_it),
+                      "cpu_ms_per_step": c, "cpu_iters_median": med(cpu_it), "cpu_iters_max": max(cpu_it),
+                      "cpu_cores": cores, "cpu_kind": "port (the C oracle, FP32 storage, FP64 dots, OpenMP)",
+                      "speedup": round(c / g, 1),
+                      "config": {"workload": "64^3 plume past a sphere: emitter sphere r 0.08 at (0.5, 0.1, 0.5), "
+                                             "obstacle sphere r 0.15 at the centre, beta 1, eps 0.3, emitter noise "
+                                             "U[0.8, 1.2], MGPCG to rel-res 1e-5, dt 1/60, seed 0"}}))
+
+
 def cpu_baseline(st_data, st_solid, cfg, forces, dt, res, step):
     """The oracle's FP32 step on the host cores, one step of the same state (a bounded sample)."""
     import numpy as np
@@ -529,6 +581,8 @@ def main():
     ap.add_argument("--profiled", type=int, default=5, help="extra step_profiled steps for the stage split")
     ap.add_argument("--roofline-kernel", default="", help="kernel for the roofline (default: largest step share)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config0", type=int, default=0, metavar="STEPS",
+                    help="the 64^3 plume past a sphere for STEPS steps on the GPU and the host cores")
     ap.add_argument("--pressure", default="pcg", choices=("pcg", "fused"),
                     help="projection: MGPCG (the bench line) or the fused stationary multigrid of "
                          "fused_config() (2 V-cycles + 4 Chebyshev sweeps, adaptive to normalised residual 5e-5)")
@@ -582,6 +636,10 @@ def main():
     if args.render > 0:
         torch.cuda.set_device(torch.device("cuda:0"))
         render_report(args.render)
+        return
+    if args.config0 > 0:
+        torch.cuda.set_device(torch.device("cuda:0"))
+        config0_report(args.config0)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
