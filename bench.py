@@ -530,6 +530,17 @@ def config0_report(steps: int):
     s2 = Solver3D(N, cfg)
     st2 = s2.reset(obst)
     prof = [s2.step_profiled(st2, forces, dt) for _ in range(steps)]
+    # the same simulation as a one-simulation batched frame: one captured hipGraph per step with a
+    # fixed PCG budget (grown when used up), no host round trip inside the step
+    from vfx.sim3d import BatchSolver3D
+    b = BatchSolver3D([N], [cfg], streams=1, graph=True)
+    bst = b.reset([obst])
+    graph_ms, graph_it = [], []
+    for k in range(steps):
+        b.step(bst, [forces], dt)
+        w = b.wait()
+        graph_ms.append(w["frame_ms"])
+        graph_it.append(int(w["iters"][0]))
     warm = min(20, steps // 2)
     cpu_ms, cpu_it = [], []
     data, phi = data0, np.zeros((N,) * 3, np.float32)
@@ -544,6 +555,7 @@ def config0_report(steps: int):
     print(json.dumps({"metric": "64^3 plume past a sphere: ms per step, GPU and host cores in the same job",
                       "steps": steps, "gpu_ms_per_step": g, "gpu_pressure_ms": med(p["projection"] for p in prof[warm:]),
                       "gpu_iters_median": med(gpu_it), "gpu_iters_max": max(gpu_it),
+                      "gpu_graph_ms_per_step": med(graph_ms[warm:]), "gpu_graph_iters_max": max(graph_it),
                       "cpu_ms_per_step": c, "cpu_iters_median": med(cpu_it), "cpu_iters_max": max(cpu_it),
                       "cpu_cores": cores, "cpu_kind": "port (the C oracle, FP32 storage, FP64 dots, OpenMP)",
                       "speedup": round(c / g, 1),
