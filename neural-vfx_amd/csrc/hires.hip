@@ -453,8 +453,8 @@ int stage_scalars(vfx_ctx* c, const real* data, hipStream_t s) {
 
 void hires_free(vfx_ctx* c) {
     if (c->hr.pool) {
-        hipDeviceSynchronize();
-        hipFree(c->hr.pool);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(c->hr.pool);
     }
     c->hr = HiresGrid{};
 }

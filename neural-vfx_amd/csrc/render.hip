@@ -955,8 +955,8 @@ struct vfx_render {
 
 static bool ralloc(void** p, size_t bytes) {
     if (*p) {
-        hipDeviceSynchronize();
-        hipFree(*p);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(*p);
         *p = nullptr;
     }
     return hip_ok(hipMalloc(p, bytes ? bytes : 4), "render buffers");
@@ -1011,11 +1011,11 @@ static int alloc_output(vfx_render* r, int out_h, int out_w, float scale) {
 }
 
 static void free_all(vfx_render* r) {
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     void* ps[] = {r->lut, r->packed, r->occ, r->sblur, r->solid, r->light, r->glow_src, r->gwork, r->glow, r->hdr,
                   r->bwork, r->bloom, r->vel, r->aux, r->hist, r->tres};
     for (void* p : ps)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
 }
 
 extern "C" {
@@ -1156,7 +1156,7 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
         hipEvent_t e[3] = {};
         ~Events() {
             for (auto& x : e)
-                if (x) hipEventDestroy(x);
+                if (x) (void)hipEventDestroy(x);
         }
     } evs;
     hipEvent_t* ev = evs.e;
