@@ -1164,7 +1164,7 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
     if (timed) {
         for (int i = 0; i < 3; ++i)
             if (!hip_ok(hipEventCreate(&ev[i]), "render events")) return VFX_E_HIP;
-        hipEventRecord(ev[0], s);
+        (void)hipEventRecord(ev[0], s);
     }
     uint8_t* occ_v = r->occ;
     uint8_t* occ_s = r->occ + BRICKS * BRICKS * BRICKS;
@@ -1220,7 +1220,7 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
         hipLaunchKernelGGL(k_blur_axis, dim3(nblk(3 * g8)), dim3(256), 0, s, g8b, g8t, 3, G8, G8, G8, 0, k3);
         hipLaunchKernelGGL(k_glow_combine, dim3(nblk(g3)), dim3(256), 0, s, gt, g8t, r->sh.glow_coarse, r->glow);
     }
-    if (timed) hipEventRecord(ev[1], s);
+    if (timed) (void)hipEventRecord(ev[1], s);
     // view
     MarchArgs a;
     a.vol = r->packed;
@@ -1270,11 +1270,11 @@ static int draw_impl(vfx_render* r, const float* dens, const float* temp, int D,
                            r->mw, r->out_h, r->out_w, 1.0e-3f, aq->out);
     if (!launch_ok("render")) return VFX_E_HIP;
     if (timed) {
-        hipEventRecord(ev[2], s);
-        hipEventSynchronize(ev[2]);
+        (void)hipEventRecord(ev[2], s);
+        (void)hipEventSynchronize(ev[2]);
         float a0 = 0.f, a1 = 0.f;
-        hipEventElapsedTime(&a0, ev[0], ev[1]);
-        hipEventElapsedTime(&a1, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&a0, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&a1, ev[1], ev[2]);
         if (ms_prep) *ms_prep = a0;
         if (ms_view) *ms_view = a1;
     }
