@@ -564,6 +564,41 @@ def config0_report(steps: int):
                                              "U[0.8, 1.2], MGPCG to rel-res 1e-5, dt 1/60, seed 0"}}))
 
 
+def seeds_report(steps: int, N: int = 256):
+    """The 256^3 fire for `steps` steps (dt 1/60) from rest with noise seeds 0-3: per seed the
+    median ms per step (HIP events around each step, steps 20+), PCG iterations (median, max),
+    the worst true relative residual and restarts, and the max normalised |div u| after the last
+    step."""
+    import torch
+    from vfx.sim3d import Solver3D
+    med = lambda v: round(float(statistics.median(v)), 3)
+    rows = []
+    f = fire_forces()
+    for seed in range(4):
+        s = Solver3D(N, fire_config(seed))
+        st = s.reset()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ms, its, worst, restarts = [], [], 0.0, 0
+        for k in range(steps):
+            ev[0].record()
+            s.step(st, f, 1.0 / 60.0)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms.append(ev[0].elapsed_time(ev[1]))
+            d = s.pressure_stats()
+            its.append(d["iters"])
+            worst = max(worst, d["true_rel_res"])
+            restarts += d["restarts"]
+        div = float(s.divergence(st).abs().max().item())
+        rows.append({"seed": seed, "ms_per_step": med(ms[20:]), "iters_median": med(its), "iters_max": max(its),
+                     "worst_true_rel_res": float(f"{worst:.3g}"), "restarts": restarts, "max_div_last": float(f"{div:.3g}")})
+        del s, st
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": f"{N}^3 fire, {steps} steps from rest per noise seed", "rows": rows,
+                      "config": {"workload": f"{N}^3 fire (fuel disc, combustion, buoyancy, confinement), MGPCG "
+                                             "rel-res 1e-5 (true residual judged), dt 1/60, seeds 0-3"}}))
+
+
 def cpu_baseline(st_data, st_solid, cfg, forces, dt, res, step):
     """The oracle's FP32 step on the host cores, one step of the same state (a bounded sample)."""
     import numpy as np
@@ -593,6 +628,8 @@ def main():
     ap.add_argument("--profiled", type=int, default=5, help="extra step_profiled steps for the stage split")
     ap.add_argument("--roofline-kernel", default="", help="kernel for the roofline (default: largest step share)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seeds", type=int, default=0, metavar="STEPS",
+                    help="the 256^3 fire for STEPS steps from rest with noise seeds 0-3")
     ap.add_argument("--config0", type=int, default=0, metavar="STEPS",
                     help="the 64^3 plume past a sphere for STEPS steps on the GPU and the host cores")
     ap.add_argument("--pressure", default="pcg", choices=("pcg", "fused"),
@@ -652,6 +689,10 @@ def main():
     if args.config0 > 0:
         torch.cuda.set_device(torch.device("cuda:0"))
         config0_report(args.config0)
+        return
+    if args.seeds > 0:
+        torch.cuda.set_device(torch.device("cuda:0"))
+        seeds_report(args.seeds, args.res)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
