@@ -12,6 +12,8 @@
  * Every function returns 0 on success or a VFX_E_* code; vfx_last_error() gives the message.
  * The library allocates device memory only in vfx_ctx_create and vfx_hires_attach; calls on one
  * context must not overlap (one context per Solver3D, as the reference keeps one solver per grid).
+ * vfx_step and vfx_project do not wait for the device (VFX_OPT_SYNC): the host runs ahead by up to
+ * 16 steps (a ring of pinned parameter slots, the reference's 4-slot ring at solver.py:331-335).
  */
 #ifndef VFXSIM3D_H
 #define VFXSIM3D_H
@@ -48,9 +50,10 @@ extern "C" {
 #define VFX_OPT_LEARNED_MAXIT 7  /* iteration budget of flexible CG before the MGPCG fallback (100) */
 #define VFX_OPT_LEARNED_SWEEPS 8 /* damped-Jacobi corrections z += 0.8 D^-1 (r - K z) after each network
                                     application (0..8, default 2) */
-#define VFX_OPT_CG_BUDGET 9      /* 0 (default): the host polls the device's PCG stop flag each
-                                    iteration; B > 0: exactly min(B, P_MAXIT) iterations are enqueued
-                                    and the ones after the stop test are no-ops on the device */
+#define VFX_OPT_CG_BUDGET 9      /* device sequencing: 0 (default) adapts the PCG iteration budget from
+                                    the counts of recent projections; B > 0 enqueues exactly
+                                    min(B, P_MAXIT) iterations (the ones after the stop test are no-ops
+                                    on the device) */
 #define VFX_OPT_CELLS_PER_THREAD 10  /* z-cells per thread of the cell kernels: 1 (default), 2, 4, 8;
                                     results equal up to FP64 reduction order */
 #define VFX_OPT_PRESSURE_MODE 11 /* 0 MGPCG (default); 1 the fused projection (pressure_fused.py):
@@ -59,6 +62,22 @@ extern "C" {
 #define VFX_OPT_FUSED_CYCLES 12  /* fused: V-cycles always run (pressure_cycles, default 2) */
 #define VFX_OPT_FUSED_SMOOTH 13  /* fused: exact-K sweeps before / between / after (pressure_smooth, 4) */
 #define VFX_OPT_FUSED_SMOOTHER 14 /* fused: 0 Chebyshev weights on [0.5, 2] (default), 1 Jacobi 0.8 */
+#define VFX_OPT_SYNC 15          /* projection sequencing.  0 (default) device: no host synchronisation
+                                    in vfx_step / vfx_project (the reference's on-device loop,
+                                    pressure.py:940-950).  The host enqueues an iteration budget (the
+                                    kernels after the device's stop test are no-ops), and with restarts
+                                    on the true residual, a device-side restart decision and one gated
+                                    restart run with its own budget.  Budgets follow the counts of recent
+                                    projections, read back lazily; a projection that runs out of its budget
+                                    stops there, like the reference at pressure_max_iters, is counted in
+                                    vfx_pressure_stats out[11], and the budget grows for the next.
+                                    1 polled: the host reads the stop flag every iteration and restarts
+                                    while the true residual is above tolerance (up to VFX_OPT_RESTARTS);
+                                    the learned path always runs this way. */
+#define VFX_OPT_GRAPH 16         /* 1 (default): in device mode vfx_step replays the whole step as a
+                                    captured hipGraph, one per set of host-side launch decisions (buffer
+                                    pointers, budgets, warm start, options), up to 4 per context
+                                    (solver.py:388-420 keeps an LRU of 4 CUDA graphs); 0: direct launches */
 
 typedef struct vfx_ctx vfx_ctx;
 
@@ -95,7 +114,12 @@ int vfx_pressure_iters(vfx_ctx* ctx, int* out);
 /* Diagnostics of the last projection (waits for it): out[0] iterations (all restarts),
  * out[1] max|u*|, out[2] ||b||_2, out[3] recurrence ||r||_2, out[4] max|r|,
  * out[5] true ||b - K phi||_2 (when VFX_OPT_TRUE_RESIDUAL is on, else -1), out[6] restarts,
- * out[7] flexible-CG iterations (learned path), out[8] 1 if it fell back to MGPCG. */
+ * out[7] flexible-CG iterations (learned path), out[8] 1 if it fell back to MGPCG,
+ * out[9] 1 if it converged (the true residual judges when computed), out[10] its first CG run's
+ * iteration budget (0 when host-polled).  Over the context's life so far: out[11] projections
+ * that stopped at a budget unconverged, out[12] unconverged projections, out[13] step graphs
+ * captured, out[14] step graph replays (out[11], out[12] count the projections read back so far). */
+#define VFX_STATS 15
 int vfx_pressure_stats(vfx_ctx* ctx, double* out, int n);
 
 int vfx_set_option(vfx_ctx* ctx, int key, double value);
@@ -286,18 +310,24 @@ int vfx_net_apply(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* r
  * graph-captures one solver's step (solver.py:388-459).  Each frame enqueues every simulation's
  * step (vfx_step's stages) on one of `streams` HIP streams forked from and joined back into the
  * caller's stream, directly or as one captured hipGraph replayed while the pointers, budgets and
- * launch shapes stay the same.  Projections are MGPCG with a fixed per-simulation iteration
- * budget (the device stops at the budget or at convergence; no host round trip inside a frame;
- * no restarts); vfx_batch_wait grows a budget that was used up.  The contexts stay usable on
- * their own between frames; a context may belong to one batch at a time. */
+ * launch shapes stay the same.  Projections are MGPCG sequenced on the device as in vfx_step
+ * (VFX_OPT_SYNC 0) with a per-simulation iteration budget (the device stops at the budget or at
+ * convergence; with restarts on, one gated restart run; no host round trip inside a frame).  A
+ * budget starts at min(P_MAXIT, 32); vfx_batch_wait reads the frame's state and doubles the budget
+ * of a simulation that stopped at it unconverged (vfx_batch_truncated counts those).  Without
+ * vfx_batch_wait the budgets stay as they are.  The contexts stay usable on their own between
+ * frames; a context may belong to one batch at a time. */
 typedef struct vfx_batch vfx_batch;
 #define VFX_TUNE_MAX_SIZES 8
 #define VFX_TUNE_REPORT 46   /* doubles of the autotuner report (layout at vfx_batch_tune) */
 int vfx_batch_create(vfx_ctx* const* ctxs, int n, int streams, vfx_batch** out);
 int vfx_batch_destroy(vfx_batch* batch);
-/* PCG iteration budget of simulation `sim` (-1: all); default 10 */
+/* PCG iteration budget of simulation `sim` (-1: all); default min(P_MAXIT, 32) at the first frame */
 int vfx_batch_set_budget(vfx_batch* batch, int sim, int budget);
-int vfx_batch_get_budget(vfx_batch* batch, int* budget);   /* n ints */
+int vfx_batch_get_budget(vfx_batch* batch, int* budget);   /* n ints (0: not set yet) */
+/* Simulation-frames, among the frames vfx_batch_wait has read, whose projection stopped at its
+ * budget unconverged (does not wait for the device). */
+int vfx_batch_truncated(vfx_batch* batch, long long* count);
 /* One frame: simulation i steps data[i] (6,N_i,N_i,N_i), solid[i], parameters prm_host[i * vfx_p_size()
  * ...]; graph 1 replays (capturing when needed) the frame's hipGraph, 0 enqueues it directly.
  * Returns once enqueued. */
@@ -305,8 +335,9 @@ int vfx_batch_step(vfx_batch* batch, float* const* data, const float* const* sol
                    int graph, void* stream);
 /* Waits for the last frame.  latency_ms[i]: frame start to the end of simulation i's step (GPU
  * wall clock); iters[i]: PCG iterations; converged[i]: the projection met its tolerance within
- * the budget; rel_res[i]: the recurrence ||r|| / ||b|| at the stop; frame_ms: frame start to the
- * last simulation's end.  Any output may be NULL. */
+ * the budget (the true residual judges when it was computed); rel_res[i]: ||r|| / ||b|| at the
+ * stop (true residual when computed, else the recurrence); frame_ms: frame start to the last
+ * simulation's end.  Any output may be NULL. */
 int vfx_batch_wait(vfx_batch* batch, float* latency_ms, int* iters, int* converged, double* rel_res,
                    float* frame_ms);
 /* Autotuner: per grid size, VFX_OPT_CELLS_PER_THREAD in {1,2,4} x VFX_OPT_COARSE_M in {4,8,16} by the

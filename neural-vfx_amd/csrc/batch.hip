@@ -2,12 +2,14 @@
 //
 // A frame enqueues every simulation's step (advection, forces, MGPCG projection) on one of a few
 // HIP streams forked from the caller's stream and joined back into it, either directly or as one
-// captured hipGraph that is replayed frame after frame.  Every projection runs a fixed PCG
-// iteration budget (the kernels after the device's stop test are no-ops, the device stops by
-// itself at the budget), so a frame never waits on the host.  Each simulation's last kernel is
-// followed by a timestamp (the constant-rate wall clock of the GPU), which gives per-simulation
-// completion times inside the frame.  After a frame the host reads every simulation's PCG state
-// at once, and a simulation that used its whole budget gets a larger one for the next capture.
+// captured hipGraph that is replayed frame after frame.  Every projection is sequenced on the
+// device with a PCG iteration budget per simulation (the kernels after the device's stop test are
+// no-ops, the device stops by itself at the budget), so a frame never waits on the host.  Each
+// simulation's last kernel is followed by a timestamp (the constant-rate wall clock of the GPU),
+// which gives per-simulation completion times inside the frame.  After a frame the device gathers
+// every simulation's PCG state into a pinned buffer; vfx_batch_wait reads it, and a simulation that
+// stopped at its budget unconverged gets twice the budget for the next frame.  A budget starts at
+// min(P_MAXIT, 32).
 // The autotuner picks, per grid size, the cells per thread of the cell kernels and the size from
 // which the multigrid coarse levels run in one workgroup, and the number of streams, by timing
 // them on snapshots of the simulations' own states.
@@ -31,7 +33,9 @@ namespace vfx {
 constexpr int BATCH_MAX = 4096;
 constexpr int BATCH_MAX_STREAMS = 16;
 constexpr int BUDGET_MARGIN = 2;   // iterations kept beyond the largest count seen
-constexpr int FRAME_STATS = 6;     // it, stop_at, r.r, b.b, max|r|, threshold per simulation
+constexpr int BUDGET_FIRST = 32;   // first budget of a simulation (capped by its P_MAXIT)
+constexpr int FRAME_STATS = 9;     // per simulation: iterations, stop_at, r.r, b.b, max|r|, threshold,
+                                   // true residual, budget-stop bits, first run's iterations
 
 // parameters of every simulation, staged in one device buffer, to each context's own slot
 __global__ void __launch_bounds__(256) k_scatter_prm(const float* __restrict__ src, float* const* __restrict__ dst) {
@@ -49,12 +53,15 @@ __global__ void __launch_bounds__(256) k_gather_state(DevState* const* __restric
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const DevState* s = st[i];
         double* o = out + (size_t)i * FRAME_STATS;
-        o[0] = (double)s->it;
+        o[0] = (double)(s->it_total + s->it);
         o[1] = (double)s->stop_at;
         o[2] = s->s[S_RR];
         o[3] = s->s[S_BB];
         o[4] = s->s[S_MAXR];
         o[5] = s->s[S_THR];
+        o[6] = s->s[S_TRUE_RR];
+        o[7] = (double)s->hit;
+        o[8] = (double)(s->restarts > 0 ? s->it_total : s->it);
     }
 }
 
@@ -85,8 +92,11 @@ struct vfx_batch {
     unsigned long long* pin_ts = nullptr;
     double* pin_stats = nullptr;
     // per simulation
-    std::vector<int> budget;
+    std::vector<int> budget;       // 0 until the first frame sets it from P_MAXIT
     std::vector<float> user_maxit, rel_tol;
+    std::vector<char> judged;      // the frame computes the simulation's true residual
+    bool absorbed = true;          // the last frame's gathered state has been read
+    long long n_hit = 0;           // simulation-frames that stopped at their budget unconverged
     double ticks_per_ms = 0.0;
     // graph
     hipGraph_t graph = nullptr;
@@ -180,9 +190,10 @@ int stage_params(vfx_batch* b, const float* prm_host, hipStream_t main) {
         float* p = slot + (size_t)i * P_SIZE;
         b->user_maxit[i] = p[P_MAXIT];
         b->rel_tol[i] = p[P_REL_TOL];
-        // the device stops at the budget: a capped projection finishes like the reference's
-        // fixed-iteration runtime configuration (config.py:90-99)
-        p[P_MAXIT] = (float)std::min((int)p[P_MAXIT], b->budget[i]);
+        if (b->budget[i] == 0) {
+            b->budget[i] = std::max(1, std::min((int)p[P_MAXIT], BUDGET_FIRST));
+            b->dirty = true;
+        }
         std::memcpy(b->sims[i]->prm_last, p, sizeof(float) * P_SIZE);
     }
     if (!hip_ok(hipMemcpyAsync(b->dprm, slot, sizeof(float) * P_SIZE * b->n, hipMemcpyHostToDevice, main),
@@ -247,14 +258,45 @@ int frame(vfx_batch* b, float* const* data, const float* const* solid, const flo
         if (!hip_ok(hipEventRecord(c->done_ev, main), "done event")) return VFX_E_HIP;
     }
     if (!hip_ok(hipEventRecord(b->done, main), "frame event")) return VFX_E_HIP;
+    for (int i = 0; i < b->n; ++i) b->judged[i] = b->sims[i]->last_true_valid ? 1 : 0;
     b->frames = true;
+    b->absorbed = false;
     return VFX_OK;
+}
+
+// converged: the true residual judges when the frame computed it
+bool sim_converged(const vfx_batch* b, int i) {
+    const double* o = b->pin_stats + (size_t)i * FRAME_STATS;
+    const double rt = (double)b->rel_tol[i];
+    if (b->judged[i] && rt > 0.0) return std::sqrt(o[6]) <= rt * std::sqrt(o[3]);
+    return rt > 0.0 ? std::sqrt(o[2]) <= rt * std::sqrt(o[3]) : o[4] <= o[5];
+}
+
+// the finished frame's gathered state -> the next budgets (a budget stop doubles the budget)
+void absorb_frame(vfx_batch* b) {
+    if (b->absorbed) return;
+    b->absorbed = true;
+    for (int i = 0; i < b->n; ++i) {
+        const double* o = b->pin_stats + (size_t)i * FRAME_STATS;
+        const int hit = (int)o[7];
+        const int first = (int)o[8];
+        if (hit) b->n_hit += 1;
+        const int cap = std::max(1, (int)b->user_maxit[i]);
+        int want = std::max(b->budget[i], first + BUDGET_MARGIN);
+        if (hit & 1) want = std::max(want, 2 * b->budget[i]);
+        want = std::min(want, cap);
+        if (want > b->budget[i]) {
+            b->budget[i] = want;
+            b->dirty = true;
+        }
+    }
 }
 
 // wait for the last frame; per-simulation results; grow the budgets that were too small
 int wait_frame(vfx_batch* b, float* latency_ms, int* iters, int* converged, double* rel_res, float* frame_ms) {
     if (!b->frames) return arg_error("no frame was enqueued");
     if (!hip_ok(hipEventSynchronize(b->done), "frame wait")) return VFX_E_HIP;
+    absorb_frame(b);
     const unsigned long long t0 = b->pin_ts[b->n];
     unsigned long long tmax = t0;
     for (int i = 0; i < b->n; ++i) {
@@ -262,21 +304,9 @@ int wait_frame(vfx_batch* b, float* latency_ms, int* iters, int* converged, doub
         tmax = std::max(tmax, te);
         if (latency_ms) latency_ms[i] = (float)((double)(te - t0) / b->ticks_per_ms);
         const double* o = b->pin_stats + (size_t)i * FRAME_STATS;
-        const int it = (int)o[0];
-        const double rt = (double)b->rel_tol[i];
-        const bool conv = rt > 0.0 ? std::sqrt(o[2]) <= rt * std::sqrt(o[3]) : o[4] <= o[5];
-        if (iters) iters[i] = it;
-        if (converged) converged[i] = conv ? 1 : 0;
-        if (rel_res) rel_res[i] = o[3] > 0.0 ? std::sqrt(o[2] / o[3]) : 0.0;
-        const int cap = std::max(1, (int)b->user_maxit[i]);
-        int want = b->budget[i];
-        if (!conv && it >= b->budget[i]) want = b->budget[i] + 2 * BUDGET_MARGIN;   // ran out
-        if (it + BUDGET_MARGIN > want) want = it + BUDGET_MARGIN;
-        want = std::min(want, cap);
-        if (want > b->budget[i]) {
-            b->budget[i] = want;
-            b->dirty = true;
-        }
+        if (iters) iters[i] = (int)o[0];
+        if (converged) converged[i] = sim_converged(b, i) ? 1 : 0;
+        if (rel_res) rel_res[i] = o[3] > 0.0 ? std::sqrt((b->judged[i] ? o[6] : o[2]) / o[3]) : 0.0;
     }
     if (frame_ms) *frame_ms = (float)((double)(tmax - t0) / b->ticks_per_ms);
     return VFX_OK;
@@ -333,7 +363,6 @@ int time_sim(vfx_batch* b, int i, float* const* data, const float* const* solid,
     vfx_ctx* c = b->sims[i];
     float p[P_SIZE];
     std::memcpy(p, prm_host + (size_t)i * P_SIZE, sizeof(p));
-    p[P_MAXIT] = (float)std::min((int)p[P_MAXIT], b->budget[i]);
     std::memcpy(c->prm_last, p, sizeof(p));
     if (int rc = upload(c, p, c->prm, s)) return rc;
     std::vector<float> t;
@@ -378,9 +407,10 @@ int vfx_batch_create(vfx_ctx* const* ctxs, int n, int streams, vfx_batch** out) 
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(),
                      [&](int a, int c) { return b->sims[a]->g.n3 > b->sims[c]->g.n3; });
-    b->budget.assign(n, 10);
+    b->budget.assign(n, 0);
     b->user_maxit.assign(n, 0.0f);
     b->rel_tol.assign(n, 0.0f);
+    b->judged.assign(n, 0);
     int dev = 0, khz = 0;
     bool ok = hip_ok(hipGetDevice(&dev), "device") &&
               hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
@@ -456,6 +486,7 @@ int vfx_batch_destroy(vfx_batch* b) {
 int vfx_batch_set_budget(vfx_batch* b, int sim, int budget) {
     if (!b) return arg_error("null batch");
     if (budget < 1 || budget > 1000) return arg_error("budget must be 1..1000");
+    b->absorbed = true;   // a frame enqueued before this call does not override the new budget
     for (int i = 0; i < b->n; ++i)
         if (sim < 0 || sim == i) {
             if (b->budget[i] != budget) b->dirty = true;
@@ -467,6 +498,12 @@ int vfx_batch_set_budget(vfx_batch* b, int sim, int budget) {
 int vfx_batch_get_budget(vfx_batch* b, int* budget) {
     if (!b || !budget) return arg_error("null pointer");
     for (int i = 0; i < b->n; ++i) budget[i] = b->budget[i];
+    return VFX_OK;
+}
+
+int vfx_batch_truncated(vfx_batch* b, long long* count) {
+    if (!b || !count) return arg_error("null pointer");
+    *count = b->n_hit;
     return VFX_OK;
 }
 

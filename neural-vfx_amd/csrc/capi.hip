@@ -123,28 +123,46 @@ int vfx::stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
     return VFX_OK;
 }
 
-// The PCG loop (PressureSolver3D.record, pressure.py:940-950).  The reference runs the loop as a
-// CUDA-graph conditional; here the host enqueues iteration k, then waits only for iteration k-1's
-// stop flag, so the device always has the next iteration queued and stopped iterations are
-// no-ops on the device.
-static int run_cg(vfx_ctx* c, hipStream_t s, int* iters) {
+// ---------------------------------------------------------------- projection sequencing
+// The reference runs its PCG loop on the device as a CUDA-graph conditional (pressure.py:940-950);
+// HIP has no conditional graph nodes.  Here a projection is sequenced in one of two ways:
+//  - device (default): the host enqueues an iteration budget; every kernel of an iteration after
+//    the device's own stop test returns at once.  With restarts on, the true residual, a restart
+//    gate and one gated restart run follow, all decided on the device.  The host never waits: the
+//    budgets come from the iteration counts of earlier projections, which the host reads back
+//    lazily (pinned copies of the device state, polled without blocking).
+//  - polled: the host enqueues iteration k, then waits for iteration k-1's stop flag (the learned
+//    path's MGPCG fallback and VFX_OPT_SYNC = 1).
+constexpr int BUDGET_START = 32;        // first budget of a context, before any read-back arrived
+constexpr int RESTART_BUDGET_START = 2;
+constexpr int FUSED_TAIL_START = 1 << 20;   // fused projection: the whole tail until a count is known
+constexpr int BUDGET_MARGIN = 1;        // iterations enqueued beyond the largest recent count
+constexpr int BUDGET_CAP = 1000;
+
+int vfx::cg_budget_main(const vfx_ctx* c, int maxit) {
+    const int b = c->cg_budget > 0 ? c->cg_budget : (c->bw.budget > 0 ? c->bw.budget : BUDGET_START);
+    return std::max(1, std::min(b, maxit));
+}
+
+int vfx::cg_budget_restart(const vfx_ctx* c, int maxit) {
+    const int b = c->bw_r.budget > 0 ? c->bw_r.budget : RESTART_BUDGET_START;
+    return std::max(1, std::min(b, maxit));
+}
+
+static int fused_tail_budget(const vfx_ctx* c) {
+    return c->cg_budget > 0 ? c->cg_budget : (c->bw.budget > 0 ? c->bw.budget : FUSED_TAIL_START);
+}
+
+// The host-polled PCG loop: iteration k is enqueued before the stop flag of iteration k-1 is
+// read, so the device always has the next iteration queued and stopped iterations are no-ops.
+static int run_cg_polled(vfx_ctx* c, hipStream_t s) {
     Pressure& P = c->P;
     const int maxit = (int)c->prm_last[P_MAXIT];
     P.iteration(0, c->prm, s);
     CHECK_LAUNCH("pcg init");
-    if (c->cg_budget > 0) {
-        // fixed budget: the iterations after the device's stop test are no-ops on the device, and
-        // the device stops by itself at P_MAXIT, so the host never waits
-        const int last = std::min(c->cg_budget, std::max(maxit, 1));
-        for (int k = 1; k <= last; ++k) P.iteration(k, c->prm, s);
-        CHECK_LAUNCH("pcg iterations");
-        *iters = last;
-        return VFX_OK;
-    }
     CHECK_HIP(hipMemcpyAsync(&c->stop_pin[0], &c->st->stop_at, sizeof(int), hipMemcpyDeviceToHost, s), "stop read");
     CHECK_HIP(hipEventRecord(c->stop_ev[0], s), "stop event");
-    int k = 1;
-    for (; k <= maxit; ++k) {
+    for (int k = 1; k <= maxit; ++k) {
         P.iteration(k, c->prm, s);
         CHECK_LAUNCH("pcg iteration");
         const int slot = k % STOP_RING;
@@ -155,7 +173,100 @@ static int run_cg(vfx_ctx* c, hipStream_t s, int* iters) {
         CHECK_HIP(hipEventSynchronize(c->stop_ev[prev]), "stop wait");
         if (c->stop_pin[prev] <= k - 1) break;
     }
-    *iters = k;   // upper bound; the exact count is DevState::it
+    return VFX_OK;
+}
+
+// Device-sequenced projection of vel (after rhs): the first CG run within its budget, then with
+// restarts on the true residual, the restart gate, one gated restart run and its true residual.
+static int project_device(vfx_ctx* c, real* vel, hipStream_t s) {
+    Pressure& P = c->P;
+    const int maxit = (int)c->prm_last[P_MAXIT];
+    const double rel_tol = (double)c->prm_last[P_REL_TOL];
+    const int B = cg_budget_main(c, maxit);
+    P.rhs(vel, c->prm, c->prm_last[P_WARM] > 0.5f, s);
+    for (int k = 0; k <= B; ++k) P.iteration(k, c->prm, s, B, 1);
+    P.finish(s);
+    const bool restart = c->opt_restarts > 0 && rel_tol > 0.0;
+    const bool judged = c->opt_true_res || restart;
+    int Br = 0;
+    if (judged) {
+        P.true_residual(vel, s);
+        if (restart) {
+            Br = cg_budget_restart(c, maxit);
+            P.restart_gate(c->prm, s);
+            P.rhs(vel, c->prm, true, s, true);
+            for (int k = 0; k <= Br; ++k) P.iteration(k, c->prm, s, Br, 2);
+            P.finish(s);
+            P.true_residual(vel, s, true);
+        }
+    }
+    CHECK_LAUNCH("pressure solve");
+    c->last_true_valid = judged;
+    c->last_info = ReadbackInfo{1, 0, B, Br, 0, judged ? 1 : 0, (float)rel_tol};
+    return VFX_OK;
+}
+
+// A read-back projection's device state -> convergence counters and the next budgets.
+static void absorb(vfx_ctx* c, const DevState& d, const ReadbackInfo& in) {
+    c->n_seen += 1;
+    if (in.fused) {
+        if (!in.device) return;
+        const bool hit = d.stop_at == INT_MAX;
+        c->bw.push(std::max(0, d.it - in.cycles));
+        int want = c->bw.max() + BUDGET_MARGIN;
+        if (hit) {
+            c->n_hit += 1;
+            want = std::max(want, 2 * std::max(in.budget, 1));
+        }
+        c->bw.budget = std::min(want, BUDGET_CAP);
+        return;
+    }
+    bool conv = d.conv != 0;
+    if (in.judged && in.rel_tol > 0.0f) conv = std::sqrt(d.s[S_TRUE_RR]) <= (double)in.rel_tol * std::sqrt(d.s[S_BB]);
+    if (!conv) c->n_unconverged += 1;
+    if (!in.device) return;
+    if (d.hit) c->n_hit += 1;
+    c->bw.push(d.restarts > 0 ? d.it_total : d.it);
+    int want = c->bw.max() + BUDGET_MARGIN;
+    if (d.hit & 1) want = std::max(want, 2 * in.budget);
+    c->bw.budget = std::min(want, BUDGET_CAP);
+    if (d.restarts > 0) {
+        c->bw_r.push(d.it);
+        int wr = std::max(RESTART_BUDGET_START, c->bw_r.max() + BUDGET_MARGIN);
+        if (d.hit & 2) wr = std::max(wr, 2 * in.budget_r);
+        c->bw_r.budget = std::min(wr, BUDGET_CAP);
+    }
+}
+
+// Absorb every finished read-back in order without blocking (wait_oldest: wait for the oldest
+// pending one first).
+static int drain_readbacks(vfx_ctx* c, bool wait_oldest) {
+    while (c->rb_count > 0) {
+        const int slot = c->rb_head;
+        const hipError_t e = wait_oldest ? hipEventSynchronize(c->rb_ev[slot]) : hipEventQuery(c->rb_ev[slot]);
+        wait_oldest = false;
+        if (e == hipErrorNotReady) {
+            (void)hipGetLastError();   // a pending event is not an error
+            break;
+        }
+        CHECK_HIP(e, "state read-back");
+        absorb(c, c->rb_pin[slot], c->rb_info[slot]);
+        c->rb_head = (slot + 1) % READBACK_RING;
+        c->rb_count -= 1;
+    }
+    return VFX_OK;
+}
+
+// After a projection was enqueued on s: its completion event and the read-back of its state.
+static int after_projection(vfx_ctx* c, hipStream_t s) {
+    CHECK_HIP(hipEventRecord(c->done_ev, s), "done event");
+    if (c->rb_count == READBACK_RING)
+        if (int rc = drain_readbacks(c, true)) return rc;
+    const int slot = (c->rb_head + c->rb_count) % READBACK_RING;
+    CHECK_HIP(hipMemcpyAsync(&c->rb_pin[slot], c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "state read-back");
+    CHECK_HIP(hipEventRecord(c->rb_ev[slot], s), "read-back event");
+    c->rb_info[slot] = c->last_info;
+    c->rb_count += 1;
     return VFX_OK;
 }
 
@@ -314,32 +425,42 @@ static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) 
 int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
     Pressure& P = c->P;
     real* vel = data + c->g.n3;
-    if (c->fused) {
-        c->last_iters = 0;
-        c->last_restarts = 0;
-        c->last_true_valid = false;
-#if !VFX_F64
-        c->last_learned = 0;
-        c->last_fallback = false;
-#endif
-        if (int rc = run_fused(c, vel, s, in_frame)) return rc;
-        if (!in_frame) CHECK_HIP(hipEventRecord(c->done_ev, s), "done event");
-        c->projected = true;
-        return VFX_OK;
-    }
-    P.rhs(vel, c->prm, c->prm_last[P_WARM] > 0.5f, s);
-    CHECK_LAUNCH("pressure rhs");
-    int upper = 0;
-    int rc = VFX_OK;
-    bool need_pcg = true;
+    const bool device = in_frame || c->sync_mode == SYNC_DEVICE;
+    c->last_iters = 0;
+    c->last_restarts = 0;
+    c->last_true_valid = false;
 #if !VFX_F64
     c->last_learned = 0;
     c->last_fallback = false;
-    if (in_frame && (c->net || c->fcg_mode)) {
+    const bool learned = c->net != nullptr || c->fcg_mode != 0;
+#else
+    const bool learned = false;
+#endif
+    if (c->fused) {
+        const int tb = device ? fused_tail_budget(c) : -1;
+        if (int rc = run_fused(c, vel, s, tb)) return rc;
+        c->last_info = ReadbackInfo{device ? 1 : 0, 1, tb, 0, c->fused_cycles, 0, 0.0f};
+        c->projected = true;
+        return VFX_OK;
+    }
+    if (in_frame && learned) {
         set_error("a batched frame runs MGPCG only (the flexible-CG loop waits on the host)");
         return VFX_E_VALUE;
     }
-    if (c->net || c->fcg_mode) {
+    if (device && !learned) {
+        if (int rc = project_device(c, vel, s)) return rc;
+        P.apply(vel, s);
+        CHECK_LAUNCH("pressure apply");
+        c->projected = true;
+        return VFX_OK;
+    }
+    // host-polled: the learned path (flexible CG, then the MGPCG fallback) and VFX_OPT_SYNC = 1
+    P.rhs(vel, c->prm, c->prm_last[P_WARM] > 0.5f, s);
+    CHECK_LAUNCH("pressure rhs");
+    int rc = VFX_OK;
+    bool need_pcg = true;
+#if !VFX_F64
+    if (learned) {
         bool conv = false;
         rc = run_fcg(c, vel, s, &conv);
         if (rc) return rc;
@@ -347,11 +468,6 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
         c->last_fallback = !conv;
         if (!conv) {
             // plain MGPCG from the current phi
-            float saved = c->prm_last[P_WARM];
-            c->prm_last[P_WARM] = 1.0f;
-            rc = upload(c, c->prm_last, c->prm, s);
-            c->prm_last[P_WARM] = saved;
-            if (rc) return rc;
             P.rhs(vel, c->prm, true, s);
             CHECK_LAUNCH("fallback rhs");
         } else {
@@ -360,20 +476,19 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
     }
 #endif
     if (need_pcg) {
-        rc = run_cg(c, s, &upper);
+        rc = run_cg_polled(c, s);
         if (rc) return rc;
         P.finish(s);
     }
-    c->last_restarts = 0;
-    c->last_true_valid = false;
     int total = 0;
     const double rel_tol = (double)c->prm_last[P_REL_TOL];
-    if (c->opt_true_res || (c->opt_restarts > 0 && rel_tol > 0.0)) {
+    const bool judged = c->opt_true_res || (c->opt_restarts > 0 && rel_tol > 0.0);
+    if (judged) {
         for (;;) {
             P.true_residual(vel, s);
             CHECK_LAUNCH("true residual");
             c->last_true_valid = true;
-            if (in_frame || !(c->opt_restarts > c->last_restarts && rel_tol > 0.0)) break;
+            if (!(c->opt_restarts > c->last_restarts && rel_tol > 0.0)) break;
             DevState hs;
             CHECK_HIP(hipMemcpyAsync(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost, s), "state read");
             CHECK_HIP(hipStreamSynchronize(s), "state sync");
@@ -381,23 +496,134 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
             total += hs.it;
             c->last_restarts += 1;
             // restart from the true residual: warm start from the current phi
-            float saved = c->prm_last[P_WARM];
-            c->prm_last[P_WARM] = RL(1.0);
-            rc = upload(c, c->prm_last, c->prm, s);
-            c->prm_last[P_WARM] = saved;
-            if (rc) return rc;
             P.rhs(vel, c->prm, true, s);
             CHECK_LAUNCH("pressure restart rhs");
-            rc = run_cg(c, s, &upper);
+            rc = run_cg_polled(c, s);
             if (rc) return rc;
             P.finish(s);
         }
     }
     c->last_iters = total;   // plus DevState::it of the final run, read lazily
+    c->last_info = ReadbackInfo{0, 0, 0, 0, 0, judged ? 1 : 0, (float)rel_tol};
     P.apply(vel, s);
     CHECK_LAUNCH("pressure apply");
-    if (!in_frame) CHECK_HIP(hipEventRecord(c->done_ev, s), "done event");
     c->projected = true;
+    return VFX_OK;
+}
+
+// ---------------------------------------------------------------- whole steps as graphs
+static int enqueue_step(vfx_ctx* c, real* data, const real* solid, hipStream_t s) {
+    if (int rc = stage_advect(c, data, solid, s)) return rc;
+    if (int rc = stage_scalars(c, data, s)) return rc;
+    if (int rc = stage_forces(c, data, s)) return rc;
+    return stage_project(c, data, s);
+}
+
+// destroy a slot's exec once its last launch has finished (the host waits for that launch if it is
+// still running; the least recently used slot's launch has normally finished long before)
+static void release_exec(StepGraph& g) {
+    if (g.exec) {
+        if (g.launched) (void)hipEventSynchronize(g.done);
+        (void)hipGraphExecDestroy(g.exec);
+    }
+    g.exec = nullptr;
+    g.launched = false;
+}
+
+void vfx::drop_graphs(vfx_ctx* c) {
+    for (StepGraph& g : c->graphs) release_exec(g);
+}
+
+// A step is replayed as a graph in device mode, except when an operator rebuild was forced (a
+// one-off that runs eagerly) and on the learned path (its loop waits on the host).
+static bool graph_capable(const vfx_ctx* c) {
+    if (!c->use_graph || c->sync_mode != SYNC_DEVICE || c->P.force_setup) return false;
+#if !VFX_F64
+    if (c->net != nullptr || c->fcg_mode != 0) return false;
+#endif
+    return true;
+}
+
+// every host-side decision that enqueue_step makes from the context and the parameters
+static StepSig step_sig(const vfx_ctx* c, const real* data, const real* solid) {
+    StepSig g;
+    std::memset(&g, 0, sizeof(g));
+    g.data = data;
+    g.solid = solid;
+    const int maxit = (int)c->prm_last[P_MAXIT];
+    const double rel_tol = (double)c->prm_last[P_REL_TOL];
+    int* v = g.v;
+    v[0] = c->prm_last[P_WARM] > 0.5f;
+    v[1] = c->fused ? 1 : 0;
+    v[2] = c->hr.S;
+    v[3] = c->g.zc;
+    v[4] = c->P.mg_kind;
+    v[5] = c->P.S;
+    v[6] = c->P.use_std ? 1 : 0;
+    if (c->fused) {
+        v[7] = fused_tail_budget(c);
+        v[8] = c->fused_cycles;
+        v[9] = c->fused_smooth;
+        v[10] = c->fused_jacobi;
+        v[11] = c->prm_last[P_TOL] > 0.0f;
+        v[12] = std::max(0, maxit - std::max(0, c->fused_cycles));
+    } else {
+        const bool restart = c->opt_restarts > 0 && rel_tol > 0.0;
+        v[7] = cg_budget_main(c, maxit);
+        v[8] = restart ? cg_budget_restart(c, maxit) : 0;
+        v[9] = (c->opt_true_res || restart) ? 1 : 0;
+    }
+    return g;
+}
+
+// Enqueue one step by replaying the captured graph of its signature (capturing it first if none
+// of the context's graphs has it; the least recently used one is replaced).
+static int graph_step(vfx_ctx* c, real* data, const real* solid, hipStream_t s) {
+    const StepSig sig = step_sig(c, data, solid);
+    StepGraph* hit = nullptr;
+    StepGraph* victim = nullptr;   // an empty slot, else the least recently used
+    for (StepGraph& g : c->graphs) {
+        if (g.exec && std::memcmp(&g.sig, &sig, sizeof(sig)) == 0) {
+            hit = &g;
+            break;
+        }
+        if (victim == nullptr || (victim->exec && (!g.exec || g.used < victim->used))) victim = &g;
+    }
+    if (hit == nullptr) {
+        if (!c->cap) CHECK_HIP(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking), "capture stream");
+        CHECK_HIP(hipStreamBeginCapture(c->cap, hipStreamCaptureModeRelaxed), "step capture");
+        const int rc = enqueue_step(c, data, solid, c->cap);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->cap, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        CHECK_HIP(e, "step capture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CHECK_HIP(ei, "step graph instantiate");
+        release_exec(*victim);
+        if (!victim->done) CHECK_HIP(hipEventCreateWithFlags(&victim->done, hipEventDisableTiming), "graph event");
+        victim->exec = exec;
+        victim->sig = sig;
+        victim->info = c->last_info;
+        victim->true_valid = c->last_true_valid;
+        hit = victim;
+        c->graph_captures += 1;
+    }
+    hit->used = ++c->graph_clock;
+    CHECK_HIP(hipGraphLaunch(hit->exec, s), "step graph launch");
+    CHECK_HIP(hipEventRecord(hit->done, s), "graph event");
+    hit->launched = true;
+    // the host-side results of the enqueue that the capture made
+    c->last_info = hit->info;
+    c->last_true_valid = hit->true_valid;
+    c->last_iters = 0;
+    c->last_restarts = 0;
+    c->projected = true;
+    c->graph_replays += 1;
     return VFX_OK;
 }
 
@@ -473,11 +699,15 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     bool ok = hip_ok(hipHostMalloc((void**)&c->ring, sizeof(float) * P_SIZE * HOST_RING, hipHostMallocDefault),
                      "pinned allocation") &&
               hip_ok(hipHostMalloc((void**)&c->stop_pin, sizeof(int) * STOP_RING, hipHostMallocDefault),
+                     "pinned allocation") &&
+              hip_ok(hipHostMalloc((void**)&c->rb_pin, sizeof(DevState) * READBACK_RING, hipHostMallocDefault),
                      "pinned allocation");
     for (int i = 0; ok && i < HOST_RING; ++i)
         ok = hip_ok(hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming), "event");
     for (int i = 0; ok && i < STOP_RING; ++i)
         ok = hip_ok(hipEventCreateWithFlags(&c->stop_ev[i], hipEventDisableTiming), "event");
+    for (int i = 0; ok && i < READBACK_RING; ++i)
+        ok = hip_ok(hipEventCreateWithFlags(&c->rb_ev[i], hipEventDisableTiming), "event");
     ok = ok && hip_ok(hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming), "event");
     std::memset(c->prm_last, 0, sizeof(c->prm_last));
     if (!ok) { VFX_FN(vfx_ctx_destroy)(c); return VFX_E_HIP; }
@@ -490,6 +720,12 @@ int VFX_FN(vfx_ctx_destroy)(vfx_ctx* c) {
     hipDeviceSynchronize();
     for (int i = 0; i < HOST_RING; ++i) if (c->ring_ev[i]) hipEventDestroy(c->ring_ev[i]);
     for (int i = 0; i < STOP_RING; ++i) if (c->stop_ev[i]) hipEventDestroy(c->stop_ev[i]);
+    for (int i = 0; i < READBACK_RING; ++i) if (c->rb_ev[i]) hipEventDestroy(c->rb_ev[i]);
+    drop_graphs(c);
+    for (StepGraph& g : c->graphs)
+        if (g.done) (void)hipEventDestroy(g.done);
+    if (c->cap) hipStreamDestroy(c->cap);
+    if (c->rb_pin) hipHostFree(c->rb_pin);
     if (c->done_ev) hipEventDestroy(c->done_ev);
     if (c->ring) hipHostFree(c->ring);
     if (c->stop_pin) hipHostFree(c->stop_pin);
@@ -529,6 +765,10 @@ int VFX_FN(vfx_set_option)(vfx_ctx* c, int key, double value) {
 #endif
         case VFX_OPT_PRESSURE_MODE:
             if (value != 0.0 && value != 1.0) return arg_error("pressure mode must be 0 (pcg) or 1 (fused)");
+            if (c->fused != (value != 0.0)) {
+                c->bw.reset();   // the budget counts CG iterations or fused tail cycles
+                c->bw_r.reset();
+            }
             c->fused = value != 0.0;
             return VFX_OK;
         case VFX_OPT_FUSED_CYCLES:
@@ -544,8 +784,16 @@ int VFX_FN(vfx_set_option)(vfx_ctx* c, int key, double value) {
             c->fused_jacobi = (int)value;
             return VFX_OK;
         case VFX_OPT_CG_BUDGET:
-            if (value < 0 || value > 1000) return arg_error("CG budget must be 0 (host-polled) or 1..1000");
+            if (value < 0 || value > 1000) return arg_error("CG budget must be 0 (adaptive) or 1..1000");
             c->cg_budget = (int)value;
+            return VFX_OK;
+        case VFX_OPT_SYNC:
+            if (value != 0.0 && value != 1.0) return arg_error("projection sync must be 0 (device) or 1 (polled)");
+            c->sync_mode = (int)value;
+            return VFX_OK;
+        case VFX_OPT_GRAPH:
+            c->use_graph = value != 0.0;
+            if (!c->use_graph) drop_graphs(c);
             return VFX_OK;
         case VFX_OPT_CELLS_PER_THREAD: {
             const int zc = (int)value;
@@ -591,6 +839,7 @@ int VFX_FN(vfx_step_stage)(vfx_ctx* c, int stage, real* data, const real* solid,
         std::memcpy(c->prm_last, prm_host, sizeof(c->prm_last));
         if (int rc = upload(c, prm_host, c->prm, s)) return rc;
     }
+    if (int rc = drain_readbacks(c, false)) return rc;
     switch (stage) {
         case VFX_STAGE_ADVECT: return stage_advect(c, data, solid, s);
         case VFX_STAGE_SCALARS: return stage_scalars(c, data, s);
@@ -598,7 +847,8 @@ int VFX_FN(vfx_step_stage)(vfx_ctx* c, int stage, real* data, const real* solid,
         case VFX_STAGE_PROJECT:
             c->P.setup(solid, s);
             CHECK_LAUNCH("obstacle setup");
-            return stage_project(c, data, s);
+            if (int rc = stage_project(c, data, s)) return rc;
+            return after_projection(c, s);
         default: set_error("unknown stage"); return VFX_E_VALUE;
     }
 }
@@ -610,10 +860,10 @@ int VFX_FN(vfx_step)(vfx_ctx* c, real* data, const real* solid, const float* prm
     hipStream_t s = S(stream);
     std::memcpy(c->prm_last, prm_host, sizeof(c->prm_last));
     if (int rc = upload(c, prm_host, c->prm, s)) return rc;
-    if (int rc = stage_advect(c, data, solid, s)) return rc;
-    if (int rc = stage_scalars(c, data, s)) return rc;
-    if (int rc = stage_forces(c, data, s)) return rc;
-    return stage_project(c, data, s);
+    if (int rc = drain_readbacks(c, false)) return rc;
+    const int rc = graph_capable(c) ? graph_step(c, data, solid, s) : enqueue_step(c, data, solid, s);
+    if (rc) return rc;
+    return after_projection(c, s);
 }
 
 int VFX_FN(vfx_project)(vfx_ctx* c, real* data, const real* solid, const float* prm_host, void* stream) {
@@ -623,9 +873,11 @@ int VFX_FN(vfx_project)(vfx_ctx* c, real* data, const real* solid, const float* 
     hipStream_t s = S(stream);
     std::memcpy(c->prm_last, prm_host, sizeof(c->prm_last));
     if (int rc = upload(c, prm_host, c->prm, s)) return rc;
+    if (int rc = drain_readbacks(c, false)) return rc;
     c->P.setup(solid, s);
     CHECK_LAUNCH("obstacle setup");
-    return stage_project(c, data, s);
+    if (int rc = stage_project(c, data, s)) return rc;
+    return after_projection(c, s);
 }
 
 int VFX_FN(vfx_advect)(vfx_ctx* c, const real* field, int C, const real* vel, real* out, float dt, void* stream) {
@@ -662,32 +914,47 @@ int VFX_FN(vfx_pressure_iters)(vfx_ctx* c, int* out) {
     CHECK_HIP(hipEventSynchronize(c->done_ev), "projection wait");
     DevState hs;
     CHECK_HIP(hipMemcpy(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost), "state read");
-    *out = c->last_iters + hs.it;
+    *out = c->last_iters + hs.it_total + hs.it;
     return VFX_OK;
 }
 
 int VFX_FN(vfx_pressure_stats)(vfx_ctx* c, double* out, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (!out || n < 1) return arg_error("null pointer");
-    double v[9] = {0, 0, 0, 0, 0, -1, 0, 0, 0};
+    double v[VFX_STATS] = {0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (c->projected) {
         CHECK_HIP(hipEventSynchronize(c->done_ev), "projection wait");
+        while (c->rb_count > 0)   // every read-back up to the last projection's (they are done or nearly)
+            if (int rc = drain_readbacks(c, true)) return rc;
         DevState hs;
         CHECK_HIP(hipMemcpy(&hs, c->st, sizeof(DevState), hipMemcpyDeviceToHost), "state read");
-        v[0] = c->last_iters + hs.it;
+        const ReadbackInfo& in = c->last_info;
+        v[0] = c->last_iters + hs.it_total + hs.it;
         v[1] = hs.s[S_MAXU];
         v[2] = std::sqrt(hs.s[S_BB]);
         v[3] = std::sqrt(hs.s[S_RR]);
         v[4] = hs.s[S_MAXR];
         v[5] = c->last_true_valid ? std::sqrt(hs.s[S_TRUE_RR]) : -1.0;
-        v[6] = c->last_restarts;
+        v[6] = c->last_restarts + hs.restarts;
+        bool conv = in.fused ? hs.stop_at != INT_MAX : hs.conv != 0;
 #if !VFX_F64
-        if ((c->net || c->fcg_mode) && !c->last_fallback && c->last_restarts == 0) v[3] = c->last_fcg_rn;
+        if ((c->net || c->fcg_mode) && !c->last_fallback && c->last_restarts == 0) {
+            v[3] = c->last_fcg_rn;
+            conv = true;
+        }
         v[7] = c->last_learned;
         v[8] = c->last_fallback ? 1.0 : 0.0;
 #endif
+        if (c->last_true_valid && in.rel_tol > 0.0f)
+            conv = std::sqrt(hs.s[S_TRUE_RR]) <= (double)in.rel_tol * std::sqrt(hs.s[S_BB]);
+        v[9] = conv ? 1.0 : 0.0;
+        v[10] = in.device ? in.budget : 0;
     }
-    for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
+    v[11] = (double)c->n_hit;
+    v[12] = (double)c->n_unconverged;
+    v[13] = c->graph_captures;
+    v[14] = c->graph_replays;
+    for (int i = 0; i < n && i < VFX_STATS; ++i) out[i] = v[i];
     return VFX_OK;
 }
 

@@ -137,12 +137,15 @@ __device__ __forceinline__ double pow2_scale(double v) { return (v > 0.0 && v < 
 // block sum of b^2 (b = -d(u*)) for the relative-residual test and block max |r| (the scale of
 // the first V-cycle's right-hand side with 16-bit level storage).  Every load is issued
 // unconditionally (clamped neighbours; a solid cell's terms are computed and discarded), so they
-// are all in flight together.
+// are all in flight together.  gated: the rhs of a device-side restart, a no-op unless the
+// restart gate found the solve above tolerance (DevState::live).
 template <bool WARM>
 __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restrict__ vel,
                                              const uint8_t* __restrict__ fm, real* __restrict__ r,
-                                             real* __restrict__ phi, double* __restrict__ part, int nparts) {
+                                             real* __restrict__ phi, double* __restrict__ part, int nparts,
+                                             const DevState* __restrict__ st, int gated) {
     __shared__ double sh[8];
+    if (gated && !st->live) return;
     const int n3 = N * N * N;
     double mx = 0.0, bb = 0.0, mr = 0.0;
     CELL_LOOP_BEGIN
@@ -169,10 +172,13 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restri
     }
 }
 
-// k_fin_maxu (:730-735) + the CG state reset.
+// k_fin_maxu (:730-735) + the CG state reset.  The first CG run of a projection (gated = 0) also
+// clears the projection's restart bookkeeping; a gated restart run keeps it.
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restrict__ part, int nparts,
-                                                 const float* __restrict__ prm, DevState* __restrict__ st) {
+                                                 const float* __restrict__ prm, DevState* __restrict__ st,
+                                                 int gated) {
     __shared__ double sh[FIN_THREADS / 64];
+    if (gated && !st->live) return;
     // the three reductions in one pass (each in reduce_parts' order): a third of the load rounds
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
 #pragma unroll 8
@@ -192,6 +198,13 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
         st->s[S_BB] = bb;
         st->it = 0;
         st->stop_at = INT_MAX;
+        st->conv = 0;
+        if (!gated) {
+            st->it_total = 0;
+            st->restarts = 0;
+            st->hit = 0;
+            st->live = 1;
+        }
     }
 }
 
@@ -490,9 +503,12 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
     }
 }
 
-// k_fin_rz (:737-771) with FP64 scalars, plus the optional relative-residual criterion.
+// k_fin_rz (:737-771) with FP64 scalars, plus the optional relative-residual criterion.  kmax is
+// the iteration budget the host enqueued (iterations after it do not exist): the run stops there
+// like at P_MAXIT, and DevState::hit records (bit hitbit) a stop that only the budget caused.
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict__ part, int nparts,
-                                                const float* __restrict__ prm, DevState* __restrict__ st, int k) {
+                                                const float* __restrict__ prm, DevState* __restrict__ st, int k,
+                                                int kmax, int hitbit) {
     __shared__ double sh[FIN_THREADS / 64];
     if (cg_inactive(st, k)) return;
     double va = 0.0, vb = 0.0, vc = 0.0;
@@ -522,8 +538,32 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict
     st->s[S_BEST] = best;
     const double rel_tol = (double)prm[P_REL_TOL];
     const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
-    const bool go = k < (int)prm[P_MAXIT] && rz_new > 0.0 && mr <= 4.0 * best && not_conv;
-    if (!go) st->stop_at = k;
+    const bool healthy = rz_new > 0.0 && mr <= 4.0 * best;
+    const int maxit = (int)prm[P_MAXIT];
+    const bool go = k < maxit && k < kmax && healthy && not_conv;
+    st->conv = not_conv ? 0 : 1;
+    if (!go) {
+        st->stop_at = k;
+        if (k >= kmax && k < maxit && healthy && not_conv) st->hit |= hitbit;
+    }
+}
+
+// Device-side restart decision after the true residual of a CG run (the host-polled form reads
+// the same test on the host): above the relative tolerance, the next gated run restarts CG from
+// the true residual with phi as its warm start; otherwise every gated kernel after it is a
+// no-op (live = 0, stop_at = -1, which also makes that run's phi update a no-op).
+__global__ void k_restart_gate(const float* __restrict__ prm, DevState* __restrict__ st) {
+    if (threadIdx.x != 0) return;
+    const double rel_tol = (double)prm[P_REL_TOL];
+    const bool done = !(rel_tol > 0.0) || sqrt(st->s[S_TRUE_RR]) <= rel_tol * sqrt(st->s[S_BB]);
+    if (done) {
+        st->live = 0;
+        st->stop_at = -1;
+    } else {
+        st->it_total += st->it;
+        st->restarts += 1;
+        st->live = 1;
+    }
 }
 
 // The phi update of the stopping iteration J (k_upd_p :723-725 of the last loop body):
@@ -559,8 +599,10 @@ __global__ void __launch_bounds__(256) k_apply(int N, int zc, real* __restrict__
 // True residual ||b - K phi||^2 with b = -d(u*) (fresh matvec, FP64 accumulation)
 __global__ void __launch_bounds__(256) k_true_res(int N, int zc, const real* __restrict__ vel,
                                                   const uint8_t* __restrict__ fm, const real* __restrict__ phi,
-                                                  double* __restrict__ part) {
+                                                  double* __restrict__ part, const DevState* __restrict__ st,
+                                                  int gated) {
     __shared__ double sh[8];
+    if (gated && !st->live) return;
     const int n3 = N * N * N;
     double s = 0.0;
     CELL_LOOP_BEGIN
@@ -573,8 +615,9 @@ __global__ void __launch_bounds__(256) k_true_res(int N, int zc, const real* __r
 }
 
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_true(const double* __restrict__ part, int nparts,
-                                                  DevState* __restrict__ st) {
+                                                  DevState* __restrict__ st, int gated) {
     __shared__ double sh[FIN_THREADS / 64];
+    if (gated && !st->live) return;
     const double s = reduce_parts_sum(part, nparts, sh);
     if (threadIdx.x == 0) st->s[S_TRUE_RR] = s;
 }
@@ -627,16 +670,21 @@ void Pressure::precond(int k, hipStream_t s) {
     });
 }
 
-void Pressure::rhs(const real* vel, const float* prm, bool warm, hipStream_t s) {
+void Pressure::rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated) {
     const Geom& g = geom;
+    const int gt = gated ? 1 : 0;
     if (warm)
-        hipLaunchKernelGGL(k_rhs<true>, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, part, nblocks(g));
+        hipLaunchKernelGGL(k_rhs<true>, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, part, nblocks(g), st, gt);
     else
-        hipLaunchKernelGGL(k_rhs<false>, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, part, nblocks(g));
-    hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), prm, st);
+        hipLaunchKernelGGL(k_rhs<false>, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, part, nblocks(g), st, gt);
+    hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), prm, st, gt);
 }
 
-void Pressure::iteration(int k, const float* prm, hipStream_t s) {
+void Pressure::restart_gate(const float* prm, hipStream_t s) {
+    hipLaunchKernelGGL(k_restart_gate, dim3(1), dim3(64), 0, s, prm, st);
+}
+
+void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int hitbit) {
     if (k > 0) {
         // p_{k-1} = z_{k-1} + beta p_{k-2} (k >= 2); p_0 = z_0 was written at k = 0
         if (k >= 2)
@@ -648,7 +696,7 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s) {
         hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, ntiles, st, k);
     }
     precond(k, s);
-    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k);
+    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, kmax, hitbit);
 }
 
 void Pressure::finish(hipStream_t s) {
@@ -661,10 +709,11 @@ void Pressure::apply(real* vel, hipStream_t s) {
     hipLaunchKernelGGL(k_apply, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi);
 }
 
-void Pressure::true_residual(const real* vel, hipStream_t s) {
+void Pressure::true_residual(const real* vel, hipStream_t s, bool gated) {
     const Geom& g = geom;
-    hipLaunchKernelGGL(k_true_res, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi, part);
-    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), st);
+    const int gt = gated ? 1 : 0;
+    hipLaunchKernelGGL(k_true_res, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi, part, st, gt);
+    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), st, gt);
 }
 
 // One kernel of the projection on the current buffers, for timing (ids relative to

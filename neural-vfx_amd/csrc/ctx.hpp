@@ -22,9 +22,63 @@ struct vfx_net {
 
 using namespace vfx;
 
-constexpr int HOST_RING = 4;   // pinned parameter slots (solver.py:81, :331-359)
+constexpr int HOST_RING = 16;  // pinned parameter slots: how many steps the host may run ahead (solver.py:331-359)
 constexpr int FCG_BLOCKS = 1024;   // blocks of the flexible-CG reductions (4 partial arrays)
-constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag
+constexpr int STOP_RING = 4;   // pinned read-back slots of the CG stop flag (host-polled mode)
+constexpr int READBACK_RING = HOST_RING;   // pinned DevState copies of finished projections
+constexpr int BUDGET_WINDOW = 8;   // projections whose iteration counts set the next budget
+constexpr int GRAPH_SLOTS = 4;     // captured step graphs kept per context (least recently used goes)
+
+// projection sequencing (VFX_OPT_SYNC)
+enum : int { SYNC_DEVICE = 0, SYNC_POLL = 1 };
+
+// what the host decides while enqueueing a step: a captured step graph is replayed only while
+// all of it is unchanged
+struct StepSig {
+    const void* data;
+    const void* solid;
+    int v[16];
+};
+
+// per projection in flight: what the host enqueued (the read-back adapts the budgets from it)
+struct ReadbackInfo {
+    int device;     // enqueued with device-side budgets (not host-polled)
+    int fused;      // the fused projection (budget = tail cycles)
+    int budget;     // iterations of the first CG run (fused: tail cycles)
+    int budget_r;   // iterations of the gated restart run (0: none)
+    int cycles;     // fused: V-cycles that always run
+    int judged;     // the true residual was computed
+    float rel_tol;
+};
+
+struct StepGraph {
+    StepSig sig;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;     // recorded after each launch: the exec is destroyed only after it
+    bool launched = false;
+    unsigned long long used = 0;   // replay clock of the last use
+    ReadbackInfo info{};           // what the captured enqueue recorded on the host
+    bool true_valid = false;
+};
+
+// Iteration counts of recent projections -> the next budget (the largest recent count plus a
+// margin, grown at once after a projection that ran out of it).
+struct BudgetWindow {
+    int w[BUDGET_WINDOW] = {};
+    int n = 0, i = 0;
+    int budget = 0;   // 0: no count seen yet
+    void push(int its) {
+        w[i] = its;
+        i = (i + 1) % BUDGET_WINDOW;
+        if (n < BUDGET_WINDOW) ++n;
+    }
+    int max() const {
+        int m = 0;
+        for (int k = 0; k < n; ++k) m = w[k] > m ? w[k] : m;
+        return m;
+    }
+    void reset() { *this = BudgetWindow{}; }
+};
 
 // The finer scalar grid (Solver3D(res, scalar_res=S), hires.py; solver.py:217-264): the caller's
 // fine fields and the library's fine scratch.
@@ -74,7 +128,23 @@ struct vfx_ctx {
     // options
     bool opt_true_res = false;
     int opt_restarts = 0;
-    int cg_budget = 0;   // > 0: exactly this many PCG iterations are enqueued, with no host polling
+    int cg_budget = 0;   // > 0: fixed budget, exactly this many PCG iterations are enqueued
+    int sync_mode = SYNC_DEVICE;
+    bool use_graph = true;   // replay whole steps as captured hipGraphs (device mode)
+    BudgetWindow bw;         // first CG run (fused: tail cycles)
+    BudgetWindow bw_r;       // gated restart run
+    // lazy read-back of finished projections (device mode): FIFO of pinned DevState slots
+    DevState* rb_pin = nullptr;
+    hipEvent_t rb_ev[READBACK_RING] = {};
+    ReadbackInfo rb_info[READBACK_RING] = {};
+    int rb_head = 0, rb_count = 0;
+    ReadbackInfo last_info{};   // of the last enqueued projection
+    long long n_seen = 0, n_hit = 0, n_unconverged = 0;   // read-back projections, budget stops, unconverged
+    // captured step graphs
+    hipStream_t cap = nullptr;
+    StepGraph graphs[GRAPH_SLOTS];
+    unsigned long long graph_clock = 0;
+    int graph_captures = 0, graph_replays = 0;
     // pressure_mode "fused" (stationary multigrid, no CG): cycles, sweeps, smoother (0 Chebyshev, 1 Jacobi)
     bool fused = false;
     int fused_cycles = 2, fused_smooth = 4, fused_jacobi = 0;
@@ -102,11 +172,18 @@ int validate_prm(const float* prm);
 // the three stages of Solver3D.step (solver.py:146-155) on stream s, parameters already in c->prm
 int stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s);
 int stage_forces(vfx_ctx* c, real* data, hipStream_t s);
-// in_frame: part of a batched frame (fixed CG budget, no host synchronisation, no restarts, the
-// caller records done_ev)
+// in_frame: part of a batched frame (the frame's CG budget, device-side sequencing; the caller
+// records done_ev).  Outside frames the caller follows it with the completion event and read-back.
 int stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame = false);
+// destroy the context's captured step graphs (after a change they cannot see, e.g. new fine buffers)
+void drop_graphs(vfx_ctx* c);
+// iteration budget of the first CG run / of the gated restart run for P_MAXIT = maxit
+int cg_budget_main(const vfx_ctx* c, int maxit);
+int cg_budget_restart(const vfx_ctx* c, int maxit);
 // the fused projection of the velocity vel (fused.hip); phi is the context's warm start
-int run_fused(vfx_ctx* c, real* vel, hipStream_t s, bool in_frame);
+// tail_budget >= 0: at most that many adaptive tail cycles are enqueued, with no host polling;
+// -1: the host polls the stop flag after each tail cycle
+int run_fused(vfx_ctx* c, real* vel, hipStream_t s, int tail_budget);
 // the finer scalar grid (hires.hip): the fine stage (advection, mass ratio, sources, diffusion) by
 // the velocity of `data`, and the restriction of the fine scalars into data's density,
 // temperature and fuel channels

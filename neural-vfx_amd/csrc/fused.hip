@@ -158,6 +158,9 @@ __global__ void k_fstart(DevState* st, int cycles) {
         st->it = cycles;
         st->stop_at = INT_MAX;
         st->s[S_MAXR] = -1.0;
+        st->it_total = 0;
+        st->restarts = 0;
+        st->hit = 0;
     }
 }
 
@@ -172,7 +175,8 @@ static std::vector<float> fused_weights(int smooth, int jacobi) {
     return w;
 }
 
-int run_fused(vfx_ctx* c, real* vel, hipStream_t s, bool in_frame) {
+int run_fused(vfx_ctx* c, real* vel, hipStream_t s, int tail_budget) {
+    const bool polled = tail_budget < 0;
     Pressure& P = c->P;
     if (P.mg_kind != MG_REAL) {
         set_error("the fused projection runs with FP32 (or FP64) multigrid levels only");
@@ -229,11 +233,11 @@ int run_fused(vfx_ctx* c, real* vel, hipStream_t s, bool in_frame) {
         hipLaunchKernelGGL(k_fresid<true>, g.grid, g.block, 0, s, g.N, g.zc, cur, vel, P.fm, rr, P.part, nb, P.st, 0);
         hipLaunchKernelGGL(k_ftest, dim3(1), dim3(FIN_THREADS), 0, s, P.part, nb, c->prm, P.st, cycles, 1, 0);
         real* const loop = cur;
-        const int tail = maxit - cycles;
-        if (!in_frame && !hip_ok(hipMemcpyAsync(&c->stop_pin[0], &c->st->stop_at, sizeof(int), hipMemcpyDeviceToHost,
+        const int tail = polled ? maxit - cycles : std::min(maxit - cycles, tail_budget);
+        if (polled && !hip_ok(hipMemcpyAsync(&c->stop_pin[0], &c->st->stop_at, sizeof(int), hipMemcpyDeviceToHost,
                                                 s), "stop read"))
             return VFX_E_HIP;
-        if (!in_frame && !hip_ok(hipEventRecord(c->stop_ev[0], s), "stop event")) return VFX_E_HIP;
+        if (polled && !hip_ok(hipEventRecord(c->stop_ev[0], s), "stop event")) return VFX_E_HIP;
         for (int j = 1; j <= tail; ++j) {
             vcycle(loop, loop, j, true);
             cur = loop;
@@ -246,7 +250,7 @@ int run_fused(vfx_ctx* c, real* vel, hipStream_t s, bool in_frame) {
                                P.st, j);
             hipLaunchKernelGGL(k_ftest, dim3(1), dim3(FIN_THREADS), 0, s, P.part, nb, c->prm, P.st, cycles, 0, j);
             if (!launch_ok("fused tail")) return VFX_E_HIP;
-            if (in_frame) continue;   // a frame enqueues the whole tail (no-ops after the stop)
+            if (!polled) continue;   // the whole budget is enqueued (no-ops after the stop)
             const int slot = j % STOP_RING, prev = (j - 1) % STOP_RING;
             if (!hip_ok(hipMemcpyAsync(&c->stop_pin[slot], &c->st->stop_at, sizeof(int), hipMemcpyDeviceToHost, s),
                         "stop read") ||

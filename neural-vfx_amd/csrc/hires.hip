@@ -483,6 +483,7 @@ extern "C" {
 int VFX_FN(vfx_hires_attach)(vfx_ctx* c, int S, real* hi, const real* hsolid) {
     if (c == nullptr) return arg_error("null context");
     if (S == 0) {
+        drop_graphs(c);
         hires_free(c);
         return VFX_OK;
     }
@@ -495,6 +496,7 @@ int VFX_FN(vfx_hires_attach)(vfx_ctx* c, int S, real* hi, const real* hsolid) {
         return VFX_E_VALUE;
     }
     if (!hi || !hsolid) return arg_error("null fine field pointer");
+    drop_graphs(c);   // captured steps hold the old fine buffers
     HiresGrid& h = c->hr;
     if (h.S != S) {
         hires_free(c);

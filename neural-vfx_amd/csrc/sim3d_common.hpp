@@ -73,10 +73,15 @@ enum : int {
 
 struct DevState {
     double s[S_COUNT];
-    int it;        // CG iterations of the current / last projection
-    int stop_at;   // iteration at which the stop test fired (INT_MAX while running)
+    int it;        // CG iterations of the current / last CG run
+    int stop_at;   // iteration at which the stop test fired (INT_MAX while running; -1 after a
+                   // restart gate that found the solve converged: every gated kernel is a no-op)
     int changed;   // solid differs from the cached face mask -> rebuild operators
-    int pad;
+    int conv;      // the last stop test found the residual at tolerance
+    int it_total;  // iterations of the earlier CG runs of this projection (device-side restarts)
+    int restarts;  // device-side restarts of this projection
+    int live;      // 1 while the projection's current CG run is live (gates the restart kernels)
+    int hit;       // bit 0: the first CG run stopped at its iteration budget unconverged; bit 1: a restart did
 };
 
 constexpr int FLUID = 64;          // face-mask fluid bit (pressure.py:32-33)

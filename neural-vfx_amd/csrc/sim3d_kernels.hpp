@@ -169,13 +169,19 @@ struct Pressure {
     size_t level_floats() const;
     void bind_levels(real* base);
     void setup(const real* solid, hipStream_t s);
-    // warm: the host's copy of the P_WARM parameter (the kernel variant is chosen on the host)
-    void rhs(const real* vel, const float* prm, bool warm, hipStream_t s);
+    // warm: the host's copy of the P_WARM parameter (the kernel variant is chosen on the host);
+    // gated: the rhs of a device-side restart (runs only when the restart gate opened)
+    void rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated = false);
     void precond(int k, hipStream_t s);
-    void iteration(int k, const float* prm, hipStream_t s);
+    // CG iteration k of a run whose host enqueued iterations 0..kmax (hitbit: DevState::hit bit
+    // set when that budget, not the stop test, ends the run)
+    void iteration(int k, const float* prm, hipStream_t s, int kmax = 1 << 30, int hitbit = 1);
     void finish(hipStream_t s);
     void apply(real* vel, hipStream_t s);
-    void true_residual(const real* vel, hipStream_t s);
+    void true_residual(const real* vel, hipStream_t s, bool gated = false);
+    // decide on the device whether the gated restart run after it runs (true residual above the
+    // relative tolerance)
+    void restart_gate(const float* prm, hipStream_t s);
     void vcycle_levels(int k, hipStream_t s);
     bool launch_single(int which, real* scratch, hipStream_t s);
     bool launch_single_mg(int which, hipStream_t s);

@@ -18,6 +18,9 @@ OPT_TRUE_RESIDUAL, OPT_RESTARTS, OPT_COARSE_M, OPT_STD_STENCIL, OPT_MG_STORAGE =
 OPT_FCG_MG, OPT_LEARNED_MAXIT, OPT_LEARNED_SWEEPS = 6, 7, 8
 OPT_CG_BUDGET, OPT_CELLS_PER_THREAD = 9, 10
 OPT_PRESSURE_MODE, OPT_FUSED_CYCLES, OPT_FUSED_SMOOTH, OPT_FUSED_SMOOTHER = 11, 12, 13, 14
+OPT_SYNC, OPT_GRAPH = 15, 16
+SYNC_MODES = {"device": 0, "poll": 1}
+STATS = 15   # doubles of vfx_pressure_stats
 TUNE_MAX_SIZES, TUNE_REPORT = 8, 46
 HIRES_MAX = 1024   # largest scalar_res (vfx_hires_attach)
 MG_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
@@ -56,7 +59,7 @@ SIGNATURES = {
 SIGNATURES.update({name + "_f64": args for name, args in list(SIGNATURES.items())})
 # the learned preconditioner, the batched frames, the paired-run guidance and the renderer (FP32 build only)
 FP32_ONLY = ("vfx_ctx_set_net", "vfx_fcg_history", "vfx_batch_create", "vfx_batch_destroy", "vfx_batch_set_budget",
-             "vfx_batch_get_budget", "vfx_batch_step", "vfx_batch_wait", "vfx_batch_tune", "vfx_guide_create",
+             "vfx_batch_get_budget", "vfx_batch_truncated", "vfx_batch_step", "vfx_batch_wait", "vfx_batch_tune", "vfx_guide_create",
              "vfx_guide_destroy", "vfx_guide_velocity", "vfx_guide_scalars", "vfx_guide_lowpass", "vfx_tracking_error",
              "vfx_render_create", "vfx_render_destroy", "vfx_render_set_output", "vfx_render_set_params",
              "vfx_render_draw", "vfx_render_draw_aux", "vfx_render_get_hdr", "vfx_render_set_temporal",
@@ -79,6 +82,7 @@ SIGNATURES.update({
     "vfx_batch_destroy": [_P],
     "vfx_batch_set_budget": [_P, _I, _I],
     "vfx_batch_get_budget": [_P, _P],
+    "vfx_batch_truncated": [_P, ctypes.POINTER(ctypes.c_longlong)],
     "vfx_batch_step": [_P, _P, _P, _P, _I, _P],
     "vfx_batch_wait": [_P, _P, _P, _P, _P, ctypes.POINTER(_F)],
     "vfx_batch_tune": [_P, _P, _P, _P, _I, _P, _P, _I],

@@ -111,6 +111,13 @@ class BatchSolver3D:
         _lib.check(self._lib.vfx_batch_get_budget(self._h, b.ctypes.data_as(ctypes.c_void_p)))
         return b
 
+    def truncated(self) -> int:
+        """Simulation-frames, among the frames wait() has read, whose projection stopped at its
+        iteration budget unconverged (no wait; wait() doubles that budget)."""
+        n = ctypes.c_longlong()
+        _lib.check(self._lib.vfx_batch_truncated(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     def set_budget(self, budget: int, sim: int = -1) -> None:
         _lib.check(self._lib.vfx_batch_set_budget(self._h, int(sim), int(budget)))
 

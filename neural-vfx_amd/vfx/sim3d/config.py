@@ -42,7 +42,8 @@ class SolverConfig3D:
     heat_diffusion: float = 5.0e-5
     confinement_floor: float = 0.5
     confinement_ref_res: int = 0
-    # reference's opt-in stationary-MG mode; only "pcg" is built here
+    # "pcg" (MGPCG), "fused" (the reference's opt-in stationary multigrid) or "learned" (flexible
+    # CG preconditioned by the network attached with Solver3D.set_preconditioner, MGPCG fallback)
     pressure_mode: str = "pcg"
     pressure_cycles: int = 2
     pressure_smooth: int = 4
@@ -61,6 +62,10 @@ class SolverConfig3D:
     # storage of the multigrid levels: "fp32" (the reference), "fp16" or "bf16" (FP32 arithmetic;
     # CG vectors stay FP32, dots and alpha/beta FP64)
     mg_storage: str = "fp32"
+    # how the host sequences a projection: "device" (no host synchronisation inside step(), as the
+    # reference's on-device loop; iteration budgets adapt to recent projections) or "poll" (the host
+    # reads the stop flag every iteration).  Results are the same while the budget suffices.
+    pressure_sync: str = "device"
 
 
 def fused_config(**kw) -> SolverConfig3D:
@@ -82,9 +87,11 @@ def solver_version_tag(cfg: SolverConfig3D, scalar_res: int | None = None) -> st
     tag = SOLVER3D_VERSION
     if cfg.confinement_ref_res:
         tag += f"+confref{cfg.confinement_ref_res}"
-    if cfg.pressure_mode != "pcg":
-        tag += f"+{cfg.pressure_mode}-c{cfg.pressure_cycles}s{cfg.pressure_smooth}{cfg.pressure_smoother[:4]}"
+    if cfg.pressure_mode == "fused":
+        tag += f"+fused-c{cfg.pressure_cycles}s{cfg.pressure_smooth}{cfg.pressure_smoother[:4]}"
         tag += "-adaptive" if cfg.pressure_tol > 0 else "-fixed"
+    elif cfg.pressure_mode == "learned":
+        tag += "+learned-fcg"
     if scalar_res is not None:
         tag += f"+scalars{scalar_res}"
     if cfg.ignition_ramp != 100.0:

@@ -19,6 +19,7 @@ the device only when `solid` changed.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 
 import numpy as np
 import torch
@@ -86,7 +87,10 @@ def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
 
 
 class Solver3D:
-    HOST_RING = 4
+    HOST_RING = 16
+    # replay each step as a captured hipGraph (device-sequenced projections; the reference's
+    # step is one CUDA graph, solver.py:388-420); False: direct launches
+    STEP_GRAPH = True
 
     def __init__(self, res: int, config: SolverConfig3D | None = None, device="cuda", scalar_res: int | None = None,
                  dtype: torch.dtype = torch.float32):
@@ -117,6 +121,7 @@ class Solver3D:
             self._chk(self._api.vfx_ctx_create(self.res, ctypes.byref(self._ctx)), "Solver3D")
         self._h = np.zeros(P_SIZE, np.float32)
         self._net = None
+        self._attached = None
         self._apply_options()
         if self.scalar_res is not None:
             self._init_hires()
@@ -132,6 +137,17 @@ class Solver3D:
 
     def _apply_options(self):
         self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_RESTARTS, float(self.config.pressure_restarts)))
+        sync = _lib.SYNC_MODES.get(self.config.pressure_sync)
+        if sync is None:
+            raise ValueError(f"pressure_sync must be one of {sorted(_lib.SYNC_MODES)}, got {self.config.pressure_sync!r}")
+        self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_SYNC, float(sync)))
+        self._chk(self._api.vfx_set_option(self._ctx, _lib.OPT_GRAPH, 1.0 if self.STEP_GRAPH else 0.0))
+        if not self._f64:
+            # the network runs only in pressure_mode "learned"; it stays attached on the Python side
+            want = self._net if self.config.pressure_mode == "learned" else None
+            if want is not self._attached:
+                self._chk(self._lib.vfx_ctx_set_net(self._ctx, want._h if want is not None else None))
+                self._attached = want
         kind = _lib.MG_STORAGE.get(self.config.mg_storage)
         if kind is None:
             raise ValueError(f"mg_storage must be one of {sorted(_lib.MG_STORAGE)}, got {self.config.mg_storage!r}")
@@ -150,15 +166,20 @@ class Solver3D:
         self._chk(self._api.vfx_set_option(self._ctx, int(key), float(value)))
 
     def set_preconditioner(self, net) -> None:
-        """Attach a learned preconditioner (vfx.sim3d.learned.UNet of this grid size) or None: the
-        projection then runs flexible CG with it, falling back to plain MGPCG when the residual
-        reduction stalls (pressure_stats()["fallback"])."""
+        """Attach a learned preconditioner (vfx.sim3d.learned.UNet of this grid size) and switch
+        config.pressure_mode to "learned", or detach it (None) and switch back to "pcg": the
+        projection then runs flexible CG with the network, falling back to plain MGPCG when the
+        residual reduction stalls (pressure_stats()["fallback"])."""
         if self._f64:
             raise ValueError("the learned preconditioner runs in the FP32 solver")
         if net is not None and net.res != self.res:
             raise ValueError(f"network grid {net.res} != solver grid {self.res}")
-        self._chk(self._lib.vfx_ctx_set_net(self._ctx, net._h if net is not None else None))
         self._net = net
+        if net is not None:
+            self.config = dataclasses.replace(self.config, pressure_mode="learned")
+        elif self.config.pressure_mode == "learned":
+            self.config = dataclasses.replace(self.config, pressure_mode="pcg")
+        self._apply_options()
 
     @property
     def version(self) -> str:
@@ -298,13 +319,19 @@ class Solver3D:
 
     def pressure_stats(self) -> dict:
         """Diagnostics of the last projection (synchronizes): iterations, max|u*|, ||b||, the
-        recurrence ||r||, max|r|, the true ||b - K phi|| (if enabled) and restarts."""
-        v = (ctypes.c_double * 9)()
-        self._chk(self._api.vfx_pressure_stats(self._ctx, v, 9))
-        keys = ("iters", "max_u", "b_norm", "r_norm", "max_r", "true_r_norm", "restarts", "learned_iters", "fallback")
+        recurrence ||r||, max|r|, the true ||b - K phi|| (if enabled), restarts, whether it
+        converged (judged by the true residual when computed) and its first CG run's iteration
+        budget; over the solver's life so far: projections that stopped at a budget unconverged,
+        unconverged projections, and step graphs captured / replayed."""
+        v = (ctypes.c_double * _lib.STATS)()
+        self._chk(self._api.vfx_pressure_stats(self._ctx, v, _lib.STATS))
+        keys = ("iters", "max_u", "b_norm", "r_norm", "max_r", "true_r_norm", "restarts", "learned_iters", "fallback",
+                "converged", "budget", "budget_stops", "unconverged", "graph_captures", "graph_replays")
         d = dict(zip(keys, list(v)))
-        for k in ("iters", "restarts", "learned_iters", "fallback"):
+        for k in ("iters", "restarts", "learned_iters", "fallback", "budget", "budget_stops", "unconverged",
+                  "graph_captures", "graph_replays"):
             d[k] = int(d[k])
+        d["converged"] = bool(d["converged"])
         d["rel_res"] = d["r_norm"] / d["b_norm"] if d["b_norm"] > 0 else 0.0
         d["true_rel_res"] = d["true_r_norm"] / d["b_norm"] if d["b_norm"] > 0 and d["true_r_norm"] >= 0 else -1.0
         return d
