@@ -103,6 +103,14 @@ int vfx_project(vfx_ctx* ctx, float* data, const float* solid, const float* prm_
  * (Solver3D.advect, solver.py:289-308).  out is (C,N,N,N) and must not alias field. */
 int vfx_advect(vfx_ctx* ctx, const float* field, int C, const float* vel, float* out, float dt, void* stream);
 
+/* vfx_advect without a context and without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and results as vfx_advect on a res^3
+ * context; the scratch is a fixed 28 MiB device array per GPU (held until the process ends), so
+ * res^3 <= 2^20 cells (res even, e.g. up to 100^3; VFX_E_VALUE above).  Calls that overlap on one
+ * GPU (concurrent streams or threads) share that array and race. */
+int vfx_advect_noalloc(const float* field, int C, const float* vel, float* out, int res, float dt, void* stream);
+
 /* |div u| dx per cell, 0 in solids (record_div + k_div_abs, pressure.py:952-956, :156-163);
  * the caller normalises by max|u| like Solver3D.divergence (solver.py:274-287). */
 int vfx_div_abs(vfx_ctx* ctx, const float* data, const float* solid, float* out, void* stream);

@@ -17,7 +17,7 @@ BUILD_DIR = os.path.join(PKG_ROOT, "_build")
 LIB_DIR = os.path.join(PKG_ROOT, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libvfxsim3d.so")
 SOURCES = ("advect.hip", "forces.hip", "pressure.hip", "cg.hip", "capi.hip", "unet.hip", "batch.hip", "fused.hip",
-           "hires.hip", "guide.hip", "render.hip")
+           "hires.hip", "guide.hip", "render.hip", "noalloc.hip")
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [

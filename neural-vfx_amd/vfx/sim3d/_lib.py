@@ -63,7 +63,7 @@ FP32_ONLY = ("vfx_ctx_set_net", "vfx_fcg_history", "vfx_batch_create", "vfx_batc
              "vfx_guide_destroy", "vfx_guide_velocity", "vfx_guide_scalars", "vfx_guide_lowpass", "vfx_tracking_error",
              "vfx_render_create", "vfx_render_destroy", "vfx_render_set_output", "vfx_render_set_params",
              "vfx_render_draw", "vfx_render_draw_aux", "vfx_render_get_hdr", "vfx_render_set_temporal",
-             "vfx_render_reset_history")
+             "vfx_render_reset_history", "vfx_advect_noalloc")
 _F = ctypes.c_float
 SIGNATURES.update({
     "vfx_net_create": [_I, ctypes.POINTER(_P)],
@@ -103,6 +103,7 @@ SIGNATURES.update({
     "vfx_render_get_hdr": [_P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I), _P],
     "vfx_render_set_temporal": [_P, _I, _F, _F, _F, _F],
     "vfx_render_reset_history": [_P],
+    "vfx_advect_noalloc": [_P, _I, _P, _P, _I, _F, _P],
 })
 GUIDE_CFG = ("k_vel", "k_scalar", "k_fuel", "cutoff", "eps_density", "eps_temp", "log_clamp")   # vfx_guide_create cfg
 

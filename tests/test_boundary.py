@@ -31,6 +31,17 @@ def test_library_exports_every_header_symbol(hiplib):
     assert set(syms) == set(_lib.SIGNATURES), "ctypes signatures and the header disagree"
 
 
+def test_advect_noalloc_rejects_grids_above_its_scratch(hiplib):
+    """vfx_advect_noalloc checks the grid against its fixed scratch (2^20 cells) before any HIP call."""
+    import ctypes
+    from vfx.sim3d import _lib
+    fake = ctypes.c_void_p(256)   # never dereferenced: the size check comes first
+    rc = hiplib.vfx_advect_noalloc(fake, 6, fake, fake, 128, ctypes.c_float(0.1), None)
+    assert rc == _lib.VFX_E_VALUE and "no-allocation scratch" in _lib.last_error()
+    rc = hiplib.vfx_advect_noalloc(fake, 6, fake, fake, 7, ctypes.c_float(0.1), None)
+    assert rc == _lib.VFX_E_ARG
+
+
 def test_library_shares_torch_hip_runtime(hiplib):
     maps = open("/proc/self/maps").read()
     rts = sorted(set(re.findall(r"/\S*libamdhip64\S*", maps)))
