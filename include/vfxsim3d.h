@@ -143,7 +143,9 @@ int vfx_get_advected(vfx_ctx* ctx, float* dst, double* mass_ratio, void* stream)
 /* Operator access for parity tests: rebuild masks and multigrid operators for `solid`, then
  * out = K p (vfx_kmul), z = M^-1 r (vfx_precond, the V(1,1) preconditioner), or the level-l
  * operators in component-fastest layout diag [m^3][8], off [m^3][24] (vfx_get_ops; returns
- * the level count in *nlev and the level size in *m; FP32 level storage only). */
+ * the level count in *nlev and the level size in *m; FP32 level storage only).  With 16-bit level
+ * storage vfx_precond scales the level-1 right-hand side by the power of two that brings max|r|
+ * over fluid cells into [1, 2) (the solver takes the scale from the CG residual). */
 int vfx_kmul(vfx_ctx* ctx, const float* solid, const float* p, float* out, void* stream);
 int vfx_precond(vfx_ctx* ctx, const float* solid, const float* r, float* z, void* stream);
 int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float* off, int* nlev, int* m,

@@ -43,6 +43,17 @@ Geom make_geom(int N, int zc) {
     return g;
 }
 
+// the level-1 scale of a standalone V-cycle: 2^-e with 2^e <= max|r| over fluid cells < 2^(e+1)
+// (the solver takes it from the CG residual; it matters only for 16-bit level storage)
+__global__ void __launch_bounds__(FIN_THREADS) k_lscale(size_t n, const real* __restrict__ r,
+                                                        const uint8_t* __restrict__ fm, DevState* __restrict__ st) {
+    __shared__ double sh[FIN_THREADS / 64];
+    double m = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) m = fm[i] ? fmax(m, fabs((double)r[i])) : m;
+    const double v = block_max(m, sh);
+    if (threadIdx.x == 0) st->s[S_LSCALE] = (v > 0.0 && v < 1e300) ? ldexp(1.0, -ilogb(v)) : 1.0;
+}
+
 __global__ void k_reset_cg(DevState* st) {
     if (threadIdx.x == 0) {
         st->it = 0;
@@ -1003,6 +1014,7 @@ int VFX_FN(vfx_precond)(vfx_ctx* c, const real* solid, const real* r, real* z, v
     c->P.setup(solid, s);
     CHECK_HIP(hipMemcpyAsync(c->P.R[0], r, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "r copy");
     hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
+    hipLaunchKernelGGL(k_lscale, dim3(1), dim3(FIN_THREADS), 0, s, n3, c->P.R[0], c->P.fm, c->st);
     c->P.precond(0, s);
     CHECK_LAUNCH("precond");
     CHECK_HIP(hipMemcpyAsync(z, c->P.z, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "z copy");

@@ -248,6 +248,29 @@ def precond_apply(fm, r, prec="f32"):
     return z, float(rz)
 
 
+MG16 = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
+def precond_apply16(fm, r, storage):
+    """z = M^-1 r with the multigrid levels stored as `storage` ("fp32", "fp16", "bf16"; FP32
+    arithmetic), the level-1 right-hand side scaled by the power of two that brings max|r| into
+    [1, 2) (or_precond_apply16)."""
+    fm = _a(fm, np.int32)
+    r = _a(r, np.float32)
+    z = np.empty_like(r)
+    rz = _fn("or_precond_apply16", "f32", ctypes.c_double)(ctypes.c_int(fm.shape[-1]), _p(fm), _p(r), _p(z),
+                                                          ctypes.c_int(MG16[storage]))
+    return z, float(rz)
+
+
+def round16(values, storage):
+    """Each float rounded to the 16-bit storage format and back (or_round16)."""
+    fn = _load().or_round16
+    fn.argtypes = [ctypes.c_float, ctypes.c_int]
+    fn.restype = ctypes.c_float
+    return np.array([fn(float(v), MG16[storage]) for v in np.ravel(values)], np.float32).reshape(np.shape(values))
+
+
 def project(data, solid, prm, phi, prec="f32"):
     """Project data[1:4] in place; phi (warm start) updated in place.  Returns (iters, stats)."""
     assert data.dtype == _dt(prec) and data.flags.c_contiguous

@@ -86,6 +86,44 @@ static inline uint32_t or_philox_u32(uint32_t seed, uint32_t em, uint32_t cell, 
 
 int or_p_size(void) { return P_SIZE; }
 
+/* ---------------------------------------------------------------- 16-bit level storage
+ * The multigrid levels may be stored in IEEE binary16 or in bfloat16 with FP32 arithmetic.  A
+ * stored value is the FP32 value rounded to the nearest representable 16-bit value, ties to even
+ * (binary16: 11 significant bits, exponent range down to the subnormal quantum 2^-24, overflow to
+ * infinity from 65520 = 65504 + half an ulp on; bfloat16: the upper 16 bits of the float after
+ * round-to-nearest-even on the lower 16).  Written from the formats' definitions (IEEE 754-2008
+ * binary16; bfloat16 = float32 with a 7-bit fraction). */
+static inline float or_round_fp16(float f) {
+    if (!(f == f) || f == INFINITY || f == -INFINITY) return f;
+    const float a = fabsf(f);
+    float q;
+    if (a >= 65520.0f) {
+        q = INFINITY;
+    } else if (a < 6.103515625e-05f) {          /* below 2^-14: subnormal binary16, quantum 2^-24 */
+        q = nearbyintf(a * 16777216.0f) / 16777216.0f;
+    } else {
+        int e;
+        frexpf(a, &e);                            /* a = m 2^e, m in [0.5, 1): quantum 2^(e - 11) */
+        const float quantum = ldexpf(1.0f, e - 11);
+        q = nearbyintf(a / quantum) * quantum;
+        if (q > 65504.0f) q = INFINITY;
+    }
+    return copysignf(q, f);
+}
+
+static inline float or_round_bf16(float f) {
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return f;   /* NaN stays NaN */
+    b = (b + 0x7fffu + ((b >> 16) & 1u)) & 0xffff0000u;
+    float r;
+    memcpy(&r, &b, 4);
+    return r;
+}
+
+/* exported for the format pins (tests/test_oracle_pressure.py) */
+float or_round16(float f, int kind) { return kind == 1 ? or_round_fp16(f) : kind == 2 ? or_round_bf16(f) : f; }
+
 #define R float
 #define SFX _f32
 #define EXPR expf

@@ -517,7 +517,19 @@ void F(or_div_abs)(int N, const int* fm, const R* vel3, R* out) {
 
 /* ---------------------------------------------------------------- multigrid (Hadamard basis)
  * pressure.py:178-268 (hsign, tv, gt, axcell, k_probe1, k_galerkin).  Level arrays use the
- * reference layout: x, r, diag (8,m,m,m); off (24,m,m,m) with off[ax*8+c] the +ax link. */
+ * reference layout: x, r, diag (8,m,m,m); off (24,m,m,m) with off[ax*8+c] the +ax link.
+ *
+ * Level storage (an extension: the reference stores FP32): F(g_mg16) = 1 or 2 stores every level
+ * value (x, r, diag, off) rounded to binary16 / bfloat16 (F(qst)), with FP32 arithmetic, and the
+ * level-1 right-hand side scaled by F(g_lscale) (a power of two; the inverse transform divides it
+ * out).  0 (the default, and always in the FP64 instantiation) is the reference's FP32 storage. */
+static int F(g_mg16) = 0;
+static R F(g_lscale) = (R)1;
+static inline R F(qst)(R v) {
+    if (F(g_mg16) == 1) return (R)or_round_fp16((float)v);
+    if (F(g_mg16) == 2) return (R)or_round_bf16((float)v);
+    return v;
+}
 static inline R F(hsign)(int c, int dz, int dy, int dx) {
     int e = (c & 1) * dx + ((c >> 1) & 1) * dy + ((c >> 2) & 1) * dz;
     return (e & 1) ? -(R)HS : (R)HS;
@@ -570,7 +582,7 @@ void F(or_probe1)(int N, const int* fm, R* diag, R* off) {
                                     dsum += v * v;
                                 }
                     size_t li = ((size_t)K * m + J) * m + I;
-                    diag[c * m3 + li] = dsum;
+                    diag[c * m3 + li] = F(qst)(dsum);
                     for (int ax = 0; ax < 3; ++ax) {
                         int nK = K + (ax == 2), nJ = J + (ax == 1), nI = I + (ax == 0);
                         R s = 0;
@@ -583,7 +595,7 @@ void F(or_probe1)(int N, const int* fm, R* diag, R* off) {
                                         s += F(gt)(fm, N, c, K, J, I, z, y, x, ax) * F(gt)(fm, N, c, nK, nJ, nI, z, y, x, ax);
                                     }
                         }
-                        off[(ax * 8 + c) * m3 + li] = s;
+                        off[(ax * 8 + c) * m3 + li] = F(qst)(s);
                     }
                 }
 }
@@ -612,9 +624,9 @@ void F(or_galerkin)(int mf, const R* diag, const R* off, R scale, R* dc, R* oc) 
                                 oo += off[(ax * 8 + c) * mf3 + FI(z, y, x)];
                             }
                         d += (R)2 * oi;
-                        oc[(ax * 8 + c) * m3 + li] = oo * scale;
+                        oc[(ax * 8 + c) * m3 + li] = F(qst)(oo * scale);
                     }
-                    dc[c * m3 + li] = d * scale;
+                    dc[c * m3 + li] = F(qst)(d * scale);
                 }
 #undef FI
 }
@@ -686,7 +698,7 @@ static void F(mg_smooth)(F(levels_t) * L, int l, int color, int zero_init) {
                     if (d <= dmin) { xv[li] = 0; continue; }
                     R acc = r[li];
                     if (!zero_init) acc -= F(offsum)(xv, off, m, c, K, J, I);
-                    xv[li] = acc / d;
+                    xv[li] = F(qst)(acc / d);
                 }
 }
 
@@ -706,7 +718,7 @@ static void F(mg_restrict)(F(levels_t) * L, int l) {
                         size_t fi = c * mf3 + (((size_t)z * mf + y) * mf + x);
                         acc += r[fi] - dg[fi] * xv[fi] - F(offsum)(xv, off, mf, c, z, y, x);
                     }
-                    rc[c * m3 + LI(K, J, I)] = (R)0.125 * acc;
+                    rc[c * m3 + LI(K, J, I)] = F(qst)((R)0.125 * acc);
                 }
 }
 
@@ -723,7 +735,7 @@ static void F(mg_prolong)(F(levels_t) * L, int l) {
                 for (int I = 0; I < m; ++I) {
                     size_t li = c * m3 + LI(K, J, I);
                     if (dg[li] <= dmin) continue;
-                    xv[li] = xv[li] + xc[c * mc3 + (((size_t)(K >> 1) * mc + (J >> 1)) * mc + (I >> 1))];
+                    xv[li] = F(qst)(xv[li] + xc[c * mc3 + (((size_t)(K >> 1) * mc + (J >> 1)) * mc + (I >> 1))]);
                 }
 }
 #undef LI
@@ -743,7 +755,7 @@ static void F(mg_vcycle)(F(levels_t) * L) {
     if (L->m[c] == 1) {
         for (int k = 0; k < 8; ++k) {
             R d = L->diag[c][k];
-            L->x[c][k] = d <= L->dmin[c] ? (R)0 : L->r[c][k] / d;
+            L->x[c][k] = d <= L->dmin[c] ? (R)0 : F(qst)(L->r[c][k] / d);
         }
     } else {
         for (int k = 0; k < 2 * L->m[c]; ++k) {
@@ -771,7 +783,7 @@ static void F(fwd_transform)(int N, const R* rr, R* r1) {
                 for (int c = 0; c < 8; ++c) {
                     R acc = 0;
                     for (int k = 0; k < 8; ++k) acc += F(hsign)(c, k >> 2, (k >> 1) & 1, k & 1) * v[k];
-                    r1[c * m3 + ((size_t)K * m + J) * m + I] = acc;
+                    r1[c * m3 + ((size_t)K * m + J) * m + I] = F(qst)(acc * F(g_lscale));
                 }
             }
 }
@@ -779,6 +791,7 @@ static void F(fwd_transform)(int N, const R* rr, R* r1) {
 /* z_out = base + T^T x1, 0 in solids (k_inv_transform :352-365) */
 static void F(inv_transform)(int N, const R* x1, const int* fm, const R* base, R* zout) {
     int m = N / 2;
+    const R isc = (R)1 / F(g_lscale);
     size_t m3 = (size_t)m * m * m;
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < N; ++z)
@@ -788,7 +801,7 @@ static void F(inv_transform)(int N, const R* x1, const int* fm, const R* base, R
                 if (fm[i] == 0) { zout[i] = 0; continue; }
                 size_t li = ((size_t)(z >> 1) * m + (y >> 1)) * m + (x >> 1);
                 R acc = base[i];
-                for (int c = 0; c < 8; ++c) acc += F(hsign)(c, z & 1, y & 1, x & 1) * x1[c * m3 + li];
+                for (int c = 0; c < 8; ++c) acc += F(hsign)(c, z & 1, y & 1, x & 1) * (x1[c * m3 + li] * isc);
                 zout[i] = acc;
             }
 }
@@ -838,6 +851,23 @@ static void F(precond)(int N, const int* fm, F(levels_t) * L, const R* r, R* z0,
     free(pm);
     *rz = s;
     *maxr = mx;
+}
+
+/* Exported: the same with level storage `kind` (0 FP32, 1 binary16, 2 bfloat16; FP32 instantiation)
+ * and the level-1 right-hand side scaled by 2^-e, 2^e <= max|r| over fluid cells < 2^(e+1) (1 for
+ * FP32 storage): the scale the solver takes from the residual, so 16-bit values sit near 1. */
+double F(or_precond_apply)(int N, const int* fm, const R* r, R* zout);
+double F(or_precond_apply16)(int N, const int* fm, const R* r, R* zout, int kind) {
+    size_t n3 = (size_t)N * N * N;
+    double mr = 0.0;
+    for (size_t i = 0; i < n3; ++i)
+        if (fm[i] && fabs((double)r[i]) > mr) mr = fabs((double)r[i]);
+    F(g_mg16) = kind;
+    F(g_lscale) = kind != 0 && mr > 0.0 && mr < 1e300 ? (R)ldexp(1.0, -ilogb(mr)) : (R)1;
+    const double rz = F(or_precond_apply)(N, fm, r, zout);
+    F(g_mg16) = 0;
+    F(g_lscale) = (R)1;
+    return rz;
 }
 
 /* Exported: build the level hierarchy for a mask and apply z = M^-1 r once (for tests of the

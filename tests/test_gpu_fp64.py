@@ -141,8 +141,10 @@ def test_fp32_against_fp64_run(cuda, hiplib, storage):
 
 @pytest.mark.parametrize("storage,tol", [("fp16", 2e-3), ("bf16", 2e-2)])
 def test_mg16_precond(cuda, hiplib, storage, tol):
-    """A V-cycle with 16-bit level storage stays close to the FP32 one, and its standard-stencil
-    path is bit-identical to reading every coefficient from the (16-bit) level arrays."""
+    """A V-cycle with 16-bit level storage matches the oracle's 16-bit storage path
+    (oracle.precond_apply16: every level value rounded where it is stored, the level-1 right-hand
+    side scaled by a power of two from max|r|) to 1e-6, stays close to the FP32 one, and its
+    standard-stencil path is bit-identical to reading every coefficient from the level arrays."""
     N = 64
     _say(f"precond with {storage} levels N={N}")
     solid = sphere_solid(N)
@@ -160,6 +162,9 @@ def test_mg16_precond(cuda, hiplib, storage, tol):
         torch.cuda.synchronize()
         outs[(kind, std)] = z.cpu()
     assert torch.equal(outs[(storage, 1)], outs[(storage, 0)])
+    ref, _ = oracle.precond_apply16(fm, r.astype(np.float32), storage)
+    eo = rel_err(outs[(storage, 1)].numpy(), ref)
     e = rel_err(outs[(storage, 1)].numpy(), outs[("fp32", 1)].numpy())
-    _say(f"  max rel difference to FP32 levels {e:.3e}")
+    _say(f"  max rel difference to the oracle's {storage} storage {eo:.3e}, to FP32 levels {e:.3e}")
+    assert eo <= 1e-6
     assert e <= tol

@@ -187,6 +187,63 @@ def test_preconditioner_spd_and_reducing(N, r):
     assert np.abs(Mx32 - Mx).max() <= 1e-5 * np.abs(Mx).max()
 
 
+def test_round16_is_the_library_conversion():
+    """The oracle's binary16 / bfloat16 rounding equals torch's float -> half / bfloat16 conversion
+    (round to nearest, ties to even) value for value: normal and subnormal binary16 ranges,
+    overflow to infinity, exact ties, and bfloat16's 7-bit fraction."""
+    import torch
+    rng = np.random.default_rng(7)
+    vals = np.concatenate([
+        rng.standard_normal(4000) * 10.0 ** rng.uniform(-9, 5, 4000),
+        (rng.integers(1, 2048, 500) + 0.5) * 2.0 ** -10,          # exact binary16 ties near 1
+        (rng.integers(1, 1024, 500) + 0.5) * 2.0 ** -24,          # ties in the subnormal range
+        (rng.integers(1, 128, 500) + 0.5) * 2.0 ** -7,            # exact bfloat16 ties near 1
+        [65504.0, 65519.0, 65520.0, 1e6, -65520.0, 6.1035e-05, 6.10351e-05, 0.0, -0.0],
+    ]).astype(np.float32)
+    t = torch.from_numpy(vals)
+    for kind, conv in (("fp16", torch.float16), ("bf16", torch.bfloat16)):
+        got = oracle.round16(vals, kind)
+        ref = t.to(conv).float().numpy()
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), kind
+
+
+def test_precond16_fp32_storage_is_the_fp32_path():
+    """With FP32 storage the 16-bit-capable preconditioner is the pinned FP32 one, bit for bit."""
+    N = 16
+    fm = oracle.mask(sphere_solid(N, r=0.25))
+    r = smooth_noise(N, 1, 3)[0] * (fm != 0)
+    a, _ = oracle.precond_apply(fm, r)
+    b, _ = oracle.precond_apply16(fm, r, "fp32")
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16", 4e-3), ("bf16", 4e-2)])
+def test_precond16_scale_and_closeness(storage, tol):
+    """16-bit level storage: (1) the level-1 right-hand side is scaled by a power of two taken from
+    max|r| and the scale is undone by the inverse transform, so M^-1 (2^k r) = 2^k M^-1 r exactly
+    even for k that would underflow or overflow the 16-bit range without it; (2) the operator
+    stays within the storage precision of the FP32 one; (3) it still reduces a smooth error in the
+    K-norm."""
+    N = 16
+    fm = oracle.mask(sphere_solid(N, r=0.25))
+    fl = (fm != 0).astype(np.float32)
+    r = (smooth_noise(N, 1, 5)[0] * fl).astype(np.float32)
+    z, _ = oracle.precond_apply16(fm, r, storage)
+    for k in (-40, 30):
+        zk, _ = oracle.precond_apply16(fm, (r * np.float32(2.0 ** k)).astype(np.float32), storage)
+        assert np.array_equal(zk, (z * np.float32(2.0 ** k)).astype(np.float32)), k
+    z32, _ = oracle.precond_apply(fm, r)
+    err = np.abs(z - z32).max() / np.abs(z32).max()
+    assert 0 < err <= tol, err
+    e = smooth_noise(N, 1, 9)[0].astype(np.float64) * fl
+    e -= e[fl > 0].mean() * fl
+    Ke = oracle.kmul(fm, e, prec="f64").astype(np.float32)
+    MKe, _ = oracle.precond_apply16(fm, Ke, storage)
+    e2 = e - MKe
+    en = lambda v: float((v * oracle.kmul(fm, v, prec="f64")).sum())
+    assert en(e2) < 0.5 * en(e)
+
+
 def _exact_projection(solid, u):
     """u - g(phi) with K phi = -d(u) solved exactly (sparse LU on the fluid cells, mean fixed)."""
     N = solid.shape[0]
