@@ -27,10 +27,11 @@ COPY_PROBE_BYTES = 256 << 20   # device copy of known size that scales the PMC b
 
 # Algorithmic (compulsory) bytes per grid cell of each kernel, per launch: every distinct input
 # element read once, every output written once, at its storage type (FP32 fields, uint8 mask,
-# FP32 multigrid levels; the level-1 multigrid has N^3 unknowns, 8 per level-1 cell, so its
-# figures are per fine cell too).  Standard multigrid cells take their coefficients from a
-# 32-float table, so the level kernels read a 1-byte flag per level-1 cell (1/8 byte per unknown)
-# instead of diag/off.  The CG kernels are the steady-state forms (iteration k >= 2).
+# multigrid levels of s bytes per value; the level-1 multigrid has N^3 unknowns, 8 per level-1
+# cell, so its figures are per fine cell too).  Standard multigrid cells take their coefficients
+# from a 32-float table, so the level kernels read a 1-byte flag per level-1 cell (1/8 byte per
+# unknown) instead of diag/off.  The CG kernels are the steady-state forms (iteration k >= 2).
+# The table is written for s = 4 (FP32 levels); kernel_bytes_per_cell rescales the level terms.
 KERNEL_BYTES_PER_CELL = {
     "adv_sl": 24 + 24 + 4,              # data (6 ch, velocity 3 of them) -> hat (6 ch), trace corner
     "adv_mc": 24 + 24 + 4 + 1 + 24,     # data + hat + trace corner + mask -> adv
@@ -43,6 +44,16 @@ KERNEL_BYTES_PER_CELL = {
     "restrict0": 4 + 4 + 0.125 + 0.5 + 0.25 + 0.25,   # level-1 r, x + flags -> level-2 r; red: diag -> x
     "prolong0": 2 + 4 + 0.5 + 0.125 + 2,   # black: r, x (all), level-2 x + flags -> x (half)
 }
+
+# level-array bytes (at s = 4) in each kernel's figure above
+LEVEL_BYTES_PER_CELL = {"inv_post": 4, "pre_resid_fwd": 4 + 2, "smooth0": 2 + 4 + 2, "restrict0": 4 + 4 + 0.5 + 0.25,
+                        "prolong0": 2 + 4 + 0.5 + 2}
+
+
+def kernel_bytes_per_cell(name: str, s: int = 4) -> float:
+    """KERNEL_BYTES_PER_CELL with the multigrid level values stored in s bytes."""
+    return KERNEL_BYTES_PER_CELL[name] - LEVEL_BYTES_PER_CELL.get(name, 0.0) * (1.0 - s / 4.0)
+
 
 # Algorithmic bytes per launch of the finer scalar grid's kernels (S^3 fine cells, N^3 coarse
 # cells, FP32, 3 fine channels): every distinct element read once and written once, the coarse
@@ -192,91 +203,121 @@ def net_flops(N: int):
     return alg, exe
 
 
-def learned_report(train_res: int, eval_res, iters: int):
-    """Learned flexible CG against plain MGPCG: the U-Net trained at train_res on solver pairs
-    (seeded scenes 20-27) plus multi-scale noise pairs, then both solvers on the same
-    pre-projection state of the held-out scene 99 at each eval_res (cold start, ||r||/||b|| <=
-    1e-5, true residual judged): pressure ms, iterations, fallback; the network's application ms
-    and its MFMA rate."""
+def train_learned(design: str, train_res: int, iters: int, say=None):
+    """Train the learned part on solver pairs of seeded scenes 20-27 plus multi-scale noise pairs at
+    train_res: design "coarse" (the default) is the coarse correction of the V-cycle (a U-Net on the
+    (train_res/4)^3 grid, trained on what the V-cycle leaves, in the fine energy norm), "full" the
+    full-resolution U-Net preconditioner.  Returns (params, training summary)."""
+    import numpy as np
+    import torch
+    from vfx.sim3d.learned import UNet, generate_pairs, noise_pairs, train, train_coarse, vcycle_pairs
+    t0 = time.perf_counter()
+    pairs = generate_pairs(train_res, seeds=range(20, 28), steps=14, every=1)
+    pairs += noise_pairs(train_res, [p[0] for p in pairs[::11]], count=len(pairs) // (2 if design == "coarse" else 1),
+                         seed=5)
+    if design == "coarse":
+        data = vcycle_pairs(train_res, pairs)
+        net = UNet(train_res // 4, seed=1)
+        losses = train_coarse(net, data, iters=iters, lr=2e-3, seed=0, log=say)
+    else:
+        data = pairs
+        net = UNet(train_res, seed=1)
+        losses = train(net, data, iters=iters, lr=2e-3, seed=0, log=say)
+    torch.cuda.synchronize()
+    summary = {"design": design, "res": train_res, "pairs": len(data), "iters": iters,
+               "seconds": round(time.perf_counter() - t0, 1),
+               "loss_first50": round(float(np.mean(losses[:50])), 4),
+               "loss_last50": round(float(np.mean(losses[-50:])), 4)}
+    params = net.params()
+    del net, pairs, data
+    torch.cuda.empty_cache()
+    return params, summary
+
+
+def compare_learned(N: int, params, design: str, reps: int = 3):
+    """Learned flexible CG (MGPCG fallback) against plain MGPCG on the same pre-projection state of
+    the held-out scene 99 after 6 steps at N^3: cold start, ||r||/||b|| <= 1e-5 with the true
+    residual judged; median wall ms of `reps` projections each, iterations, fallback."""
     import numpy as np
     import torch
     from vfx.core.fields3d import FieldState3D
     from vfx.sim3d import Solver3D, _lib
-    from vfx.sim3d.learned import UNet, generate_pairs, noise_pairs, random_scene, scene_config, train
+    from vfx.sim3d.learned import UNet, random_scene, scene_config
     from vfx.sim3d.solver import pack_params
-    say = lambda *a: print("#", *a, file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    pairs = generate_pairs(train_res, seeds=range(20, 28), steps=14, every=1)
-    pairs += noise_pairs(train_res, [p[0] for p in pairs[::11]], count=len(pairs), seed=5)
-    net = UNet(train_res, seed=1)
-    losses = train(net, pairs, iters=iters, lr=2e-3, seed=0, log=say)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    say(f"trained at {train_res}^3 on {len(pairs)} pairs, {iters} its in {t_train:.1f} s")
-    params = net.params()
-    del net, pairs
-    torch.cuda.empty_cache()
-    out = {"report": "learned flexible CG (U-Net + 2 Jacobi corrections, MGPCG fallback) vs plain MGPCG, "
-                     "held-out scene 99, cold start, rel-res 1e-5",
-           "train": {"res": train_res, "pairs": "8 scenes x 11 steps + as many noise pairs", "iters": iters,
-                     "seconds": round(t_train, 1), "loss_first50": round(float(np.mean(losses[:50])), 4),
-                     "loss_last50": round(float(np.mean(losses[-50:])), 4)}}
     forces, obst, beta = random_scene(99)
-    for N in eval_res:
-        cfg = dataclasses.replace(scene_config(beta, 99), pressure_warm_start=False, pressure_rel_tol=1e-5,
-                                  pressure_max_iters=400, pressure_restarts=2)
-        s = Solver3D(N, cfg)
-        st = s.reset(obst)
-        for _ in range(6):
-            s.step(st, forces, 1.0 / 60.0)
-        h = pack_params(cfg, N, forces, 1.0 / 60.0, st.meta.step)
-        for stage in (_lib.STAGE_ADVECT, _lib.STAGE_FORCES):
-            s._chk(s._api.vfx_step_stage(s._ctx, stage, ctypes.c_void_p(st.data.data_ptr()),
-                                          ctypes.c_void_p(st.solid.data_ptr()),
-                                          h.ctypes.data_as(ctypes.c_void_p) if stage == 0 else None, s._stream()))
-        torch.cuda.synchronize()
-        netN = UNet(N, params=params)
-        row = {}
-        for name in ("mgpcg", "learned"):
-            s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
-            s.set_preconditioner(netN if name == "learned" else None)
-            times = []
-            for r in range(3):
-                work = FieldState3D(data=st.data.clone(), solid=st.solid, meta=st.meta)
-                torch.cuda.synchronize()
-                t1 = time.perf_counter()
-                s.project(work)
-                ps = s.pressure_stats()   # synchronizes
-                times.append(1000.0 * (time.perf_counter() - t1))
-            row[name] = {"pressure_ms": round(float(np.median(times)), 3), "mgpcg_iters": ps["iters"],
-                         "learned_iters": ps["learned_iters"], "fallback": ps["fallback"],
-                         "true_rel_res": float(f"{ps['true_rel_res']:.3g}")}
-            if name == "learned":
-                row[name]["history_first"] = [float(f"{x / max(ps['b_norm'], 1e-300):.2e}")
-                                              for x in s.fcg_history()[:8]]
-        s.set_preconditioner(None)
-        # the network's application alone
-        r = torch.randn(N, N, N, device="cuda") * (st.solid < 0.5)
-        for _ in range(2):
-            netN.apply(s, st.solid, r)
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        reps = 10
-        ev[0].record()
+    cfg = dataclasses.replace(scene_config(beta, 99), pressure_warm_start=False, pressure_rel_tol=1e-5,
+                              pressure_max_iters=400, pressure_restarts=2)
+    s = Solver3D(N, cfg)
+    st = s.reset(obst)
+    for _ in range(6):
+        s.step(st, forces, 1.0 / 60.0)
+    h = pack_params(cfg, N, forces, 1.0 / 60.0, st.meta.step)
+    for stage in (_lib.STAGE_ADVECT, _lib.STAGE_FORCES):
+        s._chk(s._api.vfx_step_stage(s._ctx, stage, ctypes.c_void_p(st.data.data_ptr()),
+                                      ctypes.c_void_p(st.solid.data_ptr()),
+                                      h.ctypes.data_as(ctypes.c_void_p) if stage == 0 else None, s._stream()))
+    torch.cuda.synchronize()
+    netN = UNet(N // 4 if design == "coarse" else N, params=params)
+    s.set_option(_lib.OPT_LEARNED_SWEEPS, 0 if design == "coarse" else 2)
+    row = {}
+    for name in ("mgpcg", "learned"):
+        s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
+        s.set_preconditioner(netN if name == "learned" else None)
+        times = []
         for _ in range(reps):
-            netN.apply(s, st.solid, r)
-        ev[1].record()
-        ev[1].synchronize()
-        ms = ev[0].elapsed_time(ev[1]) / reps
-        alg, exe = net_flops(N)
-        row["net_apply_ms"] = round(ms, 3)
-        row["net_tflops_alg"] = round(alg / 1e12, 4)
-        row["net_mfma_tflops_per_s"] = round(exe / ms / 1e9, 1)
-        row["net_mfma_frac_of_bf16_dense_peak"] = round(exe / ms / 1e9 / 2500.0, 4)
-        row["speedup_learned_vs_mgpcg"] = round(row["mgpcg"]["pressure_ms"] / row["learned"]["pressure_ms"], 3)
+            work = FieldState3D(data=st.data.clone(), solid=st.solid, meta=st.meta)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            s.project(work)
+            ps = s.pressure_stats()   # synchronizes
+            times.append(1000.0 * (time.perf_counter() - t1))
+        row[name] = {"pressure_ms": round(float(np.median(times)), 3), "mgpcg_iters": ps["iters"],
+                     "learned_iters": ps["learned_iters"], "fallback": ps["fallback"],
+                     "true_rel_res": float(f"{ps['true_rel_res']:.3g}")}
+        if name == "learned":
+            row[name]["history_first"] = [float(f"{x / max(ps['b_norm'], 1e-300):.2e}") for x in s.fcg_history()[:10]]
+    s.set_preconditioner(None)
+    # the network's application alone (on its own grid)
+    Nn = netN.res
+    ss = Solver3D(Nn)
+    sol = torch.zeros(Nn, Nn, Nn, device="cuda")
+    r = torch.randn(Nn, Nn, Nn, device="cuda")
+    for _ in range(2):
+        netN.apply(ss, sol, r)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(10):
+        netN.apply(ss, sol, r)
+    ev[1].record()
+    ev[1].synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 10
+    alg, exe = net_flops(Nn)
+    row["net_grid"] = Nn
+    row["net_apply_ms"] = round(ms, 3)
+    row["net_mfma_tflops_per_s"] = round(exe / ms / 1e9, 1)
+    row["net_mfma_frac_of_bf16_dense_peak"] = round(exe / ms / 1e9 / 2500.0, 4)
+    row["speedup_learned_vs_mgpcg"] = round(row["mgpcg"]["pressure_ms"] / row["learned"]["pressure_ms"], 3)
+    del s, st, netN, ss, r
+    torch.cuda.empty_cache()
+    return row
+
+
+def learned_report(train_res: int, eval_res, iters: int, design: str = "coarse"):
+    """Learned flexible CG against plain MGPCG at each eval_res (compare_learned), after training
+    (train_learned)."""
+    import torch
+    say = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    params, summary = train_learned(design, train_res, iters, say)
+    say(f"trained: {summary}")
+    out = {"report": "learned flexible CG (" + ("MG V-cycle + coarse U-Net correction" if design == "coarse" else
+                                                "full-resolution U-Net + 2 Jacobi corrections")
+                     + ", MGPCG fallback) vs plain MGPCG, held-out scene 99, cold start, rel-res 1e-5",
+           "train": summary}
+    for N in eval_res:
+        row = compare_learned(N, params, design)
         say(f"{N}^3: {row}")
         out[str(N)] = row
-        del s, st, netN, r
-        torch.cuda.empty_cache()
+    torch.cuda.empty_cache()
     print(json.dumps(out))
 
 
@@ -648,8 +689,9 @@ def main():
     ap.add_argument("--scalar-res", type=int, default=0, metavar="S",
                     help="with the finer scalar grid (Solver3D(res, scalar_res=S)): density, temperature and "
                          "fuel carried on S^3 and restricted each step; the fine kernels join the table")
-    ap.add_argument("--mg", default="fp32", choices=("fp32", "fp16", "bf16"),
-                    help="multigrid level storage (the reference stores FP32)")
+    ap.add_argument("--mg", default="fp16", choices=("fp32", "fp16", "bf16"),
+                    help="multigrid level storage: FP16 with FP32 arithmetic (the default; the 256^3 mixed-precision "
+                         "test keeps the iteration counts and the FP64 errors), or FP32 as the reference stores it")
     ap.add_argument("--precision-report", type=int, default=0, metavar="STEPS",
                     help="instead of the bench line: FP32 (FP32/FP16/BF16 levels) against the FP64 run of the "
                          "smoke plume past a sphere after STEPS steps at --res")
@@ -664,7 +706,13 @@ def main():
     ap.add_argument("--learned", type=int, default=0, metavar="ITERS",
                     help="instead of the bench line: train the U-Net ITERS Adam steps at --train-res, then "
                          "learned flexible CG vs MGPCG at 64^3, 128^3 and 256^3")
-    ap.add_argument("--train-res", type=int, default=64)
+    ap.add_argument("--train-res", type=int, default=128)
+    ap.add_argument("--learned-design", default="coarse", choices=("coarse", "full"),
+                    help="the coarse correction of the V-cycle (a U-Net on the N/4 grid) or the full-resolution "
+                         "U-Net preconditioner")
+    ap.add_argument("--no-learned", action="store_true", help="bench line without the learned-vs-MGPCG keys")
+    ap.add_argument("--learned-iters", type=int, default=600,
+                    help="training iterations of the coarse correction for the bench line's learned-vs-MGPCG keys")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: one 256 MiB device copy first (the scale reference), "
                          "2 steps, then one launch of every timed kernel; prints no bench line")
@@ -682,7 +730,7 @@ def main():
         return
     if args.learned > 0:
         torch.cuda.set_device(torch.device("cuda:0"))
-        learned_report(args.train_res, (64, 128, 256), args.learned)
+        learned_report(args.train_res, (64, 128, 256), args.learned, args.learned_design)
         return
     if args.batch > 0:
         torch.cuda.set_device(torch.device("cuda:0"))
@@ -709,6 +757,7 @@ def main():
     N = args.res
     dt = 1.0 / 60.0
     cfg = dataclasses.replace(fire_config(args.seed), mg_storage=args.mg)
+    mg_bytes = 4 if args.mg == "fp32" else 2
     fused = args.pressure == "fused"
     if fused:
         cfg = dataclasses.replace(cfg, pressure_mode="fused", pressure_tol=5e-5, pressure_max_iters=24)
@@ -733,7 +782,8 @@ def main():
             s._lib.vfx_time_kernel(s._ctx, _lib.KERNELS.index(name), 1, s._stream(), ctypes.byref(t))
         torch.cuda.synchronize()
         print(json.dumps({"pmc_probe": True, "res": N, "copy_bytes": COPY_PROBE_BYTES,
-                          "kernel_bytes_per_launch": {k: int(v * N ** 3) for k, v in KERNEL_BYTES_PER_CELL.items()}}))
+                          "kernel_bytes_per_launch": {k: int(kernel_bytes_per_cell(k, mg_bytes) * N ** 3)
+                                                      for k in KERNEL_BYTES_PER_CELL}}))
         return
     print(f"# {N}^3 fire: {args.presim} pre-simulation + {args.warmup} warm-up steps", file=sys.stderr, flush=True)
     for i in range(args.presim + args.warmup):
@@ -767,7 +817,7 @@ def main():
         if s._lib.vfx_time_kernel(s._ctx, _lib.KERNELS.index(name), 20, s._stream(), ctypes.byref(t)) != 0:
             continue
         row = {"us": round(t.value * 1000.0, 1), "per_step_us": round(t.value * 1000.0 * launches_per_step(name, n_it), 1)}
-        nbytes = KERNEL_BYTES_PER_CELL[name] * N ** 3 if name in KERNEL_BYTES_PER_CELL else \
+        nbytes = kernel_bytes_per_cell(name, mg_bytes) * N ** 3 if name in KERNEL_BYTES_PER_CELL else \
             fine_kernel_bytes(name, N, S or N)
         if S is None and name.startswith("f"):
             continue
@@ -777,7 +827,8 @@ def main():
             gbs = nbytes / (t.value * 1e-3) / 1e9
             row.update(gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 3))
         kern[name] = row
-    byte_of = lambda k: KERNEL_BYTES_PER_CELL[k] * N ** 3 if k in KERNEL_BYTES_PER_CELL else fine_kernel_bytes(k, N, S)
+    byte_of = lambda k: kernel_bytes_per_cell(k, mg_bytes) * N ** 3 if k in KERNEL_BYTES_PER_CELL else \
+        fine_kernel_bytes(k, N, S)
     kname = args.roofline_kernel or max((k for k in kern if "gbs" in kern[k]), key=lambda k: kern[k]["per_step_us"])
     kbytes = byte_of(kname)
     kms = kern[kname]["us"] / 1000.0
@@ -785,7 +836,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(kname, N, S), "kernel": kname,
                 "kernel_ms": round(kms, 4), "bytes_per_launch": int(kbytes)}
-    sb = step_bytes_per_cell(n_it) * N ** 3
+    sb = step_bytes_per_cell(n_it, mg_bytes) * N ** 3
     step_roof = {"bytes_per_step": int(sb), "achieved": round(sb / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
                  "frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
@@ -794,6 +845,16 @@ def main():
         print("# CPU baseline (oracle, 5 steps)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(s, st.data, st.solid, cfg, forces, dt, N, st.meta.step)
 
+    learned = None
+    if not args.no_learned and S is None and not fused:
+        # learned flexible CG (V-cycle + coarse U-Net correction, MGPCG fallback) against plain MGPCG
+        # on the same 256^3 pre-projection state, both to true rel-res 1e-5 from a cold start
+        say = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+        say(f"learned vs MGPCG: training the coarse correction at {args.train_res}^3 ({args.learned_iters} iterations)")
+        params, summary = train_learned("coarse", args.train_res, args.learned_iters, None)
+        row = compare_learned(N, params, "coarse")
+        say(f"learned {row['learned']['pressure_ms']} ms vs MGPCG {row['mgpcg']['pressure_ms']} ms")
+        learned = {"train": summary, **row}
     if fused or S is not None:
         step_roof = None   # the byte formula is the plain MGPCG step's
     solve = ("fused projection (2 V-cycles + 4 Chebyshev sweeps, adaptive to normalised residual 5e-5)" if fused
@@ -816,6 +877,9 @@ def main():
         "projections": {"sync": cfg.pressure_sync, "unconverged": run_stats["unconverged"],
                         "budget_stops": run_stats["budget_stops"], "budget": run_stats["budget"],
                         "graph_captures": run_stats["graph_captures"], "graph_replays": run_stats["graph_replays"]},
+        "learned_pressure_ms": learned["learned"]["pressure_ms"] if learned else None,
+        "mgpcg_pressure_ms": learned["mgpcg"]["pressure_ms"] if learned else None,
+        "learned": learned,
         "roofline": roofline, "step_roofline": step_roof, "cpu_baseline": cpu, "kernels": kern,
     }
     print(json.dumps(out))

@@ -297,9 +297,16 @@ int vfx_net_adam(vfx_net* net, const float* grad, float lr, float beta1, float b
  * host (synchronises) when not NULL */
 int vfx_net_train_step(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* b, const float* phi, float lr,
                        double* loss, void* stream);
-/* attach a network (or NULL to detach) to a solver context of the same grid size: its
- * projections then run flexible CG (Polak-Ribiere form) with the network as the preconditioner,
- * falling back to plain MGPCG from the current pressure when ||r|| falls less than 2x over 3
+/* the same for the coarse correction: a network of a quarter of the solver's grid size trained on
+ * a fine pair (r_V, e_V), the residual and the error a V-cycle left (K e_V = r_V): c = P net(R r_V)
+ * (R the 4^3 block mean, P constant over each block's fluid cells), loss e.K e / e_V.r_V with
+ * e = c - e_V in the fine energy norm.  Uses the context's step scratch. */
+int vfx_net_train_step_coarse(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* rv, const float* ev,
+                              float lr, double* loss, void* stream);
+/* attach a network (or NULL to detach) to a solver context: its projections then run flexible CG
+ * (Polak-Ribiere form).  A network of the solver's grid size is the preconditioner; one of a
+ * quarter of it is the coarse correction of the MG V-cycle, z = V(r) + P net(R (r - K V(r))).
+ * Falls back to plain MGPCG from the current pressure when ||r|| falls less than 2x over 3
  * iterations, turns non-finite or VFX_OPT_LEARNED_MAXIT runs out; the true residual judges as for
  * MGPCG.  The context does not own the network. */
 int vfx_ctx_set_net(vfx_ctx* ctx, vfx_net* net);

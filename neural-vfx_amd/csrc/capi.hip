@@ -44,14 +44,28 @@ Geom make_geom(int N, int zc) {
 }
 
 // the level-1 scale of a standalone V-cycle: 2^-e with 2^e <= max|r| over fluid cells < 2^(e+1)
-// (the solver takes it from the CG residual; it matters only for 16-bit level storage)
-__global__ void __launch_bounds__(FIN_THREADS) k_lscale(size_t n, const real* __restrict__ r,
-                                                        const uint8_t* __restrict__ fm, DevState* __restrict__ st) {
-    __shared__ double sh[FIN_THREADS / 64];
+// (the solver takes it from the CG residual; it matters only for 16-bit level storage): block
+// maxima, then the scale in one workgroup
+constexpr int LSCALE_BLOCKS = 1024;
+__global__ void __launch_bounds__(256) k_lscale_part(size_t n, const real* __restrict__ r,
+                                                     const uint8_t* __restrict__ fm, double* __restrict__ part) {
+    __shared__ double sh[4];
     double m = 0.0;
-    for (size_t i = threadIdx.x; i < n; i += blockDim.x) m = fm[i] ? fmax(m, fabs((double)r[i])) : m;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fm[i] ? fmax(m, fabs((double)r[i])) : m;
     const double v = block_max(m, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(FIN_THREADS) k_lscale_fin(const double* __restrict__ part, DevState* __restrict__ st) {
+    __shared__ double sh[FIN_THREADS / 64];
+    const double v = reduce_parts_max(part, LSCALE_BLOCKS, sh);
     if (threadIdx.x == 0) st->s[S_LSCALE] = (v > 0.0 && v < 1e300) ? ldexp(1.0, -ilogb(v)) : 1.0;
+}
+
+static void launch_lscale(vfx_ctx* c, const real* r, hipStream_t s) {
+    hipLaunchKernelGGL(k_lscale_part, dim3(LSCALE_BLOCKS), dim3(256), 0, s, c->g.n3, r, c->P.fm, c->P.part);
+    hipLaunchKernelGGL(k_lscale_fin, dim3(1), dim3(FIN_THREADS), 0, s, c->P.part, c->st);
 }
 
 __global__ void k_reset_cg(DevState* st) {
@@ -145,7 +159,7 @@ int vfx::stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
 //  - polled: the host enqueues iteration k, then waits for iteration k-1's stop flag (the learned
 //    path's MGPCG fallback and VFX_OPT_SYNC = 1).
 constexpr int BUDGET_START = 32;        // first budget of a context, before any read-back arrived
-constexpr int RESTART_BUDGET_START = 2;
+constexpr int RESTART_BUDGET_START = 1;   // a restart from a true residual just above tolerance needs ~1
 constexpr int FUSED_TAIL_START = 1 << 20;   // fused projection: the whole tail until a count is known
 constexpr int BUDGET_MARGIN = 1;        // iterations enqueued beyond the largest recent count
 constexpr int BUDGET_CAP = 1000;
@@ -334,6 +348,12 @@ __global__ void __launch_bounds__(256) k_fcg_jacobi(size_t n, const real* __rest
     if (kopen(m)) z[i] = z[i] + OMEGA_JAC * (r[i] - t[i]) / kdiag(m);
 }
 
+__global__ void __launch_bounds__(256) k_fcg_sub(size_t n, const real* __restrict__ a, const real* __restrict__ b,
+                                                 real* __restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
 __global__ void __launch_bounds__(256) k_fcg_p(size_t n, const real* __restrict__ z, real beta, real* __restrict__ p) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = z[i] + beta * p[i];
@@ -388,11 +408,20 @@ static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) 
         // z = M(r)
         if (c->fcg_mode == 2) {
             CHECK_HIP(hipMemcpyAsync(z, r, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "fcg identity");
-        } else if (c->fcg_mode == 1) {
+        } else if (c->fcg_mode == 1 || c->net->net.N * 4 == c->N) {
+            // the MG V-cycle; with a coarse network, plus its correction of the residual the
+            // V-cycle leaves: z = V(r) + P net(R (r - K V(r)))
             CHECK_HIP(hipMemcpyAsync(P.R[0], r, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "fcg mg input");
             hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
+            launch_lscale(c, P.R[0], s);
             P.precond(0, s);
             CHECK_HIP(hipMemcpyAsync(z, P.z, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "fcg mg output");
+            if (c->fcg_mode != 1) {
+                real* rv = c->hat + 5 * n3;
+                launch_kmul(c->g, z, P.fm, q, s);
+                hipLaunchKernelGGL(k_fcg_sub, dim3(nb), dim3(256), 0, s, n3, r, q, rv);
+                c->net->net.apply_coarse(c->N, rv, P.fm, z, s);
+            }
         } else {
             c->net->net.apply(r, P.fm, z, s);
             for (int sw = 0; sw < c->learned_sweeps; ++sw) {
@@ -1014,7 +1043,7 @@ int VFX_FN(vfx_precond)(vfx_ctx* c, const real* solid, const real* r, real* z, v
     c->P.setup(solid, s);
     CHECK_HIP(hipMemcpyAsync(c->P.R[0], r, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "r copy");
     hipLaunchKernelGGL(k_reset_cg, dim3(1), dim3(64), 0, s, c->st);
-    hipLaunchKernelGGL(k_lscale, dim3(1), dim3(FIN_THREADS), 0, s, n3, c->P.R[0], c->P.fm, c->st);
+    launch_lscale(c, c->P.R[0], s);
     c->P.precond(0, s);
     CHECK_LAUNCH("precond");
     CHECK_HIP(hipMemcpyAsync(z, c->P.z, n3 * sizeof(real), hipMemcpyDeviceToDevice, s), "z copy");
@@ -1108,7 +1137,9 @@ int vfx_fcg_history(vfx_ctx* c, double* out, int n, int* count) {
 
 int vfx_ctx_set_net(vfx_ctx* c, vfx_net* n) {
     if (int rc = check_ctx(c)) return rc;
-    if (n && n->net.N != c->N) return arg_error("network and solver grid sizes differ");
+    if (n && n->net.N != c->N && n->net.N * 4 != c->N)
+        return arg_error("the network's grid must be the solver's (the preconditioner) or a quarter of it (the "
+                         "coarse correction of the V-cycle)");
     c->net = n;
     return VFX_OK;
 }
@@ -1122,6 +1153,22 @@ int vfx_net_train_step(vfx_net* n, vfx_ctx* c, const real* solid, const real* b,
     c->P.setup(solid, s);
     net_train_step(n->net, c->g, b, phi, c->P.fm, lr, s);
     CHECK_LAUNCH("network training step");
+    if (loss) {
+        CHECK_HIP(hipMemcpyAsync(loss, n->net.scal + 4, sizeof(double), hipMemcpyDeviceToHost, s), "loss copy");
+        CHECK_HIP(hipStreamSynchronize(s), "loss copy");
+    }
+    return VFX_OK;
+}
+
+int vfx_net_train_step_coarse(vfx_net* n, vfx_ctx* c, const real* solid, const real* rv, const real* ev, float lr,
+                              double* loss, void* stream) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!n || !solid || !rv || !ev) return arg_error("null pointer");
+    if (n->net.N * 4 != c->N) return arg_error("the coarse correction's grid must be a quarter of the solver's");
+    hipStream_t s = S(stream);
+    c->P.setup(solid, s);
+    net_train_step_coarse(n->net, c->g, rv, ev, c->P.fm, lr, c->hat, s);
+    CHECK_LAUNCH("coarse correction training step");
     if (loss) {
         CHECK_HIP(hipMemcpyAsync(loss, n->net.scal + 4, sizeof(double), hipMemcpyDeviceToHost, s), "loss copy");
         CHECK_HIP(hipStreamSynchronize(s), "loss copy");

@@ -124,6 +124,81 @@ __global__ void __launch_bounds__(256) k_dout(size_t n, const real* __restrict__
     dout[v] = (fm[v] && pb > 0.0) ? (float)(scal[1] * 2.0 * (double)ke[v] / pb) : 0.0f;
 }
 
+// ---------------------------------------------------------------- coarse correction
+// The network run on the (N/4)^3 grid of 4^3 blocks of an N^3 field: the fine input is the block
+// mean, the coarse output is spread back to the block's fluid cells (constant prolongation).
+
+// c[B] = mean of the 64 fine values of block B (solid cells hold 0)
+__global__ void __launch_bounds__(256) k_pool4(int N, const real* __restrict__ f, real* __restrict__ c) {
+    const int M = N >> 2;
+    const size_t m3 = (size_t)M * M * M;
+    const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (b >= m3) return;
+    const int I = (int)(b % M), J = (int)((b / M) % M), K = (int)(b / ((size_t)M * M));
+    float acc = 0.0f;
+    for (int dz = 0; dz < 4; ++dz)
+        for (int dy = 0; dy < 4; ++dy) {
+            const real* row = f + ((size_t)(4 * K + dz) * N + (4 * J + dy)) * N + 4 * I;
+            const float4 v = *reinterpret_cast<const float4*>(row);
+            acc += (v.x + v.y) + (v.z + v.w);
+        }
+    c[b] = acc * (1.0f / 64.0f);
+}
+
+// coarse fluid flag: 1 where any of the block's 64 cells is fluid
+__global__ void __launch_bounds__(256) k_mask4(int N, const uint8_t* __restrict__ fm, uint8_t* __restrict__ cm) {
+    const int M = N >> 2;
+    const size_t m3 = (size_t)M * M * M;
+    const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (b >= m3) return;
+    const int I = (int)(b % M), J = (int)((b / M) % M), K = (int)(b / ((size_t)M * M));
+    int any = 0;
+    for (int dz = 0; dz < 4; ++dz)
+        for (int dy = 0; dy < 4; ++dy) {
+            const uint32_t w = *reinterpret_cast<const uint32_t*>(fm + ((size_t)(4 * K + dz) * N + (4 * J + dy)) * N + 4 * I);
+            any |= w != 0u;
+        }
+    cm[b] = any ? 1 : 0;
+}
+
+// fine cell v of block B: ADD: z[v] += rms * out[B] (fluid cells; solids keep z); ERR: e[v] =
+// rms * out[B] - t[v] in fluid cells, 0 in solids
+template <bool ADD>
+__global__ void __launch_bounds__(256) k_up4(int N, const float* __restrict__ out, const double* __restrict__ scal,
+                                             const uint8_t* __restrict__ fm, real* __restrict__ z,
+                                             const real* __restrict__ t) {
+    const size_t n3 = (size_t)N * N * N;
+    const size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (v >= n3) return;
+    const int x = (int)(v % N), y = (int)((v / N) % N), zc = (int)(v / ((size_t)N * N));
+    const int M = N >> 2;
+    const float c = out[((size_t)(zc >> 2) * M + (y >> 2)) * M + (x >> 2)] * (float)scal[1];
+    if (ADD) {
+        if (fm[v]) z[v] = z[v] + c;
+    } else {
+        z[v] = fm[v] ? c - t[v] : RL(0.0);
+    }
+}
+
+// d loss / d out[B] = rms * sum over the block's fluid cells of 2 (K e) / (e_V . K e_V)
+__global__ void __launch_bounds__(256) k_dout4(int N, const real* __restrict__ ke, const uint8_t* __restrict__ fm,
+                                               const double* __restrict__ scal, float* __restrict__ dout) {
+    const int M = N >> 2;
+    const size_t m3 = (size_t)M * M * M;
+    const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (b >= m3) return;
+    const int I = (int)(b % M), J = (int)((b / M) % M), K = (int)(b / ((size_t)M * M));
+    double acc = 0.0;
+    for (int dz = 0; dz < 4; ++dz)
+        for (int dy = 0; dy < 4; ++dy)
+            for (int dx = 0; dx < 4; ++dx) {
+                const size_t v = ((size_t)(4 * K + dz) * N + (4 * J + dy)) * N + 4 * I + dx;
+                acc += fm[v] ? (double)ke[v] : 0.0;
+            }
+    const double pb = scal[3];
+    dout[b] = pb > 0.0 ? (float)(scal[1] * 2.0 * acc / pb) : 0.0f;
+}
+
 // ---------------------------------------------------------------- launch helpers
 static unsigned blocks_for(size_t items, int threads = 256) { return (unsigned)((items + threads - 1) / threads); }
 
@@ -219,6 +294,7 @@ int Net::create(int N_) {
     sz.push_back(al(16 * 8));            // scal
     sz.push_back(al(n0 * sizeof(real))); // work
     sz.push_back(al(n0 * sizeof(real))); // work2
+    sz.push_back(al(n0));                // cmask
     size_t total = 0;
     for (size_t b : sz) total += b;
     if (!hip_ok(hipMalloc(&pool, total), "network allocation")) return VFX_E_HIP;
@@ -238,6 +314,7 @@ int Net::create(int N_) {
     scal = (double*)take();
     work = (float*)take();
     work2 = (float*)take();
+    cmask = (uint8_t*)take();
     adam_t = 0;
     return VFX_OK;
 }
@@ -363,6 +440,45 @@ void Net::apply(const real* r, const uint8_t* fm, real* z, hipStream_t s) {
     hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, r, fm, scal, X);
     forward(s);
     hipLaunchKernelGGL(k_unpack, dim3(blocks_for(n0)), dim3(256), 0, s, n0, out, fm, scal, z);
+}
+
+// Coarse correction of a fine residual (the network's grid is N/4 of the fine grid Nf): the block
+// means of r -> scaled network input (with the coarse fluid flags), forward, and z += rms * out
+// spread over each block's fluid cells.
+void Net::apply_coarse(int Nf, const real* r, const uint8_t* fm, real* z, hipStream_t s) {
+    const size_t n0 = (size_t)N * N * N, nf = (size_t)Nf * Nf * Nf;
+    hipLaunchKernelGGL(k_pool4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, r, work2);
+    hipLaunchKernelGGL(k_mask4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, fm, cmask);
+    hipLaunchKernelGGL(k_sq_part, dim3(512), dim3(256), 0, s, n0, work2, dpart);
+    hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(FIN_THREADS), 0, s, dpart, 512, n0, scal);
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, work2, cmask, scal, X);
+    forward(s);
+    hipLaunchKernelGGL(k_up4<true>, dim3(blocks_for(nf)), dim3(256), 0, s, Nf, out, scal, fm, z, nullptr);
+}
+
+// One training step of the coarse correction on a fine pair (r_V, e_V) with K e_V = r_V (the
+// residual and the error left by the V-cycle): c = P net(R r_V), loss = e.Ke / e_V.r_V with
+// e = c - e_V in the fine energy norm, backward through P (a block sum) and the network, Adam.
+// e and K e go to the caller's fine scratch (2 Nf^3 reals).  The loss lands in scal[4].
+void net_train_step_coarse(Net& n, const Geom& gf, const real* rv, const real* ev, const uint8_t* fm, float lr,
+                           real* scratch, hipStream_t s) {
+    const int Nf = gf.N;
+    const size_t n0 = (size_t)n.N * n.N * n.N, nf = gf.n3;
+    real* e = scratch;
+    real* ke = scratch + nf;
+    hipLaunchKernelGGL(k_pool4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, rv, n.work2);
+    hipLaunchKernelGGL(k_mask4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, fm, n.cmask);
+    hipLaunchKernelGGL(k_sq_part, dim3(512), dim3(256), 0, s, n0, n.work2, n.dpart);
+    hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(FIN_THREADS), 0, s, n.dpart, 512, n0, n.scal);
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, n.work2, n.cmask, n.scal, n.X);
+    n.forward(s);
+    hipLaunchKernelGGL(k_up4<false>, dim3(blocks_for(nf)), dim3(256), 0, s, Nf, n.out, n.scal, fm, e, ev);
+    launch_kmul(gf, e, fm, ke, s);
+    hipLaunchKernelGGL(k_dot2_part, dim3(512), dim3(256), 0, s, nf, e, ke, ev, rv, n.dpart, 512);
+    hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(FIN_THREADS), 0, s, n.dpart, 512, n.scal);
+    hipLaunchKernelGGL(k_dout4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, ke, fm, n.scal, n.work);
+    n.backward(s);
+    n.adam(lr, 0.9f, 0.999f, 1e-8f, s);
 }
 
 // One training step on a (b, phi) pair in the energy norm of K (fm, geometry of the solver):

@@ -43,6 +43,7 @@ struct Net {
     double* scal = nullptr;   // device scalars (scale, dots, loss)
     float* work = nullptr;    // N^3 FP32 scratch (K e, dL/dz)
     float* work2 = nullptr;
+    uint8_t* cmask = nullptr; // N^3 fluid flags of the coarse correction's input
     int adam_t = 0;
 
     int create(int N);
@@ -55,10 +56,17 @@ struct Net {
     void adam(float lr, float b1, float b2, float eps, hipStream_t s);
     // preconditioner: z = rms(r) * net(r / rms(r), fluid) in fluid cells, 0 in solids
     void apply(const real* r, const uint8_t* fm, real* z, hipStream_t s);
+    // coarse correction of a fine field of Nf = 4 N: z += P net(R r) (R: 4^3 block mean, P:
+    // constant over each block's fluid cells)
+    void apply_coarse(int Nf, const real* r, const uint8_t* fm, real* z, hipStream_t s);
 };
 
 // one Adam step of the network on a (b, phi) pair in the energy norm of K (mask fm, geometry g of
 // a solver of the same size); the loss e.Ke / phi.b lands in n.scal[4]
 void net_train_step(Net& n, const Geom& g, const real* b, const real* phi, const uint8_t* fm, float lr, hipStream_t s);
+// one Adam step of the coarse correction (network grid N, fine grid gf.N = 4 N) on the residual r_V
+// and error e_V that a V-cycle left (K e_V = r_V); scratch: 2 fine fields
+void net_train_step_coarse(Net& n, const Geom& gf, const real* rv, const real* ev, const uint8_t* fm, float lr,
+                           real* scratch, hipStream_t s);
 
 }  // namespace vfx

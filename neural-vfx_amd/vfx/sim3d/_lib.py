@@ -76,6 +76,7 @@ SIGNATURES.update({
     "vfx_net_adam": [_P, _P, _F, _F, _F, _F, _P],
     "vfx_net_train_step": [_P, _P, _P, _P, _P, _F, ctypes.POINTER(ctypes.c_double), _P],
     "vfx_net_apply": [_P, _P, _P, _P, _P, _P],
+    "vfx_net_train_step_coarse": [_P, _P, _P, _P, _P, _F, ctypes.POINTER(ctypes.c_double), _P],
     "vfx_ctx_set_net": [_P, _P],
     "vfx_fcg_history": [_P, _P, _I, ctypes.POINTER(_I)],
     "vfx_batch_create": [_P, _I, _I, ctypes.POINTER(_P)],

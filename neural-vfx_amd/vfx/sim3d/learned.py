@@ -128,6 +128,16 @@ class UNet:
                                                 float(lr), ctypes.byref(loss), self._stream()), "UNet.train_step")
         return float(loss.value)
 
+    def train_step_coarse(self, solver, solid: torch.Tensor, rv: torch.Tensor, ev: torch.Tensor, lr: float) -> float:
+        """One Adam step of the coarse correction (this network's grid is a quarter of `solver`'s) on
+        the residual rv and error ev a V-cycle left; returns the fine energy-norm loss."""
+        loss = ctypes.c_double()
+        _lib.check(self._lib.vfx_net_train_step_coarse(self._h, solver._ctx, ctypes.c_void_p(solid.data_ptr()),
+                                                       ctypes.c_void_p(rv.data_ptr()), ctypes.c_void_p(ev.data_ptr()),
+                                                       float(lr), ctypes.byref(loss), self._stream()),
+                   "UNet.train_step_coarse")
+        return float(loss.value)
+
     def apply(self, solver, solid: torch.Tensor, r: torch.Tensor) -> torch.Tensor:
         """The preconditioner z = M(r) on `solver`'s grid."""
         z = torch.empty_like(r)
@@ -211,6 +221,44 @@ def noise_pairs(res: int, solids, count: int, seed: int = 0):
         out.append((solid, b, v))
     torch.cuda.synchronize()
     return out
+
+
+def vcycle_pairs(res: int, pairs):
+    """(solid, r_V, e_V) from (solid, b, phi) pairs of grid res: z_V = M_MG(b) (the MG
+    preconditioner), r_V = b - K z_V and e_V = phi - z_V, so that K e_V = r_V exactly as K phi = b:
+    what the V-cycle leaves for the coarse correction to remove."""
+    from .solver import Solver3D
+    s = Solver3D(res)
+    out = []
+    for solid, b, phi in pairs:
+        z = torch.empty_like(b)
+        s._chk(s._api.vfx_precond(s._ctx, ctypes.c_void_p(solid.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                  ctypes.c_void_p(z.data_ptr()), s._stream()))
+        kz = torch.empty_like(b)
+        s._chk(s._api.vfx_kmul(s._ctx, ctypes.c_void_p(solid.data_ptr()), ctypes.c_void_p(z.data_ptr()),
+                               ctypes.c_void_p(kz.data_ptr()), s._stream()))
+        out.append((solid, (b - kz).contiguous(), (phi - z).contiguous()))
+    torch.cuda.synchronize()
+    return out
+
+
+def train_coarse(net: UNet, vpairs, iters: int, lr: float = 1e-3, seed: int = 0, lr_decay: float = 0.3, log=None):
+    """Adam on the coarse correction's fine energy-norm loss over (solid, r_V, e_V) pairs of grid
+    4 net.res (shuffled each epoch; the same learning-rate schedule as train).  Returns the loss per
+    iteration."""
+    from .solver import Solver3D
+    solver = Solver3D(4 * net.res)
+    g = np.random.default_rng(int(seed))
+    order, losses = [], []
+    for it in range(int(iters)):
+        if not order:
+            order = list(g.permutation(len(vpairs)))
+        solid, rv, ev = vpairs[order.pop()]
+        rate = lr * (lr_decay if it >= 0.6 * iters else 1.0) * (lr_decay if it >= 0.85 * iters else 1.0)
+        losses.append(net.train_step_coarse(solver, solid, rv, ev, rate))
+        if log is not None and (it % 100 == 0 or it == iters - 1):
+            log(f"train {it}: loss {float(np.mean(losses[-50:])):.4f}")
+    return losses
 
 
 def train(net: UNet, pairs, iters: int, lr: float = 1e-3, seed: int = 0, lr_decay: float = 0.3, log=None):

@@ -172,8 +172,9 @@ class Solver3D:
         residual reduction stalls (pressure_stats()["fallback"])."""
         if self._f64:
             raise ValueError("the learned preconditioner runs in the FP32 solver")
-        if net is not None and net.res != self.res:
-            raise ValueError(f"network grid {net.res} != solver grid {self.res}")
+        if net is not None and net.res not in (self.res, self.res // 4):
+            raise ValueError(f"network grid {net.res}: the solver grid {self.res} (the preconditioner) or a "
+                             f"quarter of it (the coarse correction of the V-cycle)")
         self._net = net
         if net is not None:
             self.config = dataclasses.replace(self.config, pressure_mode="learned")
