@@ -219,10 +219,10 @@ static int project_device(vfx_ctx* c, real* vel, hipStream_t s) {
         if (restart) {
             Br = cg_budget_restart(c, maxit);
             P.restart_gate(c->prm, s);
-            P.rhs(vel, c->prm, true, s, true);
+            P.rhs(vel, c->prm, true, s, true, true);
             for (int k = 0; k <= Br; ++k) P.iteration(k, c->prm, s, Br, 2);
-            P.finish(s);
-            P.true_residual(vel, s, true);
+            P.finish(s, true);
+            P.true_residual(vel, s, true, true);
         }
     }
     CHECK_LAUNCH("pressure solve");
@@ -525,7 +525,7 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
     const bool judged = c->opt_true_res || (c->opt_restarts > 0 && rel_tol > 0.0);
     if (judged) {
         for (;;) {
-            P.true_residual(vel, s);
+            P.true_residual(vel, s, false, c->last_restarts > 0);
             CHECK_LAUNCH("true residual");
             c->last_true_valid = true;
             if (!(c->opt_restarts > c->last_restarts && rel_tol > 0.0)) break;
@@ -536,11 +536,11 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
             total += hs.it;
             c->last_restarts += 1;
             // restart from the true residual: warm start from the current phi
-            P.rhs(vel, c->prm, true, s);
+            P.rhs(vel, c->prm, true, s, false, true);
             CHECK_LAUNCH("pressure restart rhs");
             rc = run_cg_polled(c, s);
             if (rc) return rc;
-            P.finish(s);
+            P.finish(s, true);
         }
     }
     c->last_iters = total;   // plus DevState::it of the final run, read lazily
@@ -700,6 +700,7 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     Pressure& P = c->P;
     P.init_levels(N);
     P.geom = c->g;
+    P.geom_r = make_geom(N, 0);
     P.init_tiles();
     P.nparts = std::max({nblocks(c->g), P.ntiles, 2 * FCG_BLOCKS});   // flexible CG uses 4 * FCG_BLOCKS + 4
     const size_t n3 = c->g.n3;

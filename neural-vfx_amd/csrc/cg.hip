@@ -670,8 +670,8 @@ void Pressure::precond(int k, hipStream_t s) {
     });
 }
 
-void Pressure::rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated) {
-    const Geom& g = geom;
+void Pressure::rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated, bool restart) {
+    const Geom& g = restart ? geom_r : geom;
     const int gt = gated ? 1 : 0;
     if (warm)
         hipLaunchKernelGGL(k_rhs<true>, g.grid, g.block, 0, s, N, g.zc, vel, fm, R[0], phi, part, nblocks(g), st, gt);
@@ -699,8 +699,8 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int h
     hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, kmax, hitbit);
 }
 
-void Pressure::finish(hipStream_t s) {
-    const Geom& g = geom;
+void Pressure::finish(hipStream_t s, bool restart) {
+    const Geom& g = restart ? geom_r : geom;
     hipLaunchKernelGGL(k_phi_final, g.grid, g.block, 0, s, N, g.zc, phi, P[0], P[1], st);
 }
 
@@ -709,8 +709,8 @@ void Pressure::apply(real* vel, hipStream_t s) {
     hipLaunchKernelGGL(k_apply, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi);
 }
 
-void Pressure::true_residual(const real* vel, hipStream_t s, bool gated) {
-    const Geom& g = geom;
+void Pressure::true_residual(const real* vel, hipStream_t s, bool gated, bool restart) {
+    const Geom& g = restart ? geom_r : geom;
     const int gt = gated ? 1 : 0;
     hipLaunchKernelGGL(k_true_res, g.grid, g.block, 0, s, N, g.zc, vel, fm, phi, part, st, gt);
     hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), st, gt);

@@ -355,16 +355,30 @@ __device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K
     J = (int)(q - (uint32_t)K * m);
 }
 
+// Level launches use at most LEVEL_GRID workgroups, each striding over the items of the launch in
+// passes of LEVEL_GRID * 256 (blocks renumbered per XCD within a pass).  A launch of a CG iteration
+// that has already stopped then costs the dispatch of that many workgroups, not of one per 256
+// items (32768 at level 1 of 256^3).
+constexpr unsigned LEVEL_GRID = 4096;
+
+static unsigned level_grid(size_t items) {
+    const size_t b = (items + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > LEVEL_GRID ? LEVEL_GRID : b));
+}
+
 template <class LT>
 __global__ void __launch_bounds__(256) k_smooth_half(LvlT<LT> L, int color, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
-    int c, K, J, I;
-    if (!half_item(xcd_item(), L.m, color, c, K, J, I)) return;
-    smooth_item<true>(L, c, K, J, I, false);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = xcd_item();; t += stride) {
+        int c, K, J, I;
+        if (!half_item(t, L.m, color, c, K, J, I)) return;
+        smooth_item<true>(L, c, K, J, I, false);
+    }
 }
 
-// Restriction of the level below C.  For m >= 32 the coarse cells go by columns: workgroup b
-// (XCD-renumbered) owns 32 cells of one row (8 components each) and consecutive workgroups walk
+// Restriction of the level below C.  For m >= 32 the coarse cells go by columns: logical block b
+// (XCD-renumbered) owns 32 cells of one row (8 components each) and consecutive logical blocks walk
 // up K, then J, then along I, so the fine planes two coarse planes share are read again while
 // they are still in the L2.  Smaller levels use the natural order.
 template <class LT>
@@ -372,35 +386,41 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
                                                   int k) {
     if (cg_inactive(st, k)) return;
     const int m = C.m;
-    int c, K, J, I;
     if (m >= 32) {
-        const uint32_t um = (uint32_t)m, lb = (uint32_t)xcd_block((int)blockIdx.x, (int)gridDim.x);
-        const uint32_t col = lb / um, seg = col / um;
-        K = (int)(lb - col * um);
-        J = (int)(col - seg * um);
-        I = (int)(seg * 32u + (threadIdx.x >> 3));
-        c = (int)(threadIdx.x & 7u);
-        if (I >= m) return;
+        const uint32_t um = (uint32_t)m, total = um * um * ((um + 31u) / 32u);
+        for (uint32_t lb = (uint32_t)xcd_block((int)blockIdx.x, (int)gridDim.x); lb < total; lb += gridDim.x) {
+            const uint32_t col = lb / um, seg = col / um;
+            const int K = (int)(lb - col * um);
+            const int J = (int)(col - seg * um);
+            const int I = (int)(seg * 32u + (threadIdx.x >> 3));
+            const int c = (int)(threadIdx.x & 7u);
+            if (I < m) restrict_item<true>(F, C, c, K, J, I, do_red != 0);
+        }
     } else {
-        const size_t t = xcd_item();
-        if (t >= (size_t)m * m * m * 8) return;
-        full_item((uint32_t)t, (uint32_t)m, c, K, J, I);
+        const size_t items = (size_t)m * m * m * 8, stride = (size_t)gridDim.x * blockDim.x;
+        for (size_t t = xcd_item(); t < items; t += stride) {
+            int c, K, J, I;
+            full_item((uint32_t)t, (uint32_t)m, c, K, J, I);
+            restrict_item<true>(F, C, c, K, J, I, do_red != 0);
+        }
     }
-    restrict_item<true>(F, C, c, K, J, I, do_red != 0);
 }
 
 // workgroups of a k_restrict launch onto coarse size m
 static unsigned restrict_blocks(int m) {
-    if (m >= 32) return (unsigned)m * (unsigned)m * (unsigned)((m + 31) / 32);
-    return (unsigned)(((size_t)m * m * m * 8 + 255) / 256);
+    if (m >= 32) return std::min((unsigned)m * (unsigned)m * (unsigned)((m + 31) / 32), LEVEL_GRID);
+    return level_grid((size_t)m * m * m * 8);
 }
 
 template <class LT>
 __global__ void __launch_bounds__(256) k_prolong_black(LvlT<LT> F, LvlT<LT> C, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
-    int c, K, J, I;
-    if (!half_item(xcd_item(), F.m, 1, c, K, J, I)) return;
-    prolong_black_item(F, C, c, K, J, I);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = xcd_item();; t += stride) {
+        int c, K, J, I;
+        if (!half_item(t, F.m, 1, c, K, J, I)) return;
+        prolong_black_item(F, C, c, K, J, I);
+    }
 }
 
 // The bottom of the V-cycle in ONE workgroup (k_coarse_block :550-591 generalised to any
@@ -623,18 +643,16 @@ void Pressure::vcycle_levels(int k, hipStream_t s) {
         using LT = decltype(tag);
         for (int l = 0; l < S; ++l) {
             const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
-            hipLaunchKernelGGL(k_smooth_half<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
-                               lvl_as<LT>(lv[l]), 1, st, k);
+            hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 1, st, k);
             hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[l + 1].m)), dim3(256), 0, s,
                                lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), (l + 1 < S) ? 1 : 0, st, k);
         }
         launch_coarse<LT>(*this, s, k);
         for (int l = S - 1; l >= 0; --l) {
             const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
-            hipLaunchKernelGGL(k_prolong_black<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
-                               lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), st, k);
-            hipLaunchKernelGGL(k_smooth_half<LT>, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, s,
-                               lvl_as<LT>(lv[l]), 0, st, k);
+            hipLaunchKernelGGL(k_prolong_black<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]),
+                               lvl_as<LT>(lv[l + 1]), st, k);
+            hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 0, st, k);
         }
     });
 }
@@ -649,8 +667,8 @@ bool Pressure::launch_single_mg(int which, hipStream_t s) {
         using LT = decltype(tag);
         switch (which) {
             case 2:
-                hipLaunchKernelGGL(k_smooth_half<LT>, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s,
-                                   lvl_as<LT>(lv[0]), 1, st, 0);
+                hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half0)), dim3(256), 0, s, lvl_as<LT>(lv[0]), 1,
+                                   st, 0);
                 break;
             case 3: {
                 hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[1].m)), dim3(256), 0, s,
@@ -658,8 +676,8 @@ bool Pressure::launch_single_mg(int which, hipStream_t s) {
                 break;
             }
             case 4:
-                hipLaunchKernelGGL(k_prolong_black<LT>, dim3((unsigned)((half0 + 255) / 256)), dim3(256), 0, s,
-                                   lvl_as<LT>(lv[0]), lvl_as<LT>(lv[1]), st, 0);
+                hipLaunchKernelGGL(k_prolong_black<LT>, dim3(level_grid(half0)), dim3(256), 0, s, lvl_as<LT>(lv[0]),
+                                   lvl_as<LT>(lv[1]), st, 0);
                 break;
             default: launch_coarse<LT>(*this, s, 0); break;
         }

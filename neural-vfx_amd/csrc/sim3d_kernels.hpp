@@ -152,6 +152,7 @@ struct Pressure {
     bool force_setup = false;   // rebuild the operators at the next setup
     Lvl lv[MAX_LEVELS];
     Geom geom;
+    Geom geom_r;   // the restart run's cell kernels: about 2048 blocks, so a gated launch that does not run is cheap
     uint8_t* fm = nullptr;       // face mask (n3 bytes) followed by at least one zero byte
     real* R[2] = {nullptr, nullptr};   // residual, ping-pong by iteration parity
     real* P[2] = {nullptr, nullptr};   // search direction, ping-pong
@@ -170,15 +171,16 @@ struct Pressure {
     void bind_levels(real* base);
     void setup(const real* solid, hipStream_t s);
     // warm: the host's copy of the P_WARM parameter (the kernel variant is chosen on the host);
-    // gated: the rhs of a device-side restart (runs only when the restart gate opened)
-    void rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated = false);
+    // gated: the rhs of a device-side restart (runs only when the restart gate opened);
+    // restart: the rhs of a restart run (geom_r)
+    void rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated = false, bool restart = false);
     void precond(int k, hipStream_t s);
     // CG iteration k of a run whose host enqueued iterations 0..kmax (hitbit: DevState::hit bit
     // set when that budget, not the stop test, ends the run)
     void iteration(int k, const float* prm, hipStream_t s, int kmax = 1 << 30, int hitbit = 1);
-    void finish(hipStream_t s);
+    void finish(hipStream_t s, bool restart = false);
     void apply(real* vel, hipStream_t s);
-    void true_residual(const real* vel, hipStream_t s, bool gated = false);
+    void true_residual(const real* vel, hipStream_t s, bool gated = false, bool restart = false);
     // decide on the device whether the gated restart run after it runs (true residual above the
     // relative tolerance)
     void restart_gate(const float* prm, hipStream_t s);

@@ -4,8 +4,9 @@
   and BF16 multigrid level storage against the FP64 run of the same solver; every projection of
   every run meets the true relative residual 1e-5, and the pressure, divergence and density
   errors against FP64 are reported and bounded.
-- Learned preconditioner (128^3, held-out scene): the network trained on solver pairs, flexible CG
-  with the MGPCG fallback against plain MGPCG at equal true relative residual.
+- Learned preconditioner (128^3, held-out scenes): the V-cycle's coarse correction trained on
+  solver pairs, flexible CG with the MGPCG fallback against plain MGPCG at equal true relative
+  residual (no fallback, no more iterations than MGPCG).
 - Batched uneven sims (32 sims of sizes drawn by seed from {48, 64, 96, 128}^3): every projection
   of every frame converges, and sims of each size equal their own Solver3D.step bit for bit.
 
@@ -98,24 +99,14 @@ def test_mixed_precision_256_against_fp64(cuda, hiplib):
         assert errs["density"][1] <= 2e-3, storage
 
 
-def test_learned_preconditioner_128_held_out(cuda, hiplib):
-    """The U-Net trained on solver pairs (seeded scenes 20-27 at 64^3 and noise pairs), applied in
-    flexible CG at 128^3 on the held-out scene 99: the projection meets the true relative residual
-    1e-5 (through the MGPCG fallback when the network stalls, which is counted), the same as plain
-    MGPCG on the same pre-projection state; both times are reported."""
-    from vfx.sim3d.learned import UNet, generate_pairs, noise_pairs, random_scene, scene_config, train
+def _pre_projection(N, seed):
+    """The seeded scene after 6 steps, advected and forced for the next step: the state a
+    projection starts from (cold start, true relative residual 1e-5, up to 2 restarts)."""
+    from vfx.sim3d.learned import random_scene, scene_config
     from vfx.sim3d.solver import pack_params
-    pairs = generate_pairs(64, seeds=range(20, 24), steps=10, every=1)
-    pairs += noise_pairs(64, [p[0] for p in pairs[::7]], count=len(pairs), seed=5)
-    net = UNet(64, seed=1)
-    losses = train(net, pairs, iters=300, lr=2e-3, seed=0)
-    _say(f"learned 128^3: trained on {len(pairs)} pairs, loss {np.mean(losses[:30]):.3f} -> {np.mean(losses[-30:]):.3f}")
-    params = net.params()
-    del net, pairs
-    N = 128
-    forces, obst, beta = random_scene(99)
-    cfg = dataclasses.replace(scene_config(beta, 99), pressure_warm_start=False, pressure_max_iters=400,
-                              pressure_restarts=2)
+    forces, obst, beta = random_scene(seed)
+    cfg = dataclasses.replace(scene_config(beta, seed), pressure_warm_start=False, pressure_rel_tol=1e-5,
+                              pressure_max_iters=400, pressure_restarts=2)
     s = Solver3D(N, cfg)
     st = s.reset(obst)
     for _ in range(6):
@@ -126,25 +117,52 @@ def test_learned_preconditioner_128_held_out(cuda, hiplib):
                                       ctypes.c_void_p(st.solid.data_ptr()),
                                       h.ctypes.data_as(ctypes.c_void_p) if stage == 0 else None, s._stream()))
     torch.cuda.synchronize()
-    netN = UNet(N, params=params)
-    row = {}
-    for name in ("mgpcg", "learned"):
-        s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
-        s.set_preconditioner(netN if name == "learned" else None)
-        times, ps = [], None
-        for _ in range(3):
-            work = FieldState3D(data=st.data.clone(), solid=st.solid, meta=st.meta)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            s.project(work)
-            ps = s.pressure_stats()
-            times.append(1000.0 * (time.perf_counter() - t0))
-        row[name] = (float(np.median(times)), ps)
-        _say(f"  {name}: {row[name][0]:.2f} ms, MGPCG iterations {ps['iters']}, flexible-CG iterations "
-             f"{ps['learned_iters']}, fallback {ps['fallback']}, true rel-res {ps['true_rel_res']:.2e}")
-        assert 0 <= ps["true_rel_res"] <= 1e-5 and ps["converged"], name
-    s.set_preconditioner(None)
-    _say(f"  learned / MGPCG time {row['learned'][0] / row['mgpcg'][0]:.2f}")
+    return s, st
+
+
+def test_learned_preconditioner_128_held_out(cuda, hiplib):
+    """The coarse correction of the V-cycle (a U-Net on the 32^3 grid) trained at 128^3 on what the
+    V-cycle leaves of solver pairs (seeded scenes 20-27) and noise pairs, applied in flexible CG on
+    the held-out scenes 99 and 98: every projection meets the true relative residual 1e-5 without
+    the MGPCG fallback and in no more flexible-CG iterations than plain MGPCG takes on the same
+    pre-projection state; times are reported."""
+    from vfx.sim3d.learned import UNet, generate_pairs, noise_pairs, train_coarse, vcycle_pairs
+    N = 128
+    pairs = generate_pairs(N, seeds=range(20, 28), steps=14, every=1)
+    pairs += noise_pairs(N, [p[0] for p in pairs[::11]], count=len(pairs) // 2, seed=5)
+    data = vcycle_pairs(N, pairs)
+    net = UNet(N // 4, seed=1)
+    losses = train_coarse(net, data, iters=600, lr=2e-3, seed=0)
+    first, last = float(np.mean(losses[:50])), float(np.mean(losses[-50:]))
+    _say(f"learned 128^3: coarse correction trained on {len(data)} pairs, loss {first:.3f} -> {last:.3f}")
+    assert last < first
+    del pairs, data
+    for seed in (99, 98):
+        s, st = _pre_projection(N, seed)
+        s.set_option(_lib.OPT_LEARNED_SWEEPS, 0)
+        row = {}
+        for name in ("mgpcg", "learned"):
+            s.set_option(_lib.OPT_TRUE_RESIDUAL, 1)
+            s.set_preconditioner(net if name == "learned" else None)
+            times, ps = [], None
+            for _ in range(3):
+                work = FieldState3D(data=st.data.clone(), solid=st.solid, meta=st.meta)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                s.project(work)
+                ps = s.pressure_stats()
+                times.append(1000.0 * (time.perf_counter() - t0))
+            row[name] = (float(np.median(times)), ps)
+            _say(f"  scene {seed} {name}: {row[name][0]:.2f} ms, MGPCG iterations {ps['iters']}, flexible-CG "
+                 f"iterations {ps['learned_iters']}, fallback {ps['fallback']}, true rel-res {ps['true_rel_res']:.2e}")
+            assert 0 <= ps["true_rel_res"] <= 1e-5 and ps["converged"], (seed, name)
+        s.set_preconditioner(None)
+        lps, mps = row["learned"][1], row["mgpcg"][1]
+        _say(f"  scene {seed}: learned / MGPCG time {row['learned'][0] / row['mgpcg'][0]:.2f}")
+        assert lps["fallback"] == 0, seed
+        assert lps["learned_iters"] <= mps["iters"], (seed, lps["learned_iters"], mps["iters"])
+        del s, st
+        torch.cuda.empty_cache()
 
 
 BATCH_SIZES = (48, 64, 96, 128)
