@@ -282,27 +282,157 @@ __device__ __forceinline__ void prolong_item(const LvlT<LT>& F, const LvlT<LT>& 
     stl(F.x + ox, x + xc);
 }
 
-// prolongation fused into the black post-sweep: equal to prolonging every cell, then sweeping
-// black (a GS update ignores the cell's own old value; red cells are rewritten by the red sweep
-// that follows).  Same rounding as the unfused pair (pressure.py:396-423 vs :902-931).
-template <class LT>
-__device__ __forceinline__ void prolong_black_item(const LvlT<LT>& F, const LvlT<LT>& C, int c, int K, int J, int I) {
-    const int m = F.m;
-    const uint32_t o = lcell(m, K, J, I);
-    const LNb n = level_neighbours(m, o, K, J, I);
-    const int nI[6] = {I + 1, I - 1, I, I, I, I}, nJ[6] = {J, J, J + 1, J - 1, J, J},
-              nK[6] = {K, K, K, K, K + 1, K - 1};
-    const int f = F.sf[o];
-    const int col = (I + J + K) & 1;
-    const real r = ldl(F.r + lsto(m, o, col) * 8 + c);
-    real xn[6], xcn[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        // coordinates of the clamped neighbour (the cell itself where it is missing)
-        const int kI = n.e[k] ? nI[k] : I, kJ = n.e[k] ? nJ[k] : J, kK = n.e[k] ? nK[k] : K;
-        xn[k] = ldl(F.x + lsto(m, n.c[k], n.e[k] ? col ^ 1 : col) * 8 + c);
-        xcn[k] = ldl(C.x + lsto(C.m, kK >> 1, kJ >> 1, kI >> 1) * 8 + c);
+// The black post-sweep with the prolongation fused in (prolong_black_even below): equal to
+// prolonging every cell, then sweeping black (a GS update ignores the cell's own old value; red
+// cells are rewritten by the red sweep that follows).  Same rounding as the unfused pair
+// (pressure.py:396-423 vs :902-931).
+
+// ---------------------------------------------------------------- even-level fast paths
+// Every level launched on its own has even m (the hierarchy stops at the first odd size, which
+// the single-workgroup coarse kernel handles), so its x and r are colour-split: cell
+// o = lcell(K, J, I) of colour col = (I + J + K) & 1 is stored at col m^3/2 + h, h = o >> 1, and
+// o = 2h + b with b = I & 1.  Its six neighbours, all of the other colour, are stored at
+//   +I: sb + b,   -I: sb + b - 1,   +-J: sb +- m/2,   +-K: sb +- m^2/2,   sb = (col ^ 1) m^3/2 + h,
+// and item t of a one-colour launch has h = t >> 3, component c = t & 7.  The items below load
+// exactly the values the generic items load (a missing neighbour is read at sb instead of at the
+// cell itself, and either value is discarded), in the same order of arithmetic, so their results
+// are the same bit for bit; they spend fewer instructions on addresses, and for power-of-two m
+// the item decomposition is shifts instead of divisions.
+struct LevelShape {
+    uint32_t m, mh, sz, half;   // m, m/2, m^2/2, m^3/2
+    int lm;                     // log2 m for a power of two, else -1
+};
+
+__device__ __forceinline__ LevelShape level_shape(int m) {
+    LevelShape s;
+    s.m = (uint32_t)m;
+    s.mh = s.m >> 1;
+    s.sz = s.m * s.mh;
+    s.half = s.m * s.sz;
+    s.lm = (m & (m - 1)) == 0 ? (int)__builtin_ctz((unsigned)m) : -1;
+    return s;
+}
+
+// q = h / (m/2), Ih = h % (m/2), K = q / m, J = q % m
+__device__ __forceinline__ void split_half(const LevelShape& s, uint32_t h, int& K, int& J, uint32_t& Ih) {
+    uint32_t q;
+    if (s.lm > 0) {
+        Ih = h & (s.mh - 1u);
+        q = h >> (s.lm - 1);
+        J = (int)(q & (s.m - 1u));
+        K = (int)(q >> s.lm);
+    } else {
+        q = h / s.mh;
+        Ih = h - q * s.mh;
+        K = (int)(q / s.m);
+        J = (int)(q - (uint32_t)K * s.m);
     }
+}
+
+// cell -> (K, J, I) of a level of size m (natural order)
+__device__ __forceinline__ void split_cell(const LevelShape& s, uint32_t cell, int& K, int& J, int& I) {
+    if (s.lm >= 0) {
+        I = (int)(cell & (s.m - 1u));
+        const uint32_t q = cell >> s.lm;
+        J = (int)(q & (s.m - 1u));
+        K = (int)(q >> s.lm);
+    } else {
+        const uint32_t q = cell / s.m;
+        I = (int)(cell - q * s.m);
+        K = (int)(q / s.m);
+        J = (int)(q - (uint32_t)K * s.m);
+    }
+}
+
+struct HalfItem {
+    int c, K, J, I;
+    uint32_t h, o;   // offset in the colour half, natural cell index
+};
+
+// item t of a one-colour launch (m even); false past the last item (4 m^3 < 2^32 for m <= 512)
+__device__ __forceinline__ bool half_item_even(const LevelShape& s, uint32_t t, int color, HalfItem& it) {
+    if (t >= s.half * 8u) return false;
+    it.c = (int)(t & 7u);
+    it.h = t >> 3;
+    uint32_t Ih;
+    split_half(s, it.h, it.K, it.J, Ih);
+    const int b = (color + it.J + it.K) & 1;
+    it.I = (int)(2u * Ih) + b;
+    it.o = 2u * it.h + (uint32_t)b;
+    return true;
+}
+
+// storage offsets of the six neighbours (sb where one is missing) and their existence
+struct EvenNb {
+    uint32_t s[6];
+    bool e[6];
+};
+
+__device__ __forceinline__ EvenNb even_neighbours(const LevelShape& s, int color, const HalfItem& it) {
+    EvenNb n;
+    const uint32_t sb = (uint32_t)(color ^ 1) * s.half + it.h, b = (uint32_t)(it.I & 1);
+    n.e[0] = (uint32_t)it.I + 1u < s.m; n.e[1] = it.I > 0;
+    n.e[2] = (uint32_t)it.J + 1u < s.m; n.e[3] = it.J > 0;
+    n.e[4] = (uint32_t)it.K + 1u < s.m; n.e[5] = it.K > 0;
+    n.s[0] = n.e[0] ? sb + b : sb;
+    n.s[1] = n.e[1] ? sb + b - 1u : sb;
+    n.s[2] = n.e[2] ? sb + s.mh : sb;
+    n.s[3] = n.e[3] ? sb - s.mh : sb;
+    n.s[4] = n.e[4] ? sb + s.sz : sb;
+    n.s[5] = n.e[5] ? sb - s.sz : sb;
+    return n;
+}
+
+// smooth_item<true> (non-zero guess) of a one-colour item
+template <class LT>
+__device__ __forceinline__ void smooth_even(const LvlT<LT>& L, const LevelShape& s, int color, const HalfItem& it) {
+    const int c = it.c;
+    const uint32_t own = (uint32_t)color * s.half + it.h;
+    const EvenNb n = even_neighbours(s, color, it);
+    const real r = ldl(L.r + own * 8u + c);
+    const int f = L.sf[it.o];
+    real xn[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xn[k] = ldl(L.x + n.s[k] * 8u + c);
+    const Tab tb = load_tab(L, c);
+    real d, cf[6];
+    if (__all(f != 0)) {
+        d = tb.d;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cf[k] = tb.o[k >> 1];
+    } else {
+        const LNb nn = level_neighbours(L.m, it.o, it.K, it.J, it.I);
+        d = ldl(L.diag + it.o * 8u + c);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cf[k] = link_coef(L, nn, it.o, k, c);
+    }
+    const real acc = r - offsum_v(cf, xn, n.e);
+    stl(L.x + own * 8u + c, d <= L.dmin ? RL(0.0) : acc / d);
+}
+
+// prolong_black_item of a black item.  The coarse values: the cell's own coarse cell and, along
+// each axis, the coarse cell across the face of the 2x2x2 block the cell touches (the +ax one for
+// an odd coordinate, the -ax one for an even), which holds one of its two neighbours.
+template <class LT>
+__device__ __forceinline__ void prolong_black_even(const LvlT<LT>& F, const LvlT<LT>& C, const LevelShape& s,
+                                                   const HalfItem& it) {
+    const int c = it.c, I = it.I, J = it.J, K = it.K;
+    const uint32_t own = s.half + it.h;
+    const EvenNb n = even_neighbours(s, 1, it);
+    const int f = F.sf[it.o];
+    const real r = ldl(F.r + own * 8u + c);
+    const int Ic = I >> 1, Jc = J >> 1, Kc = K >> 1, bI = I & 1, bJ = J & 1, bK = K & 1;
+    const bool ox = bI ? n.e[0] : n.e[1], oy = bJ ? n.e[2] : n.e[3], oz = bK ? n.e[4] : n.e[5];
+    const int Io = ox ? Ic + 2 * bI - 1 : Ic, Jo = oy ? Jc + 2 * bJ - 1 : Jc, Ko = oz ? Kc + 2 * bK - 1 : Kc;
+    real xn[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xn[k] = ldl(F.x + n.s[k] * 8u + c);
+    const real xc0 = ldl(C.x + lsto(C.m, Kc, Jc, Ic) * 8 + c);
+    const real xcx = ldl(C.x + lsto(C.m, Kc, Jc, Io) * 8 + c);
+    const real xcy = ldl(C.x + lsto(C.m, Kc, Jo, Ic) * 8 + c);
+    const real xcz = ldl(C.x + lsto(C.m, Ko, Jc, Ic) * 8 + c);
+    const real xcn[6] = {bI ? xcx : xc0, bI ? xc0 : xcx, bJ ? xcy : xc0,
+                         bJ ? xc0 : xcy, bK ? xcz : xc0, bK ? xc0 : xcz};
     const Tab tb = load_tab(F, c);
     real d, cf[6], xpv[6];
     if (__all(f != 0)) {
@@ -313,16 +443,92 @@ __device__ __forceinline__ void prolong_black_item(const LvlT<LT>& F, const LvlT
             xpv[k] = as_stored<LT>(xn[k] + xcn[k]);   // a standard cell's neighbours have diag > dmin
         }
     } else {
-        d = ldl(F.diag + o * 8 + c);
+        const LNb nn = level_neighbours(F.m, it.o, K, J, I);
+        d = ldl(F.diag + it.o * 8u + c);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            cf[k] = link_coef(F, n, o, k, c);
-            // x of the neighbour after the constant prolongation (which skips diag <= dmin)
-            xpv[k] = ldl(F.diag + n.c[k] * 8 + c) <= F.dmin ? xn[k] : as_stored<LT>(xn[k] + xcn[k]);
+            cf[k] = link_coef(F, nn, it.o, k, c);
+            xpv[k] = ldl(F.diag + nn.c[k] * 8 + c) <= F.dmin ? xn[k] : as_stored<LT>(xn[k] + xcn[k]);
         }
     }
     const real a = offsum_v(cf, xpv, n.e);
-    stl(F.x + lsto(m, o, col) * 8 + c, d <= F.dmin ? RL(0.0) : (r - a) / d);
+    stl(F.x + own * 8u + c, d <= F.dmin ? RL(0.0) : (r - a) / d);
+}
+
+// restrict_item<true> onto coarse cell (K, J, I) from an even fine level.  Child k = 4 dz + 2 dy + dx
+// is fine cell (2K + dz, 2J + dy, 2I + dx) of colour (dx + dy + dz) & 1, stored at
+// colour m^3/2 + base(dz, dy) with base = ((2K + dz) m + 2J + dy) m/2 + I.  A child's neighbour
+// inside the block is a sibling (its x0 is that value); one outside is loaded from the other colour
+// at the offsets of even_neighbours.
+template <class LT>
+__device__ __forceinline__ void restrict_even(const LvlT<LT>& F, const LvlT<LT>& C, const LevelShape& s, int c, int K,
+                                              int J, int I, bool do_red) {
+    const uint32_t m = s.m, mc = s.mh;
+    uint32_t base[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) base[q] = ((2u * K + (q >> 1)) * m + 2u * J + (q & 1)) * s.mh + (uint32_t)I;
+    const bool in_x[2] = {I > 0, (uint32_t)I + 1u < mc}, in_y[2] = {J > 0, (uint32_t)J + 1u < mc},
+               in_z[2] = {K > 0, (uint32_t)K + 1u < mc};
+    real r[8], x0[8], xo[8][3];
+    int f = 1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int dz = k >> 2, dy = (k >> 1) & 1, dx = k & 1, col = (dx + dy + dz) & 1;
+        const uint32_t bs = base[k >> 1], st = (uint32_t)col * s.half + bs, sb = (uint32_t)(col ^ 1) * s.half + bs;
+        f &= (int)F.sf[2u * bs + (uint32_t)dx];
+        r[k] = ldl(F.r + st * 8u + c);
+        x0[k] = ldl(F.x + st * 8u + c);
+        // the outer neighbour along each axis: -ax for an even offset, +ax for an odd one
+        const uint32_t ax_x = in_x[dx] ? (dx ? sb + 1u : sb - 1u) : sb;
+        const uint32_t ax_y = in_y[dy] ? (dy ? sb + s.mh : sb - s.mh) : sb;
+        const uint32_t ax_z = in_z[dz] ? (dz ? sb + s.sz : sb - s.sz) : sb;
+        xo[k][0] = ldl(F.x + ax_x * 8u + c);
+        xo[k][1] = ldl(F.x + ax_y * 8u + c);
+        xo[k][2] = ldl(F.x + ax_z * 8u + c);
+    }
+    real xn[8][6];
+    bool e[8][6];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int dz = k >> 2, dy = (k >> 1) & 1, dx = k & 1;
+        // +ax, -ax along x, y, z (the order of level_neighbours)
+        xn[k][0] = dx ? xo[k][0] : x0[k | 1];  e[k][0] = dx ? in_x[1] : true;
+        xn[k][1] = dx ? x0[k & 6] : xo[k][0];  e[k][1] = dx ? true : in_x[0];
+        xn[k][2] = dy ? xo[k][1] : x0[k | 2];  e[k][2] = dy ? in_y[1] : true;
+        xn[k][3] = dy ? x0[k & 5] : xo[k][1];  e[k][3] = dy ? true : in_y[0];
+        xn[k][4] = dz ? xo[k][2] : x0[k | 4];  e[k][4] = dz ? in_z[1] : true;
+        xn[k][5] = dz ? x0[k & 3] : xo[k][2];  e[k][5] = dz ? true : in_z[0];
+    }
+    const Tab tb = load_tab(F, c);
+    real res[8];
+    if (__all(f != 0)) {
+        const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) res[k] = r[k] - tb.d * x0[k] - offsum_v(cf, xn[k], e[k]);
+    } else {
+        const uint32_t sy = m, sz = m * m;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t co = 2u * base[k >> 1] + (uint32_t)(k & 1);
+            const uint32_t nb[6] = {e[k][0] ? co + 1u : co, e[k][1] ? co - 1u : co, e[k][2] ? co + sy : co,
+                                    e[k][3] ? co - sy : co, e[k][4] ? co + sz : co, e[k][5] ? co - sz : co};
+            real cf[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) cf[j] = ldl(F.off + ((j & 1) ? nb[j] : co) * 24 + (j >> 1) * 8 + c);
+            res[k] = r[k] - ldl(F.diag + co * 8 + c) * x0[k] - offsum_v(cf, xn[k], e[k]);
+        }
+    }
+    real acc = RL(0.0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += res[k];
+    acc = RL(0.125) * acc;
+    const uint32_t oc = lcell(C.m, K, J, I) * 8 + c, xc = lsto(C.m, K, J, I) * 8 + c;
+    stl(C.r + xc, acc);
+    if (do_red && ((I + J + K) & 1) == 0) {
+        // the coarse red sweep reads the right-hand side as stored
+        const real d = ldl(C.diag + oc), ra = as_stored<LT>(acc);
+        stl(C.x + xc, d <= C.dmin ? RL(0.0) : ra / d);
+    }
 }
 
 // item of this thread in a 1D launch, blocks renumbered per XCD (neighbouring rows in one L2)
@@ -330,23 +536,9 @@ __device__ __forceinline__ size_t xcd_item() {
     return (size_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
 }
 
-// half-size colour launch: item t -> (c, K, J, I) with (I + J + K) & 1 == color (m even).  Item
-// counts stay below 2^32 (8 m^3 / 2 <= 2^30 for N <= 1024), so the decomposition is 32-bit
-// unsigned arithmetic (a 64-bit division is a long emulated sequence per lane).
-__device__ __forceinline__ bool half_item(size_t t, int m, int color, int& c, int& K, int& J, int& I) {
-    const uint32_t um = (uint32_t)m, mh = um >> 1;
-    if (t >= (size_t)um * um * mh * 8) return false;
-    const uint32_t tt = (uint32_t)t;
-    c = (int)(tt & 7u);
-    const uint32_t h = tt >> 3, q = h / mh;
-    const uint32_t Ih = h - q * mh;
-    K = (int)(q / um);
-    J = (int)(q - (uint32_t)K * um);
-    I = (int)(2u * Ih) + ((color + J + K) & 1);
-    return true;
-}
-
-// full-size launch: item t -> (c, K, J, I) (32-bit, as half_item)
+// full-size launch: item t -> (c, K, J, I).  Item counts stay below 2^32 (8 m^3 <= 2^30 for
+// N <= 1024), so the decomposition is 32-bit unsigned arithmetic (a 64-bit division is a long
+// emulated sequence per lane).
 __device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K, int& J, int& I) {
     c = (int)(t & 7u);
     const uint32_t cell = t >> 3, q = cell / m;
@@ -369,11 +561,12 @@ static unsigned level_grid(size_t items) {
 template <class LT>
 __global__ void __launch_bounds__(256) k_smooth_half(LvlT<LT> L, int color, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t t = xcd_item();; t += stride) {
-        int c, K, J, I;
-        if (!half_item(t, L.m, color, c, K, J, I)) return;
-        smooth_item<true>(L, c, K, J, I, false);
+    const LevelShape sh = level_shape(L.m);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t = (uint32_t)xcd_item();; t += stride) {
+        HalfItem it;
+        if (!half_item_even(sh, t, color, it)) return;
+        smooth_even(L, sh, color, it);
     }
 }
 
@@ -385,23 +578,31 @@ template <class LT>
 __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do_red, const DevState* __restrict__ st,
                                                   int k) {
     if (cg_inactive(st, k)) return;
+    const LevelShape fs = level_shape(F.m), cs = level_shape(C.m);
     const int m = C.m;
     if (m >= 32) {
         const uint32_t um = (uint32_t)m, total = um * um * ((um + 31u) / 32u);
         for (uint32_t lb = (uint32_t)xcd_block((int)blockIdx.x, (int)gridDim.x); lb < total; lb += gridDim.x) {
-            const uint32_t col = lb / um, seg = col / um;
+            uint32_t col, seg;
+            if (cs.lm >= 0) {
+                col = lb >> cs.lm;
+                seg = col >> cs.lm;
+            } else {
+                col = lb / um;
+                seg = col / um;
+            }
             const int K = (int)(lb - col * um);
             const int J = (int)(col - seg * um);
             const int I = (int)(seg * 32u + (threadIdx.x >> 3));
             const int c = (int)(threadIdx.x & 7u);
-            if (I < m) restrict_item<true>(F, C, c, K, J, I, do_red != 0);
+            if (I < m) restrict_even(F, C, fs, c, K, J, I, do_red != 0);
         }
     } else {
-        const size_t items = (size_t)m * m * m * 8, stride = (size_t)gridDim.x * blockDim.x;
-        for (size_t t = xcd_item(); t < items; t += stride) {
-            int c, K, J, I;
-            full_item((uint32_t)t, (uint32_t)m, c, K, J, I);
-            restrict_item<true>(F, C, c, K, J, I, do_red != 0);
+        const uint32_t items = (uint32_t)m * m * m * 8, stride = gridDim.x * blockDim.x;
+        for (uint32_t t = (uint32_t)xcd_item(); t < items; t += stride) {
+            int K, J, I;
+            split_cell(cs, t >> 3, K, J, I);
+            restrict_even(F, C, fs, (int)(t & 7u), K, J, I, do_red != 0);
         }
     }
 }
@@ -415,11 +616,12 @@ static unsigned restrict_blocks(int m) {
 template <class LT>
 __global__ void __launch_bounds__(256) k_prolong_black(LvlT<LT> F, LvlT<LT> C, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t t = xcd_item();; t += stride) {
-        int c, K, J, I;
-        if (!half_item(t, F.m, 1, c, K, J, I)) return;
-        prolong_black_item(F, C, c, K, J, I);
+    const LevelShape sh = level_shape(F.m);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t = (uint32_t)xcd_item();; t += stride) {
+        HalfItem it;
+        if (!half_item_even(sh, t, 1, it)) return;
+        prolong_black_even(F, C, sh, it);
     }
 }
 
