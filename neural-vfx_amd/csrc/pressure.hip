@@ -383,31 +383,54 @@ __device__ __forceinline__ EvenNb even_neighbours(const LevelShape& s, int color
     return n;
 }
 
-// smooth_item<true> (non-zero guess) of a one-colour item
+// value of component c of storage cell s: the byte offset 2 (8 s + c) or 4 (8 s + c) stays below
+// 2^32 (8 m^3 <= 2^30 values per array), so it is a 32-bit offset from the array's base (one
+// address register per load, the base in scalar registers)
 template <class LT>
-__device__ __forceinline__ void smooth_even(const LvlT<LT>& L, const LevelShape& s, int color, const HalfItem& it) {
+__device__ __forceinline__ real ldc(const LT* base, uint32_t s, int c) {
+    return ldl(reinterpret_cast<const LT*>(reinterpret_cast<const char*>(base) + (s * 8u + (uint32_t)c) * (uint32_t)sizeof(LT)));
+}
+template <class LT>
+__device__ __forceinline__ void stc(LT* base, uint32_t s, int c, real v) {
+    stl(reinterpret_cast<LT*>(reinterpret_cast<char*>(base) + (s * 8u + (uint32_t)c) * (uint32_t)sizeof(LT)), v);
+}
+
+// offsum_v with every neighbour present (a standard cell): the same sums without the tests
+__device__ __forceinline__ real offsum_all(const real* cf, const real* xn) {
+    real a = RL(0.0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a = a + cf[k] * xn[k];
+    return a;
+}
+
+// smooth_item<true> (non-zero guess) of a one-colour item; tb: the level table of component
+// it.c (the same for every item of a thread)
+template <class LT>
+__device__ __forceinline__ void smooth_even(const LvlT<LT>& L, const LevelShape& s, int color, const HalfItem& it,
+                                            const Tab& tb) {
     const int c = it.c;
     const uint32_t own = (uint32_t)color * s.half + it.h;
     const EvenNb n = even_neighbours(s, color, it);
-    const real r = ldl(L.r + own * 8u + c);
+    const real r = ldc(L.r, own, c);
     const int f = L.sf[it.o];
     real xn[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xn[k] = ldl(L.x + n.s[k] * 8u + c);
-    const Tab tb = load_tab(L, c);
-    real d, cf[6];
+    for (int k = 0; k < 6; ++k) xn[k] = ldc(L.x, n.s[k], c);
+    real d, acc;
     if (__all(f != 0)) {
+        // a standard cell has all six neighbours
         d = tb.d;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) cf[k] = tb.o[k >> 1];
+        const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
+        acc = r - offsum_all(cf, xn);
     } else {
+        real cf[6];
         const LNb nn = level_neighbours(L.m, it.o, it.K, it.J, it.I);
         d = ldl(L.diag + it.o * 8u + c);
 #pragma unroll
         for (int k = 0; k < 6; ++k) cf[k] = link_coef(L, nn, it.o, k, c);
+        acc = r - offsum_v(cf, xn, n.e);
     }
-    const real acc = r - offsum_v(cf, xn, n.e);
-    stl(L.x + own * 8u + c, d <= L.dmin ? RL(0.0) : acc / d);
+    stc(L.x, own, c, d <= L.dmin ? RL(0.0) : acc / d);
 }
 
 // prolong_black_item of a black item.  The coarse values: the cell's own coarse cell and, along
@@ -415,44 +438,61 @@ __device__ __forceinline__ void smooth_even(const LvlT<LT>& L, const LevelShape&
 // an odd coordinate, the -ax one for an even), which holds one of its two neighbours.
 template <class LT>
 __device__ __forceinline__ void prolong_black_even(const LvlT<LT>& F, const LvlT<LT>& C, const LevelShape& s,
-                                                   const HalfItem& it) {
+                                                   const HalfItem& it, const Tab& tb) {
     const int c = it.c, I = it.I, J = it.J, K = it.K;
     const uint32_t own = s.half + it.h;
     const EvenNb n = even_neighbours(s, 1, it);
     const int f = F.sf[it.o];
-    const real r = ldl(F.r + own * 8u + c);
+    const real r = ldc(F.r, own, c);
     const int Ic = I >> 1, Jc = J >> 1, Kc = K >> 1, bI = I & 1, bJ = J & 1, bK = K & 1;
     const bool ox = bI ? n.e[0] : n.e[1], oy = bJ ? n.e[2] : n.e[3], oz = bK ? n.e[4] : n.e[5];
-    const int Io = ox ? Ic + 2 * bI - 1 : Ic, Jo = oy ? Jc + 2 * bJ - 1 : Jc, Ko = oz ? Kc + 2 * bK - 1 : Kc;
     real xn[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xn[k] = ldl(F.x + n.s[k] * 8u + c);
-    const real xc0 = ldl(C.x + lsto(C.m, Kc, Jc, Ic) * 8 + c);
-    const real xcx = ldl(C.x + lsto(C.m, Kc, Jc, Io) * 8 + c);
-    const real xcy = ldl(C.x + lsto(C.m, Kc, Jo, Ic) * 8 + c);
-    const real xcz = ldl(C.x + lsto(C.m, Ko, Jc, Ic) * 8 + c);
+    for (int k = 0; k < 6; ++k) xn[k] = ldc(F.x, n.s[k], c);
+    // storage of the coarse cells: colour-split like the fine level (its neighbours are of the
+    // other colour) unless the coarse size is odd
+    uint32_t s0, sx, sy, sz;
+    const uint32_t mc = (uint32_t)C.m;
+    if ((mc & 1u) == 0) {
+        const uint32_t hc = mc >> 1, half_c = mc * mc * hc, oc = ((uint32_t)Kc * mc + (uint32_t)Jc) * mc + (uint32_t)Ic;
+        const uint32_t colc = (uint32_t)((Ic + Jc + Kc) & 1), sbc = (colc ^ 1u) * half_c + (oc >> 1);
+        const uint32_t bc = (uint32_t)(Ic & 1);
+        s0 = colc * half_c + (oc >> 1);
+        sx = ox ? (bI ? sbc + bc : sbc + bc - 1u) : s0;
+        sy = oy ? (bJ ? sbc + hc : sbc - hc) : s0;
+        sz = oz ? (bK ? sbc + mc * hc : sbc - mc * hc) : s0;
+    } else {
+        const int Io = ox ? Ic + 2 * bI - 1 : Ic, Jo = oy ? Jc + 2 * bJ - 1 : Jc, Ko = oz ? Kc + 2 * bK - 1 : Kc;
+        s0 = lsto(C.m, Kc, Jc, Ic);
+        sx = lsto(C.m, Kc, Jc, Io);
+        sy = lsto(C.m, Kc, Jo, Ic);
+        sz = lsto(C.m, Ko, Jc, Ic);
+    }
+    const real xc0 = ldc(C.x, s0, c), xcx = ldc(C.x, sx, c), xcy = ldc(C.x, sy, c), xcz = ldc(C.x, sz, c);
     const real xcn[6] = {bI ? xcx : xc0, bI ? xc0 : xcx, bJ ? xcy : xc0,
                          bJ ? xc0 : xcy, bK ? xcz : xc0, bK ? xc0 : xcz};
-    const Tab tb = load_tab(F, c);
-    real d, cf[6], xpv[6];
+    real d, a;
     if (__all(f != 0)) {
+        // a standard cell has all six neighbours, each with diag > dmin
         d = tb.d;
+        const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
+        real xpv[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            cf[k] = tb.o[k >> 1];
-            xpv[k] = as_stored<LT>(xn[k] + xcn[k]);   // a standard cell's neighbours have diag > dmin
-        }
+        for (int k = 0; k < 6; ++k) xpv[k] = as_stored<LT>(xn[k] + xcn[k]);
+        a = offsum_all(cf, xpv);
     } else {
         const LNb nn = level_neighbours(F.m, it.o, K, J, I);
         d = ldl(F.diag + it.o * 8u + c);
+        real cf[6], xpv[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             cf[k] = link_coef(F, nn, it.o, k, c);
+            // x of the neighbour after the constant prolongation (which skips diag <= dmin)
             xpv[k] = ldl(F.diag + nn.c[k] * 8 + c) <= F.dmin ? xn[k] : as_stored<LT>(xn[k] + xcn[k]);
         }
+        a = offsum_v(cf, xpv, n.e);
     }
-    const real a = offsum_v(cf, xpv, n.e);
-    stl(F.x + own * 8u + c, d <= F.dmin ? RL(0.0) : (r - a) / d);
+    stc(F.x, own, c, d <= F.dmin ? RL(0.0) : (r - a) / d);
 }
 
 // restrict_item<true> onto coarse cell (K, J, I) from an even fine level.  Child k = 4 dz + 2 dy + dx
@@ -462,7 +502,7 @@ __device__ __forceinline__ void prolong_black_even(const LvlT<LT>& F, const LvlT
 // at the offsets of even_neighbours.
 template <class LT>
 __device__ __forceinline__ void restrict_even(const LvlT<LT>& F, const LvlT<LT>& C, const LevelShape& s, int c, int K,
-                                              int J, int I, bool do_red) {
+                                              int J, int I, bool do_red, const Tab& tb) {
     const uint32_t m = s.m, mc = s.mh;
     uint32_t base[4];
 #pragma unroll
@@ -476,15 +516,15 @@ __device__ __forceinline__ void restrict_even(const LvlT<LT>& F, const LvlT<LT>&
         const int dz = k >> 2, dy = (k >> 1) & 1, dx = k & 1, col = (dx + dy + dz) & 1;
         const uint32_t bs = base[k >> 1], st = (uint32_t)col * s.half + bs, sb = (uint32_t)(col ^ 1) * s.half + bs;
         f &= (int)F.sf[2u * bs + (uint32_t)dx];
-        r[k] = ldl(F.r + st * 8u + c);
-        x0[k] = ldl(F.x + st * 8u + c);
+        r[k] = ldc(F.r, st, c);
+        x0[k] = ldc(F.x, st, c);
         // the outer neighbour along each axis: -ax for an even offset, +ax for an odd one
         const uint32_t ax_x = in_x[dx] ? (dx ? sb + 1u : sb - 1u) : sb;
         const uint32_t ax_y = in_y[dy] ? (dy ? sb + s.mh : sb - s.mh) : sb;
         const uint32_t ax_z = in_z[dz] ? (dz ? sb + s.sz : sb - s.sz) : sb;
-        xo[k][0] = ldl(F.x + ax_x * 8u + c);
-        xo[k][1] = ldl(F.x + ax_y * 8u + c);
-        xo[k][2] = ldl(F.x + ax_z * 8u + c);
+        xo[k][0] = ldc(F.x, ax_x, c);
+        xo[k][1] = ldc(F.x, ax_y, c);
+        xo[k][2] = ldc(F.x, ax_z, c);
     }
     real xn[8][6];
     bool e[8][6];
@@ -499,12 +539,12 @@ __device__ __forceinline__ void restrict_even(const LvlT<LT>& F, const LvlT<LT>&
         xn[k][4] = dz ? xo[k][2] : x0[k | 4];  e[k][4] = dz ? in_z[1] : true;
         xn[k][5] = dz ? x0[k & 3] : xo[k][2];  e[k][5] = dz ? true : in_z[0];
     }
-    const Tab tb = load_tab(F, c);
     real res[8];
     if (__all(f != 0)) {
+        // standard children have all six neighbours
         const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
 #pragma unroll
-        for (int k = 0; k < 8; ++k) res[k] = r[k] - tb.d * x0[k] - offsum_v(cf, xn[k], e[k]);
+        for (int k = 0; k < 8; ++k) res[k] = r[k] - tb.d * x0[k] - offsum_all(cf, xn[k]);
     } else {
         const uint32_t sy = m, sz = m * m;
 #pragma unroll
@@ -562,11 +602,13 @@ template <class LT>
 __global__ void __launch_bounds__(256) k_smooth_half(LvlT<LT> L, int color, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     const LevelShape sh = level_shape(L.m);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t t = (uint32_t)xcd_item();; t += stride) {
+    const uint32_t stride = gridDim.x * blockDim.x;   // a multiple of 8: the component is fixed
+    const uint32_t t0 = (uint32_t)xcd_item();
+    const Tab tb = load_tab(L, (int)(t0 & 7u));
+    for (uint32_t t = t0;; t += stride) {
         HalfItem it;
         if (!half_item_even(sh, t, color, it)) return;
-        smooth_even(L, sh, color, it);
+        smooth_even(L, sh, color, it, tb);
     }
 }
 
@@ -580,6 +622,8 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
     if (cg_inactive(st, k)) return;
     const LevelShape fs = level_shape(F.m), cs = level_shape(C.m);
     const int m = C.m;
+    const int c = (int)(threadIdx.x & 7u);   // blockDim and the stride are multiples of 8
+    const Tab tb = load_tab(F, c);
     if (m >= 32) {
         const uint32_t um = (uint32_t)m, total = um * um * ((um + 31u) / 32u);
         for (uint32_t lb = (uint32_t)xcd_block((int)blockIdx.x, (int)gridDim.x); lb < total; lb += gridDim.x) {
@@ -594,15 +638,14 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
             const int K = (int)(lb - col * um);
             const int J = (int)(col - seg * um);
             const int I = (int)(seg * 32u + (threadIdx.x >> 3));
-            const int c = (int)(threadIdx.x & 7u);
-            if (I < m) restrict_even(F, C, fs, c, K, J, I, do_red != 0);
+            if (I < m) restrict_even(F, C, fs, c, K, J, I, do_red != 0, tb);
         }
     } else {
         const uint32_t items = (uint32_t)m * m * m * 8, stride = gridDim.x * blockDim.x;
         for (uint32_t t = (uint32_t)xcd_item(); t < items; t += stride) {
             int K, J, I;
             split_cell(cs, t >> 3, K, J, I);
-            restrict_even(F, C, fs, (int)(t & 7u), K, J, I, do_red != 0);
+            restrict_even(F, C, fs, c, K, J, I, do_red != 0, tb);
         }
     }
 }
@@ -617,11 +660,13 @@ template <class LT>
 __global__ void __launch_bounds__(256) k_prolong_black(LvlT<LT> F, LvlT<LT> C, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     const LevelShape sh = level_shape(F.m);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t t = (uint32_t)xcd_item();; t += stride) {
+    const uint32_t stride = gridDim.x * blockDim.x;   // a multiple of 8: the component is fixed
+    const uint32_t t0 = (uint32_t)xcd_item();
+    const Tab tb = load_tab(F, (int)(t0 & 7u));
+    for (uint32_t t = t0;; t += stride) {
         HalfItem it;
         if (!half_item_even(sh, t, 1, it)) return;
-        prolong_black_even(F, C, sh, it);
+        prolong_black_even(F, C, sh, it, tb);
     }
 }
 
