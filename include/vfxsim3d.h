@@ -147,6 +147,13 @@ int vfx_get_advected(vfx_ctx* ctx, float* dst, double* mass_ratio, void* stream)
  * storage vfx_precond scales the level-1 right-hand side by the power of two that brings max|r|
  * over fluid cells into [1, 2) (the solver takes the scale from the CG residual). */
 int vfx_kmul(vfx_ctx* ctx, const float* solid, const float* p, float* out, void* stream);
+
+/* Self-check with no counterpart in the reference, for tests: runs the CG kernels' division by
+ * diag(K) = 6 at INTERIOR cells (a multiply and one fused-multiply-add correction instead of the
+ * division sequence) on every single-precision bit pattern and adds the number of results that
+ * differ from the IEEE quotient x / 6 to the device counter *mismatches (zero it first).  FP32
+ * build only. */
+int vfx_check_div6(unsigned long long* mismatches, void* stream);
 int vfx_precond(vfx_ctx* ctx, const float* solid, const float* r, float* z, void* stream);
 int vfx_get_ops(vfx_ctx* ctx, const float* solid, int level, float* diag, float* off, int* nlev, int* m,
                 void* stream);

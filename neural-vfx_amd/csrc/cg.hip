@@ -13,6 +13,7 @@
 // r and p live in two buffers each (iteration k reads one and writes the other), so a tile never
 // reads a neighbour tile's already-updated values.  Every kernel of CG iteration k returns at
 // once when the stop test fired at an earlier iteration (DevState::stop_at < k).
+#include "ctx.hpp"
 #include "pressure_ops.hpp"
 
 namespace vfx {
@@ -110,6 +111,14 @@ __device__ __forceinline__ bool wave_interior(const uint8_t* sm, int h0) {
         in = in && (m2 & (INTERIOR | (INTERIOR << 8))) == (INTERIOR | (INTERIOR << 8));
     }
     return __all(in);
+}
+
+// true when the four staged rows (two mask bytes each) of a 2x2x2 block are all INTERIOR cells
+__device__ __forceinline__ bool block_interior(const int* m2) {
+    bool in = true;
+#pragma unroll
+    for (int rw = 0; rw < 4; ++rw) in = in && (m2[rw] & (INTERIOR | (INTERIOR << 8))) == (INTERIOR | (INTERIOR << 8));
+    return in;
 }
 
 // K p at halo cell h (0 at a solid cell)
@@ -348,6 +357,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
     for (int it = 0; it < HB_IT; ++it) {
         if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
             const real in = hb[it].in ? RL(1.0) : RL(0.0);
+            // blocks of INTERIOR cells in every lane: kdiag = 6 (div6 is the division bit for bit)
+            const bool fast = __all(block_interior(hm[it]));
 #pragma unroll
             for (int rw = 0; rw < 4; ++rw) {
                 real rv[2] = {hr[it][rw].x, hr[it][rw].y};
@@ -357,11 +368,19 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
                 }
                 const int h = hb[it].h0 + row_h(rw);
                 real zv[2];
+                if (fast) {
 #pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                    const int m = (hm[it][rw] >> (8 * dx)) & 255;
-                    rv[dx] = rv[dx] * in;
-                    zv[dx] = kopen(m) ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0);
+                    for (int dx = 0; dx < 2; ++dx) {
+                        rv[dx] = rv[dx] * in;
+                        zv[dx] = div6(OMEGA_JAC * rv[dx]);
+                    }
+                } else {
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int m = (hm[it][rw] >> (8 * dx)) & 255;
+                        rv[dx] = rv[dx] * in;
+                        zv[dx] = kopen(m) ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0);
+                    }
                 }
                 st2(sz, h, zv[0], zv[1]);
                 if (hb[it].o0 >= 0) st2(sr, hb[it].o0 + row_o(rw), rv[0], rv[1]);
@@ -443,15 +462,23 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
             real hx[8];
 #pragma unroll
             for (int c = 0; c < 8; ++c) hx[c] = HS * ((xv[it][c] * isc) * in);
+            // blocks of INTERIOR cells in every lane: kdiag = 6 (div6 is the division bit for bit)
+            const bool fast_b = __all(block_interior(hm[it]));
 #pragma unroll
             for (int rw = 0; rw < 4; ++rw) {
                 const real rv[2] = {hr[it][rw].x * in, hr[it][rw].y * in};
                 real zt[2];
+                if (fast_b) {
 #pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                    const int m = (hm[it][rw] >> (8 * dx)) & 255;
-                    const real acc = inv_sum(kopen(m) ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0), hx, rw >> 1, rw & 1, dx);
-                    zt[dx] = m ? acc : RL(0.0);
+                    for (int dx = 0; dx < 2; ++dx) zt[dx] = inv_sum(div6(OMEGA_JAC * rv[dx]), hx, rw >> 1, rw & 1, dx);
+                } else {
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int m = (hm[it][rw] >> (8 * dx)) & 255;
+                        const real acc =
+                            inv_sum(kopen(m) ? OMEGA_JAC * rv[dx] / kdiag(m) : RL(0.0), hx, rw >> 1, rw & 1, dx);
+                        zt[dx] = m ? acc : RL(0.0);
+                    }
                 }
                 const int h = hb[it].h0 + row_h(rw);
                 st2(szt, h, zt[0], zt[1]);
@@ -479,9 +506,13 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
         real zv[2];
 #pragma unroll
         for (int dx = 0; dx < 2; ++dx) {
-            const int m = sm[h + dx];
             const real rv = sr[ho + dx];
-            zv[dx] = kopen(m) ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : RL(0.0);
+            if (fast) {
+                zv[dx] = szt[h + dx] + div6(OMEGA_JAC * (rv - kz[2 * rw + dx]));
+            } else {
+                const int m = sm[h + dx];
+                zv[dx] = kopen(m) ? szt[h + dx] + OMEGA_JAC * (rv - kz[2 * rw + dx]) / kdiag(m) : RL(0.0);
+            }
             s += (double)rv * (double)zv[dx];
             mx = fmax(mx, fabs((double)rv));
             rr += (double)rv * (double)rv;
@@ -753,3 +784,29 @@ void launch_kmul(const Geom& g, const real* p, const uint8_t* fm, real* out, hip
 }
 
 }  // namespace vfx
+
+#if !VFX_F64
+namespace vfx {
+
+// every single-precision bit pattern x: div6(x) against the IEEE quotient x / 6 (two NaN results
+// count as equal); the mismatches are added to *bad
+__global__ void __launch_bounds__(256) k_check_div6(unsigned long long* __restrict__ bad) {
+    unsigned long long n = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32); i += stride) {
+        const float x = __uint_as_float((uint32_t)i);
+        const float a = div6(x), b = x / 6.0f;
+        if (!(a != a && b != b) && __float_as_uint(a) != __float_as_uint(b)) ++n;
+    }
+    if (n) atomicAdd(bad, n);
+}
+
+}  // namespace vfx
+
+extern "C" int vfx_check_div6(unsigned long long* mismatches, void* stream) {
+    using namespace vfx;
+    if (!mismatches) return arg_error("null pointer");
+    hipLaunchKernelGGL(k_check_div6, dim3(16384), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), mismatches);
+    return launch_ok("div6 check") ? VFX_OK : VFX_E_HIP;
+}
+#endif  // !VFX_F64

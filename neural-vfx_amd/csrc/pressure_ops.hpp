@@ -70,6 +70,25 @@ __device__ __forceinline__ real kmul_xyz(const real* __restrict__ p, const uint8
 // any open face and 0 otherwise (small integers, exact in real)
 __device__ __forceinline__ real kdiag(int m) { return (real)(2 * __popc((m | (m >> 1)) & 0x15)); }
 
+// x / 6 correctly rounded, without the division sequence: q = x c with c = 1/6 rounded, then one
+// residual correction q + (x - 6 q) c with fused multiply-adds (Markstein's method for a constant
+// divisor).  Over all 2^32 single-precision inputs this equals IEEE x / 6 bit for bit except where
+// x / 6 is subnormal and x is not zero (|x| <= 2^-125); those, the infinities and NaN take the
+// division itself, and the sign of a zero result is x's.  diag(K) is 6 at every INTERIOR cell.
+__device__ __forceinline__ real div6(real x) {
+    if constexpr (sizeof(real) == 4) {
+        const float c6 = 1.0f / 6.0f;
+        const float q = x * c6;
+        const float r = __builtin_fmaf(-q, 6.0f, x);
+        float y = __builtin_copysignf(__builtin_fmaf(r, c6, q), x);
+        const float ax = __builtin_fabsf(x);
+        if (__builtin_expect(!(ax >= 0x1p-124f && ax <= 3.40282347e38f) && x != 0.0f, 0)) y = x / 6.0f;
+        return y;
+    } else {
+        return x / RL(6.0);
+    }
+}
+
 // A fluid cell with at least one open face.  A fluid cell sealed in by solids on all six sides has
 // diag(K) = 0 and a zero row and column of K (its right-hand side is 0 too): its Jacobi terms are
 // 0 here.  The reference divides 0 / 0 there (pressure.py:671, :702), and the NaN stops its PCG at
@@ -176,16 +195,18 @@ __device__ __forceinline__ real kmul_tile(const real* P, const uint8_t* M, int h
 }
 
 // kmul_tile at an INTERIOR cell: every open-face multiplier is 1 (x * 1 == x), so the same
-// operations without the mask
+// operations without the mask, and without the additions of +0 that start each face sum
+// (0 + a == a except that it turns -0 into +0: the values are equal, only the sign of a zero
+// result can differ)
 __device__ __forceinline__ real kmul_tile_interior(const real* P, int h) {
     real d = RL(0.0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const int st = ax == 0 ? 1 : (ax == 1 ? HX : HX * HY);
         const real m2 = P[h - 2 * st], m1 = P[h - st], c = P[h], p1 = P[h + st], p2 = P[h + 2 * st];
-        const real gi = (RL(0.0) + (p1 - c)) + (c - m1);
-        const real gp = (RL(0.0) + (p2 - p1)) + (p1 - c);
-        const real gm = (RL(0.0) + (c - m1)) + (m1 - m2);
+        const real gi = (p1 - c) + (c - m1);
+        const real gp = (p2 - p1) + (p1 - c);
+        const real gm = (c - m1) + (m1 - m2);
         d = d + (gi + gp);
         d = d - (gi + gm);
     }

@@ -63,7 +63,7 @@ FP32_ONLY = ("vfx_ctx_set_net", "vfx_fcg_history", "vfx_batch_create", "vfx_batc
              "vfx_guide_destroy", "vfx_guide_velocity", "vfx_guide_scalars", "vfx_guide_lowpass", "vfx_tracking_error",
              "vfx_render_create", "vfx_render_destroy", "vfx_render_set_output", "vfx_render_set_params",
              "vfx_render_draw", "vfx_render_draw_aux", "vfx_render_get_hdr", "vfx_render_set_temporal",
-             "vfx_render_reset_history", "vfx_advect_noalloc")
+             "vfx_render_reset_history", "vfx_advect_noalloc", "vfx_check_div6")
 _F = ctypes.c_float
 SIGNATURES.update({
     "vfx_net_create": [_I, ctypes.POINTER(_P)],
@@ -78,6 +78,7 @@ SIGNATURES.update({
     "vfx_net_apply": [_P, _P, _P, _P, _P, _P],
     "vfx_net_train_step_coarse": [_P, _P, _P, _P, _P, _F, ctypes.POINTER(ctypes.c_double), _P],
     "vfx_ctx_set_net": [_P, _P],
+    "vfx_check_div6": [_P, _P],
     "vfx_fcg_history": [_P, _P, _I, ctypes.POINTER(_I)],
     "vfx_batch_create": [_P, _I, _I, ctypes.POINTER(_P)],
     "vfx_batch_destroy": [_P],

@@ -171,6 +171,17 @@ def test_precond_parity(cuda, hiplib, N, coarse_m):
     assert e <= 1e-6
 
 
+def test_div6_equals_division_on_every_float(cuda, hiplib):
+    """The CG kernels divide by diag(K) = 6 at INTERIOR cells with a multiply and one fused
+    correction instead of the division sequence: on the device, for all 2^32 single-precision
+    inputs, the result equals the IEEE quotient x / 6 bit for bit (NaN for NaN)."""
+    cnt = torch.zeros(1, dtype=torch.int64, device=DEV)
+    _lib.check(hiplib.vfx_check_div6(_ptr(cnt), _stream()))
+    torch.cuda.synchronize()
+    _say(f"  div6 mismatches over 2^32 inputs: {int(cnt.item())}")
+    assert int(cnt.item()) == 0
+
+
 def test_sealed_fluid_cell_parity(cuda, hiplib):
     """A fluid cell sealed in by solids (diag(K) = 0): the preconditioner equals the oracle's, with
     0 in that cell, and the projection converges in the oracle's iterations."""
