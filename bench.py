@@ -458,11 +458,16 @@ def render_report(reps: int):
 
 
 def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: bool = True):
+    """Batched frames (run_batch) as one JSON line."""
+    print(json.dumps(run_batch(frames, warmup, seed, nsims, tune, compare=True)))
+
+
+def run_batch(frames: int, warmup: int, seed: int, nsims: int = 32, tune: bool = True, compare: bool = True):
     """Batched frames: nsims independent simulations with N drawn by seed from 48/64/96/128, seeded
     scenes (1-3 emitters, 0-2 sphere obstacles, beta U[0.5, 2]), MGPCG to rel-res 1e-5 with the
     adaptive iteration budget.  Aggregate cell-steps/s and per-simulation latency of graph frames
-    (after the autotuner), beside the same frames enqueued directly and the simulations stepped
-    one by one with Solver3D.step."""
+    (after the autotuner); with compare, beside the same frames enqueued directly and the
+    simulations stepped one by one with Solver3D.step."""
     import numpy as np
     import torch
     from vfx.sim3d import BatchSolver3D
@@ -513,6 +518,14 @@ def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: boo
 
     g = run(True)
     print(f"# graph frames: {g}", file=sys.stderr, flush=True)
+    r3 = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+    if not compare:
+        out = {"workload": f"{nsims} sims, sizes by seed {seed} from {{48,64,96,128}}^3, MGPCG to rel-res 1e-5, "
+                           f"graph frames", "sizes": sorted(sizes), "cells": int(cells), "frames": frames,
+               "tuned": tune, **r3(g)}
+        del batch, states
+        torch.cuda.empty_cache()
+        return out
     e = run(False)
     print(f"# direct frames: {e}", file=sys.stderr, flush=True)
     # one by one with the host-polled Solver3D.step
@@ -529,7 +542,6 @@ def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: boo
     seq = {"cell_steps_per_s": cells * frames / wall, "wall_ms_per_frame": 1000.0 * wall / frames,
            "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99))}
     print(f"# one by one: {seq}", file=sys.stderr, flush=True)
-    r3 = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
     out = {"metric": f"aggregate cell-steps/s, {nsims} sims N in {{48,64,96,128}}^3 (seed {seed}), MGPCG to "
                      f"rel-res 1e-5, graph frames",
            "value": round(g["cell_steps_per_s"], 1), "unit": "cell-steps/s", "n_gpus": 1, "frames": frames,
@@ -537,7 +549,7 @@ def batch_report(frames: int, warmup: int, seed: int, nsims: int = 32, tune: boo
            "config": {"workload": f"batched frames: {nsims} sims, sizes {sorted(sizes)}", "cells": int(cells)},
            "graph": r3(g), "direct": r3(e), "one_by_one": r3(seq),
            "tune": rep, "vs_one_by_one": round(g["cell_steps_per_s"] / seq["cell_steps_per_s"], 2)}
-    print(json.dumps(out))
+    return out
 
 
 def config0_report(steps: int):
@@ -711,6 +723,7 @@ def main():
                     help="the coarse correction of the V-cycle (a U-Net on the N/4 grid) or the full-resolution "
                          "U-Net preconditioner")
     ap.add_argument("--no-learned", action="store_true", help="bench line without the learned-vs-MGPCG keys")
+    ap.add_argument("--no-batch", action="store_true", help="bench line without the batched-frames keys")
     ap.add_argument("--learned-iters", type=int, default=600,
                     help="training iterations of the coarse correction for the bench line's learned-vs-MGPCG keys")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -855,6 +868,10 @@ def main():
         row = compare_learned(N, params, "coarse")
         say(f"learned {row['learned']['pressure_ms']} ms vs MGPCG {row['mgpcg']['pressure_ms']} ms")
         learned = {"train": summary, **row}
+    batch32 = None
+    if not args.no_batch and S is None and not fused:
+        print("# batched frames: 32 sims of {48,64,96,128}^3", file=sys.stderr, flush=True)
+        batch32 = run_batch(10, 3, 0, 32, tune=True, compare=False)
     if fused or S is not None:
         step_roof = None   # the byte formula is the plain MGPCG step's
     solve = ("fused projection (2 V-cycles + 4 Chebyshev sweeps, adaptive to normalised residual 5e-5)" if fused
@@ -880,6 +897,8 @@ def main():
         "learned_pressure_ms": learned["learned"]["pressure_ms"] if learned else None,
         "mgpcg_pressure_ms": learned["mgpcg"]["pressure_ms"] if learned else None,
         "learned": learned,
+        "batch32_cell_steps_per_s": round(batch32["cell_steps_per_s"], 1) if batch32 else None,
+        "batch32": batch32,
         "roofline": roofline, "step_roofline": step_roof, "cpu_baseline": cpu, "kernels": kern,
     }
     print(json.dumps(out))

@@ -7,9 +7,12 @@
 // no-ops, the device stops by itself at the budget), so a frame never waits on the host.  Each
 // simulation's last kernel is followed by a timestamp (the constant-rate wall clock of the GPU),
 // which gives per-simulation completion times inside the frame.  After a frame the device gathers
-// every simulation's PCG state into a pinned buffer; vfx_batch_wait reads it, and a simulation that
-// stopped at its budget unconverged gets twice the budget for the next frame.  A budget starts at
-// min(P_MAXIT, 32).
+// every simulation's PCG state into a pinned buffer; vfx_batch_wait reads it.  A budget starts at
+// min(P_MAXIT, 32); after each waited frame it is at least the largest iteration count of the
+// simulation's last 8 frames plus 2, and a simulation that stopped at its budget unconverged gets
+// twice the budget for the next frame.  Budgets grow at once; once 8 frames have been seen they
+// also shrink to that count when they sit 2 or more above it (iterations past the stop are
+// launched and return at once, so a loose budget costs launches).
 // The autotuner picks, per grid size, the cells per thread of the cell kernels and the size from
 // which the multigrid coarse levels run in one workgroup, and the number of streams, by timing
 // them on snapshots of the simulations' own states.
@@ -34,6 +37,7 @@ constexpr int BATCH_MAX = 4096;
 constexpr int BATCH_MAX_STREAMS = 16;
 constexpr int BUDGET_MARGIN = 2;   // iterations kept beyond the largest count seen
 constexpr int BUDGET_FIRST = 32;   // first budget of a simulation (capped by its P_MAXIT)
+constexpr int BUDGET_SHRINK = 2;   // a budget shrinks when it is this far above the window's need
 constexpr int FRAME_STATS = 9;     // per simulation: iterations, stop_at, r.r, b.b, max|r|, threshold,
                                    // true residual, budget-stop bits, first run's iterations
 
@@ -93,6 +97,7 @@ struct vfx_batch {
     double* pin_stats = nullptr;
     // per simulation
     std::vector<int> budget;       // 0 until the first frame sets it from P_MAXIT
+    std::vector<BudgetWindow> bwin;   // iteration counts of the last frames
     std::vector<float> user_maxit, rel_tol;
     std::vector<char> judged;      // the frame computes the simulation's true residual
     bool absorbed = true;          // the last frame's gathered state has been read
@@ -282,10 +287,14 @@ void absorb_frame(vfx_batch* b) {
         const int first = (int)o[8];
         if (hit) b->n_hit += 1;
         const int cap = std::max(1, (int)b->user_maxit[i]);
-        int want = std::max(b->budget[i], first + BUDGET_MARGIN);
+        BudgetWindow& w = b->bwin[i];
+        w.push(first);
+        int want = w.max() + BUDGET_MARGIN;
         if (hit & 1) want = std::max(want, 2 * b->budget[i]);
-        want = std::min(want, cap);
-        if (want > b->budget[i]) {
+        want = std::max(1, std::min(want, cap));
+        // grow at once; shrink once the window is full (the first frames of a flow from rest
+        // keep the start budget while their counts climb)
+        if (want > b->budget[i] || (w.n == BUDGET_WINDOW && want + BUDGET_SHRINK <= b->budget[i])) {
             b->budget[i] = want;
             b->dirty = true;
         }
@@ -408,6 +417,7 @@ int vfx_batch_create(vfx_ctx* const* ctxs, int n, int streams, vfx_batch** out) 
     std::stable_sort(b->order.begin(), b->order.end(),
                      [&](int a, int c) { return b->sims[a]->g.n3 > b->sims[c]->g.n3; });
     b->budget.assign(n, 0);
+    b->bwin.assign(n, BudgetWindow{});
     b->user_maxit.assign(n, 0.0f);
     b->rel_tol.assign(n, 0.0f);
     b->judged.assign(n, 0);
