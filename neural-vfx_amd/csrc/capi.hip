@@ -72,6 +72,7 @@ __global__ void k_reset_cg(DevState* st) {
     if (threadIdx.x == 0) {
         st->it = 0;
         st->stop_at = INT_MAX;
+        st->early = 0;
     }
 }
 

@@ -207,6 +207,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
         st->s[S_BB] = bb;
         st->it = 0;
         st->stop_at = INT_MAX;
+        st->early = 0;
         st->conv = 0;
         if (!gated) {
             st->it_total = 0;
@@ -321,10 +322,12 @@ template <bool UPDR, class LT>
 __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const real* __restrict__ r_in,
                                                                 real* __restrict__ r_out, const real* __restrict__ q,
                                                                 const uint8_t* __restrict__ fm, LvlT<LT> L, int do_red,
+                                                                double* __restrict__ part, int nparts,
                                                                 const DevState* __restrict__ st, int k) {
     __shared__ __align__(16) real sz[HN];
     __shared__ __align__(16) real sr[ON];   // r of the own region only (the halo needs z0 and the mask)
     __shared__ __align__(16) uint8_t sm[HN];
+    __shared__ double sh[TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
     const real a = (real)st->s[S_ALPHA];
@@ -389,6 +392,23 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
         }
     }
     __syncthreads();
+    // max|r| and r.r of the tile, in k_inv_post's order (the same partials it writes): the stop
+    // test of this iteration can run before the V-cycle
+    double mx = 0.0, rr2 = 0.0;
+#pragma unroll
+    for (int rw = 0; rw < 4; ++rw)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const real rv = sr[ob.o0 + row_o(rw) + dx];
+            mx = fmax(mx, fabs((double)rv));
+            rr2 += (double)rv * (double)rv;
+        }
+    const double bmx = block_max(mx, sh);
+    const double brr = block_sum(rr2, sh);
+    if (threadIdx.x == 0) {
+        part[nparts + tl.lin] = bmx;
+        part[2 * nparts + tl.lin] = brr;
+    }
     if (!ob.in) return;
     real kz[8];
     if (wave_interior(sm, ob.h0)) {
@@ -579,6 +599,34 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict
     }
 }
 
+// The stop test of iteration k on r alone, before its V-cycle (the max|r| and r.r partials of
+// k_pre_resid_fwd are k_inv_post's, reduced in k_fin_rz's order): when it fires, the run stops at
+// k as k_fin_rz would have made it, and the V-cycle, k_inv_post and k_fin_rz of iteration k are
+// skipped (their z would not be used).  The reference computes that last z.
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_r(const double* __restrict__ part, int nparts,
+                                               const float* __restrict__ prm, DevState* __restrict__ st, int k) {
+    __shared__ double sh[FIN_THREADS / 64];
+    if (cg_inactive(st, k)) return;
+    double vb = 0.0, vc = 0.0;
+#pragma unroll 8
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        vb = fmax(vb, part[nparts + i]);
+        vc += part[2 * nparts + i];
+    }
+    const double mr = block_max(vb, sh);
+    const double rr = block_sum(vc, sh);
+    if (threadIdx.x != 0) return;
+    const double rel_tol = (double)prm[P_REL_TOL];
+    const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
+    if (not_conv) return;
+    st->it = k;
+    st->s[S_MAXR] = mr;
+    st->s[S_RR] = rr;
+    st->conv = 1;
+    st->early = 1;
+    st->stop_at = k;
+}
+
 // Device-side restart decision after the true residual of a CG run (the host-polled form reads
 // the same test on the host): above the relative tolerance, the next gated run restarts CG from
 // the true residual with phi as its warm start; otherwise every gated kernel after it is a
@@ -680,7 +728,7 @@ void Pressure::init_tiles() {
 
 real* Pressure::r_buf(int k) const { return R[k & 1]; }
 
-void Pressure::precond(int k, hipStream_t s) {
+void Pressure::precond(int k, hipStream_t s, const float* prm) {
     real* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
     real* r_out = R[k & 1];
     with_mg_type(mg_kind, [&](auto tag) {
@@ -688,11 +736,12 @@ void Pressure::precond(int k, hipStream_t s) {
         const LvlT<LT> L1 = lvl_as<LT>(lv[0]);
         if (k > 0)
             hipLaunchKernelGGL((k_pre_resid_fwd<true, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q,
-                               fm, L1, S > 0 ? 1 : 0, st, k);
+                               fm, L1, S > 0 ? 1 : 0, part, ntiles, st, k);
         else
             hipLaunchKernelGGL((k_pre_resid_fwd<false, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q,
-                               fm, L1, S > 0 ? 1 : 0, st, k);
+                               fm, L1, S > 0 ? 1 : 0, part, ntiles, st, k);
     });
+    if (prm) hipLaunchKernelGGL(k_fin_r, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k);
     vcycle_levels(k, s);
     with_mg_type(mg_kind, [&](auto tag) {
         using LT = decltype(tag);
@@ -726,7 +775,7 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int h
                                part_pq, st, k);
         hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, ntiles, st, k);
     }
-    precond(k, s);
+    precond(k, s, prm);
     hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, kmax, hitbit);
 }
 
@@ -768,7 +817,7 @@ bool Pressure::launch_single(int which, real* scratch, hipStream_t s) {
             with_mg_type(mg_kind, [&](auto tag) {
                 using LT = decltype(tag);
                 hipLaunchKernelGGL((k_pre_resid_fwd<true, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, R[0],
-                                   scratch, q, fm, lvl_as<LT>(lv[0]), S > 0 ? 1 : 0, st, 2);
+                                   scratch, q, fm, lvl_as<LT>(lv[0]), S > 0 ? 1 : 0, part, ntiles, st, 2);
             });
             return true;
         default: return launch_single_mg(which, s);

@@ -157,6 +157,7 @@ __global__ void k_fstart(DevState* st, int cycles) {
     if (threadIdx.x == 0) {
         st->it = cycles;
         st->stop_at = INT_MAX;
+        st->early = 0;
         st->s[S_MAXR] = -1.0;
         st->it_total = 0;
         st->restarts = 0;

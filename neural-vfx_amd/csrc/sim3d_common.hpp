@@ -82,6 +82,8 @@ struct DevState {
     int restarts;  // device-side restarts of this projection
     int live;      // 1 while the projection's current CG run is live (gates the restart kernels)
     int hit;       // bit 0: the first CG run stopped at its iteration budget unconverged; bit 1: a restart did
+    int early;     // 1 when the stop test of iteration stop_at fired on r before that iteration's
+                   // V-cycle, whose kernels (and the rest of the iteration) are then no-ops
 };
 
 constexpr int FLUID = 64;          // face-mask fluid bit (pressure.py:32-33)

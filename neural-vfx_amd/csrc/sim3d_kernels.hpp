@@ -174,7 +174,8 @@ struct Pressure {
     // gated: the rhs of a device-side restart (runs only when the restart gate opened);
     // restart: the rhs of a restart run (geom_r)
     void rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated = false, bool restart = false);
-    void precond(int k, hipStream_t s);
+    // prm: with the parameters, iteration k's stop test also runs on r before the V-cycle (k_fin_r)
+    void precond(int k, hipStream_t s, const float* prm = nullptr);
     // CG iteration k of a run whose host enqueued iterations 0..kmax (hitbit: DevState::hit bit
     // set when that budget, not the stop test, ends the run)
     void iteration(int k, const float* prm, hipStream_t s, int kmax = 1 << 30, int hitbit = 1);
