@@ -593,6 +593,29 @@ __device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K
 // items (32768 at level 1 of 256^3).
 constexpr unsigned LEVEL_GRID = 4096;
 
+// The logical blocks [0, T) of a striding level launch, split so that each XCD (workgroup b runs
+// on XCD b % 8) sweeps one contiguous eighth of them: the neighbour planes an item reads are then
+// mostly the ones its own XCD reads, in its own L2.  A launch of one pass (or a grid that is not a
+// multiple of 8) keeps the per-pass XCD renumbering.
+struct XcdRange {
+    uint32_t first, end, step;
+};
+__device__ __forceinline__ XcdRange xcd_range(uint32_t T) {
+    const uint32_t G = gridDim.x;
+    XcdRange r;
+    if (G >= T || G % NUM_XCD != 0) {
+        r.first = (uint32_t)xcd_block((int)blockIdx.x, (int)G);
+        r.end = T;
+        r.step = G;
+        return r;
+    }
+    const uint32_t x = blockIdx.x % NUM_XCD, per = (T + NUM_XCD - 1) / NUM_XCD;
+    r.first = x * per + blockIdx.x / NUM_XCD;
+    r.end = min((x + 1) * per, T);
+    r.step = G / NUM_XCD;
+    return r;
+}
+
 static unsigned level_grid(size_t items) {
     const size_t b = (items + 255) / 256;
     return (unsigned)(b < 1 ? 1 : (b > LEVEL_GRID ? LEVEL_GRID : b));
@@ -602,12 +625,11 @@ template <class LT>
 __global__ void __launch_bounds__(256) k_smooth_half(LvlT<LT> L, int color, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     const LevelShape sh = level_shape(L.m);
-    const uint32_t stride = gridDim.x * blockDim.x;   // a multiple of 8: the component is fixed
-    const uint32_t t0 = (uint32_t)xcd_item();
-    const Tab tb = load_tab(L, (int)(t0 & 7u));
-    for (uint32_t t = t0;; t += stride) {
+    const Tab tb = load_tab(L, (int)(threadIdx.x & 7u));   // blockDim is a multiple of 8: the component is fixed
+    const XcdRange rg = xcd_range((sh.half * 8u + blockDim.x - 1) / blockDim.x);
+    for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
         HalfItem it;
-        if (!half_item_even(sh, t, color, it)) return;
+        if (!half_item_even(sh, lb * blockDim.x + threadIdx.x, color, it)) return;
         smooth_even(L, sh, color, it, tb);
     }
 }
@@ -626,7 +648,8 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
     const Tab tb = load_tab(F, c);
     if (m >= 32) {
         const uint32_t um = (uint32_t)m, total = um * um * ((um + 31u) / 32u);
-        for (uint32_t lb = (uint32_t)xcd_block((int)blockIdx.x, (int)gridDim.x); lb < total; lb += gridDim.x) {
+        const XcdRange rg = xcd_range(total);
+        for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
             uint32_t col, seg;
             if (cs.lm >= 0) {
                 col = lb >> cs.lm;
@@ -641,8 +664,11 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
             if (I < m) restrict_even(F, C, fs, c, K, J, I, do_red != 0, tb);
         }
     } else {
-        const uint32_t items = (uint32_t)m * m * m * 8, stride = gridDim.x * blockDim.x;
-        for (uint32_t t = (uint32_t)xcd_item(); t < items; t += stride) {
+        const uint32_t items = (uint32_t)m * m * m * 8;
+        const XcdRange rg = xcd_range((items + blockDim.x - 1) / blockDim.x);
+        for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
+            const uint32_t t = lb * blockDim.x + threadIdx.x;
+            if (t >= items) break;
             int K, J, I;
             split_cell(cs, t >> 3, K, J, I);
             restrict_even(F, C, fs, c, K, J, I, do_red != 0, tb);
@@ -660,12 +686,11 @@ template <class LT>
 __global__ void __launch_bounds__(256) k_prolong_black(LvlT<LT> F, LvlT<LT> C, const DevState* __restrict__ st, int k) {
     if (cg_inactive(st, k)) return;
     const LevelShape sh = level_shape(F.m);
-    const uint32_t stride = gridDim.x * blockDim.x;   // a multiple of 8: the component is fixed
-    const uint32_t t0 = (uint32_t)xcd_item();
-    const Tab tb = load_tab(F, (int)(t0 & 7u));
-    for (uint32_t t = t0;; t += stride) {
+    const Tab tb = load_tab(F, (int)(threadIdx.x & 7u));   // blockDim is a multiple of 8: the component is fixed
+    const XcdRange rg = xcd_range((sh.half * 8u + blockDim.x - 1) / blockDim.x);
+    for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
         HalfItem it;
-        if (!half_item_even(sh, t, 1, it)) return;
+        if (!half_item_even(sh, lb * blockDim.x + threadIdx.x, 1, it)) return;
         prolong_black_even(F, C, sh, it, tb);
     }
 }
