@@ -603,8 +603,13 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict
 // k_pre_resid_fwd are k_inv_post's, reduced in k_fin_rz's order): when it fires, the run stops at
 // k as k_fin_rz would have made it, and the V-cycle, k_inv_post and k_fin_rz of iteration k are
 // skipped (their z would not be used).  The reference computes that last z.
+// LAST: iteration k is the last of the budget the host enqueued (kmax = k), launched without its
+// V-cycle: the run stops here in any case, and unconverged it records the budget stop (bit
+// hitbit of DevState::hit) as k_fin_rz would (its health test on max|r|; r.z is not formed).
+template <bool LAST>
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_r(const double* __restrict__ part, int nparts,
-                                               const float* __restrict__ prm, DevState* __restrict__ st, int k) {
+                                               const float* __restrict__ prm, DevState* __restrict__ st, int k,
+                                               int hitbit) {
     __shared__ double sh[FIN_THREADS / 64];
     if (cg_inactive(st, k)) return;
     double vb = 0.0, vc = 0.0;
@@ -618,13 +623,17 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_r(const double* __restrict_
     if (threadIdx.x != 0) return;
     const double rel_tol = (double)prm[P_REL_TOL];
     const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
-    if (not_conv) return;
+    if (not_conv && !LAST) return;
     st->it = k;
     st->s[S_MAXR] = mr;
     st->s[S_RR] = rr;
-    st->conv = 1;
+    st->conv = not_conv ? 0 : 1;
     st->early = 1;
     st->stop_at = k;
+    if (LAST && not_conv) {
+        const double best = k == 0 ? mr : fmin(st->s[S_BEST], mr);
+        if (k < (int)prm[P_MAXIT] && mr <= 4.0 * best) st->hit |= hitbit;
+    }
 }
 
 // Device-side restart decision after the true residual of a CG run (the host-polled form reads
@@ -729,6 +738,12 @@ void Pressure::init_tiles() {
 real* Pressure::r_buf(int k) const { return R[k & 1]; }
 
 void Pressure::precond(int k, hipStream_t s, const float* prm) {
+    precond_front(k, s);
+    if (prm) hipLaunchKernelGGL(k_fin_r<false>, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, 0);
+    precond_back(k, s);
+}
+
+void Pressure::precond_front(int k, hipStream_t s) {
     real* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
     real* r_out = R[k & 1];
     with_mg_type(mg_kind, [&](auto tag) {
@@ -741,7 +756,10 @@ void Pressure::precond(int k, hipStream_t s, const float* prm) {
             hipLaunchKernelGGL((k_pre_resid_fwd<false, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q,
                                fm, L1, S > 0 ? 1 : 0, part, ntiles, st, k);
     });
-    if (prm) hipLaunchKernelGGL(k_fin_r, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k);
+}
+
+void Pressure::precond_back(int k, hipStream_t s) {
+    real* r_out = R[k & 1];
     vcycle_levels(k, s);
     with_mg_type(mg_kind, [&](auto tag) {
         using LT = decltype(tag);
@@ -774,6 +792,12 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int h
             hipLaunchKernelGGL(k_upd_matvec<false>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, z, P[0], P[0], phi, fm, q,
                                part_pq, st, k);
         hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, ntiles, st, k);
+    }
+    if (k >= kmax) {
+        // the budget's last iteration: the stop test on r decides; its z would not be used
+        precond_front(k, s);
+        hipLaunchKernelGGL(k_fin_r<true>, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, hitbit);
+        return;
     }
     precond(k, s, prm);
     hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, kmax, hitbit);
