@@ -616,18 +616,18 @@ def config0_report(steps: int):
                                              "U[0.8, 1.2], MGPCG to rel-res 1e-5, dt 1/60, seed 0"}}))
 
 
-def seeds_report(steps: int, N: int = 256):
-    """The 256^3 fire for `steps` steps (dt 1/60) from rest with noise seeds 0-3: per seed the
-    median ms per step (HIP events around each step, steps 20+), PCG iterations (median, max),
-    the worst true relative residual and restarts, and the max normalised |div u| after the last
-    step."""
+def seeds_report(steps: int, N: int = 256, mg: str = "fp16"):
+    """The 256^3 fire for `steps` steps (dt 1/60) from rest with noise seeds 0-3 (multigrid levels
+    stored as `mg`): per seed the median ms per step (HIP events around each step, steps 20+),
+    PCG iterations (median, max), the worst true relative residual and restarts, and the max
+    normalised |div u| after the last step."""
     import torch
     from vfx.sim3d import Solver3D
     med = lambda v: round(float(statistics.median(v)), 3)
     rows = []
     f = fire_forces()
     for seed in range(4):
-        s = Solver3D(N, fire_config(seed))
+        s = Solver3D(N, dataclasses.replace(fire_config(seed), mg_storage=mg))
         st = s.reset()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ms, its, worst, restarts = [], [], 0.0, 0
@@ -763,15 +763,17 @@ def main():
         return
     if args.seeds > 0:
         torch.cuda.set_device(torch.device("cuda:0"))
-        seeds_report(args.seeds, args.res)
+        seeds_report(args.seeds, args.res, args.mg)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     N = args.res
     dt = 1.0 / 60.0
+    fused = args.pressure == "fused"
+    if fused:
+        args.mg = "fp32"   # the fused projection keeps its levels in FP32
     cfg = dataclasses.replace(fire_config(args.seed), mg_storage=args.mg)
     mg_bytes = 4 if args.mg == "fp32" else 2
-    fused = args.pressure == "fused"
     if fused:
         cfg = dataclasses.replace(cfg, pressure_mode="fused", pressure_tol=5e-5, pressure_max_iters=24)
     forces = fire_forces()
