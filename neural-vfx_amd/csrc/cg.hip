@@ -305,6 +305,176 @@ __global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const real* 
     if (threadIdx.x == 0) part[tl.lin] = t2;
 }
 
+// coordinate c (-2 .. N+1) of a halo -> the staged cell: the nearest inside 2x2x2 block, same parity
+__device__ __forceinline__ int cpar(int c, int N) { return 2 * min(max(c >> 1, 0), (N >> 1) - 1) + (c & 1); }
+
+// ---------------------------------------------------------------- z-marching matvec
+// k_upd_matvec as a walk up z: a workgroup owns a ZX x ZY column of cells over ZC planes and keeps
+// the planes K reaches (z - 2 .. z + 2, each with a 2-cell xy halo) in a ring of 6 LDS planes.
+// While it computes plane z from the ring it has already issued the loads of plane z + 3, so the
+// memory traffic of one plane overlaps the arithmetic of the previous one, and one barrier per
+// plane suffices (the plane a lane stages is never one the other lanes still read).  Values, the
+// staging of cells outside the box (the nearest inside block's cell, scaled by 0, mask 0) and K are
+// the tile kernel's; the p.q partials are summed per plane column instead of per tile.
+constexpr int ZX = 64, ZY = 8, ZC = 32;          // own cells of a workgroup: ZX x ZY x ZC
+constexpr int ZT = (ZX / 2) * ZY;                // threads: two x cells each
+constexpr int PW = ZX + 4, PH = ZY + 4;          // a ring plane with its xy halo
+constexpr int PS = PW * PH;                      // cells per ring plane
+constexpr int PPAIRS = PS / 2;                   // x pairs per ring plane
+constexpr int PR = (PPAIRS + ZT - 1) / ZT;       // staging rounds per plane
+constexpr int RING = 6;
+
+// K at in-plane index h of the middle plane: P[j] = ring plane c + j - 2, M[j] = masks of plane
+// c + j - 1 (kmul_tile's arithmetic)
+template <bool INT>
+__device__ __forceinline__ real kmul_ring(const real* const (&P)[5], const uint8_t* const (&M)[3], int h) {
+    real d = RL(0.0);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        real m2, m1, c, p1, p2;
+        int m0 = 0, mp = 0, mm = 0;
+        if (ax < 2) {
+            const int st = ax == 0 ? 1 : PW;
+            m2 = P[2][h - 2 * st]; m1 = P[2][h - st]; c = P[2][h]; p1 = P[2][h + st]; p2 = P[2][h + 2 * st];
+            if (!INT) { m0 = M[1][h]; mp = M[1][h + st]; mm = M[1][h - st]; }
+        } else {
+            m2 = P[0][h]; m1 = P[1][h]; c = P[2][h]; p1 = P[3][h]; p2 = P[4][h];
+            if (!INT) { m0 = M[1][h]; mp = M[2][h]; mm = M[0][h]; }
+        }
+        if (INT) {
+            const real gi = (p1 - c) + (c - m1);
+            const real gp = (p2 - p1) + (p1 - c);
+            const real gm = (c - m1) + (m1 - m2);
+            d = d + (gi + gp);
+            d = d - (gi + gm);
+        } else {
+            const real op = (real)((m0 >> (2 * ax)) & 1), om = (real)((m0 >> (2 * ax + 1)) & 1);
+            const real opp = (real)((mp >> (2 * ax)) & 1), omp = (real)((mp >> (2 * ax + 1)) & 1);
+            const real opm = (real)((mm >> (2 * ax)) & 1), omm = (real)((mm >> (2 * ax + 1)) & 1);
+            const real gi = (RL(0.0) + op * (p1 - c)) + om * (c - m1);
+            const real gp = (RL(0.0) + opp * (p2 - p1)) + omp * (p1 - c);
+            const real gm = (RL(0.0) + opm * (c - m1)) + omm * (m1 - m2);
+            d = d + op * (gi + gp);
+            d = d - om * (gi + gm);
+        }
+    }
+    return -d;
+}
+
+// the raw values of one ring plane a thread stages: x pairs t, t + ZT, ... of plane zp
+struct PlaneRegs {
+    real2 p[PR], z[PR];
+    int m[PR];
+};
+
+template <bool UPD>
+__device__ __forceinline__ void plane_load(PlaneRegs& r, const real* __restrict__ p_in, const real* __restrict__ zf,
+                                           const uint8_t* __restrict__ fm, int N, int x0, int y0, int zp) {
+    const int zc = cpar(zp, N);
+#pragma unroll
+    for (int it = 0; it < PR; ++it) {
+        const int pr = min((int)threadIdx.x + it * ZT, PPAIRS - 1);
+        const int X = x0 - 2 + 2 * (pr % (PW / 2)), Y = y0 - 2 + pr / (PW / 2);
+        const int i = (zc * N + cpar(Y, N)) * N + cpar(X, N);
+        r.p[it] = ld2(p_in, i);
+        if (UPD) r.z[it] = ld2(zf, i);
+        r.m[it] = ldm2(fm, i);
+    }
+}
+
+template <bool UPD>
+__device__ __forceinline__ void plane_stage(const PlaneRegs& r, real* __restrict__ sp, uint8_t* __restrict__ sm, real b,
+                                            int N, int x0, int y0, int zp) {
+    const bool zin = (unsigned)zp < (unsigned)N;
+#pragma unroll
+    for (int it = 0; it < PR; ++it) {
+        const int pr = (int)threadIdx.x + it * ZT;
+        if (it + 1 < PR || pr < PPAIRS) {
+            const int X = x0 - 2 + 2 * (pr % (PW / 2)), Y = y0 - 2 + pr / (PW / 2);
+            const bool in = zin && (unsigned)X < (unsigned)N && (unsigned)Y < (unsigned)N;
+            const real s = in ? RL(1.0) : RL(0.0);
+            real v0 = r.p[it].x, v1 = r.p[it].y;
+            if (UPD) {
+                v0 = r.z[it].x + b * r.p[it].x;
+                v1 = r.z[it].y + b * r.p[it].y;
+            }
+            st2(sp, 2 * pr, v0 * s, v1 * s);
+            *reinterpret_cast<uint16_t*>(sm + 2 * pr) = (uint16_t)(in ? r.m[it] : 0);
+        }
+    }
+}
+
+template <bool UPD>
+__global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restrict__ z, const real* __restrict__ p_in,
+                                                     real* __restrict__ p_out, real* __restrict__ phi,
+                                                     const uint8_t* __restrict__ fm, real* __restrict__ q,
+                                                     double* __restrict__ part, const DevState* __restrict__ st, int k) {
+    __shared__ __align__(16) real sp[RING][PS];
+    __shared__ __align__(16) uint8_t sm[RING][PS];
+    __shared__ double sh[ZT / 64];
+    if (cg_inactive(st, k)) return;
+    const real a = (real)st->s[S_ALPHA], b = (real)st->s[S_BETA];
+    // column: x fastest, then y, then the z chunk (XCD-renumbered, neighbouring columns in one L2)
+    const int cx = (N + ZX - 1) / ZX, cy = (N + ZY - 1) / ZY;
+    const int lin = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int x0 = (lin % cx) * ZX, y0 = ((lin / cx) % cy) * ZY, z0 = (lin / (cx * cy)) * ZC;
+    const int z1 = min(z0 + ZC, N);
+    const int t = (int)threadIdx.x;
+    const int lx = 2 * (t % (ZX / 2)), ly = t / (ZX / 2);
+    const int ox = x0 + lx, oy = y0 + ly;
+    const bool own = ox < N && oy < N;
+    const int h = (ly + 2) * PW + lx + 2;
+    PlaneRegs pre;
+    // planes z0 - 2 .. z0 + 1 into ring slots 0 .. 3 (plane zp lives in slot (zp - z0 + 2) % RING)
+    for (int j = 0; j < 4; ++j) {
+        plane_load<UPD>(pre, p_in, z, fm, N, x0, y0, z0 - 2 + j);
+        plane_stage<UPD>(pre, sp[j], sm[j], b, N, x0, y0, z0 - 2 + j);
+    }
+    plane_load<UPD>(pre, p_in, z, fm, N, x0, y0, z0 + 2);
+    double s = 0.0;
+    for (int zc = z0; zc < z1; ++zc) {
+        const int base = zc - z0;
+        plane_stage<UPD>(pre, sp[(base + 4) % RING], sm[(base + 4) % RING], b, N, x0, y0, zc + 2);
+        __syncthreads();
+        if (zc + 3 <= z1 + 1) plane_load<UPD>(pre, p_in, z, fm, N, x0, y0, zc + 3);
+        const int oi = (min(zc, N - 1) * N + min(oy, N - 1)) * N + min(ox, N - 2);
+        real2 po, ph;
+        if (UPD) {
+            po = ld2(p_in, oi);
+            ph = ld2(phi, oi);
+        }
+        const real* P[5] = {sp[base % RING], sp[(base + 1) % RING], sp[(base + 2) % RING], sp[(base + 3) % RING],
+                            sp[(base + 4) % RING]};
+        const uint8_t* M[3] = {sm[(base + 1) % RING], sm[(base + 2) % RING], sm[(base + 3) % RING]};
+        const int m2 = *reinterpret_cast<const uint16_t*>(M[1] + h);
+        const bool interior = (m2 & (INTERIOR | (INTERIOR << 8))) == (INTERIOR | (INTERIOR << 8));
+        real q0, q1;
+        if (__all(interior)) {
+            q0 = kmul_ring<true>(P, M, h);
+            q1 = kmul_ring<true>(P, M, h + 1);
+        } else {
+            q0 = (m2 & 255) ? kmul_ring<false>(P, M, h) : RL(0.0);
+            q1 = (m2 >> 8) ? kmul_ring<false>(P, M, h + 1) : RL(0.0);
+        }
+        const real pn0 = P[2][h], pn1 = P[2][h + 1];
+        s += (double)q0 * (double)pn0;
+        s += (double)q1 * (double)pn1;
+        if (own) {
+            if (UPD) {
+                st2(phi, oi, ph.x + a * po.x, ph.y + a * po.y);
+                st2(p_out, oi, pn0, pn1);
+            }
+            st2(q, oi, q0, q1);
+        }
+    }
+    const double t2 = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t2;
+}
+
+static unsigned zmarch_blocks(int N) {
+    return (unsigned)(((N + ZX - 1) / ZX) * ((N + ZY - 1) / ZY) * ((N + ZC - 1) / ZC));
+}
+
 // alpha = rz / p.q (the finalize the reference folds into k_pre, :652-659)
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_pq(const double* __restrict__ part, int nparts,
                                                 DevState* __restrict__ st, int k) {
@@ -785,13 +955,14 @@ void Pressure::restart_gate(const float* prm, hipStream_t s) {
 void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int hitbit) {
     if (k > 0) {
         // p_{k-1} = z_{k-1} + beta p_{k-2} (k >= 2); p_0 = z_0 was written at k = 0
+        const unsigned zb = zmarch_blocks(N);
         if (k >= 2)
-            hipLaunchKernelGGL(k_upd_matvec<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, z, P[(k - 2) & 1],
-                               P[(k - 1) & 1], phi, fm, q, part_pq, st, k);
+            hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zb), dim3(ZT), 0, s, N, z, P[(k - 2) & 1], P[(k - 1) & 1], phi,
+                               fm, q, part_pq, st, k);
         else
-            hipLaunchKernelGGL(k_upd_matvec<false>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, z, P[0], P[0], phi, fm, q,
-                               part_pq, st, k);
-        hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, ntiles, st, k);
+            hipLaunchKernelGGL(k_upd_matvec_z<false>, dim3(zb), dim3(ZT), 0, s, N, z, P[0], P[0], phi, fm, q, part_pq,
+                               st, k);
+        hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, (int)zb, st, k);
     }
     if (k >= kmax) {
         // the budget's last iteration: the stop test on r decides; its z would not be used
@@ -827,7 +998,7 @@ bool Pressure::launch_single(int which, real* scratch, hipStream_t s) {
     const size_t n3 = (size_t)N * N * N;
     switch (which) {
         case 0:
-            hipLaunchKernelGGL(k_upd_matvec<true>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, z, P[0], scratch,
+            hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zmarch_blocks(N)), dim3(ZT), 0, s, N, z, P[0], scratch,
                                scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2);
             return true;
         case 1:
