@@ -218,104 +218,19 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
     }
 }
 
-// k_upd_p of iteration k-1 (:712-728, when UPD) + k_matvec_dot of iteration k (:629-646):
-// p = z + beta p_old, phi += alpha p_old; q = K p; block partial p.q (double).
-template <bool UPD>
-__global__ void __launch_bounds__(TILE_THREADS) k_upd_matvec(int N, const real* __restrict__ z,
-                                                             const real* __restrict__ p_in, real* __restrict__ p_out,
-                                                             real* __restrict__ phi, const uint8_t* __restrict__ fm,
-                                                             real* __restrict__ q, double* __restrict__ part,
-                                                             const DevState* __restrict__ st, int k) {
-    __shared__ __align__(16) real sp[HN];
-    __shared__ __align__(16) uint8_t sm[HN];
-    __shared__ double sh[TILE_THREADS / 64];
-    if (cg_inactive(st, k)) return;
-    const Tile tl = this_tile(N);
-    const real a = (real)st->s[S_ALPHA], b = (real)st->s[S_BETA];
-    const int t = (int)threadIdx.x, n3 = N * N * N;
-    const HBlock ob = halo_block(N, tl, own_hb(t));
-    real2 po[4], ph[4];
-    if (UPD) {
-#pragma unroll
-        for (int rw = 0; rw < 4; ++rw) {
-            po[rw] = ld2(p_in, ob.i0 + row_off(N, rw));
-            ph[rw] = ld2(phi, ob.i0 + row_off(N, rw));
-        }
-    }
-    HBlock hb[HB_IT];
-    real2 hp[HB_IT][4], hz[HB_IT][4];
-    int hm[HB_IT][4];
-#pragma unroll
-    for (int it = 0; it < HB_IT; ++it) {
-        hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
-#pragma unroll
-        for (int rw = 0; rw < 4; ++rw) {
-            const int o = hb[it].i0 + row_off(N, rw);
-            hp[it][rw] = ld2(p_in, o);
-            if (UPD) hz[it][rw] = ld2(z, o);
-            hm[it][rw] = ldm2(fm, hb[it].in ? o : n3);
-        }
-    }
-    __builtin_amdgcn_sched_barrier(0);   // every load above is issued before the first use
-#pragma unroll
-    for (int it = 0; it < HB_IT; ++it) {
-        if (it + 1 < HB_IT || t + it * TILE_THREADS < HBN) {
-            const real in = hb[it].in ? RL(1.0) : RL(0.0);
-#pragma unroll
-            for (int rw = 0; rw < 4; ++rw) {
-                const real2 pv = hp[it][rw];
-                real v0 = pv.x, v1 = pv.y;
-                if (UPD) {
-                    v0 = hz[it][rw].x + b * pv.x;
-                    v1 = hz[it][rw].y + b * pv.y;
-                }
-                const int h = hb[it].h0 + row_h(rw);
-                st2(sp, h, v0 * in, v1 * in);
-                *reinterpret_cast<uint16_t*>(sm + h) = (uint16_t)hm[it][rw];
-            }
-        }
-    }
-    __syncthreads();
-    // the thread's own block; a block outside the box is staged as mask 0, value 0 (terms 0)
-    real qv[8];
-    if (wave_interior(sm, ob.h0)) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) qv[c] = kq<true>(sp, sm, ob.h0 + row_h(c >> 1) + (c & 1));
-    } else {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) qv[c] = kq<false>(sp, sm, ob.h0 + row_h(c >> 1) + (c & 1));
-    }
-    double s = 0.0;
-#pragma unroll
-    for (int rw = 0; rw < 4; ++rw) {
-        const int h = ob.h0 + row_h(rw);
-        const real pn0 = sp[h], pn1 = sp[h + 1];
-        s += (double)qv[2 * rw] * (double)pn0;
-        s += (double)qv[2 * rw + 1] * (double)pn1;
-        if (ob.in) {
-            const int i = ob.i0 + row_off(N, rw);
-            if (UPD) {
-                st2(phi, i, ph[rw].x + a * po[rw].x, ph[rw].y + a * po[rw].y);
-                st2(p_out, i, pn0, pn1);
-            }
-            st2(q, i, qv[2 * rw], qv[2 * rw + 1]);
-        }
-    }
-    const double t2 = block_sum(s, sh);
-    if (threadIdx.x == 0) part[tl.lin] = t2;
-}
-
 // coordinate c (-2 .. N+1) of a halo -> the staged cell: the nearest inside 2x2x2 block, same parity
 __device__ __forceinline__ int cpar(int c, int N) { return 2 * min(max(c >> 1, 0), (N >> 1) - 1) + (c & 1); }
 
 // ---------------------------------------------------------------- z-marching matvec
-// k_upd_matvec as a walk up z: a workgroup owns a ZX x ZY column of cells over ZC planes and keeps
+// k_upd_p of iteration k-1 (:712-728, when UPD) + k_matvec_dot of iteration k (:629-646):
+// p = z + beta p_old, phi += alpha p_old; q = K p; partial p.q (double) per workgroup.
+// A walk up z: a workgroup owns a ZX x ZY column of cells over ZC planes and keeps
 // the planes K reaches (z - 2 .. z + 2, each with a 2-cell xy halo) in a ring of 6 LDS planes.
 // While it computes plane z from the ring it has already issued the loads of plane z + 3, so the
 // memory traffic of one plane overlaps the arithmetic of the previous one, and one barrier per
 // plane suffices (the plane a lane stages is never one the other lanes still read).  Values, the
 // staging of cells outside the box (the nearest inside block's cell, scaled by 0, mask 0) and K are
-// the tile kernel's; the p.q partials are summed per plane column instead of per tile.
+// those of the LDS tiles of the other CG kernels.
 constexpr int ZX = 64, ZY = 8, ZC = 32;          // own cells of a workgroup: ZX x ZY x ZC
 constexpr int ZT = (ZX / 2) * ZY;                // threads: two x cells each
 constexpr int PW = ZX + 4, PH = ZY + 4;          // a ring plane with its xy halo
