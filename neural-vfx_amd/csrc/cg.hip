@@ -1,15 +1,18 @@
 // Fine-grid kernels of the MG-preconditioned CG projection (pressure.py:595-775, :156-175) and
 // the host-side launch sequence of one CG iteration.
 //
-// The stencil kernels are LDS-tiled: a 256-thread workgroup stages a 32x8x8 fine tile with a
-// 2-cell halo (the reach of K = d d^T) and evaluates K from LDS.  Each fused kernel performs the
-// reference's arithmetic, value by value, in the reference's order:
-//   k_upd_matvec    = k_upd_p of iteration k-1 + k_matvec_dot of iteration k
-//                     (phi += alpha p_old; p = z + beta p_old; q = K p; p.q)
+// The stencil kernels stage what K = d d^T reaches (a 2-cell halo) in LDS and evaluate K there:
+// k_upd_matvec_z walks up a 64 x 8 column with a ring of planes, k_pre_resid_fwd and k_inv_post
+// stage 32 x 8 x 16 tiles.  Each fused kernel performs the reference's arithmetic, value by value,
+// in the reference's order, and the last of its workgroups runs the finaliser of its partials:
+//   k_upd_matvec_z  = k_upd_p of iteration k-1 + k_matvec_dot of iteration k
+//                     (phi += alpha p_old; p = z + beta p_old; q = K p; p.q; alpha)
 //   k_pre_resid_fwd = k_pre + k_resid + k_fwd_red
-//                     (r -= alpha q; z0 = omega D^-1 r; rr = r - K z0; level-1 transform; red sweep)
-//   k_inv_post      = k_inv_transform + k_post_rz
-//                     (zt = z0 + T^T x1; z = zt + omega D^-1 (r - K zt); r.z, max|r|, r.r)
+//                     (r -= alpha q; z0 = omega D^-1 r; rr = r - K z0; level-1 transform; red sweep;
+//                     the stop test on r)
+//   k_inv_post      = k_inv_transform + k_post_rz + k_fin_rz
+//                     (zt = z0 + T^T x1; z = zt + omega D^-1 (r - K zt); r.z, max|r|, r.r; beta and
+//                     the stop test)
 // r and p live in two buffers each (iteration k reads one and writes the other), so a tile never
 // reads a neighbour tile's already-updated values.  Every kernel of CG iteration k returns at
 // once when the stop test fired at an earlier iteration (DevState::stop_at < k).
@@ -218,6 +221,166 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
     }
 }
 
+// ---------------------------------------------------------------- finalisers
+// The finalisers of a CG iteration run in the last workgroup of the launch whose partials they
+// reduce, not as launches of their own.  The partials are written and read with agent-scope
+// atomic stores and loads, which go to the device-coherent level and pass by the per-XCD L2s (an
+// agent-scope release / acquire would write back and invalidate the whole L2 in every
+// workgroup).  Thread 0 of each workgroup waits until its partials are written, then counts the
+// workgroup in DevState::done; the workgroup that completes the count runs the finaliser and
+// resets the count for the next launch.  A launch whose workgroups return at cg_inactive counts
+// nothing.  Every workgroup reads the solver state it needs before it counts, so the finaliser's
+// writes reach the launches after this one only.
+__device__ __forceinline__ void put_part(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double get_part(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The finalisers' partial reads: PCHUNK per thread are issued before any is used (each is a
+// round trip to the coherent level).  Thread t reads i = t, t + blockDim, t + 2 blockDim, ... in
+// that order.
+constexpr int PCHUNK = 8;
+template <int NA, class F>
+__device__ __forceinline__ void each_part(const double* __restrict__ part, int nparts, const int (&off)[NA], F&& use) {
+    const int B = (int)blockDim.x;
+    for (int i0 = (int)threadIdx.x; i0 < nparts; i0 += PCHUNK * B) {
+        double v[PCHUNK][NA];
+#pragma unroll
+        for (int j = 0; j < PCHUNK; ++j) {
+            const int i = i0 + j * B;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) v[j][a] = i < nparts ? get_part(part + off[a] + i) : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < PCHUNK; ++j)
+            if (i0 + j * B < nparts) use(v[j]);
+    }
+}
+
+__device__ __forceinline__ bool last_block(DevState* st) {
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_s_waitcnt(0);   // this thread's partial stores are complete
+        const unsigned n = gridDim.x * gridDim.y * gridDim.z;
+        const unsigned prev = __hip_atomic_fetch_add(&st->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = prev + 1u == n ? 1 : 0;
+        if (last) __hip_atomic_store(&st->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return last != 0;
+}
+
+// alpha = rz / p.q (the finalize the reference folds into k_pre, :652-659)
+__device__ void fin_pq(const double* __restrict__ part, int nparts, DevState* __restrict__ st, double* sh) {
+    double v = 0.0;
+    each_part<1>(part, nparts, {0}, [&](const double (&x)[1]) { v += x[0]; });
+    const double pq = block_sum(v, sh);
+    if (threadIdx.x == 0) st->s[S_ALPHA] = pq > 0.0 ? st->s[S_RZ] / pq : 0.0;
+}
+
+// k_fin_rz (:737-771) with FP64 scalars, plus the optional relative-residual criterion.  kmax is
+// the iteration budget the host enqueued (iterations after it do not exist): the run stops there
+// like at P_MAXIT, and DevState::hit records (bit hitbit) a stop that only the budget caused.
+__device__ void fin_rz(const double* __restrict__ part, int nparts, const float* __restrict__ prm,
+                       DevState* __restrict__ st, int k, int kmax, int hitbit, double* sh) {
+    double va = 0.0, vb = 0.0, vc = 0.0;
+    each_part<3>(part, nparts, {0, nparts, 2 * nparts}, [&](const double (&x)[3]) {
+        va += x[0];
+        vb = fmax(vb, x[1]);
+        vc += x[2];
+    });
+    const double rz_new = block_sum(va, sh);
+    const double mr = block_max(vb, sh);
+    const double rr = block_sum(vc, sh);
+    if (threadIdx.x != 0) return;
+    double best = mr;
+    if (k == 0) {
+        st->s[S_BETA] = 0.0;
+    } else {
+        const double rz = st->s[S_RZ];
+        st->s[S_BETA] = rz > 0.0 ? rz_new / rz : 0.0;
+        best = fmin(st->s[S_BEST], mr);
+    }
+    st->it = k;
+    st->s[S_RZ] = rz_new;
+    st->s[S_MAXR] = mr;
+    st->s[S_LSCALE] = pow2_scale(mr);
+    st->s[S_RR] = rr;
+    st->s[S_BEST] = best;
+    const double rel_tol = (double)prm[P_REL_TOL];
+    const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
+    const bool healthy = rz_new > 0.0 && mr <= 4.0 * best;
+    const int maxit = (int)prm[P_MAXIT];
+    const bool go = k < maxit && k < kmax && healthy && not_conv;
+    st->conv = not_conv ? 0 : 1;
+    if (!go) {
+        st->stop_at = k;
+        if (k >= kmax && k < maxit && healthy && not_conv) st->hit |= hitbit;
+    }
+}
+
+// The stop test of iteration k on r alone, before its V-cycle (the max|r| and r.r partials of
+// k_pre_resid_fwd are k_inv_post's, reduced in fin_rz's order): when it fires, the run stops at
+// k as fin_rz would have made it, and the V-cycle and k_inv_post of iteration k are skipped
+// (their z would not be used).  The reference computes that last z.
+// LAST: iteration k is the last of the budget the host enqueued (kmax = k), launched without its
+// V-cycle: the run stops here in any case, and unconverged it records the budget stop (bit
+// hitbit of DevState::hit) as fin_rz would (its health test on max|r|; r.z is not formed).
+template <bool LAST>
+__device__ void fin_r(const double* __restrict__ part, int nparts, const float* __restrict__ prm,
+                      DevState* __restrict__ st, int k, int hitbit, double* sh) {
+    double vb = 0.0, vc = 0.0;
+    each_part<2>(part, nparts, {nparts, 2 * nparts}, [&](const double (&x)[2]) {
+        vb = fmax(vb, x[0]);
+        vc += x[1];
+    });
+    const double mr = block_max(vb, sh);
+    const double rr = block_sum(vc, sh);
+    if (threadIdx.x != 0) return;
+    const double rel_tol = (double)prm[P_REL_TOL];
+    const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
+    if (not_conv && !LAST) return;
+    st->it = k;
+    st->s[S_MAXR] = mr;
+    st->s[S_RR] = rr;
+    st->conv = not_conv ? 0 : 1;
+    st->early = 1;
+    st->stop_at = k;
+    if (LAST && not_conv) {
+        const double best = k == 0 ? mr : fmin(st->s[S_BEST], mr);
+        if (k < (int)prm[P_MAXIT] && mr <= 4.0 * best) st->hit |= hitbit;
+    }
+}
+
+// The same finalisers as launches of their own, for launches of many workgroups: there each
+// workgroup's wait for its partials and its count, which keep its CU slot busy, cost more than
+// the separate launch.
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_pq(const double* __restrict__ part, int nparts,
+                                                DevState* __restrict__ st, int k) {
+    __shared__ double sh[FIN_THREADS / 64];
+    if (cg_inactive(st, k)) return;
+    fin_pq(part, nparts, st, sh);
+}
+
+template <bool LAST>
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_r(const double* __restrict__ part, int nparts,
+                                               const float* __restrict__ prm, DevState* __restrict__ st, int k,
+                                               int hitbit) {
+    __shared__ double sh[FIN_THREADS / 64];
+    if (cg_inactive(st, k)) return;
+    fin_r<LAST>(part, nparts, prm, st, k, hitbit, sh);
+}
+
+__global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict__ part, int nparts,
+                                                const float* __restrict__ prm, DevState* __restrict__ st, int k,
+                                                int kmax, int hitbit) {
+    __shared__ double sh[FIN_THREADS / 64];
+    if (cg_inactive(st, k)) return;
+    fin_rz(part, nparts, prm, st, k, kmax, hitbit, sh);
+}
+
 // coordinate c (-2 .. N+1) of a halo -> the staged cell: the nearest inside 2x2x2 block, same parity
 __device__ __forceinline__ int cpar(int c, int N) { return 2 * min(max(c >> 1, 0), (N >> 1) - 1) + (c & 1); }
 
@@ -323,7 +486,8 @@ template <bool UPD>
 __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restrict__ z, const real* __restrict__ p_in,
                                                      real* __restrict__ p_out, real* __restrict__ phi,
                                                      const uint8_t* __restrict__ fm, real* __restrict__ q,
-                                                     double* __restrict__ part, const DevState* __restrict__ st, int k) {
+                                                     double* __restrict__ part, DevState* __restrict__ st, int k,
+                                                     int fin) {
     __shared__ __align__(16) real sp[RING][PS];
     __shared__ __align__(16) uint8_t sm[RING][PS];
     __shared__ double sh[ZT / 64];
@@ -383,20 +547,12 @@ __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restri
         }
     }
     const double t2 = block_sum(s, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t2;
+    if (threadIdx.x == 0) put_part(part + blockIdx.x, t2);
+    if (fin != 0 && last_block(st)) fin_pq(part, (int)gridDim.x, st, sh);
 }
 
 static unsigned zmarch_blocks(int N) {
     return (unsigned)(((N + ZX - 1) / ZX) * ((N + ZY - 1) / ZY) * ((N + ZC - 1) / ZC));
-}
-
-// alpha = rz / p.q (the finalize the reference folds into k_pre, :652-659)
-__global__ void __launch_bounds__(FIN_THREADS) k_fin_pq(const double* __restrict__ part, int nparts,
-                                                DevState* __restrict__ st, int k) {
-    __shared__ double sh[FIN_THREADS / 64];
-    if (cg_inactive(st, k)) return;
-    const double pq = reduce_parts_sum(part, nparts, sh);
-    if (threadIdx.x == 0) st->s[S_ALPHA] = pq > 0.0 ? st->s[S_RZ] / pq : 0.0;
 }
 
 // k_pre (:648-672) + k_resid (:674-685) + k_fwd_red (:426-441).  r -= alpha q (UPDR: k > 0;
@@ -408,7 +564,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
                                                                 real* __restrict__ r_out, const real* __restrict__ q,
                                                                 const uint8_t* __restrict__ fm, LvlT<LT> L, int do_red,
                                                                 double* __restrict__ part, int nparts,
-                                                                const DevState* __restrict__ st, int k) {
+                                                                DevState* __restrict__ st, int k,
+                                                                const float* __restrict__ prm, int fin, int hitbit) {
     __shared__ __align__(16) real sz[HN];
     __shared__ __align__(16) real sr[ON];   // r of the own region only (the halo needs z0 and the mask)
     __shared__ __align__(16) uint8_t sm[HN];
@@ -491,8 +648,13 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
     const double bmx = block_max(mx, sh);
     const double brr = block_sum(rr2, sh);
     if (threadIdx.x == 0) {
-        part[nparts + tl.lin] = bmx;
-        part[2 * nparts + tl.lin] = brr;
+        put_part(part + nparts + tl.lin, bmx);
+        put_part(part + 2 * nparts + tl.lin, brr);
+    }
+    // fin 1: the stop test on r (fin_r); 2: the budget's last iteration (fin_r<true>)
+    if (fin != 0 && last_block(st)) {
+        if (fin == 1) fin_r<false>(part, nparts, prm, st, k, hitbit, sh);
+        else fin_r<true>(part, nparts, prm, st, k, hitbit, sh);
     }
     if (!ob.in) return;
     real kz[8];
@@ -535,7 +697,9 @@ template <class LT>
 __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, const real* __restrict__ r,
                                                            const uint8_t* __restrict__ fm, real* __restrict__ z,
                                                            real* __restrict__ p_init, double* __restrict__ part,
-                                                           int nparts, const DevState* __restrict__ st, int k) {
+                                                           int nparts, DevState* __restrict__ st, int k,
+                                                           const float* __restrict__ prm, int fin, int kmax,
+                                                           int hitbit) {
     __shared__ __align__(16) real szt[HN];
     __shared__ __align__(16) real sr[ON];   // r of the own region only
     __shared__ __align__(16) uint8_t sm[HN];
@@ -633,92 +797,11 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
     const double c = block_sum(rr, sh);
     if (threadIdx.x == 0) {
         const int bl = tl.lin;
-        part[bl] = a;
-        part[nparts + bl] = b;
-        part[2 * nparts + bl] = c;
+        put_part(part + bl, a);
+        put_part(part + nparts + bl, b);
+        put_part(part + 2 * nparts + bl, c);
     }
-}
-
-// k_fin_rz (:737-771) with FP64 scalars, plus the optional relative-residual criterion.  kmax is
-// the iteration budget the host enqueued (iterations after it do not exist): the run stops there
-// like at P_MAXIT, and DevState::hit records (bit hitbit) a stop that only the budget caused.
-__global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict__ part, int nparts,
-                                                const float* __restrict__ prm, DevState* __restrict__ st, int k,
-                                                int kmax, int hitbit) {
-    __shared__ double sh[FIN_THREADS / 64];
-    if (cg_inactive(st, k)) return;
-    double va = 0.0, vb = 0.0, vc = 0.0;
-#pragma unroll 8
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-        va += part[i];
-        vb = fmax(vb, part[nparts + i]);
-        vc += part[2 * nparts + i];
-    }
-    const double rz_new = block_sum(va, sh);
-    const double mr = block_max(vb, sh);
-    const double rr = block_sum(vc, sh);
-    if (threadIdx.x != 0) return;
-    double best = mr;
-    if (k == 0) {
-        st->s[S_BETA] = 0.0;
-    } else {
-        const double rz = st->s[S_RZ];
-        st->s[S_BETA] = rz > 0.0 ? rz_new / rz : 0.0;
-        best = fmin(st->s[S_BEST], mr);
-    }
-    st->it = k;
-    st->s[S_RZ] = rz_new;
-    st->s[S_MAXR] = mr;
-    st->s[S_LSCALE] = pow2_scale(mr);
-    st->s[S_RR] = rr;
-    st->s[S_BEST] = best;
-    const double rel_tol = (double)prm[P_REL_TOL];
-    const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
-    const bool healthy = rz_new > 0.0 && mr <= 4.0 * best;
-    const int maxit = (int)prm[P_MAXIT];
-    const bool go = k < maxit && k < kmax && healthy && not_conv;
-    st->conv = not_conv ? 0 : 1;
-    if (!go) {
-        st->stop_at = k;
-        if (k >= kmax && k < maxit && healthy && not_conv) st->hit |= hitbit;
-    }
-}
-
-// The stop test of iteration k on r alone, before its V-cycle (the max|r| and r.r partials of
-// k_pre_resid_fwd are k_inv_post's, reduced in k_fin_rz's order): when it fires, the run stops at
-// k as k_fin_rz would have made it, and the V-cycle, k_inv_post and k_fin_rz of iteration k are
-// skipped (their z would not be used).  The reference computes that last z.
-// LAST: iteration k is the last of the budget the host enqueued (kmax = k), launched without its
-// V-cycle: the run stops here in any case, and unconverged it records the budget stop (bit
-// hitbit of DevState::hit) as k_fin_rz would (its health test on max|r|; r.z is not formed).
-template <bool LAST>
-__global__ void __launch_bounds__(FIN_THREADS) k_fin_r(const double* __restrict__ part, int nparts,
-                                               const float* __restrict__ prm, DevState* __restrict__ st, int k,
-                                               int hitbit) {
-    __shared__ double sh[FIN_THREADS / 64];
-    if (cg_inactive(st, k)) return;
-    double vb = 0.0, vc = 0.0;
-#pragma unroll 8
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-        vb = fmax(vb, part[nparts + i]);
-        vc += part[2 * nparts + i];
-    }
-    const double mr = block_max(vb, sh);
-    const double rr = block_sum(vc, sh);
-    if (threadIdx.x != 0) return;
-    const double rel_tol = (double)prm[P_REL_TOL];
-    const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
-    if (not_conv && !LAST) return;
-    st->it = k;
-    st->s[S_MAXR] = mr;
-    st->s[S_RR] = rr;
-    st->conv = not_conv ? 0 : 1;
-    st->early = 1;
-    st->stop_at = k;
-    if (LAST && not_conv) {
-        const double best = k == 0 ? mr : fmin(st->s[S_BEST], mr);
-        if (k < (int)prm[P_MAXIT] && mr <= 4.0 * best) st->hit |= hitbit;
-    }
+    if (fin != 0 && last_block(st)) fin_rz(part, nparts, prm, st, k, kmax, hitbit, sh);
 }
 
 // Device-side restart decision after the true residual of a CG run (the host-polled form reads
@@ -815,6 +898,10 @@ __global__ void __launch_bounds__(256) k_kmul(int N, int zc, const real* __restr
 }
 
 // ================================================================= host side
+// Launches of at most FOLD_MAX_BLOCKS workgroups (about one round on the GPU) run their
+// finaliser in their last workgroup; larger ones are followed by a finaliser launch.
+constexpr int FOLD_MAX_BLOCKS = 512;
+
 void Pressure::init_tiles() {
     tgrid = dim3((N + TX - 1) / TX, (N + TY - 1) / TY, (N + TZ - 1) / TZ);
     ntiles = (int)(tgrid.x * tgrid.y * tgrid.z);
@@ -823,34 +910,41 @@ void Pressure::init_tiles() {
 real* Pressure::r_buf(int k) const { return R[k & 1]; }
 
 void Pressure::precond(int k, hipStream_t s, const float* prm) {
-    precond_front(k, s);
-    if (prm) hipLaunchKernelGGL(k_fin_r<false>, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, 0);
+    precond_front(k, s, prm, prm ? 1 : 0);
     precond_back(k, s);
 }
 
-void Pressure::precond_front(int k, hipStream_t s) {
+void Pressure::precond_front(int k, hipStream_t s, const float* prm, int fin, int hitbit) {
     real* r_in = k > 0 ? R[(k - 1) & 1] : R[0];
     real* r_out = R[k & 1];
+    const int fold = ntiles <= FOLD_MAX_BLOCKS ? fin : 0;
     with_mg_type(mg_kind, [&](auto tag) {
         using LT = decltype(tag);
         const LvlT<LT> L1 = lvl_as<LT>(lv[0]);
         if (k > 0)
             hipLaunchKernelGGL((k_pre_resid_fwd<true, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q,
-                               fm, L1, S > 0 ? 1 : 0, part, ntiles, st, k);
+                               fm, L1, S > 0 ? 1 : 0, part, ntiles, st, k, prm, fold, hitbit);
         else
             hipLaunchKernelGGL((k_pre_resid_fwd<false, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, r_in, r_out, q,
-                               fm, L1, S > 0 ? 1 : 0, part, ntiles, st, k);
+                               fm, L1, S > 0 ? 1 : 0, part, ntiles, st, k, prm, fold, hitbit);
     });
+    if (fin == 1 && fold == 0)
+        hipLaunchKernelGGL(k_fin_r<false>, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, hitbit);
+    if (fin == 2 && fold == 0)
+        hipLaunchKernelGGL(k_fin_r<true>, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, hitbit);
 }
 
-void Pressure::precond_back(int k, hipStream_t s) {
+void Pressure::precond_back(int k, hipStream_t s, const float* prm, int kmax, int hitbit) {
     real* r_out = R[k & 1];
     vcycle_levels(k, s);
+    const int fold = prm != nullptr && ntiles <= FOLD_MAX_BLOCKS ? 1 : 0;
     with_mg_type(mg_kind, [&](auto tag) {
         using LT = decltype(tag);
         hipLaunchKernelGGL(k_inv_post<LT>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lvl_as<LT>(lv[0]), r_out, fm, z,
-                           P[0], part, ntiles, st, k);
+                           P[0], part, ntiles, st, k, prm, fold, kmax, hitbit);
     });
+    if (prm != nullptr && fold == 0)
+        hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, kmax, hitbit);
 }
 
 void Pressure::rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated, bool restart) {
@@ -871,22 +965,22 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int h
     if (k > 0) {
         // p_{k-1} = z_{k-1} + beta p_{k-2} (k >= 2); p_0 = z_0 was written at k = 0
         const unsigned zb = zmarch_blocks(N);
+        const int fold = (int)zb <= FOLD_MAX_BLOCKS ? 1 : 0;
         if (k >= 2)
             hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zb), dim3(ZT), 0, s, N, z, P[(k - 2) & 1], P[(k - 1) & 1], phi,
-                               fm, q, part_pq, st, k);
+                               fm, q, part_pq, st, k, fold);
         else
             hipLaunchKernelGGL(k_upd_matvec_z<false>, dim3(zb), dim3(ZT), 0, s, N, z, P[0], P[0], phi, fm, q, part_pq,
-                               st, k);
-        hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, (int)zb, st, k);
+                               st, k, fold);
+        if (!fold) hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, (int)zb, st, k);
     }
     if (k >= kmax) {
         // the budget's last iteration: the stop test on r decides; its z would not be used
-        precond_front(k, s);
-        hipLaunchKernelGGL(k_fin_r<true>, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, hitbit);
+        precond_front(k, s, prm, 2, hitbit);
         return;
     }
-    precond(k, s, prm);
-    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FIN_THREADS), 0, s, part, ntiles, prm, st, k, kmax, hitbit);
+    precond_front(k, s, prm, 1);
+    precond_back(k, s, prm, kmax, hitbit);
 }
 
 void Pressure::finish(hipStream_t s, bool restart) {
@@ -914,20 +1008,21 @@ bool Pressure::launch_single(int which, real* scratch, hipStream_t s) {
     switch (which) {
         case 0:
             hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zmarch_blocks(N)), dim3(ZT), 0, s, N, z, P[0], scratch,
-                               scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2);
+                               scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2, 0);
             return true;
         case 1:
             with_mg_type(mg_kind, [&](auto tag) {
                 using LT = decltype(tag);
                 hipLaunchKernelGGL(k_inv_post<LT>, dim3(ntiles), dim3(TILE_THREADS), 0, s, N, lvl_as<LT>(lv[0]), R[0],
-                                   fm, z, P[0], part, ntiles, st, 2);
+                                   fm, z, P[0], part, ntiles, st, 2, nullptr, 0, 0, 0);
             });
             return true;
         case 5:
             with_mg_type(mg_kind, [&](auto tag) {
                 using LT = decltype(tag);
                 hipLaunchKernelGGL((k_pre_resid_fwd<true, LT>), dim3(ntiles), dim3(TILE_THREADS), 0, s, N, R[0],
-                                   scratch, q, fm, lvl_as<LT>(lv[0]), S > 0 ? 1 : 0, part, ntiles, st, 2);
+                                   scratch, q, fm, lvl_as<LT>(lv[0]), S > 0 ? 1 : 0, part, ntiles, st, 2, nullptr,
+                                   0, 0);
             });
             return true;
         default: return launch_single_mg(which, s);

@@ -84,6 +84,8 @@ struct DevState {
     int hit;       // bit 0: the first CG run stopped at its iteration budget unconverged; bit 1: a restart did
     int early;     // 1 when the stop test of iteration stop_at fired on r before that iteration's
                    // V-cycle, whose kernels (and the rest of the iteration) are then no-ops
+    unsigned done;   // workgroups of the current finalising launch that have written their partials
+                     // (0 between launches)
 };
 
 constexpr int FLUID = 64;          // face-mask fluid bit (pressure.py:32-33)

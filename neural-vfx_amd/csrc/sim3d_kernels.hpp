@@ -174,10 +174,14 @@ struct Pressure {
     // gated: the rhs of a device-side restart (runs only when the restart gate opened);
     // restart: the rhs of a restart run (geom_r)
     void rhs(const real* vel, const float* prm, bool warm, hipStream_t s, bool gated = false, bool restart = false);
-    // prm: with the parameters, iteration k's stop test also runs on r before the V-cycle (k_fin_r)
+    // prm: with the parameters, iteration k's stop test also runs on r before the V-cycle (fin_r)
     void precond(int k, hipStream_t s, const float* prm = nullptr);
-    void precond_front(int k, hipStream_t s);   // r update, z0 = omega D^-1 r, residual, level-1 transform
-    void precond_back(int k, hipStream_t s);    // the multigrid levels, the inverse transform and post-smoothing
+    // r update, z0 = omega D^-1 r, residual, level-1 transform; fin 1: the stop test on r, 2: the
+    // budget's last iteration (its stop, bit hitbit of DevState::hit)
+    void precond_front(int k, hipStream_t s, const float* prm = nullptr, int fin = 0, int hitbit = 0);
+    // the multigrid levels, the inverse transform and post-smoothing; with prm, also iteration k's
+    // CG scalars and stop test (fin_rz, budget kmax)
+    void precond_back(int k, hipStream_t s, const float* prm = nullptr, int kmax = 0, int hitbit = 0);
     // CG iteration k of a run whose host enqueued iterations 0..kmax (hitbit: DevState::hit bit
     // set when that budget, not the stop test, ends the run)
     void iteration(int k, const float* prm, hipStream_t s, int kmax = 1 << 30, int hitbit = 1);
