@@ -387,14 +387,14 @@ __device__ __forceinline__ int cpar(int c, int N) { return 2 * min(max(c >> 1, 0
 // ---------------------------------------------------------------- z-marching matvec
 // k_upd_p of iteration k-1 (:712-728, when UPD) + k_matvec_dot of iteration k (:629-646):
 // p = z + beta p_old, phi += alpha p_old; q = K p; partial p.q (double) per workgroup.
-// A walk up z: a workgroup owns a ZX x ZY column of cells over ZC planes and keeps
+// A walk up z: a workgroup owns a ZX x ZY column of cells over nz planes and keeps
 // the planes K reaches (z - 2 .. z + 2, each with a 2-cell xy halo) in a ring of 6 LDS planes.
 // While it computes plane z from the ring it has already issued the loads of plane z + 3, so the
 // memory traffic of one plane overlaps the arithmetic of the previous one, and one barrier per
 // plane suffices (the plane a lane stages is never one the other lanes still read).  Values, the
 // staging of cells outside the box (the nearest inside block's cell, scaled by 0, mask 0) and K are
 // those of the LDS tiles of the other CG kernels.
-constexpr int ZX = 64, ZY = 8, ZC = 32;          // own cells of a workgroup: ZX x ZY x ZC
+constexpr int ZX = 64, ZY = 8, ZC = 32;          // own cells of a workgroup: ZX x ZY x (at most) ZC
 constexpr int ZT = (ZX / 2) * ZY;                // threads: two x cells each
 constexpr int PW = ZX + 4, PH = ZY + 4;          // a ring plane with its xy halo
 constexpr int PS = PW * PH;                      // cells per ring plane
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restri
                                                      real* __restrict__ p_out, real* __restrict__ phi,
                                                      const uint8_t* __restrict__ fm, real* __restrict__ q,
                                                      double* __restrict__ part, DevState* __restrict__ st, int k,
-                                                     int fin) {
+                                                     int fin, int nz) {
     __shared__ __align__(16) real sp[RING][PS];
     __shared__ __align__(16) uint8_t sm[RING][PS];
     __shared__ double sh[ZT / 64];
@@ -496,8 +496,8 @@ __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restri
     // column: x fastest, then y, then the z chunk (XCD-renumbered, neighbouring columns in one L2)
     const int cx = (N + ZX - 1) / ZX, cy = (N + ZY - 1) / ZY;
     const int lin = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    const int x0 = (lin % cx) * ZX, y0 = ((lin / cx) % cy) * ZY, z0 = (lin / (cx * cy)) * ZC;
-    const int z1 = min(z0 + ZC, N);
+    const int x0 = (lin % cx) * ZX, y0 = ((lin / cx) % cy) * ZY, z0 = (lin / (cx * cy)) * nz;
+    const int z1 = min(z0 + nz, N);
     const int t = (int)threadIdx.x;
     const int lx = 2 * (t % (ZX / 2)), ly = t / (ZX / 2);
     const int ox = x0 + lx, oy = y0 + ly;
@@ -551,8 +551,18 @@ __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restri
     if (fin != 0 && last_block(st)) fin_pq(part, (int)gridDim.x, st, sh);
 }
 
+// planes per column: ZC, halved (down to 2) while the launch has fewer than 1024 workgroups, so
+// that small grids still spread over the GPU (64^3: 256 columns of 2 planes)
+static int zmarch_planes(int N) {
+    const int cols = ((N + ZX - 1) / ZX) * ((N + ZY - 1) / ZY);
+    int nz = ZC;
+    while (nz > 2 && cols * ((N + nz - 1) / nz) < 1024) nz /= 2;
+    return nz;
+}
+
 static unsigned zmarch_blocks(int N) {
-    return (unsigned)(((N + ZX - 1) / ZX) * ((N + ZY - 1) / ZY) * ((N + ZC - 1) / ZC));
+    const int nz = zmarch_planes(N);
+    return (unsigned)(((N + ZX - 1) / ZX) * ((N + ZY - 1) / ZY) * ((N + nz - 1) / nz));
 }
 
 // k_pre (:648-672) + k_resid (:674-685) + k_fwd_red (:426-441).  r -= alpha q (UPDR: k > 0;
@@ -968,10 +978,10 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int h
         const int fold = (int)zb <= FOLD_MAX_BLOCKS ? 1 : 0;
         if (k >= 2)
             hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zb), dim3(ZT), 0, s, N, z, P[(k - 2) & 1], P[(k - 1) & 1], phi,
-                               fm, q, part_pq, st, k, fold);
+                               fm, q, part_pq, st, k, fold, zmarch_planes(N));
         else
             hipLaunchKernelGGL(k_upd_matvec_z<false>, dim3(zb), dim3(ZT), 0, s, N, z, P[0], P[0], phi, fm, q, part_pq,
-                               st, k, fold);
+                               st, k, fold, zmarch_planes(N));
         if (!fold) hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, (int)zb, st, k);
     }
     if (k >= kmax) {
@@ -1008,7 +1018,7 @@ bool Pressure::launch_single(int which, real* scratch, hipStream_t s) {
     switch (which) {
         case 0:
             hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zmarch_blocks(N)), dim3(ZT), 0, s, N, z, P[0], scratch,
-                               scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2, 0);
+                               scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2, 0, zmarch_planes(N));
             return true;
         case 1:
             with_mg_type(mg_kind, [&](auto tag) {

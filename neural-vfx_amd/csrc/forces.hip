@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256) k_curl(int N, int zc, const real* __restr
 
 // The fluid-renormalised [1,2,1]^3 smoothing B(w a) / B(w) of three channels in ONE pass: sep_x,
 // sep_y and sep_z_store / sep_z_apply (kernels.py:392-457) composed.  A 256-thread workgroup owns
-// a 64 x 8 column of cells (two rows per thread) and marches up z through SM_ZT planes.  Each
+// a 64 x 8 column of cells (two rows per thread) and marches up z through zt planes.  Each
 // input plane is staged in LDS with a one-cell x / y halo, the x pass runs on the 10 staged rows,
 // the y pass per thread, and the y-pass values of three consecutive planes stay in registers
 // for the z pass.  Every pass keeps the reference's operation order, so the result is the three
@@ -136,14 +136,15 @@ __global__ void __launch_bounds__(256) k_curl(int N, int zc, const real* __restr
 // |omega_s| at every cell, evaluated once); MODE 1: fluid cells of the state get velocity += dt *
 // the smoothed field.  Tiles are numbered through xcd_block, so the tiles that
 // share halo rows run on one XCD.
-// 8 planes per workgroup: 0.91 ms for the forces stage at 256^3 against 0.95 ms with 16 (more
-// workgroups in flight outweigh the extra halo planes; 4 is no faster)
+// zt = SM_ZT = 8 planes per workgroup: 0.91 ms for the forces stage at 256^3 against 0.95 ms with
+// 16 (more workgroups in flight outweigh the extra halo planes; 4 is no faster).  Grids that would
+// have fewer than 1024 workgroups use fewer planes (smooth3_planes).
 constexpr int SM_TX = 64, SM_TY = 8, SM_ZT = 8, SM_RY = SM_TY / 4;   // blockDim (64, 4)
 constexpr int SM_HX = SM_TX + 2, SM_HY = SM_TY + 2;
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__ a, const uint8_t* __restrict__ fm,
-                                                 const float* __restrict__ prm, real* __restrict__ out) {
+                                                 const float* __restrict__ prm, real* __restrict__ out, int zt) {
     __shared__ real sa[3][SM_HY][SM_HX];
     __shared__ real sw[SM_HY][SM_HX];
     __shared__ real sx[4][SM_HY][SM_TX];
@@ -153,8 +154,8 @@ __global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__
     const int t = xcd_block((int)blockIdx.x, nb);
     const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SM_TX + tx;
-    const int x0 = bx * SM_TX, y0 = by * SM_TY, z0 = bz * SM_ZT;
-    const int z1 = min(z0 + SM_ZT, N);
+    const int x0 = bx * SM_TX, y0 = by * SM_TY, z0 = bz * zt;
+    const int z1 = min(z0 + zt, N);
     const int x = x0 + tx;
     const real dt = MODE == 1 ? (real)prm[P_DT] : RL(0.0);
     real ym[SM_RY][4], yc[SM_RY][4], yp[SM_RY][4];
@@ -265,8 +266,17 @@ __global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__
     }
 }
 
+// planes per workgroup: SM_ZT, halved (down to 1) while the launch has fewer than 1024 workgroups
+static int smooth3_planes(int N) {
+    const int cols = ((N + SM_TX - 1) / SM_TX) * ((N + SM_TY - 1) / SM_TY);
+    int zt = SM_ZT;
+    while (zt > 1 && cols * ((N + zt - 1) / zt) < 1024) zt /= 2;
+    return zt;
+}
+
 static int smooth3_blocks(int N) {
-    return ((N + SM_TX - 1) / SM_TX) * ((N + SM_TY - 1) / SM_TY) * ((N + SM_ZT - 1) / SM_ZT);
+    const int zt = smooth3_planes(N);
+    return ((N + SM_TX - 1) / SM_TX) * ((N + SM_TY - 1) / SM_TY) * ((N + zt - 1) / zt);
 }
 
 // |omega_s| at cell i: channel 3 of the smoothed vorticity (k_smooth3<0> stores
@@ -355,9 +365,9 @@ void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const floa
                         real* omega, real* omega_s, real* force, hipStream_t s) {
     const dim3 sb(SM_TX, 4), sg(smooth3_blocks(g.N));
     hipLaunchKernelGGL(k_curl, g.grid, g.block, 0, s, g.N, g.zc, data, fm, inv2dx, omega);
-    hipLaunchKernelGGL(k_smooth3<0>, sg, sb, 0, s, g.N, omega, fm, prm, omega_s);
+    hipLaunchKernelGGL(k_smooth3<0>, sg, sb, 0, s, g.N, omega, fm, prm, omega_s, smooth3_planes(g.N));
     hipLaunchKernelGGL(k_body_forces, g.grid, g.block, 0, s, g.N, g.zc, data, fm, omega_s, prm, force);
-    hipLaunchKernelGGL(k_smooth3<1>, sg, sb, 0, s, g.N, force, fm, prm, data);
+    hipLaunchKernelGGL(k_smooth3<1>, sg, sb, 0, s, g.N, force, fm, prm, data, smooth3_planes(g.N));
 }
 
 }  // namespace vfx
