@@ -162,6 +162,7 @@ int vfx::stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
 constexpr int BUDGET_START = 32;        // first budget of a context, before any read-back arrived
 constexpr int RESTART_BUDGET_START = 1;   // a restart from a true residual just above tolerance needs ~1
 constexpr int FUSED_TAIL_START = 1 << 20;   // fused projection: the whole tail until a count is known
+constexpr int FALLBACK_BUDGET_MIN = 16;   // restart budget after a learned run's MGPCG fallback ran out of it
 constexpr int BUDGET_MARGIN = 1;        // iterations enqueued beyond the largest recent count
 constexpr int BUDGET_CAP = 1000;
 
@@ -260,6 +261,8 @@ static void absorb(vfx_ctx* c, const DevState& d, const ReadbackInfo& in) {
         c->bw_r.push(d.it);
         int wr = std::max(RESTART_BUDGET_START, c->bw_r.max() + BUDGET_MARGIN);
         if (d.hit & 2) wr = std::max(wr, 2 * in.budget_r);
+        // after a learned run, the restart run is the MGPCG fallback: a whole solve, not a touch-up
+        if ((d.hit & 2) && in.learned) wr = std::max(wr, std::max(in.budget, FALLBACK_BUDGET_MIN));
         c->bw_r.budget = std::min(wr, BUDGET_CAP);
     }
 }
@@ -461,6 +464,58 @@ static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) 
     }
     return VFX_OK;
 }
+
+// The coarse-correction network runs its flexible CG on the device (no host synchronisation);
+// the full-resolution network and the flexible-CG checks without a network are host-polled.
+static bool learned_on_device(const vfx_ctx* c) {
+    return c->net != nullptr && c->fcg_mode == 0 && c->net->net.N * 4 == c->N;
+}
+
+// Iteration k of the learned flexible CG (run budget kmax): the MGPCG iteration's kernels with
+// the search direction formed from z_V + P c, then the network's coarse correction and beta.
+static void learned_iteration(vfx_ctx* c, int k, int kmax, int hitbit, hipStream_t s) {
+    Pressure& P = c->P;
+    Net& n = c->net->net;
+    if (k > 0) launch_upd_matvec_learned(P, k, n.ccz, s);
+    if (k >= kmax) {
+        P.precond_front(k, s, c->prm, 2, hitbit);   // the stop test on r decides
+        return;
+    }
+    P.precond_front(k, s, c->prm, 1);
+    P.precond_back(k, s);   // z_V = V(r_k) and the partials r.z_V, max|r|, r.r (no finaliser)
+    launch_resid_pool(P, k, n.work2, n.csr, n.csrp, n.lpart, n.scal, s);
+    n.coarse_forward(c->st, k, s);
+    launch_fcg_fin(P, k, n.out, n.scal, n.cmask, n.csr, n.csrp, n.ccz, n.lpart, n.fpart, s);
+}
+
+// Device-sequenced learned projection of vel: the learned run within its budget (at most
+// learned_maxit iterations), the true residual, the gate and the gated MGPCG run that finishes
+// the solve from phi when the learned run stalled, broke down, stopped unconverged or left a true
+// residual above the relative tolerance.
+static int project_learned_device(vfx_ctx* c, real* vel, hipStream_t s) {
+    Pressure& P = c->P;
+    const int maxit = (int)c->prm_last[P_MAXIT];
+    const int kl = std::max(1, std::min(c->learned_maxit, maxit));
+    const double rel_tol = (double)c->prm_last[P_REL_TOL];
+    const int B = std::min(cg_budget_main(c, maxit), kl);
+    c->fcg_hist.clear();
+    P.rhs(vel, c->prm, c->prm_last[P_WARM] > 0.5f, s);
+    c->net->net.coarse_mask(c->N, P.fm, s);
+    const int hb = B < kl ? 1 : 0;   // a stop at the learned limit is not a budget stop
+    for (int k = 0; k <= B; ++k) learned_iteration(c, k, B, hb, s);
+    P.finish(s);
+    P.true_residual(vel, s);
+    const int Br = cg_budget_restart(c, maxit);
+    P.restart_gate(c->prm, s, true);
+    P.rhs(vel, c->prm, true, s, true, true);
+    for (int k = 0; k <= Br; ++k) P.iteration(k, c->prm, s, Br, 2);
+    P.finish(s, true);
+    P.true_residual(vel, s, true, true);
+    CHECK_LAUNCH("learned pressure solve");
+    c->last_true_valid = true;
+    c->last_info = ReadbackInfo{1, 0, B, Br, 0, 1, (float)rel_tol, 1};
+    return VFX_OK;
+}
 #endif
 
 int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
@@ -484,8 +539,18 @@ int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
         c->projected = true;
         return VFX_OK;
     }
+#if !VFX_F64
+    if (device && learned_on_device(c)) {
+        if (int rc = project_learned_device(c, vel, s)) return rc;
+        P.apply(vel, s);
+        CHECK_LAUNCH("pressure apply");
+        c->projected = true;
+        return VFX_OK;
+    }
+#endif
     if (in_frame && learned) {
-        set_error("a batched frame runs MGPCG only (the flexible-CG loop waits on the host)");
+        set_error("a batched frame runs MGPCG or the coarse-correction network only (the other learned "
+                  "preconditioners wait on the host)");
         return VFX_E_VALUE;
     }
     if (device && !learned) {
@@ -580,7 +645,7 @@ void vfx::drop_graphs(vfx_ctx* c) {
 static bool graph_capable(const vfx_ctx* c) {
     if (!c->use_graph || c->sync_mode != SYNC_DEVICE || c->P.force_setup) return false;
 #if !VFX_F64
-    if (c->net != nullptr || c->fcg_mode != 0) return false;
+    if ((c->net != nullptr && !learned_on_device(c)) || c->fcg_mode != 0) return false;
 #endif
     return true;
 }
@@ -591,6 +656,9 @@ static StepSig step_sig(const vfx_ctx* c, const real* data, const real* solid) {
     std::memset(&g, 0, sizeof(g));
     g.data = data;
     g.solid = solid;
+#if !VFX_F64
+    g.net = c->net;
+#endif
     const int maxit = (int)c->prm_last[P_MAXIT];
     const double rel_tol = (double)c->prm_last[P_REL_TOL];
     int* v = g.v;
@@ -613,6 +681,12 @@ static StepSig step_sig(const vfx_ctx* c, const real* data, const real* solid) {
         v[7] = cg_budget_main(c, maxit);
         v[8] = restart ? cg_budget_restart(c, maxit) : 0;
         v[9] = (c->opt_true_res || restart) ? 1 : 0;
+#if !VFX_F64
+        if (c->net != nullptr) {
+            v[8] = cg_budget_restart(c, maxit);
+            v[10] = c->learned_maxit;
+        }
+#endif
     }
     return g;
 }
@@ -980,12 +1054,19 @@ int VFX_FN(vfx_pressure_stats)(vfx_ctx* c, double* out, int n) {
         v[6] = c->last_restarts + hs.restarts;
         bool conv = in.fused ? hs.stop_at != INT_MAX : hs.conv != 0;
 #if !VFX_F64
-        if ((c->net || c->fcg_mode) && !c->last_fallback && c->last_restarts == 0) {
-            v[3] = c->last_fcg_rn;
-            conv = true;
+        if (in.learned) {
+            // the learned run, then the gated MGPCG run (counted as MGPCG iterations)
+            v[0] = hs.restarts > 0 ? hs.it : 0;
+            v[7] = hs.restarts > 0 ? hs.it_total : hs.it;
+            v[8] = hs.fallback ? 1.0 : 0.0;
+        } else {
+            if ((c->net || c->fcg_mode) && !c->last_fallback && c->last_restarts == 0) {
+                v[3] = c->last_fcg_rn;
+                conv = true;
+            }
+            v[7] = c->last_learned;
+            v[8] = c->last_fallback ? 1.0 : 0.0;
         }
-        v[7] = c->last_learned;
-        v[8] = c->last_fallback ? 1.0 : 0.0;
 #endif
         if (c->last_true_valid && in.rel_tol > 0.0f)
             conv = std::sqrt(hs.s[S_TRUE_RR]) <= (double)in.rel_tol * std::sqrt(hs.s[S_BB]);

@@ -217,6 +217,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
             st->restarts = 0;
             st->hit = 0;
             st->live = 1;
+            st->fallback = 0;
         }
     }
 }
@@ -442,25 +443,32 @@ __device__ __forceinline__ real kmul_ring(const real* const (&P)[5], const uint8
 // the raw values of one ring plane a thread stages: x pairs t, t + ZT, ... of plane zp
 struct PlaneRegs {
     real2 p[PR], z[PR];
+    real c[PR];
     int m[PR];
 };
 
-template <bool UPD>
+// LEARN: the search direction is formed from z + P c (the coarse correction c of the learned
+// preconditioner, constant over each 4^3 block's fluid cells; cz holds (N/4)^3 values)
+template <bool UPD, bool LEARN>
 __device__ __forceinline__ void plane_load(PlaneRegs& r, const real* __restrict__ p_in, const real* __restrict__ zf,
-                                           const uint8_t* __restrict__ fm, int N, int x0, int y0, int zp) {
+                                           const real* __restrict__ cz, const uint8_t* __restrict__ fm, int N, int x0,
+                                           int y0, int zp) {
     const int zc = cpar(zp, N);
+    const int M = N >> 2;
 #pragma unroll
     for (int it = 0; it < PR; ++it) {
         const int pr = min((int)threadIdx.x + it * ZT, PPAIRS - 1);
         const int X = x0 - 2 + 2 * (pr % (PW / 2)), Y = y0 - 2 + pr / (PW / 2);
-        const int i = (zc * N + cpar(Y, N)) * N + cpar(X, N);
-        r.p[it] = ld2(p_in, i);
-        if (UPD) r.z[it] = ld2(zf, i);
+        const int cx = cpar(X, N), cy = cpar(Y, N);
+        const int i = (zc * N + cy) * N + cx;
+        if (UPD || !LEARN) r.p[it] = ld2(p_in, i);
+        if (UPD || LEARN) r.z[it] = ld2(zf, i);
+        if (LEARN) r.c[it] = cz[((zc >> 2) * M + (cy >> 2)) * M + (cx >> 2)];
         r.m[it] = ldm2(fm, i);
     }
 }
 
-template <bool UPD>
+template <bool UPD, bool LEARN>
 __device__ __forceinline__ void plane_stage(const PlaneRegs& r, real* __restrict__ sp, uint8_t* __restrict__ sm, real b,
                                             int N, int x0, int y0, int zp) {
     const bool zin = (unsigned)zp < (unsigned)N;
@@ -471,10 +479,18 @@ __device__ __forceinline__ void plane_stage(const PlaneRegs& r, real* __restrict
             const int X = x0 - 2 + 2 * (pr % (PW / 2)), Y = y0 - 2 + pr / (PW / 2);
             const bool in = zin && (unsigned)X < (unsigned)N && (unsigned)Y < (unsigned)N;
             const real s = in ? RL(1.0) : RL(0.0);
-            real v0 = r.p[it].x, v1 = r.p[it].y;
-            if (UPD) {
-                v0 = r.z[it].x + b * r.p[it].x;
-                v1 = r.z[it].y + b * r.p[it].y;
+            real v0, v1;
+            if (UPD || LEARN) {
+                real z0 = r.z[it].x, z1 = r.z[it].y;
+                if (LEARN) {
+                    z0 = z0 + ((r.m[it] & 255) ? r.c[it] : RL(0.0));
+                    z1 = z1 + ((r.m[it] >> 8) ? r.c[it] : RL(0.0));
+                }
+                v0 = UPD ? z0 + b * r.p[it].x : z0;
+                v1 = UPD ? z1 + b * r.p[it].y : z1;
+            } else {
+                v0 = r.p[it].x;
+                v1 = r.p[it].y;
             }
             st2(sp, 2 * pr, v0 * s, v1 * s);
             *reinterpret_cast<uint16_t*>(sm + 2 * pr) = (uint16_t)(in ? r.m[it] : 0);
@@ -482,12 +498,28 @@ __device__ __forceinline__ void plane_stage(const PlaneRegs& r, real* __restrict
     }
 }
 
-template <bool UPD>
+// K z of the ring at the two own cells (ox, ox + 1) of the middle plane (0 at non-fluid cells)
+__device__ __forceinline__ void ring_kmul2(const real* const (&P)[5], const uint8_t* const (&M)[3], int h, real& q0,
+                                           real& q1) {
+    const int m2 = *reinterpret_cast<const uint16_t*>(M[1] + h);
+    const bool interior = (m2 & (INTERIOR | (INTERIOR << 8))) == (INTERIOR | (INTERIOR << 8));
+    if (__all(interior)) {
+        q0 = kmul_ring<true>(P, M, h);
+        q1 = kmul_ring<true>(P, M, h + 1);
+    } else {
+        q0 = (m2 & 255) ? kmul_ring<false>(P, M, h) : RL(0.0);
+        q1 = (m2 >> 8) ? kmul_ring<false>(P, M, h + 1) : RL(0.0);
+    }
+}
+
+// LEARN (the learned flexible CG): p = (z + P c) + beta p_old, and at k = 1 (!UPD) p = z + P c is
+// written to p_out, which then holds the first search direction.
+template <bool UPD, bool LEARN>
 __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restrict__ z, const real* __restrict__ p_in,
                                                      real* __restrict__ p_out, real* __restrict__ phi,
                                                      const uint8_t* __restrict__ fm, real* __restrict__ q,
                                                      double* __restrict__ part, DevState* __restrict__ st, int k,
-                                                     int fin, int nz) {
+                                                     int fin, int nz, const real* __restrict__ cz) {
     __shared__ __align__(16) real sp[RING][PS];
     __shared__ __align__(16) uint8_t sm[RING][PS];
     __shared__ double sh[ZT / 64];
@@ -506,16 +538,16 @@ __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restri
     PlaneRegs pre;
     // planes z0 - 2 .. z0 + 1 into ring slots 0 .. 3 (plane zp lives in slot (zp - z0 + 2) % RING)
     for (int j = 0; j < 4; ++j) {
-        plane_load<UPD>(pre, p_in, z, fm, N, x0, y0, z0 - 2 + j);
-        plane_stage<UPD>(pre, sp[j], sm[j], b, N, x0, y0, z0 - 2 + j);
+        plane_load<UPD, LEARN>(pre, p_in, z, cz, fm, N, x0, y0, z0 - 2 + j);
+        plane_stage<UPD, LEARN>(pre, sp[j], sm[j], b, N, x0, y0, z0 - 2 + j);
     }
-    plane_load<UPD>(pre, p_in, z, fm, N, x0, y0, z0 + 2);
+    plane_load<UPD, LEARN>(pre, p_in, z, cz, fm, N, x0, y0, z0 + 2);
     double s = 0.0;
     for (int zc = z0; zc < z1; ++zc) {
         const int base = zc - z0;
-        plane_stage<UPD>(pre, sp[(base + 4) % RING], sm[(base + 4) % RING], b, N, x0, y0, zc + 2);
+        plane_stage<UPD, LEARN>(pre, sp[(base + 4) % RING], sm[(base + 4) % RING], b, N, x0, y0, zc + 2);
         __syncthreads();
-        if (zc + 3 <= z1 + 1) plane_load<UPD>(pre, p_in, z, fm, N, x0, y0, zc + 3);
+        if (zc + 3 <= z1 + 1) plane_load<UPD, LEARN>(pre, p_in, z, cz, fm, N, x0, y0, zc + 3);
         const int oi = (min(zc, N - 1) * N + min(oy, N - 1)) * N + min(ox, N - 2);
         real2 po, ph;
         if (UPD) {
@@ -525,24 +557,14 @@ __global__ void __launch_bounds__(ZT) k_upd_matvec_z(int N, const real* __restri
         const real* P[5] = {sp[base % RING], sp[(base + 1) % RING], sp[(base + 2) % RING], sp[(base + 3) % RING],
                             sp[(base + 4) % RING]};
         const uint8_t* M[3] = {sm[(base + 1) % RING], sm[(base + 2) % RING], sm[(base + 3) % RING]};
-        const int m2 = *reinterpret_cast<const uint16_t*>(M[1] + h);
-        const bool interior = (m2 & (INTERIOR | (INTERIOR << 8))) == (INTERIOR | (INTERIOR << 8));
         real q0, q1;
-        if (__all(interior)) {
-            q0 = kmul_ring<true>(P, M, h);
-            q1 = kmul_ring<true>(P, M, h + 1);
-        } else {
-            q0 = (m2 & 255) ? kmul_ring<false>(P, M, h) : RL(0.0);
-            q1 = (m2 >> 8) ? kmul_ring<false>(P, M, h + 1) : RL(0.0);
-        }
+        ring_kmul2(P, M, h, q0, q1);
         const real pn0 = P[2][h], pn1 = P[2][h + 1];
         s += (double)q0 * (double)pn0;
         s += (double)q1 * (double)pn1;
         if (own) {
-            if (UPD) {
-                st2(phi, oi, ph.x + a * po.x, ph.y + a * po.y);
-                st2(p_out, oi, pn0, pn1);
-            }
+            if (UPD) st2(phi, oi, ph.x + a * po.x, ph.y + a * po.y);
+            if (UPD || LEARN) st2(p_out, oi, pn0, pn1);
             st2(q, oi, q0, q1);
         }
     }
@@ -818,10 +840,16 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
 // the same test on the host): above the relative tolerance, the next gated run restarts CG from
 // the true residual with phi as its warm start; otherwise every gated kernel after it is a
 // no-op (live = 0, stop_at = -1, which also makes that run's phi update a no-op).
-__global__ void k_restart_gate(const float* __restrict__ prm, DevState* __restrict__ st) {
+// learned: the run before was the learned flexible CG, which also restarts (the MGPCG fallback)
+// after a stall or breakdown, or when it stopped unconverged at its iteration limit.
+__global__ void k_restart_gate(const float* __restrict__ prm, DevState* __restrict__ st, int learned) {
     if (threadIdx.x != 0) return;
     const double rel_tol = (double)prm[P_REL_TOL];
-    const bool done = !(rel_tol > 0.0) || sqrt(st->s[S_TRUE_RR]) <= rel_tol * sqrt(st->s[S_BB]);
+    bool done = !(rel_tol > 0.0) || sqrt(st->s[S_TRUE_RR]) <= rel_tol * sqrt(st->s[S_BB]);
+    if (learned) {
+        if (!st->conv) st->fallback = 1;   // stopped unconverged at the learned limit or budget
+        done = done && !st->fallback;
+    }
     if (done) {
         st->live = 0;
         st->stop_at = -1;
@@ -967,8 +995,8 @@ void Pressure::rhs(const real* vel, const float* prm, bool warm, hipStream_t s, 
     hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FIN_THREADS), 0, s, part, nblocks(g), prm, st, gt);
 }
 
-void Pressure::restart_gate(const float* prm, hipStream_t s) {
-    hipLaunchKernelGGL(k_restart_gate, dim3(1), dim3(64), 0, s, prm, st);
+void Pressure::restart_gate(const float* prm, hipStream_t s, bool learned) {
+    hipLaunchKernelGGL(k_restart_gate, dim3(1), dim3(64), 0, s, prm, st, learned ? 1 : 0);
 }
 
 void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int hitbit) {
@@ -977,11 +1005,11 @@ void Pressure::iteration(int k, const float* prm, hipStream_t s, int kmax, int h
         const unsigned zb = zmarch_blocks(N);
         const int fold = (int)zb <= FOLD_MAX_BLOCKS ? 1 : 0;
         if (k >= 2)
-            hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zb), dim3(ZT), 0, s, N, z, P[(k - 2) & 1], P[(k - 1) & 1], phi,
-                               fm, q, part_pq, st, k, fold, zmarch_planes(N));
+            hipLaunchKernelGGL((k_upd_matvec_z<true, false>), dim3(zb), dim3(ZT), 0, s, N, z, P[(k - 2) & 1], P[(k - 1) & 1],
+                               phi, fm, q, part_pq, st, k, fold, zmarch_planes(N), nullptr);
         else
-            hipLaunchKernelGGL(k_upd_matvec_z<false>, dim3(zb), dim3(ZT), 0, s, N, z, P[0], P[0], phi, fm, q, part_pq,
-                               st, k, fold, zmarch_planes(N));
+            hipLaunchKernelGGL((k_upd_matvec_z<false, false>), dim3(zb), dim3(ZT), 0, s, N, z, P[0], P[0], phi, fm, q,
+                               part_pq, st, k, fold, zmarch_planes(N), nullptr);
         if (!fold) hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, part_pq, (int)zb, st, k);
     }
     if (k >= kmax) {
@@ -1017,8 +1045,8 @@ bool Pressure::launch_single(int which, real* scratch, hipStream_t s) {
     const size_t n3 = (size_t)N * N * N;
     switch (which) {
         case 0:
-            hipLaunchKernelGGL(k_upd_matvec_z<true>, dim3(zmarch_blocks(N)), dim3(ZT), 0, s, N, z, P[0], scratch,
-                               scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2, 0, zmarch_planes(N));
+            hipLaunchKernelGGL((k_upd_matvec_z<true, false>), dim3(zmarch_blocks(N)), dim3(ZT), 0, s, N, z, P[0], scratch,
+                               scratch + n3, fm, scratch + 2 * n3, part_pq, st, 2, 0, zmarch_planes(N), nullptr);
             return true;
         case 1:
             with_mg_type(mg_kind, [&](auto tag) {
@@ -1051,6 +1079,220 @@ void launch_kmul(const Geom& g, const real* p, const uint8_t* fm, real* out, hip
 
 #if !VFX_F64
 namespace vfx {
+
+// ---------------------------------------------------------------- learned flexible CG on the device
+// The coarse-correction preconditioner z = V(r) + P c, c = net(R (r - K V(r))) (R: 4^3 block mean,
+// P: constant over a block's fluid cells), sequenced on the device like MGPCG.  Iteration k runs
+// the MGPCG iteration's kernels up to the V-cycle's z_V = V(r_k) (k_inv_post without its
+// finaliser), then k_resid_pool, the network (gated on the same stop state) and k_fcg_fin, which
+// forms the Polak-Ribiere beta = z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1} (Golub & Ye 1999) of the
+// full z_k without writing it: z_k.r = z_V.r + sum_B c_B sum_{fluid i in B} r_i, and the same for
+// r_{k-1}.  The next iteration's k_upd_matvec_z<.., true> forms p = (z_V + P c) + beta p_old.
+
+// t = r - K z_V at fluid cells (0 elsewhere) as 4^3 block means (the network input), the block sums
+// of r and r_prev over fluid cells, the partials z_V.r_prev and sum of squared block means (the
+// input's scale).  A z-marching column like k_upd_matvec_z; nz is a multiple of 4, so every
+// 4^3 block lies in one column: each thread sums its x pair over 4 planes, and after the next
+// plane's barrier 32 threads add the 8 pair sums of each of the column's 16 x 2 blocks.
+constexpr int PBX = ZX / 4, PBY = ZY / 4;
+__global__ void __launch_bounds__(ZT) k_resid_pool(int N, const real* __restrict__ zv, const real* __restrict__ r,
+                                                   const real* __restrict__ rp, const uint8_t* __restrict__ fm,
+                                                   real* __restrict__ cin, real* __restrict__ sr, real* __restrict__ srp,
+                                                   double* __restrict__ part, DevState* __restrict__ st, int k, int nz) {
+    __shared__ __align__(16) real sp[RING][PS];
+    __shared__ __align__(16) uint8_t sm[RING][PS];
+    __shared__ real sacc[3][ZY][ZX / 2];
+    __shared__ double sh[ZT / 64];
+    if (cg_inactive(st, k)) return;
+    const int M = N >> 2;
+    const int cx = (N + ZX - 1) / ZX, cy = (N + ZY - 1) / ZY;
+    const int lin = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int x0 = (lin % cx) * ZX, y0 = ((lin / cx) % cy) * ZY, z0 = (lin / (cx * cy)) * nz;
+    const int z1 = min(z0 + nz, N);
+    const int t = (int)threadIdx.x;
+    const int lx = 2 * (t % (ZX / 2)), ly = t / (ZX / 2);
+    const int ox = x0 + lx, oy = y0 + ly;
+    const bool own = ox < N && oy < N;
+    const int h = (ly + 2) * PW + lx + 2;
+    PlaneRegs pre;
+    for (int j = 0; j < 4; ++j) {
+        plane_load<false, false>(pre, zv, nullptr, nullptr, fm, N, x0, y0, z0 - 2 + j);
+        plane_stage<false, false>(pre, sp[j], sm[j], RL(0.0), N, x0, y0, z0 - 2 + j);
+    }
+    plane_load<false, false>(pre, zv, nullptr, nullptr, fm, N, x0, y0, z0 + 2);
+    // the blocks of the plane group that starts at zg: written by threads 0..31
+    double dot = 0.0, sq = 0.0;
+    auto flush = [&](int zg) {
+        if (t < PBX * PBY) {
+            const int bx = t % PBX, by = t / PBX;
+            real a[3] = {RL(0.0), RL(0.0), RL(0.0)};
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) a[q] = a[q] + (sacc[q][4 * by + rr][2 * bx] + sacc[q][4 * by + rr][2 * bx + 1]);
+            const int BX = (x0 >> 2) + bx, BY = (y0 >> 2) + by;
+            if (BX < M && BY < M) {
+                const int B = ((zg >> 2) * M + BY) * M + BX;
+                const real mean = a[0] * RL(0.015625);
+                cin[B] = mean;
+                sr[B] = a[1];
+                srp[B] = a[2];
+                sq += (double)mean * (double)mean;
+            }
+        }
+    };
+    real at = RL(0.0), ar = RL(0.0), arp = RL(0.0);
+    for (int zc = z0; zc < z1; ++zc) {
+        const int base = zc - z0;
+        plane_stage<false, false>(pre, sp[(base + 4) % RING], sm[(base + 4) % RING], RL(0.0), N, x0, y0, zc + 2);
+        __syncthreads();
+        if (base > 0 && (base & 3) == 0) flush(zc - 4);
+        if (zc + 3 <= z1 + 1) plane_load<false, false>(pre, zv, nullptr, nullptr, fm, N, x0, y0, zc + 3);
+        const int oi = (min(zc, N - 1) * N + min(oy, N - 1)) * N + min(ox, N - 2);
+        const real2 rv = ld2(r, oi), pv = ld2(rp, oi);
+        const real* P[5] = {sp[base % RING], sp[(base + 1) % RING], sp[(base + 2) % RING], sp[(base + 3) % RING],
+                            sp[(base + 4) % RING]};
+        const uint8_t* Mk[3] = {sm[(base + 1) % RING], sm[(base + 2) % RING], sm[(base + 3) % RING]};
+        real q0, q1;
+        ring_kmul2(P, Mk, h, q0, q1);
+        const int m2 = *reinterpret_cast<const uint16_t*>(Mk[1] + h);
+        const bool f0 = own && (m2 & 255) != 0, f1 = own && (m2 >> 8) != 0;
+        at = at + ((f0 ? rv.x - q0 : RL(0.0)) + (f1 ? rv.y - q1 : RL(0.0)));
+        ar = ar + ((f0 ? rv.x : RL(0.0)) + (f1 ? rv.y : RL(0.0)));
+        arp = arp + ((f0 ? pv.x : RL(0.0)) + (f1 ? pv.y : RL(0.0)));
+        dot += f0 ? (double)P[2][h] * (double)pv.x : 0.0;
+        dot += f1 ? (double)P[2][h + 1] * (double)pv.y : 0.0;
+        if ((base & 3) == 3) {
+            sacc[0][ly][lx >> 1] = at;
+            sacc[1][ly][lx >> 1] = ar;
+            sacc[2][ly][lx >> 1] = arp;
+            at = ar = arp = RL(0.0);
+        }
+    }
+    __syncthreads();
+    flush(z1 - 4);
+    const double d = block_sum(dot, sh);
+    const double q2 = block_sum(sq, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = d;
+        part[gridDim.x + blockIdx.x] = q2;
+    }
+}
+
+// the network input's scale from k_resid_pool's partials: scal[0] = 1 / rms, scal[1] = rms of the
+// M^3 block means (both 1 for a zero input)
+__global__ void __launch_bounds__(FIN_THREADS) k_lrms_fin(const double* __restrict__ part, int nb, int m3,
+                                                  double* __restrict__ scal, const DevState* __restrict__ st, int k) {
+    __shared__ double sh[FIN_THREADS / 64];
+    if (cg_inactive(st, k)) return;
+    const double s = reduce_parts_sum(part + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(s / (double)m3);
+        scal[0] = rms > 0.0 ? 1.0 / rms : 1.0;
+        scal[1] = rms > 0.0 ? rms : 1.0;
+    }
+}
+
+// c_B = rms * out_B on blocks with a fluid cell (0 elsewhere) -> cz; partials c.sr and c.srp; the last
+// workgroup: r.z and z.r_prev of the full z, beta, the scalars of the iteration and the flexible
+// CG's own stop tests: a breakdown (r.z not positive or not finite) or a stall (||r|| fell less
+// than 2x over 3 iterations) stops the run with DevState::fallback set, and the gated MGPCG restart
+// run after the true residual finishes the solve from the current phi.
+constexpr int FCGF_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_fcg_fin(int m3, const float* __restrict__ out, const double* __restrict__ scal,
+                                                 const uint8_t* __restrict__ cmask, const real* __restrict__ sr,
+                                                 const real* __restrict__ srp, real* __restrict__ cz,
+                                                 const double* __restrict__ pinv, int nparts,
+                                                 const double* __restrict__ prp, int nrp, double* __restrict__ fpart,
+                                                 DevState* __restrict__ st, int k) {
+    __shared__ double sh[256 / 64];
+    if (cg_inactive(st, k)) return;
+    const float rms = (float)scal[1];
+    double a = 0.0, b = 0.0;
+    for (int B = (int)(blockIdx.x * blockDim.x + threadIdx.x); B < m3; B += (int)(gridDim.x * blockDim.x)) {
+        const real c = cmask[B] ? (real)(out[B] * rms) : RL(0.0);
+        cz[B] = c;
+        a += (double)c * (double)sr[B];
+        b += (double)c * (double)srp[B];
+    }
+    const double ba = block_sum(a, sh), bb = block_sum(b, sh);
+    const int G = (int)gridDim.x;
+    if (threadIdx.x == 0) {
+        put_part(fpart + blockIdx.x, ba);
+        put_part(fpart + G + blockIdx.x, bb);
+    }
+    if (!last_block(st)) return;
+    double rzv = 0.0, mrv = 0.0, rrv = 0.0, zpv = 0.0, crz = 0.0, czp = 0.0;
+    each_part<3>(pinv, nparts, {0, nparts, 2 * nparts}, [&](const double (&x)[3]) {
+        rzv += x[0];
+        mrv = fmax(mrv, x[1]);
+        rrv += x[2];
+    });
+    each_part<1>(prp, nrp, {0}, [&](const double (&x)[1]) { zpv += x[0]; });
+    each_part<2>(fpart, G, {0, G}, [&](const double (&x)[2]) {
+        crz += x[0];
+        czp += x[1];
+    });
+    const double rz_v = block_sum(rzv, sh), mr = block_max(mrv, sh), rr = block_sum(rrv, sh);
+    const double zrp_v = block_sum(zpv, sh), c_rz = block_sum(crz, sh), c_zrp = block_sum(czp, sh);
+    if (threadIdx.x != 0) return;
+    const double rz = rz_v + c_rz, zrp = zrp_v + c_zrp, rn = sqrt(rr);
+    st->it = k;
+    st->s[S_MAXR] = mr;
+    st->s[S_RR] = rr;
+    st->s[S_LSCALE] = pow2_scale(mr);
+    st->s[S_BEST] = k == 0 ? mr : fmin(st->s[S_BEST], mr);
+    const bool bad = !(rz > 0.0) || !isfinite(rz) || !isfinite(rn);
+    const bool stall = k >= 3 && rn > 0.5 * st->s[S_FCG_H0 + k % 3];
+    st->s[S_FCG_H0 + k % 3] = rn;
+    st->s[S_BETA] = k == 0 ? 0.0 : (rz - zrp) / st->s[S_RZ];
+    st->s[S_RZ] = rz;
+    st->conv = 0;
+    if (bad || stall) {
+        st->stop_at = k;
+        st->fallback = 1;
+    }
+}
+
+// planes per column of k_resid_pool: a multiple of 4
+static int rpool_planes(int N) { return std::max(4, zmarch_planes(N)); }
+
+unsigned resid_pool_blocks(int N) {
+    const int nz = rpool_planes(N);
+    return (unsigned)(((N + ZX - 1) / ZX) * ((N + ZY - 1) / ZY) * ((N + nz - 1) / nz));
+}
+
+void launch_upd_matvec_learned(const Pressure& P, int k, const real* cz, hipStream_t s) {
+    const int N = P.N;
+    const unsigned zb = zmarch_blocks(N);
+    const int fold = (int)zb <= FOLD_MAX_BLOCKS ? 1 : 0;
+    if (k >= 2)
+        hipLaunchKernelGGL((k_upd_matvec_z<true, true>), dim3(zb), dim3(ZT), 0, s, N, P.z, P.P[(k - 2) & 1],
+                           P.P[(k - 1) & 1], P.phi, P.fm, P.q, P.part_pq, P.st, k, fold, zmarch_planes(N), cz);
+    else
+        hipLaunchKernelGGL((k_upd_matvec_z<false, true>), dim3(zb), dim3(ZT), 0, s, N, P.z, P.P[0], P.P[0], P.phi, P.fm,
+                           P.q, P.part_pq, P.st, k, fold, zmarch_planes(N), cz);
+    if (!fold) hipLaunchKernelGGL(k_fin_pq, dim3(1), dim3(FIN_THREADS), 0, s, P.part_pq, (int)zb, P.st, k);
+}
+
+void launch_resid_pool(const Pressure& P, int k, real* cin, real* sr, real* srp, double* lpart, double* scal,
+                       hipStream_t s) {
+    const int N = P.N, M = N >> 2;
+    const unsigned nb = resid_pool_blocks(N);
+    const real* r = P.R[k & 1];
+    const real* rp = k > 0 ? P.R[(k - 1) & 1] : r;
+    hipLaunchKernelGGL(k_resid_pool, dim3(nb), dim3(ZT), 0, s, N, P.z, r, rp, P.fm, cin, sr, srp, lpart, P.st, k,
+                       rpool_planes(N));
+    hipLaunchKernelGGL(k_lrms_fin, dim3(1), dim3(FIN_THREADS), 0, s, lpart, (int)nb, M * M * M, scal, P.st, k);
+}
+
+void launch_fcg_fin(const Pressure& P, int k, const float* out, const double* scal, const uint8_t* cmask,
+                    const real* sr, const real* srp, real* cz, const double* lpart, double* fpart, hipStream_t s) {
+    const int M = P.N >> 2, m3 = M * M * M;
+    const int G = std::max(1, std::min(FCGF_BLOCKS, (m3 + 255) / 256));
+    hipLaunchKernelGGL(k_fcg_fin, dim3(G), dim3(256), 0, s, m3, out, scal, cmask, sr, srp, cz, P.part, P.ntiles,
+                       lpart, (int)resid_pool_blocks(P.N), fpart, P.st, k);
+}
 
 // every single-precision bit pattern x: div6(x) against the IEEE quotient x / 6 (two NaN results
 // count as equal); the mismatches are added to *bad

@@ -37,6 +37,7 @@ enum : int { SYNC_DEVICE = 0, SYNC_POLL = 1 };
 struct StepSig {
     const void* data;
     const void* solid;
+    const void* net;
     int v[16];
 };
 
@@ -49,6 +50,7 @@ struct ReadbackInfo {
     int cycles;     // fused: V-cycles that always run
     int judged;     // the true residual was computed
     float rel_tol;
+    int learned;    // the learned flexible CG ran on the device (its restart run is the MGPCG fallback)
 };
 
 struct StepGraph {
@@ -184,6 +186,17 @@ int cg_budget_restart(const vfx_ctx* c, int maxit);
 // tail_budget >= 0: at most that many adaptive tail cycles are enqueued, with no host polling;
 // -1: the host polls the stop flag after each tail cycle
 int run_fused(vfx_ctx* c, real* vel, hipStream_t s, int tail_budget);
+#if !VFX_F64
+// the learned flexible CG on the device (cg.hip): iteration k's p update and matvec with the
+// coarse correction cz; after the V-cycle, the network input (block means into cin, scale into
+// scal) and the block sums of r and r_prev; after the network, beta and the stop tests
+void launch_upd_matvec_learned(const Pressure& P, int k, const real* cz, hipStream_t s);
+unsigned resid_pool_blocks(int N);
+void launch_resid_pool(const Pressure& P, int k, real* cin, real* sr, real* srp, double* lpart, double* scal,
+                       hipStream_t s);
+void launch_fcg_fin(const Pressure& P, int k, const float* out, const double* scal, const uint8_t* cmask,
+                    const real* sr, const real* srp, real* cz, const double* lpart, double* fpart, hipStream_t s);
+#endif
 // the finer scalar grid (hires.hip): the fine stage (advection, mass ratio, sources, diffusion) by
 // the velocity of `data`, and the restriction of the fine scalars into data's density,
 // temperature and fuel channels

@@ -162,6 +162,7 @@ __global__ void k_fstart(DevState* st, int cycles) {
         st->it_total = 0;
         st->restarts = 0;
         st->hit = 0;
+        st->fallback = 0;
     }
 }
 

@@ -160,12 +160,6 @@ __device__ __forceinline__ uint32_t lsto(int m, int K, int J, int I) {
     return lsto(m, lcell(m, K, J, I), (I + J + K) & 1);
 }
 
-// the kernels of CG iteration k do nothing after the run stopped at an earlier iteration, or at k
-// itself on r before the V-cycle
-__device__ __forceinline__ bool cg_inactive(const DevState* st, int k) {
-    return st->stop_at < k || (st->stop_at == k && st->early != 0);
-}
-
 // ---------------------------------------------------------------- LDS tiles of the fine grid
 // A workgroup owns TX x TY x TZ fine cells (whole 2x2x2 blocks) and stages them with a 2-cell
 // halo, the reach of K = d d^T.  Cells outside the box are staged with mask 0 and value 0.

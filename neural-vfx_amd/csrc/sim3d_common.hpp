@@ -68,6 +68,7 @@ enum : int {
     S_MASS_RATIO = 0, S_MAXU = 1, S_RZ = 2, S_ALPHA = 3, S_BETA = 4, S_MAXR = 5, S_THR = 6,
     S_BEST = 7, S_BB = 8, S_RR = 9, S_TRUE_RR = 10, S_LSCALE = 11,
     S_MASS_RATIO_F = 12,   // mass ratio of the finer scalar grid (hires.py, solver.py:250-253)
+    S_FCG_H0 = 13,         // 13..15: ||r|| of the last three learned flexible-CG iterations (slot k % 3)
     S_COUNT = 16,
 };
 
@@ -84,9 +85,18 @@ struct DevState {
     int hit;       // bit 0: the first CG run stopped at its iteration budget unconverged; bit 1: a restart did
     int early;     // 1 when the stop test of iteration stop_at fired on r before that iteration's
                    // V-cycle, whose kernels (and the rest of the iteration) are then no-ops
+    int fallback;  // the learned flexible-CG run of this projection stalled or broke down
     unsigned done;   // workgroups of the current finalising launch that have written their partials
                      // (0 between launches)
 };
+
+// the kernels of CG iteration k do nothing after the run stopped at an earlier iteration, or at k
+// itself on r before the V-cycle
+__device__ __forceinline__ bool cg_inactive(const DevState* st, int k) {
+    return st->stop_at < k || (st->stop_at == k && st->early != 0);
+}
+// kernels that also run outside a CG iteration (the network): gated only when st is given
+__device__ __forceinline__ bool gated_off(const DevState* st, int k) { return st != nullptr && cg_inactive(st, k); }
 
 constexpr int FLUID = 64;          // face-mask fluid bit (pressure.py:32-33)
 constexpr real LAM_CAP = (real)0.16f;   // kernels.py:63 (the reference's float constant)

@@ -190,7 +190,7 @@ struct Pressure {
     void true_residual(const real* vel, hipStream_t s, bool gated = false, bool restart = false);
     // decide on the device whether the gated restart run after it runs (true residual above the
     // relative tolerance)
-    void restart_gate(const float* prm, hipStream_t s);
+    void restart_gate(const float* prm, hipStream_t s, bool learned = false);
     void vcycle_levels(int k, hipStream_t s);
     bool launch_single(int which, real* scratch, hipStream_t s);
     bool launch_single_mg(int which, hipStream_t s);

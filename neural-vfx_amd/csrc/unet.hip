@@ -36,7 +36,9 @@ __global__ void __launch_bounds__(FIN_THREADS) k_rms_fin(const double* __restric
 
 // network input: channel 0 = r / rms(r) (rounded to BF16), channel 1 = fluid indicator, rest 0
 __global__ void __launch_bounds__(256) k_pack_input(size_t n, const real* __restrict__ r, const uint8_t* __restrict__ fm,
-                                                    const double* __restrict__ scal, bf16_t* __restrict__ X) {
+                                                    const double* __restrict__ scal, bf16_t* __restrict__ X,
+                                                    const DevState* __restrict__ gst, int gk) {
+    if (gated_off(gst, gk)) return;
     const size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (v >= n) return;
     const float inv = (float)scal[0];
@@ -204,17 +206,17 @@ static unsigned blocks_for(size_t items, int threads = 256) { return (unsigned)(
 
 template <int CIN, int CT>
 static void conv_t(int N, const bf16_t* in, const bf16_t* w, int kp, const float* bias, int cout, int relu, bf16_t* out,
-                   int ocs, int ooff, float* out32, hipStream_t s) {
+                   int ocs, int ooff, float* out32, hipStream_t s, const DevState* gst, int gk) {
     const size_t n3 = (size_t)N * N * N;
     hipLaunchKernelGGL((k_conv3<CIN, CT>), dim3(blocks_for(n3, 64)), dim3(NET_THREADS), 0, s, N, in, w, kp, bias, cout,
-                       relu, out, ocs, ooff, out32);
+                       relu, out, ocs, ooff, out32, gst, gk);
 }
 
 static bool conv(int cin, int ct, int N, const bf16_t* in, const bf16_t* w, int kp, const float* bias, int cout, int relu,
-                 bf16_t* out, int ocs, int ooff, float* out32, hipStream_t s) {
+                 bf16_t* out, int ocs, int ooff, float* out32, hipStream_t s, const DevState* gst = nullptr, int gk = 0) {
 #define VFX_CONV(CI, CT_)                                                                  \
     if (cin == CI && ct == CT_) {                                                          \
-        conv_t<CI, CT_>(N, in, w, kp, bias, cout, relu, out, ocs, ooff, out32, s);         \
+        conv_t<CI, CT_>(N, in, w, kp, bias, cout, relu, out, ocs, ooff, out32, s, gst, gk); \
         return true;                                                                       \
     }
     VFX_CONV(16, 1) VFX_CONV(16, 2) VFX_CONV(16, 3) VFX_CONV(32, 1) VFX_CONV(32, 2) VFX_CONV(32, 4) VFX_CONV(32, 6)
@@ -295,6 +297,11 @@ int Net::create(int N_) {
     sz.push_back(al(n0 * sizeof(real))); // work
     sz.push_back(al(n0 * sizeof(real))); // work2
     sz.push_back(al(n0));                // cmask
+    for (int i = 0; i < 3; ++i) sz.push_back(al(n0 * sizeof(real)));   // csr csrp ccz
+    const size_t Nf = (size_t)4 * N;     // k_resid_pool's workgroups on the fine grid: at most
+    const size_t rpb = ((Nf + 63) / 64) * ((Nf + 7) / 8) * ((Nf + 3) / 4);   // columns of 4 planes
+    sz.push_back(al(2 * rpb * 8));       // lpart
+    sz.push_back(al(2 * 256 * 8));       // fpart
     size_t total = 0;
     for (size_t b : sz) total += b;
     if (!hip_ok(hipMalloc(&pool, total), "network allocation")) return VFX_E_HIP;
@@ -315,6 +322,11 @@ int Net::create(int N_) {
     work = (float*)take();
     work2 = (float*)take();
     cmask = (uint8_t*)take();
+    csr = (real*)take();
+    csrp = (real*)take();
+    ccz = (real*)take();
+    lpart = (double*)take();
+    fpart = (double*)take();
     adam_t = 0;
     return VFX_OK;
 }
@@ -335,29 +347,29 @@ void Net::pack_weights(hipStream_t s) {
 
 // forward of layer i from `in` (cinp channels) into out (stride ocs, offset ooff) or out32
 static void layer_fwd(const Net& n, int i, int N, const bf16_t* in, bf16_t* out, int ocs, int ooff, float* out32,
-                      hipStream_t s) {
+                      hipStream_t s, const DevState* gst, int gk) {
     const NetLayer& l = n.L[i];
-    conv(l.cinp, l.coutp / 16, N, in, l.wf, l.kp, n.p + l.b_off, l.cout, l.relu, out, ocs, ooff, out32, s);
+    conv(l.cinp, l.coutp / 16, N, in, l.wf, l.kp, n.p + l.b_off, l.cout, l.relu, out, ocs, ooff, out32, s, gst, gk);
 }
 
-void Net::forward(hipStream_t s) {
+void Net::forward(hipStream_t s, const DevState* gst, int gk) {
     const int n1 = N / 2, n2 = N / 4;
     const size_t v0 = (size_t)N * N * N, v1 = v0 / 8;
-    layer_fwd(*this, 0, N, X, A0, 16, 0, nullptr, s);
-    layer_fwd(*this, 1, N, A0, C0, 48, 32, nullptr, s);                 // e0b -> C0[32:48]
-    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 * 16)), dim3(256), 0, s, N, C0, 48, 32, 16, P0);
-    layer_fwd(*this, 2, n1, P0, A1, 32, 0, nullptr, s);
-    layer_fwd(*this, 3, n1, A1, C1, 96, 64, nullptr, s);                // e1b -> C1[64:96]
-    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 / 8 * 32)), dim3(256), 0, s, n1, C1, 96, 64, 32, P1);
-    layer_fwd(*this, 4, n2, P1, A2, 64, 0, nullptr, s);
-    layer_fwd(*this, 5, n2, A2, B2, 64, 0, nullptr, s);
-    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v1 * 64)), dim3(256), 0, s, n2, B2, 64, C1, 96, 0);
-    layer_fwd(*this, 6, n1, C1, D1a, 32, 0, nullptr, s);
-    layer_fwd(*this, 7, n1, D1a, D1b, 32, 0, nullptr, s);
-    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v0 * 32)), dim3(256), 0, s, n1, D1b, 32, C0, 48, 0);
-    layer_fwd(*this, 8, N, C0, D0a, 16, 0, nullptr, s);
-    layer_fwd(*this, 9, N, D0a, D0b, 16, 0, nullptr, s);
-    layer_fwd(*this, 10, N, D0b, nullptr, 16, 0, out, s);
+    layer_fwd(*this, 0, N, X, A0, 16, 0, nullptr, s, gst, gk);
+    layer_fwd(*this, 1, N, A0, C0, 48, 32, nullptr, s, gst, gk);                 // e0b -> C0[32:48]
+    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 * 16)), dim3(256), 0, s, N, C0, 48, 32, 16, P0, gst, gk);
+    layer_fwd(*this, 2, n1, P0, A1, 32, 0, nullptr, s, gst, gk);
+    layer_fwd(*this, 3, n1, A1, C1, 96, 64, nullptr, s, gst, gk);                // e1b -> C1[64:96]
+    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 / 8 * 32)), dim3(256), 0, s, n1, C1, 96, 64, 32, P1, gst, gk);
+    layer_fwd(*this, 4, n2, P1, A2, 64, 0, nullptr, s, gst, gk);
+    layer_fwd(*this, 5, n2, A2, B2, 64, 0, nullptr, s, gst, gk);
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v1 * 64)), dim3(256), 0, s, n2, B2, 64, C1, 96, 0, gst, gk);
+    layer_fwd(*this, 6, n1, C1, D1a, 32, 0, nullptr, s, gst, gk);
+    layer_fwd(*this, 7, n1, D1a, D1b, 32, 0, nullptr, s, gst, gk);
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v0 * 32)), dim3(256), 0, s, n1, D1b, 32, C0, 48, 0, gst, gk);
+    layer_fwd(*this, 8, N, C0, D0a, 16, 0, nullptr, s, gst, gk);
+    layer_fwd(*this, 9, N, D0a, D0b, 16, 0, nullptr, s, gst, gk);
+    layer_fwd(*this, 10, N, D0b, nullptr, 16, 0, out, s, gst, gk);
 }
 
 // weight + bias gradient of layer i from its input `in` and pre-activation gradient `gpre`
@@ -437,7 +449,7 @@ void Net::apply(const real* r, const uint8_t* fm, real* z, hipStream_t s) {
     const size_t n0 = (size_t)N * N * N;
     hipLaunchKernelGGL(k_sq_part, dim3(512), dim3(256), 0, s, n0, r, dpart);
     hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(FIN_THREADS), 0, s, dpart, 512, n0, scal);
-    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, r, fm, scal, X);
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, r, fm, scal, X, nullptr, 0);
     forward(s);
     hipLaunchKernelGGL(k_unpack, dim3(blocks_for(n0)), dim3(256), 0, s, n0, out, fm, scal, z);
 }
@@ -451,9 +463,20 @@ void Net::apply_coarse(int Nf, const real* r, const uint8_t* fm, real* z, hipStr
     hipLaunchKernelGGL(k_mask4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, fm, cmask);
     hipLaunchKernelGGL(k_sq_part, dim3(512), dim3(256), 0, s, n0, work2, dpart);
     hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(FIN_THREADS), 0, s, dpart, 512, n0, scal);
-    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, work2, cmask, scal, X);
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, work2, cmask, scal, X, nullptr, 0);
     forward(s);
     hipLaunchKernelGGL(k_up4<true>, dim3(blocks_for(nf)), dim3(256), 0, s, Nf, out, scal, fm, z, nullptr);
+}
+
+void Net::coarse_mask(int Nf, const uint8_t* fm, hipStream_t s) {
+    const size_t n0 = (size_t)N * N * N;
+    hipLaunchKernelGGL(k_mask4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, fm, cmask);
+}
+
+void Net::coarse_forward(const DevState* st, int k, hipStream_t s) {
+    const size_t n0 = (size_t)N * N * N;
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, work2, cmask, scal, X, st, k);
+    forward(s, st, k);
 }
 
 // One training step of the coarse correction on a fine pair (r_V, e_V) with K e_V = r_V (the
@@ -470,7 +493,7 @@ void net_train_step_coarse(Net& n, const Geom& gf, const real* rv, const real* e
     hipLaunchKernelGGL(k_mask4, dim3(blocks_for(n0)), dim3(256), 0, s, Nf, fm, n.cmask);
     hipLaunchKernelGGL(k_sq_part, dim3(512), dim3(256), 0, s, n0, n.work2, n.dpart);
     hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(FIN_THREADS), 0, s, n.dpart, 512, n0, n.scal);
-    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, n.work2, n.cmask, n.scal, n.X);
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, n.work2, n.cmask, n.scal, n.X, nullptr, 0);
     n.forward(s);
     hipLaunchKernelGGL(k_up4<false>, dim3(blocks_for(nf)), dim3(256), 0, s, Nf, n.out, n.scal, fm, e, ev);
     launch_kmul(gf, e, fm, ke, s);
@@ -487,7 +510,7 @@ void net_train_step(Net& n, const Geom& g, const real* b, const real* phi, const
     const size_t n0 = (size_t)n.N * n.N * n.N;
     hipLaunchKernelGGL(k_sq_part, dim3(512), dim3(256), 0, s, n0, b, n.dpart);
     hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(FIN_THREADS), 0, s, n.dpart, 512, n0, n.scal);
-    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, b, fm, n.scal, n.X);
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(n0)), dim3(256), 0, s, n0, b, fm, n.scal, n.X, nullptr, 0);
     n.forward(s);
     hipLaunchKernelGGL(k_err, dim3(blocks_for(n0)), dim3(256), 0, s, n0, n.out, fm, n.scal, phi, n.work2);
     launch_kmul(g, n.work2, fm, n.work, s);   // K e

@@ -44,13 +44,18 @@ struct Net {
     float* work = nullptr;    // N^3 FP32 scratch (K e, dL/dz)
     float* work2 = nullptr;
     uint8_t* cmask = nullptr; // N^3 fluid flags of the coarse correction's input
+    // the device-sequenced flexible CG of a fine grid 4 N (coarse correction): block sums of r and
+    // of r_prev over fluid cells, the scaled correction c, partials of k_resid_pool and k_fcg_fin
+    real *csr = nullptr, *csrp = nullptr, *ccz = nullptr;
+    double *lpart = nullptr, *fpart = nullptr;
     int adam_t = 0;
 
     int create(int N);
     void destroy();
     void pack_weights(hipStream_t s);
-    // raw network: X (prepared by the caller) -> out
-    void forward(hipStream_t s);
+    // raw network: X (prepared by the caller) -> out; with gst, every kernel is a no-op when
+    // iteration gk of the CG run that DevState gst describes is inactive
+    void forward(hipStream_t s, const DevState* gst = nullptr, int gk = 0);
     // d loss / d out (FP32, N^3, in `work`) -> g (flat parameter gradient)
     void backward(hipStream_t s);
     void adam(float lr, float b1, float b2, float eps, hipStream_t s);
@@ -59,6 +64,11 @@ struct Net {
     // coarse correction of a fine field of Nf = 4 N: z += P net(R r) (R: 4^3 block mean, P:
     // constant over each block's fluid cells)
     void apply_coarse(int Nf, const real* r, const uint8_t* fm, real* z, hipStream_t s);
+    // the device-sequenced flexible CG (fine grid Nf = 4 N): the coarse fluid flags of the face
+    // mask fm, once per projection; then per iteration k the network on the block means in work2
+    // (scaled by scal, written by k_resid_pool and k_lrms_fin) into out, gated on DevState st
+    void coarse_mask(int Nf, const uint8_t* fm, hipStream_t s);
+    void coarse_forward(const DevState* st, int k, hipStream_t s);
 };
 
 // one Adam step of the network on a (b, phi) pair in the energy norm of K (mask fm, geometry g of

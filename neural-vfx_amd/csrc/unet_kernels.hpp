@@ -37,7 +37,9 @@ __device__ __forceinline__ void tap_off(int t, int& dz, int& dy, int& dx) {
 template <int CIN, int CT>
 __global__ void __launch_bounds__(NET_THREADS)
 k_conv3(int N, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, int kp, const float* __restrict__ bias,
-        int cout, int relu, bf16_t* __restrict__ out, int ocs, int ooff, float* __restrict__ out32) {
+        int cout, int relu, bf16_t* __restrict__ out, int ocs, int ooff, float* __restrict__ out32,
+        const DevState* __restrict__ gst, int gk) {
+    if (gated_off(gst, gk)) return;
     const int n3 = N * N * N;
     const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
     const int v0 = ((int)blockIdx.x * (NET_THREADS / 64) + wave) * 16;
@@ -205,7 +207,9 @@ k_pack_weights(const float* __restrict__ w, int cin, int cout, int cinp, int cou
 
 // 2x2x2 average pool of channels [off, off + c) of src (stride cs) at fine size n -> dst (c channels)
 __global__ void __launch_bounds__(NET_THREADS)
-k_pool(int n, const bf16_t* __restrict__ src, int cs, int off, int c, bf16_t* __restrict__ dst) {
+k_pool(int n, const bf16_t* __restrict__ src, int cs, int off, int c, bf16_t* __restrict__ dst,
+       const DevState* __restrict__ gst, int gk) {
+    if (gated_off(gst, gk)) return;
     const int m = n / 2;
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= (size_t)m * m * m * c) return;
@@ -223,7 +227,9 @@ k_pool(int n, const bf16_t* __restrict__ src, int cs, int off, int c, bf16_t* __
 
 // nearest 2x upsample of src (c channels, coarse size m) into channels [off, off + c) of dst (stride cs)
 __global__ void __launch_bounds__(NET_THREADS)
-k_upsample(int m, const bf16_t* __restrict__ src, int c, bf16_t* __restrict__ dst, int cs, int off) {
+k_upsample(int m, const bf16_t* __restrict__ src, int c, bf16_t* __restrict__ dst, int cs, int off,
+           const DevState* __restrict__ gst, int gk) {
+    if (gated_off(gst, gk)) return;
     const int n = 2 * m;
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= (size_t)n * n * n * c) return;

@@ -128,11 +128,12 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _project(N, st, cfg, net=None, mode=0, maxit=None, sweeps=0):
-    """sweeps: damped-Jacobi corrections after each network application (0 = the bare network)."""
-    """Cold projection of a copy of st (relative residual 1e-5); returns (stats, projected data)."""
+def _project(N, st, cfg, net=None, mode=0, maxit=None, sweeps=0, sync="device", reps=1):
+    """Cold projection of a copy of st (relative residual 1e-5), `reps` times (the last is
+    returned); returns (stats, projected data).  sweeps: damped-Jacobi corrections after each
+    application of a full-resolution network (0 = the bare network)."""
     cfg = dataclasses.replace(cfg, pressure_warm_start=False, pressure_rel_tol=1e-5, pressure_max_iters=400,
-                              pressure_restarts=2)
+                              pressure_restarts=2, pressure_sync=sync)
     s = Solver3D(N, cfg)
     s.set_option(_vl.OPT_TRUE_RESIDUAL, 1)
     if mode:
@@ -142,11 +143,12 @@ def _project(N, st, cfg, net=None, mode=0, maxit=None, sweeps=0):
     s.set_option(_vl.OPT_LEARNED_SWEEPS, sweeps)
     if net is not None:
         s.set_preconditioner(net)
-    data = st.data.clone()
     from vfx.core.fields3d import FieldState3D
-    work = FieldState3D(data=data, solid=st.solid.clone(), meta=st.meta)
-    s.project(work)
-    ps = s.pressure_stats()
+    for _ in range(reps):
+        data = st.data.clone()
+        work = FieldState3D(data=data, solid=st.solid.clone(), meta=st.meta)
+        s.project(work)
+        ps = s.pressure_stats()
     ps["history"] = [float(f"{h / max(ps['b_norm'], 1e-300):.2e}") for h in s.fcg_history()]
     return ps, data
 
@@ -225,3 +227,85 @@ def test_trained_network_gains(cuda, hiplib):
     assert after["history"][kb] < before["history"][kb] or before["fallback"] == 1
     # with the Jacobi corrections (the library default) the network alone reaches 1e-5
     assert hybrid["fallback"] == 0 and hybrid["true_rel_res"] <= 1e-5
+
+
+def _coarse_net(N, iters=300):
+    """The V-cycle's coarse correction (a U-Net on the (N/4)^3 grid) trained briefly at N^3 on what
+    the V-cycle leaves of solver pairs of seeded scenes 20-23 and noise pairs."""
+    from vfx.sim3d.learned import train_coarse, vcycle_pairs
+    pairs = generate_pairs(N, seeds=range(20, 24), steps=10, every=1)
+    pairs += noise_pairs(N, [p[0] for p in pairs[::11]], count=len(pairs) // 2, seed=5)
+    net = UNet(N // 4, seed=1)
+    losses = train_coarse(net, vcycle_pairs(N, pairs), iters=iters, lr=2e-3, seed=0)
+    _say(f"  coarse correction at {N}^3: loss {np.mean(losses[:30]):.3f} -> {np.mean(losses[-30:]):.3f}")
+    return net
+
+
+@pytest.mark.parametrize("N", [64, 96])
+def test_learned_device_matches_polled(cuda, hiplib, N):
+    """The coarse correction's flexible CG sequenced on the device (no host synchronisation; the
+    full z = z_V + P c is never stored, its dots come from block sums) takes the iterations of the
+    host-polled flexible CG, which stores z and reduces its dots directly, reaches the same
+    projected velocity, and both meet the true relative residual 1e-5 without fallback."""
+    net = _coarse_net(N)
+    st, cfg = _pre_projection_state(N, 97)
+    polled, a = _project(N, st, cfg, net=net, sync="poll")
+    dev, b = _project(N, st, cfg, net=net, sync="device", reps=2)
+    mg, _ = _project(N, st, cfg)
+    _say(f"  {N}^3: polled {polled['learned_iters']} its (fallback {polled['fallback']}, true rel-res "
+         f"{polled['true_rel_res']:.2e}), device {dev['learned_iters']} its (fallback {dev['fallback']}, "
+         f"{dev['iters']} MGPCG, true rel-res {dev['true_rel_res']:.2e}), MGPCG {mg['iters']}")
+    assert polled["fallback"] == 0 and dev["fallback"] == 0
+    assert abs(dev["learned_iters"] - polled["learned_iters"]) <= 1
+    assert polled["true_rel_res"] <= 1e-5 and dev["true_rel_res"] <= 1e-5 and dev["converged"]
+    v = a[1:4].cpu().numpy()
+    e = float(np.abs(b[1:4].cpu().numpy() - v).max() / np.abs(v).max())
+    _say(f"  velocity max rel difference {e:.2e}")
+    assert e <= 1e-3
+
+
+@pytest.mark.parametrize("bad", ["nan", "limit"])
+def test_learned_device_fallback(cuda, hiplib, bad):
+    """On the device, a network that breaks the flexible CG (NaN weights) or a learned run cut at
+    one iteration hands the solve to the gated MGPCG run, which converges from the current phi."""
+    N = 64
+    p = init_params(0)
+    if bad == "nan":
+        p[:] = np.nan   # (a ReLU layer maps NaN to 0; the last layer's NaN bias reaches the output)
+    net = UNet(N // 4, params=p)
+    st, cfg = _pre_projection_state(N, 12, steps=4)
+    ps, data = _project(N, st, cfg, net=net, maxit=1 if bad == "limit" else None, reps=3)
+    _say(f"  {bad}: fallback {ps['fallback']}, learned iterations {ps['learned_iters']}, MGPCG {ps['iters']}, "
+         f"true rel-res {ps['true_rel_res']:.2e}, budget stops {ps['budget_stops']}")
+    assert ps["fallback"] == 1 and ps["iters"] > 0
+    assert 0 <= ps["true_rel_res"] <= 1e-5 and ps["converged"]
+    assert torch.isfinite(data).all()
+
+
+def test_learned_step_graph_replay(cuda, hiplib):
+    """Steps with the coarse correction on the device replay as captured step graphs and equal the
+    same steps enqueued without graphs bit for bit; step() returns before the GPU has finished."""
+    from vfx.sim3d import Solver3D
+    N = 64
+    net = _coarse_net(N, iters=100)
+    forces, obst, beta = random_scene(96)
+    cfg = dataclasses.replace(scene_config(beta, 96), pressure_rel_tol=1e-5, pressure_restarts=1)
+    runs = []
+    for graph in (True, False):
+        Solver3D.STEP_GRAPH = graph
+        try:
+            s = Solver3D(N, cfg)
+            s.set_preconditioner(net)
+            st = s.reset(obst)
+            for _ in range(12):
+                s.step(st, forces, 1.0 / 60.0)
+            ps = s.pressure_stats()
+        finally:
+            Solver3D.STEP_GRAPH = True
+        _say(f"  graph {graph}: learned its {ps['learned_iters']}, fallback {ps['fallback']}, captures "
+             f"{ps['graph_captures']}, replays {ps['graph_replays']}, unconverged {ps['unconverged']}")
+        assert ps["converged"]
+        if graph:
+            assert ps["graph_replays"] > 0
+        runs.append(st.data.clone())
+    assert torch.equal(runs[0], runs[1])
