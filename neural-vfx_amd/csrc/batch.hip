@@ -1,8 +1,9 @@
 // Batched frames: many independent simulations of any sizes stepped as one frame.
 //
-// A frame enqueues every simulation's step (advection, forces, MGPCG projection) on one of a few
-// HIP streams forked from the caller's stream and joined back into it, either directly or as one
-// captured hipGraph that is replayed frame after frame.  Every projection is sequenced on the
+// A frame enqueues every simulation's step (advection, forces, MGPCG projection, or the learned
+// flexible CG with its MGPCG fallback for a simulation with a coarse-correction network) on one of
+// a few HIP streams forked from the caller's stream and joined back into it, either directly or as
+// one captured hipGraph that is replayed frame after frame.  Every projection is sequenced on the
 // device with a PCG iteration budget per simulation (the kernels after the device's stop test are
 // no-ops, the device stops by itself at the budget), so a frame never waits on the host.  Each
 // simulation's last kernel is followed by a timestamp (the constant-rate wall clock of the GPU),
@@ -10,8 +11,8 @@
 // every simulation's PCG state into a pinned buffer; vfx_batch_wait reads it.  A budget starts at
 // min(P_MAXIT, 32); after each waited frame it is at least the largest iteration count of the
 // simulation's last 8 frames plus 2, and a simulation that stopped at its budget unconverged gets
-// twice the budget for the next frame.  Budgets grow at once; once 8 frames have been seen they
-// also shrink to that count when they sit 2 or more above it (iterations past the stop are
+// twice the budget for the next frame (a restart run that stopped at its budget: twice its own).
+// Budgets grow at once; once 8 frames have been seen they also shrink to that count when they sit 2 or more above it (iterations past the stop are
 // launched and return at once, so a loose budget costs launches).
 // The autotuner picks, per grid size, the cells per thread of the cell kernels and the size from
 // which the multigrid coarse levels run in one workgroup, and the number of streams, by timing
@@ -109,6 +110,7 @@ struct vfx_batch {
     std::vector<float*> g_data;
     std::vector<const float*> g_solid;
     std::vector<char> g_warm;   // warm-start flags the capture was made with (they pick kernel variants)
+    std::vector<const vfx_net*> g_net;   // each simulation's learned preconditioner at capture
     bool dirty = true;
     bool frames = false;   // a frame has been enqueued
 };
@@ -228,7 +230,11 @@ int capture(vfx_batch* b, float* const* data, const float* const* solid) {
     b->g_data.assign(data, data + b->n);
     b->g_solid.assign(solid, solid + b->n);
     b->g_warm.resize(b->n);
-    for (int i = 0; i < b->n; ++i) b->g_warm[i] = b->sims[i]->prm_last[P_WARM] > 0.5f;
+    b->g_net.resize(b->n);
+    for (int i = 0; i < b->n; ++i) {
+        b->g_warm[i] = b->sims[i]->prm_last[P_WARM] > 0.5f;
+        b->g_net[i] = b->sims[i]->net;
+    }
     b->dirty = false;
     return VFX_OK;
 }
@@ -242,7 +248,8 @@ int frame(vfx_batch* b, float* const* data, const float* const* solid, const flo
         // a forced operator rebuild is a one-off: it runs eagerly and stays out of the graph
         bool same = !b->dirty && b->exec && std::equal(data, data + b->n, b->g_data.begin()) &&
                     std::equal(solid, solid + b->n, b->g_solid.begin());
-        for (int i = 0; same && i < b->n; ++i) same = b->g_warm[i] == (b->sims[i]->prm_last[P_WARM] > 0.5f);
+        for (int i = 0; same && i < b->n; ++i)
+            same = b->g_warm[i] == (b->sims[i]->prm_last[P_WARM] > 0.5f) && b->g_net[i] == b->sims[i]->net;
         if (!same)
             if (int rc = capture(b, data, solid)) return rc;
         if (!hip_ok(hipGraphLaunch(b->exec, main), "frame launch")) return VFX_E_HIP;
@@ -296,6 +303,16 @@ void absorb_frame(vfx_batch* b) {
         // keep the start budget while their counts climb)
         if (want > b->budget[i] || (w.n == BUDGET_WINDOW && want + BUDGET_SHRINK <= b->budget[i])) {
             b->budget[i] = want;
+            b->dirty = true;
+        }
+        // the restart run stopped at its budget: the context's restart budget doubles (after a
+        // learned run it is the MGPCG fallback, a whole solve: at least the first run's budget)
+        if (hit & 2) {
+            vfx_ctx* c = b->sims[i];
+            const int br = cg_budget_restart(c, cap);
+            int nb = 2 * br;
+            if (c->net) nb = std::max(nb, std::max(b->budget[i], 16));
+            c->bw_r.budget = std::min(nb, cap);
             b->dirty = true;
         }
     }
@@ -524,8 +541,9 @@ int vfx_batch_step(vfx_batch* b, float* const* data, const float* const* solid, 
         if (!data[i] || !solid[i]) return arg_error("null field pointer");
         if (int rc = validate_prm(prm_host + (size_t)i * P_SIZE)) return rc;
         vfx_ctx* c = b->sims[i];
-        if (c->net || c->fcg_mode) {
-            set_error("a batched frame runs MGPCG only (detach the learned preconditioner)");
+        if ((c->net && !learned_on_device(c)) || c->fcg_mode) {
+            set_error("a batched frame runs MGPCG or the coarse-correction network only (detach the other "
+                      "learned preconditioners)");
             return VFX_E_VALUE;
         }
     }
@@ -545,8 +563,9 @@ int vfx_batch_tune(vfx_batch* b, float* const* data, const float* const* solid, 
     for (int i = 0; i < b->n; ++i) {
         if (!data[i] || !solid[i]) return arg_error("null field pointer");
         if (int rc = validate_prm(prm_host + (size_t)i * P_SIZE)) return rc;
-        if (b->sims[i]->net || b->sims[i]->fcg_mode) {
-            set_error("a batched frame runs MGPCG only (detach the learned preconditioner)");
+        if ((b->sims[i]->net && !learned_on_device(b->sims[i])) || b->sims[i]->fcg_mode) {
+            set_error("a batched frame runs MGPCG or the coarse-correction network only (detach the other "
+                      "learned preconditioners)");
             return VFX_E_VALUE;
         }
     }

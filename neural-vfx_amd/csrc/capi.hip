@@ -467,7 +467,7 @@ static int run_fcg(vfx_ctx* c, const real* vel, hipStream_t s, bool* converged) 
 
 // The coarse-correction network runs its flexible CG on the device (no host synchronisation);
 // the full-resolution network and the flexible-CG checks without a network are host-polled.
-static bool learned_on_device(const vfx_ctx* c) {
+bool vfx::learned_on_device(const vfx_ctx* c) {
     return c->net != nullptr && c->fcg_mode == 0 && c->net->net.N * 4 == c->N;
 }
 

@@ -187,6 +187,9 @@ int cg_budget_restart(const vfx_ctx* c, int maxit);
 // -1: the host polls the stop flag after each tail cycle
 int run_fused(vfx_ctx* c, real* vel, hipStream_t s, int tail_budget);
 #if !VFX_F64
+// the context's learned preconditioner runs its flexible CG on the device (the coarse correction),
+// so its projections need no host synchronisation (single steps and batched frames)
+bool learned_on_device(const vfx_ctx* c);
 // the learned flexible CG on the device (cg.hip): iteration k's p update and matvec with the
 // coarse correction cz; after the V-cycle, the network input (block means into cin, scale into
 // scal) and the block sums of r and r_prev; after the network, beta and the stop tests

@@ -7,8 +7,10 @@ simulation uses it up, so a frame never waits on the host; wait() returns per-si
 completion times measured on the GPU's wall clock.  tune() times the per-size launch shapes and
 the stream count on snapshots of the simulations' own states and keeps the fastest.
 
-The reference has no batched solver: it steps one Solver3D per call.  FP32 only; the learned
-preconditioner is not available inside frames (its flexible-CG loop waits on the host).
+The reference has no batched solver: it steps one Solver3D per call.  FP32 only.  A simulation
+with a coarse-correction network (set_preconditioner) runs the learned flexible CG with its MGPCG
+fallback, sequenced on the device like MGPCG; the full-resolution network, whose flexible-CG loop
+waits on the host, is not available inside frames.
 """
 from __future__ import annotations
 
@@ -59,6 +61,14 @@ class BatchSolver3D:
         obstacles = obstacles if obstacles is not None else [()] * self.n
         return [s.reset(o) for s, o in zip(self.solvers, obstacles)]
 
+    def set_preconditioner(self, params) -> None:
+        """Attach the learned coarse correction to every simulation: params are the trained weights
+        of a vfx.sim3d.learned.UNet (they do not depend on the grid); each simulation gets its own
+        network instance on its quarter-resolution grid.  None detaches."""
+        from .learned import UNet
+        for s in self.solvers:
+            s.set_preconditioner(None if params is None else UNet(s.res // 4, params=params))
+
     # ------------------------------------------------------------------ frames
     def _arrays(self, states, forces, dt):
         if len(states) != self.n or len(forces) != self.n:
@@ -66,8 +76,9 @@ class BatchSolver3D:
         for s, st in zip(self.solvers, states):
             s._check(st)
             s._structure()
-            if s._net is not None:
-                raise ValueError("the learned preconditioner is not available inside batched frames")
+            if s._net is not None and s._net.res * 4 != s.res:
+                raise ValueError("inside batched frames the learned preconditioner is the coarse correction "
+                                 "(a network on the quarter-resolution grid)")
             s._apply_options()
         for i, (s, st, f) in enumerate(zip(self.solvers, states, forces)):
             pack_params(s.config, s.res, f if f is not None else Forces3D(), dt, st.meta.step, self._prm[i])

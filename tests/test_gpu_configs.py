@@ -198,3 +198,49 @@ def test_batch_32_uneven_sims(cuda, hiplib):
              f"frame {info['frame_ms']:.2f} ms, latency p50 {np.percentile(info['latency_ms'], 50):.2f} ms")
     for st in states:
         assert torch.isfinite(st.data).all()
+
+
+def test_batch_32_uneven_sims_learned(cuda, hiplib):
+    """Config 4 with the learned preconditioner: the V-cycle's coarse correction trained at 64^3
+    runs in every one of the 32 sims of sizes {48, 64, 96, 128}^3 (one network instance per sim on
+    its quarter-resolution grid), sequenced on the device inside graph frames with the MGPCG
+    fallback; every projection converges, and one sim of each size equals its own Solver3D.step
+    with the same network bit for bit after every frame."""
+    from vfx.sim3d.batch import BatchSolver3D
+    from vfx.sim3d.learned import (UNet, generate_pairs, noise_pairs, random_scene, scene_config, train_coarse,
+                                   vcycle_pairs)
+    pairs = generate_pairs(64, seeds=range(20, 24), steps=10, every=1)
+    pairs += noise_pairs(64, [p[0] for p in pairs[::11]], count=len(pairs) // 2, seed=5)
+    net = UNet(16, seed=1)
+    train_coarse(net, vcycle_pairs(64, pairs), iters=200, lr=2e-3, seed=0)
+    params = net.params()
+    del pairs, net
+    rng = np.random.default_rng(0)
+    sizes = [int(x) for x in rng.choice(BATCH_SIZES, 32)]
+    scenes = [random_scene(i) for i in range(32)]
+    cfgs = [dataclasses.replace(scene_config(beta, i), pressure_restarts=1) for i, (_, _, beta) in enumerate(scenes)]
+    forces = [f for f, _, _ in scenes]
+    _say(f"learned batch: 32 sims, sizes {sorted(sizes)}")
+    batch = BatchSolver3D(sizes, cfgs, streams=4, graph=True)
+    batch.set_preconditioner(params)
+    states = batch.reset([o for _, o, _ in scenes])
+    picks = [sizes.index(n) for n in BATCH_SIZES if n in sizes]
+    singles = {i: Solver3D(sizes[i], cfgs[i]) for i in picks}
+    for i in picks:
+        singles[i].set_preconditioner(UNet(sizes[i] // 4, params=params))
+    ref = {i: singles[i].reset(scenes[i][1]) for i in picks}
+    for fr in range(6):
+        batch.step(states, forces, DT)
+        info = batch.wait()
+        for i in picks:
+            singles[i].step(ref[i], forces[i], DT)
+        torch.cuda.synchronize()
+        assert np.all(info["converged"]), (fr, np.nonzero(~info["converged"])[0], info["iters"])
+        for i in picks:
+            ps = singles[i].pressure_stats()
+            assert ps["converged"] and ps["learned_iters"] > 0, (fr, i, ps)
+            assert torch.equal(states[i].data, ref[i].data), (fr, i, sizes[i])
+        _say(f"  frame {fr}: iterations {int(info['iters'].min())}-{int(info['iters'].max())}, "
+             f"frame {info['frame_ms']:.2f} ms, latency p50 {np.percentile(info['latency_ms'], 50):.2f} ms")
+    for st in states:
+        assert torch.isfinite(st.data).all()
