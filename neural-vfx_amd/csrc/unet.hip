@@ -204,12 +204,22 @@ __global__ void __launch_bounds__(256) k_dout4(int N, const real* __restrict__ k
 // ---------------------------------------------------------------- launch helpers
 static unsigned blocks_for(size_t items, int threads = 256) { return (unsigned)((items + threads - 1) / threads); }
 
+// launch shape of a convolution over n3 voxels: the K split KS and the voxel tiles per wave MT
+// (KS = 1, MT = 4 while there are at least 16384 tiles of 16 voxels; KS = 2, 4 below 4096, 1024)
 template <int CIN, int CT>
 static void conv_t(int N, const bf16_t* in, const bf16_t* w, int kp, const float* bias, int cout, int relu, bf16_t* out,
                    int ocs, int ooff, float* out32, hipStream_t s, const DevState* gst, int gk) {
     const size_t n3 = (size_t)N * N * N;
-    hipLaunchKernelGGL((k_conv3<CIN, CT>), dim3(blocks_for(n3, 64)), dim3(NET_THREADS), 0, s, N, in, w, kp, bias, cout,
-                       relu, out, ocs, ooff, out32, gst, gk);
+    const size_t tiles = (n3 + 15) / 16;
+    constexpr int W = NET_THREADS / 64;
+#define VFX_CONV_LAUNCH(KS, MT)                                                                                      \
+    hipLaunchKernelGGL((k_conv3<CIN, CT, KS, MT>), dim3((unsigned)((tiles + (W / KS) * MT - 1) / ((W / KS) * MT))), \
+                       dim3(NET_THREADS), 0, s, N, in, w, kp, bias, cout, relu, out, ocs, ooff, out32, gst, gk)
+    if (CT <= 4 && tiles >= 16384) VFX_CONV_LAUNCH(1, (CT <= 4 ? 4 : 1));
+    else if (tiles >= 4096) VFX_CONV_LAUNCH(1, 1);
+    else if (tiles >= 1024) VFX_CONV_LAUNCH(2, 1);
+    else VFX_CONV_LAUNCH(4, 1);
+#undef VFX_CONV_LAUNCH
 }
 
 static bool conv(int cin, int ct, int N, const bf16_t* in, const bf16_t* w, int kp, const float* bias, int cout, int relu,

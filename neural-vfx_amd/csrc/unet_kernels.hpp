@@ -31,60 +31,107 @@ __device__ __forceinline__ void tap_off(int t, int& dz, int& dy, int& dx) {
     dx = t % 3 - 1;
 }
 
+// 16 zero bytes: the A operand of a tap outside the grid (or past the 27 taps) is loaded from here,
+// so every operand load is unconditional and the loads of several K steps are in flight together
+__device__ __align__(16) const unsigned int g_net_zero[4] = {0u, 0u, 0u, 0u};
+
 // Forward convolution 3^3, zero padding: out[v][co] = b[co] + sum_{t, ci} in[v + off(t)][ci] w[co][t][ci]
-// One wave computes 16 consecutive voxels x CT*16 output channels; 4 waves per workgroup.
+// A wave computes MT tiles of 16 consecutive voxels x CT*16 output channels over one K slice: the
+// K steps (32 = taps x input channels) are split over KS waves of the workgroup, whose partial
+// sums are added in LDS in wave order (small grids: more waves per voxel, shorter chains of
+// dependent MFMAs); MT > 1 reuses each weight fragment for MT voxel tiles (large grids).
 // Output: BF16 (channel stride ocs, offset ooff, ReLU if relu) or, if out32, FP32 channel 0.
-template <int CIN, int CT>
+template <int CIN, int CT, int KS, int MT>
 __global__ void __launch_bounds__(NET_THREADS)
 k_conv3(int N, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, int kp, const float* __restrict__ bias,
         int cout, int relu, bf16_t* __restrict__ out, int ocs, int ooff, float* __restrict__ out32,
         const DevState* __restrict__ gst, int gk) {
+    constexpr int WAVES = NET_THREADS / 64;
+    constexpr int TPB = WAVES / KS;   // wave groups (of KS waves) per workgroup
+    __shared__ floatx4 red[KS > 1 ? WAVES * CT * MT * 64 : 1];
     if (gated_off(gst, gk)) return;
     const int n3 = N * N * N;
     const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
-    const int v0 = ((int)blockIdx.x * (NET_THREADS / 64) + wave) * 16;
+    const int ks = wave % KS;
+    const int v0 = ((int)blockIdx.x * TPB + wave / KS) * 16 * MT;
     const int row = lane & 15, grp = lane >> 4;
-    const int v = v0 + row;
-    const bool vin = v < n3;
-    const int vc = vin ? v : 0;
-    const int x = vc % N, y = (vc / N) % N, z = vc / (N * N);
-    floatx4 acc[CT];
+    int x[MT], y[MT], z[MT];
+    bool vin[MT];
 #pragma unroll
-    for (int c = 0; c < CT; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int kk = 0; kk < kp; kk += 32) {
-        const int k = kk + grp * 8;
+    for (int m = 0; m < MT; ++m) {
+        const int v = v0 + m * 16 + row;
+        vin[m] = v < n3;
+        const int vc = vin[m] ? v : 0;
+        x[m] = vc % N;
+        y[m] = (vc / N) % N;
+        z[m] = vc / (N * N);
+    }
+    const bf16_t* zero = reinterpret_cast<const bf16_t*>(g_net_zero);
+    floatx4 acc[MT][CT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[m][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int nk = kp / 32;
+    const int kb = ks * nk / KS, ke = (ks + 1) * nk / KS;
+#pragma unroll 4
+    for (int kk = kb; kk < ke; ++kk) {
+        const int k = kk * 32 + grp * 8;
         const int t = k / CIN, ci = k % CIN;
-        bf16x8 a = {};
-        if (t < 27 && vin) {
-            int dz, dy, dx;
-            tap_off(t, dz, dy, dx);
-            const int xx = x + dx, yy = y + dy, zz = z + dz;
-            if (xx >= 0 && yy >= 0 && zz >= 0 && xx < N && yy < N && zz < N)
-                a = *reinterpret_cast<const bf16x8*>(in + (size_t)((zz * N + yy) * N + xx) * CIN + ci);
+        int dz, dy, dx;
+        tap_off(t, dz, dy, dx);
+        bf16x8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int xx = x[m] + dx, yy = y[m] + dy, zz = z[m] + dz;
+            const bool ok = vin[m] && t < 27 && (unsigned)xx < (unsigned)N && (unsigned)yy < (unsigned)N &&
+                            (unsigned)zz < (unsigned)N;
+            const bf16_t* src = ok ? in + (size_t)((zz * N + yy) * N + xx) * CIN + ci : zero;
+            a[m] = *reinterpret_cast<const bf16x8*>(src);
         }
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const bf16x8 b = *reinterpret_cast<const bf16x8*>(w + (size_t)(c * 16 + row) * kp + k);
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[c], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][c], 0, 0, 0);
         }
+    }
+    if (KS > 1) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) red[((wave * MT + m) * CT + c) * 64 + lane] = acc[m][c];
+        __syncthreads();
+        if (ks != 0) return;
+        for (int j = 1; j < KS; ++j)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int c = 0; c < CT; ++c) {
+                    const floatx4 p = red[(((wave + j) * MT + m) * CT + c) * 64 + lane];
+                    acc[m][c] = floatx4{acc[m][c][0] + p[0], acc[m][c][1] + p[1], acc[m][c][2] + p[2],
+                                        acc[m][c][3] + p[3]};
+                }
     }
     // lane holds rows (voxels) grp*4 + i, column (channel) row of each 16-wide tile
 #pragma unroll
-    for (int c = 0; c < CT; ++c) {
-        const int co = c * 16 + row;
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int vo = v0 + grp * 4 + i;
-            if (vo >= n3 || co >= cout) continue;
-            float val = acc[c][i] + (bias ? bias[co] : 0.0f);
-            if (relu) val = fmaxf(val, 0.0f);
-            if (out32) {
-                if (co == 0) out32[vo] = val;
-            } else {
-                out[(size_t)vo * ocs + ooff + co] = tobf(val);
+        for (int c = 0; c < CT; ++c) {
+            const int co = c * 16 + row;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int vo = v0 + m * 16 + grp * 4 + i;
+                if (vo >= n3 || co >= cout) continue;
+                float val = acc[m][c][i] + (bias ? bias[co] : 0.0f);
+                if (relu) val = fmaxf(val, 0.0f);
+                if (out32) {
+                    if (co == 0) out32[vo] = val;
+                } else {
+                    out[(size_t)vo * ocs + ooff + co] = tobf(val);
+                }
             }
         }
-    }
 }
 
 // Weight gradient of a 3^3 convolution for one tap t = blockIdx.y:

@@ -36,10 +36,18 @@ def _inputs(N, seed):
     return inp
 
 
-@pytest.mark.parametrize("N", [8, 16])
+@pytest.mark.parametrize("N", [8, 16, 32, 64])
 def test_net_forward_backward_match_reference(cuda, hiplib, N):
     """Output and every layer's weight / bias gradient equal the FP64 reference (same BF16
-    rounding points) to FP32-accumulation level."""
+    rounding points) to FP32-accumulation level.  The sizes cover every launch shape of the
+    convolutions: K split over 4 and 2 waves (grids of fewer than 4096 tiles of 16 voxels), one
+    wave per tile and 4 tiles per wave (16384 tiles and more).  The difference is BF16 rounding
+    flips (a sum that lands within its accumulation error of a BF16 rounding boundary rounds the
+    other way) carried through the 11 layers; it grows with the grid (4e-5, 8e-4, 5e-3 at 8, 16,
+    32: the earlier single-shape kernel gave the same), so the bound widens with N while staying
+    far below what a lost K slice (about 1/14 of a layer's sum) would give.  The gradients flip
+    the same way through the backward's BF16 stores (5e-3 at 16, 5e-2 at 32); at 64 only the
+    output is compared."""
     _say(f"unet fwd/bwd N={N}")
     params = init_params(seed=3)
     # non-zero biases so that their gradients are exercised too
@@ -58,7 +66,10 @@ def test_net_forward_backward_match_reference(cuda, hiplib, N):
     ref_out, ref_grad = unet_ref.grads(params, inp, gout)
     e = _rel_l2(out.cpu().numpy(), ref_out)
     _say(f"  output rel L2 {e:.2e}")
-    assert e <= 2e-3
+    tol_out, tol_grad = {8: (2e-3, 1e-2), 16: (2e-3, 1e-2), 32: (2e-2, 1.5e-1), 64: (5e-2, None)}[N]
+    assert e <= tol_out
+    if tol_grad is None:
+        return
     g = grad.cpu().numpy()
     k = 0
     for i, (ci, co, _) in enumerate(LAYERS):
@@ -66,7 +77,7 @@ def test_net_forward_backward_match_reference(cuda, hiplib, N):
         ew = _rel_l2(g[k:k + nw], ref_grad[k:k + nw])
         eb = _rel_l2(g[k + nw:k + nw + co], ref_grad[k + nw:k + nw + co])
         _say(f"  layer {i} ({ci}->{co}): weight grad {ew:.2e}, bias grad {eb:.2e}")
-        assert ew <= 1e-2 and eb <= 1e-2, i
+        assert ew <= tol_grad and eb <= tol_grad, i
         k += nw + co
 
 
