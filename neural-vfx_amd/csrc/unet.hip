@@ -367,16 +367,16 @@ void Net::forward(hipStream_t s, const DevState* gst, int gk) {
     const size_t v0 = (size_t)N * N * N, v1 = v0 / 8;
     layer_fwd(*this, 0, N, X, A0, 16, 0, nullptr, s, gst, gk);
     layer_fwd(*this, 1, N, A0, C0, 48, 32, nullptr, s, gst, gk);                 // e0b -> C0[32:48]
-    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 * 16)), dim3(256), 0, s, N, C0, 48, 32, 16, P0, gst, gk);
+    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 * 2)), dim3(256), 0, s, N, C0, 48, 32, 16, P0, gst, gk);
     layer_fwd(*this, 2, n1, P0, A1, 32, 0, nullptr, s, gst, gk);
     layer_fwd(*this, 3, n1, A1, C1, 96, 64, nullptr, s, gst, gk);                // e1b -> C1[64:96]
-    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 / 8 * 32)), dim3(256), 0, s, n1, C1, 96, 64, 32, P1, gst, gk);
+    hipLaunchKernelGGL(k_pool, dim3(blocks_for(v1 / 8 * 4)), dim3(256), 0, s, n1, C1, 96, 64, 32, P1, gst, gk);
     layer_fwd(*this, 4, n2, P1, A2, 64, 0, nullptr, s, gst, gk);
     layer_fwd(*this, 5, n2, A2, B2, 64, 0, nullptr, s, gst, gk);
-    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v1 * 64)), dim3(256), 0, s, n2, B2, 64, C1, 96, 0, gst, gk);
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v1 * 8)), dim3(256), 0, s, n2, B2, 64, C1, 96, 0, gst, gk);
     layer_fwd(*this, 6, n1, C1, D1a, 32, 0, nullptr, s, gst, gk);
     layer_fwd(*this, 7, n1, D1a, D1b, 32, 0, nullptr, s, gst, gk);
-    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v0 * 32)), dim3(256), 0, s, n1, D1b, 32, C0, 48, 0, gst, gk);
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v0 * 4)), dim3(256), 0, s, n1, D1b, 32, C0, 48, 0, gst, gk);
     layer_fwd(*this, 8, N, C0, D0a, 16, 0, nullptr, s, gst, gk);
     layer_fwd(*this, 9, N, D0a, D0b, 16, 0, nullptr, s, gst, gk);
     layer_fwd(*this, 10, N, D0b, nullptr, 16, 0, out, s, gst, gk);

@@ -252,38 +252,50 @@ k_pack_weights(const float* __restrict__ w, int cin, int cout, int cinp, int cou
     }
 }
 
-// 2x2x2 average pool of channels [off, off + c) of src (stride cs) at fine size n -> dst (c channels)
+// 2x2x2 average pool of channels [off, off + c) of src (stride cs) at fine size n -> dst (c
+// channels); one thread per 8 channels of a coarse voxel (16-byte loads and stores; c, cs and off
+// are multiples of 8)
 __global__ void __launch_bounds__(NET_THREADS)
 k_pool(int n, const bf16_t* __restrict__ src, int cs, int off, int c, bf16_t* __restrict__ dst,
        const DevState* __restrict__ gst, int gk) {
     if (gated_off(gst, gk)) return;
-    const int m = n / 2;
+    const int m = n / 2, c8 = c / 8;
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= (size_t)m * m * m * c) return;
-    const int ch = (int)(i % c);
-    const size_t p = i / c;
+    if (i >= (size_t)m * m * m * c8) return;
+    const int ch = (int)(i % c8) * 8;
+    const size_t p = i / c8;
     const int X = (int)(p % m), Y = (int)((p / m) % m), Z = (int)(p / ((size_t)m * m));
-    float s = 0.0f;
+    bf16x8 v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int x = 2 * X + (k & 1), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k >> 2);
-        s += bf(src[(size_t)((z * n + y) * n + x) * cs + off + ch]);
+        v[k] = *reinterpret_cast<const bf16x8*>(src + (size_t)((z * n + y) * n + x) * cs + off + ch);
     }
-    dst[i] = tobf(0.125f * s);
+    bf16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += bf(v[k][e]);
+        o[e] = tobf(0.125f * s);
+    }
+    *reinterpret_cast<bf16x8*>(dst + p * c + ch) = o;
 }
 
-// nearest 2x upsample of src (c channels, coarse size m) into channels [off, off + c) of dst (stride cs)
+// nearest 2x upsample of src (c channels, coarse size m) into channels [off, off + c) of dst
+// (stride cs); one thread per 8 channels of a fine voxel
 __global__ void __launch_bounds__(NET_THREADS)
 k_upsample(int m, const bf16_t* __restrict__ src, int c, bf16_t* __restrict__ dst, int cs, int off,
            const DevState* __restrict__ gst, int gk) {
     if (gated_off(gst, gk)) return;
-    const int n = 2 * m;
+    const int n = 2 * m, c8 = c / 8;
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= (size_t)n * n * n * c) return;
-    const int ch = (int)(i % c);
-    const size_t v = i / c;
+    if (i >= (size_t)n * n * n * c8) return;
+    const int ch = (int)(i % c8) * 8;
+    const size_t v = i / c8;
     const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((size_t)n * n));
-    dst[v * cs + off + ch] = src[(size_t)(((z / 2) * m + y / 2) * m + x / 2) * c + ch];
+    *reinterpret_cast<bf16x8*>(dst + v * cs + off + ch) =
+        *reinterpret_cast<const bf16x8*>(src + (size_t)(((z / 2) * m + y / 2) * m + x / 2) * c + ch);
 }
 
 // gradient of the upsample: out[p][ch] = sum over the 8 children of g[child][ch] (g stride gs)
