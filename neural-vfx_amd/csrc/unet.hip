@@ -204,8 +204,9 @@ __global__ void __launch_bounds__(256) k_dout4(int N, const real* __restrict__ k
 // ---------------------------------------------------------------- launch helpers
 static unsigned blocks_for(size_t items, int threads = 256) { return (unsigned)((items + threads - 1) / threads); }
 
-// launch shape of a convolution over n3 voxels: the K split KS and the voxel tiles per wave MT
-// (KS = 1, MT = 4 while there are at least 16384 tiles of 16 voxels; KS = 2, 4 below 4096, 1024)
+// launch shape of a convolution on an N^3 grid: the LDS-tiled kernel from N = 48 (up to 64 output
+// channels; smaller grids give it too few workgroups); otherwise the K split KS and the voxel tiles per wave MT (KS = 1, MT = 4 while there
+// are at least 16384 tiles of 16 voxels; KS = 2, 4 below 4096, 1024)
 template <int CIN, int CT>
 static void conv_t(int N, const bf16_t* in, const bf16_t* w, int kp, const float* bias, int cout, int relu, bf16_t* out,
                    int ocs, int ooff, float* out32, hipStream_t s, const DevState* gst, int gk) {
@@ -215,7 +216,11 @@ static void conv_t(int N, const bf16_t* in, const bf16_t* w, int kp, const float
 #define VFX_CONV_LAUNCH(KS, MT)                                                                                      \
     hipLaunchKernelGGL((k_conv3<CIN, CT, KS, MT>), dim3((unsigned)((tiles + (W / KS) * MT - 1) / ((W / KS) * MT))), \
                        dim3(NET_THREADS), 0, s, N, in, w, kp, bias, cout, relu, out, ocs, ooff, out32, gst, gk)
-    if (CT <= 4 && tiles >= 16384) VFX_CONV_LAUNCH(1, (CT <= 4 ? 4 : 1));
+    if (N >= 48 && CT <= 4) {
+        const unsigned nb = (unsigned)(((N + CX - 1) / CX) * ((N + CY - 1) / CY) * ((N + CZ - 1) / CZ));
+        hipLaunchKernelGGL((k_conv3l<CIN, CT>), dim3(nb), dim3(NET_THREADS), 0, s, N, in, w, kp, bias, cout, relu, out,
+                           ocs, ooff, out32, gst, gk);
+    } else if (CT <= 4 && tiles >= 16384) VFX_CONV_LAUNCH(1, (CT <= 4 ? 4 : 1));
     else if (tiles >= 4096) VFX_CONV_LAUNCH(1, 1);
     else if (tiles >= 1024) VFX_CONV_LAUNCH(2, 1);
     else VFX_CONV_LAUNCH(4, 1);

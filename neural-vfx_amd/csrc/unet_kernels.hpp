@@ -134,6 +134,105 @@ k_conv3(int N, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, int 
         }
 }
 
+// The same convolution with the input tile in LDS (grids of size >= 48): a workgroup computes a
+// CX x CY x CZ voxel tile (4 waves; wave w owns the tile rows y = w, each an M tile of 16 x
+// voxels, over the CZ z planes).  The input tile with its 1-voxel halo is staged 16 channels at
+// a time (zero outside the grid); a K step is two taps x those 16 channels (k = tap * Cin_pad +
+// channel, the weight layout's), so the A operand of every MFMA is one 16-byte LDS read per lane
+// and each weight fragment serves the tile's 4 z planes.  Sums: per 16-channel chunk, taps in
+// order, FP32 accumulation as in k_conv3.
+constexpr int CX = 16, CY = 4, CZ = 4;
+constexpr int SX = CX + 2, SY = CY + 2, SZ = CZ + 2, SV = SX * SY * SZ;   // staged voxels
+template <int CIN, int CT>
+__global__ void __launch_bounds__(NET_THREADS)
+k_conv3l(int N, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w, int kp, const float* __restrict__ bias,
+         int cout, int relu, bf16_t* __restrict__ out, int ocs, int ooff, float* __restrict__ out32,
+         const DevState* __restrict__ gst, int gk) {
+    static_assert(CIN % 16 == 0, "16-channel chunks");
+    __shared__ __align__(16) bf16_t sx[SV * 16];
+    if (gated_off(gst, gk)) return;
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int row = lane & 15, grp = lane >> 4;
+    const int tx = (N + CX - 1) / CX, ty = (N + CY - 1) / CY;
+    const int b = (int)blockIdx.x;
+    const int X0 = (b % tx) * CX, Y0 = ((b / tx) % ty) * CY, Z0 = (b / (tx * ty)) * CZ;
+    const bf16_t* zero = reinterpret_cast<const bf16_t*>(g_net_zero);
+    floatx4 acc[CZ][CT];
+#pragma unroll
+    for (int m = 0; m < CZ; ++m)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[m][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // staging: SV voxels x two 16-byte halves of the chunk's 16 channels
+    constexpr int SITEMS = SV * 2, SROUNDS = (SITEMS + NET_THREADS - 1) / NET_THREADS;
+    bf16x8 pre[SROUNDS];
+    auto load_chunk = [&](int c16) {
+#pragma unroll
+        for (int j = 0; j < SROUNDS; ++j) {
+            const int it = min((int)threadIdx.x + j * NET_THREADS, SITEMS - 1);
+            const int v = it >> 1, half = it & 1;
+            const int x = X0 - 1 + v % SX, y = Y0 - 1 + (v / SX) % SY, z = Z0 - 1 + v / (SX * SY);
+            const bool ok = (unsigned)x < (unsigned)N && (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N;
+            const bf16_t* src = ok ? in + (size_t)((z * N + y) * N + x) * CIN + c16 * 16 + half * 8 : zero;
+            pre[j] = *reinterpret_cast<const bf16x8*>(src);
+        }
+    };
+    auto store_chunk = [&]() {
+#pragma unroll
+        for (int j = 0; j < SROUNDS; ++j) {
+            const int it = (int)threadIdx.x + j * NET_THREADS;
+            if (j + 1 < SROUNDS || it < SITEMS) *reinterpret_cast<bf16x8*>(sx + it * 8) = pre[j];
+        }
+    };
+    load_chunk(0);
+    for (int c16 = 0; c16 < CIN / 16; ++c16) {
+        __syncthreads();   // the previous chunk's reads are done
+        store_chunk();
+        __syncthreads();
+        if (c16 + 1 < CIN / 16) load_chunk(c16 + 1);
+#pragma unroll 2
+        for (int t2 = 0; t2 < 14; ++t2) {
+            const int t = 2 * t2 + (grp >> 1);   // this lane group's tap (27: padding)
+            int dz, dy, dx;
+            tap_off(t, dz, dy, dx);
+            const int k = t * CIN + c16 * 16 + (grp & 1) * 8;
+            bf16x8 bw[CT];
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+                bw[c] = *reinterpret_cast<const bf16x8*>(t < 27 ? w + (size_t)(c * 16 + row) * kp + k : zero);
+#pragma unroll
+            for (int m = 0; m < CZ; ++m) {
+                const int sv = ((m + 1 + dz) * SY + (wave + 1 + dy)) * SX + (row + 1 + dx);
+                const bf16x8 a = t < 27 ? *reinterpret_cast<const bf16x8*>(sx + sv * 16 + (grp & 1) * 8) : bf16x8{};
+#pragma unroll
+                for (int c = 0; c < CT; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[c], acc[m][c], 0, 0, 0);
+            }
+        }
+    }
+    // lane holds rows (x voxels) grp*4 + i, column (channel) row of each 16-wide tile
+    const int y = Y0 + wave;
+#pragma unroll
+    for (int m = 0; m < CZ; ++m) {
+        const int z = Z0 + m;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int co = c * 16 + row;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = X0 + grp * 4 + i;
+                if (x >= N || y >= N || z >= N || co >= cout) continue;
+                const size_t vo = (size_t)(z * N + y) * N + x;
+                float val = acc[m][c][i] + (bias ? bias[co] : 0.0f);
+                if (relu) val = fmaxf(val, 0.0f);
+                if (out32) {
+                    if (co == 0) out32[vo] = val;
+                } else {
+                    out[vo * ocs + ooff + co] = tobf(val);
+                }
+            }
+        }
+    }
+}
+
 // Weight gradient of a 3^3 convolution for one tap t = blockIdx.y:
 //   part[g][t][co][ci] = sum over the voxels of group g = blockIdx.x of gpre[v][co] in[v + off(t)][ci]
 // The group's voxels go through LDS in chunks of 32, transposed so that both MFMA operands read 8

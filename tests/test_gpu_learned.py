@@ -41,7 +41,7 @@ def test_net_forward_backward_match_reference(cuda, hiplib, N):
     """Output and every layer's weight / bias gradient equal the FP64 reference (same BF16
     rounding points) to FP32-accumulation level.  The sizes cover every launch shape of the
     convolutions: K split over 4 and 2 waves (grids of fewer than 4096 tiles of 16 voxels), one
-    wave per tile and 4 tiles per wave (16384 tiles and more).  The difference is BF16 rounding
+    wave per tile and 4 tiles per wave (16384 tiles and more), the LDS-tiled kernel (grids from 48).  The difference is BF16 rounding
     flips (a sum that lands within its accumulation error of a BF16 rounding boundary rounds the
     other way) carried through the 11 layers; it grows with the grid (4e-5, 8e-4, 5e-3 at 8, 16,
     32: the earlier single-shape kernel gave the same), so the bound widens with N while staying
