@@ -72,8 +72,9 @@ extern "C" {
                                     stops there, like the reference at pressure_max_iters, is counted in
                                     vfx_pressure_stats out[11], and the budget grows for the next.
                                     1 polled: the host reads the stop flag every iteration and restarts
-                                    while the true residual is above tolerance (up to VFX_OPT_RESTARTS);
-                                    the learned path always runs this way. */
+                                    while the true residual is above tolerance (up to VFX_OPT_RESTARTS).
+                                    The learned coarse correction follows this setting; the full-resolution
+                                    network's flexible CG is always host-polled. */
 #define VFX_OPT_GRAPH 16         /* 1 (default): in device mode vfx_step replays the whole step as a
                                     captured hipGraph, one per set of host-side launch decisions (buffer
                                     pointers, budgets, warm start, options), up to 4 per context
@@ -315,10 +316,15 @@ int vfx_net_train_step_coarse(vfx_net* net, vfx_ctx* ctx, const float* solid, co
  * quarter of it is the coarse correction of the MG V-cycle, z = V(r) + P net(R (r - K V(r))).
  * Falls back to plain MGPCG from the current pressure when ||r|| falls less than 2x over 3
  * iterations, turns non-finite or VFX_OPT_LEARNED_MAXIT runs out; the true residual judges as for
- * MGPCG.  The context does not own the network. */
+ * MGPCG.  With VFX_OPT_SYNC 0 the coarse correction's flexible CG is sequenced on the device like
+ * MGPCG (an iteration budget, the network's kernels gated on the stop state, the fallback as the
+ * gated restart run after the true residual; vfx_step replays as a graph and batched frames
+ * accept it); the full-resolution network's is host-polled.  The context does not own the
+ * network, and a network serves one context at a time. */
 int vfx_ctx_set_net(vfx_ctx* ctx, vfx_net* net);
-/* ||r_k||_2 of every flexible-CG iteration of the last projection (convergence history): up to n
- * values into out (host), the count into *count */
+/* ||r_k||_2 of every flexible-CG iteration of the last host-polled projection (convergence
+ * history; empty after a device-sequenced one): up to n values into out (host), the count into
+ * *count */
 int vfx_fcg_history(vfx_ctx* ctx, double* out, int n, int* count);
 /* the preconditioner z = M(r): rms(r) * net(r / rms(r), fluid), 0 in solids */
 int vfx_net_apply(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* r, float* z, void* stream);
@@ -334,7 +340,8 @@ int vfx_net_apply(vfx_net* net, vfx_ctx* ctx, const float* solid, const float* r
  * graph-captures one solver's step (solver.py:388-459).  Each frame enqueues every simulation's
  * step (vfx_step's stages) on one of `streams` HIP streams forked from and joined back into the
  * caller's stream, directly or as one captured hipGraph replayed while the pointers, budgets and
- * launch shapes stay the same.  Projections are MGPCG sequenced on the device as in vfx_step
+ * launch shapes stay the same.  Projections are MGPCG (or, for a context with a coarse-correction
+ * network, the learned flexible CG with its MGPCG fallback) sequenced on the device as in vfx_step
  * (VFX_OPT_SYNC 0) with a per-simulation iteration budget (the device stops at the budget or at
  * convergence; with restarts on, one gated restart run; no host round trip inside a frame).  A
  * budget starts at min(P_MAXIT, 32); vfx_batch_wait reads the frame's state and doubles the budget
