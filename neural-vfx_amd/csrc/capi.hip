@@ -144,7 +144,7 @@ int vfx::stage_forces(vfx_ctx* c, real* data, hipStream_t s) {
     } else {
         launch_sources_diffuse(c->g, c->adv, c->P.fm, c->prm, c->st, c->hat, data, s);
     }
-    launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega, c->omega_s, c->force, s);
+    launch_vort_forces(c->g, data, c->P.fm, c->prm, c->inv2dx, c->omega_s, s);
     CHECK_LAUNCH("forces");
     return VFX_OK;
 }

@@ -92,50 +92,103 @@ __global__ void __launch_bounds__(256) k_diffuse(int N, int zc, const real* __re
     }
 }
 
-// curl (kernels.py:297-310): walls and solids count as zero velocity.
-__global__ void __launch_bounds__(256) k_curl(int N, int zc, const real* __restrict__ data,
-                                              const uint8_t* __restrict__ fm, real inv2dx,
-                                              real* __restrict__ w) {
+// curl at cell (x, y, z) (kernels.py:297-310): walls and solids count as zero velocity.
+__device__ __forceinline__ void curl_at(const real* __restrict__ data, const uint8_t* __restrict__ fm, int N, int x,
+                                        int y, int z, real inv2dx, real (&w)[3]) {
     const int n3 = N * N * N;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= N || y >= N) return;
     const real* vx = data + n3;
     const real* vy = data + 2 * n3;
     const real* vz = data + 3 * n3;
-    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
-    for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const int i = (z * N + y) * N + x;
-        const Nb6 n = nb6(N, x, y, z, i);
-        const int mxm = fm[n.xm], mxp = fm[n.xp], mym = fm[n.ym], myp = fm[n.yp], mzm = fm[n.zm], mzp = fm[n.zp];
-        const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
-        const bool syp = !n.oyp || myp == 0, szm = !n.ozm || mzm == 0, szp = !n.ozp || mzp == 0;
-        const real vz_yp = vz[n.yp], vz_ym = vz[n.ym], vy_zp = vy[n.zp], vy_zm = vy[n.zm];
-        const real vx_zp = vx[n.zp], vx_zm = vx[n.zm], vz_xp = vz[n.xp], vz_xm = vz[n.xm];
-        const real vy_xp = vy[n.xp], vy_xm = vy[n.xm], vx_yp = vx[n.yp], vx_ym = vx[n.ym];
-        const real dwdy = (syp ? RL(0.0) : vz_yp) - (sym ? RL(0.0) : vz_ym);
-        const real dvdz = (szp ? RL(0.0) : vy_zp) - (szm ? RL(0.0) : vy_zm);
-        const real dudz = (szp ? RL(0.0) : vx_zp) - (szm ? RL(0.0) : vx_zm);
-        const real dwdx = (sxp ? RL(0.0) : vz_xp) - (sxm ? RL(0.0) : vz_xm);
-        const real dvdx = (sxp ? RL(0.0) : vy_xp) - (sxm ? RL(0.0) : vy_xm);
-        const real dudy = (syp ? RL(0.0) : vx_yp) - (sym ? RL(0.0) : vx_ym);
-        w[i] = (dwdy - dvdz) * inv2dx;
-        w[n3 + i] = (dudz - dwdx) * inv2dx;
-        w[2 * n3 + i] = (dvdx - dudy) * inv2dx;
+    const int i = (z * N + y) * N + x;
+    const Nb6 n = nb6(N, x, y, z, i);
+    const int mxm = fm[n.xm], mxp = fm[n.xp], mym = fm[n.ym], myp = fm[n.yp], mzm = fm[n.zm], mzp = fm[n.zp];
+    const bool sxm = !n.oxm || mxm == 0, sxp = !n.oxp || mxp == 0, sym = !n.oym || mym == 0;
+    const bool syp = !n.oyp || myp == 0, szm = !n.ozm || mzm == 0, szp = !n.ozp || mzp == 0;
+    const real vz_yp = vz[n.yp], vz_ym = vz[n.ym], vy_zp = vy[n.zp], vy_zm = vy[n.zm];
+    const real vx_zp = vx[n.zp], vx_zm = vx[n.zm], vz_xp = vz[n.xp], vz_xm = vz[n.xm];
+    const real vy_xp = vy[n.xp], vy_xm = vy[n.xm], vx_yp = vx[n.yp], vx_ym = vx[n.ym];
+    const real dwdy = (syp ? RL(0.0) : vz_yp) - (sym ? RL(0.0) : vz_ym);
+    const real dvdz = (szp ? RL(0.0) : vy_zp) - (szm ? RL(0.0) : vy_zm);
+    const real dudz = (szp ? RL(0.0) : vx_zp) - (szm ? RL(0.0) : vx_zm);
+    const real dwdx = (sxp ? RL(0.0) : vz_xp) - (sxm ? RL(0.0) : vz_xm);
+    const real dvdx = (sxp ? RL(0.0) : vy_xp) - (sxm ? RL(0.0) : vy_xm);
+    const real dudy = (syp ? RL(0.0) : vx_yp) - (sym ? RL(0.0) : vx_ym);
+    w[0] = (dwdy - dvdz) * inv2dx;
+    w[1] = (dudz - dwdx) * inv2dx;
+    w[2] = (dvdx - dudy) * inv2dx;
+}
+
+// |omega_s| at cell i: channel 3 of the smoothed vorticity (k_smooth3<0> stores
+// sqrt(a * a + b * b + c * c) of the values it writes, the expression of kernels.py:362-365)
+__device__ __forceinline__ real wmag(const real* __restrict__ o, size_t n3, size_t i) { return o[3 * n3 + i]; }
+
+// the parameters body_force_at reads
+struct BodyPrm {
+    real amb, alpha, beta, eps, cfloor, dxp, wdx, wdy, wdz, wmark;
+    bool wind;
+};
+
+__device__ __forceinline__ BodyPrm body_prm(const float* __restrict__ prm) {
+    BodyPrm b;
+    b.amb = (real)prm[P_AMB]; b.alpha = (real)prm[P_ALPHA]; b.beta = (real)prm[P_BETA]; b.eps = (real)prm[P_EPS];
+    b.cfloor = (real)prm[P_CFLOOR]; b.dxp = (real)prm[P_DX];
+    b.wdx = (real)prm[P_WIND_X]; b.wdy = (real)prm[P_WIND_Y]; b.wdz = (real)prm[P_WIND_Z]; b.wmark = (real)prm[P_WMARK];
+    b.wind = b.wdx * b.wdx + b.wdy * b.wdy + b.wdz * b.wdz > RL(0.0);
+    return b;
+}
+
+// body_forces (kernels.py:346-387) at fluid cell (x, y, z): buoyancy + vorticity confinement +
+// wind marker force from the state f and the smoothed vorticity om (4 channels)
+__device__ __forceinline__ void body_force_at(const real* __restrict__ f, const real* __restrict__ om,
+                                              const BodyPrm& b, int N, int x, int y, int z, real (&F)[3]) {
+    const size_t n3 = (size_t)N * N * N;
+    const size_t sz = (size_t)N * N;
+    const size_t i = idx3(N, z, y, x);
+    const real d = f[i], T = f[4 * n3 + i];
+    real fx = RL(0.0), fy = -b.alpha * d + b.beta * (T - b.amb), fz = RL(0.0);
+    if (b.eps != RL(0.0)) {
+        const int xr = min(x + 1, N - 1), xl = max(x - 1, 0);
+        const int yt = min(y + 1, N - 1), yb = max(y - 1, 0);
+        const int zf = min(z + 1, N - 1), zb = max(z - 1, 0);
+        const size_t rowb = (size_t)z * sz + (size_t)y * N;
+        const real gx = (wmag(om, n3, rowb + xr) - wmag(om, n3, rowb + xl)) / (real)max(xr - xl, 1);
+        const real gy = (wmag(om, n3, (size_t)z * sz + (size_t)yt * N + x) -
+                         wmag(om, n3, (size_t)z * sz + (size_t)yb * N + x)) / (real)max(yt - yb, 1);
+        const real gz = (wmag(om, n3, (size_t)zf * sz + (size_t)y * N + x) -
+                         wmag(om, n3, (size_t)zb * sz + (size_t)y * N + x)) / (real)max(zf - zb, 1);
+        const real wx = om[i], wy = om[n3 + i], wz = om[2 * n3 + i];
+        const real fl = b.cfloor * sqrt(wx * wx + wy * wy + wz * wz);
+        const real mag = sqrt(gx * gx + gy * gy + gz * gz + fl * fl) + RL(1.0e-20);
+        const real k = b.eps * b.dxp / mag;
+        fx += k * (gy * wz - gz * wy);
+        fy += k * (gz * wx - gx * wz);
+        fz += k * (gx * wy - gy * wx);
     }
+    if (b.wind) {
+        const real mk = clampf_(d + f[5 * n3 + i] + (T - b.amb) / b.wmark, RL(0.0), RL(1.0));
+        fx += b.wdx * mk;
+        fy += b.wdy * mk;
+        fz += b.wdz * mk;
+    }
+    F[0] = fx;
+    F[1] = fy;
+    F[2] = fz;
 }
 
 // The fluid-renormalised [1,2,1]^3 smoothing B(w a) / B(w) of three channels in ONE pass: sep_x,
-// sep_y and sep_z_store / sep_z_apply (kernels.py:392-457) composed.  A 256-thread workgroup owns
-// a 64 x 8 column of cells (two rows per thread) and marches up z through zt planes.  Each
-// input plane is staged in LDS with a one-cell x / y halo, the x pass runs on the 10 staged rows,
-// the y pass per thread, and the y-pass values of three consecutive planes stay in registers
-// for the z pass.  Every pass keeps the reference's operation order, so the result is the three
-// separate passes' result, while a and the mask are read once and no intermediate is written.
-// MODE 0: out (4 channels) = the smoothed field and its magnitude sqrt(a^2 + b^2 + c^2) (body_forces'
-// |omega_s| at every cell, evaluated once); MODE 1: fluid cells of the state get velocity += dt *
-// the smoothed field.  Tiles are numbered through xcd_block, so the tiles that
-// share halo rows run on one XCD.
+// sep_y and sep_z_store / sep_z_apply (kernels.py:392-457) composed, with the field a computed
+// where it is staged.  A 256-thread workgroup owns a 64 x 8 column of cells (two rows per thread)
+// and marches up z through zt planes.  Each input plane is staged in LDS with a one-cell x / y
+// halo, the x pass runs on the 10 staged rows, the y pass per thread, and the y-pass values of
+// three consecutive planes stay in registers for the z pass.  Every pass keeps the reference's
+// operation order, so the result is the separate kernels' result bit for bit, while no
+// intermediate field is written.
+// MODE 0: a = the curl of the state's velocity (curl_at, kernels.py:297-310); out (4 channels) =
+// the smoothed vorticity and its magnitude sqrt(a^2 + b^2 + c^2) (body_forces' |omega_s| at every
+// cell, evaluated once).  MODE 1: a = the body force (body_force_at, kernels.py:346-387, from the
+// state and the smoothed vorticity om); fluid cells of the state get velocity += dt * the
+// smoothed force.  Tiles are numbered through xcd_block, so the tiles that share halo rows run on
+// one XCD.
 // zt = SM_ZT = 8 planes per workgroup: 0.91 ms for the forces stage at 256^3 against 0.95 ms with
 // 16 (more workgroups in flight outweigh the extra halo planes; 4 is no faster).  Grids that would
 // have fewer than 1024 workgroups use fewer planes (smooth3_planes).
@@ -143,8 +196,9 @@ constexpr int SM_TX = 64, SM_TY = 8, SM_ZT = 8, SM_RY = SM_TY / 4;   // blockDim
 constexpr int SM_HX = SM_TX + 2, SM_HY = SM_TY + 2;
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__ a, const uint8_t* __restrict__ fm,
-                                                 const float* __restrict__ prm, real* __restrict__ out, int zt) {
+__global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__ f, const uint8_t* __restrict__ fm,
+                                                 const float* __restrict__ prm, real* __restrict__ out, int zt,
+                                                 real inv2dx, const real* __restrict__ om) {
     __shared__ real sa[3][SM_HY][SM_HX];
     __shared__ real sw[SM_HY][SM_HX];
     __shared__ real sx[4][SM_HY][SM_TX];
@@ -158,6 +212,8 @@ __global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__
     const int z1 = min(z0 + zt, N);
     const int x = x0 + tx;
     const real dt = MODE == 1 ? (real)prm[P_DT] : RL(0.0);
+    BodyPrm bp;
+    if (MODE == 1) bp = body_prm(prm);
     real ym[SM_RY][4], yc[SM_RY][4], yp[SM_RY][4];
 #pragma unroll
     for (int j = 0; j < SM_RY; ++j)
@@ -176,11 +232,19 @@ __global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__
             const int r = k / SM_HX, c = k - r * SM_HX;
             const int py = y0 - 1 + r, px = x0 - 1 + c;
             const bool in = k < SM_HY * SM_HX && zin && px >= 0 && px < N && py >= 0 && py < N;
-            const int gi = in ? (zi * N + py) * N + px : 0;   // clamped address: always loaded
-            pv[j][0] = a[gi];
-            pv[j][1] = a[n3 + gi];
-            pv[j][2] = a[2 * n3 + gi];
+            // outside the box: cell (0, 0, 0) is computed and weighted 0 (every load is issued)
+            const int cx = in ? px : 0, cy = in ? py : 0, cz = in ? zi : 0;
+            const int gi = (cz * N + cy) * N + cx;
             const int m = fm[gi];
+            real v[3];
+            if (MODE == 0) {
+                curl_at(f, fm, N, cx, cy, cz, inv2dx, v);
+            } else {
+                body_force_at(f, om, bp, N, cx, cy, cz, v);   // (a solid cell's force is weighted 0)
+            }
+            pv[j][0] = v[0];
+            pv[j][1] = v[1];
+            pv[j][2] = v[2];
             pw[j] = (in && m != 0) ? RL(1.0) : RL(0.0);
         }
     };
@@ -279,61 +343,6 @@ static int smooth3_blocks(int N) {
     return ((N + SM_TX - 1) / SM_TX) * ((N + SM_TY - 1) / SM_TY) * ((N + zt - 1) / zt);
 }
 
-// |omega_s| at cell i: channel 3 of the smoothed vorticity (k_smooth3<0> stores
-// sqrt(a * a + b * b + c * c) of the values it writes, the expression of kernels.py:362-365)
-__device__ __forceinline__ real wmag(const real* __restrict__ o, size_t n3, size_t i) { return o[3 * n3 + i]; }
-
-// body_forces (kernels.py:346-387): buoyancy + vorticity confinement + wind marker force.
-__global__ void __launch_bounds__(256) k_body_forces(int N, int zc, const real* __restrict__ f,
-                                                     const uint8_t* __restrict__ fm, const real* __restrict__ om,
-                                                     const float* __restrict__ prm, real* __restrict__ force) {
-    const size_t n3 = (size_t)N * N * N;
-    const size_t sz = (size_t)N * N;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= N || y >= N) return;
-    const real amb = (real)prm[P_AMB], alpha = (real)prm[P_ALPHA], beta = (real)prm[P_BETA], eps = (real)prm[P_EPS];
-    const real cfloor = (real)prm[P_CFLOOR], dxp = (real)prm[P_DX];
-    const real wdx = (real)prm[P_WIND_X], wdy = (real)prm[P_WIND_Y], wdz = (real)prm[P_WIND_Z],
-               wmark = (real)prm[P_WMARK];
-    const bool wind = wdx * wdx + wdy * wdy + wdz * wdz > RL(0.0);
-    const int z1 = min((int)(blockIdx.z + 1) * zc, N);
-    for (int z = blockIdx.z * zc; z < z1; ++z) {
-        const size_t i = idx3(N, z, y, x);
-        if (fm[i] == 0) {
-            force[i] = RL(0.0); force[n3 + i] = RL(0.0); force[2 * n3 + i] = RL(0.0);
-            continue;
-        }
-        const real d = f[i], T = f[4 * n3 + i];
-        real fx = RL(0.0), fy = -alpha * d + beta * (T - amb), fz = RL(0.0);
-        if (eps != RL(0.0)) {
-            const int xr = min(x + 1, N - 1), xl = max(x - 1, 0);
-            const int yt = min(y + 1, N - 1), yb = max(y - 1, 0);
-            const int zf = min(z + 1, N - 1), zb = max(z - 1, 0);
-            const size_t rowb = (size_t)z * sz + (size_t)y * N;
-            const real gx = (wmag(om, n3, rowb + xr) - wmag(om, n3, rowb + xl)) / (real)max(xr - xl, 1);
-            const real gy = (wmag(om, n3, (size_t)z * sz + (size_t)yt * N + x) -
-                              wmag(om, n3, (size_t)z * sz + (size_t)yb * N + x)) / (real)max(yt - yb, 1);
-            const real gz = (wmag(om, n3, (size_t)zf * sz + (size_t)y * N + x) -
-                              wmag(om, n3, (size_t)zb * sz + (size_t)y * N + x)) / (real)max(zf - zb, 1);
-            const real wx = om[i], wy = om[n3 + i], wz = om[2 * n3 + i];
-            const real fl = cfloor * sqrt(wx * wx + wy * wy + wz * wz);
-            const real mag = sqrt(gx * gx + gy * gy + gz * gz + fl * fl) + RL(1.0e-20);
-            const real k = eps * dxp / mag;
-            fx += k * (gy * wz - gz * wy);
-            fy += k * (gz * wx - gx * wz);
-            fz += k * (gx * wy - gy * wx);
-        }
-        if (wind) {
-            const real mk = clampf_(d + f[5 * n3 + i] + (T - amb) / wmark, RL(0.0), RL(1.0));
-            fx += wdx * mk;
-            fy += wdy * mk;
-            fz += wdz * mk;
-        }
-        force[i] = fx; force[n3 + i] = fy; force[2 * n3 + i] = fz;
-    }
-}
-
 void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, const float* prm,
                             const DevState* st, real* hat, real* data, hipStream_t s) {
     // n_diff = max(1, ceil(N / 256)) substeps, ping-pong ending in data (solver.py:107-109,436-442)
@@ -361,13 +370,14 @@ void launch_sources_pass(int pass, const Geom& g, const real* src, const uint8_t
         hipLaunchKernelGGL(k_diffuse, g.grid, g.block, 0, s, g.N, g.zc, src, fm, prm, RL(1.0), dst);
 }
 
-void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx,
-                        real* omega, real* omega_s, real* force, hipStream_t s) {
+// the vorticity-confinement chain (kernels.py:297-457): curl + smoothing of omega into omega_s,
+// then body forces + smoothing + apply to the velocity; omega and the force are never stored
+void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx, real* omega_s,
+                        hipStream_t s) {
     const dim3 sb(SM_TX, 4), sg(smooth3_blocks(g.N));
-    hipLaunchKernelGGL(k_curl, g.grid, g.block, 0, s, g.N, g.zc, data, fm, inv2dx, omega);
-    hipLaunchKernelGGL(k_smooth3<0>, sg, sb, 0, s, g.N, omega, fm, prm, omega_s, smooth3_planes(g.N));
-    hipLaunchKernelGGL(k_body_forces, g.grid, g.block, 0, s, g.N, g.zc, data, fm, omega_s, prm, force);
-    hipLaunchKernelGGL(k_smooth3<1>, sg, sb, 0, s, g.N, force, fm, prm, data, smooth3_planes(g.N));
+    hipLaunchKernelGGL(k_smooth3<0>, sg, sb, 0, s, g.N, data, fm, prm, omega_s, smooth3_planes(g.N), inv2dx,
+                       nullptr);
+    hipLaunchKernelGGL(k_smooth3<1>, sg, sb, 0, s, g.N, data, fm, prm, data, smooth3_planes(g.N), inv2dx, omega_s);
 }
 
 }  // namespace vfx

@@ -762,31 +762,35 @@ __global__ void __launch_bounds__(CB_THREADS) k_coarse(CoarseLevelsT<LT> cl, con
     coarse_vcycle<LT>(cl.n, [&](int j) -> const LvlT<LT>& { return cl.lv[j]; });
 }
 
-// The same V-cycle with every coarse level resident in LDS (FP32 / FP64 level storage): x, r,
-// diag and off of each level in the global layout, 48 reals per cell.  The operators and the
+// The same V-cycle with every coarse level resident in LDS: x, r, diag and off of each level in
+// the global layout and storage type (48 values per cell), so every load and store rounds as the
+// global kernel's does and the results are the same bit for bit.  The operators and the
 // right-hand side of the top level are read once, and only the top level's x is written back
 // (the next finer level prolongs it); every sweep then waits on LDS instead of the L2.
-__host__ __device__ inline size_t coarse_lds_reals(const CoarseLevelsT<real>& cl, int upto) {
+template <class LT>
+__host__ __device__ inline size_t coarse_lds_vals(const CoarseLevelsT<LT>& cl, int upto) {
     size_t off = 0;
     for (int i = 0; i < upto; ++i) off += (size_t)48 * cl.lv[i].m * cl.lv[i].m * cl.lv[i].m;
     return off;
 }
 
-__global__ void __launch_bounds__(CB_THREADS) k_coarse_lds(CoarseLevelsT<real> cl, const DevState* __restrict__ st,
+template <class LT>
+__global__ void __launch_bounds__(CB_THREADS) k_coarse_lds(CoarseLevelsT<LT> cl, const DevState* __restrict__ st,
                                                            int k) {
-    extern __shared__ __align__(16) real cb_lds[];
+    extern __shared__ __align__(16) unsigned char cb_raw[];
+    LT* cb_lds = reinterpret_cast<LT*>(cb_raw);
     if (cg_inactive(st, k)) return;
     auto lvl = [&](int j) {
-        LvlT<real> L = cl.lv[j];
+        LvlT<LT> L = cl.lv[j];
         const size_t u = (size_t)L.m * L.m * L.m * 8;
-        L.x = cb_lds + coarse_lds_reals(cl, j);
+        L.x = cb_lds + coarse_lds_vals(cl, j);
         L.r = L.x + u;
         L.diag = L.r + u;
         L.off = L.diag + u;
         return L;
     };
     for (int j = 0; j < cl.n; ++j) {
-        const LvlT<real> G = cl.lv[j], L = lvl(j);
+        const LvlT<LT> G = cl.lv[j], L = lvl(j);
         const uint32_t u = (uint32_t)G.m * G.m * G.m * 8;
         for (uint32_t t = threadIdx.x; t < u; t += CB_THREADS) {
             L.diag[t] = G.diag[t];
@@ -795,26 +799,24 @@ __global__ void __launch_bounds__(CB_THREADS) k_coarse_lds(CoarseLevelsT<real> c
         for (uint32_t t = threadIdx.x; t < 3 * u; t += CB_THREADS) L.off[t] = G.off[t];
     }
     __syncthreads();
-    coarse_vcycle<real>(cl.n, lvl);
-    const LvlT<real> L0 = lvl(0);
+    coarse_vcycle<LT>(cl.n, lvl);
+    const LvlT<LT> L0 = lvl(0);
     const uint32_t u0 = (uint32_t)L0.m * L0.m * L0.m * 8;
     for (uint32_t t = threadIdx.x; t < u0; t += CB_THREADS) cl.lv[0].x[t] = L0.x[t];
 }
 
 constexpr size_t COARSE_LDS_MAX = 64 * 1024;   // bytes of dynamic LDS the resident kernel may use
 
-// the coarse V-cycle of levels S.. on stream s: LDS-resident when the storage is `real` and fits
+// the coarse V-cycle of levels S.. on stream s: LDS-resident when it fits
 template <class LT>
 static void launch_coarse(const Pressure& P, hipStream_t s, int k) {
     CoarseLevelsT<LT> cl{};
     cl.n = P.nlev - P.S;
     for (int j = 0; j < cl.n; ++j) cl.lv[j] = lvl_as<LT>(P.lv[P.S + j]);
-    if constexpr (std::is_same<LT, real>::value) {
-        const size_t bytes = coarse_lds_reals(cl, cl.n) * sizeof(real);
-        if (bytes <= COARSE_LDS_MAX) {
-            hipLaunchKernelGGL(k_coarse_lds, dim3(1), dim3(CB_THREADS), bytes, s, cl, P.st, k);
-            return;
-        }
+    const size_t bytes = coarse_lds_vals(cl, cl.n) * sizeof(LT);
+    if (bytes <= COARSE_LDS_MAX) {
+        hipLaunchKernelGGL(k_coarse_lds<LT>, dim3(1), dim3(CB_THREADS), bytes, s, cl, P.st, k);
+        return;
     }
     hipLaunchKernelGGL(k_coarse<LT>, dim3(1), dim3(CB_THREADS), 0, s, cl, P.st, k);
 }

@@ -133,8 +133,8 @@ void launch_sources_diffuse(const Geom& g, const real* adv, const uint8_t* fm, c
 // supplies density, temperature and fuel; no diffusion, which leaves the velocity as it is)
 void launch_sources_vel(const Geom& g, const real* adv, const uint8_t* fm, const float* prm, real* data,
                         hipStream_t s);
-void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx,
-                        real* omega, real* omega_s, real* force, hipStream_t s);
+void launch_vort_forces(const Geom& g, real* data, const uint8_t* fm, const float* prm, real inv2dx, real* omega_s,
+                        hipStream_t s);
 
 // ---------------------------------------------------------------- projection
 void launch_div_abs(const Geom& g, const real* vel, const real* solid, real* out, hipStream_t s);
