@@ -32,6 +32,14 @@ namespace vfx {
 #define CELL_LOOP_END }
 
 constexpr int HB_IT = (HBN + TILE_THREADS - 1) / TILE_THREADS;   // halo blocks staged per thread
+// Thread t stages halo blocks t + it TILE_THREADS.  The last round holds the HB_TAIL blocks left
+// over; only the first waves of the workgroup, which hold them, load it (a whole wave skips it, a
+// wave-uniform test that waits on no load), so the other waves spend no address arithmetic or
+// memory instructions on a round with nothing to stage.
+constexpr int HB_TAIL = HBN - (HB_IT - 1) * TILE_THREADS;
+__device__ __forceinline__ bool stages_round(int t, int it) {
+    return it + 1 < HB_IT || t < ((HB_TAIL + 63) & ~63);
+}
 
 // The tile of this workgroup.  Tiles are launched as a 1D grid and renumbered with xcd_block, so
 // each XCD sweeps a contiguous run of tiles (x fastest, then y, then z) and neighbouring tiles
@@ -620,6 +628,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
+        if (!stages_round(t, it)) continue;
         hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
 #pragma unroll
         for (int rw = 0; rw < 4; ++rw) {
@@ -746,6 +755,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
     int hm[HB_IT][4];
 #pragma unroll
     for (int it = 0; it < HB_IT; ++it) {
+        if (!stages_round(t, it)) continue;
         hb[it] = halo_block(N, tl, min(t + it * TILE_THREADS, HBN - 1));
         ld8(L.x + (size_t)hb[it].lx * 8, xv[it]);
 #pragma unroll
