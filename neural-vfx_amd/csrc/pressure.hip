@@ -349,11 +349,28 @@ struct HalfItem {
     uint32_t h, o;   // offset in the colour half, natural cell index
 };
 
-// item t of a one-colour launch (m even); false past the last item (4 m^3 < 2^32 for m <= 512)
+// Components per lane of the even-level sweeps: 2 with 16-bit storage (the two components of a
+// lane are one 4-byte access, so a wave's access is one contiguous 256-byte run, as with one
+// 4-byte component per lane at FP32), 1 otherwise.  The arithmetic of each component is the
+// one-component item's, so the results do not depend on CP.  The restriction keeps one component
+// per lane: with two, its 40 loads per item need 167 VGPRs, and at 3 waves per SIMD it ran slower
+// (30.9 against 28.3 us at level 1 of 256^3; the prolongation went from 39.3 to 34.7 us).
+template <class LT>
+constexpr int level_cp() {
+    return sizeof(LT) == 2 ? 2 : 1;
+}
+template <int CP>
+constexpr uint32_t cp_shift() {
+    return CP == 2 ? 2u : 3u;   // log2 of the lanes per cell (8 / CP)
+}
+
+// item t of a one-colour launch (m even), CP components from it.c; false past the last item
+// (4 m^3 < 2^32 for m <= 512)
+template <int CP>
 __device__ __forceinline__ bool half_item_even(const LevelShape& s, uint32_t t, int color, HalfItem& it) {
-    if (t >= s.half * 8u) return false;
-    it.c = (int)(t & 7u);
-    it.h = t >> 3;
+    if (t >= s.half * (8u / CP)) return false;
+    it.c = (int)(t & ((8u / CP) - 1u)) * CP;
+    it.h = t >> cp_shift<CP>();
     uint32_t Ih;
     split_half(s, it.h, it.K, it.J, Ih);
     const int b = (color + it.J + it.K) & 1;
@@ -395,6 +412,42 @@ __device__ __forceinline__ void stc(LT* base, uint32_t s, int c, real v) {
     stl(reinterpret_cast<LT*>(reinterpret_cast<char*>(base) + (s * 8u + (uint32_t)c) * (uint32_t)sizeof(LT)), v);
 }
 
+// CP consecutive values from p (CP values aligned to CP elements): one load or store of them all
+template <class LT>
+struct alignas(2 * sizeof(LT)) LPair {
+    LT a, b;
+};
+template <int CP, class LT>
+__device__ __forceinline__ void ldpv(const LT* p, real (&v)[CP]) {
+    if constexpr (CP == 1) {
+        v[0] = ldl(p);
+    } else {
+        const LPair<LT> w = *reinterpret_cast<const LPair<LT>*>(p);
+        v[0] = ldl(&w.a);
+        v[1] = ldl(&w.b);
+    }
+}
+template <int CP, class LT>
+__device__ __forceinline__ void stpv(LT* p, const real (&v)[CP]) {
+    if constexpr (CP == 1) {
+        stl(p, v[0]);
+    } else {
+        LPair<LT> w;
+        stl(&w.a, v[0]);
+        stl(&w.b, v[1]);
+        *reinterpret_cast<LPair<LT>*>(p) = w;
+    }
+}
+// components c .. c + CP - 1 of storage cell s (ldc / stc)
+template <int CP, class LT>
+__device__ __forceinline__ void ldcv(const LT* base, uint32_t s, int c, real (&v)[CP]) {
+    ldpv<CP>(reinterpret_cast<const LT*>(reinterpret_cast<const char*>(base) + (s * 8u + (uint32_t)c) * (uint32_t)sizeof(LT)), v);
+}
+template <int CP, class LT>
+__device__ __forceinline__ void stcv(LT* base, uint32_t s, int c, const real (&v)[CP]) {
+    stpv<CP>(reinterpret_cast<LT*>(reinterpret_cast<char*>(base) + (s * 8u + (uint32_t)c) * (uint32_t)sizeof(LT)), v);
+}
+
 // offsum_v with every neighbour present (a standard cell): the same sums without the tests
 __device__ __forceinline__ real offsum_all(const real* cf, const real* xn) {
     real a = RL(0.0);
@@ -403,52 +456,66 @@ __device__ __forceinline__ real offsum_all(const real* cf, const real* xn) {
     return a;
 }
 
-// smooth_item<true> (non-zero guess) of a one-colour item; tb: the level table of component
-// it.c (the same for every item of a thread)
-template <class LT>
+// smooth_item<true> (non-zero guess) of a one-colour item, components it.c .. it.c + CP - 1; tb:
+// the level table of those components (the same for every item of a thread)
+template <int CP, class LT>
 __device__ __forceinline__ void smooth_even(const LvlT<LT>& L, const LevelShape& s, int color, const HalfItem& it,
-                                            const Tab& tb) {
+                                            const Tab (&tb)[CP]) {
     const int c = it.c;
     const uint32_t own = (uint32_t)color * s.half + it.h;
     const EvenNb n = even_neighbours(s, color, it);
-    const real r = ldc(L.r, own, c);
+    real r[CP];
+    ldcv<CP>(L.r, own, c, r);
     const int f = L.sf[it.o];
-    real xn[6];
+    real xn[6][CP];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xn[k] = ldc(L.x, n.s[k], c);
-    real d, acc;
+    for (int k = 0; k < 6; ++k) ldcv<CP>(L.x, n.s[k], c, xn[k]);
+    real out[CP];
     if (__all(f != 0)) {
         // a standard cell has all six neighbours
-        d = tb.d;
-        const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
-        acc = r - offsum_all(cf, xn);
-    } else {
-        real cf[6];
-        const LNb nn = level_neighbours(L.m, it.o, it.K, it.J, it.I);
-        d = ldl(L.diag + it.o * 8u + c);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) cf[k] = link_coef(L, nn, it.o, k, c);
-        acc = r - offsum_v(cf, xn, n.e);
+        for (int q = 0; q < CP; ++q) {
+            const real d = tb[q].d;
+            const real cf[6] = {tb[q].o[0], tb[q].o[0], tb[q].o[1], tb[q].o[1], tb[q].o[2], tb[q].o[2]};
+            const real xq[6] = {xn[0][q], xn[1][q], xn[2][q], xn[3][q], xn[4][q], xn[5][q]};
+            const real acc = r[q] - offsum_all(cf, xq);
+            out[q] = d <= L.dmin ? RL(0.0) : acc / d;
+        }
+    } else {
+        const LNb nn = level_neighbours(L.m, it.o, it.K, it.J, it.I);
+        real dg[CP], cf[6][CP];
+        ldpv<CP>(L.diag + it.o * 8u + c, dg);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ldpv<CP>(L.off + ((k & 1) ? nn.c[k] : it.o) * 24u + (k >> 1) * 8 + c, cf[k]);
+#pragma unroll
+        for (int q = 0; q < CP; ++q) {
+            const real cq[6] = {cf[0][q], cf[1][q], cf[2][q], cf[3][q], cf[4][q], cf[5][q]};
+            const real xq[6] = {xn[0][q], xn[1][q], xn[2][q], xn[3][q], xn[4][q], xn[5][q]};
+            const real acc = r[q] - offsum_v(cq, xq, n.e);
+            out[q] = dg[q] <= L.dmin ? RL(0.0) : acc / dg[q];
+        }
     }
-    stc(L.x, own, c, d <= L.dmin ? RL(0.0) : acc / d);
+    stcv<CP>(L.x, own, c, out);
 }
 
-// prolong_black_item of a black item.  The coarse values: the cell's own coarse cell and, along
-// each axis, the coarse cell across the face of the 2x2x2 block the cell touches (the +ax one for
-// an odd coordinate, the -ax one for an even), which holds one of its two neighbours.
-template <class LT>
+// prolong_black_item of a black item, components it.c .. it.c + CP - 1.  The coarse values: the
+// cell's own coarse cell and, along each axis, the coarse cell across the face of the 2x2x2 block
+// the cell touches (the +ax one for an odd coordinate, the -ax one for an even), which holds one
+// of its two neighbours.
+template <int CP, class LT>
 __device__ __forceinline__ void prolong_black_even(const LvlT<LT>& F, const LvlT<LT>& C, const LevelShape& s,
-                                                   const HalfItem& it, const Tab& tb) {
+                                                   const HalfItem& it, const Tab (&tb)[CP]) {
     const int c = it.c, I = it.I, J = it.J, K = it.K;
     const uint32_t own = s.half + it.h;
     const EvenNb n = even_neighbours(s, 1, it);
     const int f = F.sf[it.o];
-    const real r = ldc(F.r, own, c);
+    real r[CP];
+    ldcv<CP>(F.r, own, c, r);
     const int Ic = I >> 1, Jc = J >> 1, Kc = K >> 1, bI = I & 1, bJ = J & 1, bK = K & 1;
     const bool ox = bI ? n.e[0] : n.e[1], oy = bJ ? n.e[2] : n.e[3], oz = bK ? n.e[4] : n.e[5];
-    real xn[6];
+    real xn[6][CP];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xn[k] = ldc(F.x, n.s[k], c);
+    for (int k = 0; k < 6; ++k) ldcv<CP>(F.x, n.s[k], c, xn[k]);
     // storage of the coarse cells: colour-split like the fine level (its neighbours are of the
     // other colour) unless the coarse size is odd
     uint32_t s0, sx, sy, sz;
@@ -468,83 +535,122 @@ __device__ __forceinline__ void prolong_black_even(const LvlT<LT>& F, const LvlT
         sy = lsto(C.m, Kc, Jo, Ic);
         sz = lsto(C.m, Ko, Jc, Ic);
     }
-    const real xc0 = ldc(C.x, s0, c), xcx = ldc(C.x, sx, c), xcy = ldc(C.x, sy, c), xcz = ldc(C.x, sz, c);
-    const real xcn[6] = {bI ? xcx : xc0, bI ? xc0 : xcx, bJ ? xcy : xc0,
-                         bJ ? xc0 : xcy, bK ? xcz : xc0, bK ? xc0 : xcz};
-    real d, a;
+    real xc0[CP], xcx[CP], xcy[CP], xcz[CP];
+    ldcv<CP>(C.x, s0, c, xc0);
+    ldcv<CP>(C.x, sx, c, xcx);
+    ldcv<CP>(C.x, sy, c, xcy);
+    ldcv<CP>(C.x, sz, c, xcz);
+    real out[CP];
     if (__all(f != 0)) {
         // a standard cell has all six neighbours, each with diag > dmin
-        d = tb.d;
-        const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
-        real xpv[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) xpv[k] = as_stored<LT>(xn[k] + xcn[k]);
-        a = offsum_all(cf, xpv);
+        for (int q = 0; q < CP; ++q) {
+            const real xcn[6] = {bI ? xcx[q] : xc0[q], bI ? xc0[q] : xcx[q], bJ ? xcy[q] : xc0[q],
+                                 bJ ? xc0[q] : xcy[q], bK ? xcz[q] : xc0[q], bK ? xc0[q] : xcz[q]};
+            const real d = tb[q].d;
+            const real cf[6] = {tb[q].o[0], tb[q].o[0], tb[q].o[1], tb[q].o[1], tb[q].o[2], tb[q].o[2]};
+            real xpv[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) xpv[k] = as_stored<LT>(xn[k][q] + xcn[k]);
+            const real a = offsum_all(cf, xpv);
+            out[q] = d <= F.dmin ? RL(0.0) : (r[q] - a) / d;
+        }
     } else {
         const LNb nn = level_neighbours(F.m, it.o, K, J, I);
-        d = ldl(F.diag + it.o * 8u + c);
-        real cf[6], xpv[6];
+        real dg[CP], cf[6][CP], dn[6][CP];
+        ldpv<CP>(F.diag + it.o * 8u + c, dg);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            cf[k] = link_coef(F, nn, it.o, k, c);
-            // x of the neighbour after the constant prolongation (which skips diag <= dmin)
-            xpv[k] = ldl(F.diag + nn.c[k] * 8 + c) <= F.dmin ? xn[k] : as_stored<LT>(xn[k] + xcn[k]);
+            ldpv<CP>(F.off + ((k & 1) ? nn.c[k] : it.o) * 24u + (k >> 1) * 8 + c, cf[k]);
+            ldpv<CP>(F.diag + nn.c[k] * 8u + c, dn[k]);
         }
-        a = offsum_v(cf, xpv, n.e);
+#pragma unroll
+        for (int q = 0; q < CP; ++q) {
+            const real xcn[6] = {bI ? xcx[q] : xc0[q], bI ? xc0[q] : xcx[q], bJ ? xcy[q] : xc0[q],
+                                 bJ ? xc0[q] : xcy[q], bK ? xcz[q] : xc0[q], bK ? xc0[q] : xcz[q]};
+            real cq[6], xpv[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                cq[k] = cf[k][q];
+                // x of the neighbour after the constant prolongation (which skips diag <= dmin)
+                xpv[k] = dn[k][q] <= F.dmin ? xn[k][q] : as_stored<LT>(xn[k][q] + xcn[k]);
+            }
+            const real a = offsum_v(cq, xpv, n.e);
+            out[q] = dg[q] <= F.dmin ? RL(0.0) : (r[q] - a) / dg[q];
+        }
     }
-    stc(F.x, own, c, d <= F.dmin ? RL(0.0) : (r - a) / d);
+    stcv<CP>(F.x, own, c, out);
 }
 
-// restrict_item<true> onto coarse cell (K, J, I) from an even fine level.  Child k = 4 dz + 2 dy + dx
-// is fine cell (2K + dz, 2J + dy, 2I + dx) of colour (dx + dy + dz) & 1, stored at
-// colour m^3/2 + base(dz, dy) with base = ((2K + dz) m + 2J + dy) m/2 + I.  A child's neighbour
-// inside the block is a sibling (its x0 is that value); one outside is loaded from the other colour
-// at the offsets of even_neighbours.
-template <class LT>
+// restrict_item<true> onto coarse cell (K, J, I) from an even fine level, components c ..
+// c + CP - 1.  Child k = 4 dz + 2 dy + dx is fine cell (2K + dz, 2J + dy, 2I + dx) of colour
+// (dx + dy + dz) & 1, stored at colour m^3/2 + base(dz, dy) with base = ((2K + dz) m + 2J + dy)
+// m/2 + I.  A child's neighbour inside the block is a sibling (its x0 is that value); one outside
+// is loaded from the other colour at the offsets of even_neighbours.
+template <int CP, class LT>
 __device__ __forceinline__ void restrict_even(const LvlT<LT>& F, const LvlT<LT>& C, const LevelShape& s, int c, int K,
-                                              int J, int I, bool do_red, const Tab& tb) {
+                                              int J, int I, bool do_red, const Tab (&tb)[CP]) {
     const uint32_t m = s.m, mc = s.mh;
     uint32_t base[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) base[q] = ((2u * K + (q >> 1)) * m + 2u * J + (q & 1)) * s.mh + (uint32_t)I;
     const bool in_x[2] = {I > 0, (uint32_t)I + 1u < mc}, in_y[2] = {J > 0, (uint32_t)J + 1u < mc},
                in_z[2] = {K > 0, (uint32_t)K + 1u < mc};
-    real r[8], x0[8], xo[8][3];
+    real r[8][CP], x0[8][CP], xo[8][3][CP];
     int f = 1;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int dz = k >> 2, dy = (k >> 1) & 1, dx = k & 1, col = (dx + dy + dz) & 1;
         const uint32_t bs = base[k >> 1], st = (uint32_t)col * s.half + bs, sb = (uint32_t)(col ^ 1) * s.half + bs;
         f &= (int)F.sf[2u * bs + (uint32_t)dx];
-        r[k] = ldc(F.r, st, c);
-        x0[k] = ldc(F.x, st, c);
+        ldcv<CP>(F.r, st, c, r[k]);
+        ldcv<CP>(F.x, st, c, x0[k]);
         // the outer neighbour along each axis: -ax for an even offset, +ax for an odd one
         const uint32_t ax_x = in_x[dx] ? (dx ? sb + 1u : sb - 1u) : sb;
         const uint32_t ax_y = in_y[dy] ? (dy ? sb + s.mh : sb - s.mh) : sb;
         const uint32_t ax_z = in_z[dz] ? (dz ? sb + s.sz : sb - s.sz) : sb;
-        xo[k][0] = ldc(F.x, ax_x, c);
-        xo[k][1] = ldc(F.x, ax_y, c);
-        xo[k][2] = ldc(F.x, ax_z, c);
+        ldcv<CP>(F.x, ax_x, c, xo[k][0]);
+        ldcv<CP>(F.x, ax_y, c, xo[k][1]);
+        ldcv<CP>(F.x, ax_z, c, xo[k][2]);
     }
-    real xn[8][6];
     bool e[8][6];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int dz = k >> 2, dy = (k >> 1) & 1, dx = k & 1;
         // +ax, -ax along x, y, z (the order of level_neighbours)
-        xn[k][0] = dx ? xo[k][0] : x0[k | 1];  e[k][0] = dx ? in_x[1] : true;
-        xn[k][1] = dx ? x0[k & 6] : xo[k][0];  e[k][1] = dx ? true : in_x[0];
-        xn[k][2] = dy ? xo[k][1] : x0[k | 2];  e[k][2] = dy ? in_y[1] : true;
-        xn[k][3] = dy ? x0[k & 5] : xo[k][1];  e[k][3] = dy ? true : in_y[0];
-        xn[k][4] = dz ? xo[k][2] : x0[k | 4];  e[k][4] = dz ? in_z[1] : true;
-        xn[k][5] = dz ? x0[k & 3] : xo[k][2];  e[k][5] = dz ? true : in_z[0];
+        e[k][0] = dx ? in_x[1] : true;
+        e[k][1] = dx ? true : in_x[0];
+        e[k][2] = dy ? in_y[1] : true;
+        e[k][3] = dy ? true : in_y[0];
+        e[k][4] = dz ? in_z[1] : true;
+        e[k][5] = dz ? true : in_z[0];
     }
-    real res[8];
+    // neighbour j of child k, component q: a sibling's x0 or the loaded outer value
+    auto xnb = [&](int k, int j, int q) -> real {
+        const int dz = k >> 2, dy = (k >> 1) & 1, dx = k & 1;
+        switch (j) {
+            case 0: return dx ? xo[k][0][q] : x0[k | 1][q];
+            case 1: return dx ? x0[k & 6][q] : xo[k][0][q];
+            case 2: return dy ? xo[k][1][q] : x0[k | 2][q];
+            case 3: return dy ? x0[k & 5][q] : xo[k][1][q];
+            case 4: return dz ? xo[k][2][q] : x0[k | 4][q];
+            default: return dz ? x0[k & 3][q] : xo[k][2][q];
+        }
+    };
+    real res[CP][8];
     if (__all(f != 0)) {
         // standard children have all six neighbours
-        const real cf[6] = {tb.o[0], tb.o[0], tb.o[1], tb.o[1], tb.o[2], tb.o[2]};
 #pragma unroll
-        for (int k = 0; k < 8; ++k) res[k] = r[k] - tb.d * x0[k] - offsum_all(cf, xn[k]);
+        for (int q = 0; q < CP; ++q) {
+            const real cf[6] = {tb[q].o[0], tb[q].o[0], tb[q].o[1], tb[q].o[1], tb[q].o[2], tb[q].o[2]};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                real xq[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) xq[j] = xnb(k, j, q);
+                res[q][k] = r[k][q] - tb[q].d * x0[k][q] - offsum_all(cf, xq);
+            }
+        }
     } else {
         const uint32_t sy = m, sz = m * m;
 #pragma unroll
@@ -552,22 +658,39 @@ __device__ __forceinline__ void restrict_even(const LvlT<LT>& F, const LvlT<LT>&
             const uint32_t co = 2u * base[k >> 1] + (uint32_t)(k & 1);
             const uint32_t nb[6] = {e[k][0] ? co + 1u : co, e[k][1] ? co - 1u : co, e[k][2] ? co + sy : co,
                                     e[k][3] ? co - sy : co, e[k][4] ? co + sz : co, e[k][5] ? co - sz : co};
-            real cf[6];
+            real cf[6][CP], dg[CP];
 #pragma unroll
-            for (int j = 0; j < 6; ++j) cf[j] = ldl(F.off + ((j & 1) ? nb[j] : co) * 24 + (j >> 1) * 8 + c);
-            res[k] = r[k] - ldl(F.diag + co * 8 + c) * x0[k] - offsum_v(cf, xn[k], e[k]);
+            for (int j = 0; j < 6; ++j) ldpv<CP>(F.off + ((j & 1) ? nb[j] : co) * 24u + (j >> 1) * 8 + c, cf[j]);
+            ldpv<CP>(F.diag + co * 8u + c, dg);
+#pragma unroll
+            for (int q = 0; q < CP; ++q) {
+                real cq[6], xq[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    cq[j] = cf[j][q];
+                    xq[j] = xnb(k, j, q);
+                }
+                res[q][k] = r[k][q] - dg[q] * x0[k][q] - offsum_v(cq, xq, e[k]);
+            }
         }
     }
-    real acc = RL(0.0);
+    real acc[CP];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc += res[k];
-    acc = RL(0.125) * acc;
+    for (int q = 0; q < CP; ++q) {
+        real a = RL(0.0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a += res[q][k];
+        acc[q] = RL(0.125) * a;
+    }
     const uint32_t oc = lcell(C.m, K, J, I) * 8 + c, xc = lsto(C.m, K, J, I) * 8 + c;
-    stl(C.r + xc, acc);
+    stpv<CP>(C.r + xc, acc);
     if (do_red && ((I + J + K) & 1) == 0) {
         // the coarse red sweep reads the right-hand side as stored
-        const real d = ldl(C.diag + oc), ra = as_stored<LT>(acc);
-        stl(C.x + xc, d <= C.dmin ? RL(0.0) : ra / d);
+        real dg[CP], xr[CP];
+        ldpv<CP>(C.diag + oc, dg);
+#pragma unroll
+        for (int q = 0; q < CP; ++q) xr[q] = dg[q] <= C.dmin ? RL(0.0) : as_stored<LT>(acc[q]) / dg[q];
+        stpv<CP>(C.x + xc, xr);
     }
 }
 
@@ -587,11 +710,15 @@ __device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K
     J = (int)(q - (uint32_t)K * m);
 }
 
-// Level launches use at most LEVEL_GRID workgroups, each striding over the items of the launch in
-// passes of LEVEL_GRID * 256 (blocks renumbered per XCD within a pass).  A launch of a CG iteration
-// that has already stopped then costs the dispatch of that many workgroups, not of one per 256
-// items (32768 at level 1 of 256^3).
-constexpr unsigned LEVEL_GRID = 4096;
+// Level launches use at most LEVEL_GRID workgroups (the restriction RESTRICT_GRID), each striding
+// over the items of the launch in passes of that many * 256 (blocks renumbered per XCD within a
+// pass).  A launch of a CG iteration that has already stopped then costs the dispatch of that many
+// workgroups, not of one per 256 items.  Level 1 of 256^3 with FP16 storage: 8192 workgroups of
+// two items per thread ran the sweep in 19.8 us and the prolongation in 28.2 us, against 24.6 and
+// 34.0 us with 4096 of four and 20.9 and 30.1 us with 16384 of one; the restriction (one component
+// per lane) took 28.0 us with 4096 and 28.8 us with 8192.
+constexpr unsigned LEVEL_GRID = 8192;
+constexpr unsigned RESTRICT_GRID = 4096;
 
 // The logical blocks [0, T) of a striding level launch, split so that each XCD (workgroup b runs
 // on XCD b % 8) sweeps one contiguous eighth of them: the neighbour planes an item reads are then
@@ -621,33 +748,50 @@ static unsigned level_grid(size_t items) {
     return (unsigned)(b < 1 ? 1 : (b > LEVEL_GRID ? LEVEL_GRID : b));
 }
 
+// the level tables of the CP components of this thread (blockDim and every stride are multiples
+// of 8 / CP, so a thread's components are fixed)
+template <int CP, class LT>
+__device__ __forceinline__ void load_tabs(const LvlT<LT>& L, Tab (&tb)[CP]) {
+    const int c = (int)(threadIdx.x & ((8u / CP) - 1u)) * CP;
+#pragma unroll
+    for (int q = 0; q < CP; ++q) tb[q] = load_tab(L, c + q);
+}
+
 template <class LT>
 __global__ void __launch_bounds__(256) k_smooth_half(LvlT<LT> L, int color, const DevState* __restrict__ st, int k) {
+    constexpr int CP = level_cp<LT>();
     if (cg_inactive(st, k)) return;
     const LevelShape sh = level_shape(L.m);
-    const Tab tb = load_tab(L, (int)(threadIdx.x & 7u));   // blockDim is a multiple of 8: the component is fixed
-    const XcdRange rg = xcd_range((sh.half * 8u + blockDim.x - 1) / blockDim.x);
+    Tab tb[CP];
+    load_tabs<CP>(L, tb);
+    const XcdRange rg = xcd_range((sh.half * (8u / CP) + blockDim.x - 1) / blockDim.x);
     for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
         HalfItem it;
-        if (!half_item_even(sh, lb * blockDim.x + threadIdx.x, color, it)) return;
-        smooth_even(L, sh, color, it, tb);
+        if (!half_item_even<CP>(sh, lb * blockDim.x + threadIdx.x, color, it)) return;
+        smooth_even<CP>(L, sh, color, it, tb);
     }
 }
 
-// Restriction of the level below C.  For m >= 32 the coarse cells go by columns: logical block b
-// (XCD-renumbered) owns 32 cells of one row (8 components each) and consecutive logical blocks walk
-// up K, then J, then along I, so the fine planes two coarse planes share are read again while
-// they are still in the L2.  Smaller levels use the natural order.
+// Restriction of the level below C.  For m >= RESTRICT_ROW the coarse cells go by columns: logical
+// block b (XCD-renumbered) owns RESTRICT_ROW cells of one row (8 components each) and consecutive
+// logical blocks walk up K, then J, then along I, so the fine planes two coarse planes share are
+// read again while they are still in the L2.  Smaller levels use the natural order.
+constexpr int RESTRICT_CP = 1;
+constexpr int RESTRICT_ROW = 32 * RESTRICT_CP;   // coarse cells of a 256-thread workgroup
+
 template <class LT>
 __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do_red, const DevState* __restrict__ st,
                                                   int k) {
+    constexpr int CP = RESTRICT_CP;
+    constexpr uint32_t ROW = RESTRICT_ROW, SH = cp_shift<CP>();
     if (cg_inactive(st, k)) return;
     const LevelShape fs = level_shape(F.m), cs = level_shape(C.m);
     const int m = C.m;
-    const int c = (int)(threadIdx.x & 7u);   // blockDim and the stride are multiples of 8
-    const Tab tb = load_tab(F, c);
-    if (m >= 32) {
-        const uint32_t um = (uint32_t)m, total = um * um * ((um + 31u) / 32u);
+    const int c = (int)(threadIdx.x & ((8u / CP) - 1u)) * CP;   // blockDim and the stride are multiples of 8 / CP
+    Tab tb[CP];
+    load_tabs<CP>(F, tb);
+    if (m >= (int)ROW) {
+        const uint32_t um = (uint32_t)m, total = um * um * ((um + ROW - 1u) / ROW);
         const XcdRange rg = xcd_range(total);
         for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
             uint32_t col, seg;
@@ -660,38 +804,41 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
             }
             const int K = (int)(lb - col * um);
             const int J = (int)(col - seg * um);
-            const int I = (int)(seg * 32u + (threadIdx.x >> 3));
-            if (I < m) restrict_even(F, C, fs, c, K, J, I, do_red != 0, tb);
+            const int I = (int)(seg * ROW + (threadIdx.x >> SH));
+            if (I < m) restrict_even<CP>(F, C, fs, c, K, J, I, do_red != 0, tb);
         }
     } else {
-        const uint32_t items = (uint32_t)m * m * m * 8;
+        const uint32_t items = (uint32_t)m * m * m * (8u / CP);
         const XcdRange rg = xcd_range((items + blockDim.x - 1) / blockDim.x);
         for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
             const uint32_t t = lb * blockDim.x + threadIdx.x;
             if (t >= items) break;
             int K, J, I;
-            split_cell(cs, t >> 3, K, J, I);
-            restrict_even(F, C, fs, c, K, J, I, do_red != 0, tb);
+            split_cell(cs, t >> SH, K, J, I);
+            restrict_even<CP>(F, C, fs, c, K, J, I, do_red != 0, tb);
         }
     }
 }
 
 // workgroups of a k_restrict launch onto coarse size m
 static unsigned restrict_blocks(int m) {
-    if (m >= 32) return std::min((unsigned)m * (unsigned)m * (unsigned)((m + 31) / 32), LEVEL_GRID);
-    return level_grid((size_t)m * m * m * 8);
+    constexpr int ROW = RESTRICT_ROW;
+    if (m >= ROW) return std::min((unsigned)m * (unsigned)m * (unsigned)((m + ROW - 1) / ROW), RESTRICT_GRID);
+    return std::min(level_grid((size_t)m * m * m * (8 / RESTRICT_CP)), RESTRICT_GRID);
 }
 
 template <class LT>
 __global__ void __launch_bounds__(256) k_prolong_black(LvlT<LT> F, LvlT<LT> C, const DevState* __restrict__ st, int k) {
+    constexpr int CP = level_cp<LT>();
     if (cg_inactive(st, k)) return;
     const LevelShape sh = level_shape(F.m);
-    const Tab tb = load_tab(F, (int)(threadIdx.x & 7u));   // blockDim is a multiple of 8: the component is fixed
-    const XcdRange rg = xcd_range((sh.half * 8u + blockDim.x - 1) / blockDim.x);
+    Tab tb[CP];
+    load_tabs<CP>(F, tb);
+    const XcdRange rg = xcd_range((sh.half * (8u / CP) + blockDim.x - 1) / blockDim.x);
     for (uint32_t lb = rg.first; lb < rg.end; lb += rg.step) {
         HalfItem it;
-        if (!half_item_even(sh, lb * blockDim.x + threadIdx.x, 1, it)) return;
-        prolong_black_even(F, C, sh, it, tb);
+        if (!half_item_even<CP>(sh, lb * blockDim.x + threadIdx.x, 1, it)) return;
+        prolong_black_even<CP>(F, C, sh, it, tb);
     }
 }
 
@@ -915,15 +1062,16 @@ namespace vfx {
 void Pressure::vcycle_levels(int k, hipStream_t s) {
     with_mg_type(mg_kind, [&](auto tag) {
         using LT = decltype(tag);
+        constexpr int CP = level_cp<LT>();
         for (int l = 0; l < S; ++l) {
-            const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
+            const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * (8 / CP);
             hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 1, st, k);
             hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[l + 1].m)), dim3(256), 0, s,
                                lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), (l + 1 < S) ? 1 : 0, st, k);
         }
         launch_coarse<LT>(*this, s, k);
         for (int l = S - 1; l >= 0; --l) {
-            const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * 8;
+            const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * (8 / CP);
             hipLaunchKernelGGL(k_prolong_black<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]),
                                lvl_as<LT>(lv[l + 1]), st, k);
             hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 0, st, k);
@@ -934,11 +1082,11 @@ void Pressure::vcycle_levels(int k, hipStream_t s) {
 // Multigrid kernels for timing (ids relative to VFX_K_MATVEC).
 bool Pressure::launch_single_mg(int which, hipStream_t s) {
     const size_t m0 = (size_t)lv[0].m;
-    const size_t half0 = m0 * m0 * (m0 / 2) * 8;
     if (which != 10 && (which < 2 || which > 4)) return false;
     if (which != 10 && S < 1) return false;
     with_mg_type(mg_kind, [&](auto tag) {
         using LT = decltype(tag);
+        const size_t half0 = m0 * m0 * (m0 / 2) * (8 / level_cp<LT>());
         switch (which) {
             case 2:
                 hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half0)), dim3(256), 0, s, lvl_as<LT>(lv[0]), 1,
