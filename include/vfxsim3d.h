@@ -86,7 +86,8 @@ typedef struct vfx_ctx vfx_ctx;
 int vfx_p_size(void);
 
 /* Allocate every buffer of a res^3 solver (Solver3D.__init__, solver.py:86-125;
- * PressureSolver3D.__init__, pressure.py:789-818).  res must be even (pressure.py:790). */
+ * PressureSolver3D.__init__, pressure.py:789-818).  res must be even (pressure.py:790) and at
+ * most 512 (VFX_E_ARG above: the kernels' channel offsets c res^3 + i are 32-bit). */
 int vfx_ctx_create(int res, vfx_ctx** out);
 int vfx_ctx_destroy(vfx_ctx* ctx);
 

@@ -762,7 +762,8 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     if (out == nullptr) return arg_error("out is NULL");
     *out = nullptr;
     if (res < 2 || res % 2 != 0) return arg_error("even grid size >= 2 required");
-    if (res > 1024) return arg_error("grid size above 1024");
+    // the kernels index channel c of a field as c N^3 + i in 32-bit integers: 6 N^3 < 2^31 up to 512
+    if (res > 512) return arg_error("grid size above 512 (channel offsets are 32-bit)");
     vfx_ctx* c = new (std::nothrow) vfx_ctx();
     if (!c) return arg_error("out of host memory");
     const int N = res;

@@ -118,9 +118,6 @@ __device__ __forceinline__ void curl_at(const real* __restrict__ data, const uin
     w[2] = (dvdx - dudy) * inv2dx;
 }
 
-// |omega_s| at cell i: channel 3 of the smoothed vorticity (k_smooth3<0> stores
-// sqrt(a * a + b * b + c * c) of the values it writes, the expression of kernels.py:362-365)
-__device__ __forceinline__ real wmag(const real* __restrict__ o, size_t n3, size_t i) { return o[3 * n3 + i]; }
 
 // the parameters body_force_at reads
 struct BodyPrm {
@@ -138,25 +135,30 @@ __device__ __forceinline__ BodyPrm body_prm(const float* __restrict__ prm) {
 }
 
 // body_forces (kernels.py:346-387) at fluid cell (x, y, z): buoyancy + vorticity confinement +
-// wind marker force from the state f and the smoothed vorticity om (4 channels)
+// wind marker force from the state f and the smoothed vorticity om (4 channels; channel 3 holds
+// |omega_s| = sqrt(a * a + b * b + c * c) of the values k_smooth3<0> wrote, the expression of
+// kernels.py:342-343).  Indices within a channel are 32-bit (N <= 512), channel bases are
+// pointers.  The gradient's divisor max(xr - xl, 1) is 1 or 2, and a quotient by 1 or 2 equals
+// the product by 1 or 0.5 exactly, so it is computed as that product.
 __device__ __forceinline__ void body_force_at(const real* __restrict__ f, const real* __restrict__ om,
                                               const BodyPrm& b, int N, int x, int y, int z, real (&F)[3]) {
     const size_t n3 = (size_t)N * N * N;
-    const size_t sz = (size_t)N * N;
-    const size_t i = idx3(N, z, y, x);
-    const real d = f[i], T = f[4 * n3 + i];
+    const int sz = N * N;
+    const int i = (z * N + y) * N + x;
+    const real* __restrict__ wm = om + 3 * n3;
+    const real d = f[i], T = (f + 4 * n3)[i];
     real fx = RL(0.0), fy = -b.alpha * d + b.beta * (T - b.amb), fz = RL(0.0);
     if (b.eps != RL(0.0)) {
         const int xr = min(x + 1, N - 1), xl = max(x - 1, 0);
         const int yt = min(y + 1, N - 1), yb = max(y - 1, 0);
         const int zf = min(z + 1, N - 1), zb = max(z - 1, 0);
-        const size_t rowb = (size_t)z * sz + (size_t)y * N;
-        const real gx = (wmag(om, n3, rowb + xr) - wmag(om, n3, rowb + xl)) / (real)max(xr - xl, 1);
-        const real gy = (wmag(om, n3, (size_t)z * sz + (size_t)yt * N + x) -
-                         wmag(om, n3, (size_t)z * sz + (size_t)yb * N + x)) / (real)max(yt - yb, 1);
-        const real gz = (wmag(om, n3, (size_t)zf * sz + (size_t)y * N + x) -
-                         wmag(om, n3, (size_t)zb * sz + (size_t)y * N + x)) / (real)max(zf - zb, 1);
-        const real wx = om[i], wy = om[n3 + i], wz = om[2 * n3 + i];
+        const int rowb = z * sz + y * N;
+        const real hx = xr - xl == 2 ? RL(0.5) : RL(1.0), hy = yt - yb == 2 ? RL(0.5) : RL(1.0);
+        const real hz = zf - zb == 2 ? RL(0.5) : RL(1.0);
+        const real gx = (wm[rowb + xr] - wm[rowb + xl]) * hx;
+        const real gy = (wm[z * sz + yt * N + x] - wm[z * sz + yb * N + x]) * hy;
+        const real gz = (wm[zf * sz + y * N + x] - wm[zb * sz + y * N + x]) * hz;
+        const real wx = om[i], wy = (om + n3)[i], wz = (om + 2 * n3)[i];
         const real fl = b.cfloor * sqrt(wx * wx + wy * wy + wz * wz);
         const real mag = sqrt(gx * gx + gy * gy + gz * gz + fl * fl) + RL(1.0e-20);
         const real k = b.eps * b.dxp / mag;
@@ -165,7 +167,7 @@ __device__ __forceinline__ void body_force_at(const real* __restrict__ f, const 
         fz += k * (gx * wy - gy * wx);
     }
     if (b.wind) {
-        const real mk = clampf_(d + f[5 * n3 + i] + (T - b.amb) / b.wmark, RL(0.0), RL(1.0));
+        const real mk = clampf_(d + (f + 5 * n3)[i] + (T - b.amb) / b.wmark, RL(0.0), RL(1.0));
         fx += b.wdx * mk;
         fy += b.wdy * mk;
         fz += b.wdz * mk;
@@ -315,7 +317,13 @@ __global__ void __launch_bounds__(256) k_smooth3(int N, const real* __restrict__
                     sv[c] = u;
                 }
                 real r0 = RL(0.0), r1 = RL(0.0), r2 = RL(0.0);
-                if (sv[3] > RL(0.0)) { r0 = sv[0] / sv[3]; r1 = sv[1] / sv[3]; r2 = sv[2] / sv[3]; }
+                if (__all(sv[3] == RL(64.0))) {
+                    // every cell of the 3x3x3 neighbourhood is fluid: the weight sum is 4^3, and a
+                    // quotient by 64 equals the product by 1/64 exactly
+                    r0 = sv[0] * RL(0.015625); r1 = sv[1] * RL(0.015625); r2 = sv[2] * RL(0.015625);
+                } else if (sv[3] > RL(0.0)) {
+                    r0 = sv[0] / sv[3]; r1 = sv[1] / sv[3]; r2 = sv[2] / sv[3];
+                }
                 const int i = (zo * N + y) * N + x;
                 if (MODE == 0) {
                     out[i] = r0; out[n3 + i] = r1; out[2 * n3 + i] = r2;

@@ -57,6 +57,9 @@ def test_p_size_and_error_string(hiplib):
     assert "even" in _lib.last_error()
     with pytest.raises(AssertionError):
         _lib.check(rc)
+    # above 512 the kernels' 32-bit channel offsets (c N^3 + i) would overflow
+    rc = hiplib.vfx_ctx_create(1024, ctypes.byref(ctypes.c_void_p()))
+    assert rc == 1 and "512" in _lib.last_error()
 
 
 def test_api_surface():
