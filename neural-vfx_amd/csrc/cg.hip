@@ -201,12 +201,11 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
     if (gated && !st->live) return;
     // the three reductions in one pass (each in reduce_parts' order): a third of the load rounds
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-#pragma unroll 8
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-        v0 = fmax(v0, part[i]);
-        v1 += part[nparts + i];
-        v2 = fmax(v2, part[2 * nparts + i]);
-    }
+    for_parts<3, 16>(part, nparts, {0, nparts, 2 * nparts}, [&](const double (&x)[3]) {
+        v0 = fmax(v0, x[0]);
+        v1 += x[1];
+        v2 = fmax(v2, x[2]);
+    });
     const double mx = block_max(v0, sh);
     const double bb = block_sum(v1, sh);
     const double mr = block_max(v2, sh);

@@ -198,16 +198,36 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 // order for a fixed n.
 constexpr int FIN_THREADS = 1024;
 
+// Block partials of NA arrays (array a at p + off[a]), n each: thread t uses i = t, t + blockDim,
+// t + 2 blockDim, ... in that order, and issues the reads of CH of them before it uses any (each
+// read is a round trip to the L2 or HBM; a single workgroup reducing 65536 partials one round of
+// 8 at a time took 20-30 us).  Indices past n are read at n - 1 and not used.
+template <int NA, int CH, class F>
+__device__ __forceinline__ void for_parts(const double* __restrict__ p, int n, const int (&off)[NA], F&& use) {
+    const int B = (int)blockDim.x;
+    for (int i0 = (int)threadIdx.x; i0 < n; i0 += CH * B) {
+        double v[CH][NA];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int i = min(i0 + j * B, n - 1);
+#pragma unroll
+            for (int a = 0; a < NA; ++a) v[j][a] = p[off[a] + i];
+        }
+        __builtin_amdgcn_sched_barrier(0);   // every read above is issued before the first use
+#pragma unroll
+        for (int j = 0; j < CH; ++j)
+            if (i0 + j * B < n) use(v[j]);
+    }
+}
+
 __device__ __forceinline__ double reduce_parts_sum(const double* p, int n, double* sh) {
     double v = 0.0;
-#pragma unroll 8
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
+    for_parts<1, 32>(p, n, {0}, [&](const double (&x)[1]) { v += x[0]; });
     return block_sum(v, sh);
 }
 __device__ __forceinline__ double reduce_parts_max(const double* p, int n, double* sh) {
     double v = 0.0;
-#pragma unroll 8
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, p[i]);
+    for_parts<1, 32>(p, n, {0}, [&](const double (&x)[1]) { v = fmax(v, x[0]); });
     return block_max(v, sh);
 }
 
