@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restri
                                              const uint8_t* __restrict__ fm, real* __restrict__ r,
                                              real* __restrict__ phi, double* __restrict__ part, int nparts,
                                              const DevState* __restrict__ st, int gated) {
-    __shared__ double sh[8];
+    __shared__ double sh[3 * 8];
     if (gated && !st->live) return;
     const int n3 = N * N * N;
     double mx = 0.0, bb = 0.0, mr = 0.0;
@@ -182,9 +182,9 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restri
         mr = fl ? fmax(mr, fabs((double)rv)) : mr;
         mx = fl ? fmax(mx, sp) : mx;
     CELL_LOOP_END
-    const double a = block_max(mx, sh);
-    const double b2 = block_sum(bb, sh);
-    const double c = block_max(mr, sh);
+    double v3[3] = {mx, bb, mr};
+    block_reduce<3, 0b101u>(v3, sh);
+    const double a = v3[0], b2 = v3[1], c = v3[2];
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         part[block_linear()] = a;
         part[nparts + block_linear()] = b2;
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256) k_rhs(int N, int zc, const real* __restri
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restrict__ part, int nparts,
                                                  const float* __restrict__ prm, DevState* __restrict__ st,
                                                  int gated) {
-    __shared__ double sh[FIN_THREADS / 64];
+    __shared__ double sh[3 * FIN_THREADS / 64];
     if (gated && !st->live) return;
     // the three reductions in one pass (each in reduce_parts' order): a third of the load rounds
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
@@ -206,9 +206,9 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_rhs(const double* __restric
         v1 += x[1];
         v2 = fmax(v2, x[2]);
     });
-    const double mx = block_max(v0, sh);
-    const double bb = block_sum(v1, sh);
-    const double mr = block_max(v2, sh);
+    double v3[3] = {v0, v1, v2};
+    block_reduce<3, 0b101u>(v3, sh);
+    const double mx = v3[0], bb = v3[1], mr = v3[2];
     if (threadIdx.x == 0) {
         const float tol = prm[P_TOL] > 1.0e-6f ? prm[P_TOL] : 1.0e-6f;
         st->s[S_LSCALE] = pow2_scale(mr);
@@ -299,9 +299,9 @@ __device__ void fin_rz(const double* __restrict__ part, int nparts, const float*
         vb = fmax(vb, x[1]);
         vc += x[2];
     });
-    const double rz_new = block_sum(va, sh);
-    const double mr = block_max(vb, sh);
-    const double rr = block_sum(vc, sh);
+    double v3[3] = {va, vb, vc};
+    block_reduce<3, 0b010u>(v3, sh);
+    const double rz_new = v3[0], mr = v3[1], rr = v3[2];
     if (threadIdx.x != 0) return;
     double best = mr;
     if (k == 0) {
@@ -344,8 +344,9 @@ __device__ void fin_r(const double* __restrict__ part, int nparts, const float* 
         vb = fmax(vb, x[0]);
         vc += x[1];
     });
-    const double mr = block_max(vb, sh);
-    const double rr = block_sum(vc, sh);
+    double v2[2] = {vb, vc};
+    block_reduce<2, 0b01u>(v2, sh);
+    const double mr = v2[0], rr = v2[1];
     if (threadIdx.x != 0) return;
     const double rel_tol = (double)prm[P_REL_TOL];
     const bool not_conv = rel_tol > 0.0 ? sqrt(rr) > rel_tol * sqrt(st->s[S_BB]) : mr > st->s[S_THR];
@@ -376,7 +377,7 @@ template <bool LAST>
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_r(const double* __restrict__ part, int nparts,
                                                const float* __restrict__ prm, DevState* __restrict__ st, int k,
                                                int hitbit) {
-    __shared__ double sh[FIN_THREADS / 64];
+    __shared__ double sh[2 * FIN_THREADS / 64];
     if (cg_inactive(st, k)) return;
     fin_r<LAST>(part, nparts, prm, st, k, hitbit, sh);
 }
@@ -384,7 +385,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_fin_r(const double* __restrict_
 __global__ void __launch_bounds__(FIN_THREADS) k_fin_rz(const double* __restrict__ part, int nparts,
                                                 const float* __restrict__ prm, DevState* __restrict__ st, int k,
                                                 int kmax, int hitbit) {
-    __shared__ double sh[FIN_THREADS / 64];
+    __shared__ double sh[3 * FIN_THREADS / 64];
     if (cg_inactive(st, k)) return;
     fin_rz(part, nparts, prm, st, k, kmax, hitbit, sh);
 }
@@ -608,7 +609,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
     __shared__ __align__(16) real sz[HN];
     __shared__ __align__(16) real sr[ON];   // r of the own region only (the halo needs z0 and the mask)
     __shared__ __align__(16) uint8_t sm[HN];
-    __shared__ double sh[TILE_THREADS / 64];
+    __shared__ double sh[2 * TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
     const real a = (real)st->s[S_ALPHA];
@@ -685,8 +686,9 @@ __global__ void __launch_bounds__(TILE_THREADS) k_pre_resid_fwd(int N, const rea
             mx = fmax(mx, fabs((double)rv));
             rr2 += (double)rv * (double)rv;
         }
-    const double bmx = block_max(mx, sh);
-    const double brr = block_sum(rr2, sh);
+    double v2[2] = {mx, rr2};
+    block_reduce<2, 0b01u>(v2, sh);
+    const double bmx = v2[0], brr = v2[1];
     if (threadIdx.x == 0) {
         put_part(part + nparts + tl.lin, bmx);
         put_part(part + 2 * nparts + tl.lin, brr);
@@ -743,7 +745,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
     __shared__ __align__(16) real szt[HN];
     __shared__ __align__(16) real sr[ON];   // r of the own region only
     __shared__ __align__(16) uint8_t sm[HN];
-    __shared__ double sh[TILE_THREADS / 64];
+    __shared__ double sh[3 * TILE_THREADS / 64];
     if (cg_inactive(st, k)) return;
     const Tile tl = this_tile(N);
     const int t = (int)threadIdx.x, n3 = N * N * N;
@@ -833,9 +835,9 @@ __global__ void __launch_bounds__(TILE_THREADS) k_inv_post(int N, LvlT<LT> L, co
             if (k == 0) st2(p_init, i, zv[0], zv[1]);
         }
     }
-    const double a = block_sum(s, sh);
-    const double b = block_max(mx, sh);
-    const double c = block_sum(rr, sh);
+    double v3[3] = {s, mx, rr};
+    block_reduce<3, 0b010u>(v3, sh);
+    const double a = v3[0], b = v3[1], c = v3[2];
     if (threadIdx.x == 0) {
         const int bl = tl.lin;
         put_part(part + bl, a);

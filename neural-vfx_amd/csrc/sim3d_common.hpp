@@ -193,6 +193,35 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return r;
 }
 
+// NV block reductions in one pass: value j is a sum (bit j of MAXMASK clear) or a max (set), each
+// with block_sum's / block_max's butterfly and wave order, so the results are theirs bit for bit,
+// behind two barriers instead of two per value.  Results valid in thread 0; `sh` needs NV
+// doubles per wave.
+template <int NV, unsigned MAXMASK>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double* sh) {
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const double u = __shfl_xor(v[j], o, 64);
+            v[j] = ((MAXMASK >> j) & 1u) ? fmax(v[j], u) : v[j] + u;
+        }
+    __syncthreads();
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) sh[j * nw + (t >> 6)] = v[j];
+    __syncthreads();
+    if (t == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double r = 0.0;
+            for (int w = 0; w < nw; ++w) r = ((MAXMASK >> j) & 1u) ? fmax(r, sh[j * nw + w]) : r + sh[j * nw + w];
+            v[j] = r;
+        }
+}
+
 // Final reductions of block partials run in one workgroup of FIN_THREADS threads; the strided
 // loop is unrolled so that a thread's loads are in flight together.  Result in thread 0, fixed
 // order for a fixed n.
