@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
                                                 const real* __restrict__ hat, const int* __restrict__ qidx,
                                                 real* __restrict__ out, const uint8_t* __restrict__ fm,
                                                 double* __restrict__ part, int nparts) {
-    __shared__ double sh[8];
+    __shared__ double sh[2 * 8];
     const int n3 = N * N * N;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -128,8 +128,9 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
         }
     }
     if (part != nullptr) {
-        const double a = block_sum(sa, sh);
-        const double b = block_sum(sb, sh);
+        double v2[2] = {sa, sb};
+        block_reduce<2, 0u>(v2, sh);
+        const double a = v2[0], b = v2[1];
         if (threadIdx.x == 0 && threadIdx.y == 0) {
             const int bl = block_linear();
             part[bl] = a;
@@ -141,7 +142,7 @@ __global__ void __launch_bounds__(256) k_adv_mc(int N, const real* __restrict__ 
 // First reduction level: block b sums partials [b*per, (b+1)*per) of both halves.
 __global__ void __launch_bounds__(256) k_mass_part2(const double* __restrict__ part, int nparts, int per,
                                                     double* __restrict__ part2, int n2) {
-    __shared__ double sh[4];
+    __shared__ double sh[2 * 4];
     const int b = blockIdx.x;
     const int lo = b * per, hi = min(lo + per, nparts);
     double a = 0.0, c = 0.0;
@@ -149,8 +150,9 @@ __global__ void __launch_bounds__(256) k_mass_part2(const double* __restrict__ p
         a += part[i];
         c += part[nparts + i];
     }
-    const double ra = block_sum(a, sh);
-    const double rc = block_sum(c, sh);
+    double v2[2] = {a, c};
+    block_reduce<2, 0u>(v2, sh);
+    const double ra = v2[0], rc = v2[1];
     if (threadIdx.x == 0) {
         part2[b] = ra;
         part2[n2 + b] = rc;
@@ -160,9 +162,15 @@ __global__ void __launch_bounds__(256) k_mass_part2(const double* __restrict__ p
 // ratio = a / b if a > 0 and b > 1e-30, else 1 (solver.py:67-74).
 __global__ void __launch_bounds__(FIN_THREADS) k_mass_fin(const double* __restrict__ part, int nparts,
                                                           DevState* __restrict__ st, int slot) {
-    __shared__ double sh[FIN_THREADS / 64];
-    double a = reduce_parts_sum(part, nparts, sh);
-    double b = reduce_parts_sum(part + nparts, nparts, sh);
+    __shared__ double sh[2 * FIN_THREADS / 64];
+    // both sums in one pass, each in reduce_parts_sum's order
+    double v2[2] = {0.0, 0.0};
+    for_parts<2, 16>(part, nparts, {0, nparts}, [&](const double (&x)[2]) {
+        v2[0] += x[0];
+        v2[1] += x[1];
+    });
+    block_reduce<2, 0u>(v2, sh);
+    const double a = v2[0], b = v2[1];
     if (threadIdx.x == 0) st->s[slot] = (a > 0.0 && b > 1e-30) ? a / b : 1.0;
 }
 

@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __res
                                                  real inv_dx, real a, real ia, real b, const real* __restrict__ hat,
                                                  const int* __restrict__ qidx, const uint8_t* __restrict__ ffm,
                                                  real* __restrict__ out, double* __restrict__ part, int nparts) {
-    __shared__ double sh[8];
+    __shared__ double sh[2 * 8];
     __shared__ real vt[3 * VT_N];
     const size_t n3 = (size_t)S * S * S;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -192,8 +192,9 @@ __global__ void __launch_bounds__(256) k_fadv_mc(int S, int N, const real* __res
             sb = (double)v0;
         }
     }
-    const double ra = block_sum(sa, sh);
-    const double rb = block_sum(sb, sh);
+    double v2[2] = {sa, sb};
+    block_reduce<2, 0u>(v2, sh);
+    const double ra = v2[0], rb = v2[1];
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const int bl = block_linear();
         part[bl] = ra;
