@@ -12,9 +12,8 @@
  * Every function returns 0 on success or a VFX_E_* code; vfx_last_error() gives the message.
  * The library allocates device memory only in vfx_ctx_create and vfx_hires_attach; calls on one
  * context must not overlap (one context per Solver3D, as the reference keeps one solver per grid).
- * vfx_step does not wait for the device (VFX_OPT_SYNC): the host runs ahead by up to 16 steps (a
- * ring of pinned parameter slots, the reference's 4-slot ring at solver.py:331-335).  vfx_project,
- * a standalone projection of an arbitrary field, runs its PCG loop host-polled to the tolerance.
+ * vfx_step and vfx_project do not wait for the device (VFX_OPT_SYNC): the host runs ahead by up to
+ * 16 steps (a ring of pinned parameter slots, the reference's 4-slot ring at solver.py:331-335).
  */
 #ifndef VFXSIM3D_H
 #define VFXSIM3D_H
@@ -64,7 +63,7 @@ extern "C" {
 #define VFX_OPT_FUSED_SMOOTH 13  /* fused: exact-K sweeps before / between / after (pressure_smooth, 4) */
 #define VFX_OPT_FUSED_SMOOTHER 14 /* fused: 0 Chebyshev weights on [0.5, 2] (default), 1 Jacobi 0.8 */
 #define VFX_OPT_SYNC 15          /* projection sequencing.  0 (default) device: no host synchronisation
-                                    in vfx_step (the reference's on-device loop,
+                                    in vfx_step / vfx_project (the reference's on-device loop,
                                     pressure.py:940-950).  The host enqueues an iteration budget (the
                                     kernels after the device's stop test are no-ops), and with restarts
                                     on the true residual, a device-side restart decision and one gated
@@ -99,9 +98,7 @@ int vfx_step(vfx_ctx* ctx, float* data, const float* solid, const float* prm_hos
  * stage uploads prm_host; the later stages reuse the uploaded parameters (prm_host may be NULL). */
 int vfx_step_stage(vfx_ctx* ctx, int stage, float* data, const float* solid, const float* prm_host, void* stream);
 
-/* Projection only, with dt = 0 forces (Solver3D.project, solver.py:266-272).  Host-polled PCG
- * whatever VFX_OPT_SYNC says: the field is arbitrary, so there are no recent iteration counts to
- * budget from, and the projection runs to P_REL_TOL / P_TOL or P_MAXIT like the reference's. */
+/* Projection only, with dt = 0 forces (Solver3D.project, solver.py:266-272). */
 int vfx_project(vfx_ctx* ctx, float* data, const float* solid, const float* prm_host, void* stream);
 
 /* MacCormack advection of C channels by `vel` (3,N,N,N), no obstacles, no mass correction

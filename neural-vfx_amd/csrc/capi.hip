@@ -518,10 +518,10 @@ static int project_learned_device(vfx_ctx* c, real* vel, hipStream_t s) {
 }
 #endif
 
-int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame, bool poll) {
+int vfx::stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame) {
     Pressure& P = c->P;
     real* vel = data + c->g.n3;
-    const bool device = in_frame || (c->sync_mode == SYNC_DEVICE && !poll);
+    const bool device = in_frame || c->sync_mode == SYNC_DEVICE;
     c->last_iters = 0;
     c->last_restarts = 0;
     c->last_true_valid = false;
@@ -993,9 +993,7 @@ int VFX_FN(vfx_project)(vfx_ctx* c, real* data, const real* solid, const float* 
     if (int rc = drain_readbacks(c, false)) return rc;
     c->P.setup(solid, s);
     CHECK_LAUNCH("obstacle setup");
-    // A standalone projection of an arbitrary field has no recent iteration counts to budget
-    // from: it runs host-polled to the tolerance (or P_MAXIT), as the reference's does.
-    if (int rc = stage_project(c, data, s, false, true)) return rc;
+    if (int rc = stage_project(c, data, s)) return rc;
     return after_projection(c, s);
 }
 

@@ -176,8 +176,7 @@ int stage_advect(vfx_ctx* c, real* data, const real* solid, hipStream_t s);
 int stage_forces(vfx_ctx* c, real* data, hipStream_t s);
 // in_frame: part of a batched frame (the frame's CG budget, device-side sequencing; the caller
 // records done_ev).  Outside frames the caller follows it with the completion event and read-back.
-// poll: host-polled PCG loop whatever VFX_OPT_SYNC says (a standalone projection, vfx_project)
-int stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame = false, bool poll = false);
+int stage_project(vfx_ctx* c, real* data, hipStream_t s, bool in_frame = false);
 // destroy the context's captured step graphs (after a change they cannot see, e.g. new fine buffers)
 void drop_graphs(vfx_ctx* c);
 // iteration budget of the first CG run / of the gated restart run for P_MAXIT = maxit
