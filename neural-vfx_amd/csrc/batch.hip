@@ -10,7 +10,7 @@
 // which gives per-simulation completion times inside the frame.  After a frame the device gathers
 // every simulation's PCG state into a pinned buffer; vfx_batch_wait reads it.  A budget starts at
 // min(P_MAXIT, 32); after each waited frame it is at least the largest iteration count of the
-// simulation's last 8 frames plus 2, and a simulation that stopped at its budget unconverged gets
+// simulation's last 8 frames plus 1, and a simulation that stopped at its budget unconverged gets
 // twice the budget for the next frame (a restart run that stopped at its budget: twice its own).
 // Budgets grow at once; once 8 frames have been seen they also shrink to that count when they sit 2 or more above it (iterations past the stop are
 // launched and return at once, so a loose budget costs launches).
@@ -36,7 +36,7 @@ namespace vfx {
 
 constexpr int BATCH_MAX = 4096;
 constexpr int BATCH_MAX_STREAMS = 16;
-constexpr int BUDGET_MARGIN = 2;   // iterations kept beyond the largest count seen
+constexpr int BUDGET_MARGIN = 1;   // iterations kept beyond the largest count seen
 constexpr int BUDGET_FIRST = 32;   // first budget of a simulation (capped by its P_MAXIT)
 constexpr int BUDGET_SHRINK = 2;   // a budget shrinks when it is this far above the window's need
 constexpr int FRAME_STATS = 9;     // per simulation: iterations, stop_at, r.r, b.b, max|r|, threshold,
