@@ -139,15 +139,16 @@ def test_fp32_against_fp64_run(cuda, hiplib, storage):
         assert el2 <= 1e-2, name
 
 
-@pytest.mark.parametrize("N", [48, 64])
-@pytest.mark.parametrize("storage,tol", [("fp16", 2e-3), ("bf16", 2e-2)])
+@pytest.mark.parametrize("N,storage,tol", [(48, "fp16", 2e-3), (48, "bf16", 2e-2), (64, "fp16", 2e-3),
+                                           (64, "bf16", 2e-2), (256, "fp16", 2e-3)])
 def test_mg16_precond(cuda, hiplib, storage, tol, N):
     """A V-cycle with 16-bit level storage matches the oracle's 16-bit storage path
     (oracle.precond_apply16: every level value rounded where it is stored, the level-1 right-hand
     side scaled by a power of two from max|r|) to 1e-6, stays close to the FP32 one, and its
     standard-stencil path is bit-identical to reading every coefficient from the level arrays.
     N = 48 has even levels of sizes that are not powers of two (24, 12, 6; the item decomposition
-    divides) and an odd coarsest level (3); N = 64 has power-of-two levels down to 1."""
+    divides) and an odd coarsest level (3); N = 64 has power-of-two levels down to 1; at N = 256
+    the level-1 launches stride over their items in two passes."""
     _say(f"precond with {storage} levels N={N}")
     solid = sphere_solid(N)
     fm = oracle.mask(solid)
