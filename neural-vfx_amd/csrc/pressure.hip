@@ -710,14 +710,16 @@ __device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K
     J = (int)(q - (uint32_t)K * m);
 }
 
-// Level launches use at most LEVEL_GRID workgroups (the restriction RESTRICT_GRID), each striding
-// over the items of the launch in passes of that many * 256 (blocks renumbered per XCD within a
-// pass).  A launch of a CG iteration that has already stopped then costs the dispatch of that many
-// workgroups, not of one per 256 items.  Level 1 of 256^3 with FP16 storage: 8192 workgroups of
-// two items per thread ran the sweep in 19.8 us and the prolongation in 28.2 us, against 24.6 and
-// 34.0 us with 4096 of four and 20.9 and 30.1 us with 16384 of one; the restriction (one component
-// per lane) took 28.0 us with 4096 and 28.8 us with 8192.
-constexpr unsigned LEVEL_GRID = 8192;
+// Level launches use at most LEVEL_GRID / CP workgroups (the restriction RESTRICT_GRID), each
+// striding over the items of the launch in passes of that many * 256 (blocks renumbered per XCD
+// within a pass): two items per thread at level 1 of 256^3 for either storage.  A launch of a CG
+// iteration that has already stopped then costs the dispatch of that many workgroups, not of one
+// per 256 items.  Level 1 of 256^3 with FP16 storage (CP = 2): 8192 workgroups ran the sweep in
+// 19.8 us and the prolongation in 28.2 us, against 24.6 and 34.0 us with 4096 (four items per
+// thread) and 20.9 and 30.1 us with 16384 (one); with FP32 storage (CP = 1) 16384 workgroups took
+// 29.0 and 38.1 us against 35.6 and 44.5 us with 8192.  The restriction (one component per lane)
+// took 28.0 us with 4096 workgroups and 28.8 us with 8192 at FP16.
+constexpr unsigned LEVEL_GRID = 16384;
 constexpr unsigned RESTRICT_GRID = 4096;
 
 // The logical blocks [0, T) of a striding level launch, split so that each XCD (workgroup b runs
@@ -743,9 +745,9 @@ __device__ __forceinline__ XcdRange xcd_range(uint32_t T) {
     return r;
 }
 
-static unsigned level_grid(size_t items) {
-    const size_t b = (items + 255) / 256;
-    return (unsigned)(b < 1 ? 1 : (b > LEVEL_GRID ? LEVEL_GRID : b));
+static unsigned level_grid(size_t items, int cp = 1) {
+    const size_t b = (items + 255) / 256, cap = LEVEL_GRID / (unsigned)cp;
+    return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
 // the level tables of the CP components of this thread (blockDim and every stride are multiples
@@ -1065,16 +1067,16 @@ void Pressure::vcycle_levels(int k, hipStream_t s) {
         constexpr int CP = level_cp<LT>();
         for (int l = 0; l < S; ++l) {
             const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * (8 / CP);
-            hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 1, st, k);
+            hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half, CP)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 1, st, k);
             hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[l + 1].m)), dim3(256), 0, s,
                                lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), (l + 1 < S) ? 1 : 0, st, k);
         }
         launch_coarse<LT>(*this, s, k);
         for (int l = S - 1; l >= 0; --l) {
             const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * (8 / CP);
-            hipLaunchKernelGGL(k_prolong_black<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]),
+            hipLaunchKernelGGL(k_prolong_black<LT>, dim3(level_grid(half, CP)), dim3(256), 0, s, lvl_as<LT>(lv[l]),
                                lvl_as<LT>(lv[l + 1]), st, k);
-            hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 0, st, k);
+            hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half, CP)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 0, st, k);
         }
     });
 }
@@ -1089,8 +1091,8 @@ bool Pressure::launch_single_mg(int which, hipStream_t s) {
         const size_t half0 = m0 * m0 * (m0 / 2) * (8 / level_cp<LT>());
         switch (which) {
             case 2:
-                hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half0)), dim3(256), 0, s, lvl_as<LT>(lv[0]), 1,
-                                   st, 0);
+                hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half0, level_cp<LT>())), dim3(256), 0, s,
+                                   lvl_as<LT>(lv[0]), 1, st, 0);
                 break;
             case 3: {
                 hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[1].m)), dim3(256), 0, s,
@@ -1098,8 +1100,8 @@ bool Pressure::launch_single_mg(int which, hipStream_t s) {
                 break;
             }
             case 4:
-                hipLaunchKernelGGL(k_prolong_black<LT>, dim3(level_grid(half0)), dim3(256), 0, s, lvl_as<LT>(lv[0]),
-                                   lvl_as<LT>(lv[1]), st, 0);
+                hipLaunchKernelGGL(k_prolong_black<LT>, dim3(level_grid(half0, level_cp<LT>())), dim3(256), 0, s,
+                                   lvl_as<LT>(lv[0]), lvl_as<LT>(lv[1]), st, 0);
                 break;
             default: launch_coarse<LT>(*this, s, 0); break;
         }
