@@ -710,7 +710,7 @@ __device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K
     J = (int)(q - (uint32_t)K * m);
 }
 
-// Level launches use at most LEVEL_GRID / CP workgroups (the restriction RESTRICT_GRID), each
+// Level launches use at most LEVEL_GRID / CP workgroups (the restriction restrict_grid), each
 // striding over the items of the launch in passes of that many * 256 (blocks renumbered per XCD
 // within a pass): two items per thread at level 1 of 256^3 for either storage.  A launch of a CG
 // iteration that has already stopped then costs the dispatch of that many workgroups, not of one
@@ -720,7 +720,12 @@ __device__ __forceinline__ void full_item(uint32_t t, uint32_t m, int& c, int& K
 // 29.0 and 38.1 us against 35.6 and 44.5 us with 8192.  The restriction (one component per lane)
 // took 28.0 us with 4096 workgroups and 28.8 us with 8192 at FP16.
 constexpr unsigned LEVEL_GRID = 16384;
-constexpr unsigned RESTRICT_GRID = 4096;
+// the restriction's workgroup cap: 4096 for 16-bit levels (28.0 us at level 1 of 256^3 against 28.8
+// with 8192), 8192 for FP32 (42.1 us against 48.2 with 4096)
+template <class LT>
+constexpr unsigned restrict_grid() {
+    return sizeof(LT) == 2 ? 4096u : 8192u;
+}
 
 // The logical blocks [0, T) of a striding level launch, split so that each XCD (workgroup b runs
 // on XCD b % 8) sweeps one contiguous eighth of them: the neighbour planes an item reads are then
@@ -823,10 +828,12 @@ __global__ void __launch_bounds__(256) k_restrict(LvlT<LT> F, LvlT<LT> C, int do
 }
 
 // workgroups of a k_restrict launch onto coarse size m
+template <class LT>
 static unsigned restrict_blocks(int m) {
     constexpr int ROW = RESTRICT_ROW;
-    if (m >= ROW) return std::min((unsigned)m * (unsigned)m * (unsigned)((m + ROW - 1) / ROW), RESTRICT_GRID);
-    return std::min(level_grid((size_t)m * m * m * (8 / RESTRICT_CP)), RESTRICT_GRID);
+    constexpr unsigned G = restrict_grid<LT>();
+    if (m >= ROW) return std::min((unsigned)m * (unsigned)m * (unsigned)((m + ROW - 1) / ROW), G);
+    return std::min(level_grid((size_t)m * m * m * (8 / RESTRICT_CP)), G);
 }
 
 template <class LT>
@@ -1068,7 +1075,7 @@ void Pressure::vcycle_levels(int k, hipStream_t s) {
         for (int l = 0; l < S; ++l) {
             const size_t half = (size_t)lv[l].m * lv[l].m * (lv[l].m / 2) * (8 / CP);
             hipLaunchKernelGGL(k_smooth_half<LT>, dim3(level_grid(half, CP)), dim3(256), 0, s, lvl_as<LT>(lv[l]), 1, st, k);
-            hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[l + 1].m)), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks<LT>(lv[l + 1].m)), dim3(256), 0, s,
                                lvl_as<LT>(lv[l]), lvl_as<LT>(lv[l + 1]), (l + 1 < S) ? 1 : 0, st, k);
         }
         launch_coarse<LT>(*this, s, k);
@@ -1095,7 +1102,7 @@ bool Pressure::launch_single_mg(int which, hipStream_t s) {
                                    lvl_as<LT>(lv[0]), 1, st, 0);
                 break;
             case 3: {
-                hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks(lv[1].m)), dim3(256), 0, s,
+                hipLaunchKernelGGL(k_restrict<LT>, dim3(restrict_blocks<LT>(lv[1].m)), dim3(256), 0, s,
                                    lvl_as<LT>(lv[0]), lvl_as<LT>(lv[1]), S > 1 ? 1 : 0, st, 0);
                 break;
             }
