@@ -54,8 +54,8 @@ extern "C" {
                                     the counts of recent projections; B > 0 enqueues exactly
                                     min(B, P_MAXIT) iterations (the ones after the stop test are no-ops
                                     on the device) */
-#define VFX_OPT_CELLS_PER_THREAD 10  /* z-cells per thread of the cell kernels: 1 (default), 2, 4, 8;
-                                    results equal up to FP64 reduction order */
+#define VFX_OPT_CELLS_PER_THREAD 10  /* z-cells per thread of the cell kernels: 1, 2, 4, 8 (default 2
+                                    from 128^3, 1 below); results equal up to FP64 reduction order */
 #define VFX_OPT_PRESSURE_MODE 11 /* 0 MGPCG (default); 1 the fused projection (pressure_fused.py):
                                     sweeps + V-cycles as a stationary iteration, adaptive tail while
                                     P_TOL > 0 (up to P_MAXIT cycles); FP32 / FP64 level storage */

@@ -828,6 +828,12 @@ int VFX_FN(vfx_ctx_create)(int res, vfx_ctx** out) {
     ok = ok && hip_ok(hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming), "event");
     std::memset(c->prm_last, 0, sizeof(c->prm_last));
     if (!ok) { VFX_FN(vfx_ctx_destroy)(c); return VFX_E_HIP; }
+    // The cell kernels' default from 128^3 is two cells per thread (the 256^3 fire step 6.87 ->
+    // 6.79 ms; four: 6.82 ms).  Every buffer above is sized for one, the geometry with the most blocks.
+    if (N >= 128) {
+        c->g = make_geom(N, 2);
+        P.geom = c->g;
+    }
     *out = c;
     return VFX_OK;
 }
